@@ -1,0 +1,249 @@
+/*
+ * piny_amd.h -- C ABI of the MI355X force evaluation for path-integral water.
+ *
+ * This library replaces PINY_MD's classical force call
+ *     void energy_control(CLASS*, BONDED*, GENERAL_DATA*)
+ *         src/energy/control/energy_control.c:33-77,
+ *         prototype src/energy/control/proto_energy_ctrl_entry.h:4
+ *     void energy_control_pimd(CLASS*, BONDED*, GENERAL_DATA*)
+ *         src/energy/control/energy_control_pimd.c:37-278,
+ *         prototype src/energy/control/proto_energy_ctrl_entry.h:17
+ * for the water force fields (LJ + erfc-Ewald real space, PME, bonds,
+ * bends, excluded-pair Ewald corrections, massless ghost sites, ring-polymer
+ * springs).  The reference keeps everything inside its CLASS / BONDED /
+ * GENERAL_DATA structs; here the same data crosses the boundary as plain
+ * pointers and sizes, in PINY's internal units (Bohr, Hartree, elementary
+ * charge, electron masses, Kelvin) and 0-based indices.
+ *
+ * Arrays that PINY keeps per bead (clatoms_pos[ip].x[1..N],
+ * typedefs_class.h:209-220) are passed bead-major: x[ip*N + i] for bead
+ * ip = 0..P-1 and site i = 0..N-1.
+ *
+ * Every entry point returns PMX_OK (0) or an error code; the text of the
+ * last error of a context is available from pmx_last_error().  Nothing in
+ * this library calls exit() (the reference does, e.g. force_npol.c:174-182).
+ * A context is not re-entrant; distinct contexts may be used concurrently
+ * from different host threads (as PINYEmpirical does with replicas,
+ * python/piny/piny.pyx:426-436).
+ */
+#ifndef PINY_AMD_H
+#define PINY_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PMX_ABI_VERSION 1
+
+/* return codes */
+enum {
+  PMX_OK = 0,
+  PMX_ERR_ARG = 1,      /* invalid argument or inconsistent input        */
+  PMX_ERR_DEVICE = 2,   /* HIP runtime error                             */
+  PMX_ERR_STATE = 3,    /* call out of order (e.g. no positions yet)     */
+  PMX_ERR_RANGE = 4,    /* a size exceeds what was allocated             */
+  PMX_ERR_NOMEM = 5,    /* allocation failure                            */
+  PMX_ERR_NOTIMPL = 6   /* option not supported by this build            */
+};
+
+/* intermolecular potential labels, as PINY's inter_label
+ * (control_inter_params.c:447-470) */
+enum { PMX_INTER_LJ = 2, PMX_INTER_NULL = 4 };
+
+/* shift_inter_pe (set_sim_params.c:1185-1197): 0 none, 1 shift, 2 swit */
+enum { PMX_SHIFT_NONE = 0, PMX_SHIFT_SHIFT = 1, PMX_SHIFT_SWIT = 2 };
+
+/* arithmetic of the pair, spline and FFT work */
+enum { PMX_PREC_FP64 = 0, PMX_PREC_MIXED = 1 };
+
+/* path_int_md_typ (transform_cnt.c:104,190): 1 staging, 2 centroid */
+enum { PMX_PIMD_STAGING = 1, PMX_PIMD_CENTROID = 2 };
+
+/* flags of pmx_energy_control */
+enum {
+  PMX_EVAL_PIMD = 1u << 0,        /* energy_control_pimd semantics: shared
+                                     list from bead 1 + spread, xmod image
+                                     convention, f/P, E/P, pvten/P       */
+  PMX_EVAL_FORCE_LIST = 1u << 1,  /* rebuild the Verlet list regardless of
+                                     the skin criterion                   */
+  PMX_EVAL_NO_RECIP = 1u << 2,    /* skip the k-space sum (real, intra
+                                     terms only)                          */
+  PMX_EVAL_MODE_FORCES = 1u << 3  /* PIMD: also transform the forces to
+                                     mode coordinates and add the spring
+                                     forces (energy_control_pimd.c:200-218) */
+};
+
+/*
+ * Topology and force field after PINY's parse(): the values PINY holds in
+ * CLATOMS_INFO (typedefs_class.h:124-177), INTERACT (:610-690), BOND, BEND,
+ * GHOST_ATOMS (:225-243).  Type pairs are numbered like PINY's inter list
+ * (control_inter_params.c:118-126): for 0-based types a <= b,
+ *     pair(a,b) = a*ntype - a*(a-1)/2 + (b - a).
+ */
+typedef struct pmx_system {
+  int natm;                     /* sites, ghosts included                  */
+  int ntype;                    /* atom types                              */
+  const int *atm_type;          /* [natm] 0..ntype-1                       */
+  const double *q;              /* [natm] charge                           */
+  const double *mass;           /* [natm] electron masses (amu*PROT_MASS)  */
+
+  /* per type pair, ntype*(ntype+1)/2 entries */
+  const int *inter_label;       /* PMX_INTER_LJ or PMX_INTER_NULL          */
+  const double *inter_sig;      /* LJ sigma, Bohr                          */
+  const double *inter_eps;      /* LJ epsilon, Hartree                     */
+  const double *inter_cutti;    /* spline start (min_dist), Bohr           */
+  const double *inter_cutoff;   /* cutoff (max_dist), Bohr                 */
+  const double *inter_cutoff_res; /* RESPA cutoff (res_dist), Bohr         */
+
+  /* bonds: V = sum_m c_m (r - eq)^m, m = 0..6 (bond.c:150-166) */
+  int nbond;
+  const int *bond_j1, *bond_j2, *bond_type;  /* [nbond]                   */
+  int nbond_type;
+  const double *bond_c;         /* [nbond_type*7]                          */
+  const double *bond_eq;        /* [nbond_type] Bohr                       */
+
+  /* bends j1-j2-j3 (j2 the apex):
+   * V = sum_m c_m cos^m + cos * sum_{m>=2} s_m sin^(m-1) + s_1 sin
+   * (bend.c:253-270) */
+  int nbend;
+  const int *bend_j1, *bend_j2, *bend_j3, *bend_type;
+  int nbend_type;
+  const double *bend_c;         /* [nbend_type*7]                          */
+  const double *bend_s;         /* [nbend_type*7]                          */
+
+  /* massless ghost sites r_g = sum_k coef_k r_comp_k (ghost_control.c) */
+  int nghost;
+  const int *ghost_atom;        /* [nghost]                                */
+  const int *ghost_ncomp;       /* [nghost] <= 4                           */
+  const int *ghost_comp;        /* [nghost*4]                              */
+  const double *ghost_coef;     /* [nghost*4]                              */
+} pmx_system;
+
+/* Simulation settings (PINY input keywords in brackets). */
+typedef struct pmx_params {
+  double hmat[9];       /* cell, column-major like PINY hmat[1..9], Bohr   */
+  int pi_beads;         /* [path_int_beads] P                              */
+  double temperature;   /* [temperature] K, for spring constants           */
+  int pimd_typ;         /* [path_int_md_typ] PMX_PIMD_*                    */
+  int nsplin;           /* [inter_spline_pts]                              */
+  int ishift;           /* [shift_inter_pe] PMX_SHIFT_*                    */
+  double rheal;         /* [respa_rheal] switching length, Bohr            */
+  double skin;          /* [verlist_skin] Bohr                             */
+  double ewald_alpha;   /* [ewald_alpha] dimensionless alpha * V^(1/3)     */
+  int kmax_ewd;         /* [ewald_kmax]                                    */
+  int pme_on;           /* [ewald_pme_opt] 1 on (0: standard Ewald)        */
+  int kmax_pme;         /* [ewald_kmax_pme]                                */
+  int n_interp;         /* [ewald_interp_pme] B-spline order               */
+  int intra_perds;      /* [intra_perds] 1: minimum image on bonded terms  */
+  int precision;        /* PMX_PREC_*                                      */
+} pmx_params;
+
+/* Energies (Hartree) and pressure tensors (Hartree), stat_avg / ptens. */
+typedef struct pmx_energies {
+  double vintert;       /* inter: vreal + vlong + vrecip+vself+vbgr+vecor  */
+  double vintrat;       /* intra: vbond + vbend                            */
+  double vreal, vvdw, vcoul, vlong;
+  double vrecip;        /* k-space sum only                                */
+  double vself, vbgr, vecor;
+  double vbond, vbend;
+  double kin_harm;      /* spring energy (mode_energy_control)             */
+  double pi_ke_prim;    /* primitive kinetic estimator (mode forces only)  */
+  double pi_ke_vir;     /* virial kinetic estimator (PIMD)                 */
+  double pvten[9];      /* weighted pressure*volume tensor                 */
+  double pvten_tot[9];
+  double spread;        /* PIMD bead spread used for the list skin         */
+  int nver;             /* Verlet-list pairs                               */
+  int list_rebuilt;     /* 1 if this call rebuilt the list                 */
+} pmx_energies;
+
+typedef struct pmx_ctx pmx_ctx;
+
+/* ABI version of the loaded library (== PMX_ABI_VERSION). */
+int pmx_abi_version(void);
+
+/* Number of HIP devices visible (0 when none). */
+int pmx_device_count(void);
+
+/*
+ * Builds a context: PINY's setup (control_inter_params spline tables,
+ * control_set_cp_ewald k-vectors / PME grid / B-spline weights,
+ * set_exclude exclusions and Ewald-correction list, splin_ecor table,
+ * path_integral_init spring constants), then device allocation.
+ * init_x/y/z ([P*N], may be NULL) are the initial positions used only to
+ * size the excluded-pair table as splin_ecor does (set_exclude.c:600-622).
+ * Returns NULL on failure and writes a message into err (if not NULL).
+ */
+pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par,
+                    const double *init_x, const double *init_y,
+                    const double *init_z, int device, char *err,
+                    size_t errlen);
+
+void pmx_destroy(pmx_ctx *ctx);
+
+const char *pmx_last_error(const pmx_ctx *ctx);
+
+/* Positions of all beads, bead-major [P*N], from host memory.  Ghost
+ * positions are recomputed on the device (get_ghost_pos). */
+int pmx_set_positions(pmx_ctx *ctx, const double *x, const double *y,
+                      const double *z);
+
+/* Same, from device memory (three [P*N] arrays on the context's device). */
+int pmx_set_positions_device(pmx_ctx *ctx, const double *x, const double *y,
+                             const double *z);
+
+/* Mode-1 coordinates xmod ([N]) used by the PIMD minimum-image convention
+ * (period.c:121-206).  If never set, the bead centroid is used. */
+int pmx_set_xmod(pmx_ctx *ctx, const double *x, const double *y,
+                 const double *z);
+
+/* The force call. */
+int pmx_energy_control(pmx_ctx *ctx, unsigned flags, pmx_energies *out);
+
+/* Forces of all beads [P*N] into host memory (ghost forces redistributed). */
+int pmx_get_forces(pmx_ctx *ctx, double *fx, double *fy, double *fz);
+
+/* Device pointers of the context's force arrays ([P*N] each). */
+int pmx_get_forces_device(pmx_ctx *ctx, const double **fx, const double **fy,
+                          const double **fz);
+
+/* Virial-estimator forces fxt of all beads (PIMD only). */
+int pmx_get_fxt(pmx_ctx *ctx, double *fx, double *fy, double *fz);
+
+/* Per-bead energies [P*4]: vreal, vrecip(k-sum), vecor, vintra. */
+int pmx_get_bead_energies(pmx_ctx *ctx, double *out);
+
+/* Verlet half list in PINY's form (verlist_create.c:373-503): for every
+ * site i, nter[i] partners j < i in ascending order, stored at
+ * jter[jver_off[i] ...].  Returns the pair count in *nver. */
+int pmx_get_verlet_list(pmx_ctx *ctx, int *nter, int *jver_off, int *jter,
+                        int cap, int *nver);
+
+/* Setup products for parity checks. */
+int pmx_get_nktot(pmx_ctx *ctx, int *nktot, int *ngrid);   /* ngrid[3]   */
+int pmx_get_kvectors(pmx_ctx *ctx, int *ka, int *kb, int *kc,
+                     double *bweight, int cap);
+/* which: 0 cv0, 1 cdv0, 2 cv0_c, 3 cdv0_c for type pair ipair;
+ *        4 ecor cv0, 5 ecor cdv0 (ipair ignored). */
+int pmx_get_spline(pmx_ctx *ctx, int which, int ipair, double *out, int cap);
+int pmx_get_setup_scalars(pmx_ctx *ctx, double *out); /* see setup_scalars */
+
+/* Wall time (ms) of the last pmx_energy_control per stage:
+ * [0] list, [1] real, [2] recip, [3] intra, [4] total. */
+int pmx_get_timings(pmx_ctx *ctx, double *ms5);
+
+/* Real-space pairs inside the cutoff evaluated by the last force call,
+ * summed over beads (the work unit of the pair kernel). */
+int pmx_get_pair_count(pmx_ctx *ctx, double *npair);
+
+/* Coefficient helpers restating intra_coefs.c (units: Hartree, Bohr). */
+int pmx_bond_coef_harm(double fk, double *c7);
+int pmx_bond_coef_morse(double d0, double alpha, double *c7);
+int pmx_bend_coef_harm(double fk, double theta0, double *c7, double *s7);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PINY_AMD_H */

@@ -1,0 +1,128 @@
+"""ctypes mirror of include/piny_amd.h (structs, enums) and helpers that turn a
+``water.System`` into the plain-pointer structs the C ABI takes."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+PMX_OK = 0
+PMX_EVAL_PIMD = 1 << 0
+PMX_EVAL_FORCE_LIST = 1 << 1
+PMX_EVAL_NO_RECIP = 1 << 2
+PMX_EVAL_MODE_FORCES = 1 << 3
+PMX_PREC_FP64 = 0
+PMX_PREC_MIXED = 1
+
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+
+class PmxSystem(C.Structure):
+    _fields_ = [
+        ("natm", C.c_int), ("ntype", C.c_int),
+        ("atm_type", _ip), ("q", _dp), ("mass", _dp),
+        ("inter_label", _ip), ("inter_sig", _dp), ("inter_eps", _dp),
+        ("inter_cutti", _dp), ("inter_cutoff", _dp), ("inter_cutoff_res", _dp),
+        ("nbond", C.c_int), ("bond_j1", _ip), ("bond_j2", _ip), ("bond_type", _ip),
+        ("nbond_type", C.c_int), ("bond_c", _dp), ("bond_eq", _dp),
+        ("nbend", C.c_int), ("bend_j1", _ip), ("bend_j2", _ip), ("bend_j3", _ip),
+        ("bend_type", _ip), ("nbend_type", C.c_int), ("bend_c", _dp), ("bend_s", _dp),
+        ("nghost", C.c_int), ("ghost_atom", _ip), ("ghost_ncomp", _ip),
+        ("ghost_comp", _ip), ("ghost_coef", _dp),
+    ]
+
+
+class PmxParams(C.Structure):
+    _fields_ = [
+        ("hmat", C.c_double * 9), ("pi_beads", C.c_int), ("temperature", C.c_double),
+        ("pimd_typ", C.c_int), ("nsplin", C.c_int), ("ishift", C.c_int),
+        ("rheal", C.c_double), ("skin", C.c_double), ("ewald_alpha", C.c_double),
+        ("kmax_ewd", C.c_int), ("pme_on", C.c_int), ("kmax_pme", C.c_int),
+        ("n_interp", C.c_int), ("intra_perds", C.c_int), ("precision", C.c_int),
+    ]
+
+
+class PmxEnergies(C.Structure):
+    _fields_ = [
+        ("vintert", C.c_double), ("vintrat", C.c_double),
+        ("vreal", C.c_double), ("vvdw", C.c_double), ("vcoul", C.c_double),
+        ("vlong", C.c_double), ("vrecip", C.c_double), ("vself", C.c_double),
+        ("vbgr", C.c_double), ("vecor", C.c_double), ("vbond", C.c_double),
+        ("vbend", C.c_double), ("kin_harm", C.c_double), ("pi_ke_prim", C.c_double),
+        ("pi_ke_vir", C.c_double), ("pvten", C.c_double * 9),
+        ("pvten_tot", C.c_double * 9), ("spread", C.c_double), ("nver", C.c_int),
+        ("list_rebuilt", C.c_int),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("pvten", "pvten_tot")}
+        d["pvten"] = np.array(self.pvten[:])
+        d["pvten_tot"] = np.array(self.pvten_tot[:])
+        return d
+
+
+ENERGY_KEYS = ("vintert", "vintrat", "vreal", "vvdw", "vcoul", "vlong", "vrecip",
+               "vself", "vbgr", "vecor", "vbond", "vbend")
+
+
+def _iarr(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def _darr(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+class SystemStruct:
+    """Holds a PmxSystem / PmxParams pair and the numpy arrays they point to."""
+
+    def __init__(self, s, precision: int = PMX_PREC_FP64):
+        self.keep = []
+
+        def ia(x):
+            arr, p = _iarr(x)
+            self.keep.append(arr)
+            return p
+
+        def da(x):
+            arr, p = _darr(x)
+            self.keep.append(arr)
+            return p
+
+        st = s.settings
+        self.sys = PmxSystem(
+            natm=s.natm, ntype=s.ntype, atm_type=ia(s.atm_type), q=da(s.q), mass=da(s.mass),
+            inter_label=ia(s.inter_label), inter_sig=da(s.inter_sig), inter_eps=da(s.inter_eps),
+            inter_cutti=da(s.inter_cutti), inter_cutoff=da(s.inter_cutoff),
+            inter_cutoff_res=da(s.inter_cutoff_res),
+            nbond=len(s.bond_j1), bond_j1=ia(s.bond_j1), bond_j2=ia(s.bond_j2),
+            bond_type=ia(s.bond_type), nbond_type=len(s.bond_eq),
+            bond_c=da(np.asarray(s.bond_c).reshape(-1)), bond_eq=da(s.bond_eq),
+            nbend=len(s.bend_j1), bend_j1=ia(s.bend_j1), bend_j2=ia(s.bend_j2),
+            bend_j3=ia(s.bend_j3), bend_type=ia(s.bend_type),
+            nbend_type=max(1, np.asarray(s.bend_c).size // 7),
+            bend_c=da(np.asarray(s.bend_c).reshape(-1)), bend_s=da(np.asarray(s.bend_s).reshape(-1)),
+            nghost=len(s.ghost_atom), ghost_atom=ia(s.ghost_atom), ghost_ncomp=ia(s.ghost_ncomp),
+            ghost_comp=ia(s.ghost_comp), ghost_coef=da(s.ghost_coef))
+        bohr = 0.529177
+        self.par = PmxParams(
+            pi_beads=s.pi_beads, temperature=st.temperature, pimd_typ=st.pimd_typ,
+            nsplin=st.nsplin, ishift=st.ishift, rheal=st.rheal / bohr, skin=st.skin / bohr,
+            ewald_alpha=s.ewald_alpha, kmax_ewd=st.kmax_ewd, pme_on=st.pme_on,
+            kmax_pme=st.kmax_pme, n_interp=st.n_interp, intra_perds=st.intra_perds,
+            precision=precision)
+        for k, v in enumerate(s.hmat):
+            self.par.hmat[k] = float(v)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_ip)

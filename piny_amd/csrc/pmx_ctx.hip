@@ -1,0 +1,925 @@
+// Context, device memory and the C ABI (include/piny_amd.h).
+//
+// pmx_energy_control follows the stage order of energy_control
+// (energy_control.c:49-74) and energy_control_pimd (energy_control_pimd.c:
+// 91-273): initial (zero forces) -> real space (list check/build, pair
+// forces) -> reciprocal space (self/background, PME, excluded-pair
+// corrections) -> intramolecular -> PIMD /P and virial correction -> ghost
+// force redistribution -> optional normal-mode forces.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pmx_internal.h"
+#include "pmx_kernels.h"
+
+using namespace pmx;
+
+struct pmx_ctx {
+  std::string err;
+  int device = 0;
+  Setup s;
+  pmx_params par{};
+  hipStream_t st = nullptr;
+  hipEvent_t ev[6] = {};
+  double ms[5] = {0, 0, 0, 0, 0};
+  std::vector<void *> allocs;
+  // positions / forces, atom-major [N*P]
+  double *x = nullptr, *y = nullptr, *z = nullptr, *stage = nullptr;
+  double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  double *xm = nullptr, *ym = nullptr, *zm = nullptr;
+  double *b0x = nullptr, *b0y = nullptr, *b0z = nullptr, *x0 = nullptr, *y0 = nullptr, *z0 = nullptr;
+  unsigned long long *maxes = nullptr;
+  int *type = nullptr, *ichrg = nullptr, *ex_off = nullptr, *ex_num = nullptr, *ex_j = nullptr;
+  double *q = nullptr;
+  double2 *tab_lj = nullptr, *tab_c = nullptr, *ecor_tab = nullptr;
+  int cap = 0, *jfull = nullptr, *jforce = nullptr, *nfull = nullptr, *nforce = nullptr, *overflow = nullptr;
+  int *b1 = nullptr, *b2 = nullptr, *btyp = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr,
+      *atyp = nullptr, *e1 = nullptr, *e2 = nullptr;
+  double *bc = nullptr, *beq = nullptr, *ac = nullptr, *as = nullptr;
+  int nghost = 0, *gatom = nullptr, *gncomp = nullptr, *gcomp = nullptr;
+  double *gcoef = nullptr;
+  // PME
+  int pme = 0, logK[3] = {0, 0, 0};
+  size_t gstride = 0, sstride = 0;
+  double *grid = nullptr, *infl = nullptr;
+  double2 *spec = nullptr, *tw[3] = {nullptr, nullptr, nullptr};
+  // accumulators [P][ACC_NUM] + ake
+  double *acc = nullptr, *ake = nullptr;
+  std::vector<double> acc_h;
+  // springs
+  double *prekf = nullptr, *veig = nullptr, *cs = nullptr;
+  // state
+  bool have_pos = false, have_list = false, xmod_given = false;
+  double spread = 0.0;
+  int nver = 0;
+  PairTypes pt{};
+  Cell cell{};
+  int nbond = 0, nbend = 0;
+};
+
+namespace {
+
+void set_err(pmx_ctx *c, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  c->err = buf;
+}
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      set_err(c, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return PMX_ERR_DEVICE;                                                       \
+    }                                                                              \
+  } while (0)
+
+template <typename T>
+int dalloc(pmx_ctx *c, T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    set_err(c, "hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    return PMX_ERR_NOMEM;
+  }
+  c->allocs.push_back((void *)*p);
+  return PMX_OK;
+}
+
+template <typename T>
+int upload(pmx_ctx *c, T **p, const T *src, size_t n) {
+  int rc = dalloc(c, p, n);
+  if (rc) return rc;
+  if (n) HIPCHK(hipMemcpy(*p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return PMX_OK;
+}
+
+int ilog2_exact(int k) {
+  int l = 0;
+  while ((1 << l) < k) l++;
+  return (1 << l) == k ? l : -1;
+}
+
+unsigned blocks(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+int alloc_lists(pmx_ctx *c, int cap) {
+  const size_t n = (size_t)c->s.N * cap;
+  if (c->jfull) {
+    hipFree(c->jfull);
+    hipFree(c->jforce);
+    c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->jfull), c->allocs.end());
+    c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->jforce), c->allocs.end());
+  }
+  c->cap = cap;
+  int rc = dalloc(c, &c->jfull, n);
+  if (!rc) rc = dalloc(c, &c->jforce, n);
+  return rc;
+}
+
+// Initial ELL width: 1.3 x the mean full-neighbour count within the largest
+// list radius, plus a margin, in multiples of 64.
+int estimate_cap(const Setup &s, double rskin) {
+  const double dens = (double)s.N / s.vol;
+  const double nb = dens * 4.0 / 3.0 * M_PI * rskin * rskin * rskin;
+  int cap = (int)(1.3 * nb) + 64;
+  cap = ((cap + 63) / 64) * 64;
+  return std::min(std::max(cap, 64), ((s.N + 63) / 64) * 64);
+}
+
+int build_device(pmx_ctx *c, const pmx_system *sys) {
+  Setup &s = c->s;
+  const int N = s.N, P = s.P;
+  const size_t NP = (size_t)N * P;
+  int rc = 0;
+#define TRY(x)        \
+  do {                \
+    rc = (x);         \
+    if (rc) return rc; \
+  } while (0)
+  TRY(dalloc(c, &c->x, NP));
+  TRY(dalloc(c, &c->y, NP));
+  TRY(dalloc(c, &c->z, NP));
+  TRY(dalloc(c, &c->stage, NP));
+  TRY(dalloc(c, &c->fx, NP));
+  TRY(dalloc(c, &c->fy, NP));
+  TRY(dalloc(c, &c->fz, NP));
+  TRY(dalloc(c, &c->xm, N));
+  TRY(dalloc(c, &c->ym, N));
+  TRY(dalloc(c, &c->zm, N));
+  TRY(dalloc(c, &c->b0x, N));
+  TRY(dalloc(c, &c->b0y, N));
+  TRY(dalloc(c, &c->b0z, N));
+  TRY(dalloc(c, &c->x0, N));
+  TRY(dalloc(c, &c->y0, N));
+  TRY(dalloc(c, &c->z0, N));
+  TRY(dalloc(c, &c->maxes, 2));
+  TRY(upload(c, &c->type, sys->atm_type, N));
+  TRY(upload(c, &c->q, sys->q, N));
+  TRY(upload(c, &c->ichrg, s.ichrg.data(), s.ichrg.size()));
+  TRY(upload(c, &c->ex_off, s.ex_off.data(), N));
+  TRY(upload(c, &c->ex_num, s.ex_num.data(), N));
+  TRY(upload(c, &c->ex_j, s.ex_j.data(), s.ex_j.size()));
+  // tables as (V, -V'/r) pairs
+  {
+    const size_t nt = (size_t)s.npair * s.nsplin;
+    std::vector<double2> lj(nt), cc(nt);
+    for (size_t k = 0; k < nt; k++) {
+      lj[k] = make_double2(s.tab[4 * k + 0], s.tab[4 * k + 1]);
+      cc[k] = make_double2(s.tab[4 * k + 2], s.tab[4 * k + 3]);
+    }
+    TRY(upload(c, &c->tab_lj, lj.data(), nt));
+    TRY(upload(c, &c->tab_c, cc.data(), nt));
+    std::vector<double2> et(s.nsplin);
+    for (int k = 0; k < s.nsplin; k++) et[k] = make_double2(s.ecor_tab[2 * k], s.ecor_tab[2 * k + 1]);
+    TRY(upload(c, &c->ecor_tab, et.data(), et.size()));
+  }
+  TRY(dalloc(c, &c->nfull, N));
+  TRY(dalloc(c, &c->nforce, N));
+  TRY(dalloc(c, &c->overflow, 2));
+  TRY(alloc_lists(c, estimate_cap(s, s.cutoff_max + c->par.skin + 1.0 / BOHR * (P > 1 ? 1.0 : 0.0))));
+  // bonded
+  c->nbond = sys->nbond;
+  c->nbend = sys->nbend;
+  TRY(upload(c, &c->b1, sys->bond_j1, sys->nbond));
+  TRY(upload(c, &c->b2, sys->bond_j2, sys->nbond));
+  TRY(upload(c, &c->btyp, sys->bond_type, sys->nbond));
+  TRY(upload(c, &c->bc, sys->bond_c, (size_t)7 * sys->nbond_type));
+  TRY(upload(c, &c->beq, sys->bond_eq, sys->nbond_type));
+  TRY(upload(c, &c->a1, sys->bend_j1, sys->nbend));
+  TRY(upload(c, &c->a2, sys->bend_j2, sys->nbend));
+  TRY(upload(c, &c->a3, sys->bend_j3, sys->nbend));
+  TRY(upload(c, &c->atyp, sys->bend_type, sys->nbend));
+  TRY(upload(c, &c->ac, sys->bend_c, (size_t)7 * sys->nbend_type));
+  TRY(upload(c, &c->as, sys->bend_s, (size_t)7 * sys->nbend_type));
+  TRY(upload(c, &c->e1, s.e1.data(), s.e1.size()));
+  TRY(upload(c, &c->e2, s.e2.data(), s.e2.size()));
+  c->nghost = sys->nghost;
+  TRY(upload(c, &c->gatom, sys->ghost_atom, sys->nghost));
+  TRY(upload(c, &c->gncomp, sys->ghost_ncomp, sys->nghost));
+  TRY(upload(c, &c->gcomp, sys->ghost_comp, (size_t)4 * sys->nghost));
+  TRY(upload(c, &c->gcoef, sys->ghost_coef, (size_t)4 * sys->nghost));
+  // PME grids, twiddles and the influence function on the sphere list
+  c->pme = (!s.ichrg.empty() && c->par.pme_on);
+  if (!s.ichrg.empty() && !c->par.pme_on) {
+    set_err(c, "standard Ewald (ewald_pme_opt off) is not implemented on the device; use PME");
+    return PMX_ERR_NOTIMPL;
+  }
+  if (c->pme) {
+    const int Ka = s.ngrid[0], Kb = s.ngrid[1], Kc = s.ngrid[2], Ha = Ka / 2 + 1;
+    for (int d = 0; d < 3; d++) {
+      c->logK[d] = ilog2_exact(s.ngrid[d]);
+      if (c->logK[d] < 0) {
+        set_err(c, "PME grid %dx%dx%d: only power-of-two grids are implemented on the device", Ka, Kb, Kc);
+        return PMX_ERR_NOTIMPL;
+      }
+    }
+    if ((size_t)Ka * Kb * sizeof(double2) > 65536 || (size_t)Kc * Ha * sizeof(double2) > 65536) {
+      set_err(c, "PME grid %dx%dx%d exceeds the 64 KB LDS plane of this build", Ka, Kb, Kc);
+      return PMX_ERR_NOTIMPL;
+    }
+    c->gstride = (size_t)Ka * Kb * Kc;
+    c->sstride = (size_t)Kc * Kb * Ha;
+    TRY(dalloc(c, &c->grid, c->gstride * P));
+    TRY(dalloc(c, &c->spec, c->sstride * P));
+    for (int d = 0; d < 3; d++) {
+      const int K = s.ngrid[d];
+      std::vector<double2> t(K);
+      for (int m = 0; m < K; m++) t[m] = make_double2(cos(2.0 * M_PI * m / K), -sin(2.0 * M_PI * m / K));
+      TRY(upload(c, &c->tw[d], t.data(), K));
+    }
+    std::vector<double> W(c->sstride, 0.0);
+    const double *hi = s.hmati;
+    const double falp2 = 4.0 * s.alp_ewd * s.alp_ewd, tpi = 2.0 * M_PI, pivol = s.vol / (4.0 * M_PI);
+    for (int k = 0; k < s.nktot; k++) {
+      const double aka = tpi * ((double)s.ka[k]), akb = tpi * ((double)s.kb[k]), akc = tpi * ((double)s.kc[k]);
+      const double xk = (aka * hi[1] + akb * hi[2] + akc * hi[3]);
+      const double yk = (aka * hi[4] + akb * hi[5] + akc * hi[6]);
+      const double zk = (aka * hi[7] + akb * hi[8] + akc * hi[9]);
+      const double g2 = xk * xk + yk * yk + zk * zk;
+      const double preg = exp((-g2 / falp2)) / (g2 * pivol);
+      const double w = preg * s.bweight[k];
+      const int ia = s.ka[k], ib = ((s.kb[k] % Kb) + Kb) % Kb, ic = ((s.kc[k] % Kc) + Kc) % Kc;
+      W[((size_t)ic * Kb + ib) * Ha + ia] = w;
+      if (ia == 0) {
+        const int jb = ((-s.kb[k] % Kb) + Kb) % Kb, jc = ((-s.kc[k] % Kc) + Kc) % Kc;
+        W[((size_t)jc * Kb + jb) * Ha] = w;
+      }
+    }
+    TRY(upload(c, &c->infl, W.data(), W.size()));
+  }
+  TRY(dalloc(c, &c->acc, (size_t)P * ACC_NUM));
+  TRY(dalloc(c, &c->ake, 1));
+  c->acc_h.assign((size_t)P * ACC_NUM, 0.0);
+  TRY(upload(c, &c->prekf, s.prekf.data(), N));
+  TRY(upload(c, &c->veig, s.veig.data(), s.veig.size()));
+  {
+    std::vector<double> csv(2 * P);
+    for (int m = 0; m < P; m++) {
+      csv[2 * m] = cos(2.0 * M_PI * m / P);
+      csv[2 * m + 1] = sin(2.0 * M_PI * m / P);
+    }
+    TRY(upload(c, &c->cs, csv.data(), csv.size()));
+  }
+  // kernel parameter blocks
+  for (int k = 0; k < 9; k++) {
+    c->cell.h[k] = s.hmat[k + 1];
+    c->cell.hi[k] = s.hmati[k + 1];
+  }
+  c->cell.cubic = s.cubic;
+  c->pt.ntype = s.ntype;
+  for (int t = 0; t < s.npair; t++) {
+    c->pt.lj[t] = (s.label[t] == PMX_INTER_LJ);
+    c->pt.rmin[t] = s.rmin[t];
+    c->pt.dr[t] = s.dr[t];
+    c->pt.dri[t] = s.dri[t];
+    c->pt.cut[t] = s.cutoff[t];
+    const double cs = s.cutoff[t] + c->par.skin;
+    c->pt.cutskin2[t] = cs * cs;
+  }
+  return PMX_OK;
+#undef TRY
+}
+
+// (Re)build the Verlet list from the bead-1 copy; grows the ELL width when a
+// row overflows.
+int build_list(pmx_ctx *c) {
+  Setup &s = c->s;
+  for (int attempt = 0; attempt < 8; attempt++) {
+    HIPCHK(hipMemsetAsync(c->overflow, 0, 2 * sizeof(int), c->st));
+    ListArgs a{};
+    a.N = s.N;
+    a.cap = c->cap;
+    a.pimd = (s.P > 1);
+    a.bx = c->b0x;
+    a.by = c->b0y;
+    a.bz = c->b0z;
+    a.xm = c->xm;
+    a.ym = c->ym;
+    a.zm = c->zm;
+    a.type = c->type;
+    a.q = c->q;
+    a.ex_off = c->ex_off;
+    a.ex_num = c->ex_num;
+    a.ex_j = c->ex_j;
+    a.pt = c->pt;
+    a.cell = c->cell;
+    a.jfull = c->jfull;
+    a.jforce = c->jforce;
+    a.nfull = c->nfull;
+    a.nforce = c->nforce;
+    a.overflow = c->overflow;
+    hipLaunchKernelGGL(k_list_build, dim3(blocks((size_t)s.N * 64)), dim3(256), 0, c->st, a);
+    HIPCHK(hipGetLastError());
+    int ov[2];
+    HIPCHK(hipMemcpyAsync(ov, c->overflow, sizeof ov, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (ov[0] == 0) {
+      c->nver = ov[1];
+      return PMX_OK;
+    }
+    int rc = alloc_lists(c, ((ov[0] * 5 / 4 + 63) / 64) * 64);
+    if (rc) return rc;
+  }
+  set_err(c, "Verlet list does not fit after growing its rows");
+  return PMX_ERR_RANGE;
+}
+
+IntraArgs intra_args(pmx_ctx *c, bool pimd) {
+  Setup &s = c->s;
+  IntraArgs a{};
+  a.N = s.N;
+  a.P = s.P;
+  a.pimd = pimd && s.P > 1;
+  a.intra_perds = c->par.intra_perds;
+  a.nsplin = s.nsplin;
+  a.x = c->x;
+  a.y = c->y;
+  a.z = c->z;
+  a.xm = c->xm;
+  a.ym = c->ym;
+  a.zm = c->zm;
+  a.q = c->q;
+  a.cell = c->cell;
+  a.fx = c->fx;
+  a.fy = c->fy;
+  a.fz = c->fz;
+  a.acc = c->acc;
+  a.nbond = c->nbond;
+  a.b1 = c->b1;
+  a.b2 = c->b2;
+  a.btyp = c->btyp;
+  a.bc = c->bc;
+  a.beq = c->beq;
+  a.nbend = c->nbend;
+  a.a1 = c->a1;
+  a.a2 = c->a2;
+  a.a3 = c->a3;
+  a.atyp = c->atyp;
+  a.ac = c->ac;
+  a.as = c->as;
+  a.necor = (int)s.e1.size();
+  a.e1 = c->e1;
+  a.e2 = c->e2;
+  a.ecor_tab = c->ecor_tab;
+  a.e_rmin = s.e_rmin;
+  a.e_dr = s.e_dr;
+  a.e_dri = s.e_dri;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmx_abi_version(void) { return PMX_ABI_VERSION; }
+
+int pmx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *ix, const double *iy,
+                    const double *iz, int device, char *err, size_t errlen) {
+  auto fail = [&](const std::string &m) -> pmx_ctx * {
+    if (err && errlen) snprintf(err, errlen, "%s", m.c_str());
+    return nullptr;
+  };
+  if (!sys || !par) return fail("pmx_create: null argument");
+  if (sys->natm < 2 || sys->ntype < 1 || par->pi_beads < 1 || par->nsplin < 10)
+    return fail("pmx_create: invalid sizes");
+  if (par->pi_beads > 64) return fail("pmx_create: more than 64 beads per context is not supported");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("pmx_create: no HIP device");
+  if (device < 0 || device >= ndev) return fail("pmx_create: device index out of range");
+  pmx_ctx *c = new pmx_ctx();
+  c->device = device;
+  c->par = *par;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return fail("pmx_create: hipSetDevice failed");
+  }
+  std::string m = build_setup(sys, par, ix, iy, iz, c->s);
+  if (!m.empty()) {
+    delete c;
+    return fail("pmx_create: " + m);
+  }
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail("pmx_create: stream creation failed");
+  }
+  for (auto &e : c->ev) hipEventCreate(&e);
+  if (build_device(c, sys) != PMX_OK) {
+    std::string e = c->err;
+    pmx_destroy(c);
+    return fail("pmx_create: " + e);
+  }
+  return c;
+}
+
+void pmx_destroy(pmx_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->st) hipStreamSynchronize(c->st);
+  for (void *p : c->allocs) hipFree(p);
+  for (auto &e : c->ev)
+    if (e) hipEventDestroy(e);
+  if (c->st) hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char *pmx_last_error(const pmx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+static int set_pos_common(pmx_ctx *c, const double *x, const double *y, const double *z, hipMemcpyKind kind) {
+  Setup &s = c->s;
+  const size_t NP = (size_t)s.N * s.P;
+  HIPCHK(hipSetDevice(c->device));
+  const double *src[3] = {x, y, z};
+  double *dst[3] = {c->x, c->y, c->z};
+  for (int d = 0; d < 3; d++) {
+    HIPCHK(hipMemcpyAsync(c->stage, src[d], NP * sizeof(double), kind, c->st));
+    hipLaunchKernelGGL(k_bead_to_atom, dim3(blocks(NP)), dim3(256), 0, c->st, s.N, s.P, c->stage, dst[d]);
+    HIPCHK(hipGetLastError());
+  }
+  if (c->nghost) {
+    hipLaunchKernelGGL(k_ghost_pos, dim3(blocks((size_t)c->nghost * s.P)), dim3(256), 0, c->st, c->nghost, s.P,
+                       c->gatom, c->gncomp, c->gcomp, c->gcoef, c->x, c->y, c->z);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->have_pos = true;
+  return PMX_OK;
+}
+
+int pmx_set_positions(pmx_ctx *c, const double *x, const double *y, const double *z) {
+  if (!c || !x || !y || !z) return PMX_ERR_ARG;
+  return set_pos_common(c, x, y, z, hipMemcpyHostToDevice);
+}
+
+int pmx_set_positions_device(pmx_ctx *c, const double *x, const double *y, const double *z) {
+  if (!c || !x || !y || !z) return PMX_ERR_ARG;
+  return set_pos_common(c, x, y, z, hipMemcpyDeviceToDevice);
+}
+
+int pmx_set_xmod(pmx_ctx *c, const double *x, const double *y, const double *z) {
+  if (!c) return PMX_ERR_ARG;
+  if (!x) {
+    c->xmod_given = false;
+    return PMX_OK;
+  }
+  const size_t n = (size_t)c->s.N * sizeof(double);
+  HIPCHK(hipMemcpy(c->xm, x, n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ym, y, n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->zm, z, n, hipMemcpyHostToDevice));
+  c->xmod_given = true;
+  return PMX_OK;
+}
+
+int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
+  if (!c || !out) return PMX_ERR_ARG;
+  if (!c->have_pos) {
+    set_err(c, "pmx_energy_control: no positions set");
+    return PMX_ERR_STATE;
+  }
+  Setup &s = c->s;
+  const int N = s.N, P = s.P;
+  const bool pimd = (flags & PMX_EVAL_PIMD) != 0;
+  if (P > 1 && !pimd) {
+    set_err(c, "pmx_energy_control: more than one bead needs PMX_EVAL_PIMD");
+    return PMX_ERR_ARG;
+  }
+  const size_t NP = (size_t)N * P;
+  HIPCHK(hipSetDevice(c->device));
+  memset(out, 0, sizeof(*out));
+  hipStream_t st = c->st;
+  HIPCHK(hipEventRecord(c->ev[0], st));
+  // --- initial (energy_control_initial.c:168-208)
+  HIPCHK(hipMemsetAsync(c->fx, 0, NP * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(c->fy, 0, NP * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(c->fz, 0, NP * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(c->acc, 0, (size_t)P * ACC_NUM * sizeof(double), st));
+  HIPCHK(hipMemsetAsync(c->ake, 0, sizeof(double), st));
+  if (P > 1 && !c->xmod_given) {
+    hipLaunchKernelGGL(k_centroid, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm);
+    HIPCHK(hipGetLastError());
+  }
+  // --- neighbour list (nbr_list_control.c:32-118)
+  HIPCHK(hipMemsetAsync(c->maxes, 0, 2 * sizeof(unsigned long long), st));
+  hipLaunchKernelGGL(k_list_check, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->x0, c->y0, c->z0,
+                     c->b0x, c->b0y, c->b0z, c->maxes);
+  HIPCHK(hipGetLastError());
+  unsigned long long mx[2];
+  HIPCHK(hipMemcpyAsync(mx, c->maxes, sizeof mx, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  double dmax, smax;
+  memcpy(&dmax, &mx[0], sizeof(double));
+  memcpy(&smax, &mx[1], sizeof(double));
+  const double spread_now = (P > 1) ? sqrt(smax) : 0.0;
+  if (P > 1) {  // get_cut_skin with the current spread (nbr_list_control.c:249-283)
+    for (int t = 0; t < s.npair; t++) {
+      const double cs = s.cutoff[t] + c->par.skin + spread_now;
+      c->pt.cutskin2[t] = cs * cs;
+    }
+  }
+  bool rebuild = !c->have_list || (flags & PMX_EVAL_FORCE_LIST);
+  if (!rebuild) {
+    const double dspread = (P > 1) ? std::max(0.0, spread_now - c->spread) : 0.0;
+    rebuild = sqrt(dmax) > c->par.skin / 2.0 + dspread;
+  }
+  if (rebuild) {
+    if (P > 1) c->spread = spread_now;
+    HIPCHK(hipMemcpyAsync(c->x0, c->b0x, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->y0, c->b0y, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->z0, c->b0z, N * sizeof(double), hipMemcpyDeviceToDevice, st));
+    int rc = build_list(c);
+    if (rc) return rc;
+    c->have_list = true;
+  }
+  out->list_rebuilt = rebuild;
+  out->nver = c->nver;
+  out->spread = c->spread;
+  HIPCHK(hipEventRecord(c->ev[1], st));
+  // --- real space (force_control -> force_verlst -> force_npol)
+  {
+    PairArgs a{};
+    a.N = N;
+    a.P = P;
+    a.cap = c->cap;
+    a.nsplin = s.nsplin;
+    a.pimd = (P > 1);
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.xm = c->xm;
+    a.ym = c->ym;
+    a.zm = c->zm;
+    a.type = c->type;
+    a.q = c->q;
+    a.jlist = c->jforce;
+    a.cnt = c->nforce;
+    a.tab_lj = c->tab_lj;
+    a.tab_c = c->tab_c;
+    a.pt = c->pt;
+    a.cell = c->cell;
+    a.fx = c->fx;
+    a.fy = c->fy;
+    a.fz = c->fz;
+    a.acc = c->acc;
+    HIPCHK(launch_pair(a, st));
+  }
+  HIPCHK(hipEventRecord(c->ev[2], st));
+  // --- reciprocal space (energy_control_inter_recip.c:119-355)
+  const int nchrg = (int)s.ichrg.size();
+  if (c->pme && !(flags & PMX_EVAL_NO_RECIP)) {
+    PmeArgs a{};
+    a.N = N;
+    a.P = P;
+    a.nchrg = nchrg;
+    a.n_interp = c->par.n_interp;
+    for (int d = 0; d < 3; d++) a.K[d] = s.ngrid[d];
+    a.gstride = c->gstride;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.q = c->q;
+    a.ichrg = c->ichrg;
+    a.cell = c->cell;
+    a.grid = c->grid;
+    a.fx = c->fx;
+    a.fy = c->fy;
+    a.fz = c->fz;
+    HIPCHK(hipMemsetAsync(c->grid, 0, c->gstride * P * sizeof(double), st));
+    hipLaunchKernelGGL(k_pme_spread, dim3(blocks((size_t)nchrg * P)), dim3(256), 0, st, a);
+    HIPCHK(hipGetLastError());
+    FftArgs f{};
+    f.P = P;
+    for (int d = 0; d < 3; d++) {
+      f.K[d] = s.ngrid[d];
+      f.logK[d] = c->logK[d];
+      f.tw[d] = c->tw[d];
+    }
+    f.gstride = c->gstride;
+    f.sstride = c->sstride;
+    f.grid = c->grid;
+    f.spec = c->spec;
+    f.infl = c->infl;
+    f.falp2 = 4.0 * s.alp_ewd * s.alp_ewd;
+    f.cell = c->cell;
+    f.acc = c->acc;
+    const int Ka = s.ngrid[0], Kb = s.ngrid[1], Kc = s.ngrid[2], Ha = Ka / 2 + 1;
+    hipLaunchKernelGGL(k_fft_fwd_ab, dim3(Kc, P), dim3(256), (size_t)Ka * Kb * sizeof(double2), st, f);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_fft_c_conv, dim3(Kb, P), dim3(256), (size_t)Kc * Ha * sizeof(double2), st, f);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_fft_inv_ba, dim3(Kc, P), dim3(256), (size_t)Ka * Kb * sizeof(double2), st, f);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pme_gather, dim3(blocks((size_t)nchrg * P)), dim3(256), 0, st, a);
+    HIPCHK(hipGetLastError());
+  }
+  IntraArgs ia = intra_args(c, pimd);
+  if (nchrg > 0 && ia.necor > 0) {
+    hipLaunchKernelGGL(k_ecor, dim3(blocks((size_t)ia.necor * P)), dim3(256), 0, st, ia);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(c->ev[3], st));
+  // --- intramolecular (energy_control_intra.c:33-331)
+  if (c->nbond) {
+    hipLaunchKernelGGL(k_bond, dim3(blocks((size_t)c->nbond * P)), dim3(256), 0, st, ia);
+    HIPCHK(hipGetLastError());
+  }
+  if (c->nbend) {
+    hipLaunchKernelGGL(k_bend, dim3(blocks((size_t)c->nbend * P)), dim3(256), 0, st, ia);
+    HIPCHK(hipGetLastError());
+  }
+  // --- PIMD: f/P and virial correction, then ghosts, then modes
+  if (pimd) {
+    FinishArgs a{};
+    a.N = N;
+    a.P = P;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.xm = c->xm;
+    a.ym = c->ym;
+    a.zm = c->zm;
+    a.fx = c->fx;
+    a.fy = c->fy;
+    a.fz = c->fz;
+    a.acc = c->acc;
+    hipLaunchKernelGGL(k_pimd_finish, dim3(blocks(NP)), dim3(256), 0, st, a);
+    HIPCHK(hipGetLastError());
+  }
+  if (c->nghost) {
+    hipLaunchKernelGGL(k_ghost_force, dim3(blocks((size_t)c->nghost * P)), dim3(256), 0, st, c->nghost, P,
+                       c->gatom, c->gncomp, c->gcomp, c->gcoef, c->fx, c->fy, c->fz);
+    HIPCHK(hipGetLastError());
+  }
+  const bool modes = pimd && (flags & PMX_EVAL_MODE_FORCES) && P > 1;
+  if (modes) {
+    ModeArgs a{};
+    a.N = N;
+    a.P = P;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.fx = c->fx;
+    a.fy = c->fy;
+    a.fz = c->fz;
+    a.prekf = c->prekf;
+    a.veig = c->veig;
+    a.cs = c->cs;
+    a.ake = c->ake;
+    hipLaunchKernelGGL(k_mode_forces, dim3(blocks((size_t)N * 3)), dim3(256), 0, st, a);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(c->ev[4], st));
+  HIPCHK(hipMemcpyAsync(c->acc_h.data(), c->acc, c->acc_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  double ake = 0.0;
+  HIPCHK(hipMemcpyAsync(&ake, c->ake, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  {
+    float t;
+    for (int k = 0; k < 4; k++) {
+      hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]);
+      c->ms[k] = t;
+    }
+    hipEventElapsedTime(&t, c->ev[0], c->ev[4]);
+    c->ms[4] = t;
+  }
+  // --- assemble (energy_control_inter_real.c:188-197, _recip.c:340-347, _intra.c)
+  const double dP = (double)P;
+  double sum[ACC_NUM] = {0};
+  for (int b = 0; b < P; b++)
+    for (int k = 0; k < ACC_NUM; k++) sum[k] += c->acc_h[(size_t)b * ACC_NUM + k];
+  out->vvdw = sum[ACC_VVDW] / dP;
+  out->vcoul = sum[ACC_VCOUL] / dP;
+  out->vreal = out->vvdw + out->vcoul;
+  out->vlong = s.clong / s.vol;
+  out->vrecip = sum[ACC_VRECIP] / dP;
+  out->vecor = sum[ACC_VECOR] / dP;
+  out->vbond = sum[ACC_VBOND] / dP;
+  out->vbend = sum[ACC_VBEND] / dP;
+  double pv[10] = {0};
+  pv[1] = sum[ACC_P11];
+  pv[5] = sum[ACC_P22];
+  pv[9] = sum[ACC_P33];
+  pv[2] = pv[4] = sum[ACC_P12];
+  pv[3] = pv[7] = sum[ACC_P13];
+  pv[6] = pv[8] = sum[ACC_P23];
+  {
+    const double amult = (c->par.ishift == PMX_SHIFT_SWIT) ? 1.0 : 2.0;
+    const double pd = (s.clong / s.vol) * amult * dP;
+    pv[1] += pd;
+    pv[5] += pd;
+    pv[9] += pd;
+  }
+  if (nchrg > 0) {
+    // ewald3d_selfbgr (ewald3d_self_bgr.c:29-83) over all sites
+    const double qq = s.qsq, qs = s.qsum;
+    out->vself = -qq * (s.alp_ewd) * (s.self_erf) / sqrt(M_PI);
+    out->vbgr = -0.5 * qs * qs * M_PI / (s.alp_ewd * s.alp_ewd * s.vol);
+    pv[1] += out->vbgr * dP;
+    pv[5] += out->vbgr * dP;
+    pv[9] += out->vbgr * dP;
+  }
+  out->vintert = out->vreal + out->vlong + out->vrecip + out->vself + out->vbgr + out->vecor;
+  out->vintrat = out->vbond + out->vbend;
+  if (pimd) {
+    for (int k = 1; k <= 9; k++) pv[k] /= dP;
+    if (P > 1) {
+      pv[1] += sum[ACC_C11];
+      pv[5] += sum[ACC_C22];
+      pv[9] += sum[ACC_C33];
+      pv[2] += sum[ACC_C12];
+      pv[4] += sum[ACC_C21];
+      pv[3] += sum[ACC_C13];
+      pv[7] += sum[ACC_C31];
+      pv[6] += sum[ACC_C23];
+      pv[8] += sum[ACC_C32];
+      const double T = c->par.temperature > 0 ? c->par.temperature : 300.0;
+      const double beta = BOLTZ / T;
+      out->pi_ke_vir = 0.5 * (3.0 * ((double)(N - c->nghost)) / beta + sum[ACC_AKVIR]);
+    }
+  }
+  for (int k = 0; k < 9; k++) {
+    out->pvten[k] = pv[k + 1];
+    out->pvten_tot[k] = pv[k + 1];
+  }
+  if (modes) {
+    const double T = c->par.temperature > 0 ? c->par.temperature : 300.0;
+    const double tau = (BOLTZ / T) / dP;
+    out->kin_harm = ake;
+    out->pi_ke_prim = 1.5 * ((double)(N - c->nghost)) / tau - ake;
+  }
+  return PMX_OK;
+}
+
+int pmx_get_forces(pmx_ctx *c, double *fx, double *fy, double *fz) {
+  if (!c || !fx || !fy || !fz) return PMX_ERR_ARG;
+  Setup &s = c->s;
+  const size_t NP = (size_t)s.N * s.P;
+  HIPCHK(hipSetDevice(c->device));
+  double *src[3] = {c->fx, c->fy, c->fz};
+  double *dst[3] = {fx, fy, fz};
+  for (int d = 0; d < 3; d++) {
+    hipLaunchKernelGGL(k_atom_to_bead, dim3(blocks(NP)), dim3(256), 0, c->st, s.N, s.P, src[d], c->stage);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst[d], c->stage, NP * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+  return PMX_OK;
+}
+
+int pmx_get_forces_device(pmx_ctx *c, const double **fx, const double **fy, const double **fz) {
+  if (!c) return PMX_ERR_ARG;
+  *fx = c->fx;
+  *fy = c->fy;
+  *fz = c->fz;
+  return PMX_OK;
+}
+
+int pmx_get_fxt(pmx_ctx *c, double *fx, double *fy, double *fz) {
+  // With RESPA off every force term enters fxt with the weight it has in f
+  // (force_npol.c:1222-1283, ewald3d_pme.c:568-573, bond.c:272-285), so the
+  // virial-estimator forces equal the forces.
+  return pmx_get_forces(c, fx, fy, fz);
+}
+
+int pmx_get_bead_energies(pmx_ctx *c, double *out) {
+  if (!c || !out) return PMX_ERR_ARG;
+  for (int b = 0; b < c->s.P; b++) {
+    const double *a = &c->acc_h[(size_t)b * ACC_NUM];
+    out[4 * b + 0] = a[ACC_VVDW] + a[ACC_VCOUL];
+    out[4 * b + 1] = a[ACC_VRECIP];
+    out[4 * b + 2] = a[ACC_VECOR];
+    out[4 * b + 3] = a[ACC_VBOND] + a[ACC_VBEND];
+  }
+  return PMX_OK;
+}
+
+int pmx_get_verlet_list(pmx_ctx *c, int *nter, int *jver_off, int *jter, int cap, int *nver) {
+  if (!c || !nter || !jver_off || !nver) return PMX_ERR_ARG;
+  if (!c->have_list) {
+    set_err(c, "no Verlet list yet");
+    return PMX_ERR_STATE;
+  }
+  const int N = c->s.N;
+  HIPCHK(hipMemcpy(nter, c->nfull, N * sizeof(int), hipMemcpyDeviceToHost));
+  long tot = 0;
+  for (int i = 0; i < N; i++) {
+    jver_off[i] = (int)tot;
+    tot += nter[i];
+  }
+  *nver = (int)tot;
+  if (cap < tot || !jter) return cap < tot ? PMX_ERR_RANGE : PMX_OK;
+  std::vector<int> row(c->cap);
+  for (int i = 0; i < N; i++) {
+    if (nter[i] == 0) continue;
+    HIPCHK(hipMemcpy(jter + jver_off[i], c->jfull + (size_t)i * c->cap, nter[i] * sizeof(int),
+                     hipMemcpyDeviceToHost));
+  }
+  return PMX_OK;
+}
+
+int pmx_get_nktot(pmx_ctx *c, int *nktot, int *ngrid) {
+  if (!c || !nktot || !ngrid) return PMX_ERR_ARG;
+  *nktot = c->s.nktot;
+  for (int d = 0; d < 3; d++) ngrid[d] = c->s.ngrid[d];
+  return PMX_OK;
+}
+
+int pmx_get_kvectors(pmx_ctx *c, int *ka, int *kb, int *kc, double *bw, int cap) {
+  if (!c) return PMX_ERR_ARG;
+  if (cap < c->s.nktot) return PMX_ERR_RANGE;
+  for (int k = 0; k < c->s.nktot; k++) {
+    ka[k] = c->s.ka[k];
+    kb[k] = c->s.kb[k];
+    kc[k] = c->s.kc[k];
+    bw[k] = c->s.bweight[k];
+  }
+  return PMX_OK;
+}
+
+int pmx_get_spline(pmx_ctx *c, int which, int ipair, double *out, int cap) {
+  if (!c || !out) return PMX_ERR_ARG;
+  const Setup &s = c->s;
+  if (cap < s.nsplin) return PMX_ERR_RANGE;
+  if (which >= 0 && which <= 3) {
+    if (ipair < 0 || ipair >= s.npair) return PMX_ERR_ARG;
+    for (int k = 0; k < s.nsplin; k++) out[k] = s.tab[((size_t)ipair * s.nsplin + k) * 4 + which];
+    return PMX_OK;
+  }
+  if (which == 4 || which == 5) {
+    for (int k = 0; k < s.nsplin; k++) out[k] = s.ecor_tab[2 * k + (which - 4)];
+    return PMX_OK;
+  }
+  return PMX_ERR_ARG;
+}
+
+int pmx_get_setup_scalars(pmx_ctx *c, double *o) {
+  if (!c || !o) return PMX_ERR_ARG;
+  const Setup &s = c->s;
+  o[0] = s.alp_ewd;
+  o[1] = s.ecut;
+  o[2] = s.vol;
+  o[3] = s.clong;
+  o[4] = s.clong_res;
+  o[5] = s.self_erf;
+  o[6] = s.e_rmin;
+  o[7] = s.e_rmax;
+  o[8] = s.e_dr;
+  o[9] = s.nktot;
+  for (int d = 0; d < 3; d++) {
+    o[10 + d] = s.ngrid[d];
+    o[13 + d] = s.kmaxv[d + 1];
+  }
+  o[16] = (double)s.e1.size();
+  o[17] = s.nexcl;
+  o[18] = s.cutoff_max;
+  return PMX_OK;
+}
+
+int pmx_get_timings(pmx_ctx *c, double *ms5) {
+  if (!c || !ms5) return PMX_ERR_ARG;
+  for (int k = 0; k < 5; k++) ms5[k] = c->ms[k];
+  return PMX_OK;
+}
+
+int pmx_get_pair_count(pmx_ctx *c, double *npair) {
+  if (!c || !npair) return PMX_ERR_ARG;
+  double s = 0.0;
+  for (int b = 0; b < c->s.P; b++) s += c->acc_h[(size_t)b * ACC_NUM + ACC_NPAIR];
+  *npair = s;
+  return PMX_OK;
+}
+
+int pmx_bond_coef_harm(double fk, double *c7) {
+  if (!c7) return PMX_ERR_ARG;
+  bond_coef_harm(fk, c7);
+  return PMX_OK;
+}
+
+int pmx_bond_coef_morse(double d0, double alpha, double *c7) {
+  if (!c7) return PMX_ERR_ARG;
+  bond_coef_morse(d0, alpha, c7);
+  return PMX_OK;
+}
+
+int pmx_bend_coef_harm(double fk, double theta0, double *c7, double *s7) {
+  if (!c7 || !s7) return PMX_ERR_ARG;
+  bend_coef_harm(fk, theta0, c7, s7);
+  return PMX_OK;
+}
+
+}  // extern "C"

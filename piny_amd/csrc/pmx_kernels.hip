@@ -1,0 +1,920 @@
+// HIP kernels of the force evaluation, written for gfx950 (wave64).
+//
+// Device layout: positions and forces are atom-major with the bead index
+// fastest, x[i*P + b], so that the lanes of a wavefront that handle the P
+// beads of one pair read and write contiguous memory.  Reference citations
+// name the PINY routine whose arithmetic each kernel restates.
+#include <hip/hip_runtime.h>
+
+#include "pmx_kernels.h"
+
+namespace pmx {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+__device__ __forceinline__ int nint_piny(double x) {
+  // NINT with SIMP_NINT, defines.h:96-97
+  return (int)((x) >= 0.0 ? ((x) + 0.5) : ((x)-0.5));
+}
+
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// Sum of one value over the lanes that hold the same bead (lane mod P) and
+// one atomic add per bead into dst[bead*stride].  Every lane of the wave must
+// call it.  P must divide 64; other bead counts take the per-lane path.
+__device__ __forceinline__ void bead_reduce_add(double v, int P, int bead, bool active, double *dst,
+                                                int stride) {
+  if (P <= 64 && (64 % P) == 0) {
+    v = active ? v : 0.0;
+    for (int off = 32; off >= P; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63;
+    if (lane < P && v != 0.0) atomic_add(dst + (size_t)lane * stride, v);
+  } else if (active && v != 0.0) {
+    atomic_add(dst + (size_t)bead * stride, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion
+
+__global__ void k_bead_to_atom(int N, int P, const double *__restrict__ src, double *__restrict__ dst) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * P) return;
+  const int i = (int)(t / P), b = (int)(t % P);
+  dst[t] = src[(size_t)b * N + i];
+}
+
+__global__ void k_atom_to_bead(int N, int P, const double *__restrict__ src, double *__restrict__ dst) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * P) return;
+  const int b = (int)(t / N), i = (int)(t % N);
+  dst[t] = src[(size_t)i * P + b];
+}
+
+// get_ghost_pos, ghost_control.c:28-64 (same summation order, no FMA)
+__global__ void k_ghost_pos(int ng, int P, const int *__restrict__ gatom, const int *__restrict__ gncomp,
+                            const int *__restrict__ gcomp, const double *__restrict__ gcoef, double *x,
+                            double *y, double *z) {
+#pragma clang fp contract(off)
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ng * P) return;
+  const int g = t / P, b = t % P;
+  const size_t ig = (size_t)gatom[g] * P + b;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int k = 0; k < gncomp[g]; k++) {
+    const size_t ia = (size_t)gcomp[4 * g + k] * P + b;
+    const double c = gcoef[4 * g + k];
+    sx += c * x[ia];
+    sy += c * y[ia];
+    sz += c * z[ia];
+  }
+  x[ig] = sx;
+  y[ig] = sy;
+  z[ig] = sz;
+}
+
+// bead centroid as the mode-1 coordinate of the normal-mode transform
+// (control_pimd_trans.c:175-181 takes xmod from mode 1)
+__global__ void k_centroid(int N, int P, const double *__restrict__ x, const double *__restrict__ y,
+                           const double *__restrict__ z, double *xm, double *ym, double *zm) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int b = 0; b < P; b++) {
+    sx += x[(size_t)i * P + b];
+    sy += y[(size_t)i * P + b];
+    sz += z[(size_t)i * P + b];
+  }
+  xm[i] = sx / P;
+  ym[i] = sy / P;
+  zm[i] = sz / P;
+}
+
+// Bead-1 copy, largest displacement since the last list build
+// (check_ver_list, nbr_list_control.c:182-239) and bead spread
+// (get_pimd_spread, pimd_estimators.c:121-222).  Maxima of squares are
+// reduced as the bit patterns of non-negative doubles.
+__global__ void k_list_check(int N, int P, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, const double *__restrict__ x0,
+                             const double *__restrict__ y0, const double *__restrict__ z0, double *b0x,
+                             double *b0y, double *b0z, unsigned long long *maxes) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dmax = 0.0, smax = 0.0;
+  if (i < N) {
+    const double xi = x[(size_t)i * P], yi = y[(size_t)i * P], zi = z[(size_t)i * P];
+    b0x[i] = xi;
+    b0y[i] = yi;
+    b0z[i] = zi;
+    const double dx = xi - x0[i], dy = yi - y0[i], dz = zi - z0[i];
+    dmax = dx * dx + dy * dy + dz * dz;
+    for (int b = 1; b < P; b++) {
+      const double ex = x[(size_t)i * P + b] - xi, ey = y[(size_t)i * P + b] - yi, ez = z[(size_t)i * P + b] - zi;
+      const double r2 = ex * ex + ey * ey + ez * ez;
+      smax = fmax(smax, r2);
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+    smax = fmax(smax, __shfl_xor(smax, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&maxes[0], (unsigned long long)__double_as_longlong(dmax));
+    atomicMax(&maxes[1], (unsigned long long)__double_as_longlong(smax));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Verlet list: nolst_ver_gen + verlist_create (verlist_control.c:134-441,
+// verlist_create.c:223-503).  One wavefront per site i scans j = 0..i-1 in
+// ascending chunks of 64; a ballot keeps the reference's order, so each
+// per-site segment comes out sorted.  Two lists are written in ELL form
+// (row i at i*cap): every pair with r^2 < cutskin^2 (the reference's list)
+// and the subset that can interact (LJ table non-null or q_i q_j != 0),
+// which the force kernel walks.  No FMA contraction: the r^2 test must see
+// the reference's bits.
+__global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= a.N) return;
+  const double xi = a.bx[i], yi = a.by[i], zi = a.bz[i];
+  double xmi = 0.0, ymi = 0.0, zmi = 0.0;
+  if (a.pimd) {
+    xmi = a.xm[i];
+    ymi = a.ym[i];
+    zmi = a.zm[i];
+  }
+  const int ti = a.type[i];
+  const double qi = a.q[i];
+  const int eo = a.ex_off[i], en = a.ex_num[i];
+  const Cell &c = a.cell;
+  int nfull = 0, nforce = 0;
+  int *rowf = a.jfull + (size_t)i * a.cap;
+  int *rowi = a.jforce + (size_t)i * a.cap;
+  for (int j0 = 0; j0 < i; j0 += 64) {
+    const int j = j0 + lane;
+    bool keep = false, inter = false;
+    if (j < i) {
+      bool excl = false;
+      for (int k = 0; k < en; k++) excl |= (a.ex_j[eo + k] == j);
+      if (!excl) {
+        double dx = xi - a.bx[j], dy = yi - a.by[j], dz = zi - a.bz[j];
+        if (!a.pimd) {
+          if (c.cubic) {
+            const double asx = dx * c.hi[0], asy = dy * c.hi[4], asz = dz * c.hi[8];
+            const double sx = asx - nint_piny(asx), sy = asy - nint_piny(asy), sz = asz - nint_piny(asz);
+            dx = sx * c.h[0];
+            dy = sy * c.h[4];
+            dz = sz * c.h[8];
+          } else {
+            const double asx = dx * c.hi[0] + dy * c.hi[3] + dz * c.hi[6];
+            const double asy = dx * c.hi[1] + dy * c.hi[4] + dz * c.hi[7];
+            const double asz = dx * c.hi[2] + dy * c.hi[5] + dz * c.hi[8];
+            const double sx = asx - nint_piny(asx), sy = asy - nint_piny(asy), sz = asz - nint_piny(asz);
+            dx = sx * c.h[0] + sy * c.h[3] + sz * c.h[6];
+            dy = sx * c.h[1] + sy * c.h[4] + sz * c.h[7];
+            dz = sx * c.h[2] + sy * c.h[5] + sz * c.h[8];
+          }
+        } else {
+          const double dxs = xmi - a.xm[j], dys = ymi - a.ym[j], dzs = zmi - a.zm[j];
+          if (c.cubic) {
+            const double sx = -nint_piny(dxs * c.hi[0]), sy = -nint_piny(dys * c.hi[4]),
+                         sz = -nint_piny(dzs * c.hi[8]);
+            dx += (sx * c.h[0]);
+            dy += (sy * c.h[4]);
+            dz += (sz * c.h[8]);
+          } else {
+            const double asx = dxs * c.hi[0] + dys * c.hi[3] + dzs * c.hi[6];
+            const double asy = dxs * c.hi[1] + dys * c.hi[4] + dzs * c.hi[7];
+            const double asz = dxs * c.hi[2] + dys * c.hi[5] + dzs * c.hi[8];
+            const double sx = -nint_piny(asx), sy = -nint_piny(asy), sz = -nint_piny(asz);
+            dx += (sx * c.h[0] + sy * c.h[3] + sz * c.h[6]);
+            dy += (sx * c.h[1] + sy * c.h[4] + sz * c.h[7]);
+            dz += (sx * c.h[2] + sy * c.h[5] + sz * c.h[8]);
+          }
+        }
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const int tj = a.type[j];
+        const int t = pair_index(a.pt.ntype, ti, tj);
+        keep = r2 < a.pt.cutskin2[t];
+        inter = keep && (a.pt.lj[t] || (qi * a.q[j] != 0.0));
+      }
+    }
+    const unsigned long long mk = __ballot(keep), mi = __ballot(inter);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    if (keep) {
+      const int p = nfull + __popcll(mk & below);
+      if (p < a.cap) rowf[p] = j;
+    }
+    if (inter) {
+      const int p = nforce + __popcll(mi & below);
+      if (p < a.cap) rowi[p] = j;
+    }
+    nfull += __popcll(mk);
+    nforce += __popcll(mi);
+  }
+  if (lane == 0) {
+    a.nfull[i] = nfull;
+    a.nforce[i] = nforce;
+    if (nfull > a.cap) atomicMax(a.overflow, nfull);
+    atomicAdd(a.overflow + 1, nfull);  // pair count of the reference list
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Real-space pair forces over the interacting list: force_npol
+// (force_npol.c:28-328) with shave_skin_opt on and RESPA off.  A wavefront
+// owns one site i at a time; its 64 lanes are (slot, bead) with B beads per
+// slot, so the B lanes of a slot evaluate the same pair for B beads and
+// touch contiguous memory.  j forces are scattered with FP64 atomics, the i
+// force is reduced over slots with shuffles.  FP64 throughout.
+
+__device__ __forceinline__ double spline4(const double2 *__restrict__ tab, int k, double del, bool second) {
+  // npol_vspl_fetch, force_npol.c:339-376 (k is PINY's 1-based index)
+  const double2 tm2 = tab[k - 3], tm1 = tab[k - 2], t0 = tab[k - 1], tp1 = tab[k];
+  const double fm2 = second ? tm2.y : tm2.x, fm1 = second ? tm1.y : tm1.x;
+  const double c0 = second ? t0.y : t0.x, fp1 = second ? tp1.y : tp1.x;
+  const double oneth = 1.0 / 3.0, onesi = 1.0 / 6.0;
+  const double c1 = oneth * fp1 + 0.5 * c0 - fm1 + onesi * fm2;
+  const double c2 = 0.5 * fp1 - c0 + 0.5 * fm1;
+  const double c3 = onesi * fp1 - 0.5 * c0 + 0.5 * fm1 - onesi * fm2;
+  return (c0 + del * (c1 + del * (c2 + del * c3)));
+}
+
+template <int B>
+__global__ void __launch_bounds__(256) k_pair_f64(PairArgs a) {
+  constexpr int S = 64 / B;
+  const int lane = threadIdx.x & 63;
+  const int slot = lane / B;
+  const int bead = (lane % B) + blockIdx.y * B;
+  const bool bact = bead < a.P;
+  const int P = a.P;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
+  const Cell &c = a.cell;
+  double vlj = 0.0, vc = 0.0, p11 = 0.0, p22 = 0.0, p33 = 0.0, p12 = 0.0, p13 = 0.0, p23 = 0.0, npr = 0.0;
+  for (int i = wave; i < a.N; i += nwave) {
+    const size_t ib = (size_t)i * P + (bact ? bead : 0);
+    const double xi = a.x[ib], yi = a.y[ib], zi = a.z[ib];
+    const int ti = a.type[i];
+    const double qi = a.q[i];
+    double xmi = 0.0, ymi = 0.0, zmi = 0.0;
+    if (a.pimd) {
+      xmi = a.xm[i];
+      ymi = a.ym[i];
+      zmi = a.zm[i];
+    }
+    const int cnt = a.cnt[i];
+    const int *row = a.jlist + (size_t)i * a.cap;
+    double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+    for (int k = slot; k < cnt; k += S) {
+      const int j = row[k];
+      const size_t jb = (size_t)j * P + (bact ? bead : 0);
+      double dx = xi - a.x[jb], dy = yi - a.y[jb], dz = zi - a.z[jb];
+      if (!a.pimd) {  // period, period.c:27-111
+        if (c.cubic) {
+          const double asx = dx * c.hi[0], asy = dy * c.hi[4], asz = dz * c.hi[8];
+          dx = (asx - nint_piny(asx)) * c.h[0];
+          dy = (asy - nint_piny(asy)) * c.h[4];
+          dz = (asz - nint_piny(asz)) * c.h[8];
+        } else {
+          const double asx = dx * c.hi[0] + dy * c.hi[3] + dz * c.hi[6];
+          const double asy = dx * c.hi[1] + dy * c.hi[4] + dz * c.hi[7];
+          const double asz = dx * c.hi[2] + dy * c.hi[5] + dz * c.hi[8];
+          const double sx = asx - nint_piny(asx), sy = asy - nint_piny(asy), sz = asz - nint_piny(asz);
+          dx = sx * c.h[0] + sy * c.h[3] + sz * c.h[6];
+          dy = sx * c.h[1] + sy * c.h[4] + sz * c.h[7];
+          dz = sx * c.h[2] + sy * c.h[5] + sz * c.h[8];
+        }
+      } else {  // period_pimd, period.c:121-206: image of the xmod pair
+        const double dxs = xmi - a.xm[j], dys = ymi - a.ym[j], dzs = zmi - a.zm[j];
+        if (c.cubic) {
+          dx += -nint_piny(dxs * c.hi[0]) * c.h[0];
+          dy += -nint_piny(dys * c.hi[4]) * c.h[4];
+          dz += -nint_piny(dzs * c.hi[8]) * c.h[8];
+        } else {
+          const double sx = -nint_piny(dxs * c.hi[0] + dys * c.hi[3] + dzs * c.hi[6]);
+          const double sy = -nint_piny(dxs * c.hi[1] + dys * c.hi[4] + dzs * c.hi[7]);
+          const double sz = -nint_piny(dxs * c.hi[2] + dys * c.hi[5] + dzs * c.hi[8]);
+          dx += (sx * c.h[0] + sy * c.h[3] + sz * c.h[6]);
+          dy += (sx * c.h[1] + sy * c.h[4] + sz * c.h[7]);
+          dz += (sx * c.h[2] + sy * c.h[5] + sz * c.h[8]);
+        }
+      }
+      const double r = sqrt(dx * dx + dy * dy + dz * dz);
+      const int t = pair_index(a.pt.ntype, ti, a.type[j]);
+      if (bact && r < a.pt.cut[t]) {
+        const double rmin = a.pt.rmin[t], dri = a.pt.dri[t];
+        int kk = (int)(((r - rmin) * dri) + 0.5) + 3;
+        kk = min(kk, a.nsplin - 2);
+        kk = max(kk, 3);
+        const double rsp = ((double)(kk - 3)) * a.pt.dr[t] + rmin;
+        const double del = (r - rsp) * dri;
+        const size_t toff = (size_t)t * a.nsplin;
+        double dvt = 0.0;
+        if (a.pt.lj[t]) {
+          vlj += spline4(a.tab_lj + toff, kk, del, false);
+          dvt += spline4(a.tab_lj + toff, kk, del, true);
+        }
+        const double q2 = qi * a.q[j];
+        if (q2 != 0.0) {
+          vc += q2 * spline4(a.tab_c + toff, kk, del, false);
+          dvt += q2 * spline4(a.tab_c + toff, kk, del, true);
+        }
+        const double fx1 = dx * dvt, fy1 = dy * dvt, fz1 = dz * dvt;
+        fxi += fx1;
+        fyi += fy1;
+        fzi += fz1;
+        atomic_add(a.fx + jb, -fx1);
+        atomic_add(a.fy + jb, -fy1);
+        atomic_add(a.fz + jb, -fz1);
+        p11 += dx * fx1;
+        p22 += dy * fy1;
+        p33 += dz * fz1;
+        p12 += dx * fy1;
+        p13 += dx * fz1;
+        p23 += dy * fz1;
+        npr += 1.0;
+      }
+    }
+    for (int off = 32; off >= B; off >>= 1) {
+      fxi += __shfl_xor(fxi, off, 64);
+      fyi += __shfl_xor(fyi, off, 64);
+      fzi += __shfl_xor(fzi, off, 64);
+    }
+    if (slot == 0 && bact && cnt > 0) {
+      atomic_add(a.fx + ib, fxi);
+      atomic_add(a.fy + ib, fyi);
+      atomic_add(a.fz + ib, fzi);
+    }
+  }
+  double vals[9] = {vlj, vc, p11, p22, p33, p12, p13, p23, npr};
+  const int slots[9] = {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR};
+#pragma unroll
+  for (int v = 0; v < 9; v++) {
+    double s = vals[v];
+    for (int off = 32; off >= B; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (slot == 0 && bact && s != 0.0) atomic_add(a.acc + (size_t)bead * ACC_NUM + slots[v], s);
+  }
+}
+
+template __global__ void k_pair_f64<1>(PairArgs);
+template __global__ void k_pair_f64<2>(PairArgs);
+template __global__ void k_pair_f64<4>(PairArgs);
+template __global__ void k_pair_f64<8>(PairArgs);
+template __global__ void k_pair_f64<16>(PairArgs);
+template __global__ void k_pair_f64<32>(PairArgs);
+template __global__ void k_pair_f64<64>(PairArgs);
+
+// ---------------------------------------------------------------------------
+// PME.  B-spline weights of one charged site and one axis,
+// ewald3d_pme.c:241-337 (spread) and :516-530 (derivatives).
+struct Bsp {
+  double mn[MAX_INTERP + 1], dmn[MAX_INTERP + 1];
+  int ig[MAX_INTERP + 1];  // 0-based grid index of point j
+};
+
+__device__ __forceinline__ void bspline_axis(double s, int K, int n, bool deriv, Bsp &b) {
+  double t = s - nint_piny((s - 0.5));
+  const double sc = t * (double)K;
+  const int it = (int)(sc);
+  const double frac = sc - (double)(it);
+  double u[MAX_INTERP + 1];
+  for (int j = 1; j <= n; j++) {
+    u[j] = frac + (double)(j - 1);
+    int ia = it - (j - 2);
+    ia = (ia > 0 ? ia : K + ia);
+    if (ia > K) ia -= K;  // only for a coordinate exactly on the cell face
+    b.ig[j] = ia - 1;
+  }
+  double *mn = b.mn, *dmn = b.dmn;
+  mn[1] = 1.0 - fabs(u[1] - 1.0);
+  mn[2] = 1.0 - fabs(u[2] - 1.0);
+  for (int j = 3; j <= n; j++) mn[j] = 0.0;
+  for (int o = 3; o <= n; o++) {
+    const double rn = (double)o, rn1 = 1.0 / ((double)(o - 1));
+    for (int j = o; j >= 2; j--) mn[j] = (u[j] * mn[j] + (rn - u[j]) * mn[j - 1]) * rn1;
+    mn[1] = u[1] * mn[1] * rn1;
+    if (deriv && o == n - 1) {
+      dmn[1] = mn[1];
+      for (int j = 2; j <= n; j++) dmn[j] = mn[j] - mn[j - 1];
+    }
+  }
+}
+
+// charge spreading (ewald3d_pme.c:342-356), one thread per (charge, bead)
+__global__ void __launch_bounds__(256) k_pme_spread(PmeArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)a.nchrg * a.P) return;
+  const int ic = (int)(t / a.P), b = (int)(t % a.P);
+  const int i = a.ichrg[ic];
+  const size_t ib = (size_t)i * a.P + b;
+  const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
+  const double *hi = a.cell.hi;
+  const double sa = x * hi[0] + y * hi[3] + z * hi[6];
+  const double sb = x * hi[1] + y * hi[4] + z * hi[7];
+  const double sc = x * hi[2] + y * hi[5] + z * hi[8];
+  Bsp ba, bb, bc;
+  const int n = a.n_interp;
+  bspline_axis(sa, a.K[0], n, false, ba);
+  bspline_axis(sb, a.K[1], n, false, bb);
+  bspline_axis(sc, a.K[2], n, false, bc);
+  const double q = a.q[i];
+  double *g = a.grid + (size_t)b * a.gstride;
+  const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
+  for (int jc = 1; jc <= n; jc++) {
+    for (int jb = 1; jb <= n; jb++) {
+      const double wbc = bb.mn[jb] * bc.mn[jc];
+      const int obc = bc.ig[jc] * Kab + bb.ig[jb] * Ka;
+      for (int ja = 1; ja <= n; ja++) atomic_add(g + obc + ba.ig[ja], ba.mn[ja] * wbc * q);
+    }
+  }
+}
+
+// force gather (ewald3d_pme.c:534-587), one thread per (charge, bead)
+__global__ void __launch_bounds__(256) k_pme_gather(PmeArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)a.nchrg * a.P) return;
+  const int ic = (int)(t / a.P), b = (int)(t % a.P);
+  const int i = a.ichrg[ic];
+  const size_t ib = (size_t)i * a.P + b;
+  const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
+  const double *hi = a.cell.hi;
+  const double sa = x * hi[0] + y * hi[3] + z * hi[6];
+  const double sb = x * hi[1] + y * hi[4] + z * hi[7];
+  const double sc = x * hi[2] + y * hi[5] + z * hi[8];
+  Bsp ba, bb, bc;
+  const int n = a.n_interp;
+  bspline_axis(sa, a.K[0], n, true, ba);
+  bspline_axis(sb, a.K[1], n, true, bb);
+  bspline_axis(sc, a.K[2], n, true, bc);
+  const double q = a.q[i];
+  const double *g = a.grid + (size_t)b * a.gstride;
+  const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
+  const double ga = (double)a.K[0], gb = (double)a.K[1], gc = (double)a.K[2];
+  double da = 0.0, db = 0.0, dc = 0.0;
+  for (int jc = 1; jc <= n; jc++) {
+    for (int jb = 1; jb <= n; jb++) {
+      const int obc = bc.ig[jc] * Kab + bb.ig[jb] * Ka;
+      const double mbc = bb.mn[jb] * bc.mn[jc], dbc = bb.dmn[jb] * bc.mn[jc], bdc = bb.mn[jb] * bc.dmn[jc];
+      for (int ja = 1; ja <= n; ja++) {
+        const double phi = g[obc + ba.ig[ja]];
+        da += ba.dmn[ja] * mbc * phi;
+        db += ba.mn[ja] * dbc * phi;
+        dc += ba.mn[ja] * bdc * phi;
+      }
+    }
+  }
+  da *= ga * q;
+  db *= gb * q;
+  dc *= gc * q;
+  a.fx[ib] -= da * hi[0] + db * hi[1] + dc * hi[2];
+  a.fy[ib] -= da * hi[3] + db * hi[4] + dc * hi[5];
+  a.fz[ib] -= da * hi[6] + db * hi[7] + dc * hi[8];
+}
+
+// --- 1D complex FFTs of power-of-two length held in LDS, cooperative over
+// the block: in-place radix-2 decimation in time after a bit reversal.
+// Line l element e lives at buf[l*dist + e*stride].  sign -1: exp(-i),
+// +1: exp(+i); unnormalised, like FFTPACK's DCFFTF/DCFFTB.
+__device__ void lds_fft_pow2(double2 *buf, int K, int logK, int nlines, int stride, int dist, int sign,
+                             const double2 *__restrict__ tw) {
+  const int nt = blockDim.x, tid = threadIdx.x;
+  for (int w = tid; w < nlines * K; w += nt) {
+    const int l = w / K, e = w % K;
+    const int r = (int)(__brev((unsigned)e) >> (32 - logK));
+    if (r > e) {
+      double2 *p = buf + (size_t)l * dist;
+      const double2 t0 = p[e * stride];
+      p[e * stride] = p[r * stride];
+      p[r * stride] = t0;
+    }
+  }
+  __syncthreads();
+  for (int len = 2, s = 1; len <= K; len <<= 1, s++) {
+    const int half = len >> 1;
+    const int tstep = K / len;
+    for (int w = tid; w < nlines * (K >> 1); w += nt) {
+      const int l = w / (K >> 1), bfly = w % (K >> 1);
+      const int grp = bfly / half, pos = bfly % half;
+      const int e0 = grp * len + pos, e1 = e0 + half;
+      const double2 twv = tw[pos * tstep];  // exp(-2 pi i m/K) = (cos, -sin)
+      const double wr = twv.x, wi = sign < 0 ? twv.y : -twv.y;
+      double2 *p = buf + (size_t)l * dist;
+      const double2 u = p[e0 * stride], v = p[e1 * stride];
+      const double vr = v.x * wr - v.y * wi, vi = v.x * wi + v.y * wr;
+      p[e0 * stride] = make_double2(u.x + vr, u.y + vi);
+      p[e1 * stride] = make_double2(u.x - vr, u.y - vi);
+    }
+    __syncthreads();
+  }
+}
+
+// forward transform of one c-plane of one bead: real along a, then b.
+// LDS holds the plane as Kb lines of Ka complex values; only a' = 0..Ka/2
+// is kept (Hermitian half).  spec layout: [bead][c][b][a'] complex.
+__global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
+  extern __shared__ double2 sm[];
+  const int c = blockIdx.x, b = blockIdx.y;
+  const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
+  const double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
+  for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) sm[w] = make_double2(g[w], 0.0);
+  __syncthreads();
+  lds_fft_pow2(sm, Ka, a.logK[0], Kb, 1, Ka, -1, a.tw[0]);
+  lds_fft_pow2(sm, Kb, a.logK[1], Ha, Ka, 1, -1, a.tw[1]);
+  double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
+  for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
+    const int bb = w / Ha, aa = w % Ha;
+    sp[w] = sm[bb * Ka + aa];
+  }
+  (void)Kc;
+}
+
+// along c for one (bead, b) row: forward FFT, influence function over the
+// half-space sphere list (ewald3d_pme.c:367-430): energy, virial, and the
+// product with preg*bweight, then the inverse FFT.
+__global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
+  extern __shared__ double2 sm[];
+  const int bb = blockIdx.x, b = blockIdx.y;
+  const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
+  double2 *sp = a.spec + (size_t)b * a.sstride;
+  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
+    const int cc = w / Ha, aa = w % Ha;
+    sm[w] = sp[((size_t)cc * Kb + bb) * Ha + aa];
+  }
+  __syncthreads();
+  lds_fft_pow2(sm, Kc, a.logK[2], Ha, Ha, 1, -1, a.tw[2]);
+  const double *hi = a.cell.hi;
+  const double tpi = 2.0 * M_PI, falp2 = a.falp2;
+  const int kb = bb <= Kb / 2 ? bb : bb - Kb;
+  double e = 0.0, v11 = 0.0, v22 = 0.0, v33 = 0.0, v12 = 0.0, v13 = 0.0, v23 = 0.0;
+  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
+    const int cc = w / Ha, aa = w % Ha;
+    const double W = a.infl[((size_t)cc * Kb + bb) * Ha + aa];
+    double2 s = sm[w];
+    if (W != 0.0) {
+      const int kc = cc <= Kc / 2 ? cc : cc - Kc;
+      const double aka = tpi * (double)aa, akb = tpi * (double)kb, akc = tpi * (double)kc;
+      const double xk = (aka * hi[0] + akb * hi[1] + akc * hi[2]);
+      const double yk = (aka * hi[3] + akb * hi[4] + akc * hi[5]);
+      const double zk = (aka * hi[6] + akb * hi[7] + akc * hi[8]);
+      const double g2 = xk * xk + yk * yk + zk * zk;
+      const double wsum = (aa == 0) ? 0.5 : 1.0;  // a' = 0 holds both +k and -k
+      const double sm2 = (s.x * s.x + s.y * s.y) * wsum;
+      e += W * sm2;
+      const double prep = -2.0 * W * ((g2 / falp2) + 1.0) / g2 * sm2;
+      v11 += prep * xk * xk;
+      v22 += prep * yk * yk;
+      v33 += prep * zk * zk;
+      v12 += prep * xk * yk;
+      v13 += prep * xk * zk;
+      v23 += prep * yk * zk;
+    }
+    sm[w] = make_double2(s.x * W, s.y * W);
+  }
+  __syncthreads();
+  lds_fft_pow2(sm, Kc, a.logK[2], Ha, Ha, 1, +1, a.tw[2]);
+  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
+    const int cc = w / Ha, aa = w % Ha;
+    sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
+  }
+  // block reduction of the energy and virial of this row
+  __shared__ double red[8][4];
+  double vals[7] = {e, v11, v22, v33, v12, v13, v23};
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 7; k++) {
+    double s = vals[k];
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) red[k][wv] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 7) {
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += red[threadIdx.x][k];
+    const int slots[7] = {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23};
+    if (s != 0.0) atomic_add(a.acc + (size_t)b * ACC_NUM + slots[threadIdx.x], s);
+    // the k-space pressure also carries vnow on the diagonal (ewald3d_pme.c:412-414)
+    if (threadIdx.x == 0 && s != 0.0) {
+      atomic_add(a.acc + (size_t)b * ACC_NUM + ACC_P11, s);
+      atomic_add(a.acc + (size_t)b * ACC_NUM + ACC_P22, s);
+      atomic_add(a.acc + (size_t)b * ACC_NUM + ACC_P33, s);
+    }
+  }
+}
+
+// inverse transform of one c-plane: along b, then complex-to-real along a.
+__global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
+  extern __shared__ double2 sm[];
+  const int c = blockIdx.x, b = blockIdx.y;
+  const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
+  const double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
+  for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
+    const int bb = w / Ka, aa = w % Ka;
+    sm[w] = aa < Ha ? sp[bb * Ha + aa] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  lds_fft_pow2(sm, Kb, a.logK[1], Ha, Ka, 1, +1, a.tw[1]);
+  // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
+  for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
+    const int bb = w / Ka, aa = w % Ka;
+    if (aa >= Ha) {
+      const double2 v = sm[bb * Ka + (Ka - aa)];
+      sm[w] = make_double2(v.x, -v.y);
+    }
+  }
+  __syncthreads();
+  lds_fft_pow2(sm, Ka, a.logK[0], Kb, 1, Ka, +1, a.tw[0]);
+  double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
+  for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) g[w] = sm[w].x;
+}
+
+// ---------------------------------------------------------------------------
+// bonded terms, one thread per (term, bead)
+
+__device__ __forceinline__ void intra_sep(const IntraArgs &a, int i, int j, int b, double &dx, double &dy,
+                                          double &dz) {
+  const size_t ib = (size_t)i * a.P + b, jb = (size_t)j * a.P + b;
+  dx = a.x[ib] - a.x[jb];
+  dy = a.y[ib] - a.y[jb];
+  dz = a.z[ib] - a.z[jb];
+  if (a.intra_perds) {
+    const Cell &c = a.cell;
+    if (!a.pimd) {
+      const double asx = dx * c.hi[0] + dy * c.hi[3] + dz * c.hi[6];
+      const double asy = dx * c.hi[1] + dy * c.hi[4] + dz * c.hi[7];
+      const double asz = dx * c.hi[2] + dy * c.hi[5] + dz * c.hi[8];
+      const double sx = asx - nint_piny(asx), sy = asy - nint_piny(asy), sz = asz - nint_piny(asz);
+      dx = sx * c.h[0] + sy * c.h[3] + sz * c.h[6];
+      dy = sx * c.h[1] + sy * c.h[4] + sz * c.h[7];
+      dz = sx * c.h[2] + sy * c.h[5] + sz * c.h[8];
+    } else {
+      const double dxs = a.xm[i] - a.xm[j], dys = a.ym[i] - a.ym[j], dzs = a.zm[i] - a.zm[j];
+      const double sx = -nint_piny(dxs * c.hi[0] + dys * c.hi[3] + dzs * c.hi[6]);
+      const double sy = -nint_piny(dxs * c.hi[1] + dys * c.hi[4] + dzs * c.hi[7]);
+      const double sz = -nint_piny(dxs * c.hi[2] + dys * c.hi[5] + dzs * c.hi[8]);
+      dx += (sx * c.h[0] + sy * c.h[3] + sz * c.h[6]);
+      dy += (sx * c.h[1] + sy * c.h[4] + sz * c.h[7]);
+      dz += (sx * c.h[2] + sy * c.h[5] + sz * c.h[8]);
+    }
+  }
+}
+
+__device__ __forceinline__ void add_force(const IntraArgs &a, int i, int b, double fx, double fy, double fz) {
+  const size_t ib = (size_t)i * a.P + b;
+  atomic_add(a.fx + ib, fx);
+  atomic_add(a.fy + ib, fy);
+  atomic_add(a.fz + ib, fz);
+}
+
+__device__ __forceinline__ void reduce_virial(const IntraArgs &a, int slot_e, double e, double p11, double p22,
+                                              double p33, double p12, double p13, double p23, int b,
+                                              bool act) {
+  double *acc = a.acc;
+  bead_reduce_add(e, a.P, b, act, acc + slot_e, ACC_NUM);
+  bead_reduce_add(p11, a.P, b, act, acc + ACC_P11, ACC_NUM);
+  bead_reduce_add(p22, a.P, b, act, acc + ACC_P22, ACC_NUM);
+  bead_reduce_add(p33, a.P, b, act, acc + ACC_P33, ACC_NUM);
+  bead_reduce_add(p12, a.P, b, act, acc + ACC_P12, ACC_NUM);
+  bead_reduce_add(p13, a.P, b, act, acc + ACC_P13, ACC_NUM);
+  bead_reduce_add(p23, a.P, b, act, acc + ACC_P23, ACC_NUM);
+}
+
+// ecor, ecorr.c:28-265 (no switching on the table)
+__global__ void __launch_bounds__(256) k_ecor(IntraArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = t < (size_t)a.necor * a.P;
+  const int k = act ? (int)(t / a.P) : 0, b = act ? (int)(t % a.P) : 0;
+  double e = 0, p11 = 0, p22 = 0, p33 = 0, p12 = 0, p13 = 0, p23 = 0;
+  if (act) {
+    const int i = a.e1[k], j = a.e2[k];
+    double dx, dy, dz;
+    intra_sep(a, i, j, b, dx, dy, dz);
+    const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
+    int kk = (int)(((r12 - a.e_rmin) * a.e_dri) + 0.5) + 3;
+    kk = min(kk, a.nsplin - 2);
+    kk = max(kk, 3);
+    const double rsp = ((double)(kk - 3)) * a.e_dr + a.e_rmin;
+    const double del = (r12 - rsp) * a.e_dri;
+    const double q12 = a.q[i] * a.q[j];
+    e = spline4(a.ecor_tab, kk, del, false) * q12;
+    const double dvt = spline4(a.ecor_tab, kk, del, true) * q12;
+    const double fx = dx * dvt, fy = dy * dvt, fz = dz * dvt;
+    add_force(a, i, b, fx, fy, fz);
+    add_force(a, j, b, -fx, -fy, -fz);
+    p11 = dx * fx;
+    p22 = dy * fy;
+    p33 = dz * fz;
+    p12 = dx * fy;
+    p13 = dx * fz;
+    p23 = dy * fz;
+  }
+  reduce_virial(a, ACC_VECOR, e, p11, p22, p33, p12, p13, p23, b, act);
+}
+
+// bond, bond.c:29-312
+__global__ void __launch_bounds__(256) k_bond(IntraArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = t < (size_t)a.nbond * a.P;
+  const int k = act ? (int)(t / a.P) : 0, b = act ? (int)(t % a.P) : 0;
+  double e = 0, p11 = 0, p22 = 0, p33 = 0, p12 = 0, p13 = 0, p23 = 0;
+  if (act) {
+    const int i = a.b1[k], j = a.b2[k], ty = a.btyp[k];
+    const double *cc = a.bc + 7 * ty;
+    double dx, dy, dz;
+    intra_sep(a, i, j, b, dx, dy, dz);
+    const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
+    const double rr0 = r12 - a.beq[ty];
+    e = (((((cc[6] * rr0 + cc[5]) * rr0 + cc[4]) * rr0 + cc[3]) * rr0 + cc[2]) * rr0 + cc[1]) * rr0 + cc[0];
+    const double dvb = (((((6.0 * cc[6]) * rr0 + 5.0 * cc[5]) * rr0 + 4.0 * cc[4]) * rr0 + 3.0 * cc[3]) * rr0 +
+                        2.0 * cc[2]) * rr0 + cc[1];
+    const double pre = -dvb / r12;
+    const double fx = dx * pre, fy = dy * pre, fz = dz * pre;
+    add_force(a, i, b, fx, fy, fz);
+    add_force(a, j, b, -fx, -fy, -fz);
+    p11 = dx * fx;
+    p22 = dy * fy;
+    p33 = dz * fz;
+    p12 = dx * fy;
+    p13 = dx * fz;
+    p23 = dy * fz;
+  }
+  reduce_virial(a, ACC_VBOND, e, p11, p22, p33, p12, p13, p23, b, act);
+}
+
+// bend, bend.c:28-434
+__global__ void __launch_bounds__(256) k_bend(IntraArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = t < (size_t)a.nbend * a.P;
+  const int k = act ? (int)(t / a.P) : 0, b = act ? (int)(t % a.P) : 0;
+  double e = 0, p11 = 0, p22 = 0, p33 = 0, p12 = 0, p13 = 0, p23 = 0;
+  if (act) {
+    const int i1 = a.a1[k], i2 = a.a2[k], i3 = a.a3[k], ty = a.atyp[k];
+    const double *cc = a.ac + 7 * ty, *ss = a.as + 7 * ty;
+    double dx12, dy12, dz12, dx23, dy23, dz23;
+    intra_sep(a, i1, i2, b, dx12, dy12, dz12);
+    intra_sep(a, i3, i2, b, dx23, dy23, dz23);
+    const double r122i = 1.0 / (dx12 * dx12 + dy12 * dy12 + dz12 * dz12);
+    const double r322i = 1.0 / (dx23 * dx23 + dy23 * dy23 + dz23 * dz23);
+    const double rpmag = sqrt(r122i) * sqrt(r322i);
+    double cost = (dx12 * dx23 + dy12 * dy23 + dz12 * dz23) * rpmag;
+    cost = (cost < 1.0 ? cost : 1.0);
+    cost = (cost > -1.0 ? cost : -1.0);
+    double sint = sqrt(1.0 - cost * cost);
+    sint = (sint > 1.0e-8 ? sint : 1.0e-8);
+    const double cos122 = cost * r122i, cos322 = cost * r322i;
+    const double vbc = ((((((cc[6] * cost + cc[5]) * cost + cc[4]) * cost + cc[3]) * cost + cc[2]) * cost + cc[1]) *
+                            cost + cc[0]);
+    const double sisum = (((((ss[6] * sint + ss[5]) * sint + ss[4]) * sint + ss[3]) * sint + ss[2]) * sint);
+    e = vbc + sisum * cost + ss[1] * sint;
+    const double dvbc = (((((6.0 * cc[6]) * cost + 5.0 * cc[5]) * cost + 4.0 * cc[4]) * cost + 3.0 * cc[3]) * cost +
+                         2.0 * cc[2]) * cost + cc[1];
+    double dvbs = ((((5.0 * ss[6]) * sint + 4.0 * ss[5]) * sint + 3.0 * ss[4]) * sint + 2.0 * ss[3]) * sint + ss[2];
+    dvbs = dvbs * cost + ss[1];
+    const double pre = -(dvbc + sisum - dvbs * cost / sint);
+    const double f1x = (dx23 * rpmag - dx12 * cos122) * pre;
+    const double f1y = (dy23 * rpmag - dy12 * cos122) * pre;
+    const double f1z = (dz23 * rpmag - dz12 * cos122) * pre;
+    const double f3x = (dx12 * rpmag - dx23 * cos322) * pre;
+    const double f3y = (dy12 * rpmag - dy23 * cos322) * pre;
+    const double f3z = (dz12 * rpmag - dz23 * cos322) * pre;
+    add_force(a, i1, b, f1x, f1y, f1z);
+    add_force(a, i2, b, -(f1x + f3x), -(f1y + f3y), -(f1z + f3z));
+    add_force(a, i3, b, f3x, f3y, f3z);
+    p11 = dx12 * f1x + dx23 * f3x;
+    p22 = dy12 * f1y + dy23 * f3y;
+    p33 = dz12 * f1z + dz23 * f3z;
+    p12 = dx12 * f1y + dx23 * f3y;
+    p13 = dx12 * f1z + dx23 * f3z;
+    p23 = dy12 * f1z + dy23 * f3z;
+  }
+  reduce_virial(a, ACC_VBEND, e, p11, p22, p33, p12, p13, p23, b, act);
+}
+
+// ---------------------------------------------------------------------------
+// PIMD finish: f /= P (energy_control_pimd.c:116-126) and the virial
+// correction -sum (x - xmod) (x) f (pimd_estimators.c:223-323), one thread
+// per (site, bead).
+__global__ void __launch_bounds__(256) k_pimd_finish(FinishArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = t < (size_t)a.N * a.P;
+  const int i = act ? (int)(t / a.P) : 0, b = act ? (int)(t % a.P) : 0;
+  double c[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (act) {
+    const double fx = a.fx[t] / (double)a.P, fy = a.fy[t] / (double)a.P, fz = a.fz[t] / (double)a.P;
+    a.fx[t] = fx;
+    a.fy[t] = fy;
+    a.fz[t] = fz;
+    if (a.P > 1) {
+      const double dx = a.x[t] - a.xm[i], dy = a.y[t] - a.ym[i], dz = a.z[t] - a.zm[i];
+      c[0] = -(dx * fx);
+      c[1] = -(dy * fy);
+      c[2] = -(dz * fz);
+      c[3] = -(dx * fy);
+      c[4] = -(dy * fx);
+      c[5] = -(dx * fz);
+      c[6] = -(dz * fx);
+      c[7] = -(dy * fz);
+      c[8] = -(dz * fy);
+      c[9] = -(dx * fx + dy * fy + dz * fz);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 10; k++) bead_reduce_add(c[k], a.P, b, act, a.acc + ACC_C11 + k, ACC_NUM);
+}
+
+// distrib_ghost_force, ghost_control.c:71-125
+__global__ void k_ghost_force(int ng, int P, const int *__restrict__ gatom, const int *__restrict__ gncomp,
+                              const int *__restrict__ gcomp, const double *__restrict__ gcoef, double *fx,
+                              double *fy, double *fz) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ng * P) return;
+  const int g = t / P, b = t % P;
+  const size_t ig = (size_t)gatom[g] * P + b;
+  const double gx = fx[ig], gy = fy[ig], gz = fz[ig];
+  for (int k = 0; k < gncomp[g]; k++) {
+    const size_t ia = (size_t)gcomp[4 * g + k] * P + b;
+    const double c = gcoef[4 * g + k];
+    atomic_add(fx + ia, c * gx);
+    atomic_add(fy + ia, c * gy);
+    atomic_add(fz + ia, c * gz);
+  }
+  fx[ig] = 0.0;
+  fy[ig] = 0.0;
+  fz[ig] = 0.0;
+}
+
+// normal-mode forces and springs for one site and one Cartesian direction:
+// convert_pimd_force_cent (transform_cnt.c:415-489) of the bead forces,
+// mode coordinates of convert_pimd_mode_cent (:240-316) and the spring
+// force -2 prekf veig u of mode_energy_control (pimd_estimators.c:71-95).
+__global__ void k_mode_forces(ModeArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = t < a.N * 3;
+  double ake = 0.0;
+  if (act) {
+    const int i = t / 3, d = t % 3;
+    const double *x = d == 0 ? a.x : (d == 1 ? a.y : a.z);
+    double *f = d == 0 ? a.fx : (d == 1 ? a.fy : a.fz);
+    const int P = a.P;
+    double xr[64], fr[64];
+    for (int b = 0; b < P; b++) {
+      xr[b] = x[(size_t)i * P + b];
+      fr[b] = f[(size_t)i * P + b];
+    }
+    const double pk = a.prekf[i];
+    for (int m = 0; m < P; m++) {
+      // mode m in PINY's storage order: 0 centroid, P-1 alternating,
+      // 2k-1 real part and 2k imaginary part of frequency k
+      int k = 0, im = 0;
+      if (m == 0) k = 0;
+      else if (m == P - 1) k = P / 2;
+      else {
+        k = (m + 1) / 2;
+        im = (m % 2 == 0);
+      }
+      double su = 0.0, sf = 0.0;
+      for (int b = 0; b < P; b++) {
+        const int ph = (k * b) % P;
+        const double cs = a.cs[2 * ph], sn = a.cs[2 * ph + 1];  // cos, sin of 2 pi ph/P
+        const double w = im ? sn : cs;                          // exp(+i) real / imaginary part
+        su += xr[b] * w;
+        sf += fr[b] * w;
+      }
+      su /= (double)P;
+      if (m != 0 && m != P - 1) sf *= 2.0;
+      const double fk = pk * a.veig[m + 1];
+      ake += fk * su * su;
+      f[(size_t)i * P + m] = sf + (-2.0 * fk * su);
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) ake += __shfl_xor(ake, off, 64);
+  if ((threadIdx.x & 63) == 0 && ake != 0.0) atomic_add(a.ake, ake);
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+
+hipError_t launch_pair(const PairArgs &a, hipStream_t st) {
+  // beads per slot: the smallest power of two covering P, at most 64
+  int B = 1;
+  while (B < a.P && B < 64) B <<= 1;
+  const int nbg = (a.P + B - 1) / B;
+  const int waves = a.N < 8192 ? a.N : 8192;
+  dim3 grid((waves + 3) / 4, nbg), block(256);
+  switch (B) {
+    case 1: hipLaunchKernelGGL(k_pair_f64<1>, grid, block, 0, st, a); break;
+    case 2: hipLaunchKernelGGL(k_pair_f64<2>, grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(k_pair_f64<4>, grid, block, 0, st, a); break;
+    case 8: hipLaunchKernelGGL(k_pair_f64<8>, grid, block, 0, st, a); break;
+    case 16: hipLaunchKernelGGL(k_pair_f64<16>, grid, block, 0, st, a); break;
+    case 32: hipLaunchKernelGGL(k_pair_f64<32>, grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL(k_pair_f64<64>, grid, block, 0, st, a); break;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pmx

@@ -1,0 +1,36 @@
+"""Named test systems shared by the CPU and GPU tests (seeded, synthetic).
+
+Grid sizes: PINY sizes the PME grid as 4*(NINT(kmax_pme/2)+1) snapped to
+its radix table (set_cp_ewald.c:204-293, 815-1008), so kmax_pme = 6 gives
+16, 14 gives 32 and 30 gives 64; kmax_ewd <= K/2 - 2 keeps the sphere
+inside the grid (set_cp_ewald.c:259-280).
+"""
+from piny_amd import water as w
+
+
+def _st(**kw):
+    return w.Settings(**kw)
+
+
+CASES = {
+    # 27 SPC/Fw, P = 1, K = 16
+    "spc27": lambda: w.build_config(w.SPC_FW, 27, w.density_box(27), 1, 11,
+                                    _st(cutoff=4.0, alpha=0.85, kmax_ewd=6, kmax_pme=6)),
+    # 64 SPC/Fw, P = 1, K = 16
+    "spc64": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
+                                    _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
+    # 64 q-TIP4P/F, P = 4, K = 16
+    "tip64_p4": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
+                                       _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
+    # configuration 1: 256 SPC/Fw, L = 19.73 A, P = 1, K = 32, order 6
+    "cfg1": lambda: w.config1(),
+    # configuration 2 at P = 8 (the oracle's size budget), K = 32
+    "cfg2_p8": lambda: w.config2(P=8),
+    # 256 q-TIP4P/F at P = 32, K = 32
+    "tip256_p32": lambda: w.build_config(w.Q_TIP4P_F, 256, w.density_box(256), 32, 14,
+                                         _st(cutoff=9.0, alpha=0.35, kmax_ewd=14, kmax_pme=14)),
+}
+
+
+def get(name):
+    return CASES[name]()

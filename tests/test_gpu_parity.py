@@ -1,0 +1,128 @@
+"""Parity of the HIP path with the oracle, through the C ABI, on the GPU.
+
+Bars: Verlet lists, k-vectors, grid sizes and spline tables bit-exact;
+energies, forces and virials in FP64 within 1e-9 relative (the north-star
+bound for the FP64 path is a force RMS error below 1e-5 relative)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from piny_amd import abi
+from piny_amd.engine import Engine
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+ETOL = 1e-9
+FTOL = 1e-9
+
+
+def _run(case, flags_extra=0):
+    c = cases.get(case)
+    s = c.system
+    flags = (abi.PMX_EVAL_PIMD if s.pi_beads > 1 else 0) | flags_extra
+    print(f"[case {case}] N={s.natm} P={s.pi_beads}", flush=True)
+    o = Oracle(s, c.x, c.y, c.z)
+    eo, fo, _ = o.energy_control(c.x, c.y, c.z, flags)
+    g = Engine(s, c.x, c.y, c.z)
+    g.set_positions(c.x, c.y, c.z)
+    eg = g.energy_control(flags)
+    fg = g.forces()
+    return c, o, g, eo, fo, eg, fg
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("case", ["spc27", "spc64", "tip64_p4", "cfg1", "cfg2_p8", "tip256_p32"])
+def test_energy_control_parity(case):
+    c, o, g, eo, fo, eg, fg = _run(case)
+    scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
+    for k in abi.ENERGY_KEYS:
+        assert abs(eg[k] - eo[k]) <= ETOL * max(abs(eo[k]), scale), (k, eg[k], eo[k])
+    assert eg["nver"] == eo["nver"]
+    rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
+    print(f"  force rms rel err {rms:.3e}", flush=True)
+    assert rms < FTOL
+    pv_scale = np.max(np.abs(eo["pvten"]))
+    assert np.max(np.abs(eg["pvten"] - eo["pvten"])) <= ETOL * pv_scale
+    if c.system.pi_beads > 1:
+        assert _rel(eg["pi_ke_vir"], eo["pi_ke_vir"]) < ETOL
+    assert np.allclose(g.bead_energies(), o.bead_energies(), rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["spc64", "tip64_p4", "cfg1", "cfg2_p8"])
+def test_verlet_list_bit_exact(case):
+    c, o, g, eo, fo, eg, fg = _run(case)
+    for a, b in zip(g.verlet_list(), o.verlet_list()):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("case", ["spc64", "cfg1", "tip256_p32"])
+def test_setup_bit_exact(case):
+    c = cases.get(case)
+    s = c.system
+    print(f"[case {case}] setup", flush=True)
+    o = Oracle(s, c.x, c.y, c.z)
+    g = Engine(s, c.x, c.y, c.z)
+    assert np.array_equal(g.scalars()[:19], o.scalars()[:19])
+    for a, b in zip(g.kvectors(), o.kvectors()):
+        assert np.array_equal(a, b)
+    npair = s.ntype * (s.ntype + 1) // 2
+    for which in range(4):
+        for ip in range(npair):
+            assert np.array_equal(g.spline(which, ip), o.spline(which, ip)), (which, ip)
+    for which in (4, 5):
+        assert np.array_equal(g.spline(which), o.spline(which)), which
+
+
+@pytest.mark.parametrize("case", ["tip64_p4", "cfg2_p8"])
+def test_mode_forces_and_springs(case):
+    flags = abi.PMX_EVAL_MODE_FORCES
+    c, o, g, eo, fo, eg, fg = _run(case, flags)
+    assert _rel(eg["kin_harm"], eo["kin_harm"]) < 1e-10
+    assert _rel(eg["pi_ke_prim"], eo["pi_ke_prim"]) < 1e-10
+    rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
+    assert rms < FTOL
+
+
+def test_list_rebuild_after_motion():
+    c = cases.get("spc64")
+    s = c.system
+    print("[case spc64 rebuild]", flush=True)
+    o = Oracle(s, c.x, c.y, c.z)
+    g = Engine(s, c.x, c.y, c.z)
+    g.set_positions(c.x, c.y, c.z)
+    g.energy_control()
+    o.energy_control(c.x, c.y, c.z)
+    rng = np.random.default_rng(5)
+    x, y, z = (v + rng.normal(scale=0.05, size=v.shape) for v in (c.x, c.y, c.z))   # small: no rebuild
+    g.set_positions(x, y, z)
+    e1 = g.energy_control()
+    eo1, fo1, _ = o.energy_control(x, y, z)
+    assert e1["list_rebuilt"] == 0 and eo1["list_rebuilt"] == 0
+    x2 = x.copy()
+    x2[5] += 1.5  # more than half the skin: rebuild
+    g.set_positions(x2, y, z)
+    e2 = g.energy_control()
+    eo2, fo2, _ = o.energy_control(x2, y, z)
+    assert e2["list_rebuilt"] == 1 and eo2["list_rebuilt"] == 1
+    assert _rel(e2["vintert"], eo2["vintert"]) < ETOL
+    for a, b in zip(g.verlet_list(), o.verlet_list()):
+        assert np.array_equal(a, b)
+
+
+def test_repeat_runs_reproduce():
+    c = cases.get("cfg1")
+    print("[case cfg1 repeat]", flush=True)
+    g = Engine(c.system, c.x, c.y, c.z)
+    g.set_positions(c.x, c.y, c.z)
+    e1 = g.energy_control()
+    f1 = g.forces()
+    e2 = g.energy_control()
+    f2 = g.forces()
+    assert np.max(np.abs(f1 - f2)) <= 1e-12 * np.max(np.abs(f1))
+    assert abs(e1["vintert"] - e2["vintert"]) <= 1e-12 * abs(e1["vintert"])

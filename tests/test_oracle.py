@@ -1,0 +1,299 @@
+"""CPU checks that pin the oracle (the FP64 C restatement of the reference
+path) with closed forms, independent restatements and finite differences,
+so that a dropped term, a wrong sign or a wrong index fails somewhere."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, gauss_legendre, nm_force_to_mode, nm_mode_to_pos, nm_pos_to_mode
+from oracle import oracle as orc
+from piny_amd import water as w
+from tests import cases
+
+
+def _total(e):
+    return e["vintert"] + e["vintrat"]
+
+
+# --------------------------------------------------------------------------
+# spline tables: properties fixed by the mathematics, not a retyped formula
+
+def test_lj_table_properties():
+    st = w.Settings(cutoff=9.0, nsplin=5000)
+    s = w.make_system(w.SPC_FW, 8, 12.0, 1, st)
+    o = Oracle(s)
+    v, dv = o.spline(0, 0), o.spline(1, 0)          # O-O pair is pair 0
+    rmin, rmax = s.inter_cutti[0], s.inter_cutoff[0]
+    dr = (rmax - rmin) / (st.nsplin - 5)
+    r = dr * (np.arange(1, st.nsplin + 1) - 3) + rmin
+    sig, eps = s.inter_sig[0], s.inter_eps[0]
+    rheal = st.rheal / w.BOHR
+    inner = r < rmax - rheal
+    # V(sigma) = 0 and the minimum -eps at 2^(1/6) sigma (no switching there)
+    k0 = np.argmin(np.abs(r - sig))
+    lin = -24.0 * eps / sig * (r[k0] - sig)        # V'(sigma) = -24 eps / sigma
+    assert abs(v[k0] - lin) < 1e-2 * abs(lin)
+    km = np.argmin(np.where(inner, v, np.inf))
+    assert abs(r[km] - 2 ** (1 / 6) * sig) < 2 * dr
+    assert abs(v[km] + eps) < 1e-6 * eps
+    # -(1/r) dV/dr from the V table by central differences agrees with dV
+    fd = -(v[2:] - v[:-2]) / (2 * dr) / r[1:-1]
+    sel = (r[1:-1] > 0.9 * sig)
+    # O(dr^2) difference error on the steep repulsive wall: ~2e-5 relative
+    assert np.max(np.abs(fd[sel] - dv[1:-1][sel])) < 1e-4 * np.max(np.abs(dv[1:-1][sel]))
+    # switched to zero at the cutoff, untouched inside rmax - rheal
+    assert abs(v[np.argmin(np.abs(r - rmax))]) < 1e-12
+    sig6 = sig ** 6
+    vlj = 4 * eps * (sig6 / r ** 6) * (sig6 / r ** 6 - 1)
+    assert np.allclose(v[inner & (r > 0.8 * sig)], vlj[inner & (r > 0.8 * sig)], rtol=1e-12, atol=1e-12 * eps)
+    # null pairs are zero
+    assert np.all(o.spline(0, 1) == 0.0) and np.all(o.spline(1, 2) == 0.0)
+
+
+def test_erfc_table_against_math_erfc():
+    st = w.Settings(cutoff=9.0, nsplin=5000)
+    s = w.make_system(w.SPC_FW, 8, 12.0, 1, st)
+    o = Oracle(s)
+    alp = o.scalars()[0]
+    assert abs(alp - s.ewald_alpha / (12.0 / w.BOHR)) < 1e-15
+    vc = o.spline(2, 0)
+    rmin, rmax = s.inter_cutti[0], s.inter_cutoff[0]
+    dr = (rmax - rmin) / (st.nsplin - 5)
+    r = dr * (np.arange(1, st.nsplin + 1) - 3) + rmin
+    inner = (r > 1.0) & (r < rmax - st.rheal / w.BOHR)
+    exact = np.array([math.erfc(alp * ri) / ri for ri in r[inner]])
+    # PINY's 9-term erfc polynomial (spline_fit.c:327-339) is accurate to ~1e-9 abs
+    assert np.max(np.abs(vc[inner] - exact)) < 5e-9
+
+
+# --------------------------------------------------------------------------
+# k-vectors, grid sizes, B-spline weights
+
+def _bspline(n, x):
+    """Cardinal B-spline M_n(x) (Essmann et al. 1995, eq. 4.1 closed form)."""
+    s = 0.0
+    for k in range(n + 1):
+        t = x - k
+        if t > 0:
+            s += (-1) ** k * math.comb(n, k) * t ** (n - 1)
+    return s / math.factorial(n - 1)
+
+
+@pytest.mark.parametrize("kmax_pme,grid", [(6, 16), (14, 32), (17, 40), (22, 48), (25, 60), (30, 64)])
+def test_pme_grid_sizes(kmax_pme, grid):
+    box = 24.832
+    s = w.make_system(w.Q_TIP4P_F, 8, box, 1, w.Settings(kmax_ewd=min(6, grid // 2 - 2), kmax_pme=kmax_pme,
+                                                          n_interp=8, alpha=10.0 / box))
+    o = Oracle(s)
+    assert o.scalars()[10:13].tolist() == [grid] * 3
+
+
+def test_kvectors_half_space_sphere():
+    box = 19.73
+    s = w.make_system(w.SPC_FW, 8, box, 1, w.Settings(kmax_ewd=14, kmax_pme=14))
+    o = Oracle(s)
+    ng, ka, kb, kc, bw = o.kvectors()
+    got = set(zip(ka.tolist(), kb.tolist(), kc.tolist()))
+    # brute force: half space (ka>0 | ka=0,kb>0 | ka=kb=0,kc>0) inside |k| <= kmax
+    L = box / w.BOHR
+    vol = L ** 3
+    ecut = math.pi * 0.5 * math.pi * 14 * 14 * vol ** (-2.0 / 3.0)
+    want = set()
+    for a in range(0, 15):
+        for b in range(-14, 15):
+            for c in range(-14, 15):
+                if (a, b, c) <= (0, 0, 0) and not (a > 0 or (a == 0 and (b > 0 or (b == 0 and c > 0)))):
+                    continue
+                if not (a > 0 or (a == 0 and (b > 0 or (b == 0 and c > 0)))):
+                    continue
+                g2 = (2 * math.pi / L) ** 2 * (a * a + b * b + c * c)
+                if 0.5 * g2 <= 4 * ecut:
+                    want.add((a, b, c))
+    assert got == want
+    assert len(ka) == len(got)
+    # separable B-spline weights from the closed-form cardinal spline
+    n = s.settings.n_interp
+    K = int(ng[0])
+    mk = [_bspline(n, k + 1) for k in range(n - 1)]
+
+    def b1(m):
+        den = sum(mk[k] * complex(math.cos(2 * math.pi * k * m / K), math.sin(2 * math.pi * k * m / K))
+                  for k in range(n - 1))
+        return 1.0 / abs(den) ** 2
+
+    ref = np.array([b1(a) * b1(b) * b1(c) for a, b, c in zip(ka, kb, kc)])
+    assert np.max(np.abs(bw - ref) / ref) < 1e-13
+
+
+def test_gauss_legendre_nodes():
+    a, wts = gauss_legendre(128)
+    ra, rw = np.polynomial.legendre.leggauss(128)
+    assert np.max(np.abs(a - ra)) < 1e-14 and np.max(np.abs(wts - rw) / rw) < 1e-10
+
+
+# --------------------------------------------------------------------------
+# bonded coefficients
+
+def test_morse_taylor_coefficients():
+    # (1 - e^{-x})^2 as a truncated power series from exact rational products
+    from fractions import Fraction
+    e = [Fraction((-1) ** k, math.factorial(k)) for k in range(8)]   # e^{-x}
+    one_minus = [-c for c in e]
+    one_minus[0] += 1
+    sq = [sum(one_minus[i] * one_minus[m - i] for i in range(m + 1)) for m in range(7)]
+    D, a = 0.7, 1.3
+    c = np.zeros(7)
+    orc.lib().or_bond_coef_morse(D, a, orc.abi.dptr(c))
+    for m in range(7):
+        assert abs(c[m] - D * float(sq[m]) * a ** m) < 1e-14 * max(1.0, abs(c[m]))
+
+
+def test_harmonic_bend_series():
+    fk, th0 = 0.3, math.radians(107.4)
+    cc, ss = np.zeros(7), np.zeros(7)
+    orc.lib().or_bend_coef_harm(fk, th0, orc.abi.dptr(cc), orc.abi.dptr(ss))
+
+    def V(t):
+        ct, st = math.cos(t), math.sin(t)
+        # bend.c:253-270 evaluates cos * sum_{m>=2} s_m sin^(m-1) + s_1 sin
+        return (sum(cc[m] * ct ** m for m in range(7)) + ct * sum(ss[m] * st ** (m - 1) for m in range(2, 7))
+                + ss[1] * st)
+
+    assert abs(V(th0)) < 1e-12 * fk
+    for d in (0.02, -0.05, 0.1):
+        assert abs(V(th0 + d) - 0.5 * fk * d * d) < 0.02 * 0.5 * fk * d * d
+
+
+# --------------------------------------------------------------------------
+# normal-mode transform: against numpy.fft and the chain rule
+
+def test_nm_transforms():
+    rng = np.random.default_rng(1)
+    for P in (4, 8, 32):
+        x = rng.normal(size=P)
+        u = nm_pos_to_mode(x)
+        X = np.fft.ifft(x)
+        ref = np.zeros(P)
+        ref[0] = X[0].real
+        ref[P - 1] = X[P // 2].real
+        for k in range(1, P // 2):
+            ref[2 * k - 1] = X[k].real
+            ref[2 * k] = X[k].imag
+        assert np.allclose(u, ref, atol=1e-14)
+        assert np.allclose(nm_mode_to_pos(u), x, atol=1e-13)
+        # forces: f_mode = J^T f with J = dx/du of the inverse transform
+        J = np.stack([nm_mode_to_pos(np.eye(P)[m]) for m in range(P)], 1)
+        f = rng.normal(size=P)
+        assert np.allclose(nm_force_to_mode(f), J.T @ f, atol=1e-12)
+
+
+# --------------------------------------------------------------------------
+# Verlet list against an independent numpy minimum-image search
+
+def test_verlet_list_bruteforce():
+    c = cases.get("spc64")
+    s = c.system
+    o = Oracle(s, c.x, c.y, c.z)
+    o.energy_control(c.x, c.y, c.z)
+    nter, off, jter = o.verlet_list()
+    L = s.box / w.BOHR
+    pos = np.stack([c.x, c.y, c.z], 1)
+    mol = np.arange(s.natm) // 3
+    cs2 = (s.inter_cutoff[0] + s.settings.skin / w.BOHR) ** 2
+    for i in range(s.natm):
+        d = pos[i] - pos[:i]
+        a = d / L
+        nint = np.where(a >= 0, np.floor(a + 0.5), np.ceil(a - 0.5))
+        d = (a - nint) * L
+        r2 = d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]
+        want = np.nonzero((r2 < cs2) & (mol[:i] != mol[i]))[0]
+        got = jter[off[i]:off[i] + nter[i]]
+        assert np.array_equal(got, want), i
+
+
+# --------------------------------------------------------------------------
+# forces are minus the gradient of the energy (every term, both models, PIMD)
+
+@pytest.mark.parametrize("case", ["spc64", "tip64_p4"])
+def test_forces_are_energy_gradient(case):
+    c = cases.get(case)
+    s = c.system
+    o = Oracle(s, c.x, c.y, c.z)
+    flags = 1 if s.pi_beads > 1 else 0
+    e, f, (x, y, z) = o.energy_control(c.x, c.y, c.z, flags)
+    h = 1e-5
+    rng = np.random.default_rng(2)
+    real = np.nonzero(s.atm_type != 2)[0] if s.ntype == 3 else np.arange(s.natm)
+    worst = 0.0
+    for i in rng.choice(real, 5, replace=False):
+        ip = int(rng.integers(s.pi_beads))
+        for d in range(3):
+            xp = [x.copy(), y.copy(), z.copy()]
+            xm = [x.copy(), y.copy(), z.copy()]
+            xp[d][ip * s.natm + i] += h
+            xm[d][ip * s.natm + i] -= h
+            ep, _, _ = o.energy_control(*xp, flags)
+            em, _, _ = o.energy_control(*xm, flags)
+            fd = -(_total(ep) - _total(em)) / (2 * h)
+            worst = max(worst, abs(fd - f[ip, i, d]))
+    assert worst < 2e-9
+
+
+# --------------------------------------------------------------------------
+# the pressure tensor is minus the strain derivative of the energy
+
+def test_virial_is_strain_derivative():
+    # kmax 14 puts k_cut/2alpha near 5.7, so the box dependence of self_erf
+    # and of the finite-k ecor correction (set_exclude.c:690-720), which
+    # PINY's virial leaves out, is below 1e-14.
+    c0 = cases.get("spc64")
+    s0 = c0.system
+    st0 = w.Settings(**{**s0.settings.__dict__, "kmax_ewd": 14, "kmax_pme": 14})
+    s = w.make_system(s0.model, s0.nmol, s0.box, 1, st0)
+    c = w.Config(s, c0.x, c0.y, c0.z)
+    e, f, _ = Oracle(s, c.x, c.y, c.z).energy_control(c.x, c.y, c.z)
+    eps = 1e-6
+    vals = []
+    for sgn in (+1, -1):
+        lam = 1 + sgn * eps
+        st = w.Settings(**{**s.settings.__dict__})
+        s2 = w.make_system(s.model, s.nmol, s.box * lam, 1, st)
+        o2 = Oracle(s2, c.x * lam, c.y * lam, c.z * lam)
+        e2, _, _ = o2.energy_control(c.x * lam, c.y * lam, c.z * lam)
+        vals.append(_total(e2))
+    dE = (vals[0] - vals[1]) / (2 * eps)
+    tr = e["pvten"][0] + e["pvten"][4] + e["pvten"][8]
+    assert abs(-dE - tr) < 1e-6 * abs(tr) + 1e-7
+
+
+# --------------------------------------------------------------------------
+# PME converges to standard Ewald; Ewald total does not depend on alpha
+
+def test_pme_converges_to_ewald():
+    c = cases.get("spc64")
+    s = c.system
+    ew = Oracle(w.make_system(s.model, s.nmol, s.box, 1, w.Settings(**{**s.settings.__dict__, "pme_on": 0})))
+    e0, f0, _ = ew.recip_energy(c.x, c.y, c.z, use_pme=False)
+    errs = []
+    for kp, n in ((6, 6), (14, 6), (14, 8)):
+        st = w.Settings(**{**s.settings.__dict__, "kmax_pme": kp, "n_interp": n})
+        o = Oracle(w.make_system(s.model, s.nmol, s.box, 1, st))
+        v, f1, _ = o.recip_energy(c.x, c.y, c.z, use_pme=True)
+        errs.append((abs(v - e0) / abs(e0), np.sqrt(((f1 - f0) ** 2).sum() / (f0 ** 2).sum())))
+    assert errs[0][0] > errs[1][0] > errs[2][0]
+    assert errs[2][0] < 1e-6 and errs[2][1] < 1e-5
+
+
+def test_ewald_total_independent_of_alpha():
+    c = cases.get("spc64")
+    s = c.system
+    # alpha (r_c - 1 A) >= 4 keeps the switched region of the erfc table
+    # negligible; kmax 22 converges the k-sum for alpha up to 1.1 / A
+    tot = []
+    for alpha in (0.9, 1.1):
+        st = w.Settings(**{**s.settings.__dict__, "alpha": alpha, "pme_on": 0, "kmax_ewd": 22})
+        o = Oracle(w.make_system(s.model, s.nmol, s.box, 1, st), c.x, c.y, c.z)
+        e, _, _ = o.energy_control(c.x, c.y, c.z)
+        tot.append(e["vcoul"] + e["vrecip"] + e["vself"] + e["vbgr"] + e["vecor"])
+    assert abs(tot[0] - tot[1]) < 1e-6 * abs(tot[0])
