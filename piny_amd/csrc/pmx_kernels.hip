@@ -387,8 +387,10 @@ __device__ __forceinline__ void bspline_axis(double s, int K, int n, bool deriv,
     u[j] = frac + (double)(j - 1);
     int ia = it - (j - 2);
     ia = (ia > 0 ? ia : K + ia);
-    if (ia > K) ia -= K;  // only for a coordinate exactly on the cell face
-    b.ig[j] = ia - 1;
+    // PINY's index is ia in 1..K; the wrap below only acts for a coordinate
+    // exactly on the cell face (ia = K+1) and keeps a non-finite coordinate
+    // inside the grid.
+    b.ig[j] = (((ia - 1) % K) + K) % K;
   }
   double *mn = b.mn, *dmn = b.dmn;
   mn[1] = 1.0 - fabs(u[1] - 1.0);

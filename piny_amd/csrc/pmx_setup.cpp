@@ -311,6 +311,25 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
   hi[8] = (h[7] * h[2] - h[8] * h[1]) / vol;
   hi[6] = (h[3] * h[4] - h[6] * h[1]) / vol;
   s.cubic = (h[4] == 0.0 && h[7] == 0.0 && h[2] == 0.0 && h[8] == 0.0 && h[3] == 0.0 && h[6] == 0.0);
+  // --- topology indices must address existing sites / types
+  {
+    auto bad = [&](int v, int n) { return v < 0 || v >= n; };
+    for (int i = 0; i < s.N; i++)
+      if (bad(sys->atm_type[i], s.ntype)) return "atom type index out of range";
+    for (int b = 0; b < sys->nbond; b++)
+      if (bad(sys->bond_j1[b], s.N) || bad(sys->bond_j2[b], s.N) || bad(sys->bond_type[b], sys->nbond_type))
+        return "bond index out of range";
+    for (int b = 0; b < sys->nbend; b++)
+      if (bad(sys->bend_j1[b], s.N) || bad(sys->bend_j2[b], s.N) || bad(sys->bend_j3[b], s.N) ||
+          bad(sys->bend_type[b], sys->nbend_type))
+        return "bend index out of range";
+    for (int g = 0; g < sys->nghost; g++) {
+      if (bad(sys->ghost_atom[g], s.N) || sys->ghost_ncomp[g] < 1 || sys->ghost_ncomp[g] > 4)
+        return "ghost definition out of range";
+      for (int k = 0; k < sys->ghost_ncomp[g]; k++)
+        if (bad(sys->ghost_comp[4 * g + k], s.N)) return "ghost constituent out of range";
+    }
+  }
   // --- charged sites
   s.ichrg.clear();
   s.qsq = 0.0;
