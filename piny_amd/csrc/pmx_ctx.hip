@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -50,7 +51,15 @@ struct pmx_ctx {
   size_t gstride = 0, sstride = 0;
   double *grid = nullptr, *infl = nullptr;
   double2 *spec = nullptr, *tw[3] = {nullptr, nullptr, nullptr};
-  // accumulators [P][ACC_NUM] + ake
+  // binned PME: tiling, per-item data, bins, halo boxes
+  PmeArgs pa{};
+  int *item_s = nullptr, *item_tile = nullptr, *item_pos = nullptr, *bin_cnt = nullptr, *bin_off = nullptr,
+      *sorted = nullptr;
+  double *item_f = nullptr, *hbuf = nullptr;
+  // per-block partial sums of every stage, then accumulators [P][ACC_NUM] + ake
+  double *part = nullptr;
+  size_t off_pair = 0, off_fft = 0, off_ecor = 0, off_bond = 0, off_bend = 0, off_fin = 0;
+  int g_intra = 1024;
   double *acc = nullptr, *ake = nullptr;
   std::vector<double> acc_h;
   // springs
@@ -116,8 +125,8 @@ unsigned blocks(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / 
 int alloc_lists(pmx_ctx *c, int cap) {
   const size_t n = (size_t)c->s.N * cap;
   if (c->jfull) {
-    hipFree(c->jfull);
-    hipFree(c->jforce);
+    (void)hipFree(c->jfull);
+    (void)hipFree(c->jforce);
     c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->jfull), c->allocs.end());
     c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->jforce), c->allocs.end());
   }
@@ -135,6 +144,86 @@ int estimate_cap(const Setup &s, double rskin) {
   int cap = (int)(1.3 * nb) + 64;
   cap = ((cap + 63) / 64) * 64;
   return std::min(std::max(cap, 64), ((s.N + 63) / 64) * 64);
+}
+
+#define TRY_RC(x)      \
+  do {                 \
+    int rc_ = (x);     \
+    if (rc_) return rc_; \
+  } while (0)
+
+// Chooses the PME tile (largest of 16 x 16 x 8 and its halvings whose halo
+// box and weight chunk fit 64 KB of LDS), allocates the binning buffers and
+// fills the constant part of the PME argument block.
+int setup_pme_tiles(pmx_ctx *c) {
+  Setup &s = c->s;
+  PmeArgs &a = c->pa;
+  const int n = c->par.n_interp;
+  if (n < 4 || n > MAX_INTERP) {
+    set_err(c, "PME interpolation order %d: the device path supports orders 4..%d", n, MAX_INTERP);
+    return PMX_ERR_NOTIMPL;
+  }
+  for (int d = 0; d < 3; d++) {
+    if (s.ngrid[d] < n) {
+      set_err(c, "PME grid %d is smaller than the interpolation order %d", s.ngrid[d], n);
+      return PMX_ERR_RANGE;
+    }
+  }
+  a.N = s.N;
+  a.P = s.P;
+  a.nchrg = (int)s.ichrg.size();
+  a.n = n;
+  int T[3] = {16, 16, 8};
+  const char *env = getenv("PMX_PME_TILE");  // tuning override "Ta,Tb,Tc"
+  if (env) sscanf(env, "%d,%d,%d", &T[0], &T[1], &T[2]);
+  for (int d = 0; d < 3; d++) T[d] = std::max(1, std::min(T[d], s.ngrid[d]));
+  for (;;) {
+    for (int d = 0; d < 3; d++) {
+      a.K[d] = s.ngrid[d];
+      a.T[d] = T[d];
+      a.nt[d] = (s.ngrid[d] + T[d] - 1) / T[d];
+      a.H[d] = T[d] + n - 1;
+    }
+    a.hvol = (size_t)a.H[0] * a.H[1] * a.H[2];
+    bool ok = true;
+    for (int d = 0; d < 3; d++) ok = ok && pme_max_candidates(a.K[d], a.T[d], n) <= 4;
+    a.ch = 64;
+    while (a.ch > 8 && pme_lds_bytes(a, true) > 65536) a.ch /= 2;
+    ok = ok && pme_lds_bytes(a, true) <= 65536;
+    if (ok) break;
+    int dm = 0;
+    for (int d = 1; d < 3; d++)
+      if (T[d] > T[dm]) dm = d;
+    if (T[dm] == 1) {
+      set_err(c, "PME order %d on a %dx%dx%d grid does not fit the tiled spread", n, s.ngrid[0], s.ngrid[1],
+              s.ngrid[2]);
+      return PMX_ERR_NOTIMPL;
+    }
+    T[dm] = (T[dm] + 1) / 2;
+  }
+  a.ntile = a.nt[0] * a.nt[1] * a.nt[2];
+  a.gstride = c->gstride;
+  const size_t nitem = (size_t)s.P * a.nchrg;
+  TRY_RC(dalloc(c, &c->item_s, 3 * nitem));
+  TRY_RC(dalloc(c, &c->item_f, 3 * nitem));
+  TRY_RC(dalloc(c, &c->item_tile, nitem));
+  TRY_RC(dalloc(c, &c->item_pos, nitem));
+  TRY_RC(dalloc(c, &c->sorted, nitem));
+  TRY_RC(dalloc(c, &c->bin_cnt, (size_t)s.P * a.ntile));
+  TRY_RC(dalloc(c, &c->bin_off, (size_t)s.P * a.ntile));
+  TRY_RC(dalloc(c, &c->hbuf, (size_t)s.P * a.ntile * a.hvol));
+  a.item_s = c->item_s;
+  a.item_f = c->item_f;
+  a.item_tile = c->item_tile;
+  a.item_pos = c->item_pos;
+  a.sorted = c->sorted;
+  a.bin_cnt = c->bin_cnt;
+  a.bin_off = c->bin_off;
+  a.hbuf = c->hbuf;
+  a.grid = c->grid;
+  a.q = c->q;
+  a.ichrg = c->ichrg;
+  return PMX_OK;
 }
 
 int build_device(pmx_ctx *c, const pmx_system *sys) {
@@ -257,6 +346,20 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
       }
     }
     TRY(upload(c, &c->infl, W.data(), W.size()));
+    TRY(setup_pme_tiles(c));
+  }
+  {
+    const size_t pair = (size_t)pair_grid_x(N) * P * PAIR_NV;
+    const size_t fft = (size_t)(c->pme ? s.ngrid[1] : 0) * P * FFT_NV;
+    const size_t intra = (size_t)c->g_intra * P * INTRA_NV;
+    const size_t fin = (size_t)c->g_intra * P * FIN_NV;
+    c->off_pair = 0;
+    c->off_fft = pair;
+    c->off_ecor = c->off_fft + fft;
+    c->off_bond = c->off_ecor + intra;
+    c->off_bend = c->off_bond + intra;
+    c->off_fin = c->off_bend + intra;
+    TRY(dalloc(c, &c->part, c->off_fin + fin));
   }
   TRY(dalloc(c, &c->acc, (size_t)P * ACC_NUM));
   TRY(dalloc(c, &c->ake, 1));
@@ -335,6 +438,14 @@ int build_list(pmx_ctx *c) {
   return PMX_ERR_RANGE;
 }
 
+void add_stage(RedArgs &r, size_t off, int G, int nv, const std::vector<int> &slots) {
+  RedStage &st = r.st[r.nstage++];
+  st.off = off;
+  st.G = G;
+  st.nv = nv;
+  for (int v = 0; v < nv; v++) st.slot[v] = slots[v];
+}
+
 IntraArgs intra_args(pmx_ctx *c, bool pimd) {
   Setup &s = c->s;
   IntraArgs a{};
@@ -354,7 +465,6 @@ IntraArgs intra_args(pmx_ctx *c, bool pimd) {
   a.fx = c->fx;
   a.fy = c->fy;
   a.fz = c->fz;
-  a.acc = c->acc;
   a.nbond = c->nbond;
   a.b1 = c->b1;
   a.b2 = c->b2;
@@ -419,7 +529,12 @@ pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *
     delete c;
     return fail("pmx_create: stream creation failed");
   }
-  for (auto &e : c->ev) hipEventCreate(&e);
+  for (auto &e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      pmx_destroy(c);
+      return fail("pmx_create: event creation failed");
+    }
+  }
   if (build_device(c, sys) != PMX_OK) {
     std::string e = c->err;
     pmx_destroy(c);
@@ -430,12 +545,12 @@ pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *
 
 void pmx_destroy(pmx_ctx *c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  if (c->st) hipStreamSynchronize(c->st);
-  for (void *p : c->allocs) hipFree(p);
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (void *p : c->allocs) (void)hipFree(p);
   for (auto &e : c->ev)
-    if (e) hipEventDestroy(e);
-  if (c->st) hipStreamDestroy(c->st);
+    if (e) (void)hipEventDestroy(e);
+  if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
 
@@ -503,12 +618,12 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   HIPCHK(hipSetDevice(c->device));
   memset(out, 0, sizeof(*out));
   hipStream_t st = c->st;
+  RedArgs red{};
   HIPCHK(hipEventRecord(c->ev[0], st));
   // --- initial (energy_control_initial.c:168-208)
   HIPCHK(hipMemsetAsync(c->fx, 0, NP * sizeof(double), st));
   HIPCHK(hipMemsetAsync(c->fy, 0, NP * sizeof(double), st));
   HIPCHK(hipMemsetAsync(c->fz, 0, NP * sizeof(double), st));
-  HIPCHK(hipMemsetAsync(c->acc, 0, (size_t)P * ACC_NUM * sizeof(double), st));
   HIPCHK(hipMemsetAsync(c->ake, 0, sizeof(double), st));
   if (P > 1 && !c->xmod_given) {
     hipLaunchKernelGGL(k_centroid, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm);
@@ -575,32 +690,34 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     a.fx = c->fx;
     a.fy = c->fy;
     a.fz = c->fz;
-    a.acc = c->acc;
+    a.part = c->part + c->off_pair;
     HIPCHK(launch_pair(a, st));
+    add_stage(red, c->off_pair, pair_grid_x(N), PAIR_NV,
+              {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR});
   }
   HIPCHK(hipEventRecord(c->ev[2], st));
   // --- reciprocal space (energy_control_inter_recip.c:119-355)
   const int nchrg = (int)s.ichrg.size();
   if (c->pme && !(flags & PMX_EVAL_NO_RECIP)) {
-    PmeArgs a{};
-    a.N = N;
-    a.P = P;
-    a.nchrg = nchrg;
-    a.n_interp = c->par.n_interp;
-    for (int d = 0; d < 3; d++) a.K[d] = s.ngrid[d];
-    a.gstride = c->gstride;
+    PmeArgs a = c->pa;
     a.x = c->x;
     a.y = c->y;
     a.z = c->z;
-    a.q = c->q;
-    a.ichrg = c->ichrg;
     a.cell = c->cell;
-    a.grid = c->grid;
     a.fx = c->fx;
     a.fy = c->fy;
     a.fz = c->fz;
-    HIPCHK(hipMemsetAsync(c->grid, 0, c->gstride * P * sizeof(double), st));
-    hipLaunchKernelGGL(k_pme_spread, dim3(blocks((size_t)nchrg * P)), dim3(256), 0, st, a);
+    const size_t nitem = (size_t)P * nchrg, nbin = (size_t)P * a.ntile;
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks(nbin)), dim3(256), 0, st, c->bin_cnt, 0, nbin);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pme_bin, dim3(blocks(nitem)), dim3(256), 0, st, a);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pme_scan, dim3(P), dim3(256), 0, st, a);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pme_scatter, dim3(blocks(nitem)), dim3(256), 0, st, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(launch_pme_spread(a, st));
+    hipLaunchKernelGGL(k_pme_fold, dim3(blocks((size_t)P * c->gstride)), dim3(256), 0, st, a);
     HIPCHK(hipGetLastError());
     FftArgs f{};
     f.P = P;
@@ -616,7 +733,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     f.infl = c->infl;
     f.falp2 = 4.0 * s.alp_ewd * s.alp_ewd;
     f.cell = c->cell;
-    f.acc = c->acc;
+    f.part = c->part + c->off_fft;
     const int Ka = s.ngrid[0], Kb = s.ngrid[1], Kc = s.ngrid[2], Ha = Ka / 2 + 1;
     hipLaunchKernelGGL(k_fft_fwd_ab, dim3(Kc, P), dim3(256), (size_t)Ka * Kb * sizeof(double2), st, f);
     HIPCHK(hipGetLastError());
@@ -624,23 +741,40 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_fft_inv_ba, dim3(Kc, P), dim3(256), (size_t)Ka * Kb * sizeof(double2), st, f);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_pme_gather, dim3(blocks((size_t)nchrg * P)), dim3(256), 0, st, a);
-    HIPCHK(hipGetLastError());
+    HIPCHK(launch_pme_gather(a, st));
+    add_stage(red, c->off_fft, Kb, FFT_NV, {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
   }
   IntraArgs ia = intra_args(c, pimd);
+  const int bsP = bead_block(P);
+  auto igrid = [&](size_t items) { return (unsigned)std::min<size_t>(c->g_intra, (items + bsP - 1) / bsP); };
+  const std::initializer_list<int> intra_slots = {0, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23};
+  auto intra_stage = [&](size_t off, unsigned G, int eslot) {
+    std::vector<int> sl(intra_slots);
+    sl[0] = eslot;
+    add_stage(red, off, (int)G, INTRA_NV, sl);
+  };
   if (nchrg > 0 && ia.necor > 0) {
-    hipLaunchKernelGGL(k_ecor, dim3(blocks((size_t)ia.necor * P)), dim3(256), 0, st, ia);
+    const unsigned G = igrid((size_t)ia.necor * P);
+    ia.part = c->part + c->off_ecor;
+    hipLaunchKernelGGL(k_ecor, dim3(G), dim3(bsP), 0, st, ia);
     HIPCHK(hipGetLastError());
+    intra_stage(c->off_ecor, G, ACC_VECOR);
   }
   HIPCHK(hipEventRecord(c->ev[3], st));
   // --- intramolecular (energy_control_intra.c:33-331)
   if (c->nbond) {
-    hipLaunchKernelGGL(k_bond, dim3(blocks((size_t)c->nbond * P)), dim3(256), 0, st, ia);
+    const unsigned G = igrid((size_t)c->nbond * P);
+    ia.part = c->part + c->off_bond;
+    hipLaunchKernelGGL(k_bond, dim3(G), dim3(bsP), 0, st, ia);
     HIPCHK(hipGetLastError());
+    intra_stage(c->off_bond, G, ACC_VBOND);
   }
   if (c->nbend) {
-    hipLaunchKernelGGL(k_bend, dim3(blocks((size_t)c->nbend * P)), dim3(256), 0, st, ia);
+    const unsigned G = igrid((size_t)c->nbend * P);
+    ia.part = c->part + c->off_bend;
+    hipLaunchKernelGGL(k_bend, dim3(G), dim3(bsP), 0, st, ia);
     HIPCHK(hipGetLastError());
+    intra_stage(c->off_bend, G, ACC_VBEND);
   }
   // --- PIMD: f/P and virial correction, then ghosts, then modes
   if (pimd) {
@@ -656,9 +790,12 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     a.fx = c->fx;
     a.fy = c->fy;
     a.fz = c->fz;
-    a.acc = c->acc;
-    hipLaunchKernelGGL(k_pimd_finish, dim3(blocks(NP)), dim3(256), 0, st, a);
+    a.part = c->part + c->off_fin;
+    const unsigned G = igrid(NP);
+    hipLaunchKernelGGL(k_pimd_finish, dim3(G), dim3(bsP), 0, st, a);
     HIPCHK(hipGetLastError());
+    add_stage(red, c->off_fin, (int)G, FIN_NV,
+              {ACC_C11, ACC_C22, ACC_C33, ACC_C12, ACC_C21, ACC_C13, ACC_C31, ACC_C23, ACC_C32, ACC_AKVIR});
   }
   if (c->nghost) {
     hipLaunchKernelGGL(k_ghost_force, dim3(blocks((size_t)c->nghost * P)), dim3(256), 0, st, c->nghost, P,
@@ -683,6 +820,11 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     hipLaunchKernelGGL(k_mode_forces, dim3(blocks((size_t)N * 3)), dim3(256), 0, st, a);
     HIPCHK(hipGetLastError());
   }
+  red.P = P;
+  red.part = c->part;
+  red.acc = c->acc;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(blocks((size_t)P * ACC_NUM * 64)), dim3(256), 0, st, red);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(c->ev[4], st));
   HIPCHK(hipMemcpyAsync(c->acc_h.data(), c->acc, c->acc_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   double ake = 0.0;
@@ -691,10 +833,10 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   {
     float t;
     for (int k = 0; k < 4; k++) {
-      hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]);
+      (void)hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]);
       c->ms[k] = t;
     }
-    hipEventElapsedTime(&t, c->ev[0], c->ev[4]);
+    (void)hipEventElapsedTime(&t, c->ev[0], c->ev[4]);
     c->ms[4] = t;
   }
   // --- assemble (energy_control_inter_real.c:188-197, _recip.c:340-347, _intra.c)

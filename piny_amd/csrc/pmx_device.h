@@ -1,0 +1,51 @@
+// Device helpers shared by the kernel files (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pmx_kernels.h"
+
+namespace pmx {
+
+__device__ __forceinline__ int nint_piny(double x) {
+  // NINT with SIMP_NINT, defines.h:96-97
+  return (int)((x) >= 0.0 ? ((x) + 0.5) : ((x)-0.5));
+}
+
+__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// Per-bead sums over the threads of one workgroup, written as one row per
+// (block, bead): part[(blk*P + bead0 + k)*NV + v] for k < nb.  Thread t holds
+// bead slot t % nb, so a workgroup covers nb beads.  The order of the sums
+// is fixed, so the result does not depend on scheduling.  Every thread of
+// the block must call it.
+template <int NV>
+__device__ __forceinline__ void block_partials(const double (&v)[NV], int nb, int bead0, int P, double *part,
+                                               size_t blk) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x, lane = tid & 63, bs = blockDim.x;
+  const bool shfl = (64 % nb) == 0 && (bs & 63) == 0;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = v[k];
+    if (shfl) {
+      for (int off = 32; off >= nb; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (lane < nb) red[(tid >> 6) * nb + lane] = s;
+    } else {
+      red[tid] = s;
+    }
+    __syncthreads();
+    if (tid < nb && bead0 + tid < P) {
+      double t = 0.0;
+      if (shfl) {
+        for (int w = 0; w < (bs >> 6); w++) t += red[w * nb + tid];
+      } else {
+        for (int u = tid; u < bs; u += nb) t += red[u];
+      }
+      part[(blk * P + bead0 + tid) * NV + k] = t;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace pmx

@@ -32,18 +32,31 @@ struct PairArgs {
   PairTypes pt;
   Cell cell;
   double *fx, *fy, *fz;
-  double *acc;  // [P][ACC_NUM]
+  double *part;  // per-block partial sums [gridDim.x][P][PAIR_NV]
 };
 
+// PME on a grid cut into tiles of T[0] x T[1] x T[2] points.  Charge (c, b)
+// (charge c, bead b) is binned by the tile that holds the first point of its
+// interpolation support; its support then lies in that tile's halo box of
+// H = T + n - 1 points per axis, which one workgroup holds in LDS.
 struct PmeArgs {
-  int N, P, nchrg, n_interp;
-  int K[3];
-  size_t gstride;  // doubles per bead grid
+  int N, P, nchrg, n;  // n: interpolation order
+  int K[3], T[3], nt[3], H[3];
+  int ntile;           // tiles per bead
+  int ch;              // charges per weight chunk in LDS
+  size_t gstride;      // doubles per bead grid
+  size_t hvol;         // H[0]*H[1]*H[2]
   const double *x, *y, *z;
   const double *q;
   const int *ichrg;
   Cell cell;
-  double *grid;
+  int *item_s;         // [P*nchrg][3] first support point per axis (0-based)
+  double *item_f;      // [P*nchrg][3] fractional grid offset per axis
+  int *item_tile, *item_pos;
+  int *bin_cnt, *bin_off;  // [P*ntile]
+  int *sorted;         // [P*nchrg] item ids in bin order
+  double *hbuf;        // [P*ntile*hvol] halo boxes of spread charge
+  double *grid;        // [P*gstride]
   double *fx, *fy, *fz;
 };
 
@@ -57,7 +70,7 @@ struct FftArgs {
   const double *infl;    // [Kc][Kb][Ka/2+1] preg*bweight on the sphere list
   double falp2;
   Cell cell;
-  double *acc;
+  double *part;  // [Kb][P][FFT_NV]
 };
 
 struct IntraArgs {
@@ -67,7 +80,7 @@ struct IntraArgs {
   const double *q;
   Cell cell;
   double *fx, *fy, *fz;
-  double *acc;
+  double *part;  // [gridDim.x][P][INTRA_NV], set per kernel
   // bonds
   int nbond;
   const int *b1, *b2, *btyp;
@@ -87,7 +100,7 @@ struct FinishArgs {
   int N, P;
   const double *x, *y, *z, *xm, *ym, *zm;
   double *fx, *fy, *fz;
-  double *acc;
+  double *part;  // [gridDim.x][P][FIN_NV]
 };
 
 struct ModeArgs {
@@ -97,6 +110,27 @@ struct ModeArgs {
   const double *prekf, *veig;
   const double *cs;  // [2P] cos, sin of 2 pi m/P
   double *ake;
+};
+
+// Values per (block, bead) partial-sum row of each stage.
+constexpr int PAIR_NV = 9;   // vvdw, vcoul, 6 virial, pair count
+constexpr int FFT_NV = 7;    // vrecip, 6 virial
+constexpr int INTRA_NV = 7;  // energy, 6 virial
+constexpr int FIN_NV = 10;   // 9 virial corrections, akvir
+
+// One stage of partial sums: part[off + (g*P + b)*nv + v] for g < G holds
+// the contribution of block g to accumulator slot[v] of bead b.
+struct RedStage {
+  size_t off;
+  int G, nv;
+  int slot[12];
+};
+constexpr int MAX_RED_STAGES = 8;
+struct RedArgs {
+  int P, nstage;
+  RedStage st[MAX_RED_STAGES];
+  const double *part;
+  double *acc;  // [P][ACC_NUM], every entry written
 };
 
 __global__ void k_bead_to_atom(int N, int P, const double *src, double *dst);
@@ -109,8 +143,12 @@ __global__ void k_list_check(int N, int P, const double *x, const double *y, con
                              const double *x0, const double *y0, const double *z0, double *b0x, double *b0y,
                              double *b0z, unsigned long long *maxes);
 __global__ void k_list_build(ListArgs a);
-__global__ void k_pme_spread(PmeArgs a);
-__global__ void k_pme_gather(PmeArgs a);
+__global__ void k_pme_bin(PmeArgs a);
+__global__ void k_pme_scan(PmeArgs a);
+__global__ void k_pme_scatter(PmeArgs a);
+__global__ void k_pme_fold(PmeArgs a);
+__global__ void k_fill_int(int *p, int v, size_t n);
+__global__ void k_reduce_partials(RedArgs a);
 __global__ void k_fft_fwd_ab(FftArgs a);
 __global__ void k_fft_c_conv(FftArgs a);
 __global__ void k_fft_inv_ba(FftArgs a);
@@ -123,5 +161,14 @@ __global__ void k_ghost_force(int ng, int P, const int *gatom, const int *gncomp
 __global__ void k_mode_forces(ModeArgs a);
 
 hipError_t launch_pair(const PairArgs &a, hipStream_t st);
+int pair_grid_x(int N);
+// LDS bytes of the spread and gather workgroups for a given tiling
+size_t pme_lds_bytes(const PmeArgs &a, bool gather);
+int pme_max_candidates(int K, int T, int n);
+hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st);
+hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st);
+// block size for grid-stride kernels whose threads keep one bead: the
+// largest multiple of P not above 256
+inline int bead_block(int P) { return 256 - (256 % P); }
 
 }  // namespace pmx

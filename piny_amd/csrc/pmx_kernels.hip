@@ -6,33 +6,34 @@
 // name the PINY routine whose arithmetic each kernel restates.
 #include <hip/hip_runtime.h>
 
-#include "pmx_kernels.h"
+#include "pmx_device.h"
 
 namespace pmx {
 
 // ---------------------------------------------------------------------------
 // helpers
 
-__device__ __forceinline__ int nint_piny(double x) {
-  // NINT with SIMP_NINT, defines.h:96-97
-  return (int)((x) >= 0.0 ? ((x) + 0.5) : ((x)-0.5));
+__global__ void k_fill_int(int *p, int v, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
 }
 
-__device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
-
-// Sum of one value over the lanes that hold the same bead (lane mod P) and
-// one atomic add per bead into dst[bead*stride].  Every lane of the wave must
-// call it.  P must divide 64; other bead counts take the per-lane path.
-__device__ __forceinline__ void bead_reduce_add(double v, int P, int bead, bool active, double *dst,
-                                                int stride) {
-  if (P <= 64 && (64 % P) == 0) {
-    v = active ? v : 0.0;
-    for (int off = 32; off >= P; off >>= 1) v += __shfl_xor(v, off, 64);
-    const int lane = threadIdx.x & 63;
-    if (lane < P && v != 0.0) atomic_add(dst + (size_t)lane * stride, v);
-  } else if (active && v != 0.0) {
-    atomic_add(dst + (size_t)bead * stride, v);
+// Sums the partial rows of every stage into the per-bead accumulators, one
+// wavefront per (bead, slot); blocks are summed in a fixed order.
+__global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {
+  const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (w >= a.P * ACC_NUM) return;  // uniform over the wavefront
+  const int b = w / ACC_NUM, slot = w % ACC_NUM;
+  double s = 0.0;
+  for (int si = 0; si < a.nstage; si++) {
+    const RedStage &r = a.st[si];
+    for (int v = 0; v < r.nv; v++) {
+      if (r.slot[v] != slot) continue;
+      for (int g = lane; g < r.G; g += 64) s += a.part[r.off + ((size_t)g * a.P + b) * r.nv + v];
+    }
   }
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) a.acc[(size_t)b * ACC_NUM + slot] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -351,14 +352,8 @@ __global__ void __launch_bounds__(256) k_pair_f64(PairArgs a) {
       atomic_add(a.fz + ib, fzi);
     }
   }
-  double vals[9] = {vlj, vc, p11, p22, p33, p12, p13, p23, npr};
-  const int slots[9] = {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR};
-#pragma unroll
-  for (int v = 0; v < 9; v++) {
-    double s = vals[v];
-    for (int off = 32; off >= B; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (slot == 0 && bact && s != 0.0) atomic_add(a.acc + (size_t)bead * ACC_NUM + slots[v], s);
-  }
+  const double vals[PAIR_NV] = {vlj, vc, p11, p22, p33, p12, p13, p23, npr};
+  block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x);
 }
 
 template __global__ void k_pair_f64<1>(PairArgs);
@@ -370,272 +365,7 @@ template __global__ void k_pair_f64<32>(PairArgs);
 template __global__ void k_pair_f64<64>(PairArgs);
 
 // ---------------------------------------------------------------------------
-// PME.  B-spline weights of one charged site and one axis,
-// ewald3d_pme.c:241-337 (spread) and :516-530 (derivatives).
-struct Bsp {
-  double mn[MAX_INTERP + 1], dmn[MAX_INTERP + 1];
-  int ig[MAX_INTERP + 1];  // 0-based grid index of point j
-};
-
-__device__ __forceinline__ void bspline_axis(double s, int K, int n, bool deriv, Bsp &b) {
-  double t = s - nint_piny((s - 0.5));
-  const double sc = t * (double)K;
-  const int it = (int)(sc);
-  const double frac = sc - (double)(it);
-  double u[MAX_INTERP + 1];
-  for (int j = 1; j <= n; j++) {
-    u[j] = frac + (double)(j - 1);
-    int ia = it - (j - 2);
-    ia = (ia > 0 ? ia : K + ia);
-    // PINY's index is ia in 1..K; the wrap below only acts for a coordinate
-    // exactly on the cell face (ia = K+1) and keeps a non-finite coordinate
-    // inside the grid.
-    b.ig[j] = (((ia - 1) % K) + K) % K;
-  }
-  double *mn = b.mn, *dmn = b.dmn;
-  mn[1] = 1.0 - fabs(u[1] - 1.0);
-  mn[2] = 1.0 - fabs(u[2] - 1.0);
-  for (int j = 3; j <= n; j++) mn[j] = 0.0;
-  for (int o = 3; o <= n; o++) {
-    const double rn = (double)o, rn1 = 1.0 / ((double)(o - 1));
-    for (int j = o; j >= 2; j--) mn[j] = (u[j] * mn[j] + (rn - u[j]) * mn[j - 1]) * rn1;
-    mn[1] = u[1] * mn[1] * rn1;
-    if (deriv && o == n - 1) {
-      dmn[1] = mn[1];
-      for (int j = 2; j <= n; j++) dmn[j] = mn[j] - mn[j - 1];
-    }
-  }
-}
-
-// charge spreading (ewald3d_pme.c:342-356), one thread per (charge, bead)
-__global__ void __launch_bounds__(256) k_pme_spread(PmeArgs a) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)a.nchrg * a.P) return;
-  const int ic = (int)(t / a.P), b = (int)(t % a.P);
-  const int i = a.ichrg[ic];
-  const size_t ib = (size_t)i * a.P + b;
-  const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
-  const double *hi = a.cell.hi;
-  const double sa = x * hi[0] + y * hi[3] + z * hi[6];
-  const double sb = x * hi[1] + y * hi[4] + z * hi[7];
-  const double sc = x * hi[2] + y * hi[5] + z * hi[8];
-  Bsp ba, bb, bc;
-  const int n = a.n_interp;
-  bspline_axis(sa, a.K[0], n, false, ba);
-  bspline_axis(sb, a.K[1], n, false, bb);
-  bspline_axis(sc, a.K[2], n, false, bc);
-  const double q = a.q[i];
-  double *g = a.grid + (size_t)b * a.gstride;
-  const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
-  for (int jc = 1; jc <= n; jc++) {
-    for (int jb = 1; jb <= n; jb++) {
-      const double wbc = bb.mn[jb] * bc.mn[jc];
-      const int obc = bc.ig[jc] * Kab + bb.ig[jb] * Ka;
-      for (int ja = 1; ja <= n; ja++) atomic_add(g + obc + ba.ig[ja], ba.mn[ja] * wbc * q);
-    }
-  }
-}
-
-// force gather (ewald3d_pme.c:534-587), one thread per (charge, bead)
-__global__ void __launch_bounds__(256) k_pme_gather(PmeArgs a) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)a.nchrg * a.P) return;
-  const int ic = (int)(t / a.P), b = (int)(t % a.P);
-  const int i = a.ichrg[ic];
-  const size_t ib = (size_t)i * a.P + b;
-  const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
-  const double *hi = a.cell.hi;
-  const double sa = x * hi[0] + y * hi[3] + z * hi[6];
-  const double sb = x * hi[1] + y * hi[4] + z * hi[7];
-  const double sc = x * hi[2] + y * hi[5] + z * hi[8];
-  Bsp ba, bb, bc;
-  const int n = a.n_interp;
-  bspline_axis(sa, a.K[0], n, true, ba);
-  bspline_axis(sb, a.K[1], n, true, bb);
-  bspline_axis(sc, a.K[2], n, true, bc);
-  const double q = a.q[i];
-  const double *g = a.grid + (size_t)b * a.gstride;
-  const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
-  const double ga = (double)a.K[0], gb = (double)a.K[1], gc = (double)a.K[2];
-  double da = 0.0, db = 0.0, dc = 0.0;
-  for (int jc = 1; jc <= n; jc++) {
-    for (int jb = 1; jb <= n; jb++) {
-      const int obc = bc.ig[jc] * Kab + bb.ig[jb] * Ka;
-      const double mbc = bb.mn[jb] * bc.mn[jc], dbc = bb.dmn[jb] * bc.mn[jc], bdc = bb.mn[jb] * bc.dmn[jc];
-      for (int ja = 1; ja <= n; ja++) {
-        const double phi = g[obc + ba.ig[ja]];
-        da += ba.dmn[ja] * mbc * phi;
-        db += ba.mn[ja] * dbc * phi;
-        dc += ba.mn[ja] * bdc * phi;
-      }
-    }
-  }
-  da *= ga * q;
-  db *= gb * q;
-  dc *= gc * q;
-  a.fx[ib] -= da * hi[0] + db * hi[1] + dc * hi[2];
-  a.fy[ib] -= da * hi[3] + db * hi[4] + dc * hi[5];
-  a.fz[ib] -= da * hi[6] + db * hi[7] + dc * hi[8];
-}
-
-// --- 1D complex FFTs of power-of-two length held in LDS, cooperative over
-// the block: in-place radix-2 decimation in time after a bit reversal.
-// Line l element e lives at buf[l*dist + e*stride].  sign -1: exp(-i),
-// +1: exp(+i); unnormalised, like FFTPACK's DCFFTF/DCFFTB.
-__device__ void lds_fft_pow2(double2 *buf, int K, int logK, int nlines, int stride, int dist, int sign,
-                             const double2 *__restrict__ tw) {
-  const int nt = blockDim.x, tid = threadIdx.x;
-  for (int w = tid; w < nlines * K; w += nt) {
-    const int l = w / K, e = w % K;
-    const int r = (int)(__brev((unsigned)e) >> (32 - logK));
-    if (r > e) {
-      double2 *p = buf + (size_t)l * dist;
-      const double2 t0 = p[e * stride];
-      p[e * stride] = p[r * stride];
-      p[r * stride] = t0;
-    }
-  }
-  __syncthreads();
-  for (int len = 2, s = 1; len <= K; len <<= 1, s++) {
-    const int half = len >> 1;
-    const int tstep = K / len;
-    for (int w = tid; w < nlines * (K >> 1); w += nt) {
-      const int l = w / (K >> 1), bfly = w % (K >> 1);
-      const int grp = bfly / half, pos = bfly % half;
-      const int e0 = grp * len + pos, e1 = e0 + half;
-      const double2 twv = tw[pos * tstep];  // exp(-2 pi i m/K) = (cos, -sin)
-      const double wr = twv.x, wi = sign < 0 ? twv.y : -twv.y;
-      double2 *p = buf + (size_t)l * dist;
-      const double2 u = p[e0 * stride], v = p[e1 * stride];
-      const double vr = v.x * wr - v.y * wi, vi = v.x * wi + v.y * wr;
-      p[e0 * stride] = make_double2(u.x + vr, u.y + vi);
-      p[e1 * stride] = make_double2(u.x - vr, u.y - vi);
-    }
-    __syncthreads();
-  }
-}
-
-// forward transform of one c-plane of one bead: real along a, then b.
-// LDS holds the plane as Kb lines of Ka complex values; only a' = 0..Ka/2
-// is kept (Hermitian half).  spec layout: [bead][c][b][a'] complex.
-__global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
-  extern __shared__ double2 sm[];
-  const int c = blockIdx.x, b = blockIdx.y;
-  const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
-  const double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
-  for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) sm[w] = make_double2(g[w], 0.0);
-  __syncthreads();
-  lds_fft_pow2(sm, Ka, a.logK[0], Kb, 1, Ka, -1, a.tw[0]);
-  lds_fft_pow2(sm, Kb, a.logK[1], Ha, Ka, 1, -1, a.tw[1]);
-  double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
-  for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
-    const int bb = w / Ha, aa = w % Ha;
-    sp[w] = sm[bb * Ka + aa];
-  }
-  (void)Kc;
-}
-
-// along c for one (bead, b) row: forward FFT, influence function over the
-// half-space sphere list (ewald3d_pme.c:367-430): energy, virial, and the
-// product with preg*bweight, then the inverse FFT.
-__global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
-  extern __shared__ double2 sm[];
-  const int bb = blockIdx.x, b = blockIdx.y;
-  const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
-  double2 *sp = a.spec + (size_t)b * a.sstride;
-  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
-    const int cc = w / Ha, aa = w % Ha;
-    sm[w] = sp[((size_t)cc * Kb + bb) * Ha + aa];
-  }
-  __syncthreads();
-  lds_fft_pow2(sm, Kc, a.logK[2], Ha, Ha, 1, -1, a.tw[2]);
-  const double *hi = a.cell.hi;
-  const double tpi = 2.0 * M_PI, falp2 = a.falp2;
-  const int kb = bb <= Kb / 2 ? bb : bb - Kb;
-  double e = 0.0, v11 = 0.0, v22 = 0.0, v33 = 0.0, v12 = 0.0, v13 = 0.0, v23 = 0.0;
-  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
-    const int cc = w / Ha, aa = w % Ha;
-    const double W = a.infl[((size_t)cc * Kb + bb) * Ha + aa];
-    double2 s = sm[w];
-    if (W != 0.0) {
-      const int kc = cc <= Kc / 2 ? cc : cc - Kc;
-      const double aka = tpi * (double)aa, akb = tpi * (double)kb, akc = tpi * (double)kc;
-      const double xk = (aka * hi[0] + akb * hi[1] + akc * hi[2]);
-      const double yk = (aka * hi[3] + akb * hi[4] + akc * hi[5]);
-      const double zk = (aka * hi[6] + akb * hi[7] + akc * hi[8]);
-      const double g2 = xk * xk + yk * yk + zk * zk;
-      const double wsum = (aa == 0) ? 0.5 : 1.0;  // a' = 0 holds both +k and -k
-      const double sm2 = (s.x * s.x + s.y * s.y) * wsum;
-      e += W * sm2;
-      const double prep = -2.0 * W * ((g2 / falp2) + 1.0) / g2 * sm2;
-      v11 += prep * xk * xk;
-      v22 += prep * yk * yk;
-      v33 += prep * zk * zk;
-      v12 += prep * xk * yk;
-      v13 += prep * xk * zk;
-      v23 += prep * yk * zk;
-    }
-    sm[w] = make_double2(s.x * W, s.y * W);
-  }
-  __syncthreads();
-  lds_fft_pow2(sm, Kc, a.logK[2], Ha, Ha, 1, +1, a.tw[2]);
-  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
-    const int cc = w / Ha, aa = w % Ha;
-    sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
-  }
-  // block reduction of the energy and virial of this row
-  __shared__ double red[8][4];
-  double vals[7] = {e, v11, v22, v33, v12, v13, v23};
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-#pragma unroll
-  for (int k = 0; k < 7; k++) {
-    double s = vals[k];
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) red[k][wv] = s;
-  }
-  __syncthreads();
-  if (threadIdx.x < 7) {
-    double s = 0.0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); k++) s += red[threadIdx.x][k];
-    const int slots[7] = {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23};
-    if (s != 0.0) atomic_add(a.acc + (size_t)b * ACC_NUM + slots[threadIdx.x], s);
-    // the k-space pressure also carries vnow on the diagonal (ewald3d_pme.c:412-414)
-    if (threadIdx.x == 0 && s != 0.0) {
-      atomic_add(a.acc + (size_t)b * ACC_NUM + ACC_P11, s);
-      atomic_add(a.acc + (size_t)b * ACC_NUM + ACC_P22, s);
-      atomic_add(a.acc + (size_t)b * ACC_NUM + ACC_P33, s);
-    }
-  }
-}
-
-// inverse transform of one c-plane: along b, then complex-to-real along a.
-__global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
-  extern __shared__ double2 sm[];
-  const int c = blockIdx.x, b = blockIdx.y;
-  const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
-  const double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
-  for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
-    const int bb = w / Ka, aa = w % Ka;
-    sm[w] = aa < Ha ? sp[bb * Ha + aa] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  lds_fft_pow2(sm, Kb, a.logK[1], Ha, Ka, 1, +1, a.tw[1]);
-  // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
-  for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
-    const int bb = w / Ka, aa = w % Ka;
-    if (aa >= Ha) {
-      const double2 v = sm[bb * Ka + (Ka - aa)];
-      sm[w] = make_double2(v.x, -v.y);
-    }
-  }
-  __syncthreads();
-  lds_fft_pow2(sm, Ka, a.logK[0], Kb, 1, Ka, +1, a.tw[0]);
-  double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
-  for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) g[w] = sm[w].x;
-}
-
-// ---------------------------------------------------------------------------
-// bonded terms, one thread per (term, bead)
+// bonded terms and excluded-pair corrections
 
 __device__ __forceinline__ void intra_sep(const IntraArgs &a, int i, int j, int b, double &dx, double &dy,
                                           double &dz) {
@@ -672,26 +402,30 @@ __device__ __forceinline__ void add_force(const IntraArgs &a, int i, int b, doub
   atomic_add(a.fz + ib, fz);
 }
 
-__device__ __forceinline__ void reduce_virial(const IntraArgs &a, int slot_e, double e, double p11, double p22,
-                                              double p33, double p12, double p13, double p23, int b,
-                                              bool act) {
-  double *acc = a.acc;
-  bead_reduce_add(e, a.P, b, act, acc + slot_e, ACC_NUM);
-  bead_reduce_add(p11, a.P, b, act, acc + ACC_P11, ACC_NUM);
-  bead_reduce_add(p22, a.P, b, act, acc + ACC_P22, ACC_NUM);
-  bead_reduce_add(p33, a.P, b, act, acc + ACC_P33, ACC_NUM);
-  bead_reduce_add(p12, a.P, b, act, acc + ACC_P12, ACC_NUM);
-  bead_reduce_add(p13, a.P, b, act, acc + ACC_P13, ACC_NUM);
-  bead_reduce_add(p23, a.P, b, act, acc + ACC_P23, ACC_NUM);
+// Grid-stride loop over (term, bead) items t = term*P + bead.  The block size
+// is a multiple of P and so is the grid stride: each thread keeps one bead,
+// accumulates its terms' energy and virial in registers, and the block
+// writes one partial row per bead.
+#define PMX_TERM_LOOP(nterm)                                                                       \
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)(nterm) * a.P;       \
+       t += (size_t)gridDim.x * blockDim.x)
+
+__device__ __forceinline__ void add_virial(double (&acc)[INTRA_NV], double e, double dx, double dy, double dz,
+                                           double fx, double fy, double fz) {
+  acc[0] += e;
+  acc[1] += dx * fx;
+  acc[2] += dy * fy;
+  acc[3] += dz * fz;
+  acc[4] += dx * fy;
+  acc[5] += dx * fz;
+  acc[6] += dy * fz;
 }
 
 // ecor, ecorr.c:28-265 (no switching on the table)
 __global__ void __launch_bounds__(256) k_ecor(IntraArgs a) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = t < (size_t)a.necor * a.P;
-  const int k = act ? (int)(t / a.P) : 0, b = act ? (int)(t % a.P) : 0;
-  double e = 0, p11 = 0, p22 = 0, p33 = 0, p12 = 0, p13 = 0, p23 = 0;
-  if (act) {
+  double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
+  PMX_TERM_LOOP(a.necor) {
+    const int k = (int)(t / a.P), b = (int)(t % a.P);
     const int i = a.e1[k], j = a.e2[k];
     double dx, dy, dz;
     intra_sep(a, i, j, b, dx, dy, dz);
@@ -702,58 +436,45 @@ __global__ void __launch_bounds__(256) k_ecor(IntraArgs a) {
     const double rsp = ((double)(kk - 3)) * a.e_dr + a.e_rmin;
     const double del = (r12 - rsp) * a.e_dri;
     const double q12 = a.q[i] * a.q[j];
-    e = spline4(a.ecor_tab, kk, del, false) * q12;
+    const double e = spline4(a.ecor_tab, kk, del, false) * q12;
     const double dvt = spline4(a.ecor_tab, kk, del, true) * q12;
     const double fx = dx * dvt, fy = dy * dvt, fz = dz * dvt;
     add_force(a, i, b, fx, fy, fz);
     add_force(a, j, b, -fx, -fy, -fz);
-    p11 = dx * fx;
-    p22 = dy * fy;
-    p33 = dz * fz;
-    p12 = dx * fy;
-    p13 = dx * fz;
-    p23 = dy * fz;
+    add_virial(acc, e, dx, dy, dz, fx, fy, fz);
   }
-  reduce_virial(a, ACC_VECOR, e, p11, p22, p33, p12, p13, p23, b, act);
+  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x);
 }
 
 // bond, bond.c:29-312
 __global__ void __launch_bounds__(256) k_bond(IntraArgs a) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = t < (size_t)a.nbond * a.P;
-  const int k = act ? (int)(t / a.P) : 0, b = act ? (int)(t % a.P) : 0;
-  double e = 0, p11 = 0, p22 = 0, p33 = 0, p12 = 0, p13 = 0, p23 = 0;
-  if (act) {
+  double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
+  PMX_TERM_LOOP(a.nbond) {
+    const int k = (int)(t / a.P), b = (int)(t % a.P);
     const int i = a.b1[k], j = a.b2[k], ty = a.btyp[k];
     const double *cc = a.bc + 7 * ty;
     double dx, dy, dz;
     intra_sep(a, i, j, b, dx, dy, dz);
     const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
     const double rr0 = r12 - a.beq[ty];
-    e = (((((cc[6] * rr0 + cc[5]) * rr0 + cc[4]) * rr0 + cc[3]) * rr0 + cc[2]) * rr0 + cc[1]) * rr0 + cc[0];
+    const double e =
+        (((((cc[6] * rr0 + cc[5]) * rr0 + cc[4]) * rr0 + cc[3]) * rr0 + cc[2]) * rr0 + cc[1]) * rr0 + cc[0];
     const double dvb = (((((6.0 * cc[6]) * rr0 + 5.0 * cc[5]) * rr0 + 4.0 * cc[4]) * rr0 + 3.0 * cc[3]) * rr0 +
                         2.0 * cc[2]) * rr0 + cc[1];
     const double pre = -dvb / r12;
     const double fx = dx * pre, fy = dy * pre, fz = dz * pre;
     add_force(a, i, b, fx, fy, fz);
     add_force(a, j, b, -fx, -fy, -fz);
-    p11 = dx * fx;
-    p22 = dy * fy;
-    p33 = dz * fz;
-    p12 = dx * fy;
-    p13 = dx * fz;
-    p23 = dy * fz;
+    add_virial(acc, e, dx, dy, dz, fx, fy, fz);
   }
-  reduce_virial(a, ACC_VBOND, e, p11, p22, p33, p12, p13, p23, b, act);
+  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x);
 }
 
 // bend, bend.c:28-434
 __global__ void __launch_bounds__(256) k_bend(IntraArgs a) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = t < (size_t)a.nbend * a.P;
-  const int k = act ? (int)(t / a.P) : 0, b = act ? (int)(t % a.P) : 0;
-  double e = 0, p11 = 0, p22 = 0, p33 = 0, p12 = 0, p13 = 0, p23 = 0;
-  if (act) {
+  double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
+  PMX_TERM_LOOP(a.nbend) {
+    const int k = (int)(t / a.P), b = (int)(t % a.P);
     const int i1 = a.a1[k], i2 = a.a2[k], i3 = a.a3[k], ty = a.atyp[k];
     const double *cc = a.ac + 7 * ty, *ss = a.as + 7 * ty;
     double dx12, dy12, dz12, dx23, dy23, dz23;
@@ -771,7 +492,7 @@ __global__ void __launch_bounds__(256) k_bend(IntraArgs a) {
     const double vbc = ((((((cc[6] * cost + cc[5]) * cost + cc[4]) * cost + cc[3]) * cost + cc[2]) * cost + cc[1]) *
                             cost + cc[0]);
     const double sisum = (((((ss[6] * sint + ss[5]) * sint + ss[4]) * sint + ss[3]) * sint + ss[2]) * sint);
-    e = vbc + sisum * cost + ss[1] * sint;
+    const double e = vbc + sisum * cost + ss[1] * sint;
     const double dvbc = (((((6.0 * cc[6]) * cost + 5.0 * cc[5]) * cost + 4.0 * cc[4]) * cost + 3.0 * cc[3]) * cost +
                          2.0 * cc[2]) * cost + cc[1];
     double dvbs = ((((5.0 * ss[6]) * sint + 4.0 * ss[5]) * sint + 3.0 * ss[4]) * sint + 2.0 * ss[3]) * sint + ss[2];
@@ -786,46 +507,45 @@ __global__ void __launch_bounds__(256) k_bend(IntraArgs a) {
     add_force(a, i1, b, f1x, f1y, f1z);
     add_force(a, i2, b, -(f1x + f3x), -(f1y + f3y), -(f1z + f3z));
     add_force(a, i3, b, f3x, f3y, f3z);
-    p11 = dx12 * f1x + dx23 * f3x;
-    p22 = dy12 * f1y + dy23 * f3y;
-    p33 = dz12 * f1z + dz23 * f3z;
-    p12 = dx12 * f1y + dx23 * f3y;
-    p13 = dx12 * f1z + dx23 * f3z;
-    p23 = dy12 * f1z + dy23 * f3z;
+    acc[0] += e;
+    acc[1] += dx12 * f1x + dx23 * f3x;
+    acc[2] += dy12 * f1y + dy23 * f3y;
+    acc[3] += dz12 * f1z + dz23 * f3z;
+    acc[4] += dx12 * f1y + dx23 * f3y;
+    acc[5] += dx12 * f1z + dx23 * f3z;
+    acc[6] += dy12 * f1z + dy23 * f3z;
   }
-  reduce_virial(a, ACC_VBEND, e, p11, p22, p33, p12, p13, p23, b, act);
+  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
 // PIMD finish: f /= P (energy_control_pimd.c:116-126) and the virial
-// correction -sum (x - xmod) (x) f (pimd_estimators.c:223-323), one thread
-// per (site, bead).
+// correction -sum (x - xmod) (x) f (pimd_estimators.c:223-323), grid-stride
+// over (site, bead) items.
 __global__ void __launch_bounds__(256) k_pimd_finish(FinishArgs a) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = t < (size_t)a.N * a.P;
-  const int i = act ? (int)(t / a.P) : 0, b = act ? (int)(t % a.P) : 0;
-  double c[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (act) {
-    const double fx = a.fx[t] / (double)a.P, fy = a.fy[t] / (double)a.P, fz = a.fz[t] / (double)a.P;
+  double c[FIN_NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const double dP = (double)a.P;
+  PMX_TERM_LOOP(a.N) {
+    const int i = (int)(t / a.P);
+    const double fx = a.fx[t] / dP, fy = a.fy[t] / dP, fz = a.fz[t] / dP;
     a.fx[t] = fx;
     a.fy[t] = fy;
     a.fz[t] = fz;
     if (a.P > 1) {
       const double dx = a.x[t] - a.xm[i], dy = a.y[t] - a.ym[i], dz = a.z[t] - a.zm[i];
-      c[0] = -(dx * fx);
-      c[1] = -(dy * fy);
-      c[2] = -(dz * fz);
-      c[3] = -(dx * fy);
-      c[4] = -(dy * fx);
-      c[5] = -(dx * fz);
-      c[6] = -(dz * fx);
-      c[7] = -(dy * fz);
-      c[8] = -(dz * fy);
-      c[9] = -(dx * fx + dy * fy + dz * fz);
+      c[0] -= dx * fx;
+      c[1] -= dy * fy;
+      c[2] -= dz * fz;
+      c[3] -= dx * fy;
+      c[4] -= dy * fx;
+      c[5] -= dx * fz;
+      c[6] -= dz * fx;
+      c[7] -= dy * fz;
+      c[8] -= dz * fy;
+      c[9] -= dx * fx + dy * fy + dz * fz;
     }
   }
-#pragma unroll
-  for (int k = 0; k < 10; k++) bead_reduce_add(c[k], a.P, b, act, a.acc + ACC_C11 + k, ACC_NUM);
+  block_partials<FIN_NV>(c, a.P, 0, a.P, a.part, blockIdx.x);
 }
 
 // distrib_ghost_force, ghost_control.c:71-125
@@ -900,13 +620,17 @@ __global__ void k_mode_forces(ModeArgs a) {
 // ---------------------------------------------------------------------------
 // launch wrappers
 
+int pair_grid_x(int N) {
+  const int waves = N < 8192 ? N : 8192;
+  return (waves + 3) / 4;
+}
+
 hipError_t launch_pair(const PairArgs &a, hipStream_t st) {
   // beads per slot: the smallest power of two covering P, at most 64
   int B = 1;
   while (B < a.P && B < 64) B <<= 1;
   const int nbg = (a.P + B - 1) / B;
-  const int waves = a.N < 8192 ? a.N : 8192;
-  dim3 grid((waves + 3) / 4, nbg), block(256);
+  dim3 grid(pair_grid_x(a.N), nbg), block(256);
   switch (B) {
     case 1: hipLaunchKernelGGL(k_pair_f64<1>, grid, block, 0, st, a); break;
     case 2: hipLaunchKernelGGL(k_pair_f64<2>, grid, block, 0, st, a); break;

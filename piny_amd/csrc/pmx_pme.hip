@@ -1,0 +1,508 @@
+// Particle-mesh Ewald on gfx950: charge binning, tiled spread and gather,
+// and the 3-D FFT with the influence-function convolution.
+//
+// Arithmetic restates ewald3d_pme.c (PINY): fractional coordinates and
+// B-spline weights (:241-337), charge stuffing (:342-356), the k-space sum
+// over the half-space sphere list (:367-430) and the force gather (:500-560).
+// The MI355X layout differs from the reference's loop order:
+//  - every (charge, bead) is binned by the grid tile holding the first point
+//    of its support (k_pme_bin / k_pme_scan / k_pme_scatter, a counting sort);
+//  - one workgroup per (tile, bead) spreads its charges into an LDS halo box
+//    of (T + n - 1)^3 points with LDS atomics, then writes the box out once
+//    (no global atomics, no grid clear);
+//  - k_pme_fold sums the overlapping halo boxes into the periodic grid;
+//  - the gather loads the same halo box of the potential into LDS and each
+//    wavefront reduces one charge's n^3 products.
+#include <hip/hip_runtime.h>
+
+#include "pmx_device.h"
+
+namespace pmx {
+
+namespace {
+
+constexpr int MAX_CAND = 4;  // halo boxes that can cover one grid point, per axis
+
+// B-spline weights M_n(frac + j - 1), j = 1..NI, by the recursion of
+// ewald3d_pme.c:296-337; dmn (order NI-1 differences, :516-530) if DERIV.
+template <int NI, bool DERIV>
+__device__ __forceinline__ void bspline(double frac, double (&mn)[NI + 1], double (&dmn)[NI + 1]) {
+  double u[NI + 1];
+#pragma unroll
+  for (int j = 1; j <= NI; j++) u[j] = frac + (double)(j - 1);
+  mn[1] = 1.0 - fabs(u[1] - 1.0);
+  mn[2] = 1.0 - fabs(u[2] - 1.0);
+#pragma unroll
+  for (int j = 3; j <= NI; j++) mn[j] = 0.0;
+#pragma unroll
+  for (int o = 3; o <= NI; o++) {
+    const double rn = (double)o, rn1 = 1.0 / ((double)(o - 1));
+#pragma unroll
+    for (int j = o; j >= 2; j--) mn[j] = (u[j] * mn[j] + (rn - u[j]) * mn[j - 1]) * rn1;
+    mn[1] = u[1] * mn[1] * rn1;
+    if (DERIV && o == NI - 1) {
+      dmn[1] = mn[1];
+#pragma unroll
+      for (int j = 2; j <= NI; j++) dmn[j] = mn[j] - mn[j - 1];
+    }
+  }
+}
+
+__device__ __forceinline__ void tile_origin(const PmeArgs &a, int tl, int &o0, int &o1, int &o2) {
+  const int ta = tl % a.nt[0], tb = (tl / a.nt[0]) % a.nt[1], tc = tl / (a.nt[0] * a.nt[1]);
+  o0 = ta * a.T[0];
+  o1 = tb * a.T[1];
+  o2 = tc * a.T[2];
+}
+
+// Weights of one chunk of a bin's charges into LDS: w[(c*3 + d)*NI + m] is
+// the weight of support point s_d + m (that is M_n index j = NI - m), with
+// the charge folded into axis 0; ls[c*3 + d] is the support start relative
+// to the tile origin.
+template <int NI, bool DERIV>
+__device__ __forceinline__ void chunk_weights(const PmeArgs &a, int lo, int base, int m, int o0, int o1, int o2,
+                                              double *w, double *dw, int *ls, int *atom) {
+  const int c = threadIdx.x;
+  if (c >= m) return;
+  const int it = a.sorted[lo + base + c];
+  const int ic = it % a.nchrg;
+  const int i = a.ichrg[ic];
+  const double q = a.q[i];
+  const int org[3] = {o0, o1, o2};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double mn[NI + 1], dmn[NI + 1];
+    bspline<NI, DERIV>(a.item_f[3 * (size_t)it + d], mn, dmn);
+    ls[c * 3 + d] = a.item_s[3 * (size_t)it + d] - org[d];
+    const double sc = (d == 0 && !DERIV) ? q : 1.0;
+#pragma unroll
+    for (int mm = 0; mm < NI; mm++) {
+      w[(c * 3 + d) * NI + mm] = mn[NI - mm] * sc;
+      if (DERIV) dw[(c * 3 + d) * NI + mm] = dmn[NI - mm];
+    }
+  }
+  if (DERIV) atom[c] = (int)(((size_t)i * a.P) + (it / a.nchrg));
+}
+
+}  // namespace
+
+// --- binning -----------------------------------------------------------------
+
+// one thread per item it = bead*nchrg + charge: fractional grid coordinates
+// (ewald3d_pme.c:241-262, 270-276), first support point per axis, bin slot
+__global__ void __launch_bounds__(256) k_pme_bin(PmeArgs a) {
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= a.P * a.nchrg) return;
+  const int b = it / a.nchrg, ic = it % a.nchrg;
+  const size_t ib = (size_t)a.ichrg[ic] * a.P + b;
+  const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
+  const double *hi = a.cell.hi;
+  double s[3];
+  s[0] = x * hi[0] + y * hi[3] + z * hi[6];
+  s[1] = x * hi[1] + y * hi[4] + z * hi[7];
+  s[2] = x * hi[2] + y * hi[5] + z * hi[8];
+  int tcoord[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int K = a.K[d];
+    const double t = s[d] - nint_piny((s[d] - 0.5));
+    const double sc = t * (double)K;
+    int ig = (int)(sc);
+    const double frac = sc - (double)ig;
+    // support points (0-based) ig - n + 1 .. ig, periodic; ig = K (a
+    // coordinate rounding onto the upper face) is the same set as ig = 0
+    ig = ((ig % K) + K) % K;
+    int s0 = ig - a.n + 1;
+    s0 = ((s0 % K) + K) % K;
+    a.item_f[3 * (size_t)it + d] = frac;
+    a.item_s[3 * (size_t)it + d] = s0;
+    tcoord[d] = s0 / a.T[d];
+  }
+  const int tile = (tcoord[2] * a.nt[1] + tcoord[1]) * a.nt[0] + tcoord[0];
+  a.item_tile[it] = tile;
+  a.item_pos[it] = atomicAdd(a.bin_cnt + (size_t)b * a.ntile + tile, 1);
+}
+
+// exclusive scan of the bin counts of one bead (one workgroup per bead)
+__global__ void __launch_bounds__(256) k_pme_scan(PmeArgs a) {
+  __shared__ int sums[256];
+  const int b = blockIdx.x, tid = threadIdx.x, nt = a.ntile;
+  const int per = (nt + 255) / 256;
+  const int lo = min(nt, tid * per), hi = min(nt, lo + per);
+  const int *cnt = a.bin_cnt + (size_t)b * nt;
+  int s = 0;
+  for (int t = lo; t < hi; t++) s += cnt[t];
+  sums[tid] = s;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int k = 0; k < 256; k++) {
+      const int v = sums[k];
+      sums[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = b * a.nchrg + sums[tid];
+  for (int t = lo; t < hi; t++) {
+    a.bin_off[(size_t)b * nt + t] = run;
+    run += cnt[t];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pme_scatter(PmeArgs a) {
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= a.P * a.nchrg) return;
+  const int b = it / a.nchrg;
+  a.sorted[a.bin_off[(size_t)b * a.ntile + a.item_tile[it]] + a.item_pos[it]] = it;
+}
+
+// --- spread --------------------------------------------------------------------
+
+// one workgroup per (tile, bead); LDS: halo box | weights | starts
+template <int NI>
+__global__ void __launch_bounds__(256) k_pme_spread(PmeArgs a) {
+  extern __shared__ double lds[];
+  const size_t hvol = a.hvol;
+  double *box = lds;
+  double *w = lds + hvol;
+  int *ls = (int *)(w + (size_t)a.ch * 3 * NI);
+  const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  for (size_t k = tid; k < hvol; k += blockDim.x) box[k] = 0.0;
+  const size_t bin = (size_t)b * a.ntile + tl;
+  const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
+  int o0, o1, o2;
+  tile_origin(a, tl, o0, o1, o2);
+  const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
+  constexpr int NP = NI * NI * NI;
+  for (int base = 0; base < cnt; base += a.ch) {
+    const int m = min(a.ch, cnt - base);
+    __syncthreads();
+    chunk_weights<NI, false>(a, lo, base, m, o0, o1, o2, w, nullptr, ls, nullptr);
+    __syncthreads();
+    for (int c = wv; c < m; c += nw) {
+      const int l0 = ls[c * 3], l1 = ls[c * 3 + 1], l2 = ls[c * 3 + 2];
+      const double *wa = w + (c * 3) * NI, *wb = wa + NI, *wc = wb + NI;
+#pragma unroll
+      for (int p0 = 0; p0 < NP; p0 += 64) {
+        const int p = p0 + lane;
+        if (p < NP) {
+          const int ma = p % NI, mb = (p / NI) % NI, mc = p / (NI * NI);
+          const double v = wa[ma] * wb[mb] * wc[mc];
+          atomicAdd(box + (l2 + mc) * H01 + (l1 + mb) * H0 + (l0 + ma), v);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  double *dst = a.hbuf + bin * hvol;
+  for (size_t k = tid; k < hvol; k += blockDim.x) dst[k] = box[k];
+}
+
+// Halo boxes of one axis that hold grid point p: (tile, local index) pairs.
+__device__ __forceinline__ int axis_candidates(int p, int K, int T, int nt, int n, int *tt, int *ll) {
+  int nc = 0;
+  for (int t = 0; t < nt; t++) {
+    const int org = t * T;
+    const int len = min(T, K - org) + n - 1;
+    int l = p - org;
+    if (l < 0) l += K;
+    for (; l < len && nc < MAX_CAND; l += K) {
+      tt[nc] = t;
+      ll[nc] = l;
+      nc++;
+    }
+  }
+  return nc;
+}
+
+// sums the halo boxes into the periodic grid, one thread per (bead, point)
+__global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)a.P * a.gstride) return;
+  const int b = (int)(t / a.gstride);
+  const size_t r = t % a.gstride;
+  const int Ka = a.K[0], Kb = a.K[1];
+  const int pa = (int)(r % Ka), pb = (int)((r / Ka) % Kb), pc = (int)(r / ((size_t)Ka * Kb));
+  int ta[MAX_CAND], la[MAX_CAND], tb[MAX_CAND], lb[MAX_CAND], tc[MAX_CAND], lc[MAX_CAND];
+  const int na = axis_candidates(pa, Ka, a.T[0], a.nt[0], a.n, ta, la);
+  const int nb = axis_candidates(pb, Kb, a.T[1], a.nt[1], a.n, tb, lb);
+  const int nc = axis_candidates(pc, a.K[2], a.T[2], a.nt[2], a.n, tc, lc);
+  const double *hb = a.hbuf + (size_t)b * a.ntile * a.hvol;
+  double s = 0.0;
+  for (int kc = 0; kc < nc; kc++)
+    for (int kb = 0; kb < nb; kb++)
+      for (int ka = 0; ka < na; ka++) {
+        const int tile = (tc[kc] * a.nt[1] + tb[kb]) * a.nt[0] + ta[ka];
+        s += hb[(size_t)tile * a.hvol + ((size_t)lc[kc] * a.H[1] + lb[kb]) * a.H[0] + la[ka]];
+      }
+  a.grid[t] = s;
+}
+
+// --- gather ---------------------------------------------------------------------
+
+// one workgroup per (tile, bead): the potential on the halo box into LDS,
+// then one wavefront per charge (ewald3d_pme.c:534-560)
+template <int NI>
+__global__ void __launch_bounds__(256) k_pme_gather(PmeArgs a) {
+  extern __shared__ double lds[];
+  const size_t hvol = a.hvol;
+  double *box = lds;
+  double *w = lds + hvol;
+  double *dw = w + (size_t)a.ch * 3 * NI;
+  int *ls = (int *)(dw + (size_t)a.ch * 3 * NI);
+  int *atom = ls + 3 * a.ch;
+  const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+  const size_t bin = (size_t)b * a.ntile + tl;
+  const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
+  if (cnt == 0) return;  // uniform over the workgroup
+  int o0, o1, o2;
+  tile_origin(a, tl, o0, o1, o2);
+  const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2];
+  const int H0 = a.H[0], H1 = a.H[1], H01 = H0 * H1;
+  const double *g = a.grid + (size_t)b * a.gstride;
+  for (size_t k = tid; k < hvol; k += blockDim.x) {
+    const int la = (int)(k % H0), lb = (int)((k / H0) % H1), lc = (int)(k / H01);
+    int ga = o0 + la, gb = o1 + lb, gc = o2 + lc;
+    ga %= Ka;
+    gb %= Kb;
+    gc %= Kc;
+    box[k] = g[((size_t)gc * Kb + gb) * Ka + ga];
+  }
+  const double *hi = a.cell.hi;
+  const double fa = (double)Ka, fb = (double)Kb, fc = (double)Kc;
+  constexpr int NP = NI * NI * NI;
+  for (int base = 0; base < cnt; base += a.ch) {
+    const int m = min(a.ch, cnt - base);
+    __syncthreads();
+    chunk_weights<NI, true>(a, lo, base, m, o0, o1, o2, w, dw, ls, atom);
+    __syncthreads();
+    for (int c = wv; c < m; c += nw) {
+      const int l0 = ls[c * 3], l1 = ls[c * 3 + 1], l2 = ls[c * 3 + 2];
+      const double *wa = w + (c * 3) * NI, *wb = wa + NI, *wc = wb + NI;
+      const double *dwa = dw + (c * 3) * NI, *dwb = dwa + NI, *dwc = dwb + NI;
+      double da = 0.0, db = 0.0, dc = 0.0;
+#pragma unroll
+      for (int p0 = 0; p0 < NP; p0 += 64) {
+        const int p = p0 + lane;
+        if (p < NP) {
+          const int ma = p % NI, mb = (p / NI) % NI, mc = p / (NI * NI);
+          const double phi = box[(l2 + mc) * H01 + (l1 + mb) * H0 + (l0 + ma)];
+          const double xa = wa[ma], xb = wb[mb], xc = wc[mc];
+          da += dwa[ma] * xb * xc * phi;
+          db += xa * dwb[mb] * xc * phi;
+          dc += xa * xb * dwc[mc] * phi;
+        }
+      }
+      for (int off = 32; off >= 1; off >>= 1) {
+        da += __shfl_xor(da, off, 64);
+        db += __shfl_xor(db, off, 64);
+        dc += __shfl_xor(dc, off, 64);
+      }
+      if (lane == 0) {
+        const int ib = atom[c];
+        const double q = a.q[ib / a.P];
+        da *= fa * q;
+        db *= fb * q;
+        dc *= fc * q;
+        a.fx[ib] -= da * hi[0] + db * hi[1] + dc * hi[2];
+        a.fy[ib] -= da * hi[3] + db * hi[4] + dc * hi[5];
+        a.fz[ib] -= da * hi[6] + db * hi[7] + dc * hi[8];
+      }
+    }
+  }
+}
+
+size_t pme_lds_bytes(const PmeArgs &a, bool gather) {
+  const size_t wt = (size_t)a.ch * 3 * a.n * sizeof(double) * (gather ? 2 : 1);
+  const size_t meta = (size_t)a.ch * (gather ? 4 : 3) * sizeof(int);
+  return a.hvol * sizeof(double) + wt + meta;
+}
+
+int pme_max_candidates(int K, int T, int n) {
+  const int nt = (K + T - 1) / T;
+  int mx = 0;
+  for (int p = 0; p < K; p++) {
+    int nc = 0;
+    for (int t = 0; t < nt; t++) {
+      const int org = t * T;
+      const int len = (T < K - org ? T : K - org) + n - 1;
+      int l = p - org;
+      if (l < 0) l += K;
+      for (; l < len; l += K) nc++;
+    }
+    mx = nc > mx ? nc : mx;
+  }
+  return mx;
+}
+
+#define PMX_PME_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
+
+hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
+  const dim3 grid(a.ntile, a.P), block(256);
+  const size_t lds = pme_lds_bytes(a, false);
+  switch (a.n) {
+#define X(NI) \
+  case NI: hipLaunchKernelGGL(k_pme_spread<NI>, grid, block, lds, st, a); break;
+    PMX_PME_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {
+  const dim3 grid(a.ntile, a.P), block(256);
+  const size_t lds = pme_lds_bytes(a, true);
+  switch (a.n) {
+#define X(NI) \
+  case NI: hipLaunchKernelGGL(k_pme_gather<NI>, grid, block, lds, st, a); break;
+    PMX_PME_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// --- FFT --------------------------------------------------------------------------
+
+// 1D complex FFTs of power-of-two length held in LDS, cooperative over the
+// block: in-place radix-2 decimation in time after a bit reversal.  Line l
+// element e lives at buf[l*dist + e*stride].  sign -1: exp(-i), +1: exp(+i);
+// unnormalised, like FFTPACK's DCFFTF/DCFFTB.
+__device__ void lds_fft_pow2(double2 *buf, int K, int logK, int nlines, int stride, int dist, int sign,
+                             const double2 *__restrict__ tw) {
+  const int nt = blockDim.x, tid = threadIdx.x;
+  for (int w = tid; w < nlines * K; w += nt) {
+    const int l = w / K, e = w % K;
+    const int r = (int)(__brev((unsigned)e) >> (32 - logK));
+    if (r > e) {
+      double2 *p = buf + (size_t)l * dist;
+      const double2 t0 = p[e * stride];
+      p[e * stride] = p[r * stride];
+      p[r * stride] = t0;
+    }
+  }
+  __syncthreads();
+  for (int len = 2; len <= K; len <<= 1) {
+    const int half = len >> 1;
+    const int tstep = K / len;
+    for (int w = tid; w < nlines * (K >> 1); w += nt) {
+      const int l = w / (K >> 1), bfly = w % (K >> 1);
+      const int grp = bfly / half, pos = bfly % half;
+      const int e0 = grp * len + pos, e1 = e0 + half;
+      const double2 twv = tw[pos * tstep];  // exp(-2 pi i m/K) = (cos, -sin)
+      const double wr = twv.x, wi = sign < 0 ? twv.y : -twv.y;
+      double2 *p = buf + (size_t)l * dist;
+      const double2 u = p[e0 * stride], v = p[e1 * stride];
+      const double vr = v.x * wr - v.y * wi, vi = v.x * wi + v.y * wr;
+      p[e0 * stride] = make_double2(u.x + vr, u.y + vi);
+      p[e1 * stride] = make_double2(u.x - vr, u.y - vi);
+    }
+    __syncthreads();
+  }
+}
+
+// forward transform of one c-plane of one bead: real along a, then b.
+// LDS holds the plane as Kb lines of Ka complex values; only a' = 0..Ka/2
+// is kept (Hermitian half).  spec layout: [bead][c][b][a'] complex.
+__global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
+  extern __shared__ double2 sm[];
+  const int c = blockIdx.x, b = blockIdx.y;
+  const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
+  const double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
+  for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) sm[w] = make_double2(g[w], 0.0);
+  __syncthreads();
+  lds_fft_pow2(sm, Ka, a.logK[0], Kb, 1, Ka, -1, a.tw[0]);
+  lds_fft_pow2(sm, Kb, a.logK[1], Ha, Ka, 1, -1, a.tw[1]);
+  double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
+  for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
+    const int bb = w / Ha, aa = w % Ha;
+    sp[w] = sm[bb * Ka + aa];
+  }
+}
+
+// along c for one (bead, b) row: forward FFT, influence function over the
+// half-space sphere list (ewald3d_pme.c:367-430): energy, virial, and the
+// product with preg*bweight, then the inverse FFT.  Energy and virial go to
+// one partial row per (b row, bead).
+__global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
+  extern __shared__ double2 sm[];
+  const int bb = blockIdx.x, b = blockIdx.y;
+  const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
+  double2 *sp = a.spec + (size_t)b * a.sstride;
+  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
+    const int cc = w / Ha, aa = w % Ha;
+    sm[w] = sp[((size_t)cc * Kb + bb) * Ha + aa];
+  }
+  __syncthreads();
+  lds_fft_pow2(sm, Kc, a.logK[2], Ha, Ha, 1, -1, a.tw[2]);
+  const double *hi = a.cell.hi;
+  const double tpi = 2.0 * M_PI, falp2 = a.falp2;
+  const int kb = bb <= Kb / 2 ? bb : bb - Kb;
+  double e = 0.0, v11 = 0.0, v22 = 0.0, v33 = 0.0, v12 = 0.0, v13 = 0.0, v23 = 0.0;
+  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
+    const int cc = w / Ha, aa = w % Ha;
+    const double W = a.infl[((size_t)cc * Kb + bb) * Ha + aa];
+    double2 s = sm[w];
+    if (W != 0.0) {
+      const int kc = cc <= Kc / 2 ? cc : cc - Kc;
+      const double aka = tpi * (double)aa, akb = tpi * (double)kb, akc = tpi * (double)kc;
+      const double xk = (aka * hi[0] + akb * hi[1] + akc * hi[2]);
+      const double yk = (aka * hi[3] + akb * hi[4] + akc * hi[5]);
+      const double zk = (aka * hi[6] + akb * hi[7] + akc * hi[8]);
+      const double g2 = xk * xk + yk * yk + zk * zk;
+      const double wsum = (aa == 0) ? 0.5 : 1.0;  // a' = 0 holds both +k and -k
+      const double sm2 = (s.x * s.x + s.y * s.y) * wsum;
+      e += W * sm2;
+      const double prep = -2.0 * W * ((g2 / falp2) + 1.0) / g2 * sm2;
+      v11 += prep * xk * xk;
+      v22 += prep * yk * yk;
+      v33 += prep * zk * zk;
+      v12 += prep * xk * yk;
+      v13 += prep * xk * zk;
+      v23 += prep * yk * zk;
+    }
+    sm[w] = make_double2(s.x * W, s.y * W);
+  }
+  __syncthreads();
+  lds_fft_pow2(sm, Kc, a.logK[2], Ha, Ha, 1, +1, a.tw[2]);
+  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
+    const int cc = w / Ha, aa = w % Ha;
+    sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
+  }
+  // the k-space pressure also carries the energy on its diagonal
+  // (ewald3d_pme.c:412-414)
+  const double vals[FFT_NV] = {e, v11 + e, v22 + e, v33 + e, v12, v13, v23};
+  block_partials<FFT_NV>(vals, 1, b, a.P, a.part, bb);
+}
+
+// inverse transform of one c-plane: along b, then complex-to-real along a.
+__global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
+  extern __shared__ double2 sm[];
+  const int c = blockIdx.x, b = blockIdx.y;
+  const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
+  const double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
+  for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
+    const int bb = w / Ka, aa = w % Ka;
+    sm[w] = aa < Ha ? sp[bb * Ha + aa] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  lds_fft_pow2(sm, Kb, a.logK[1], Ha, Ka, 1, +1, a.tw[1]);
+  // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
+  for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
+    const int bb = w / Ka, aa = w % Ka;
+    if (aa >= Ha) {
+      const double2 v = sm[bb * Ka + (Ka - aa)];
+      sm[w] = make_double2(v.x, -v.y);
+    }
+  }
+  __syncthreads();
+  lds_fft_pow2(sm, Ka, a.logK[0], Kb, 1, Ka, +1, a.tw[0]);
+  double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
+  for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) g[w] = sm[w].x;
+}
+
+}  // namespace pmx
