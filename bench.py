@@ -2,13 +2,21 @@
 
 Workload (the headline configuration): 4096 q-TIP4P/F molecules (16384
 sites incl. the massless M site), cubic L = 49.66 A, P = 32 beads at 300 K,
-r_c = 9 A, Ewald alpha = 0.35/A, PME 64^3 of order 6, FP64.  One step is one
-energy_control_pimd call over all 32 beads (32 bead-force evaluations:
+r_c = 9 A, skin 1 A, Ewald alpha = 0.35/A, PME 64^3 of order 6.  One step is
+one energy_control_pimd call over all 32 beads (32 bead-force evaluations:
 forces with ghost forces redistributed, energies and the 3x3 virial); the
 Verlet list is checked every step and rebuilt by the skin criterion, as the
 reference does.  Positions are resident on the device when timing starts.
 
-Prints one JSON line (see the keys below).
+Three paths are timed on the same box:
+  fp64        PINY's arithmetic: FP64 with cubic spline-table interpolation
+              (the drop-in path, parity with the reference to ~1e-12);
+  fp64_direct FP64 with the tabulated pair functions evaluated directly;
+  mixed       FP32 pair terms with FP64 sums (PME, bonded, springs FP64).
+`value` is the mixed path's throughput, reported with the fp64 path's and
+the mixed path's force RMS error relative to fp64.
+
+Prints one JSON line.
 """
 import argparse
 import json
@@ -22,8 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "bead-force evals/s, 4096 q-TIP4P/F P=32: mixed vs FP64 GPU; force RMS err"
-PEAK_FP64_TFLOPS = 78.6     # MI355X FP64 (vector and matrix)
-FLOP_PER_PAIR = 100.0       # nominal per-pair operation count of the pair kernel
+PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X vector TFLOP/s (FP64 vector = FP64 matrix peak)
+FLOP_PER_PAIR = 100.0                 # nominal algorithmic flops per pair term inside the cutoff
 DT_FS = 0.25
 
 
@@ -51,6 +59,31 @@ def cpu_baseline(cfg, seconds=12.0):
                       f"{el:.1f} s on one host core"}
 
 
+def run_path(cfg, steps, warmup, **engine_kw):
+    from piny_amd import abi
+    from piny_amd.engine import Engine
+    s = cfg.system
+    eng = Engine(s, cfg.x, cfg.y, cfg.z, **engine_kw)
+    eng.set_positions(cfg.x, cfg.y, cfg.z)
+    flags = abi.PMX_EVAL_PIMD
+    for _ in range(warmup):
+        e = eng.energy_control(flags)
+    stage = {"list": 0.0, "real": 0.0, "recip": 0.0, "intra": 0.0, "total": 0.0}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e = eng.energy_control(flags)
+        tm = eng.timings()
+        for k in stage:
+            stage[k] += tm[k]
+    el = time.perf_counter() - t0
+    for k in stage:
+        stage[k] /= steps
+    out = {"value": steps * s.pi_beads / el, "ms_per_step": el / steps * 1e3, "stage_ms": stage,
+           "pairs": eng.pair_count(), "energies": e, "forces": eng.forces()}
+    eng.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,66 +91,79 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--beads", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--paths", default="fp64,fp64_direct,mixed")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a device copy of 512 MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(beads and boxes would shard across GPUs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        import torch
+        a = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copies": 3}))
+        return
 
     from piny_amd import abi, water as w
-    from piny_amd.engine import Engine
 
     cfg = w.config3(P=args.beads)
     s = cfg.system
     P = s.pi_beads
-    eng = Engine(s, cfg.x, cfg.y, cfg.z)
-    eng.set_positions(cfg.x, cfg.y, cfg.z)
-    flags = abi.PMX_EVAL_PIMD
-    for _ in range(args.warmup):
-        e = eng.energy_control(flags)
-    stage = {"list": 0.0, "real": 0.0, "recip": 0.0, "intra": 0.0, "total": 0.0}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e = eng.energy_control(flags)
-        tm = eng.timings()
-        for k in stage:
-            stage[k] += tm[k]
-    t1 = time.perf_counter()
-    el = t1 - t0
-    for k in stage:
-        stage[k] /= args.steps
-    npairs = eng.pair_count()
-    value = args.steps * P / el
-    ms_step = el / args.steps * 1e3
-    pair_ms = stage["real"]
+    kw = {"fp64": {}, "fp64_direct": {"pair_eval": abi.PMX_PAIR_DIRECT},
+          "mixed": {"precision": abi.PMX_PREC_MIXED}}
+    paths = [p for p in args.paths.split(",") if p]
+    res = {p: run_path(cfg, args.steps, args.warmup, **kw[p]) for p in paths}
+    head = res["mixed"] if "mixed" in res else res[paths[-1]]
+    head_name = "mixed" if "mixed" in res else paths[-1]
+    ref = res.get("fp64")
+    fo = ref["forces"] if ref else None
+
+    def rms(f):
+        return float(np.sqrt(((f - fo) ** 2).sum() / (fo ** 2).sum())) if fo is not None else None
+
+    pair_ms = head["stage_ms"]["real"]
+    npairs = head["pairs"]
+    prec = "f32" if head_name == "mixed" else "f64"
     achieved = FLOP_PER_PAIR * npairs / (pair_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": head["value"],
         "unit": "bead-force evals/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if head_name == "mixed" else "f64",
         "data": "synthetic",
         "config": {"workload": "q-TIP4P/F 4096 molecules, 16384 sites, L=49.66 A, P=32, 300 K, "
-                               "r_c=9 A, skin 1 A, PME 64^3 order 6, FP64",
-                   "molecules": s.nmol, "sites": s.natm, "beads": P, "precision": "fp64"},
-        "ns_per_day": (args.steps / el) * DT_FS * 86400 * 1e-6,
-        "stage_ms": stage,
+                               "r_c=9 A, skin 1 A, PME 64^3 order 6",
+                   "molecules": s.nmol, "sites": s.natm, "beads": P, "path": head_name},
+        "ns_per_day": (1e3 / head["ms_per_step"]) * DT_FS * 86400 * 1e-6,
+        "paths": {p: {"value": r["value"], "ms_per_step": r["ms_per_step"], "stage_ms": r["stage_ms"],
+                      "force_rms_err_vs_fp64": rms(r["forces"]),
+                      "vintert_rel_err_vs_fp64": (abs(r["energies"]["vintert"] - ref["energies"]["vintert"])
+                                                  / abs(ref["energies"]["vintert"])) if ref else None}
+                  for p, r in res.items()},
+        "speedup_vs_fp64": head["value"] / ref["value"] if ref else None,
+        "force_rms_err": rms(head["forces"]),
         "pairs_in_cutoff_all_beads": npairs,
-        "verlet_pairs": e["nver"],
-        "energies_per_bead_hartree": {"vintert": e["vintert"], "vintrat": e["vintrat"]},
-        "mixed": None,
-        "roofline": {"kernel": "k_pair_f64 (real-space LJ + erfc pair forces, all beads)",
-                     "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "note": "FP64 vector-ALU bound; MI355X's FP64 vector peak equals its FP64 matrix "
-                             "peak (78.6 TFLOP/s); 100 flop/pair nominal"},
+        "verlet_pairs": head["energies"]["nver"],
+        "energies_per_bead_hartree": {"vintert": head["energies"]["vintert"],
+                                      "vintrat": head["energies"]["vintrat"]},
+        "roofline": {"kernel": f"pair kernel of the {head_name} path (real-space LJ + erfc, all beads)",
+                     "bound": "mfma", "achieved": achieved, "peak": PEAK[prec], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[prec], "traffic": None,
+                     "note": "vector-ALU bound (no MFMA work); peak is MI355X's published vector peak "
+                             f"for {prec}; {FLOP_PER_PAIR:.0f} algorithmic flop per pair inside the cutoff; "
+                             "duration from HIP events around the launch on the context's stream"},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)

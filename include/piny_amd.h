@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PMX_ABI_VERSION 1
+#define PMX_ABI_VERSION 2
 
 /* return codes */
 enum {
@@ -57,6 +57,13 @@ enum { PMX_SHIFT_NONE = 0, PMX_SHIFT_SHIFT = 1, PMX_SHIFT_SWIT = 2 };
 
 /* arithmetic of the pair, spline and FFT work */
 enum { PMX_PREC_FP64 = 0, PMX_PREC_MIXED = 1 };
+
+/* how FP64 pair terms are evaluated: PINY's 4-point cubic interpolation of
+ * the spline tables (force_npol.c:339-376), or the tabulated function itself
+ * (LJ and PINY's erfc polynomial with the same switch and shift,
+ * spline_fit.c:28-398) computed per pair; the two agree to the tables'
+ * interpolation error.  The mixed precision always evaluates directly. */
+enum { PMX_PAIR_TABLE = 0, PMX_PAIR_DIRECT = 1 };
 
 /* path_int_md_typ (transform_cnt.c:104,190): 1 staging, 2 centroid */
 enum { PMX_PIMD_STAGING = 1, PMX_PIMD_CENTROID = 2 };
@@ -138,6 +145,7 @@ typedef struct pmx_params {
   int n_interp;         /* [ewald_interp_pme] B-spline order               */
   int intra_perds;      /* [intra_perds] 1: minimum image on bonded terms  */
   int precision;        /* PMX_PREC_*                                      */
+  int pair_eval;        /* PMX_PAIR_* (FP64 only)                          */
 } pmx_params;
 
 /* Energies (Hartree) and pressure tensors (Hartree), stat_avg / ptens. */

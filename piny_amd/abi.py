@@ -13,6 +13,8 @@ PMX_EVAL_NO_RECIP = 1 << 2
 PMX_EVAL_MODE_FORCES = 1 << 3
 PMX_PREC_FP64 = 0
 PMX_PREC_MIXED = 1
+PMX_PAIR_TABLE = 0
+PMX_PAIR_DIRECT = 1
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -40,6 +42,7 @@ class PmxParams(C.Structure):
         ("rheal", C.c_double), ("skin", C.c_double), ("ewald_alpha", C.c_double),
         ("kmax_ewd", C.c_int), ("pme_on", C.c_int), ("kmax_pme", C.c_int),
         ("n_interp", C.c_int), ("intra_perds", C.c_int), ("precision", C.c_int),
+        ("pair_eval", C.c_int),
     ]
 
 
@@ -79,7 +82,7 @@ def _darr(a):
 class SystemStruct:
     """Holds a PmxSystem / PmxParams pair and the numpy arrays they point to."""
 
-    def __init__(self, s, precision: int = PMX_PREC_FP64):
+    def __init__(self, s, precision: int = PMX_PREC_FP64, pair_eval: int = PMX_PAIR_TABLE):
         self.keep = []
 
         def ia(x):
@@ -113,7 +116,7 @@ class SystemStruct:
             nsplin=st.nsplin, ishift=st.ishift, rheal=st.rheal / bohr, skin=st.skin / bohr,
             ewald_alpha=s.ewald_alpha, kmax_ewd=st.kmax_ewd, pme_on=st.pme_on,
             kmax_pme=st.kmax_pme, n_interp=st.n_interp, intra_perds=st.intra_perds,
-            precision=precision)
+            precision=precision, pair_eval=pair_eval)
         for k, v in enumerate(s.hmat):
             self.par.hmat[k] = float(v)
 

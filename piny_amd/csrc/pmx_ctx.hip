@@ -40,6 +40,10 @@ struct pmx_ctx {
   int *type = nullptr, *ichrg = nullptr, *ex_off = nullptr, *ex_num = nullptr, *ex_j = nullptr;
   double *q = nullptr;
   double2 *tab_lj = nullptr, *tab_c = nullptr, *ecor_tab = nullptr;
+  double *pair_pp = nullptr;   // [npair][PP_NUM]
+  double4 *site = nullptr;     // [N] (xmod, q)
+  int pair_mode = 0;       // 0 FP64 tables, 1 FP64 direct, 2 FP32 direct
+  bool full_list = true;   // force list with both directions (no atomics)
   int cap = 0, *jfull = nullptr, *jforce = nullptr, *nfull = nullptr, *nforce = nullptr, *overflow = nullptr;
   int *b1 = nullptr, *b2 = nullptr, *btyp = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr,
       *atyp = nullptr, *e1 = nullptr, *e2 = nullptr;
@@ -231,6 +235,10 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   const int N = s.N, P = s.P;
   const size_t NP = (size_t)N * P;
   int rc = 0;
+  if (N > ENTRY_MASK) {  // force-list entries hold the site index in 24 bits
+    set_err(c, "%d sites exceed the %d sites of one context", N, ENTRY_MASK);
+    return PMX_ERR_RANGE;
+  }
 #define TRY(x)        \
   do {                \
     rc = (x);         \
@@ -256,9 +264,14 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(upload(c, &c->type, sys->atm_type, N));
   TRY(upload(c, &c->q, sys->q, N));
   TRY(upload(c, &c->ichrg, s.ichrg.data(), s.ichrg.size()));
-  TRY(upload(c, &c->ex_off, s.ex_off.data(), N));
-  TRY(upload(c, &c->ex_num, s.ex_num.data(), N));
-  TRY(upload(c, &c->ex_j, s.ex_j.data(), s.ex_j.size()));
+  // the list kernel tests exclusions in both directions
+  TRY(upload(c, &c->ex_off, s.exs_off.data(), N));
+  TRY(upload(c, &c->ex_num, s.exs_num.data(), N));
+  TRY(upload(c, &c->ex_j, s.exs_j.data(), s.exs_j.size()));
+  {
+    const char *env = getenv("PMX_PAIR_LIST");  // tuning override: "half" or "full"
+    if (env) c->full_list = strcmp(env, "half") != 0;
+  }
   // tables as (V, -V'/r) pairs
   {
     const size_t nt = (size_t)s.npair * s.nsplin;
@@ -272,6 +285,24 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     std::vector<double2> et(s.nsplin);
     for (int k = 0; k < s.nsplin; k++) et[k] = make_double2(s.ecor_tab[2 * k], s.ecor_tab[2 * k + 1]);
     TRY(upload(c, &c->ecor_tab, et.data(), et.size()));
+  }
+  {
+    std::vector<double> pp((size_t)s.npair * PP_NUM, 0.0);
+    for (int t = 0; t < s.npair; t++) {
+      double *e = &pp[(size_t)t * PP_NUM];
+      e[PP_CUT] = s.cutoff[t];
+      e[PP_RMIN] = s.rmin[t];
+      e[PP_DRI] = s.dri[t];
+      e[PP_DR] = s.dr[t];
+      e[PP_EPS] = s.lj_eps[t];
+      e[PP_SIG6] = s.lj_sig6[t];
+      e[PP_VSLJ] = s.vshift_lj[t];
+      e[PP_VSC] = s.vshift_c[t];
+      e[PP_LJ] = (s.label[t] == PMX_INTER_LJ) ? 1.0 : 0.0;
+    }
+    TRY(upload(c, &c->pair_pp, pp.data(), pp.size()));
+    TRY(dalloc(c, &c->site, N));
+    c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
   }
   TRY(dalloc(c, &c->nfull, N));
   TRY(dalloc(c, &c->nforce, N));
@@ -417,6 +448,7 @@ int build_list(pmx_ctx *c) {
     a.ex_j = c->ex_j;
     a.pt = c->pt;
     a.cell = c->cell;
+    a.full = c->full_list ? 1 : 0;
     a.jfull = c->jfull;
     a.jforce = c->jforce;
     a.nfull = c->nfull;
@@ -621,9 +653,11 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   RedArgs red{};
   HIPCHK(hipEventRecord(c->ev[0], st));
   // --- initial (energy_control_initial.c:168-208)
-  HIPCHK(hipMemsetAsync(c->fx, 0, NP * sizeof(double), st));
-  HIPCHK(hipMemsetAsync(c->fy, 0, NP * sizeof(double), st));
-  HIPCHK(hipMemsetAsync(c->fz, 0, NP * sizeof(double), st));
+  if (!c->full_list) {  // with the full list the pair kernel stores every force first
+    HIPCHK(hipMemsetAsync(c->fx, 0, NP * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(c->fy, 0, NP * sizeof(double), st));
+    HIPCHK(hipMemsetAsync(c->fz, 0, NP * sizeof(double), st));
+  }
   HIPCHK(hipMemsetAsync(c->ake, 0, sizeof(double), st));
   if (P > 1 && !c->xmod_given) {
     hipLaunchKernelGGL(k_centroid, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm);
@@ -667,31 +701,38 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   HIPCHK(hipEventRecord(c->ev[1], st));
   // --- real space (force_control -> force_verlst -> force_npol)
   {
+    hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
+                       c->q, c->site);
+    HIPCHK(hipGetLastError());
     PairArgs a{};
     a.N = N;
     a.P = P;
     a.cap = c->cap;
     a.nsplin = s.nsplin;
     a.pimd = (P > 1);
+    a.npair = s.npair;
+    a.alp = s.alp_ewd;
+    a.palp = kErfP * s.alp_ewd;
+    a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
+    a.rheal = c->par.rheal;
+    a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
+    a.swit = (c->par.ishift == PMX_SHIFT_SWIT);
+    a.pp = c->pair_pp;
     a.x = c->x;
     a.y = c->y;
     a.z = c->z;
-    a.xm = c->xm;
-    a.ym = c->ym;
-    a.zm = c->zm;
+    a.site = c->site;
     a.type = c->type;
-    a.q = c->q;
     a.jlist = c->jforce;
     a.cnt = c->nforce;
     a.tab_lj = c->tab_lj;
     a.tab_c = c->tab_c;
-    a.pt = c->pt;
     a.cell = c->cell;
     a.fx = c->fx;
     a.fy = c->fy;
     a.fz = c->fz;
     a.part = c->part + c->off_pair;
-    HIPCHK(launch_pair(a, st));
+    HIPCHK(launch_pair(a, c->pair_mode, c->full_list, st));
     add_stage(red, c->off_pair, pair_grid_x(N), PAIR_NV,
               {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR});
   }

@@ -12,6 +12,12 @@ __device__ __forceinline__ int nint_piny(double x) {
   return (int)((x) >= 0.0 ? ((x) + 0.5) : ((x)-0.5));
 }
 
+// The same rounding (half away from zero) as a double, without the round
+// trip through an integer register: trunc(x + copysign(0.5, x)) equals
+// (int)(x >= 0 ? x + 0.5 : x - 0.5) for every |x| < 2^31, up to the sign of
+// a zero result.
+__device__ __forceinline__ double nint_d(double x) { return trunc(x + copysign(0.5, x)); }
+
 __device__ __forceinline__ void atomic_add(double *p, double v) { unsafeAtomicAdd(p, v); }
 
 // Per-bead sums over the threads of one workgroup, written as one row per

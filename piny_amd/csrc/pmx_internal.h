@@ -16,6 +16,18 @@ constexpr double BOLTZ = 315777.0;    // defines.h:80
 constexpr int MAX_TYPES = 8;
 constexpr int MAX_PAIRS = MAX_TYPES * (MAX_TYPES + 1) / 2;
 constexpr int MAX_INTERP = 12;
+// PINY's erfc(x) ~ exp(-x^2) sum_k e_k t^k, t = 1/(1 + p x) (spline_fit.c:327-339)
+constexpr double kErfP = 0.3614;
+constexpr double kErfE[10] = {0.0,
+                              0.2041422096422003,
+                              0.1997535956961481,
+                              0.2213176596405576,
+                              0.03360430734640255,
+                              0.4732592578721755,
+                              -0.509078520069735,
+                              0.6772631491947646,
+                              -0.369912979092217,
+                              0.06965131976970335};
 
 // ---------------------------------------------------------------------------
 // Host-side setup products (restating PINY's parse-time setup).
@@ -29,6 +41,9 @@ struct Setup {
   std::vector<double> rmin, dr, dri, cutoff, cutoff_res, cutskin2;
   // tables [npair][nsplin][4] = {cv0, cdv0, cv0_c, cdv0_c}
   std::vector<double> tab;
+  // the tabulated functions' parameters, for direct evaluation: LJ eps and
+  // sigma^6, and the constant removed from V by shift_inter_pe = shift
+  std::vector<double> lj_eps, lj_sig6, vshift_lj, vshift_c;
   double cutoff_max = 0.0, clong = 0.0, clong_res = 0.0;
   // charges and the sums of ewald3d_selfbgr (ewald3d_self_bgr.c:55-61)
   std::vector<int> ichrg;
@@ -36,6 +51,8 @@ struct Setup {
   // exclusions: per atom (0-based) sorted list of j < i
   std::vector<int> ex_off, ex_num, ex_j;
   int nexcl = 0;
+  // the same exclusions listed under both sites (j < i and j > i)
+  std::vector<int> exs_off, exs_num, exs_j;
   // Ewald-corrected excluded pairs (0-based, e1 > e2)
   std::vector<int> e1, e2;
   std::vector<double> ecor_tab;  // [nsplin][2] = {cv0, cdv0}

@@ -9,11 +9,12 @@ namespace pmx {
 
 struct ListArgs {
   int N, cap, pimd;
+  int full;  // 1: jforce holds every interacting j != i (both directions)
   const double *bx, *by, *bz;  // bead-1 positions [N]
   const double *xm, *ym, *zm;  // xmod [N]
   const int *type;
   const double *q;
-  const int *ex_off, *ex_num, *ex_j;
+  const int *ex_off, *ex_num, *ex_j;  // exclusions (both directions when full)
   PairTypes pt;
   Cell cell;
   int *jfull, *jforce;  // ELL rows of width cap
@@ -21,15 +22,27 @@ struct ListArgs {
   int *overflow;
 };
 
+// Per type pair constants of the force kernels, PP_NUM doubles each: cutoff,
+// table origin and spacing, and the tabulated functions' parameters for
+// direct evaluation (spline_fit.c:28-398): LJ 4 eps (s^6/r^6)((s^6/r^6) - 1)
+// and the PINY erfc polynomial, both times the switch of width rheal below
+// the cutoff when swit is set, minus vshift (shift_inter_pe = shift).
+enum { PP_CUT = 0, PP_RMIN, PP_DRI, PP_DR, PP_EPS, PP_SIG6, PP_VSLJ, PP_VSC, PP_LJ, PP_NUM };
+
+// List entries of the force kernels carry the type pair in the top byte.
+constexpr int ENTRY_SHIFT = 24;
+constexpr int ENTRY_MASK = (1 << ENTRY_SHIFT) - 1;
+
 struct PairArgs {
-  int N, P, cap, nsplin, pimd;
+  int N, P, cap, nsplin, pimd, npair;
+  double alp, palp, talp2, rheal, irheal;  // erfc and switch constants
+  int swit;
+  const double *pp;            // [npair][PP_NUM]
   const double *x, *y, *z;     // [N*P]
-  const double *xm, *ym, *zm;  // [N]
+  const double4 *site;         // [N] (xmod, ymod, zmod, q)
   const int *type;
-  const double *q;
-  const int *jlist, *cnt;
+  const int *jlist, *cnt;      // entries j | t << ENTRY_SHIFT
   const double2 *tab_lj, *tab_c;  // [npair*nsplin] (V, -V'/r)
-  PairTypes pt;
   Cell cell;
   double *fx, *fy, *fz;
   double *part;  // per-block partial sums [gridDim.x][P][PAIR_NV]
@@ -139,6 +152,8 @@ __global__ void k_ghost_pos(int ng, int P, const int *gatom, const int *gncomp, 
                             const double *gcoef, double *x, double *y, double *z);
 __global__ void k_centroid(int N, int P, const double *x, const double *y, const double *z, double *xm,
                            double *ym, double *zm);
+__global__ void k_pack_sites(int N, int pimd, const double *xm, const double *ym, const double *zm,
+                             const double *q, double4 *site);
 __global__ void k_list_check(int N, int P, const double *x, const double *y, const double *z,
                              const double *x0, const double *y0, const double *z0, double *b0x, double *b0y,
                              double *b0z, unsigned long long *maxes);
@@ -160,7 +175,10 @@ __global__ void k_ghost_force(int ng, int P, const int *gatom, const int *gncomp
                               const double *gcoef, double *fx, double *fy, double *fz);
 __global__ void k_mode_forces(ModeArgs a);
 
-hipError_t launch_pair(const PairArgs &a, hipStream_t st);
+// mode: 0 FP64 tables, 1 FP64 direct, 2 FP32 direct with FP64 accumulation;
+// full: the list holds both directions and each wave stores its sites'
+// forces (no atomics); otherwise j forces are scattered with atomics
+hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st);
 int pair_grid_x(int N);
 // LDS bytes of the spread and gather workgroups for a given tiling
 size_t pme_lds_bytes(const PmeArgs &a, bool gather);

@@ -6,6 +6,8 @@
 // name the PINY routine whose arithmetic each kernel restates.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "pmx_device.h"
 
 namespace pmx {
@@ -93,6 +95,14 @@ __global__ void k_centroid(int N, int P, const double *__restrict__ x, const dou
   zm[i] = sz / P;
 }
 
+// (xmod, q) of every site in one 32-byte record for the force kernels
+__global__ void k_pack_sites(int N, int pimd, const double *__restrict__ xm, const double *__restrict__ ym,
+                             const double *__restrict__ zm, const double *__restrict__ q, double4 *site) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  site[i] = pimd ? make_double4(xm[i], ym[i], zm[i], q[i]) : make_double4(0.0, 0.0, 0.0, q[i]);
+}
+
 // Bead-1 copy, largest displacement since the last list build
 // (check_ver_list, nbr_list_control.c:182-239) and bead spread
 // (get_pimd_spread, pimd_estimators.c:121-222).  Maxima of squares are
@@ -129,13 +139,59 @@ __global__ void k_list_check(int N, int P, const double *__restrict__ x, const d
 
 // ---------------------------------------------------------------------------
 // Verlet list: nolst_ver_gen + verlist_create (verlist_control.c:134-441,
-// verlist_create.c:223-503).  One wavefront per site i scans j = 0..i-1 in
-// ascending chunks of 64; a ballot keeps the reference's order, so each
-// per-site segment comes out sorted.  Two lists are written in ELL form
-// (row i at i*cap): every pair with r^2 < cutskin^2 (the reference's list)
-// and the subset that can interact (LJ table non-null or q_i q_j != 0),
-// which the force kernel walks.  No FMA contraction: the r^2 test must see
-// the reference's bits.
+// verlist_create.c:223-503).  One wavefront per site i scans j in ascending
+// chunks of 64; a ballot keeps the reference's order, so each per-site
+// segment comes out sorted.  Two lists are written in ELL form (row i at
+// i*cap): the reference's half list (j < i, r^2 < cutskin^2) and the list the
+// force kernel walks, which keeps only pairs that can interact (LJ table
+// non-null or q_i q_j != 0) and, when full is set, holds both directions
+// (every j != i).  The minimum image is odd in the separation, so the full
+// list is exactly the half list and its transpose.  No FMA contraction: the
+// r^2 test must see the reference's bits.
+__device__ __forceinline__ double min_image_sep(const Cell &c, int pimd, double xi, double yi, double zi,
+                                                double xj, double yj, double zj, double xmi, double ymi,
+                                                double zmi, double xmj, double ymj, double zmj, double &dx,
+                                                double &dy, double &dz) {
+#pragma clang fp contract(off)
+  dx = xi - xj;
+  dy = yi - yj;
+  dz = zi - zj;
+  if (!pimd) {
+    if (c.cubic) {
+      const double asx = dx * c.hi[0], asy = dy * c.hi[4], asz = dz * c.hi[8];
+      const double sx = asx - nint_d(asx), sy = asy - nint_d(asy), sz = asz - nint_d(asz);
+      dx = sx * c.h[0];
+      dy = sy * c.h[4];
+      dz = sz * c.h[8];
+    } else {
+      const double asx = dx * c.hi[0] + dy * c.hi[3] + dz * c.hi[6];
+      const double asy = dx * c.hi[1] + dy * c.hi[4] + dz * c.hi[7];
+      const double asz = dx * c.hi[2] + dy * c.hi[5] + dz * c.hi[8];
+      const double sx = asx - nint_d(asx), sy = asy - nint_d(asy), sz = asz - nint_d(asz);
+      dx = sx * c.h[0] + sy * c.h[3] + sz * c.h[6];
+      dy = sx * c.h[1] + sy * c.h[4] + sz * c.h[7];
+      dz = sx * c.h[2] + sy * c.h[5] + sz * c.h[8];
+    }
+  } else {
+    const double dxs = xmi - xmj, dys = ymi - ymj, dzs = zmi - zmj;
+    if (c.cubic) {
+      const double sx = -nint_d(dxs * c.hi[0]), sy = -nint_d(dys * c.hi[4]), sz = -nint_d(dzs * c.hi[8]);
+      dx += (sx * c.h[0]);
+      dy += (sy * c.h[4]);
+      dz += (sz * c.h[8]);
+    } else {
+      const double asx = dxs * c.hi[0] + dys * c.hi[3] + dzs * c.hi[6];
+      const double asy = dxs * c.hi[1] + dys * c.hi[4] + dzs * c.hi[7];
+      const double asz = dxs * c.hi[2] + dys * c.hi[5] + dzs * c.hi[8];
+      const double sx = -nint_d(asx), sy = -nint_d(asy), sz = -nint_d(asz);
+      dx += (sx * c.h[0] + sy * c.h[3] + sz * c.h[6]);
+      dy += (sx * c.h[1] + sy * c.h[4] + sz * c.h[7]);
+      dz += (sx * c.h[2] + sy * c.h[5] + sz * c.h[8]);
+    }
+  }
+  return dx * dx + dy * dy + dz * dz;
+}
+
 __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
@@ -151,57 +207,31 @@ __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
   const int ti = a.type[i];
   const double qi = a.q[i];
   const int eo = a.ex_off[i], en = a.ex_num[i];
-  const Cell &c = a.cell;
   int nfull = 0, nforce = 0;
   int *rowf = a.jfull + (size_t)i * a.cap;
   int *rowi = a.jforce + (size_t)i * a.cap;
-  for (int j0 = 0; j0 < i; j0 += 64) {
+  const int jend = a.full ? a.N : i;
+  for (int j0 = 0; j0 < jend; j0 += 64) {
     const int j = j0 + lane;
     bool keep = false, inter = false;
-    if (j < i) {
+    int t = 0;
+    if (j < jend && j != i) {
       bool excl = false;
       for (int k = 0; k < en; k++) excl |= (a.ex_j[eo + k] == j);
       if (!excl) {
-        double dx = xi - a.bx[j], dy = yi - a.by[j], dz = zi - a.bz[j];
-        if (!a.pimd) {
-          if (c.cubic) {
-            const double asx = dx * c.hi[0], asy = dy * c.hi[4], asz = dz * c.hi[8];
-            const double sx = asx - nint_piny(asx), sy = asy - nint_piny(asy), sz = asz - nint_piny(asz);
-            dx = sx * c.h[0];
-            dy = sy * c.h[4];
-            dz = sz * c.h[8];
-          } else {
-            const double asx = dx * c.hi[0] + dy * c.hi[3] + dz * c.hi[6];
-            const double asy = dx * c.hi[1] + dy * c.hi[4] + dz * c.hi[7];
-            const double asz = dx * c.hi[2] + dy * c.hi[5] + dz * c.hi[8];
-            const double sx = asx - nint_piny(asx), sy = asy - nint_piny(asy), sz = asz - nint_piny(asz);
-            dx = sx * c.h[0] + sy * c.h[3] + sz * c.h[6];
-            dy = sx * c.h[1] + sy * c.h[4] + sz * c.h[7];
-            dz = sx * c.h[2] + sy * c.h[5] + sz * c.h[8];
-          }
-        } else {
-          const double dxs = xmi - a.xm[j], dys = ymi - a.ym[j], dzs = zmi - a.zm[j];
-          if (c.cubic) {
-            const double sx = -nint_piny(dxs * c.hi[0]), sy = -nint_piny(dys * c.hi[4]),
-                         sz = -nint_piny(dzs * c.hi[8]);
-            dx += (sx * c.h[0]);
-            dy += (sy * c.h[4]);
-            dz += (sz * c.h[8]);
-          } else {
-            const double asx = dxs * c.hi[0] + dys * c.hi[3] + dzs * c.hi[6];
-            const double asy = dxs * c.hi[1] + dys * c.hi[4] + dzs * c.hi[7];
-            const double asz = dxs * c.hi[2] + dys * c.hi[5] + dzs * c.hi[8];
-            const double sx = -nint_piny(asx), sy = -nint_piny(asy), sz = -nint_piny(asz);
-            dx += (sx * c.h[0] + sy * c.h[3] + sz * c.h[6]);
-            dy += (sx * c.h[1] + sy * c.h[4] + sz * c.h[7]);
-            dz += (sx * c.h[2] + sy * c.h[5] + sz * c.h[8]);
-          }
+        double dx, dy, dz;
+        double xmj = 0.0, ymj = 0.0, zmj = 0.0;
+        if (a.pimd) {
+          xmj = a.xm[j];
+          ymj = a.ym[j];
+          zmj = a.zm[j];
         }
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const int tj = a.type[j];
-        const int t = pair_index(a.pt.ntype, ti, tj);
-        keep = r2 < a.pt.cutskin2[t];
-        inter = keep && (a.pt.lj[t] || (qi * a.q[j] != 0.0));
+        const double r2 = min_image_sep(a.cell, a.pimd, xi, yi, zi, a.bx[j], a.by[j], a.bz[j], xmi, ymi, zmi, xmj,
+                                        ymj, zmj, dx, dy, dz);
+        t = pair_index(a.pt.ntype, ti, a.type[j]);
+        const bool in = r2 < a.pt.cutskin2[t];
+        keep = in && j < i;
+        inter = in && (a.pt.lj[t] || (qi * a.q[j] != 0.0));
       }
     }
     const unsigned long long mk = __ballot(keep), mi = __ballot(inter);
@@ -212,7 +242,7 @@ __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
     }
     if (inter) {
       const int p = nforce + __popcll(mi & below);
-      if (p < a.cap) rowi[p] = j;
+      if (p < a.cap) rowi[p] = j | (t << ENTRY_SHIFT);
     }
     nfull += __popcll(mk);
     nforce += __popcll(mi);
@@ -220,7 +250,8 @@ __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
   if (lane == 0) {
     a.nfull[i] = nfull;
     a.nforce[i] = nforce;
-    if (nfull > a.cap) atomicMax(a.overflow, nfull);
+    const int mx = nfull > nforce ? nfull : nforce;
+    if (mx > a.cap) atomicMax(a.overflow, mx);
     atomicAdd(a.overflow + 1, nfull);  // pair count of the reference list
   }
 }
@@ -230,8 +261,15 @@ __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
 // (force_npol.c:28-328) with shave_skin_opt on and RESPA off.  A wavefront
 // owns one site i at a time; its 64 lanes are (slot, bead) with B beads per
 // slot, so the B lanes of a slot evaluate the same pair for B beads and
-// touch contiguous memory.  j forces are scattered with FP64 atomics, the i
-// force is reduced over slots with shuffles.  FP64 throughout.
+// touch contiguous memory.  The i force is reduced over slots with shuffles.
+// With the full list every pair is met from both sites: each wavefront
+// stores its own sites' forces and the energies and virial carry a factor
+// 1/2, so the kernel has no atomics.  With the half list the j forces are
+// scattered with FP64 atomics.
+//
+// MODE 0 interpolates PINY's spline tables in FP64 (npol_vspl_fetch); MODE 1
+// evaluates the tabulated functions directly in FP64; MODE 2 does that in
+// FP32 with the separation taken in FP64 and every sum kept in FP64.
 
 __device__ __forceinline__ double spline4(const double2 *__restrict__ tab, int k, double del, bool second) {
   // npol_vspl_fetch, force_npol.c:339-376 (k is PINY's 1-based index)
@@ -245,8 +283,141 @@ __device__ __forceinline__ double spline4(const double2 *__restrict__ tab, int k
   return (c0 + del * (c1 + del * (c2 + del * c3)));
 }
 
-template <int B>
-__global__ void __launch_bounds__(256) k_pair_f64(PairArgs a) {
+template <typename R>
+__device__ __forceinline__ R rcp_r(R x);
+template <>
+__device__ __forceinline__ double rcp_r<double>(double x) {
+  return 1.0 / x;
+}
+template <>
+__device__ __forceinline__ float rcp_r<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);  // v_rcp_f32, 1 ulp
+}
+template <typename R>
+__device__ __forceinline__ R exp_r(R x);
+template <>
+__device__ __forceinline__ double exp_r<double>(double x) {
+  return exp(x);
+}
+template <>
+__device__ __forceinline__ float exp_r<float>(float x) {
+  return __expf(x);
+}
+
+// energy (ev) and -(1/r) dV/dr (returned) of one pair at distance r, for
+// the LJ term (lj) and the erfc term with charge product q2; pp holds the
+// type pair's constants
+template <typename R>
+__device__ __forceinline__ R pair_direct(const PairArgs &a, const double *pp, R q2, R r, R rinv, R &vlj, R &vc) {
+  R sw = (R)1, dsw = (R)0;
+  if (a.swit) {
+    const R rheal = (R)a.rheal, irheal = (R)a.irheal;
+    R rp = (r - (R)pp[PP_CUT] + rheal) * irheal;
+    rp = rp < (R)1 ? rp : (R)1;
+    rp = rp > (R)0 ? rp : (R)0;
+    sw = (R)1 + rp * rp * ((R)2 * rp - (R)3);
+    dsw = (R)(-6) * (rp * irheal) * (rp - (R)1) * rinv;
+  }
+  const R r2i = rinv * rinv;
+  R dvt = (R)0;
+  vlj = (R)0;
+  vc = (R)0;
+  if (pp[PP_LJ] != 0.0) {
+    const R eps = (R)pp[PP_EPS];
+    const R s6 = (R)pp[PP_SIG6] * (r2i * r2i * r2i);
+    const R v = (R)4 * eps * s6 * (s6 - (R)1);
+    const R dv = (R)24 * eps * s6 * (s6 * (R)2 - (R)1) * r2i;
+    vlj = v * sw - (R)pp[PP_VSLJ];
+    dvt = dv * sw + v * dsw;
+  }
+  if (q2 != (R)0) {
+    const R ralp = r * (R)a.alp;
+    const R eee = exp_r<R>(-ralp * ralp);
+    const R tt = rcp_r<R>((R)1 + (R)kErfP * ralp);
+    R g = (R)kErfE[9], dg = (R)(9.0 * kErfE[9]);
+#pragma unroll
+    for (int k = 8; k >= 1; k--) {
+      g = g * tt + (R)kErfE[k];
+      dg = dg * tt + (R)(k * kErfE[k]);
+    }
+    g = g * tt * eee;
+    dg = dg * tt * tt * eee * (R)a.palp + (R)a.talp2 * g * r;
+    const R v = g * rinv;
+    const R dv = (g * r2i + dg * rinv) * rinv;
+    vc = q2 * (v * sw - (R)pp[PP_VSC]);
+    dvt += q2 * (dv * sw + v * dsw);
+  }
+  return dvt;
+}
+
+// Accumulators of one lane of the pair kernel.
+struct PairAcc {
+  double vlj, vc, p11, p22, p33, p12, p13, p23, npr, fx, fy, fz;
+};
+
+// One pair term for one bead: separation (dx, dy, dz) and squared distance
+// r2d (FP64), type pair constants pp; returns the scalar -(1/r) dV/dr and
+// adds the energies, the i force and the virial.
+template <int MODE>
+__device__ __forceinline__ bool pair_term(const PairArgs &a, const double *pp, int t, double qq, double dx,
+                                          double dy, double dz, double r2d, PairAcc &s, double &dvd) {
+  using R = typename std::conditional<MODE == 2, float, double>::type;
+  const double cut = pp[PP_CUT];
+  if constexpr (MODE == 2) {
+    if (!(r2d < cut * cut)) return false;
+    const float r2 = (float)r2d;
+    const float rinv = __frsqrt_rn(r2);
+    R ev_lj, ev_c;
+    dvd = (double)pair_direct<float>(a, pp, (float)qq, r2 * rinv, rinv, ev_lj, ev_c);
+    s.vlj += (double)ev_lj;
+    s.vc += (double)ev_c;
+  } else {
+    const double rd = sqrt(r2d);
+    if (!(rd < cut)) return false;
+    if constexpr (MODE == 0) {
+      const double rmin = pp[PP_RMIN], dri = pp[PP_DRI];
+      int kk = (int)(((rd - rmin) * dri) + 0.5) + 3;
+      kk = min(kk, a.nsplin - 2);
+      kk = max(kk, 3);
+      const double rsp = ((double)(kk - 3)) * pp[PP_DR] + rmin;
+      const double del = (rd - rsp) * dri;
+      const size_t toff = (size_t)t * a.nsplin;
+      double dvt = 0.0;
+      if (pp[PP_LJ] != 0.0) {
+        s.vlj += spline4(a.tab_lj + toff, kk, del, false);
+        dvt += spline4(a.tab_lj + toff, kk, del, true);
+      }
+      if (qq != 0.0) {
+        s.vc += qq * spline4(a.tab_c + toff, kk, del, false);
+        dvt += qq * spline4(a.tab_c + toff, kk, del, true);
+      }
+      dvd = dvt;
+    } else {
+      double ev_lj, ev_c;
+      dvd = pair_direct<double>(a, pp, qq, rd, 1.0 / rd, ev_lj, ev_c);
+      s.vlj += ev_lj;
+      s.vc += ev_c;
+    }
+  }
+  const double fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
+  s.fx += fx1;
+  s.fy += fy1;
+  s.fz += fz1;
+  s.p11 += dx * fx1;
+  s.p22 += dy * fy1;
+  s.p33 += dz * fz1;
+  s.p12 += dx * fy1;
+  s.p13 += dx * fz1;
+  s.p23 += dy * fz1;
+  s.npr += 1.0;
+  return true;
+}
+
+template <int B, int MODE, bool FULL>
+__global__ void __launch_bounds__(256) k_pair(PairArgs a) {
+  __shared__ double spp[MAX_PAIRS * PP_NUM];
+  for (int k = threadIdx.x; k < a.npair * PP_NUM; k += blockDim.x) spp[k] = a.pp[k];
+  __syncthreads();
   constexpr int S = 64 / B;
   const int lane = threadIdx.x & 63;
   const int slot = lane / B;
@@ -256,113 +427,76 @@ __global__ void __launch_bounds__(256) k_pair_f64(PairArgs a) {
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
   const Cell &c = a.cell;
-  double vlj = 0.0, vc = 0.0, p11 = 0.0, p22 = 0.0, p33 = 0.0, p12 = 0.0, p13 = 0.0, p23 = 0.0, npr = 0.0;
+  PairAcc s{};
   for (int i = wave; i < a.N; i += nwave) {
     const size_t ib = (size_t)i * P + (bact ? bead : 0);
     const double xi = a.x[ib], yi = a.y[ib], zi = a.z[ib];
-    const int ti = a.type[i];
-    const double qi = a.q[i];
-    double xmi = 0.0, ymi = 0.0, zmi = 0.0;
-    if (a.pimd) {
-      xmi = a.xm[i];
-      ymi = a.ym[i];
-      zmi = a.zm[i];
-    }
+    const double4 si = a.site[i];
     const int cnt = a.cnt[i];
     const int *row = a.jlist + (size_t)i * a.cap;
-    double fxi = 0.0, fyi = 0.0, fzi = 0.0;
-    for (int k = slot; k < cnt; k += S) {
-      const int j = row[k];
-      const size_t jb = (size_t)j * P + (bact ? bead : 0);
-      double dx = xi - a.x[jb], dy = yi - a.y[jb], dz = zi - a.z[jb];
-      if (!a.pimd) {  // period, period.c:27-111
-        if (c.cubic) {
-          const double asx = dx * c.hi[0], asy = dy * c.hi[4], asz = dz * c.hi[8];
-          dx = (asx - nint_piny(asx)) * c.h[0];
-          dy = (asy - nint_piny(asy)) * c.h[4];
-          dz = (asz - nint_piny(asz)) * c.h[8];
-        } else {
-          const double asx = dx * c.hi[0] + dy * c.hi[3] + dz * c.hi[6];
-          const double asy = dx * c.hi[1] + dy * c.hi[4] + dz * c.hi[7];
-          const double asz = dx * c.hi[2] + dy * c.hi[5] + dz * c.hi[8];
-          const double sx = asx - nint_piny(asx), sy = asy - nint_piny(asy), sz = asz - nint_piny(asz);
-          dx = sx * c.h[0] + sy * c.h[3] + sz * c.h[6];
-          dy = sx * c.h[1] + sy * c.h[4] + sz * c.h[7];
-          dz = sx * c.h[2] + sy * c.h[5] + sz * c.h[8];
-        }
-      } else {  // period_pimd, period.c:121-206: image of the xmod pair
-        const double dxs = xmi - a.xm[j], dys = ymi - a.ym[j], dzs = zmi - a.zm[j];
-        if (c.cubic) {
-          dx += -nint_piny(dxs * c.hi[0]) * c.h[0];
-          dy += -nint_piny(dys * c.hi[4]) * c.h[4];
-          dz += -nint_piny(dzs * c.hi[8]) * c.h[8];
-        } else {
-          const double sx = -nint_piny(dxs * c.hi[0] + dys * c.hi[3] + dzs * c.hi[6]);
-          const double sy = -nint_piny(dxs * c.hi[1] + dys * c.hi[4] + dzs * c.hi[7]);
-          const double sz = -nint_piny(dxs * c.hi[2] + dys * c.hi[5] + dzs * c.hi[8]);
-          dx += (sx * c.h[0] + sy * c.h[3] + sz * c.h[6]);
-          dy += (sx * c.h[1] + sy * c.h[4] + sz * c.h[7]);
-          dz += (sx * c.h[2] + sy * c.h[5] + sz * c.h[8]);
-        }
+    s.fx = s.fy = s.fz = 0.0;
+    // two list entries per slot and iteration; the next iteration's entries
+    // are loaded while the current pairs are evaluated
+    int k0 = slot;
+    int en0 = k0 < cnt ? row[k0] : 0, en1 = k0 + S < cnt ? row[k0 + S] : 0;
+    for (; k0 < cnt; k0 += 2 * S) {
+      const int e[2] = {en0, en1};
+      const bool v1 = k0 + S < cnt;
+      const int kn = k0 + 2 * S;
+      en0 = kn < cnt ? row[kn] : 0;
+      en1 = kn + S < cnt ? row[kn + S] : 0;
+      double xj[2], yj[2], zj[2];
+      double4 sj[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int j = e[u] & ENTRY_MASK;
+        const size_t jb = (size_t)j * P + (bact ? bead : 0);
+        xj[u] = a.x[jb];
+        yj[u] = a.y[jb];
+        zj[u] = a.z[jb];
+        sj[u] = a.site[j];
       }
-      const double r = sqrt(dx * dx + dy * dy + dz * dz);
-      const int t = pair_index(a.pt.ntype, ti, a.type[j]);
-      if (bact && r < a.pt.cut[t]) {
-        const double rmin = a.pt.rmin[t], dri = a.pt.dri[t];
-        int kk = (int)(((r - rmin) * dri) + 0.5) + 3;
-        kk = min(kk, a.nsplin - 2);
-        kk = max(kk, 3);
-        const double rsp = ((double)(kk - 3)) * a.pt.dr[t] + rmin;
-        const double del = (r - rsp) * dri;
-        const size_t toff = (size_t)t * a.nsplin;
-        double dvt = 0.0;
-        if (a.pt.lj[t]) {
-          vlj += spline4(a.tab_lj + toff, kk, del, false);
-          dvt += spline4(a.tab_lj + toff, kk, del, true);
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (u == 1 && !v1) break;
+        const int j = e[u] & ENTRY_MASK, t = e[u] >> ENTRY_SHIFT;
+        double dx, dy, dz;
+        const double r2d = min_image_sep(c, a.pimd, xi, yi, zi, xj[u], yj[u], zj[u], si.x, si.y, si.z, sj[u].x,
+                                         sj[u].y, sj[u].z, dx, dy, dz);
+        double dvd;
+        if (bact && pair_term<MODE>(a, spp + t * PP_NUM, t, si.w * sj[u].w, dx, dy, dz, r2d, s, dvd)) {
+          if constexpr (!FULL) {
+            const size_t jb = (size_t)j * P + bead;
+            atomic_add(a.fx + jb, -(dx * dvd));
+            atomic_add(a.fy + jb, -(dy * dvd));
+            atomic_add(a.fz + jb, -(dz * dvd));
+          }
         }
-        const double q2 = qi * a.q[j];
-        if (q2 != 0.0) {
-          vc += q2 * spline4(a.tab_c + toff, kk, del, false);
-          dvt += q2 * spline4(a.tab_c + toff, kk, del, true);
-        }
-        const double fx1 = dx * dvt, fy1 = dy * dvt, fz1 = dz * dvt;
-        fxi += fx1;
-        fyi += fy1;
-        fzi += fz1;
-        atomic_add(a.fx + jb, -fx1);
-        atomic_add(a.fy + jb, -fy1);
-        atomic_add(a.fz + jb, -fz1);
-        p11 += dx * fx1;
-        p22 += dy * fy1;
-        p33 += dz * fz1;
-        p12 += dx * fy1;
-        p13 += dx * fz1;
-        p23 += dy * fz1;
-        npr += 1.0;
       }
     }
+    double fxi = s.fx, fyi = s.fy, fzi = s.fz;
     for (int off = 32; off >= B; off >>= 1) {
       fxi += __shfl_xor(fxi, off, 64);
       fyi += __shfl_xor(fyi, off, 64);
       fzi += __shfl_xor(fzi, off, 64);
     }
-    if (slot == 0 && bact && cnt > 0) {
-      atomic_add(a.fx + ib, fxi);
-      atomic_add(a.fy + ib, fyi);
-      atomic_add(a.fz + ib, fzi);
+    if (slot == 0 && bact) {
+      if constexpr (FULL) {  // the first writer of this step's forces
+        a.fx[ib] = fxi;
+        a.fy[ib] = fyi;
+        a.fz[ib] = fzi;
+      } else if (cnt > 0) {
+        atomic_add(a.fx + ib, fxi);
+        atomic_add(a.fy + ib, fyi);
+        atomic_add(a.fz + ib, fzi);
+      }
     }
   }
-  const double vals[PAIR_NV] = {vlj, vc, p11, p22, p33, p12, p13, p23, npr};
+  const double h = FULL ? 0.5 : 1.0;
+  const double vals[PAIR_NV] = {s.vlj * h, s.vc * h, s.p11 * h, s.p22 * h, s.p33 * h,
+                                s.p12 * h, s.p13 * h, s.p23 * h, s.npr * h};
   block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x);
 }
-
-template __global__ void k_pair_f64<1>(PairArgs);
-template __global__ void k_pair_f64<2>(PairArgs);
-template __global__ void k_pair_f64<4>(PairArgs);
-template __global__ void k_pair_f64<8>(PairArgs);
-template __global__ void k_pair_f64<16>(PairArgs);
-template __global__ void k_pair_f64<32>(PairArgs);
-template __global__ void k_pair_f64<64>(PairArgs);
 
 // ---------------------------------------------------------------------------
 // bonded terms and excluded-pair corrections
@@ -379,15 +513,15 @@ __device__ __forceinline__ void intra_sep(const IntraArgs &a, int i, int j, int 
       const double asx = dx * c.hi[0] + dy * c.hi[3] + dz * c.hi[6];
       const double asy = dx * c.hi[1] + dy * c.hi[4] + dz * c.hi[7];
       const double asz = dx * c.hi[2] + dy * c.hi[5] + dz * c.hi[8];
-      const double sx = asx - nint_piny(asx), sy = asy - nint_piny(asy), sz = asz - nint_piny(asz);
+      const double sx = asx - nint_d(asx), sy = asy - nint_d(asy), sz = asz - nint_d(asz);
       dx = sx * c.h[0] + sy * c.h[3] + sz * c.h[6];
       dy = sx * c.h[1] + sy * c.h[4] + sz * c.h[7];
       dz = sx * c.h[2] + sy * c.h[5] + sz * c.h[8];
     } else {
       const double dxs = a.xm[i] - a.xm[j], dys = a.ym[i] - a.ym[j], dzs = a.zm[i] - a.zm[j];
-      const double sx = -nint_piny(dxs * c.hi[0] + dys * c.hi[3] + dzs * c.hi[6]);
-      const double sy = -nint_piny(dxs * c.hi[1] + dys * c.hi[4] + dzs * c.hi[7]);
-      const double sz = -nint_piny(dxs * c.hi[2] + dys * c.hi[5] + dzs * c.hi[8]);
+      const double sx = -nint_d(dxs * c.hi[0] + dys * c.hi[3] + dzs * c.hi[6]);
+      const double sy = -nint_d(dxs * c.hi[1] + dys * c.hi[4] + dzs * c.hi[7]);
+      const double sz = -nint_d(dxs * c.hi[2] + dys * c.hi[5] + dzs * c.hi[8]);
       dx += (sx * c.h[0] + sy * c.h[3] + sz * c.h[6]);
       dy += (sx * c.h[1] + sy * c.h[4] + sz * c.h[7]);
       dz += (sx * c.h[2] + sy * c.h[5] + sz * c.h[8]);
@@ -625,20 +759,36 @@ int pair_grid_x(int N) {
   return (waves + 3) / 4;
 }
 
-hipError_t launch_pair(const PairArgs &a, hipStream_t st) {
+template <int B>
+void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_t st) {
+  const dim3 block(256);
+#define PMX_PAIR_LAUNCH(M, F) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<B, M, F>), grid, block, 0, st, a)
+  if (full) {
+    if (mode == 0) PMX_PAIR_LAUNCH(0, true);
+    else if (mode == 1) PMX_PAIR_LAUNCH(1, true);
+    else PMX_PAIR_LAUNCH(2, true);
+  } else {
+    if (mode == 0) PMX_PAIR_LAUNCH(0, false);
+    else if (mode == 1) PMX_PAIR_LAUNCH(1, false);
+    else PMX_PAIR_LAUNCH(2, false);
+  }
+#undef PMX_PAIR_LAUNCH
+}
+
+hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st) {
   // beads per slot: the smallest power of two covering P, at most 64
   int B = 1;
   while (B < a.P && B < 64) B <<= 1;
   const int nbg = (a.P + B - 1) / B;
-  dim3 grid(pair_grid_x(a.N), nbg), block(256);
+  const dim3 grid(pair_grid_x(a.N), nbg);
   switch (B) {
-    case 1: hipLaunchKernelGGL(k_pair_f64<1>, grid, block, 0, st, a); break;
-    case 2: hipLaunchKernelGGL(k_pair_f64<2>, grid, block, 0, st, a); break;
-    case 4: hipLaunchKernelGGL(k_pair_f64<4>, grid, block, 0, st, a); break;
-    case 8: hipLaunchKernelGGL(k_pair_f64<8>, grid, block, 0, st, a); break;
-    case 16: hipLaunchKernelGGL(k_pair_f64<16>, grid, block, 0, st, a); break;
-    case 32: hipLaunchKernelGGL(k_pair_f64<32>, grid, block, 0, st, a); break;
-    default: hipLaunchKernelGGL(k_pair_f64<64>, grid, block, 0, st, a); break;
+    case 1: launch_pair_b<1>(a, mode, full, grid, st); break;
+    case 2: launch_pair_b<2>(a, mode, full, grid, st); break;
+    case 4: launch_pair_b<4>(a, mode, full, grid, st); break;
+    case 8: launch_pair_b<8>(a, mode, full, grid, st); break;
+    case 16: launch_pair_b<16>(a, mode, full, grid, st); break;
+    case 32: launch_pair_b<32>(a, mode, full, grid, st); break;
+    default: launch_pair_b<64>(a, mode, full, grid, st); break;
   }
   return hipGetLastError();
 }

@@ -105,7 +105,7 @@ __global__ void __launch_bounds__(256) k_pme_bin(PmeArgs a) {
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const int K = a.K[d];
-    const double t = s[d] - nint_piny((s[d] - 0.5));
+    const double t = s[d] - nint_d((s[d] - 0.5));
     const double sc = t * (double)K;
     int ig = (int)(sc);
     const double frac = sc - (double)ig;

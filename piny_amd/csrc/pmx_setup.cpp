@@ -20,17 +20,7 @@ namespace {
 
 int nint_piny(double x) { return (int)((x) >= 0.0 ? ((x) + 0.5) : ((x)-0.5)); }
 
-const double kErfP = 0.3614;
-const double kErfE[10] = {0.0,
-                          0.2041422096422003,
-                          0.1997535956961481,
-                          0.2213176596405576,
-                          0.03360430734640255,
-                          0.4732592578721755,
-                          -0.509078520069735,
-                          0.6772631491947646,
-                          -0.369912979092217,
-                          0.06965131976970335};
+
 
 double erfc_poly(double tt) {
   const double *e = kErfE;
@@ -50,7 +40,7 @@ double derfc_poly(double tt) {
 // itype: 1 erfc-Coulomb with unit charge product, 2 LJ, 4 null.
 void spline_table(const Setup &s, const pmx_params &par, int itype, double rmin, double rmax,
                   double sig, double eps, std::vector<double> &v, std::vector<double> &dv,
-                  double &dr_out, double &dri_out) {
+                  double &dr_out, double &dri_out, double *shift_out = nullptr) {
   const int n = s.nsplin;
   const double rheal = par.rheal;
   v.assign(n, 0.0);
@@ -96,9 +86,11 @@ void spline_table(const Setup &s, const pmx_params &par, int itype, double rmin,
       dv[i] = dvnow * sw[i] + vnow * dsw[i];
     }
   }
+  if (shift_out) *shift_out = 0.0;
   if (par.ishift == PMX_SHIFT_SHIFT) {
     const double shift = v[(n - 2) - 1];
     for (int i = 0; i < n; i++) v[i] -= shift;
+    if (shift_out) *shift_out = shift;
   }
 }
 
@@ -399,6 +391,10 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
   s.cutoff_res.assign(s.npair, 0);
   s.cutskin2.assign(s.npair, 0);
   s.tab.assign((size_t)s.npair * s.nsplin * 4, 0.0);
+  s.lj_eps.assign(s.npair, 0.0);
+  s.lj_sig6.assign(s.npair, 0.0);
+  s.vshift_lj.assign(s.npair, 0.0);
+  s.vshift_c.assign(s.npair, 0.0);
   s.cutoff_max = 0.0;
   for (int t = 0; t < s.npair; t++) {
     const int lab = sys->inter_label[t];
@@ -413,12 +409,15 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
     s.cutoff_max = std::max(s.cutoff[t], s.cutoff_max);
     std::vector<double> v, dv, vc, dvc;
     spline_table(s, *par, lab, s.rmin[t], s.cutoff[t], sys->inter_sig[t], sys->inter_eps[t], v, dv,
-                 s.dr[t], s.dri[t]);
+                 s.dr[t], s.dri[t], &s.vshift_lj[t]);
     double drt, drit;
     if (!s.ichrg.empty())
-      spline_table(s, *par, 1, s.rmin[t], s.cutoff[t], 1.0, 0.0, vc, dvc, drt, drit);
+      spline_table(s, *par, 1, s.rmin[t], s.cutoff[t], 1.0, 0.0, vc, dvc, drt, drit, &s.vshift_c[t]);
     else
       vc.assign(s.nsplin, 0.0), dvc.assign(s.nsplin, 0.0);
+    const double sg = sys->inter_sig[t];
+    s.lj_eps[t] = sys->inter_eps[t];
+    s.lj_sig6[t] = sg * sg * sg * sg * sg * sg;
     for (int i = 0; i < s.nsplin; i++) {
       double *e = &s.tab[((size_t)t * s.nsplin + i) * 4];
       e[0] = v[i];
@@ -494,6 +493,24 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
       s.ex_num[pr[k].first]++;
     }
     for (int i = 1; i < s.N; i++) s.ex_off[i] = s.ex_off[i - 1] + s.ex_num[i - 1];
+    // both directions, for the full list of the force kernels
+    {
+      std::vector<std::pair<int, int>> sym;
+      sym.reserve(2 * pr.size());
+      for (auto &p : pr) {
+        sym.emplace_back(p.first, p.second);
+        sym.emplace_back(p.second, p.first);
+      }
+      std::sort(sym.begin(), sym.end());
+      s.exs_num.assign(s.N, 0);
+      s.exs_off.assign(s.N, 0);
+      s.exs_j.resize(sym.size());
+      for (size_t k = 0; k < sym.size(); k++) {
+        s.exs_j[k] = sym[k].second;
+        s.exs_num[sym[k].first]++;
+      }
+      for (int i = 1; i < s.N; i++) s.exs_off[i] = s.exs_off[i - 1] + s.exs_num[i - 1];
+    }
     s.e1.clear();
     s.e2.clear();
     for (auto &p : pr) {

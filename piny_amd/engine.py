@@ -60,7 +60,7 @@ def lib():
     L.pmx_bond_coef_harm.argtypes = [C.c_double, dp]
     L.pmx_bond_coef_morse.argtypes = [C.c_double, C.c_double, dp]
     L.pmx_bend_coef_harm.argtypes = [C.c_double, C.c_double, dp, dp]
-    if L.pmx_abi_version() != 1:
+    if L.pmx_abi_version() != 2:
         raise PinyAmdError("libpiny_amd.so ABI version mismatch")
     _lib = L
     return L
@@ -79,10 +79,11 @@ class Engine:
     """One device context: setup at construction, then set_positions /
     energy_control / forces, like PINY's energy_control on its structs."""
 
-    def __init__(self, system, x=None, y=None, z=None, precision=abi.PMX_PREC_FP64, device=0):
+    def __init__(self, system, x=None, y=None, z=None, precision=abi.PMX_PREC_FP64, device=0,
+                 pair_eval=abi.PMX_PAIR_TABLE):
         L = lib()
         self.system = system
-        self.st = abi.SystemStruct(system, precision)
+        self.st = abi.SystemStruct(system, precision, pair_eval)
         self.N = system.natm
         self.P = system.pi_beads
         args = [None, None, None]

@@ -19,6 +19,11 @@ CASES = {
     # 64 SPC/Fw, P = 1, K = 16
     "spc64": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
                                     _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
+    # the same box with shift_inter_pe = shift and = none
+    "spc64_shift": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
+                                          _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, ishift=1)),
+    "spc64_none": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
+                                         _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, ishift=0)),
     # 64 q-TIP4P/F, P = 4, K = 16
     "tip64_p4": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
                                        _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),

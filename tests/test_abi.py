@@ -19,7 +19,9 @@ def test_library_exports_every_symbol():
     lib = ctypes.CDLL(engine.LIB_PATH)
     for name in engine.EXPORTED:
         assert hasattr(lib, name), name
-    assert lib.pmx_abi_version() == 1
+    hdr = open(os.path.join(ROOT, "include", "piny_amd.h")).read()
+    ver = int(re.search(r"#define PMX_ABI_VERSION (\d+)", hdr).group(1))
+    assert lib.pmx_abi_version() == ver
 
 
 def test_coefficient_helpers_match_oracle():

@@ -19,14 +19,14 @@ ETOL = 1e-9
 FTOL = 1e-9
 
 
-def _run(case, flags_extra=0):
+def _run(case, flags_extra=0, **engine_kw):
     c = cases.get(case)
     s = c.system
     flags = (abi.PMX_EVAL_PIMD if s.pi_beads > 1 else 0) | flags_extra
-    print(f"[case {case}] N={s.natm} P={s.pi_beads}", flush=True)
+    print(f"[case {case}] N={s.natm} P={s.pi_beads} {engine_kw}", flush=True)
     o = Oracle(s, c.x, c.y, c.z)
     eo, fo, _ = o.energy_control(c.x, c.y, c.z, flags)
-    g = Engine(s, c.x, c.y, c.z)
+    g = Engine(s, c.x, c.y, c.z, **engine_kw)
     g.set_positions(c.x, c.y, c.z)
     eg = g.energy_control(flags)
     fg = g.forces()
@@ -37,7 +37,8 @@ def _rel(a, b):
     return abs(a - b) / max(abs(b), 1e-300)
 
 
-@pytest.mark.parametrize("case", ["spc27", "spc64", "tip64_p4", "cfg1", "cfg2_p8", "tip256_p32"])
+@pytest.mark.parametrize("case", ["spc27", "spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg1", "cfg2_p8",
+                                  "tip256_p32"])
 def test_energy_control_parity(case):
     c, o, g, eo, fo, eg, fg = _run(case)
     scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
@@ -52,6 +53,57 @@ def test_energy_control_parity(case):
     if c.system.pi_beads > 1:
         assert _rel(eg["pi_ke_vir"], eo["pi_ke_vir"]) < ETOL
     assert np.allclose(g.bead_energies(), o.bead_energies(), rtol=1e-9, atol=1e-12)
+
+
+# Direct evaluation of the tabulated functions differs from PINY's cubic
+# interpolation by the tables' interpolation error: about 1e-10 relative for
+# the energy table, but up to 1e-7 relative for the force table next to the
+# switching onset r_c - r_heal, where the switch's second derivative jumps
+# and the 4-point cubic across that kink is first-order accurate (the table
+# equals the direct function at its nodes to 1e-15).  Bars: energies 1e-7,
+# virial 1e-6 of its largest element, force RMS 2e-6 (north-star FP64 bound:
+# 1e-5).  The mixed path (FP32 pair terms, FP64 sums) is held to the
+# north-star bound of 1e-4 relative force RMS error and virial, and its
+# energies to 1e-5 of the largest term.
+DIRECT_ETOL, DIRECT_VTOL, DIRECT_FTOL = 1e-7, 1e-6, 2e-6
+MIXED_ETOL, MIXED_VTOL, MIXED_FTOL = 1e-5, 1e-4, 1e-4
+
+
+@pytest.mark.parametrize("mode", ["direct", "mixed"])
+@pytest.mark.parametrize("case", ["spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg2_p8", "tip256_p32"])
+def test_pair_direct_and_mixed(case, mode):
+    kw = {"pair_eval": abi.PMX_PAIR_DIRECT} if mode == "direct" else {"precision": abi.PMX_PREC_MIXED}
+    c, o, g, eo, fo, eg, fg = _run(case, **kw)
+    etol, vtol, ftol = ((DIRECT_ETOL, DIRECT_VTOL, DIRECT_FTOL) if mode == "direct"
+                        else (MIXED_ETOL, MIXED_VTOL, MIXED_FTOL))
+    scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
+    for k in abi.ENERGY_KEYS:
+        assert abs(eg[k] - eo[k]) <= etol * max(abs(eo[k]), scale), (k, eg[k], eo[k])
+    rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
+    de = abs(eg["vintert"] - eo["vintert"]) / abs(eo["vintert"])
+    print(f"  {mode}: force rms rel err {rms:.3e}, vintert rel err {de:.3e}", flush=True)
+    assert rms < ftol
+    pv_scale = np.max(np.abs(eo["pvten"]))
+    assert np.max(np.abs(eg["pvten"] - eo["pvten"])) <= vtol * pv_scale
+
+
+@pytest.mark.parametrize("mode", ["table", "mixed"])
+@pytest.mark.parametrize("case", ["spc64", "tip64_p4", "cfg2_p8"])
+def test_half_list_with_atomics(case, mode, monkeypatch):
+    # the half-list force kernel (j forces scattered with atomics), selected
+    # by the tuning switch, against the oracle
+    monkeypatch.setenv("PMX_PAIR_LIST", "half")
+    kw = {} if mode == "table" else {"precision": abi.PMX_PREC_MIXED}
+    c, o, g, eo, fo, eg, fg = _run(case, **kw)
+    etol, ftol = (ETOL, FTOL) if mode == "table" else (MIXED_ETOL, MIXED_FTOL)
+    scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
+    for k in abi.ENERGY_KEYS:
+        assert abs(eg[k] - eo[k]) <= etol * max(abs(eo[k]), scale), (k, eg[k], eo[k])
+    rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
+    print(f"  half list {mode}: force rms rel err {rms:.3e}", flush=True)
+    assert rms < ftol
+    for a, b in zip(g.verlet_list(), o.verlet_list()):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("case", ["spc64", "tip64_p4", "cfg1", "cfg2_p8"])
