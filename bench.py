@@ -31,6 +31,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "bead-force evals/s, 4096 q-TIP4P/F P=32: mixed vs FP64 GPU; force RMS err"
 PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X vector TFLOP/s (FP64 vector = FP64 matrix peak)
+# HBM bytes per launch of the pair kernel, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (separate runs) scaled by a 512 MiB copy probe (FETCH x 2.00,
+# WRITE x 1.00); profiles/kernels_q-tip4p-f_4096_p32_{mixed,fp64}.md
+PAIR_TRAFFIC = {"mixed": (124078 * 2.0 + 16904) * 1024, "fp64": (134511 * 2.0 + 16897) * 1024}
 FLOP_PER_PAIR = 100.0                 # nominal algorithmic flops per pair term inside the cutoff
 DT_FS = 0.25
 
@@ -160,7 +164,7 @@ def main():
                                       "vintrat": head["energies"]["vintrat"]},
         "roofline": {"kernel": f"pair kernel of the {head_name} path (real-space LJ + erfc, all beads)",
                      "bound": "mfma", "achieved": achieved, "peak": PEAK[prec], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[prec], "traffic": None,
+                     "frac": achieved / PEAK[prec], "traffic": PAIR_TRAFFIC.get(head_name),
                      "note": "vector-ALU bound (no MFMA work); peak is MI355X's published vector peak "
                              f"for {prec}; {FLOP_PER_PAIR:.0f} algorithmic flop per pair inside the cutoff; "
                              "duration from HIP events around the launch on the context's stream"},
