@@ -51,7 +51,7 @@ struct pmx_ctx {
   int nghost = 0, *gatom = nullptr, *gncomp = nullptr, *gcomp = nullptr;
   double *gcoef = nullptr;
   // PME
-  int pme = 0, logK[3] = {0, 0, 0};
+  int pme = 0, npass[3] = {0, 0, 0}, radix[3][FFT_MAX_PASSES] = {};
   size_t gstride = 0, sstride = 0;
   double *grid = nullptr, *infl = nullptr;
   double2 *spec = nullptr, *tw[3] = {nullptr, nullptr, nullptr};
@@ -118,10 +118,17 @@ int upload(pmx_ctx *c, T **p, const T *src, size_t n) {
   return PMX_OK;
 }
 
-int ilog2_exact(int k) {
-  int l = 0;
-  while ((1 << l) < k) l++;
-  return (1 << l) == k ? l : -1;
+// Radices 4, 2, 3, 5 (in that order of preference) whose product is k; the
+// number of passes, or -1 when k has another prime factor.
+int fft_factor(int k, int *radix) {
+  int n = 0;
+  while (k > 1 && n < FFT_MAX_PASSES) {
+    int r = (k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : (k % 3 == 0) ? 3 : (k % 5 == 0) ? 5 : 0;
+    if (r == 0) return -1;
+    radix[n++] = r;
+    k /= r;
+  }
+  return k == 1 ? n : -1;
 }
 
 unsigned blocks(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
@@ -157,7 +164,7 @@ int estimate_cap(const Setup &s, double rskin) {
   } while (0)
 
 // Chooses the PME tile (largest of 16 x 16 x 8 and its halvings whose halo
-// box and weight chunk fit 64 KB of LDS), allocates the binning buffers and
+// box fits 64 KB of LDS), allocates the binning buffers and
 // fills the constant part of the PME argument block.
 int setup_pme_tiles(pmx_ctx *c) {
   Setup &s = c->s;
@@ -191,8 +198,6 @@ int setup_pme_tiles(pmx_ctx *c) {
     a.hvol = (size_t)a.H[0] * a.H[1] * a.H[2];
     bool ok = true;
     for (int d = 0; d < 3; d++) ok = ok && pme_max_candidates(a.K[d], a.T[d], n) <= 4;
-    a.ch = 64;
-    while (a.ch > 8 && pme_lds_bytes(a, true) > 65536) a.ch /= 2;
     ok = ok && pme_lds_bytes(a, true) <= 65536;
     if (ok) break;
     int dm = 0;
@@ -338,13 +343,13 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   if (c->pme) {
     const int Ka = s.ngrid[0], Kb = s.ngrid[1], Kc = s.ngrid[2], Ha = Ka / 2 + 1;
     for (int d = 0; d < 3; d++) {
-      c->logK[d] = ilog2_exact(s.ngrid[d]);
-      if (c->logK[d] < 0) {
-        set_err(c, "PME grid %dx%dx%d: only power-of-two grids are implemented on the device", Ka, Kb, Kc);
+      c->npass[d] = fft_factor(s.ngrid[d], c->radix[d]);
+      if (c->npass[d] < 0 || (s.ngrid[d] & 1)) {
+        set_err(c, "PME grid %dx%dx%d: the device FFT takes even sizes with prime factors 2, 3, 5", Ka, Kb, Kc);
         return PMX_ERR_NOTIMPL;
       }
     }
-    if ((size_t)Ka * Kb * sizeof(double2) > 65536 || (size_t)Kc * Ha * sizeof(double2) > 65536) {
+    if ((size_t)Ka * Kb > FFT_MAX_PLANE || (size_t)Kc * Ha > FFT_MAX_PLANE) {
       set_err(c, "PME grid %dx%dx%d exceeds the 64 KB LDS plane of this build", Ka, Kb, Kc);
       return PMX_ERR_NOTIMPL;
     }
@@ -764,7 +769,8 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     f.P = P;
     for (int d = 0; d < 3; d++) {
       f.K[d] = s.ngrid[d];
-      f.logK[d] = c->logK[d];
+      f.npass[d] = c->npass[d];
+      for (int k = 0; k < c->npass[d]; k++) f.radix[d][k] = c->radix[d][k];
       f.tw[d] = c->tw[d];
     }
     f.gstride = c->gstride;

@@ -56,7 +56,6 @@ struct PmeArgs {
   int N, P, nchrg, n;  // n: interpolation order
   int K[3], T[3], nt[3], H[3];
   int ntile;           // tiles per bead
-  int ch;              // charges per weight chunk in LDS
   size_t gstride;      // doubles per bead grid
   size_t hvol;         // H[0]*H[1]*H[2]
   const double *x, *y, *z;
@@ -73,9 +72,13 @@ struct PmeArgs {
   double *fx, *fy, *fz;
 };
 
+constexpr int FFT_MAX_PASSES = 12;
+constexpr int FFT_MAX_PLANE = 4096;  // complex values per LDS plane (64 KB)
+
 struct FftArgs {
   int P;
-  int K[3], logK[3];
+  int K[3];
+  int npass[3], radix[3][FFT_MAX_PASSES];  // Stockham passes per axis (radices 2..5)
   size_t gstride, sstride;
   double *grid;
   double2 *spec;

@@ -7,12 +7,12 @@
 // The MI355X layout differs from the reference's loop order:
 //  - every (charge, bead) is binned by the grid tile holding the first point
 //    of its support (k_pme_bin / k_pme_scan / k_pme_scatter, a counting sort);
-//  - one workgroup per (tile, bead) spreads its charges into an LDS halo box
-//    of (T + n - 1)^3 points with LDS atomics, then writes the box out once
-//    (no global atomics, no grid clear);
+//  - one workgroup per (tile, bead) spreads its charges, one lane per charge,
+//    into an LDS halo box of (T + n - 1)^3 points with LDS atomics, then
+//    writes the box out once (no global atomics, no grid clear);
 //  - k_pme_fold sums the overlapping halo boxes into the periodic grid;
 //  - the gather loads the same halo box of the potential into LDS and each
-//    wavefront reduces one charge's n^3 products.
+//    lane reduces one charge's n^3 products in registers.
 #include <hip/hip_runtime.h>
 
 #include "pmx_device.h"
@@ -21,7 +21,8 @@ namespace pmx {
 
 namespace {
 
-constexpr int MAX_CAND = 4;  // halo boxes that can cover one grid point, per axis
+constexpr int MAX_CAND = 4;    // halo boxes that can cover one grid point, per axis
+constexpr int PME_BLOCK = 128;  // lanes per (tile, bead) workgroup: about one per charge
 
 // B-spline weights M_n(frac + j - 1), j = 1..NI, by the recursion of
 // ewald3d_pme.c:296-337; dmn (order NI-1 differences, :516-530) if DERIV.
@@ -53,35 +54,6 @@ __device__ __forceinline__ void tile_origin(const PmeArgs &a, int tl, int &o0, i
   o0 = ta * a.T[0];
   o1 = tb * a.T[1];
   o2 = tc * a.T[2];
-}
-
-// Weights of one chunk of a bin's charges into LDS: w[(c*3 + d)*NI + m] is
-// the weight of support point s_d + m (that is M_n index j = NI - m), with
-// the charge folded into axis 0; ls[c*3 + d] is the support start relative
-// to the tile origin.
-template <int NI, bool DERIV>
-__device__ __forceinline__ void chunk_weights(const PmeArgs &a, int lo, int base, int m, int o0, int o1, int o2,
-                                              double *w, double *dw, int *ls, int *atom) {
-  const int c = threadIdx.x;
-  if (c >= m) return;
-  const int it = a.sorted[lo + base + c];
-  const int ic = it % a.nchrg;
-  const int i = a.ichrg[ic];
-  const double q = a.q[i];
-  const int org[3] = {o0, o1, o2};
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    double mn[NI + 1], dmn[NI + 1];
-    bspline<NI, DERIV>(a.item_f[3 * (size_t)it + d], mn, dmn);
-    ls[c * 3 + d] = a.item_s[3 * (size_t)it + d] - org[d];
-    const double sc = (d == 0 && !DERIV) ? q : 1.0;
-#pragma unroll
-    for (int mm = 0; mm < NI; mm++) {
-      w[(c * 3 + d) * NI + mm] = mn[NI - mm] * sc;
-      if (DERIV) dw[(c * 3 + d) * NI + mm] = dmn[NI - mm];
-    }
-  }
-  if (DERIV) atom[c] = (int)(((size_t)i * a.P) + (it / a.nchrg));
 }
 
 }  // namespace
@@ -159,45 +131,59 @@ __global__ void __launch_bounds__(256) k_pme_scatter(PmeArgs a) {
 
 // --- spread --------------------------------------------------------------------
 
-// one workgroup per (tile, bead); LDS: halo box | weights | starts
+// Weights of one item along the three axes: w[d][m] is the weight of support
+// point s_d + m (M_n index j = n - m); l[d] is s_d relative to the tile origin.
+template <int NI, bool DERIV>
+__device__ __forceinline__ void item_weights(const PmeArgs &a, int it, const int (&org)[3], double (&w)[3][NI],
+                                             double (&dw)[3][NI], int (&l)[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double mn[NI + 1], dmn[NI + 1];
+    bspline<NI, DERIV>(a.item_f[3 * (size_t)it + d], mn, dmn);
+    l[d] = a.item_s[3 * (size_t)it + d] - org[d];
+#pragma unroll
+    for (int m = 0; m < NI; m++) {
+      w[d][m] = mn[NI - m];
+      if (DERIV) dw[d][m] = dmn[NI - m];
+    }
+  }
+}
+
+// one workgroup per (tile, bead), one lane per charge of the bin; the
+// charge's n^3 products go into the LDS halo box with LDS atomics
 template <int NI>
-__global__ void __launch_bounds__(256) k_pme_spread(PmeArgs a) {
-  extern __shared__ double lds[];
-  const size_t hvol = a.hvol;
-  double *box = lds;
-  double *w = lds + hvol;
-  int *ls = (int *)(w + (size_t)a.ch * 3 * NI);
+__global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
+  extern __shared__ double box[];
+  const int hvol = (int)a.hvol;
   const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
-  for (size_t k = tid; k < hvol; k += blockDim.x) box[k] = 0.0;
+  for (int k = tid; k < hvol; k += blockDim.x) box[k] = 0.0;
   const size_t bin = (size_t)b * a.ntile + tl;
   const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
-  int o0, o1, o2;
-  tile_origin(a, tl, o0, o1, o2);
+  int org[3];
+  tile_origin(a, tl, org[0], org[1], org[2]);
   const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
-  constexpr int NP = NI * NI * NI;
-  for (int base = 0; base < cnt; base += a.ch) {
-    const int m = min(a.ch, cnt - base);
-    __syncthreads();
-    chunk_weights<NI, false>(a, lo, base, m, o0, o1, o2, w, nullptr, ls, nullptr);
-    __syncthreads();
-    for (int c = wv; c < m; c += nw) {
-      const int l0 = ls[c * 3], l1 = ls[c * 3 + 1], l2 = ls[c * 3 + 2];
-      const double *wa = w + (c * 3) * NI, *wb = wa + NI, *wc = wb + NI;
+  __syncthreads();
+  for (int c = tid; c < cnt; c += blockDim.x) {
+    const int it = a.sorted[lo + c];
+    const double q = a.q[a.ichrg[it % a.nchrg]];
+    double w[3][NI], dw[3][NI];
+    int l[3];
+    item_weights<NI, false>(a, it, org, w, dw, l);
 #pragma unroll
-      for (int p0 = 0; p0 < NP; p0 += 64) {
-        const int p = p0 + lane;
-        if (p < NP) {
-          const int ma = p % NI, mb = (p / NI) % NI, mc = p / (NI * NI);
-          const double v = wa[ma] * wb[mb] * wc[mc];
-          atomicAdd(box + (l2 + mc) * H01 + (l1 + mb) * H0 + (l0 + ma), v);
-        }
+    for (int m = 0; m < NI; m++) w[0][m] *= q;
+    for (int mc = 0; mc < NI; mc++) {
+#pragma unroll
+      for (int mb = 0; mb < NI; mb++) {
+        const double wbc = w[1][mb] * w[2][mc];
+        double *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
+#pragma unroll
+        for (int ma = 0; ma < NI; ma++) atomicAdd(row + ma, w[0][ma] * wbc);
       }
     }
   }
   __syncthreads();
-  double *dst = a.hbuf + bin * hvol;
-  for (size_t k = tid; k < hvol; k += blockDim.x) dst[k] = box[k];
+  double *dst = a.hbuf + bin * a.hvol;
+  for (int k = tid; k < hvol; k += blockDim.x) dst[k] = box[k];
 }
 
 // Halo boxes of one axis that hold grid point p: (tile, local index) pairs.
@@ -243,82 +229,68 @@ __global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
 // --- gather ---------------------------------------------------------------------
 
 // one workgroup per (tile, bead): the potential on the halo box into LDS,
-// then one wavefront per charge (ewald3d_pme.c:534-560)
+// then one lane per charge (ewald3d_pme.c:534-560)
 template <int NI>
-__global__ void __launch_bounds__(256) k_pme_gather(PmeArgs a) {
-  extern __shared__ double lds[];
-  const size_t hvol = a.hvol;
-  double *box = lds;
-  double *w = lds + hvol;
-  double *dw = w + (size_t)a.ch * 3 * NI;
-  int *ls = (int *)(dw + (size_t)a.ch * 3 * NI);
-  int *atom = ls + 3 * a.ch;
+__global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
+  extern __shared__ double box[];
+  const int hvol = (int)a.hvol;
   const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   const size_t bin = (size_t)b * a.ntile + tl;
   const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
   if (cnt == 0) return;  // uniform over the workgroup
-  int o0, o1, o2;
-  tile_origin(a, tl, o0, o1, o2);
+  int org[3];
+  tile_origin(a, tl, org[0], org[1], org[2]);
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2];
   const int H0 = a.H[0], H1 = a.H[1], H01 = H0 * H1;
   const double *g = a.grid + (size_t)b * a.gstride;
-  for (size_t k = tid; k < hvol; k += blockDim.x) {
-    const int la = (int)(k % H0), lb = (int)((k / H0) % H1), lc = (int)(k / H01);
-    int ga = o0 + la, gb = o1 + lb, gc = o2 + lc;
-    ga %= Ka;
-    gb %= Kb;
-    gc %= Kc;
+  for (int k = tid; k < hvol; k += blockDim.x) {
+    const int la = k % H0, lb = (k / H0) % H1, lc = k / H01;
+    int ga = org[0] + la, gb = org[1] + lb, gc = org[2] + lc;
+    while (ga >= Ka) ga -= Ka;
+    while (gb >= Kb) gb -= Kb;
+    while (gc >= Kc) gc -= Kc;
     box[k] = g[((size_t)gc * Kb + gb) * Ka + ga];
   }
+  __syncthreads();
   const double *hi = a.cell.hi;
-  const double fa = (double)Ka, fb = (double)Kb, fc = (double)Kc;
-  constexpr int NP = NI * NI * NI;
-  for (int base = 0; base < cnt; base += a.ch) {
-    const int m = min(a.ch, cnt - base);
-    __syncthreads();
-    chunk_weights<NI, true>(a, lo, base, m, o0, o1, o2, w, dw, ls, atom);
-    __syncthreads();
-    for (int c = wv; c < m; c += nw) {
-      const int l0 = ls[c * 3], l1 = ls[c * 3 + 1], l2 = ls[c * 3 + 2];
-      const double *wa = w + (c * 3) * NI, *wb = wa + NI, *wc = wb + NI;
-      const double *dwa = dw + (c * 3) * NI, *dwb = dwa + NI, *dwc = dwb + NI;
-      double da = 0.0, db = 0.0, dc = 0.0;
+  for (int c = tid; c < cnt; c += blockDim.x) {
+    const int it = a.sorted[lo + c];
+    const int ic = it % a.nchrg, bead = it / a.nchrg;
+    const int i = a.ichrg[ic];
+    double w[3][NI], dw[3][NI];
+    int l[3];
+    item_weights<NI, true>(a, it, org, w, dw, l);
+    double da = 0.0, db = 0.0, dc = 0.0;
+    for (int mc = 0; mc < NI; mc++) {
 #pragma unroll
-      for (int p0 = 0; p0 < NP; p0 += 64) {
-        const int p = p0 + lane;
-        if (p < NP) {
-          const int ma = p % NI, mb = (p / NI) % NI, mc = p / (NI * NI);
-          const double phi = box[(l2 + mc) * H01 + (l1 + mb) * H0 + (l0 + ma)];
-          const double xa = wa[ma], xb = wb[mb], xc = wc[mc];
-          da += dwa[ma] * xb * xc * phi;
-          db += xa * dwb[mb] * xc * phi;
-          dc += xa * xb * dwc[mc] * phi;
+      for (int mb = 0; mb < NI; mb++) {
+        const double *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
+        double r0 = 0.0, r1 = 0.0;
+#pragma unroll
+        for (int ma = 0; ma < NI; ma++) {
+          const double phi = row[ma];
+          r0 += dw[0][ma] * phi;
+          r1 += w[0][ma] * phi;
         }
-      }
-      for (int off = 32; off >= 1; off >>= 1) {
-        da += __shfl_xor(da, off, 64);
-        db += __shfl_xor(db, off, 64);
-        dc += __shfl_xor(dc, off, 64);
-      }
-      if (lane == 0) {
-        const int ib = atom[c];
-        const double q = a.q[ib / a.P];
-        da *= fa * q;
-        db *= fb * q;
-        dc *= fc * q;
-        a.fx[ib] -= da * hi[0] + db * hi[1] + dc * hi[2];
-        a.fy[ib] -= da * hi[3] + db * hi[4] + dc * hi[5];
-        a.fz[ib] -= da * hi[6] + db * hi[7] + dc * hi[8];
+        da += r0 * (w[1][mb] * w[2][mc]);
+        db += r1 * (dw[1][mb] * w[2][mc]);
+        dc += r1 * (w[1][mb] * dw[2][mc]);
       }
     }
+    const double q = a.q[i];
+    da *= (double)Ka * q;
+    db *= (double)Kb * q;
+    dc *= (double)Kc * q;
+    const size_t ib = (size_t)i * a.P + bead;
+    a.fx[ib] -= da * hi[0] + db * hi[1] + dc * hi[2];
+    a.fy[ib] -= da * hi[3] + db * hi[4] + dc * hi[5];
+    a.fz[ib] -= da * hi[6] + db * hi[7] + dc * hi[8];
   }
 }
 
 size_t pme_lds_bytes(const PmeArgs &a, bool gather) {
-  const size_t wt = (size_t)a.ch * 3 * a.n * sizeof(double) * (gather ? 2 : 1);
-  const size_t meta = (size_t)a.ch * (gather ? 4 : 3) * sizeof(int);
-  return a.hvol * sizeof(double) + wt + meta;
+  (void)gather;
+  return a.hvol * sizeof(double);
 }
 
 int pme_max_candidates(int K, int T, int n) {
@@ -341,7 +313,7 @@ int pme_max_candidates(int K, int T, int n) {
 #define PMX_PME_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
 hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
-  const dim3 grid(a.ntile, a.P), block(256);
+  const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
   const size_t lds = pme_lds_bytes(a, false);
   switch (a.n) {
 #define X(NI) \
@@ -354,7 +326,7 @@ hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {
-  const dim3 grid(a.ntile, a.P), block(256);
+  const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
   const size_t lds = pme_lds_bytes(a, true);
   switch (a.n) {
 #define X(NI) \
@@ -368,40 +340,114 @@ hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {
 
 // --- FFT --------------------------------------------------------------------------
 
-// 1D complex FFTs of power-of-two length held in LDS, cooperative over the
-// block: in-place radix-2 decimation in time after a bit reversal.  Line l
-// element e lives at buf[l*dist + e*stride].  sign -1: exp(-i), +1: exp(+i);
-// unnormalised, like FFTPACK's DCFFTF/DCFFTB.
-__device__ void lds_fft_pow2(double2 *buf, int K, int logK, int nlines, int stride, int dist, int sign,
-                             const double2 *__restrict__ tw) {
-  const int nt = blockDim.x, tid = threadIdx.x;
-  for (int w = tid; w < nlines * K; w += nt) {
-    const int l = w / K, e = w % K;
-    const int r = (int)(__brev((unsigned)e) >> (32 - logK));
-    if (r > e) {
-      double2 *p = buf + (size_t)l * dist;
-      const double2 t0 = p[e * stride];
-      p[e * stride] = p[r * stride];
-      p[r * stride] = t0;
+// Batched 1D complex FFTs held in LDS, cooperative over the workgroup:
+// mixed-radix Stockham passes (radix 2, 3, 4 or 5; the radices of FFTPACK,
+// fft_generic.f:14-84), unnormalised, sign -1 for exp(-i) (DCFFTF) and +1 for
+// exp(+i) (DCFFTB).  Line l element e lives at buf[l*dist + e*stride].  A
+// pass reads every butterfly's inputs into registers, waits, then writes the
+// outputs in place, so no permutation pass is needed.  Lanes run along the
+// contiguous direction (lines when stride > 1), which keeps LDS accesses
+// free of bank conflicts.  nlines * K <= FFT_MAX_PLANE and blockDim.x == 256.
+template <int R>
+__device__ __forceinline__ void dft_small(double2 (&v)[R], int sign) {
+  if constexpr (R == 2) {
+    const double2 a = v[0], b = v[1];
+    v[0] = make_double2(a.x + b.x, a.y + b.y);
+    v[1] = make_double2(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    const double2 a0 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+    const double2 a1 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const double2 b0 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+    const double2 b1 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+    // b1 * (sign * i)
+    const double2 c1 = sign < 0 ? make_double2(b1.y, -b1.x) : make_double2(-b1.y, b1.x);
+    v[0] = make_double2(a0.x + b0.x, a0.y + b0.y);
+    v[2] = make_double2(a0.x - b0.x, a0.y - b0.y);
+    v[1] = make_double2(a1.x + c1.x, a1.y + c1.y);
+    v[3] = make_double2(a1.x - c1.x, a1.y - c1.y);
+  } else {
+    // direct DFT: out[k] = sum_m v[m] exp(sign 2 pi i k m / R)
+    constexpr double c3[3] = {1.0, -0.5, -0.5};
+    constexpr double s3[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
+    constexpr double c5[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410, -0.80901699437494742410,
+                              0.30901699437494742410};
+    constexpr double s5[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
+                              -0.95105651629515357212};
+    double2 out[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int m = 0; m < R; m++) {
+        const int km = (k * m) % R;
+        const double cr = (R == 3) ? c3[km] : c5[km];
+        const double ci = ((R == 3) ? s3[km] : s5[km]) * (double)sign;
+        re += v[m].x * cr - v[m].y * ci;
+        im += v[m].x * ci + v[m].y * cr;
+      }
+      out[k] = make_double2(re, im);
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = out[k];
+  }
+}
+
+template <int R>
+__device__ void stockham_pass(double2 *buf, int K, int nlines, int stride, int dist, int Ns, int sign,
+                              const double2 *__restrict__ tw) {
+  constexpr int MAXB = (FFT_MAX_PLANE / R + 255) / 256;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int KR = K / R, nb = nlines * KR;
+  const bool lines_fast = stride != 1;
+  double2 v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int bi = tid + q * nt;
+    if (bi < nb) {
+      const int l = lines_fast ? bi % nlines : bi / KR;
+      const int j = lines_fast ? bi / nlines : bi % KR;
+      const double2 *p = buf + (size_t)l * dist;
+#pragma unroll
+      for (int k = 0; k < R; k++) v[q][k] = p[(j + k * KR) * stride];
     }
   }
   __syncthreads();
-  for (int len = 2; len <= K; len <<= 1) {
-    const int half = len >> 1;
-    const int tstep = K / len;
-    for (int w = tid; w < nlines * (K >> 1); w += nt) {
-      const int l = w / (K >> 1), bfly = w % (K >> 1);
-      const int grp = bfly / half, pos = bfly % half;
-      const int e0 = grp * len + pos, e1 = e0 + half;
-      const double2 twv = tw[pos * tstep];  // exp(-2 pi i m/K) = (cos, -sin)
-      const double wr = twv.x, wi = sign < 0 ? twv.y : -twv.y;
+  const int tstep = K / (Ns * R);
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int bi = tid + q * nt;
+    if (bi < nb) {
+      const int l = lines_fast ? bi % nlines : bi / KR;
+      const int j = lines_fast ? bi / nlines : bi % KR;
+      const int jm = j % Ns;
+#pragma unroll
+      for (int k = 1; k < R; k++) {  // twiddle exp(sign 2 pi i jm k / (Ns R))
+        const double2 t = tw[jm * k * tstep];  // (cos, -sin) of 2 pi m / K
+        const double wr = t.x, wi = sign < 0 ? t.y : -t.y;
+        const double2 a = v[q][k];
+        v[q][k] = make_double2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+      }
+      dft_small<R>(v[q], sign);
       double2 *p = buf + (size_t)l * dist;
-      const double2 u = p[e0 * stride], v = p[e1 * stride];
-      const double vr = v.x * wr - v.y * wi, vi = v.x * wi + v.y * wr;
-      p[e0 * stride] = make_double2(u.x + vr, u.y + vi);
-      p[e1 * stride] = make_double2(u.x - vr, u.y - vi);
+      const int base = (j / Ns) * Ns * R + jm;
+#pragma unroll
+      for (int k = 0; k < R; k++) p[(base + k * Ns) * stride] = v[q][k];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+}
+
+__device__ void lds_fft(double2 *buf, int K, int nlines, int stride, int dist, int sign, const double2 *tw,
+                        int npass, const int *radix) {
+  int Ns = 1;
+  for (int s = 0; s < npass; s++) {
+    switch (radix[s]) {
+      case 2: stockham_pass<2>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      case 3: stockham_pass<3>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      case 4: stockham_pass<4>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      default: stockham_pass<5>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+    }
+    Ns *= radix[s];
   }
 }
 
@@ -415,8 +461,8 @@ __global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
   const double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
   for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) sm[w] = make_double2(g[w], 0.0);
   __syncthreads();
-  lds_fft_pow2(sm, Ka, a.logK[0], Kb, 1, Ka, -1, a.tw[0]);
-  lds_fft_pow2(sm, Kb, a.logK[1], Ha, Ka, 1, -1, a.tw[1]);
+  lds_fft(sm, Ka, Kb, 1, Ka, -1, a.tw[0], a.npass[0], a.radix[0]);
+  lds_fft(sm, Kb, Ha, Ka, 1, -1, a.tw[1], a.npass[1], a.radix[1]);
   double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
   for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
     const int bb = w / Ha, aa = w % Ha;
@@ -438,7 +484,7 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
     sm[w] = sp[((size_t)cc * Kb + bb) * Ha + aa];
   }
   __syncthreads();
-  lds_fft_pow2(sm, Kc, a.logK[2], Ha, Ha, 1, -1, a.tw[2]);
+  lds_fft(sm, Kc, Ha, Ha, 1, -1, a.tw[2], a.npass[2], a.radix[2]);
   const double *hi = a.cell.hi;
   const double tpi = 2.0 * M_PI, falp2 = a.falp2;
   const int kb = bb <= Kb / 2 ? bb : bb - Kb;
@@ -468,7 +514,7 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
     sm[w] = make_double2(s.x * W, s.y * W);
   }
   __syncthreads();
-  lds_fft_pow2(sm, Kc, a.logK[2], Ha, Ha, 1, +1, a.tw[2]);
+  lds_fft(sm, Kc, Ha, Ha, 1, +1, a.tw[2], a.npass[2], a.radix[2]);
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
     const int cc = w / Ha, aa = w % Ha;
     sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
@@ -490,7 +536,7 @@ __global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
     sm[w] = aa < Ha ? sp[bb * Ha + aa] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  lds_fft_pow2(sm, Kb, a.logK[1], Ha, Ka, 1, +1, a.tw[1]);
+  lds_fft(sm, Kb, Ha, Ka, 1, +1, a.tw[1], a.npass[1], a.radix[1]);
   // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
   for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
     const int bb = w / Ka, aa = w % Ka;
@@ -500,7 +546,7 @@ __global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft_pow2(sm, Ka, a.logK[0], Kb, 1, Ka, +1, a.tw[0]);
+  lds_fft(sm, Ka, Kb, 1, Ka, +1, a.tw[0], a.npass[0], a.radix[0]);
   double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
   for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) g[w] = sm[w].x;
 }

@@ -31,6 +31,14 @@ CASES = {
     "cfg1": lambda: w.config1(),
     # configuration 2 at P = 8 (the oracle's size budget), K = 32
     "cfg2_p8": lambda: w.config2(P=8),
+    # PME grids that are not powers of two: 20 (2^2 5), 24 (2^3 3), 40, 48
+    "spc64_k20": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 15,
+                                        _st(cutoff=5.5, alpha=0.62, kmax_ewd=8, kmax_pme=8)),
+    "tip64_p4_k24": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 16,
+                                           _st(cutoff=5.5, alpha=0.62, kmax_ewd=10, kmax_pme=10)),
+    "cfg1_k40": lambda: w.config1(_st(kmax_ewd=14, kmax_pme=17)),
+    "tip128_p4_k48": lambda: w.build_config(w.Q_TIP4P_F, 128, w.density_box(128), 4, 17,
+                                            _st(cutoff=6.0, alpha=0.55, kmax_ewd=16, kmax_pme=22, n_interp=8)),
     # 256 q-TIP4P/F at P = 32, K = 32
     "tip256_p32": lambda: w.build_config(w.Q_TIP4P_F, 256, w.density_box(256), 32, 14,
                                          _st(cutoff=9.0, alpha=0.35, kmax_ewd=14, kmax_pme=14)),

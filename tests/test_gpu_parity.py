@@ -38,7 +38,7 @@ def _rel(a, b):
 
 
 @pytest.mark.parametrize("case", ["spc27", "spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg1", "cfg2_p8",
-                                  "tip256_p32"])
+                                  "tip256_p32", "spc64_k20", "tip64_p4_k24", "cfg1_k40", "tip128_p4_k48"])
 def test_energy_control_parity(case):
     c, o, g, eo, fo, eg, fg = _run(case)
     scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
