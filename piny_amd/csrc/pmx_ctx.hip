@@ -59,7 +59,7 @@ struct pmx_ctx {
   PmeArgs pa{};
   int *item_s = nullptr, *item_tile = nullptr, *item_pos = nullptr, *bin_cnt = nullptr, *bin_off = nullptr,
       *sorted = nullptr;
-  double *item_f = nullptr, *hbuf = nullptr;
+  double *item_f = nullptr;
   // per-block partial sums of every stage, then accumulators [P][ACC_NUM] + ake
   double *part = nullptr;
   size_t off_pair = 0, off_fft = 0, off_ecor = 0, off_bond = 0, off_bend = 0, off_fin = 0;
@@ -184,11 +184,19 @@ int setup_pme_tiles(pmx_ctx *c) {
   a.P = s.P;
   a.nchrg = (int)s.ichrg.size();
   a.n = n;
-  int T[3] = {16, 16, 8};
-  const char *env = getenv("PMX_PME_TILE");  // tuning override "Ta,Tb,Tc"
-  if (env) sscanf(env, "%d,%d,%d", &T[0], &T[1], &T[2]);
-  for (int d = 0; d < 3; d++) T[d] = std::max(1, std::min(T[d], s.ngrid[d]));
+  // tiles divide the grid, so every tile is whole and a support of n points
+  // starting in a tile reaches at most ceil((n-1)/T) tiles beyond it
+  int cap[3] = {16, 16, 8};
+  const char *env = getenv("PMX_PME_TILE");  // tuning override "Ta,Tb,Tc" (upper bounds)
+  if (env) sscanf(env, "%d,%d,%d", &cap[0], &cap[1], &cap[2]);
+  auto divisor_below = [](int K, int m) {
+    int t = std::max(1, std::min(m, K));
+    while (K % t) t--;
+    return t;
+  };
+  int T[3];
   for (;;) {
+    for (int d = 0; d < 3; d++) T[d] = divisor_below(s.ngrid[d], cap[d]);
     for (int d = 0; d < 3; d++) {
       a.K[d] = s.ngrid[d];
       a.T[d] = T[d];
@@ -197,7 +205,8 @@ int setup_pme_tiles(pmx_ctx *c) {
     }
     a.hvol = (size_t)a.H[0] * a.H[1] * a.H[2];
     bool ok = true;
-    for (int d = 0; d < 3; d++) ok = ok && pme_max_candidates(a.K[d], a.T[d], n) <= 4;
+    for (int d = 0; d < 3; d++) a.nbr[d] = 1 + (n - 1 + a.T[d] - 1) / a.T[d];
+    ok = ok && (size_t)a.nt[0] * a.nt[1] * a.nt[2] * sizeof(int) <= 65536;  // LDS histogram of k_pme_bin
     ok = ok && pme_lds_bytes(a, true) <= 65536;
     if (ok) break;
     int dm = 0;
@@ -208,7 +217,7 @@ int setup_pme_tiles(pmx_ctx *c) {
               s.ngrid[2]);
       return PMX_ERR_NOTIMPL;
     }
-    T[dm] = (T[dm] + 1) / 2;
+    cap[dm] = T[dm] - 1;
   }
   a.ntile = a.nt[0] * a.nt[1] * a.nt[2];
   a.gstride = c->gstride;
@@ -220,7 +229,6 @@ int setup_pme_tiles(pmx_ctx *c) {
   TRY_RC(dalloc(c, &c->sorted, nitem));
   TRY_RC(dalloc(c, &c->bin_cnt, (size_t)s.P * a.ntile));
   TRY_RC(dalloc(c, &c->bin_off, (size_t)s.P * a.ntile));
-  TRY_RC(dalloc(c, &c->hbuf, (size_t)s.P * a.ntile * a.hvol));
   a.item_s = c->item_s;
   a.item_f = c->item_f;
   a.item_tile = c->item_tile;
@@ -228,7 +236,6 @@ int setup_pme_tiles(pmx_ctx *c) {
   a.sorted = c->sorted;
   a.bin_cnt = c->bin_cnt;
   a.bin_off = c->bin_off;
-  a.hbuf = c->hbuf;
   a.grid = c->grid;
   a.q = c->q;
   a.ichrg = c->ichrg;
@@ -756,15 +763,13 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     const size_t nitem = (size_t)P * nchrg, nbin = (size_t)P * a.ntile;
     hipLaunchKernelGGL(k_fill_int, dim3(blocks(nbin)), dim3(256), 0, st, c->bin_cnt, 0, nbin);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_pme_bin, dim3(blocks(nitem)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_pme_bin, dim3(blocks(nchrg), P), dim3(256), (size_t)a.ntile * sizeof(int), st, a);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_pme_scan, dim3(P), dim3(256), 0, st, a);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_pme_scatter, dim3(blocks(nitem)), dim3(256), 0, st, a);
     HIPCHK(hipGetLastError());
     HIPCHK(launch_pme_spread(a, st));
-    hipLaunchKernelGGL(k_pme_fold, dim3(blocks((size_t)P * c->gstride)), dim3(256), 0, st, a);
-    HIPCHK(hipGetLastError());
     FftArgs f{};
     f.P = P;
     for (int d = 0; d < 3; d++) {

@@ -51,10 +51,12 @@ struct PairArgs {
 // PME on a grid cut into tiles of T[0] x T[1] x T[2] points.  Charge (c, b)
 // (charge c, bead b) is binned by the tile that holds the first point of its
 // interpolation support; its support then lies in that tile's halo box of
-// H = T + n - 1 points per axis, which one workgroup holds in LDS.
+// H = T + n - 1 points per axis (held in LDS by the gather), and reaches
+// nbr[d] tiles along axis d (the spread of a tile reads nbr[0]*nbr[1]*nbr[2]
+// bins).
 struct PmeArgs {
   int N, P, nchrg, n;  // n: interpolation order
-  int K[3], T[3], nt[3], H[3];
+  int K[3], T[3], nt[3], H[3], nbr[3];
   int ntile;           // tiles per bead
   size_t gstride;      // doubles per bead grid
   size_t hvol;         // H[0]*H[1]*H[2]
@@ -67,7 +69,6 @@ struct PmeArgs {
   int *item_tile, *item_pos;
   int *bin_cnt, *bin_off;  // [P*ntile]
   int *sorted;         // [P*nchrg] item ids in bin order
-  double *hbuf;        // [P*ntile*hvol] halo boxes of spread charge
   double *grid;        // [P*gstride]
   double *fx, *fy, *fz;
 };
@@ -164,7 +165,6 @@ __global__ void k_list_build(ListArgs a);
 __global__ void k_pme_bin(PmeArgs a);
 __global__ void k_pme_scan(PmeArgs a);
 __global__ void k_pme_scatter(PmeArgs a);
-__global__ void k_pme_fold(PmeArgs a);
 __global__ void k_fill_int(int *p, int v, size_t n);
 __global__ void k_reduce_partials(RedArgs a);
 __global__ void k_fft_fwd_ab(FftArgs a);
@@ -185,7 +185,6 @@ hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st);
 int pair_grid_x(int N);
 // LDS bytes of the spread and gather workgroups for a given tiling
 size_t pme_lds_bytes(const PmeArgs &a, bool gather);
-int pme_max_candidates(int K, int T, int n);
 hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st);
 hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st);
 // block size for grid-stride kernels whose threads keep one bead: the

@@ -7,12 +7,13 @@
 // The MI355X layout differs from the reference's loop order:
 //  - every (charge, bead) is binned by the grid tile holding the first point
 //    of its support (k_pme_bin / k_pme_scan / k_pme_scatter, a counting sort);
-//  - one workgroup per (tile, bead) spreads its charges, one lane per charge,
-//    into an LDS halo box of (T + n - 1)^3 points with LDS atomics, then
-//    writes the box out once (no global atomics, no grid clear);
-//  - k_pme_fold sums the overlapping halo boxes into the periodic grid;
-//  - the gather loads the same halo box of the potential into LDS and each
-//    lane reduces one charge's n^3 products in registers.
+//  - one workgroup per (tile, bead) owns the tile's grid points: one lane
+//    per charge of the tile's bin and of the neighbouring bins whose supports
+//    reach into the tile adds the charge's in-tile products to an LDS copy of
+//    the tile with LDS atomics; the tile is then stored once (no global
+//    atomics, no grid clear, no partial grids);
+//  - the gather loads the tile's halo box of (T + n - 1)^3 potential values
+//    into LDS and each lane reduces one charge's n^3 products in registers.
 #include <hip/hip_runtime.h>
 
 #include "pmx_device.h"
@@ -21,7 +22,6 @@ namespace pmx {
 
 namespace {
 
-constexpr int MAX_CAND = 4;    // halo boxes that can cover one grid point, per axis
 constexpr int PME_BLOCK = 128;  // lanes per (tile, bead) workgroup: about one per charge
 
 // B-spline weights M_n(frac + j - 1), j = 1..NI, by the recursion of
@@ -60,39 +60,57 @@ __device__ __forceinline__ void tile_origin(const PmeArgs &a, int tl, int &o0, i
 
 // --- binning -----------------------------------------------------------------
 
-// one thread per item it = bead*nchrg + charge: fractional grid coordinates
-// (ewald3d_pme.c:241-262, 270-276), first support point per axis, bin slot
+// one thread per item it = bead*nchrg + charge (grid: charge chunks x beads):
+// fractional grid coordinates (ewald3d_pme.c:241-262, 270-276), first support
+// point per axis, and the item's slot in its bin.  Slots are counted in an
+// LDS histogram per workgroup first, so each (block, tile) pair makes one
+// global atomic instead of one per item.
 __global__ void __launch_bounds__(256) k_pme_bin(PmeArgs a) {
-  const int it = blockIdx.x * blockDim.x + threadIdx.x;
-  if (it >= a.P * a.nchrg) return;
-  const int b = it / a.nchrg, ic = it % a.nchrg;
-  const size_t ib = (size_t)a.ichrg[ic] * a.P + b;
-  const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
-  const double *hi = a.cell.hi;
-  double s[3];
-  s[0] = x * hi[0] + y * hi[3] + z * hi[6];
-  s[1] = x * hi[1] + y * hi[4] + z * hi[7];
-  s[2] = x * hi[2] + y * hi[5] + z * hi[8];
-  int tcoord[3];
+  extern __shared__ int hist[];  // [ntile]
+  const int b = blockIdx.y, ic = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = ic < a.nchrg;
+  for (int t = threadIdx.x; t < a.ntile; t += blockDim.x) hist[t] = 0;
+  __syncthreads();
+  const int it = b * a.nchrg + ic;
+  int tile = 0, lpos = 0;
+  if (act) {
+    const size_t ib = (size_t)a.ichrg[ic] * a.P + b;
+    const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
+    const double *hi = a.cell.hi;
+    double s[3];
+    s[0] = x * hi[0] + y * hi[3] + z * hi[6];
+    s[1] = x * hi[1] + y * hi[4] + z * hi[7];
+    s[2] = x * hi[2] + y * hi[5] + z * hi[8];
+    int tcoord[3];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const int K = a.K[d];
-    const double t = s[d] - nint_d((s[d] - 0.5));
-    const double sc = t * (double)K;
-    int ig = (int)(sc);
-    const double frac = sc - (double)ig;
-    // support points (0-based) ig - n + 1 .. ig, periodic; ig = K (a
-    // coordinate rounding onto the upper face) is the same set as ig = 0
-    ig = ((ig % K) + K) % K;
-    int s0 = ig - a.n + 1;
-    s0 = ((s0 % K) + K) % K;
-    a.item_f[3 * (size_t)it + d] = frac;
-    a.item_s[3 * (size_t)it + d] = s0;
-    tcoord[d] = s0 / a.T[d];
+    for (int d = 0; d < 3; d++) {
+      const int K = a.K[d];
+      const double t = s[d] - nint_d((s[d] - 0.5));
+      const double sc = t * (double)K;
+      int ig = (int)(sc);
+      const double frac = sc - (double)ig;
+      // support points (0-based) ig - n + 1 .. ig, periodic; ig = K (a
+      // coordinate rounding onto the upper face) is the same set as ig = 0
+      ig = ((ig % K) + K) % K;
+      int s0 = ig - a.n + 1;
+      s0 = ((s0 % K) + K) % K;
+      a.item_f[3 * (size_t)it + d] = frac;
+      a.item_s[3 * (size_t)it + d] = s0;
+      tcoord[d] = s0 / a.T[d];
+    }
+    tile = (tcoord[2] * a.nt[1] + tcoord[1]) * a.nt[0] + tcoord[0];
+    lpos = atomicAdd(hist + tile, 1);
   }
-  const int tile = (tcoord[2] * a.nt[1] + tcoord[1]) * a.nt[0] + tcoord[0];
-  a.item_tile[it] = tile;
-  a.item_pos[it] = atomicAdd(a.bin_cnt + (size_t)b * a.ntile + tile, 1);
+  __syncthreads();
+  for (int t = threadIdx.x; t < a.ntile; t += blockDim.x) {
+    const int n = hist[t];
+    hist[t] = n ? atomicAdd(a.bin_cnt + (size_t)b * a.ntile + t, n) : 0;
+  }
+  __syncthreads();
+  if (act) {
+    a.item_tile[it] = tile;
+    a.item_pos[it] = hist[tile] + lpos;
+  }
 }
 
 // exclusive scan of the bin counts of one bead (one workgroup per bead)
@@ -149,81 +167,77 @@ __device__ __forceinline__ void item_weights(const PmeArgs &a, int it, const int
   }
 }
 
-// one workgroup per (tile, bead), one lane per charge of the bin; the
-// charge's n^3 products go into the LDS halo box with LDS atomics
+// One workgroup per (tile, bead) owns the tile's points.  Its lanes take the
+// charges of the bins whose supports reach into the tile: the tile's own bin
+// and, per axis, the nbr[d] - 1 bins below it (periodically), whose charges
+// start at most n - 1 points before the tile.  Each lane adds the in-tile
+// part of its charge's n^3 products to the LDS tile; the tile is then stored.
 template <int NI>
 __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
-  extern __shared__ double box[];
-  const int hvol = (int)a.hvol;
+  extern __shared__ double box[];  // [T2][T1][T0]
   const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  for (int k = tid; k < hvol; k += blockDim.x) box[k] = 0.0;
-  const size_t bin = (size_t)b * a.ntile + tl;
-  const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
-  int org[3];
-  tile_origin(a, tl, org[0], org[1], org[2]);
-  const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
-  __syncthreads();
-  for (int c = tid; c < cnt; c += blockDim.x) {
-    const int it = a.sorted[lo + c];
-    const double q = a.q[a.ichrg[it % a.nchrg]];
-    double w[3][NI], dw[3][NI];
-    int l[3];
-    item_weights<NI, false>(a, it, org, w, dw, l);
+  const int tcoord[3] = {tl % a.nt[0], (tl / a.nt[0]) % a.nt[1], tl / (a.nt[0] * a.nt[1])};
+  int org[3], len[3];
 #pragma unroll
-    for (int m = 0; m < NI; m++) w[0][m] *= q;
-    for (int mc = 0; mc < NI; mc++) {
-#pragma unroll
-      for (int mb = 0; mb < NI; mb++) {
-        const double wbc = w[1][mb] * w[2][mc];
-        double *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
-#pragma unroll
-        for (int ma = 0; ma < NI; ma++) atomicAdd(row + ma, w[0][ma] * wbc);
-      }
-    }
+  for (int d = 0; d < 3; d++) {
+    org[d] = tcoord[d] * a.T[d];
+    len[d] = min(a.T[d], a.K[d] - org[d]);
   }
+  const int T0 = a.T[0], T01 = a.T[0] * a.T[1], vol = a.T[0] * a.T[1] * a.T[2];
+  for (int k = tid; k < vol; k += blockDim.x) box[k] = 0.0;
   __syncthreads();
-  double *dst = a.hbuf + bin * a.hvol;
-  for (int k = tid; k < hvol; k += blockDim.x) dst[k] = box[k];
-}
-
-// Halo boxes of one axis that hold grid point p: (tile, local index) pairs.
-__device__ __forceinline__ int axis_candidates(int p, int K, int T, int nt, int n, int *tt, int *ll) {
-  int nc = 0;
-  for (int t = 0; t < nt; t++) {
-    const int org = t * T;
-    const int len = min(T, K - org) + n - 1;
-    int l = p - org;
-    if (l < 0) l += K;
-    for (; l < len && nc < MAX_CAND; l += K) {
-      tt[nc] = t;
-      ll[nc] = l;
-      nc++;
-    }
-  }
-  return nc;
-}
-
-// sums the halo boxes into the periodic grid, one thread per (bead, point)
-__global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)a.P * a.gstride) return;
-  const int b = (int)(t / a.gstride);
-  const size_t r = t % a.gstride;
-  const int Ka = a.K[0], Kb = a.K[1];
-  const int pa = (int)(r % Ka), pb = (int)((r / Ka) % Kb), pc = (int)(r / ((size_t)Ka * Kb));
-  int ta[MAX_CAND], la[MAX_CAND], tb[MAX_CAND], lb[MAX_CAND], tc[MAX_CAND], lc[MAX_CAND];
-  const int na = axis_candidates(pa, Ka, a.T[0], a.nt[0], a.n, ta, la);
-  const int nb = axis_candidates(pb, Kb, a.T[1], a.nt[1], a.n, tb, lb);
-  const int nc = axis_candidates(pc, a.K[2], a.T[2], a.nt[2], a.n, tc, lc);
-  const double *hb = a.hbuf + (size_t)b * a.ntile * a.hvol;
-  double s = 0.0;
-  for (int kc = 0; kc < nc; kc++)
-    for (int kb = 0; kb < nb; kb++)
-      for (int ka = 0; ka < na; ka++) {
-        const int tile = (tc[kc] * a.nt[1] + tb[kb]) * a.nt[0] + ta[ka];
-        s += hb[(size_t)tile * a.hvol + ((size_t)lc[kc] * a.H[1] + lb[kb]) * a.H[0] + la[ka]];
+  for (int oc = 0; oc < a.nbr[2]; oc++)
+    for (int ob = 0; ob < a.nbr[1]; ob++)
+      for (int oa = 0; oa < a.nbr[0]; oa++) {
+        const int off[3] = {oa, ob, oc};
+        int src[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) src[d] = ((tcoord[d] - off[d]) % a.nt[d] + a.nt[d]) % a.nt[d];
+        const size_t bin = (size_t)b * a.ntile + (src[2] * a.nt[1] + src[1]) * a.nt[0] + src[0];
+        const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
+        for (int c = tid; c < cnt; c += blockDim.x) {
+          const int it = a.sorted[lo + c];
+          int rel[3], m0[3], m1[3];
+          bool any = true;
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            rel[d] = a.item_s[3 * (size_t)it + d] - org[d];
+            if (off[d] > 0 && rel[d] >= 0) rel[d] -= a.K[d];  // wrapped from the top of the grid
+            m0[d] = max(0, -rel[d]);
+            m1[d] = min(NI, len[d] - rel[d]);
+            any = any && (m0[d] < m1[d]);
+          }
+          if (!any) continue;
+          const double q = a.q[a.ichrg[it % a.nchrg]];
+          double w[3][NI];
+#pragma unroll
+          for (int d = 0; d < 3; d++) {
+            double mn[NI + 1], dmn[NI + 1];
+            bspline<NI, false>(a.item_f[3 * (size_t)it + d], mn, dmn);
+#pragma unroll
+            for (int m = 0; m < NI; m++) w[d][m] = mn[NI - m];
+          }
+#pragma unroll
+          for (int m = 0; m < NI; m++) w[0][m] *= q;
+          for (int mc = m0[2]; mc < m1[2]; mc++) {
+            for (int mb = m0[1]; mb < m1[1]; mb++) {
+              const double wbc = w[1][mb] * w[2][mc];
+              double *row = box + (rel[2] + mc) * T01 + (rel[1] + mb) * T0 + rel[0];
+#pragma unroll
+              for (int ma = 0; ma < NI; ma++)
+                if (ma >= m0[0] && ma < m1[0]) atomicAdd(row + ma, w[0][ma] * wbc);
+            }
+          }
+        }
       }
-  a.grid[t] = s;
+  __syncthreads();
+  double *g = a.grid + (size_t)b * a.gstride;
+  const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
+  for (int k = tid; k < vol; k += blockDim.x) {
+    const int la = k % T0, lb = (k / T0) % a.T[1], lc = k / T01;
+    if (la < len[0] && lb < len[1] && lc < len[2])
+      g[(size_t)(org[2] + lc) * Kab + (size_t)(org[1] + lb) * Ka + org[0] + la] = box[k];
+  }
 }
 
 // --- gather ---------------------------------------------------------------------
@@ -243,13 +257,27 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2];
   const int H0 = a.H[0], H1 = a.H[1], H01 = H0 * H1;
   const double *g = a.grid + (size_t)b * a.gstride;
-  for (int k = tid; k < hvol; k += blockDim.x) {
-    const int la = k % H0, lb = (k / H0) % H1, lc = k / H01;
-    int ga = org[0] + la, gb = org[1] + lb, gc = org[2] + lc;
-    while (ga >= Ka) ga -= Ka;
-    while (gb >= Kb) gb -= Kb;
-    while (gc >= Kc) gc -= Kc;
-    box[k] = g[((size_t)gc * Kb + gb) * Ka + ga];
+  // batches of 8 loads in flight per lane
+  for (int k0 = tid; k0 < hvol; k0 += 8 * blockDim.x) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int k = k0 + u * blockDim.x;
+      v[u] = 0.0;
+      if (k < hvol) {
+        const int la = k % H0, lb = (k / H0) % H1, lc = k / H01;
+        int ga = org[0] + la, gb = org[1] + lb, gc = org[2] + lc;
+        while (ga >= Ka) ga -= Ka;
+        while (gb >= Kb) gb -= Kb;
+        while (gc >= Kc) gc -= Kc;
+        v[u] = g[((size_t)gc * Kb + gb) * Ka + ga];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int k = k0 + u * blockDim.x;
+      if (k < hvol) box[k] = v[u];
+    }
   }
   __syncthreads();
   const double *hi = a.cell.hi;
@@ -289,25 +317,7 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
 }
 
 size_t pme_lds_bytes(const PmeArgs &a, bool gather) {
-  (void)gather;
-  return a.hvol * sizeof(double);
-}
-
-int pme_max_candidates(int K, int T, int n) {
-  const int nt = (K + T - 1) / T;
-  int mx = 0;
-  for (int p = 0; p < K; p++) {
-    int nc = 0;
-    for (int t = 0; t < nt; t++) {
-      const int org = t * T;
-      const int len = (T < K - org ? T : K - org) + n - 1;
-      int l = p - org;
-      if (l < 0) l += K;
-      for (; l < len; l += K) nc++;
-    }
-    mx = nc > mx ? nc : mx;
-  }
-  return mx;
+  return gather ? a.hvol * sizeof(double) : (size_t)a.T[0] * a.T[1] * a.T[2] * sizeof(double);
 }
 
 #define PMX_PME_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
@@ -459,7 +469,19 @@ __global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
   const int c = blockIdx.x, b = blockIdx.y;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
   const double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
-  for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) sm[w] = make_double2(g[w], 0.0);
+  for (int w0 = threadIdx.x; w0 < Ka * Kb; w0 += 8 * blockDim.x) {  // 8 loads in flight per lane
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int w = w0 + u * blockDim.x;
+      v[u] = w < Ka * Kb ? g[w] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int w = w0 + u * blockDim.x;
+      if (w < Ka * Kb) sm[w] = make_double2(v[u], 0.0);
+    }
+  }
   __syncthreads();
   lds_fft(sm, Ka, Kb, 1, Ka, -1, a.tw[0], a.npass[0], a.radix[0]);
   lds_fft(sm, Kb, Ha, Ka, 1, -1, a.tw[1], a.npass[1], a.radix[1]);
@@ -479,9 +501,18 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
   const int bb = blockIdx.x, b = blockIdx.y;
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
   double2 *sp = a.spec + (size_t)b * a.sstride;
-  for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
-    const int cc = w / Ha, aa = w % Ha;
-    sm[w] = sp[((size_t)cc * Kb + bb) * Ha + aa];
+  for (int w0 = threadIdx.x; w0 < Kc * Ha; w0 += 8 * blockDim.x) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int w = w0 + u * blockDim.x;
+      if (w < Kc * Ha) v[u] = sp[((size_t)(w / Ha) * Kb + bb) * Ha + (w % Ha)];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int w = w0 + u * blockDim.x;
+      if (w < Kc * Ha) sm[w] = v[u];
+    }
   }
   __syncthreads();
   lds_fft(sm, Kc, Ha, Ha, 1, -1, a.tw[2], a.npass[2], a.radix[2]);
@@ -531,9 +562,19 @@ __global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
   const int c = blockIdx.x, b = blockIdx.y;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
   const double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
-  for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
-    const int bb = w / Ka, aa = w % Ka;
-    sm[w] = aa < Ha ? sp[bb * Ha + aa] : make_double2(0.0, 0.0);
+  for (int w0 = threadIdx.x; w0 < Kb * Ka; w0 += 8 * blockDim.x) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int w = w0 + u * blockDim.x;
+      const int bb = w / Ka, aa = w % Ka;
+      v[u] = (w < Kb * Ka && aa < Ha) ? sp[bb * Ha + aa] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int w = w0 + u * blockDim.x;
+      if (w < Kb * Ka) sm[w] = v[u];
+    }
   }
   __syncthreads();
   lds_fft(sm, Kb, Ha, Ka, 1, +1, a.tw[1], a.npass[1], a.radix[1]);
