@@ -59,7 +59,7 @@ struct pmx_ctx {
   PmeArgs pa{};
   int *item_s = nullptr, *item_tile = nullptr, *item_pos = nullptr, *bin_cnt = nullptr, *bin_off = nullptr,
       *sorted = nullptr;
-  double *item_f = nullptr;
+  double *item_f = nullptr, *hbuf = nullptr;
   // per-block partial sums of every stage, then accumulators [P][ACC_NUM] + ake
   double *part = nullptr;
   size_t off_pair = 0, off_fft = 0, off_ecor = 0, off_bond = 0, off_bend = 0, off_fin = 0;
@@ -229,6 +229,8 @@ int setup_pme_tiles(pmx_ctx *c) {
   TRY_RC(dalloc(c, &c->sorted, nitem));
   TRY_RC(dalloc(c, &c->bin_cnt, (size_t)s.P * a.ntile));
   TRY_RC(dalloc(c, &c->bin_off, (size_t)s.P * a.ntile));
+  TRY_RC(dalloc(c, &c->hbuf, (size_t)s.P * a.ntile * a.hvol));
+  a.hbuf = c->hbuf;
   a.item_s = c->item_s;
   a.item_f = c->item_f;
   a.item_tile = c->item_tile;
@@ -770,6 +772,8 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     hipLaunchKernelGGL(k_pme_scatter, dim3(blocks(nitem)), dim3(256), 0, st, a);
     HIPCHK(hipGetLastError());
     HIPCHK(launch_pme_spread(a, st));
+    hipLaunchKernelGGL(k_pme_fold, dim3(a.ntile, P), dim3(256), 0, st, a);
+    HIPCHK(hipGetLastError());
     FftArgs f{};
     f.P = P;
     for (int d = 0; d < 3; d++) {

@@ -69,6 +69,7 @@ struct PmeArgs {
   int *item_tile, *item_pos;
   int *bin_cnt, *bin_off;  // [P*ntile]
   int *sorted;         // [P*nchrg] item ids in bin order
+  double *hbuf;        // [P*ntile*hvol] halo boxes of spread charge
   double *grid;        // [P*gstride]
   double *fx, *fy, *fz;
 };
@@ -165,6 +166,7 @@ __global__ void k_list_build(ListArgs a);
 __global__ void k_pme_bin(PmeArgs a);
 __global__ void k_pme_scan(PmeArgs a);
 __global__ void k_pme_scatter(PmeArgs a);
+__global__ void k_pme_fold(PmeArgs a);
 __global__ void k_fill_int(int *p, int v, size_t n);
 __global__ void k_reduce_partials(RedArgs a);
 __global__ void k_fft_fwd_ab(FftArgs a);

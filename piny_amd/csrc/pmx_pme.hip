@@ -7,11 +7,11 @@
 // The MI355X layout differs from the reference's loop order:
 //  - every (charge, bead) is binned by the grid tile holding the first point
 //    of its support (k_pme_bin / k_pme_scan / k_pme_scatter, a counting sort);
-//  - one workgroup per (tile, bead) owns the tile's grid points: one lane
-//    per charge of the tile's bin and of the neighbouring bins whose supports
-//    reach into the tile adds the charge's in-tile products to an LDS copy of
-//    the tile with LDS atomics; the tile is then stored once (no global
-//    atomics, no grid clear, no partial grids);
+//  - one workgroup per (tile, bead) spreads the charges of its bin, one lane
+//    per charge, into an LDS halo box of (T + n - 1)^3 points with LDS
+//    atomics and writes the box out once (no global atomics, no grid clear);
+//  - k_pme_fold sums, per tile, the tile's own box and the overhang of the
+//    boxes below it into the periodic grid;
 //  - the gather loads the tile's halo box of (T + n - 1)^3 potential values
 //    into LDS and each lane reduces one charge's n^3 products in registers.
 #include <hip/hip_runtime.h>
@@ -167,76 +167,77 @@ __device__ __forceinline__ void item_weights(const PmeArgs &a, int it, const int
   }
 }
 
-// One workgroup per (tile, bead) owns the tile's points.  Its lanes take the
-// charges of the bins whose supports reach into the tile: the tile's own bin
-// and, per axis, the nbr[d] - 1 bins below it (periodically), whose charges
-// start at most n - 1 points before the tile.  Each lane adds the in-tile
-// part of its charge's n^3 products to the LDS tile; the tile is then stored.
+// one workgroup per (tile, bead), one lane per charge of the bin; the
+// charge's n^3 products go into the LDS halo box with LDS atomics
 template <int NI>
 __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
-  extern __shared__ double box[];  // [T2][T1][T0]
+  extern __shared__ double box[];
+  const int hvol = (int)a.hvol;
   const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  const int tcoord[3] = {tl % a.nt[0], (tl / a.nt[0]) % a.nt[1], tl / (a.nt[0] * a.nt[1])};
-  int org[3], len[3];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    org[d] = tcoord[d] * a.T[d];
-    len[d] = min(a.T[d], a.K[d] - org[d]);
-  }
-  const int T0 = a.T[0], T01 = a.T[0] * a.T[1], vol = a.T[0] * a.T[1] * a.T[2];
-  for (int k = tid; k < vol; k += blockDim.x) box[k] = 0.0;
+  for (int k = tid; k < hvol; k += blockDim.x) box[k] = 0.0;
+  const size_t bin = (size_t)b * a.ntile + tl;
+  const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
+  int org[3];
+  tile_origin(a, tl, org[0], org[1], org[2]);
+  const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
   __syncthreads();
-  for (int oc = 0; oc < a.nbr[2]; oc++)
-    for (int ob = 0; ob < a.nbr[1]; ob++)
-      for (int oa = 0; oa < a.nbr[0]; oa++) {
-        const int off[3] = {oa, ob, oc};
-        int src[3];
+  for (int c = tid; c < cnt; c += blockDim.x) {
+    const int it = a.sorted[lo + c];
+    const double q = a.q[a.ichrg[it % a.nchrg]];
+    double w[3][NI], dw[3][NI];
+    int l[3];
+    item_weights<NI, false>(a, it, org, w, dw, l);
 #pragma unroll
-        for (int d = 0; d < 3; d++) src[d] = ((tcoord[d] - off[d]) % a.nt[d] + a.nt[d]) % a.nt[d];
-        const size_t bin = (size_t)b * a.ntile + (src[2] * a.nt[1] + src[1]) * a.nt[0] + src[0];
-        const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
-        for (int c = tid; c < cnt; c += blockDim.x) {
-          const int it = a.sorted[lo + c];
-          int rel[3], m0[3], m1[3];
-          bool any = true;
+    for (int m = 0; m < NI; m++) w[0][m] *= q;
+    for (int mc = 0; mc < NI; mc++) {
 #pragma unroll
-          for (int d = 0; d < 3; d++) {
-            rel[d] = a.item_s[3 * (size_t)it + d] - org[d];
-            if (off[d] > 0 && rel[d] >= 0) rel[d] -= a.K[d];  // wrapped from the top of the grid
-            m0[d] = max(0, -rel[d]);
-            m1[d] = min(NI, len[d] - rel[d]);
-            any = any && (m0[d] < m1[d]);
-          }
-          if (!any) continue;
-          const double q = a.q[a.ichrg[it % a.nchrg]];
-          double w[3][NI];
+      for (int mb = 0; mb < NI; mb++) {
+        const double wbc = w[1][mb] * w[2][mc];
+        double *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
 #pragma unroll
-          for (int d = 0; d < 3; d++) {
-            double mn[NI + 1], dmn[NI + 1];
-            bspline<NI, false>(a.item_f[3 * (size_t)it + d], mn, dmn);
-#pragma unroll
-            for (int m = 0; m < NI; m++) w[d][m] = mn[NI - m];
-          }
-#pragma unroll
-          for (int m = 0; m < NI; m++) w[0][m] *= q;
-          for (int mc = m0[2]; mc < m1[2]; mc++) {
-            for (int mb = m0[1]; mb < m1[1]; mb++) {
-              const double wbc = w[1][mb] * w[2][mc];
-              double *row = box + (rel[2] + mc) * T01 + (rel[1] + mb) * T0 + rel[0];
-#pragma unroll
-              for (int ma = 0; ma < NI; ma++)
-                if (ma >= m0[0] && ma < m1[0]) atomicAdd(row + ma, w[0][ma] * wbc);
-            }
-          }
-        }
+        for (int ma = 0; ma < NI; ma++) atomicAdd(row + ma, w[0][ma] * wbc);
       }
+    }
+  }
   __syncthreads();
+  double *dst = a.hbuf + bin * a.hvol;
+  for (int k = tid; k < hvol; k += blockDim.x) dst[k] = box[k];
+}
+
+// Sums the halo boxes into the periodic grid, one workgroup per (tile, bead):
+// core point l of tile t collects l + o*T from the box of tile t - o (per
+// axis, periodically, for o < nbr), which holds that point when
+// l + o*T < T + n - 1.  Tiles divide the grid, so the offset is exact also
+// across the periodic boundary.
+__global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
+  const int tl = blockIdx.x, b = blockIdx.y;
+  const int tcoord[3] = {tl % a.nt[0], (tl / a.nt[0]) % a.nt[1], tl / (a.nt[0] * a.nt[1])};
+  const int T0 = a.T[0], T1 = a.T[1], T01 = T0 * T1, vol = T01 * a.T[2];
+  const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
+  const double *hb = a.hbuf + (size_t)b * a.ntile * a.hvol;
   double *g = a.grid + (size_t)b * a.gstride;
   const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
-  for (int k = tid; k < vol; k += blockDim.x) {
-    const int la = k % T0, lb = (k / T0) % a.T[1], lc = k / T01;
-    if (la < len[0] && lb < len[1] && lc < len[2])
-      g[(size_t)(org[2] + lc) * Kab + (size_t)(org[1] + lb) * Ka + org[0] + la] = box[k];
+  for (int k = threadIdx.x; k < vol; k += blockDim.x) {
+    const int l[3] = {k % T0, (k / T0) % T1, k / T01};
+    double s = 0.0;
+    for (int oc = 0; oc < a.nbr[2]; oc++) {
+      const int lc = l[2] + oc * a.T[2];
+      if (lc >= a.H[2]) break;
+      const int sc = ((tcoord[2] - oc) % a.nt[2] + a.nt[2]) % a.nt[2];
+      for (int ob = 0; ob < a.nbr[1]; ob++) {
+        const int lb = l[1] + ob * T1;
+        if (lb >= a.H[1]) break;
+        const int sb = ((tcoord[1] - ob) % a.nt[1] + a.nt[1]) % a.nt[1];
+        for (int oa = 0; oa < a.nbr[0]; oa++) {
+          const int la = l[0] + oa * T0;
+          if (la >= H0) break;
+          const int sa = ((tcoord[0] - oa) % a.nt[0] + a.nt[0]) % a.nt[0];
+          const size_t src = (size_t)(sc * a.nt[1] + sb) * a.nt[0] + sa;
+          s += hb[src * a.hvol + (size_t)lc * H01 + lb * H0 + la];
+        }
+      }
+    }
+    g[(size_t)(tcoord[2] * a.T[2] + l[2]) * Kab + (size_t)(tcoord[1] * T1 + l[1]) * Ka + tcoord[0] * T0 + l[0]] = s;
   }
 }
 
@@ -317,7 +318,8 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
 }
 
 size_t pme_lds_bytes(const PmeArgs &a, bool gather) {
-  return gather ? a.hvol * sizeof(double) : (size_t)a.T[0] * a.T[1] * a.T[2] * sizeof(double);
+  (void)gather;
+  return a.hvol * sizeof(double);
 }
 
 #define PMX_PME_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
