@@ -1,0 +1,121 @@
+/*
+ * ref_harness.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A driver for the reference build (oracle/ref.mk): it links the
+ * reference's own objects, reads a PINY input directory with the
+ * reference's parse(), overwrites the bead positions with exact values, and
+ * runs the reference force call once, as python/piny/piny.pyx:170-188,357-359
+ * and 426-436 do.  Results are written as raw doubles for the golden-vector
+ * script under tests/golden/.  Nothing in the product links it.
+ *
+ * usage: piny_ref <input file> <positions.bin> <out.bin>
+ *   positions.bin: P*N*3 doubles, Bohr, [bead][site][x,y,z] (ghost sites
+ *                  are recomputed by get_ghost_pos)
+ *   out.bin:       header {N, P, nver} as doubles, then 24 energy/virial
+ *                  doubles (see the list below), pvten_tot, then forces
+ *                  [bead][site][3] (for P > 1: normal-mode forces including
+ *                  the springs, as energy_control_pimd leaves them) and,
+ *                  for P > 1, the PIMD virial forces fxt in the same layout
+ */
+#include "standard_include.h"
+#include "../typ_defs/typedefs_class.h"
+#include "../typ_defs/typedefs_bnd.h"
+#include "../typ_defs/typedefs_gen.h"
+#include "../typ_defs/typedefs_cp.h"
+#include "../typ_defs/typedefs_par.h"
+#include "../typ_defs/typedefs_stat.h"
+#include "../proto_defs/proto_parse_entry.h"
+#include "../proto_defs/proto_communicate_wrappers.h"
+#include "../proto_defs/proto_energy_ctrl_entry.h"
+#include "../proto_defs/proto_intra_con_entry.h"
+#include "../proto_defs/proto_pimd_entry.h"
+
+static CLASS cls;
+static BONDED bonded;
+static GENERAL_DATA gd;
+static CP cp;
+static ANALYSIS analysis;
+
+int main(int argc, char *argv[]) {
+  if (argc != 4) {
+    fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin>\n", argv[0]);
+    return 2;
+  }
+  int zero = 0;
+  char **noargs = NULL;
+  Init(&zero, &noargs, &(cls.communicate.world));
+  Comm_size(cls.communicate.world, &(cls.communicate.np));
+  Comm_rank(cls.communicate.world, &(cls.communicate.myid));
+  gd.error_check_on = 1;
+  parse(&cls, &bonded, &gd, &cp, &analysis, argv[1]);
+
+  cls.energy_ctrl.iget_full_inter = 1;
+  cls.energy_ctrl.iget_res_inter = 0;
+  cls.energy_ctrl.iget_full_intra = 1;
+  cls.energy_ctrl.iget_res_intra = 0;
+  /* real-space energy and pressure every call (test_energy.c:506-507) */
+  cls.energy_ctrl.iget_pe_real_inter = 1;
+  cls.energy_ctrl.iget_pv_real_inter = 1;
+
+  const int N = cls.clatoms_info.natm_tot, P = cls.clatoms_info.pi_beads;
+  FILE *f = fopen(argv[2], "rb");
+  if (!f) {
+    fprintf(stderr, "cannot open %s\n", argv[2]);
+    return 1;
+  }
+  double *pos = (double *)malloc(sizeof(double) * 3 * (size_t)N * P);
+  if (fread(pos, sizeof(double), 3 * (size_t)N * P, f) != 3 * (size_t)N * P) {
+    fprintf(stderr, "short positions file\n");
+    return 1;
+  }
+  fclose(f);
+  for (int ip = 1; ip <= P; ip++) {
+    for (int i = 1; i <= N; i++) {
+      const double *p = pos + 3 * ((size_t)(ip - 1) * N + (i - 1));
+      cls.clatoms_pos[ip].x[i] = p[0];
+      cls.clatoms_pos[ip].y[i] = p[1];
+      cls.clatoms_pos[ip].z[i] = p[2];
+    }
+    get_ghost_pos(&(cls.clatoms_info), &(cls.clatoms_pos[ip]), &(cls.ghost_atoms));
+  }
+  if (P > 1) {
+    /* the integrator's order (int_0_to_dt2_pimd.c:156-175): beads to
+     * modes, spring forces fxm and kin_harm in mode coordinates, modes back
+     * to beads (which sets xmod from mode 1), then the force call, which
+     * returns mode forces plus fxm (energy_control_pimd.c:198-214) */
+    cls.clatoms_info.wght_pimd = 1.0; /* as control_pimd.c:311 */
+    control_pimd_trans_mode(&cls, &gd);
+    mode_energy_control(&cls, &gd);
+    control_pimd_trans_pos(&cls, &gd);
+    for (int ip = 1; ip <= P; ip++) get_ghost_pos(&(cls.clatoms_info), &(cls.clatoms_pos[ip]), &(cls.ghost_atoms));
+    energy_control_pimd(&cls, &bonded, &gd);
+  } else {
+    energy_control(&cls, &bonded, &gd);
+  }
+
+  STAT_AVG *s = &gd.stat_avg;
+  double hdr[3] = {(double)N, (double)P, (double)cls.nbr_list.verlist.nver_lst_now};
+  double e[24] = {s->vintert, s->vintrat, s->vreal, s->vrecip, s->vvdw, s->vcoul, s->vlong,
+                  s->vbondt,  s->vbendt,  s->pi_ke_prim, s->pi_ke_vir, s->kin_harm};
+  /* e[12..20]: pvten[1..9]; e[21..23] unused */
+  for (int k = 0; k < 9; k++) e[12 + k] = gd.ptens.pvten[k + 1];
+  f = fopen(argv[3], "wb");
+  if (!f) return 1;
+  fwrite(hdr, sizeof(double), 3, f);
+  fwrite(e, sizeof(double), 24, f);
+  for (int k = 0; k < 9; k++) fwrite(&gd.ptens.pvten_tot[k + 1], sizeof(double), 1, f);
+  for (int ip = 1; ip <= P; ip++)
+    for (int i = 1; i <= N; i++) {
+      double v[3] = {cls.clatoms_pos[ip].fx[i], cls.clatoms_pos[ip].fy[i], cls.clatoms_pos[ip].fz[i]};
+      fwrite(v, sizeof(double), 3, f);
+    }
+  if (P > 1)
+    for (int ip = 1; ip <= P; ip++)
+      for (int i = 1; i <= N; i++) {
+        double v[3] = {cls.clatoms_pos[ip].fxt[i], cls.clatoms_pos[ip].fyt[i], cls.clatoms_pos[ip].fzt[i]};
+        fwrite(v, sizeof(double), 3, f);
+      }
+  fclose(f);
+  free(pos);
+  return 0;
+}

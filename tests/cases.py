@@ -24,6 +24,11 @@ CASES = {
                                           _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, ishift=1)),
     "spc64_none": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
                                          _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, ishift=0)),
+    # 64 q-TIP4P/F, P = 1, K = 16
+    "tip64": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 1, 18,
+                                    _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
+    # configuration 2 at P = 4, K = 32
+    "cfg2_p4": lambda: w.config2(P=4),
     # 64 q-TIP4P/F, P = 4, K = 16
     "tip64_p4": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
                                        _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),

@@ -1,0 +1,177 @@
+"""Writes golden vectors from the reference itself: oracle/_ref/piny_ref is
+the reference's own C and Fortran compiled from /root/reference by
+oracle/ref.mk and linked with oracle/ref_harness.c, which runs the
+reference's parse() on the input files written here and then one
+energy_control() (P = 1) or energy_control_pimd() (P > 1) on the case's
+exact positions.
+
+Run in a container where /root/reference is mounted and the reference has
+been built (make -C oracle -f ref.mk); the .npz files it writes next to this
+script are the committed fixtures, holding the inputs (case name and
+positions) and the reference's outputs.
+
+The input files follow the formats of python/piny/tools.py:137-182 and the
+example inputs examples/q-SPC-Fw/classical/build.py and
+examples/q-TIP4P-F/build.py; the force-field values are the ones
+piny_amd/water.py converts (Angstrom, Kelvin via kB = 0.001987204118
+kcal/mol/K, amu, degrees).
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from piny_amd import water as w  # noqa: E402
+from tests import cases  # noqa: E402
+
+EXE = os.path.join(ROOT, "oracle", "_ref", "piny_ref")
+CASES = ["spc64", "spc64_shift", "spc64_none", "tip64", "cfg1", "spc64_k20", "tip64_p4", "cfg2_p4"]
+SHIFT = {0: "off", 1: "on", 2: "swit"}  # shift_inter_pe values (set_sim_dict.c:736)
+
+
+def section(name, items):
+    body = "\n".join("    \\%s{%s}" % (k, v) for k, v in items)
+    return "~%s[\n%s]" % (name, body)
+
+
+def write_inputs(d, c):
+    s = c.system
+    st = s.settings
+    m = s.model
+    P = s.pi_beads
+    # PIMD needs a thermostatted ensemble in the reference's input checks; the
+    # thermostat does not enter the force call
+    gen = [("simulation_typ", "pimd" if P > 1 else "md"), ("ensemble_typ", "nvt" if P > 1 else "nve"),
+           ("num_time_step", 0),
+           ("restart_type", "initial"), ("time_step", 0.25), ("temperature", st.temperature),
+           ("generic_fft_opt", "on"), ("num_proc_tot", 1), ("num_proc_beads", 1), ("num_proc_class_forc", 1)]
+    parts = [section("sim_gen_def", gen)]
+    if P > 1:
+        parts.append(section("sim_pimd_def", [("path_int_beads", P), ("path_int_md_typ", "centroid"),
+                                              ("initial_spread_opt", "off")]))
+    if P > 1:
+        parts.append(section("sim_nhc_def", [("atm_nhc_tau_def", 20.0), ("atm_nhc_len", 4), ("respa_steps_nhc", 4),
+                                             ("yosh_steps_nhc", 3), ("respa_xi_opt", 1)]))
+    parts.append(section("sim_list_def", [("neighbor_list", "ver_list"), ("verlist_skin", repr(st.skin)),
+                                          ("update_type", "no_list")]))
+    parts.append(section("sim_run_def", [("respa_rheal", repr(st.rheal))]))
+    parts.append(section("sim_vol_def", [("periodicity", 3), ("intra_perds", "on" if st.intra_perds else "off")]))
+    parts.append(section("sim_class_PE_def", [
+        ("shift_inter_pe", SHIFT[st.ishift]), ("inter_spline_pts", st.nsplin),
+        ("ewald_alpha", repr(s.ewald_alpha)), ("ewald_kmax", st.kmax_ewd),
+        ("ewald_pme_opt", "on" if st.pme_on else "off"), ("ewald_kmax_pme", st.kmax_pme),
+        ("ewald_interp_pme", st.n_interp), ("ewald_respa_pme_opt", "off"), ("inter_PE_calc_freq", 1)]))
+    parts.append(section("sim_write_def", [
+        ("sim_name", "water.input-all"), ("write_screen_freq", 1000000), ("instant_file", "water.iavg"),
+        ("write_inst_freq", 1000000), ("atm_pos_file", "water.confp"), ("write_pos_freq", 1000000),
+        ("path_cent_file", "water-centroid.confp"), ("path_cent_freq", 1000000),
+        ("atm_vel_file", "water.confv"), ("write_vel_freq", 1000000), ("atm_force_file", "water.conff"),
+        ("write_force_freq", 1000000), ("out_restart_file", "water.restart"), ("write_dump_freq", 1000000),
+        ("in_restart_file", "water.initial"), ("mol_set_file", "water.set"),
+        ("conf_file_format", "formatted")]))
+    open(os.path.join(d, "water.input"), "w").write("\n\n".join(parts) + "\n")
+
+    kb = w.KB_KCAL
+    parm = [section("molecule_name_def", [("molecule_name", "water"), ("natom", len(m.sites))])]
+    ghost = dict()
+    if m.ghost is not None:
+        ghost[m.ghost[0]] = m.ghost[1]
+    for k, (name, q, mass) in enumerate(zip(m.sites, m.charges, m.masses_amu)):
+        items = [("atom_typ", name), ("atom_ind", k + 1), ("mass", repr(mass)), ("charge", repr(q))]
+        for g, (comp, coef) in enumerate(ghost.get(k, ())):
+            items.append(("def_ghost%d" % (g + 1), "%d, %r" % (comp + 1, coef)))
+        parm.append(section("atom_def", items))
+    parm.append(section("bond_def", [("atom1", 1), ("atom2", 2)]))
+    parm.append(section("bond_def", [("atom1", 1), ("atom2", 3)]))
+    parm.append(section("bend_def", [("atom1", 2), ("atom2", 1), ("atom3", 3)]))
+    open(os.path.join(d, "water.parm"), "w").write("\n\n".join(parm) + "\n")
+
+    if m.bond[0] == "harm":
+        bond = [("atom1", "O"), ("atom2", "H"), ("pot_type", "harm"), ("eq", repr(m.bond[2])),
+                ("fk", repr(m.bond[1] / kb))]
+    else:
+        bond = [("atom1", "O"), ("atom2", "H"), ("pot_type", "morse"), ("eq", repr(m.bond[3])),
+                ("alpha", repr(m.bond[2])), ("d0", repr(m.bond[1] / kb))]
+    open(os.path.join(d, "water.bond"), "w").write(section("bond_parm", bond) + "\n")
+    bend = [("atom1", "H"), ("atom2", "O"), ("atom3", "H"), ("pot_type_bend", "harm"),
+            ("eq_bend", repr(m.bend[2])), ("fk_bend", repr(m.bend[1] / kb))]
+    open(os.path.join(d, "water.bend"), "w").write(section("bend_parm", bend) + "\n")
+
+    types = []
+    for n in m.sites:
+        if n not in types:
+            types.append(n)
+    inter = []
+    for a in range(len(types)):
+        for b in range(a, len(types)):
+            items = [("atom1", types[a]), ("atom2", types[b])]
+            if types[a] == "O" and types[b] == "O":
+                items += [("pot_type", "lennard-jones"), ("eps", repr(m.lj_eps_kcal / kb)), ("sig", repr(m.lj_sig))]
+            else:
+                items += [("pot_type", "null")]
+            items += [("min_dist", repr(st.min_dist)), ("max_dist", repr(st.cutoff)), ("res_dist", repr(st.res_dist))]
+            inter.append(section("inter_parm", items))
+    open(os.path.join(d, "water.inter"), "w").write("\n\n".join(inter) + "\n")
+
+    open(os.path.join(d, "water.set"), "w").write("\n\n".join([
+        section("molecule_def", [("mol_parm_file", "water.parm"), ("mol_opt_nhc", "global"),
+                                 ("num_mol", s.nmol), ("mol_index", 1), ("mol_name", "water")]),
+        section("data_base_def", [("bond_file", "water.bond"), ("bend_file", "water.bend"),
+                                  ("inter_file", "water.inter")])]) + "\n")
+
+    # initial file (Angstrom): P blocks of N sites (read_coord.c:383-400); the
+    # harness then sets every bead to the exact Bohr values
+    N = s.natm
+    lines = ["%d 1 %d" % (N, P)]
+    for i in range(N * P):
+        lines.append("%.15f %.15f %.15f" % (c.x[i] * w.BOHR, c.y[i] * w.BOHR, c.z[i] * w.BOHR))
+    for r in range(3):
+        row = [s.box if k == r else 0.0 for k in range(3)]
+        lines.append("%.15f %.15f %.15f" % tuple(row))
+    open(os.path.join(d, "water.initial"), "w").write("\n".join(lines) + "\n")
+
+    pos = np.stack([c.x, c.y, c.z], 1).reshape(P, N, 3)
+    pos.astype(np.float64).tofile(os.path.join(d, "pos.bin"))
+
+
+def run_case(name):
+    c = cases.get(name)
+    s = c.system
+    with tempfile.TemporaryDirectory() as d:
+        write_inputs(d, c)
+        with open(os.path.join(d, "stdout.txt"), "w") as log:
+            r = subprocess.run([EXE, "water.input", "pos.bin", "out.bin"], cwd=d, stdout=log,
+                               stderr=subprocess.STDOUT, timeout=600)
+        if r.returncode != 0:
+            print(open(os.path.join(d, "stdout.txt")).read()[-3000:])
+            raise RuntimeError(f"{name}: reference run failed ({r.returncode})")
+        raw = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64)
+    N, P, nver = int(raw[0]), int(raw[1]), int(raw[2])
+    assert N == s.natm and P == s.pi_beads, (N, P)
+    e = raw[3:27]
+    pvten_tot = raw[27:36]
+    f = raw[36:36 + 3 * N * P].reshape(P, N, 3)
+    fxt = raw[36 + 3 * N * P:36 + 6 * N * P].reshape(P, N, 3) if P > 1 else np.zeros((0, N, 3))
+    keys = ["vintert", "vintrat", "vreal", "vrecip", "vvdw", "vcoul", "vlong", "vbond", "vbend",
+            "pi_ke_prim", "pi_ke_vir", "kin_harm"]
+    out = os.path.join(HERE, f"ref_{name}.npz")
+    np.savez_compressed(out, case=name, x=c.x, y=c.y, z=c.z, nver=nver,
+                        energy_keys=np.array(keys), energies=e[:12], pvten=e[12:21], pvten_tot=pvten_tot,
+                        forces=f, fxt=fxt)
+    print(f"{name}: N={N} P={P} nver={nver} vintert={e[0]:.12g} vintrat={e[1]:.12g} -> {out}")
+
+
+def main():
+    names = sys.argv[1:] or CASES
+    for n in names:
+        run_case(n)
+
+
+if __name__ == "__main__":
+    main()

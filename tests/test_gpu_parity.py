@@ -178,3 +178,26 @@ def test_repeat_runs_reproduce():
     f2 = g.forces()
     assert np.max(np.abs(f1 - f2)) <= 1e-12 * np.max(np.abs(f1))
     assert abs(e1["vintert"] - e2["vintert"]) <= 1e-12 * abs(e1["vintert"])
+
+
+# --- the device against the reference itself (tests/golden/ref_*.npz) -------
+
+from tests.test_ref_golden import FIXTURES, compare, load  # noqa: E402
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_device_matches_reference(name, mode):
+    d, ref = load(name)
+    c = cases.get(name)
+    s = c.system
+    print(f"[reference fixture {name}] N={s.natm} P={s.pi_beads} {mode}", flush=True)
+    kw = {} if mode == "fp64" else {"precision": abi.PMX_PREC_MIXED}
+    g = Engine(s, c.x, c.y, c.z, **kw)
+    g.set_positions(c.x, c.y, c.z)
+    flags = (abi.PMX_EVAL_PIMD | abi.PMX_EVAL_MODE_FORCES) if s.pi_beads > 1 else 0
+    e = g.energy_control(flags)
+    f = g.forces()
+    etol, ftol, vtol = (ETOL, FTOL, ETOL) if mode == "fp64" else (MIXED_ETOL, MIXED_FTOL, MIXED_VTOL)
+    rms = compare(name, e, f, d, ref, etol, ftol, vtol)
+    print(f"  force rms rel err vs the reference {rms:.3e}", flush=True)
