@@ -8,7 +8,7 @@
  * and 426-436 do.  Results are written as raw doubles for the golden-vector
  * script under tests/golden/.  Nothing in the product links it.
  *
- * usage: piny_ref <input file> <positions.bin> <out.bin>
+ * usage: piny_ref <input file> <positions.bin> <out.bin> [repeat]
  *   positions.bin: P*N*3 doubles, Bohr, [bead][site][x,y,z] (ghost sites
  *                  are recomputed by get_ghost_pos)
  *   out.bin:       header {N, P, nver} as doubles, then 24 energy/virial
@@ -16,7 +16,13 @@
  *                  [bead][site][3] (for P > 1: normal-mode forces including
  *                  the springs, as energy_control_pimd leaves them) and,
  *                  for P > 1, the PIMD virial forces fxt in the same layout
+ *   repeat:        optional; after the first call (which builds the Verlet
+ *                  list) the force call is made this many more times on the
+ *                  same positions and the wall time of those calls
+ *                  (CLOCK_MONOTONIC, as src/output/timer.c:55-64) is printed
+ *                  as "REPEAT <calls> <seconds>"
  */
+#include <time.h>
 #include "standard_include.h"
 #include "../typ_defs/typedefs_class.h"
 #include "../typ_defs/typedefs_bnd.h"
@@ -37,10 +43,11 @@ static CP cp;
 static ANALYSIS analysis;
 
 int main(int argc, char *argv[]) {
-  if (argc != 4) {
-    fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin>\n", argv[0]);
+  if (argc != 4 && argc != 5) {
+    fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin> [repeat]\n", argv[0]);
     return 2;
   }
+  const int repeat = argc == 5 ? atoi(argv[4]) : 0;
   int zero = 0;
   char **noargs = NULL;
   Init(&zero, &noargs, &(cls.communicate.world));
@@ -91,6 +98,14 @@ int main(int argc, char *argv[]) {
     energy_control_pimd(&cls, &bonded, &gd);
   } else {
     energy_control(&cls, &bonded, &gd);
+    if (repeat > 0) {
+      struct timespec t0, t1;
+      clock_gettime(CLOCK_MONOTONIC, &t0);
+      for (int r = 0; r < repeat; r++) energy_control(&cls, &bonded, &gd);
+      clock_gettime(CLOCK_MONOTONIC, &t1);
+      printf("REPEAT %d %.6f\n", repeat, (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec));
+      fflush(stdout);
+    }
   }
 
   STAT_AVG *s = &gd.stat_avg;
