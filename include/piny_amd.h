@@ -245,6 +245,26 @@ int pmx_get_timings(pmx_ctx *ctx, double *ms5);
  * summed over beads (the work unit of the pair kernel). */
 int pmx_get_pair_count(pmx_ctx *ctx, double *npair);
 
+/* Velocities of all beads, bead-major [P*N], Bohr per atomic time unit
+ * (clatoms_pos[ip].vx); zero after pmx_create. */
+int pmx_set_velocities(pmx_ctx *ctx, const double *vx, const double *vy,
+                       const double *vz);
+int pmx_get_velocities(pmx_ctx *ctx, double *vx, double *vy, double *vz);
+
+/* Current positions, bead-major [P*N], Bohr (ghost sites included). */
+int pmx_get_positions(pmx_ctx *ctx, double *x, double *y, double *z);
+
+/* nsteps of velocity Verlet in the NVE ensemble, as int_NVE
+ * (md/int_NVE.c:29-125 with int_0_to_dt2_nve and int_dt2_to_dt_nve): per
+ * step v += f dt/2 / m, x += v dt, ghost positions, the force call
+ * (energy_control), v += f dt/2 / m.  dt in atomic time units (PINY's
+ * timeinfo.dt = time_step / TIME_CONV, defines.h:85).  Classical contexts
+ * (P = 1) only; forces at the starting positions are computed first when
+ * the context has none.  out receives the energies of the last force call,
+ * kinetic (if not NULL) 1/2 sum m v^2 at the end (get_tvten). */
+int pmx_md_nve(pmx_ctx *ctx, int nsteps, double dt, pmx_energies *out,
+               double *kinetic);
+
 /* Coefficient helpers restating intra_coefs.c (units: Hartree, Bohr). */
 int pmx_bond_coef_harm(double fk, double *c7);
 int pmx_bond_coef_morse(double d0, double alpha, double *c7);

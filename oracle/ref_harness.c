@@ -9,6 +9,7 @@
  * script under tests/golden/.  Nothing in the product links it.
  *
  * usage: piny_ref <input file> <positions.bin> <out.bin> [repeat]
+ *        piny_ref <input file> <positions.bin> <out.bin> md <nsteps> <velocities.bin>
  *   positions.bin: P*N*3 doubles, Bohr, [bead][site][x,y,z] (ghost sites
  *                  are recomputed by get_ghost_pos)
  *   out.bin:       header {N, P, nver} as doubles, then 24 energy/virial
@@ -21,6 +22,12 @@
  *                  same positions and the wall time of those calls
  *                  (CLOCK_MONOTONIC, as src/output/timer.c:55-64) is printed
  *                  as "REPEAT <calls> <seconds>"
+ *   md:            P = 1; after the first force call the velocities
+ *                  (N*3 doubles, Bohr per atomic time unit) are set and
+ *                  int_NVE (md/int_NVE.c) runs nsteps times with the
+ *                  input's time step; out.bin then holds the header, the
+ *                  energies of the last force call, kinet, and the final
+ *                  positions, velocities and forces, [site][3] each
  */
 #include <time.h>
 #include "standard_include.h"
@@ -35,6 +42,7 @@
 #include "../proto_defs/proto_energy_ctrl_entry.h"
 #include "../proto_defs/proto_intra_con_entry.h"
 #include "../proto_defs/proto_pimd_entry.h"
+#include "../proto_defs/proto_integrate_md_entry.h"
 
 static CLASS cls;
 static BONDED bonded;
@@ -43,8 +51,10 @@ static CP cp;
 static ANALYSIS analysis;
 
 int main(int argc, char *argv[]) {
-  if (argc != 4 && argc != 5) {
-    fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin> [repeat]\n", argv[0]);
+  const int md = argc == 7 && strcmp(argv[4], "md") == 0;
+  if (argc != 4 && argc != 5 && !md) {
+    fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin> [repeat | md nsteps velocities.bin]\n",
+            argv[0]);
     return 2;
   }
   const int repeat = argc == 5 ? atoi(argv[4]) : 0;
@@ -98,6 +108,37 @@ int main(int argc, char *argv[]) {
     energy_control_pimd(&cls, &bonded, &gd);
   } else {
     energy_control(&cls, &bonded, &gd);
+    if (md) {
+      const int nsteps = atoi(argv[5]);
+      FILE *fv = fopen(argv[6], "rb");
+      if (!fv) return 1;
+      double *vel = (double *)malloc(sizeof(double) * 3 * (size_t)N);
+      if (fread(vel, sizeof(double), 3 * (size_t)N, fv) != 3 * (size_t)N) return 1;
+      fclose(fv);
+      for (int i = 1; i <= N; i++) {
+        cls.clatoms_pos[1].vx[i] = vel[3 * (i - 1)];
+        cls.clatoms_pos[1].vy[i] = vel[3 * (i - 1) + 1];
+        cls.clatoms_pos[1].vz[i] = vel[3 * (i - 1) + 2];
+      }
+      free(vel);
+      for (int st = 0; st < nsteps; st++) int_NVE(&cls, &bonded, &gd);
+      STAT_AVG *s = &gd.stat_avg;
+      double hdr[3] = {(double)N, (double)P, (double)cls.nbr_list.verlist.nver_lst_now};
+      double e[4] = {s->vintert, s->vintrat, s->kinet, gd.timeinfo.dt};
+      FILE *fo = fopen(argv[3], "wb");
+      if (!fo) return 1;
+      fwrite(hdr, sizeof(double), 3, fo);
+      fwrite(e, sizeof(double), 4, fo);
+      for (int i = 1; i <= N; i++) {
+        double v[9] = {cls.clatoms_pos[1].x[i],  cls.clatoms_pos[1].y[i],  cls.clatoms_pos[1].z[i],
+                       cls.clatoms_pos[1].vx[i], cls.clatoms_pos[1].vy[i], cls.clatoms_pos[1].vz[i],
+                       cls.clatoms_pos[1].fx[i], cls.clatoms_pos[1].fy[i], cls.clatoms_pos[1].fz[i]};
+        fwrite(v, sizeof(double), 9, fo);
+      }
+      fclose(fo);
+      free(pos);
+      return 0;
+    }
     if (repeat > 0) {
       struct timespec t0, t1;
       clock_gettime(CLOCK_MONOTONIC, &t0);

@@ -34,11 +34,15 @@ struct pmx_ctx {
   // positions / forces, atom-major [N*P]
   double *x = nullptr, *y = nullptr, *z = nullptr, *stage = nullptr;
   double *fx = nullptr, *fy = nullptr, *fz = nullptr;
+  double *fxt = nullptr, *fyt = nullptr, *fzt = nullptr;  // bead forces kept when modes are returned
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr;     // velocities, atom-major
+  double *ke = nullptr;                                   // kinetic energy scratch
+  bool last_modes = false, forces_valid = false;
   double *xm = nullptr, *ym = nullptr, *zm = nullptr;
   double *b0x = nullptr, *b0y = nullptr, *b0z = nullptr, *x0 = nullptr, *y0 = nullptr, *z0 = nullptr;
   unsigned long long *maxes = nullptr;
   int *type = nullptr, *ichrg = nullptr, *ex_off = nullptr, *ex_num = nullptr, *ex_j = nullptr;
-  double *q = nullptr;
+  double *q = nullptr, *mass = nullptr;
   double2 *tab_lj = nullptr, *tab_c = nullptr, *ecor_tab = nullptr;
   double *pair_pp = nullptr;   // [npair][PP_NUM]
   double4 *site = nullptr;     // [N] (xmod, q)
@@ -55,6 +59,8 @@ struct pmx_ctx {
   size_t gstride = 0, sstride = 0;
   double *grid = nullptr, *infl = nullptr;
   double2 *spec = nullptr, *tw[3] = {nullptr, nullptr, nullptr};
+  float2 *twf[3] = {nullptr, nullptr, nullptr};
+  int pme_f32 = 0;  // grids, B-splines and FFT in FP32 (the mixed precision)
   // binned PME: tiling, per-item data, bins, halo boxes
   PmeArgs pa{};
   int *item_s = nullptr, *item_tile = nullptr, *item_pos = nullptr, *bin_cnt = nullptr, *bin_off = nullptr,
@@ -184,6 +190,7 @@ int setup_pme_tiles(pmx_ctx *c) {
   a.P = s.P;
   a.nchrg = (int)s.ichrg.size();
   a.n = n;
+  a.f32 = c->pme_f32;
   // tiles divide the grid, so every tile is whole and a support of n points
   // starting in a tile reaches at most ceil((n-1)/T) tiles beyond it
   int cap[3] = {16, 16, 8};
@@ -265,6 +272,16 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(dalloc(c, &c->fx, NP));
   TRY(dalloc(c, &c->fy, NP));
   TRY(dalloc(c, &c->fz, NP));
+  TRY(dalloc(c, &c->fxt, NP));
+  TRY(dalloc(c, &c->fyt, NP));
+  TRY(dalloc(c, &c->fzt, NP));
+  TRY(dalloc(c, &c->vx, NP));
+  TRY(dalloc(c, &c->vy, NP));
+  TRY(dalloc(c, &c->vz, NP));
+  TRY(dalloc(c, &c->ke, 1));
+  HIPCHK(hipMemset(c->vx, 0, NP * sizeof(double)));
+  HIPCHK(hipMemset(c->vy, 0, NP * sizeof(double)));
+  HIPCHK(hipMemset(c->vz, 0, NP * sizeof(double)));
   TRY(dalloc(c, &c->xm, N));
   TRY(dalloc(c, &c->ym, N));
   TRY(dalloc(c, &c->zm, N));
@@ -277,6 +294,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(dalloc(c, &c->maxes, 2));
   TRY(upload(c, &c->type, sys->atm_type, N));
   TRY(upload(c, &c->q, sys->q, N));
+  TRY(upload(c, &c->mass, sys->mass, N));
   TRY(upload(c, &c->ichrg, s.ichrg.data(), s.ichrg.size()));
   // the list kernel tests exclusions in both directions
   TRY(upload(c, &c->ex_off, s.exs_off.data(), N));
@@ -345,6 +363,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(upload(c, &c->gcoef, sys->ghost_coef, (size_t)4 * sys->nghost));
   // PME grids, twiddles and the influence function on the sphere list
   c->pme = (!s.ichrg.empty() && c->par.pme_on);
+  c->pme_f32 = (c->par.precision == PMX_PREC_MIXED) ? 1 : 0;
   if (!s.ichrg.empty() && !c->par.pme_on) {
     set_err(c, "standard Ewald (ewald_pme_opt off) is not implemented on the device; use PME");
     return PMX_ERR_NOTIMPL;
@@ -371,6 +390,9 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
       std::vector<double2> t(K);
       for (int m = 0; m < K; m++) t[m] = make_double2(cos(2.0 * M_PI * m / K), -sin(2.0 * M_PI * m / K));
       TRY(upload(c, &c->tw[d], t.data(), K));
+      std::vector<float2> tf(K);
+      for (int m = 0; m < K; m++) tf[m] = make_float2((float)t[m].x, (float)t[m].y);
+      TRY(upload(c, &c->twf[d], tf.data(), K));
     }
     std::vector<double> W(c->sstride, 0.0);
     const double *hi = s.hmati;
@@ -772,15 +794,15 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     hipLaunchKernelGGL(k_pme_scatter, dim3(blocks(nitem)), dim3(256), 0, st, a);
     HIPCHK(hipGetLastError());
     HIPCHK(launch_pme_spread(a, st));
-    hipLaunchKernelGGL(k_pme_fold, dim3(a.ntile, P), dim3(256), 0, st, a);
-    HIPCHK(hipGetLastError());
     FftArgs f{};
     f.P = P;
+    f.f32 = c->pme_f32;
     for (int d = 0; d < 3; d++) {
       f.K[d] = s.ngrid[d];
       f.npass[d] = c->npass[d];
       for (int k = 0; k < c->npass[d]; k++) f.radix[d][k] = c->radix[d][k];
       f.tw[d] = c->tw[d];
+      f.twf[d] = c->twf[d];
     }
     f.gstride = c->gstride;
     f.sstride = c->sstride;
@@ -790,14 +812,9 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     f.falp2 = 4.0 * s.alp_ewd * s.alp_ewd;
     f.cell = c->cell;
     f.part = c->part + c->off_fft;
-    const int Ka = s.ngrid[0], Kb = s.ngrid[1], Kc = s.ngrid[2], Ha = Ka / 2 + 1;
-    hipLaunchKernelGGL(k_fft_fwd_ab, dim3(Kc, P), dim3(256), (size_t)Ka * Kb * sizeof(double2), st, f);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_fft_c_conv, dim3(Kb, P), dim3(256), (size_t)Kc * Ha * sizeof(double2), st, f);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_fft_inv_ba, dim3(Kc, P), dim3(256), (size_t)Ka * Kb * sizeof(double2), st, f);
-    HIPCHK(hipGetLastError());
+    HIPCHK(launch_pme_fft(f, st));
     HIPCHK(launch_pme_gather(a, st));
+    const int Kb = s.ngrid[1];
     add_stage(red, c->off_fft, Kb, FFT_NV, {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
   }
   IntraArgs ia = intra_args(c, pimd);
@@ -859,6 +876,13 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     HIPCHK(hipGetLastError());
   }
   const bool modes = pimd && (flags & PMX_EVAL_MODE_FORCES) && P > 1;
+  if (modes) {  // fxt: the bead forces before the transform (energy_control_pimd.c:116-144)
+    HIPCHK(hipMemcpyAsync(c->fxt, c->fx, NP * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->fyt, c->fy, NP * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(c->fzt, c->fz, NP * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  c->last_modes = modes;
+  c->forces_valid = true;
   if (modes) {
     ModeArgs a{};
     a.N = N;
@@ -987,11 +1011,113 @@ int pmx_get_forces_device(pmx_ctx *c, const double **fx, const double **fy, cons
   return PMX_OK;
 }
 
+static int get_bead_major(pmx_ctx *c, double *const src[3], double *fx, double *fy, double *fz) {
+  Setup &s = c->s;
+  const size_t NP = (size_t)s.N * s.P;
+  HIPCHK(hipSetDevice(c->device));
+  double *dst[3] = {fx, fy, fz};
+  for (int d = 0; d < 3; d++) {
+    hipLaunchKernelGGL(k_atom_to_bead, dim3(blocks(NP)), dim3(256), 0, c->st, s.N, s.P, src[d], c->stage);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst[d], c->stage, NP * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+  }
+  return PMX_OK;
+}
+
 int pmx_get_fxt(pmx_ctx *c, double *fx, double *fy, double *fz) {
   // With RESPA off every force term enters fxt with the weight it has in f
   // (force_npol.c:1222-1283, ewald3d_pme.c:568-573, bond.c:272-285), so the
-  // virial-estimator forces equal the forces.
+  // virial-estimator forces are the bead forces; after a call that returned
+  // mode forces they are the bead forces kept before the transform.
+  if (!c || !fx || !fy || !fz) return PMX_ERR_ARG;
+  if (c->last_modes) {
+    double *src[3] = {c->fxt, c->fyt, c->fzt};
+    return get_bead_major(c, src, fx, fy, fz);
+  }
   return pmx_get_forces(c, fx, fy, fz);
+}
+
+static int set_bead_major(pmx_ctx *c, double *const dst[3], const double *x, const double *y, const double *z) {
+  Setup &s = c->s;
+  const size_t NP = (size_t)s.N * s.P;
+  HIPCHK(hipSetDevice(c->device));
+  const double *src[3] = {x, y, z};
+  for (int d = 0; d < 3; d++) {
+    HIPCHK(hipMemcpyAsync(c->stage, src[d], NP * sizeof(double), hipMemcpyHostToDevice, c->st));
+    hipLaunchKernelGGL(k_bead_to_atom, dim3(blocks(NP)), dim3(256), 0, c->st, s.N, s.P, c->stage, dst[d]);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return PMX_OK;
+}
+
+int pmx_set_velocities(pmx_ctx *c, const double *vx, const double *vy, const double *vz) {
+  if (!c || !vx || !vy || !vz) return PMX_ERR_ARG;
+  double *dst[3] = {c->vx, c->vy, c->vz};
+  return set_bead_major(c, dst, vx, vy, vz);
+}
+
+int pmx_get_velocities(pmx_ctx *c, double *vx, double *vy, double *vz) {
+  if (!c || !vx || !vy || !vz) return PMX_ERR_ARG;
+  double *src[3] = {c->vx, c->vy, c->vz};
+  return get_bead_major(c, src, vx, vy, vz);
+}
+
+int pmx_get_positions(pmx_ctx *c, double *x, double *y, double *z) {
+  if (!c || !x || !y || !z) return PMX_ERR_ARG;
+  double *src[3] = {c->x, c->y, c->z};
+  return get_bead_major(c, src, x, y, z);
+}
+
+// Velocity Verlet in the NVE ensemble, int_NVE (md/int_NVE.c:29-125):
+// int_0_to_dt2_nve (int_0_to_dt2.c:35-124: v += f dt/2 / m, x += v dt, ghost
+// positions), the force call, int_dt2_to_dt_nve (int_dt2_to_dt.c:33-100:
+// v += f dt/2 / m) and the kinetic energy of get_tvten.  Classical (P = 1).
+int pmx_md_nve(pmx_ctx *c, int nsteps, double dt, pmx_energies *out, double *kinetic) {
+  if (!c || !out || nsteps < 0) return PMX_ERR_ARG;
+  Setup &s = c->s;
+  if (s.P != 1) {
+    set_err(c, "pmx_md_nve: the device integrator is classical (P = 1); use the force call for beads");
+    return PMX_ERR_NOTIMPL;
+  }
+  if (!c->have_pos) {
+    set_err(c, "pmx_md_nve: no positions set");
+    return PMX_ERR_STATE;
+  }
+  const size_t N = (size_t)s.N;
+  int rc = PMX_OK;
+  if (!c->forces_valid || c->last_modes) {
+    rc = pmx_energy_control(c, 0, out);
+    if (rc) return rc;
+  }
+  const double dt2 = dt / 2.0;
+  for (int step = 0; step < nsteps; step++) {
+    hipLaunchKernelGGL(k_vv_kick, dim3(blocks(N)), dim3(256), 0, c->st, (int)N, dt2, c->mass, c->fx, c->fy, c->fz,
+                       c->vx, c->vy, c->vz);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_vv_drift, dim3(blocks(N)), dim3(256), 0, c->st, (int)N, dt, c->vx, c->vy, c->vz, c->x,
+                       c->y, c->z);
+    HIPCHK(hipGetLastError());
+    if (c->nghost) {
+      hipLaunchKernelGGL(k_ghost_pos, dim3(blocks((size_t)c->nghost)), dim3(256), 0, c->st, c->nghost, 1, c->gatom,
+                         c->gncomp, c->gcomp, c->gcoef, c->x, c->y, c->z);
+      HIPCHK(hipGetLastError());
+    }
+    rc = pmx_energy_control(c, 0, out);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_vv_kick, dim3(blocks(N)), dim3(256), 0, c->st, (int)N, dt2, c->mass, c->fx, c->fy, c->fz,
+                       c->vx, c->vy, c->vz);
+    HIPCHK(hipGetLastError());
+  }
+  if (kinetic) {
+    HIPCHK(hipMemsetAsync(c->ke, 0, sizeof(double), c->st));
+    hipLaunchKernelGGL(k_kinetic, dim3(blocks(N)), dim3(256), 0, c->st, (int)N, c->mass, c->vx, c->vy, c->vz, c->ke);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(kinetic, c->ke, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  return PMX_OK;
 }
 
 int pmx_get_bead_energies(pmx_ctx *c, double *out) {

@@ -56,6 +56,7 @@ struct PairArgs {
 // bins).
 struct PmeArgs {
   int N, P, nchrg, n;  // n: interpolation order
+  int f32;             // grids in FP32 (mixed precision) instead of FP64
   int K[3], T[3], nt[3], H[3], nbr[3];
   int ntile;           // tiles per bead
   size_t gstride;      // doubles per bead grid
@@ -69,8 +70,8 @@ struct PmeArgs {
   int *item_tile, *item_pos;
   int *bin_cnt, *bin_off;  // [P*ntile]
   int *sorted;         // [P*nchrg] item ids in bin order
-  double *hbuf;        // [P*ntile*hvol] halo boxes of spread charge
-  double *grid;        // [P*gstride]
+  void *hbuf;          // [P*ntile*hvol] halo boxes of spread charge (grid type)
+  void *grid;          // [P*gstride] (grid type)
   double *fx, *fy, *fz;
 };
 
@@ -78,13 +79,14 @@ constexpr int FFT_MAX_PASSES = 12;
 constexpr int FFT_MAX_PLANE = 4096;  // complex values per LDS plane (64 KB)
 
 struct FftArgs {
-  int P;
+  int P, f32;
   int K[3];
   int npass[3], radix[3][FFT_MAX_PASSES];  // Stockham passes per axis (radices 2..5)
   size_t gstride, sstride;
-  double *grid;
-  double2 *spec;
-  const double2 *tw[3];  // exp(-2 pi i m/K) per axis
+  void *grid;             // [P][Kc][Kb][Ka] real (grid type)
+  void *spec;             // [P][Kc][Kb][Ka/2+1] complex (grid type)
+  const double2 *tw[3];   // exp(-2 pi i m/K) per axis
+  const float2 *twf[3];   // the same in FP32
   const double *infl;    // [Kc][Kb][Ka/2+1] preg*bweight on the sphere list
   double falp2;
   Cell cell;
@@ -166,12 +168,8 @@ __global__ void k_list_build(ListArgs a);
 __global__ void k_pme_bin(PmeArgs a);
 __global__ void k_pme_scan(PmeArgs a);
 __global__ void k_pme_scatter(PmeArgs a);
-__global__ void k_pme_fold(PmeArgs a);
 __global__ void k_fill_int(int *p, int v, size_t n);
 __global__ void k_reduce_partials(RedArgs a);
-__global__ void k_fft_fwd_ab(FftArgs a);
-__global__ void k_fft_c_conv(FftArgs a);
-__global__ void k_fft_inv_ba(FftArgs a);
 __global__ void k_ecor(IntraArgs a);
 __global__ void k_bond(IntraArgs a);
 __global__ void k_bend(IntraArgs a);
@@ -179,6 +177,12 @@ __global__ void k_pimd_finish(FinishArgs a);
 __global__ void k_ghost_force(int ng, int P, const int *gatom, const int *gncomp, const int *gcomp,
                               const double *gcoef, double *fx, double *fy, double *fz);
 __global__ void k_mode_forces(ModeArgs a);
+__global__ void k_vv_kick(int n, double dt2, const double *mass, const double *fx, const double *fy,
+                          const double *fz, double *vx, double *vy, double *vz);
+__global__ void k_vv_drift(int n, double dt, const double *vx, const double *vy, const double *vz, double *x,
+                           double *y, double *z);
+__global__ void k_kinetic(int n, const double *mass, const double *vx, const double *vy, const double *vz,
+                          double *ke);
 
 // mode: 0 FP64 tables, 1 FP64 direct, 2 FP32 direct with FP64 accumulation;
 // full: the list holds both directions and each wave stores its sites'
@@ -189,6 +193,7 @@ int pair_grid_x(int N);
 size_t pme_lds_bytes(const PmeArgs &a, bool gather);
 hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st);
 hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st);
+hipError_t launch_pme_fft(const FftArgs &f, hipStream_t st);
 // block size for grid-stride kernels whose threads keep one bead: the
 // largest multiple of P not above 256
 inline int bead_block(int P) { return 256 - (256 % P); }

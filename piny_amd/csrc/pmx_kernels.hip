@@ -752,6 +752,41 @@ __global__ void k_mode_forces(ModeArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// velocity Verlet pieces (int_0_to_dt2.c:83-104, int_dt2_to_dt.c:76-80):
+// v += f dt/2 / m and x += v dt, in the reference's order of operations
+__global__ void k_vv_kick(int n, double dt2, const double *__restrict__ mass, const double *__restrict__ fx,
+                          const double *__restrict__ fy, const double *__restrict__ fz, double *vx, double *vy,
+                          double *vz) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double m = mass[i];
+  vx[i] += fx[i] * dt2 / m;
+  vy[i] += fy[i] * dt2 / m;
+  vz[i] += fz[i] * dt2 / m;
+}
+
+__global__ void k_vv_drift(int n, double dt, const double *__restrict__ vx, const double *__restrict__ vy,
+                           const double *__restrict__ vz, double *x, double *y, double *z) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] += vx[i] * dt;
+  y[i] += vy[i] * dt;
+  z[i] += vz[i] * dt;
+}
+
+// kinetic energy 1/2 sum m v^2 (get_tvten), one atomic per wavefront
+__global__ void k_kinetic(int n, const double *__restrict__ mass, const double *__restrict__ vx,
+                          const double *__restrict__ vy, const double *__restrict__ vz, double *ke) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0;
+  if (i < n) e = 0.5 * mass[i] * (vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+  for (int off = 32; off >= 1; off >>= 1) e += __shfl_xor(e, off, 64);
+  if ((threadIdx.x & 63) == 0) atomic_add(ke, e);
+}
+
+// ---------------------------------------------------------------------------
 // launch wrappers
 
 int pair_grid_x(int N) {

@@ -24,20 +24,35 @@ namespace {
 
 constexpr int PME_BLOCK = 128;  // lanes per (tile, bead) workgroup: about one per charge
 
+// Complex type of a grid scalar T (double2 for FP64, float2 for the mixed
+// precision's FP32 grids).
+template <typename T>
+struct Cx;
+template <>
+struct Cx<double> {
+  using C = double2;
+  __device__ static C make(double x, double y) { return make_double2(x, y); }
+};
+template <>
+struct Cx<float> {
+  using C = float2;
+  __device__ static C make(float x, float y) { return make_float2(x, y); }
+};
+
 // B-spline weights M_n(frac + j - 1), j = 1..NI, by the recursion of
 // ewald3d_pme.c:296-337; dmn (order NI-1 differences, :516-530) if DERIV.
-template <int NI, bool DERIV>
-__device__ __forceinline__ void bspline(double frac, double (&mn)[NI + 1], double (&dmn)[NI + 1]) {
-  double u[NI + 1];
+template <int NI, bool DERIV, typename T>
+__device__ __forceinline__ void bspline(T frac, T (&mn)[NI + 1], T (&dmn)[NI + 1]) {
+  T u[NI + 1];
 #pragma unroll
-  for (int j = 1; j <= NI; j++) u[j] = frac + (double)(j - 1);
-  mn[1] = 1.0 - fabs(u[1] - 1.0);
-  mn[2] = 1.0 - fabs(u[2] - 1.0);
+  for (int j = 1; j <= NI; j++) u[j] = frac + (T)(j - 1);
+  mn[1] = (T)1 - fabs(u[1] - (T)1);
+  mn[2] = (T)1 - fabs(u[2] - (T)1);
 #pragma unroll
-  for (int j = 3; j <= NI; j++) mn[j] = 0.0;
+  for (int j = 3; j <= NI; j++) mn[j] = (T)0;
 #pragma unroll
   for (int o = 3; o <= NI; o++) {
-    const double rn = (double)o, rn1 = 1.0 / ((double)(o - 1));
+    const T rn = (T)o, rn1 = (T)1 / ((T)(o - 1));
 #pragma unroll
     for (int j = o; j >= 2; j--) mn[j] = (u[j] * mn[j] + (rn - u[j]) * mn[j - 1]) * rn1;
     mn[1] = u[1] * mn[1] * rn1;
@@ -151,13 +166,13 @@ __global__ void __launch_bounds__(256) k_pme_scatter(PmeArgs a) {
 
 // Weights of one item along the three axes: w[d][m] is the weight of support
 // point s_d + m (M_n index j = n - m); l[d] is s_d relative to the tile origin.
-template <int NI, bool DERIV>
-__device__ __forceinline__ void item_weights(const PmeArgs &a, int it, const int (&org)[3], double (&w)[3][NI],
-                                             double (&dw)[3][NI], int (&l)[3]) {
+template <int NI, bool DERIV, typename T>
+__device__ __forceinline__ void item_weights(const PmeArgs &a, int it, const int (&org)[3], T (&w)[3][NI],
+                                             T (&dw)[3][NI], int (&l)[3]) {
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    double mn[NI + 1], dmn[NI + 1];
-    bspline<NI, DERIV>(a.item_f[3 * (size_t)it + d], mn, dmn);
+    T mn[NI + 1], dmn[NI + 1];
+    bspline<NI, DERIV, T>((T)a.item_f[3 * (size_t)it + d], mn, dmn);
     l[d] = a.item_s[3 * (size_t)it + d] - org[d];
 #pragma unroll
     for (int m = 0; m < NI; m++) {
@@ -169,12 +184,13 @@ __device__ __forceinline__ void item_weights(const PmeArgs &a, int it, const int
 
 // one workgroup per (tile, bead), one lane per charge of the bin; the
 // charge's n^3 products go into the LDS halo box with LDS atomics
-template <int NI>
+template <int NI, typename T>
 __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
-  extern __shared__ double box[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *box = (T *)lds_raw;
   const int hvol = (int)a.hvol;
   const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  for (int k = tid; k < hvol; k += blockDim.x) box[k] = 0.0;
+  for (int k = tid; k < hvol; k += blockDim.x) box[k] = (T)0;
   const size_t bin = (size_t)b * a.ntile + tl;
   const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
   int org[3];
@@ -183,24 +199,24 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
   __syncthreads();
   for (int c = tid; c < cnt; c += blockDim.x) {
     const int it = a.sorted[lo + c];
-    const double q = a.q[a.ichrg[it % a.nchrg]];
-    double w[3][NI], dw[3][NI];
+    const T q = (T)a.q[a.ichrg[it % a.nchrg]];
+    T w[3][NI], dw[3][NI];
     int l[3];
-    item_weights<NI, false>(a, it, org, w, dw, l);
+    item_weights<NI, false, T>(a, it, org, w, dw, l);
 #pragma unroll
     for (int m = 0; m < NI; m++) w[0][m] *= q;
     for (int mc = 0; mc < NI; mc++) {
 #pragma unroll
       for (int mb = 0; mb < NI; mb++) {
-        const double wbc = w[1][mb] * w[2][mc];
-        double *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
+        const T wbc = w[1][mb] * w[2][mc];
+        T *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
 #pragma unroll
         for (int ma = 0; ma < NI; ma++) atomicAdd(row + ma, w[0][ma] * wbc);
       }
     }
   }
   __syncthreads();
-  double *dst = a.hbuf + bin * a.hvol;
+  T *dst = (T *)a.hbuf + bin * a.hvol;
   for (int k = tid; k < hvol; k += blockDim.x) dst[k] = box[k];
 }
 
@@ -209,17 +225,18 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
 // axis, periodically, for o < nbr), which holds that point when
 // l + o*T < T + n - 1.  Tiles divide the grid, so the offset is exact also
 // across the periodic boundary.
+template <typename T>
 __global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
   const int tl = blockIdx.x, b = blockIdx.y;
   const int tcoord[3] = {tl % a.nt[0], (tl / a.nt[0]) % a.nt[1], tl / (a.nt[0] * a.nt[1])};
   const int T0 = a.T[0], T1 = a.T[1], T01 = T0 * T1, vol = T01 * a.T[2];
   const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
-  const double *hb = a.hbuf + (size_t)b * a.ntile * a.hvol;
-  double *g = a.grid + (size_t)b * a.gstride;
+  const T *hb = (const T *)a.hbuf + (size_t)b * a.ntile * a.hvol;
+  T *g = (T *)a.grid + (size_t)b * a.gstride;
   const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
   for (int k = threadIdx.x; k < vol; k += blockDim.x) {
     const int l[3] = {k % T0, (k / T0) % T1, k / T01};
-    double s = 0.0;
+    T s = (T)0;
     for (int oc = 0; oc < a.nbr[2]; oc++) {
       const int lc = l[2] + oc * a.T[2];
       if (lc >= a.H[2]) break;
@@ -245,9 +262,10 @@ __global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
 
 // one workgroup per (tile, bead): the potential on the halo box into LDS,
 // then one lane per charge (ewald3d_pme.c:534-560)
-template <int NI>
+template <int NI, typename T>
 __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
-  extern __shared__ double box[];
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *box = (T *)lds_raw;
   const int hvol = (int)a.hvol;
   const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
   const size_t bin = (size_t)b * a.ntile + tl;
@@ -257,14 +275,14 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
   tile_origin(a, tl, org[0], org[1], org[2]);
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2];
   const int H0 = a.H[0], H1 = a.H[1], H01 = H0 * H1;
-  const double *g = a.grid + (size_t)b * a.gstride;
+  const T *g = (const T *)a.grid + (size_t)b * a.gstride;
   // batches of 8 loads in flight per lane
   for (int k0 = tid; k0 < hvol; k0 += 8 * blockDim.x) {
-    double v[8];
+    T v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int k = k0 + u * blockDim.x;
-      v[u] = 0.0;
+      v[u] = (T)0;
       if (k < hvol) {
         const int la = k % H0, lb = (k / H0) % H1, lc = k / H01;
         int ga = org[0] + la, gb = org[1] + lb, gc = org[2] + lc;
@@ -286,24 +304,25 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
     const int it = a.sorted[lo + c];
     const int ic = it % a.nchrg, bead = it / a.nchrg;
     const int i = a.ichrg[ic];
-    double w[3][NI], dw[3][NI];
+    T w[3][NI], dw[3][NI];
     int l[3];
-    item_weights<NI, true>(a, it, org, w, dw, l);
+    item_weights<NI, true, T>(a, it, org, w, dw, l);
+    // row sums in T, accumulated over rows in FP64
     double da = 0.0, db = 0.0, dc = 0.0;
     for (int mc = 0; mc < NI; mc++) {
 #pragma unroll
       for (int mb = 0; mb < NI; mb++) {
-        const double *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
-        double r0 = 0.0, r1 = 0.0;
+        const T *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
+        T r0 = (T)0, r1 = (T)0;
 #pragma unroll
         for (int ma = 0; ma < NI; ma++) {
-          const double phi = row[ma];
+          const T phi = row[ma];
           r0 += dw[0][ma] * phi;
           r1 += w[0][ma] * phi;
         }
-        da += r0 * (w[1][mb] * w[2][mc]);
-        db += r1 * (dw[1][mb] * w[2][mc]);
-        dc += r1 * (w[1][mb] * dw[2][mc]);
+        da += (double)(r0 * (w[1][mb] * w[2][mc]));
+        db += (double)(r1 * (dw[1][mb] * w[2][mc]));
+        dc += (double)(r1 * (w[1][mb] * dw[2][mc]));
       }
     }
     const double q = a.q[i];
@@ -319,17 +338,33 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
 
 size_t pme_lds_bytes(const PmeArgs &a, bool gather) {
   (void)gather;
-  return a.hvol * sizeof(double);
+  return a.hvol * (a.f32 ? sizeof(float) : sizeof(double));
 }
 
 #define PMX_PME_CASES(X) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12)
 
-hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
+template <typename T>
+hipError_t launch_spread_t(const PmeArgs &a, hipStream_t st) {
   const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
   const size_t lds = pme_lds_bytes(a, false);
   switch (a.n) {
 #define X(NI) \
-  case NI: hipLaunchKernelGGL(k_pme_spread<NI>, grid, block, lds, st, a); break;
+  case NI: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_spread<NI, T>), grid, block, lds, st, a); break;
+    PMX_PME_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_fold<T>), dim3(a.ntile, a.P), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_gather_t(const PmeArgs &a, hipStream_t st) {
+  const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
+  const size_t lds = pme_lds_bytes(a, true);
+  switch (a.n) {
+#define X(NI) \
+  case NI: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_gather<NI, T>), grid, block, lds, st, a); break;
     PMX_PME_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -337,17 +372,13 @@ hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// spread into halo boxes, then the fold into the grid
+hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
+  return a.f32 ? launch_spread_t<float>(a, st) : launch_spread_t<double>(a, st);
+}
+
 hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {
-  const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
-  const size_t lds = pme_lds_bytes(a, true);
-  switch (a.n) {
-#define X(NI) \
-  case NI: hipLaunchKernelGGL(k_pme_gather<NI>, grid, block, lds, st, a); break;
-    PMX_PME_CASES(X)
-#undef X
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return a.f32 ? launch_gather_t<float>(a, st) : launch_gather_t<double>(a, st);
 }
 
 // --- FFT --------------------------------------------------------------------------
@@ -360,23 +391,24 @@ hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {
 // outputs in place, so no permutation pass is needed.  Lanes run along the
 // contiguous direction (lines when stride > 1), which keeps LDS accesses
 // free of bank conflicts.  nlines * K <= FFT_MAX_PLANE and blockDim.x == 256.
-template <int R>
-__device__ __forceinline__ void dft_small(double2 (&v)[R], int sign) {
+template <int R, typename T>
+__device__ __forceinline__ void dft_small(typename Cx<T>::C (&v)[R], int sign) {
+  using C = typename Cx<T>::C;
   if constexpr (R == 2) {
-    const double2 a = v[0], b = v[1];
-    v[0] = make_double2(a.x + b.x, a.y + b.y);
-    v[1] = make_double2(a.x - b.x, a.y - b.y);
+    const C a = v[0], b = v[1];
+    v[0] = Cx<T>::make(a.x + b.x, a.y + b.y);
+    v[1] = Cx<T>::make(a.x - b.x, a.y - b.y);
   } else if constexpr (R == 4) {
-    const double2 a0 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
-    const double2 a1 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
-    const double2 b0 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
-    const double2 b1 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+    const C a0 = Cx<T>::make(v[0].x + v[2].x, v[0].y + v[2].y);
+    const C a1 = Cx<T>::make(v[0].x - v[2].x, v[0].y - v[2].y);
+    const C b0 = Cx<T>::make(v[1].x + v[3].x, v[1].y + v[3].y);
+    const C b1 = Cx<T>::make(v[1].x - v[3].x, v[1].y - v[3].y);
     // b1 * (sign * i)
-    const double2 c1 = sign < 0 ? make_double2(b1.y, -b1.x) : make_double2(-b1.y, b1.x);
-    v[0] = make_double2(a0.x + b0.x, a0.y + b0.y);
-    v[2] = make_double2(a0.x - b0.x, a0.y - b0.y);
-    v[1] = make_double2(a1.x + c1.x, a1.y + c1.y);
-    v[3] = make_double2(a1.x - c1.x, a1.y - c1.y);
+    const C c1 = sign < 0 ? Cx<T>::make(b1.y, -b1.x) : Cx<T>::make(-b1.y, b1.x);
+    v[0] = Cx<T>::make(a0.x + b0.x, a0.y + b0.y);
+    v[2] = Cx<T>::make(a0.x - b0.x, a0.y - b0.y);
+    v[1] = Cx<T>::make(a1.x + c1.x, a1.y + c1.y);
+    v[3] = Cx<T>::make(a1.x - c1.x, a1.y - c1.y);
   } else {
     // direct DFT: out[k] = sum_m v[m] exp(sign 2 pi i k m / R)
     constexpr double c3[3] = {1.0, -0.5, -0.5};
@@ -385,40 +417,41 @@ __device__ __forceinline__ void dft_small(double2 (&v)[R], int sign) {
                               0.30901699437494742410};
     constexpr double s5[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917, -0.58778525229247312917,
                               -0.95105651629515357212};
-    double2 out[R];
+    C out[R];
 #pragma unroll
     for (int k = 0; k < R; k++) {
-      double re = 0.0, im = 0.0;
+      T re = (T)0, im = (T)0;
 #pragma unroll
       for (int m = 0; m < R; m++) {
         const int km = (k * m) % R;
-        const double cr = (R == 3) ? c3[km] : c5[km];
-        const double ci = ((R == 3) ? s3[km] : s5[km]) * (double)sign;
+        const T cr = (T)((R == 3) ? c3[km] : c5[km]);
+        const T ci = (T)(((R == 3) ? s3[km] : s5[km]) * (double)sign);
         re += v[m].x * cr - v[m].y * ci;
         im += v[m].x * ci + v[m].y * cr;
       }
-      out[k] = make_double2(re, im);
+      out[k] = Cx<T>::make(re, im);
     }
 #pragma unroll
     for (int k = 0; k < R; k++) v[k] = out[k];
   }
 }
 
-template <int R>
-__device__ void stockham_pass(double2 *buf, int K, int nlines, int stride, int dist, int Ns, int sign,
-                              const double2 *__restrict__ tw) {
+template <int R, typename T>
+__device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int stride, int dist, int Ns, int sign,
+                              const typename Cx<T>::C *__restrict__ tw) {
+  using C = typename Cx<T>::C;
   constexpr int MAXB = (FFT_MAX_PLANE / R + 255) / 256;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int KR = K / R, nb = nlines * KR;
   const bool lines_fast = stride != 1;
-  double2 v[MAXB][R];
+  C v[MAXB][R];
 #pragma unroll
   for (int q = 0; q < MAXB; q++) {
     const int bi = tid + q * nt;
     if (bi < nb) {
       const int l = lines_fast ? bi % nlines : bi / KR;
       const int j = lines_fast ? bi / nlines : bi % KR;
-      const double2 *p = buf + (size_t)l * dist;
+      const C *p = buf + (size_t)l * dist;
 #pragma unroll
       for (int k = 0; k < R; k++) v[q][k] = p[(j + k * KR) * stride];
     }
@@ -434,13 +467,13 @@ __device__ void stockham_pass(double2 *buf, int K, int nlines, int stride, int d
       const int jm = j % Ns;
 #pragma unroll
       for (int k = 1; k < R; k++) {  // twiddle exp(sign 2 pi i jm k / (Ns R))
-        const double2 t = tw[jm * k * tstep];  // (cos, -sin) of 2 pi m / K
-        const double wr = t.x, wi = sign < 0 ? t.y : -t.y;
-        const double2 a = v[q][k];
-        v[q][k] = make_double2(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+        const C t = tw[jm * k * tstep];  // (cos, -sin) of 2 pi m / K
+        const T wr = t.x, wi = sign < 0 ? t.y : -t.y;
+        const C a = v[q][k];
+        v[q][k] = Cx<T>::make(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
       }
-      dft_small<R>(v[q], sign);
-      double2 *p = buf + (size_t)l * dist;
+      dft_small<R, T>(v[q], sign);
+      C *p = buf + (size_t)l * dist;
       const int base = (j / Ns) * Ns * R + jm;
 #pragma unroll
       for (int k = 0; k < R; k++) p[(base + k * Ns) * stride] = v[q][k];
@@ -449,45 +482,57 @@ __device__ void stockham_pass(double2 *buf, int K, int nlines, int stride, int d
   __syncthreads();
 }
 
-__device__ void lds_fft(double2 *buf, int K, int nlines, int stride, int dist, int sign, const double2 *tw,
-                        int npass, const int *radix) {
+template <typename T>
+__device__ void lds_fft(typename Cx<T>::C *buf, int K, int nlines, int stride, int dist, int sign,
+                        const typename Cx<T>::C *tw, int npass, const int *radix) {
   int Ns = 1;
   for (int s = 0; s < npass; s++) {
     switch (radix[s]) {
-      case 2: stockham_pass<2>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
-      case 3: stockham_pass<3>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
-      case 4: stockham_pass<4>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
-      default: stockham_pass<5>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      case 2: stockham_pass<2, T>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      case 3: stockham_pass<3, T>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      case 4: stockham_pass<4, T>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      default: stockham_pass<5, T>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
     }
     Ns *= radix[s];
   }
 }
 
+template <typename T>
+__device__ __forceinline__ const typename Cx<T>::C *twiddles(const FftArgs &a, int d) {
+  if constexpr (sizeof(T) == 8)
+    return a.tw[d];
+  else
+    return a.twf[d];
+}
+
 // forward transform of one c-plane of one bead: real along a, then b.
 // LDS holds the plane as Kb lines of Ka complex values; only a' = 0..Ka/2
 // is kept (Hermitian half).  spec layout: [bead][c][b][a'] complex.
+template <typename T>
 __global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
-  extern __shared__ double2 sm[];
+  using C = typename Cx<T>::C;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *sm = (C *)lds_raw;
   const int c = blockIdx.x, b = blockIdx.y;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
-  const double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
+  const T *g = (const T *)a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
   for (int w0 = threadIdx.x; w0 < Ka * Kb; w0 += 8 * blockDim.x) {  // 8 loads in flight per lane
-    double v[8];
+    T v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int w = w0 + u * blockDim.x;
-      v[u] = w < Ka * Kb ? g[w] : 0.0;
+      v[u] = w < Ka * Kb ? g[w] : (T)0;
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int w = w0 + u * blockDim.x;
-      if (w < Ka * Kb) sm[w] = make_double2(v[u], 0.0);
+      if (w < Ka * Kb) sm[w] = Cx<T>::make(v[u], (T)0);
     }
   }
   __syncthreads();
-  lds_fft(sm, Ka, Kb, 1, Ka, -1, a.tw[0], a.npass[0], a.radix[0]);
-  lds_fft(sm, Kb, Ha, Ka, 1, -1, a.tw[1], a.npass[1], a.radix[1]);
-  double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
+  lds_fft<T>(sm, Ka, Kb, 1, Ka, -1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
+  lds_fft<T>(sm, Kb, Ha, Ka, 1, -1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
+  C *sp = (C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
   for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
     const int bb = w / Ha, aa = w % Ha;
     sp[w] = sm[bb * Ka + aa];
@@ -496,15 +541,18 @@ __global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
 
 // along c for one (bead, b) row: forward FFT, influence function over the
 // half-space sphere list (ewald3d_pme.c:367-430): energy, virial, and the
-// product with preg*bweight, then the inverse FFT.  Energy and virial go to
-// one partial row per (b row, bead).
+// product with preg*bweight, then the inverse FFT.  Energy and virial are
+// summed in FP64 into one partial row per (b row, bead).
+template <typename T>
 __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
-  extern __shared__ double2 sm[];
+  using C = typename Cx<T>::C;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *sm = (C *)lds_raw;
   const int bb = blockIdx.x, b = blockIdx.y;
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
-  double2 *sp = a.spec + (size_t)b * a.sstride;
+  C *sp = (C *)a.spec + (size_t)b * a.sstride;
   for (int w0 = threadIdx.x; w0 < Kc * Ha; w0 += 8 * blockDim.x) {
-    double2 v[8];
+    C v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int w = w0 + u * blockDim.x;
@@ -517,7 +565,7 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft(sm, Kc, Ha, Ha, 1, -1, a.tw[2], a.npass[2], a.radix[2]);
+  lds_fft<T>(sm, Kc, Ha, Ha, 1, -1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
   const double *hi = a.cell.hi;
   const double tpi = 2.0 * M_PI, falp2 = a.falp2;
   const int kb = bb <= Kb / 2 ? bb : bb - Kb;
@@ -525,7 +573,7 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
     const int cc = w / Ha, aa = w % Ha;
     const double W = a.infl[((size_t)cc * Kb + bb) * Ha + aa];
-    double2 s = sm[w];
+    const C s = sm[w];
     if (W != 0.0) {
       const int kc = cc <= Kc / 2 ? cc : cc - Kc;
       const double aka = tpi * (double)aa, akb = tpi * (double)kb, akc = tpi * (double)kc;
@@ -534,7 +582,8 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
       const double zk = (aka * hi[6] + akb * hi[7] + akc * hi[8]);
       const double g2 = xk * xk + yk * yk + zk * zk;
       const double wsum = (aa == 0) ? 0.5 : 1.0;  // a' = 0 holds both +k and -k
-      const double sm2 = (s.x * s.x + s.y * s.y) * wsum;
+      const double sx = (double)s.x, sy = (double)s.y;
+      const double sm2 = (sx * sx + sy * sy) * wsum;
       e += W * sm2;
       const double prep = -2.0 * W * ((g2 / falp2) + 1.0) / g2 * sm2;
       v11 += prep * xk * xk;
@@ -544,10 +593,10 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
       v13 += prep * xk * zk;
       v23 += prep * yk * zk;
     }
-    sm[w] = make_double2(s.x * W, s.y * W);
+    sm[w] = Cx<T>::make((T)(s.x * W), (T)(s.y * W));
   }
   __syncthreads();
-  lds_fft(sm, Kc, Ha, Ha, 1, +1, a.tw[2], a.npass[2], a.radix[2]);
+  lds_fft<T>(sm, Kc, Ha, Ha, 1, +1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
     const int cc = w / Ha, aa = w % Ha;
     sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
@@ -559,18 +608,21 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
 }
 
 // inverse transform of one c-plane: along b, then complex-to-real along a.
+template <typename T>
 __global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
-  extern __shared__ double2 sm[];
+  using C = typename Cx<T>::C;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *sm = (C *)lds_raw;
   const int c = blockIdx.x, b = blockIdx.y;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
-  const double2 *sp = a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
+  const C *sp = (const C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
   for (int w0 = threadIdx.x; w0 < Kb * Ka; w0 += 8 * blockDim.x) {
-    double2 v[8];
+    C v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int w = w0 + u * blockDim.x;
       const int bb = w / Ka, aa = w % Ka;
-      v[u] = (w < Kb * Ka && aa < Ha) ? sp[bb * Ha + aa] : make_double2(0.0, 0.0);
+      v[u] = (w < Kb * Ka && aa < Ha) ? sp[bb * Ha + aa] : Cx<T>::make((T)0, (T)0);
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -579,19 +631,37 @@ __global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft(sm, Kb, Ha, Ka, 1, +1, a.tw[1], a.npass[1], a.radix[1]);
+  lds_fft<T>(sm, Kb, Ha, Ka, 1, +1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
   // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
   for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
     const int bb = w / Ka, aa = w % Ka;
     if (aa >= Ha) {
-      const double2 v = sm[bb * Ka + (Ka - aa)];
-      sm[w] = make_double2(v.x, -v.y);
+      const C v = sm[bb * Ka + (Ka - aa)];
+      sm[w] = Cx<T>::make(v.x, -v.y);
     }
   }
   __syncthreads();
-  lds_fft(sm, Ka, Kb, 1, Ka, +1, a.tw[0], a.npass[0], a.radix[0]);
-  double *g = a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
+  lds_fft<T>(sm, Ka, Kb, 1, Ka, +1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
+  T *g = (T *)a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
   for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) g[w] = sm[w].x;
+}
+
+template <typename T>
+hipError_t launch_fft_t(const FftArgs &f, hipStream_t st) {
+  using C = typename Cx<T>::C;
+  const int Ka = f.K[0], Kb = f.K[1], Kc = f.K[2], Ha = Ka / 2 + 1;
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_fwd_ab<T>), dim3(Kc, f.P), dim3(256), (size_t)Ka * Kb * sizeof(C), st,
+                     f);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_c_conv<T>), dim3(Kb, f.P), dim3(256), (size_t)Kc * Ha * sizeof(C), st,
+                     f);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_inv_ba<T>), dim3(Kc, f.P), dim3(256), (size_t)Ka * Kb * sizeof(C), st,
+                     f);
+  return hipGetLastError();
+}
+
+// forward 3-D FFT, convolution with energy and virial, inverse 3-D FFT
+hipError_t launch_pme_fft(const FftArgs &f, hipStream_t st) {
+  return f.f32 ? launch_fft_t<float>(f, st) : launch_fft_t<double>(f, st);
 }
 
 }  // namespace pmx

@@ -57,6 +57,10 @@ def lib():
     L.pmx_get_setup_scalars.argtypes = [C.c_void_p, dp]
     L.pmx_get_timings.argtypes = [C.c_void_p, dp]
     L.pmx_get_pair_count.argtypes = [C.c_void_p, dp]
+    L.pmx_set_velocities.argtypes = [C.c_void_p, dp, dp, dp]
+    L.pmx_get_velocities.argtypes = [C.c_void_p, dp, dp, dp]
+    L.pmx_get_positions.argtypes = [C.c_void_p, dp, dp, dp]
+    L.pmx_md_nve.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(abi.PmxEnergies), dp]
     L.pmx_bond_coef_harm.argtypes = [C.c_double, dp]
     L.pmx_bond_coef_morse.argtypes = [C.c_double, C.c_double, dp]
     L.pmx_bend_coef_harm.argtypes = [C.c_double, C.c_double, dp, dp]
@@ -71,7 +75,8 @@ EXPORTED = ("pmx_abi_version", "pmx_device_count", "pmx_create", "pmx_destroy", 
             "pmx_set_positions", "pmx_set_positions_device", "pmx_set_xmod", "pmx_energy_control",
             "pmx_get_forces", "pmx_get_forces_device", "pmx_get_fxt", "pmx_get_bead_energies",
             "pmx_get_verlet_list", "pmx_get_nktot", "pmx_get_kvectors", "pmx_get_spline",
-            "pmx_get_setup_scalars", "pmx_get_timings", "pmx_get_pair_count", "pmx_bond_coef_harm", "pmx_bond_coef_morse",
+            "pmx_get_setup_scalars", "pmx_get_timings", "pmx_get_pair_count", "pmx_set_velocities",
+            "pmx_get_velocities", "pmx_get_positions", "pmx_md_nve", "pmx_bond_coef_harm", "pmx_bond_coef_morse",
             "pmx_bend_coef_harm")
 
 
@@ -141,6 +146,33 @@ class Engine:
         self._chk(lib().pmx_get_forces_device(self.h, C.byref(px), C.byref(py), C.byref(pz)),
                   "pmx_get_forces_device")
         return [C.cast(p, C.c_void_p).value for p in (px, py, pz)]
+
+    def _bead_major(self, fn, what):
+        n = self.N * self.P
+        a, b, c = np.zeros(n), np.zeros(n), np.zeros(n)
+        self._chk(fn(self.h, abi.dptr(a), abi.dptr(b), abi.dptr(c)), what)
+        return np.stack([a, b, c], 1).reshape(self.P, self.N, 3)
+
+    def fxt(self):
+        return self._bead_major(lib().pmx_get_fxt, "pmx_get_fxt")
+
+    def positions(self):
+        return self._bead_major(lib().pmx_get_positions, "pmx_get_positions")
+
+    def velocities(self):
+        return self._bead_major(lib().pmx_get_velocities, "pmx_get_velocities")
+
+    def set_velocities(self, vx, vy, vz):
+        self._vel = [np.ascontiguousarray(v, dtype=np.float64) for v in (vx, vy, vz)]
+        self._chk(lib().pmx_set_velocities(self.h, *[abi.dptr(v) for v in self._vel]), "pmx_set_velocities")
+
+    def md_nve(self, nsteps, dt):
+        """nsteps of velocity Verlet (int_NVE); dt in atomic time units.
+        Returns (energies of the last force call, kinetic energy)."""
+        e = abi.PmxEnergies()
+        ke = C.c_double(0.0)
+        self._chk(lib().pmx_md_nve(self.h, int(nsteps), float(dt), C.byref(e), C.byref(ke)), "pmx_md_nve")
+        return e.as_dict(), ke.value
 
     def bead_energies(self):
         out = np.zeros(4 * self.P)

@@ -167,10 +167,57 @@ def run_case(name):
     print(f"{name}: N={N} P={P} nver={nver} vintert={e[0]:.12g} vintrat={e[1]:.12g} -> {out}")
 
 
+MD_CASES = {"cfg1": 100}   # configuration 1: 100 velocity-Verlet steps at 0.25 fs
+
+
+def md_velocities(c, seed):
+    """Seeded Maxwell-Boltzmann velocities at the case temperature (Bohr per
+    atomic time unit), no centre-of-mass motion, zero on ghost sites."""
+    s = c.system
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kt = s.settings.temperature / w.BOLTZ
+    m = np.asarray(s.mass, dtype=np.float64)
+    v = rng.normal(size=(s.natm, 3)) * np.sqrt(kt / m)[:, None]
+    ghost = np.zeros(s.natm, bool)
+    ghost[np.asarray(s.ghost_atom, dtype=int)] = True
+    v[ghost] = 0.0
+    real = ~ghost
+    v[real] -= (m[real, None] * v[real]).sum(0) / m[real].sum()
+    return v
+
+
+def run_md_case(name, nsteps):
+    c = cases.get(name)
+    s = c.system
+    assert s.pi_beads == 1
+    v0 = md_velocities(c, w.SEED_BASE + 100)
+    with tempfile.TemporaryDirectory() as d:
+        write_inputs(d, c)
+        v0.astype(np.float64).tofile(os.path.join(d, "vel.bin"))
+        with open(os.path.join(d, "stdout.txt"), "w") as log:
+            r = subprocess.run([EXE, "water.input", "pos.bin", "out.bin", "md", str(nsteps), "vel.bin"], cwd=d,
+                               stdout=log, stderr=subprocess.STDOUT, timeout=1200)
+        if r.returncode != 0:
+            print(open(os.path.join(d, "stdout.txt")).read()[-3000:])
+            raise RuntimeError(f"md {name}: reference run failed ({r.returncode})")
+        raw = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64)
+    N = int(raw[0])
+    vintert, vintrat, kinet, dt = raw[3:7]
+    xvf = raw[7:7 + 9 * N].reshape(N, 9)
+    out = os.path.join(HERE, f"ref_md_{name}.npz")
+    np.savez_compressed(out, case=name, nsteps=nsteps, dt=dt, v0=v0, vintert=vintert, vintrat=vintrat,
+                        kinet=kinet, x=xvf[:, 0:3], v=xvf[:, 3:6], f=xvf[:, 6:9])
+    print(f"md {name}: {nsteps} steps dt={dt:.6g} au, vintert={vintert:.12g} kinet={kinet:.12g} -> {out}")
+
+
 def main():
-    names = sys.argv[1:] or CASES
+    names = sys.argv[1:] or CASES + ["md"]
     for n in names:
-        run_case(n)
+        if n == "md":
+            for m, k in MD_CASES.items():
+                run_md_case(m, k)
+        else:
+            run_case(n)
 
 
 if __name__ == "__main__":

@@ -201,3 +201,48 @@ def test_device_matches_reference(name, mode):
     etol, ftol, vtol = (ETOL, FTOL, ETOL) if mode == "fp64" else (MIXED_ETOL, MIXED_FTOL, MIXED_VTOL)
     rms = compare(name, e, f, d, ref, etol, ftol, vtol)
     print(f"  force rms rel err vs the reference {rms:.3e}", flush=True)
+
+
+# --- velocity Verlet on the device -------------------------------------------
+
+from tests.test_md_nve import load as load_md  # noqa: E402
+
+
+def test_device_nve_matches_reference():
+    d = load_md()
+    c = cases.get(str(d["case"]))
+    s = c.system
+    print(f"[md {d['case']}] {int(d['nsteps'])} NVE steps", flush=True)
+    g = Engine(s, c.x, c.y, c.z)
+    g.set_positions(c.x, c.y, c.z)
+    v0 = d["v0"]
+    g.set_velocities(v0[:, 0].copy(), v0[:, 1].copy(), v0[:, 2].copy())
+    e, ke = g.md_nve(int(d["nsteps"]), float(d["dt"]))
+    x, v, f = g.positions()[0], g.velocities()[0], g.forces()[0]
+    print(f"  x {np.abs(x - d['x']).max():.2e}  v {np.abs(v - d['v']).max():.2e}  ke {ke} vs {float(d['kinet'])}",
+          flush=True)
+    assert np.max(np.abs(x - d["x"])) < 1e-9
+    assert np.max(np.abs(v - d["v"])) < 1e-9 * np.max(np.abs(d["v"]))
+    assert abs(e["vintert"] - float(d["vintert"])) < 1e-9 * abs(float(d["vintert"]))
+    assert abs(ke - float(d["kinet"])) < 1e-9 * float(d["kinet"])
+
+
+def test_mixed_energy_conservation_within_2x_of_fp64():
+    # total energy vintert + vintrat + kinetic along 1000 steps of 0.25 fs
+    # (configuration 1), in FP64 and in the mixed precision
+    d = load_md()
+    c = cases.get(str(d["case"]))
+    v0 = d["v0"]
+    spread = {}
+    for mode, kw in (("fp64", {}), ("mixed", {"precision": abi.PMX_PREC_MIXED})):
+        g = Engine(c.system, c.x, c.y, c.z, **kw)
+        g.set_positions(c.x, c.y, c.z)
+        g.set_velocities(v0[:, 0].copy(), v0[:, 1].copy(), v0[:, 2].copy())
+        tot = []
+        for _ in range(20):
+            e, ke = g.md_nve(50, float(d["dt"]))
+            tot.append(e["vintert"] + e["vintrat"] + ke)
+        tot = np.array(tot)
+        spread[mode] = np.max(np.abs(tot - tot[0]))
+        print(f"  {mode}: max |E(t) - E(0)| = {spread[mode]:.3e} Ha over 1000 steps", flush=True)
+    assert spread["mixed"] <= 2.0 * spread["fp64"] + 1e-6
