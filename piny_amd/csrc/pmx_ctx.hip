@@ -49,6 +49,9 @@ struct pmx_ctx {
   int pair_mode = 0;       // 0 FP64 tables, 1 FP64 direct, 2 FP32 direct
   bool full_list = true;   // force list with both directions (no atomics)
   int cap = 0, *jfull = nullptr, *jforce = nullptr, *nfull = nullptr, *nforce = nullptr, *overflow = nullptr;
+  // link cells of the list build
+  int *site_cell = nullptr, *site_pos = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_sites = nullptr;
+  bool list_cells = false;  // the last build used link cells
   int *b1 = nullptr, *b2 = nullptr, *btyp = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr,
       *atyp = nullptr, *e1 = nullptr, *e2 = nullptr;
   double *bc = nullptr, *beq = nullptr, *ac = nullptr, *as = nullptr;
@@ -162,6 +165,8 @@ int estimate_cap(const Setup &s, double rskin) {
   cap = ((cap + 63) / 64) * 64;
   return std::min(std::max(cap, 64), ((s.N + 63) / 64) * 64);
 }
+
+constexpr int MAX_LIST_CELLS = 1 << 18;
 
 #define TRY_RC(x)      \
   do {                 \
@@ -336,6 +341,11 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
   }
+  TRY(dalloc(c, &c->site_cell, N));
+  TRY(dalloc(c, &c->site_pos, N));
+  TRY(dalloc(c, &c->cell_sites, N));
+  TRY(dalloc(c, &c->cell_cnt, (size_t)MAX_LIST_CELLS));
+  TRY(dalloc(c, &c->cell_start, (size_t)MAX_LIST_CELLS));
   TRY(dalloc(c, &c->nfull, N));
   TRY(dalloc(c, &c->nforce, N));
   TRY(dalloc(c, &c->overflow, 2));
@@ -461,11 +471,40 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
 #undef TRY
 }
 
+// Link cells for the list build (make_lnk_lst.c / make_lnk_map.c): cells of
+// fractional space at least one list radius wide, at least 3 per axis, so
+// the 27 cells around a site hold every site with an image inside the list
+// radius, each once.  Returns false when the box is too small for that (or
+// the rows would not fit the sort), and the ascending O(N^2) scan is used.
+bool choose_cells(pmx_ctx *c, int nc[3], int *n2) {
+  const char *env = getenv("PMX_LIST");  // tuning override: "brute" or "cells"
+  if (env && strcmp(env, "brute") == 0) return false;
+  double rl = 0.0;
+  for (int t = 0; t < c->s.npair; t++) rl = std::max(rl, sqrt(c->pt.cutskin2[t]));
+  int ncell = 1;
+  for (int d = 0; d < 3; d++) {
+    const double *hi = c->cell.hi;
+    const double w = 1.0 / sqrt(hi[d] * hi[d] + hi[d + 3] * hi[d + 3] + hi[d + 6] * hi[d + 6]);
+    nc[d] = std::min(64, (int)floor(w / rl));
+    if (nc[d] < 3) return false;
+    ncell *= nc[d];
+  }
+  if (ncell > MAX_LIST_CELLS) return false;
+  int p2 = 64;
+  while (p2 < c->cap) p2 <<= 1;
+  if ((size_t)p2 * sizeof(int) > 65536) return false;
+  *n2 = p2;
+  // below this many cells the 27-cell scan saves little over the plain one
+  return ncell >= 64 || (env && strcmp(env, "cells") == 0);
+}
+
 // (Re)build the Verlet list from the bead-1 copy; grows the ELL width when a
 // row overflows.
 int build_list(pmx_ctx *c) {
   Setup &s = c->s;
   for (int attempt = 0; attempt < 8; attempt++) {
+    int nc[3] = {1, 1, 1}, n2 = 64;
+    const bool cells = choose_cells(c, nc, &n2);
     HIPCHK(hipMemsetAsync(c->overflow, 0, 2 * sizeof(int), c->st));
     ListArgs a{};
     a.N = s.N;
@@ -490,12 +529,33 @@ int build_list(pmx_ctx *c) {
     a.nfull = c->nfull;
     a.nforce = c->nforce;
     a.overflow = c->overflow;
+    a.use_cells = cells ? 1 : 0;
+    for (int d = 0; d < 3; d++) a.nc[d] = nc[d];
+    a.site_cell = c->site_cell;
+    a.site_pos = c->site_pos;
+    a.cell_cnt = c->cell_cnt;
+    a.cell_start = c->cell_start;
+    a.cell_sites = c->cell_sites;
+    if (cells) {
+      const int ncell = nc[0] * nc[1] * nc[2];
+      hipLaunchKernelGGL(k_fill_int, dim3(blocks(ncell)), dim3(256), 0, c->st, c->cell_cnt, 0, (size_t)ncell);
+      hipLaunchKernelGGL(k_cell_count, dim3(blocks(s.N)), dim3(256), 0, c->st, a);
+      hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(256), 0, c->st, ncell, c->cell_cnt, c->cell_start);
+      hipLaunchKernelGGL(k_cell_scatter, dim3(blocks(s.N)), dim3(256), 0, c->st, a);
+      HIPCHK(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_list_build, dim3(blocks((size_t)s.N * 64)), dim3(256), 0, c->st, a);
     HIPCHK(hipGetLastError());
     int ov[2];
     HIPCHK(hipMemcpyAsync(ov, c->overflow, sizeof ov, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
     if (ov[0] == 0) {
+      if (cells) {  // ascending rows, as the plain scan writes them
+        hipLaunchKernelGGL(k_sort_rows, dim3(s.N), dim3(256), (size_t)n2 * sizeof(int), c->st, c->cap, n2,
+                           c->nfull, c->nforce, c->jfull, c->jforce);
+        HIPCHK(hipGetLastError());
+      }
+      c->list_cells = cells;
       c->nver = ov[1];
       return PMX_OK;
     }

@@ -10,6 +10,10 @@ namespace pmx {
 struct ListArgs {
   int N, cap, pimd;
   int full;  // 1: jforce holds every interacting j != i (both directions)
+  // link cells (when use_cells): nc[3] cells per axis of fractional space,
+  // sites of cell k at cell_sites[cell_start[k] .. + cell_cnt[k]]
+  int use_cells, nc[3];
+  int *site_cell, *site_pos, *cell_cnt, *cell_start, *cell_sites;
   const double *bx, *by, *bz;  // bead-1 positions [N]
   const double *xm, *ym, *zm;  // xmod [N]
   const int *type;
@@ -165,6 +169,10 @@ __global__ void k_list_check(int N, int P, const double *x, const double *y, con
                              const double *x0, const double *y0, const double *z0, double *b0x, double *b0y,
                              double *b0z, unsigned long long *maxes);
 __global__ void k_list_build(ListArgs a);
+__global__ void k_cell_count(ListArgs a);
+__global__ void k_cell_scan(int ncell, const int *cnt, int *start);
+__global__ void k_cell_scatter(ListArgs a);
+__global__ void k_sort_rows(int cap, int n2, const int *nfull, const int *nforce, int *jfull, int *jforce);
 __global__ void k_pme_bin(PmeArgs a);
 __global__ void k_pme_scan(PmeArgs a);
 __global__ void k_pme_scatter(PmeArgs a);

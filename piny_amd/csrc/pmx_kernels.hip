@@ -210,42 +210,67 @@ __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
   int nfull = 0, nforce = 0;
   int *rowf = a.jfull + (size_t)i * a.cap;
   int *rowi = a.jforce + (size_t)i * a.cap;
+  // candidates: every j (ascending, so rows come out sorted) or the sites of
+  // the 27 link cells around i's cell (rows sorted afterwards by k_sort_rows)
   const int jend = a.full ? a.N : i;
-  for (int j0 = 0; j0 < jend; j0 += 64) {
-    const int j = j0 + lane;
-    bool keep = false, inter = false;
-    int t = 0;
-    if (j < jend && j != i) {
-      bool excl = false;
-      for (int k = 0; k < en; k++) excl |= (a.ex_j[eo + k] == j);
-      if (!excl) {
-        double dx, dy, dz;
-        double xmj = 0.0, ymj = 0.0, zmj = 0.0;
-        if (a.pimd) {
-          xmj = a.xm[j];
-          ymj = a.ym[j];
-          zmj = a.zm[j];
+  const int ncand_cells = a.use_cells ? 27 : 1;
+  int ci[3] = {0, 0, 0};
+  if (a.use_cells) {
+    const int c0 = a.site_cell[i];
+    ci[0] = c0 % a.nc[0];
+    ci[1] = (c0 / a.nc[0]) % a.nc[1];
+    ci[2] = c0 / (a.nc[0] * a.nc[1]);
+  }
+  for (int cc = 0; cc < ncand_cells; cc++) {
+    int base = 0, count = jend;
+    const int *sites = nullptr;
+    if (a.use_cells) {
+      int n3[3];
+      const int off[3] = {cc % 3 - 1, (cc / 3) % 3 - 1, cc / 9 - 1};
+#pragma unroll
+      for (int d = 0; d < 3; d++) n3[d] = (ci[d] + off[d] + a.nc[d]) % a.nc[d];
+      const int cell = (n3[2] * a.nc[1] + n3[1]) * a.nc[0] + n3[0];
+      base = a.cell_start[cell];
+      count = a.cell_cnt[cell];
+      sites = a.cell_sites;
+    }
+    for (int j0 = 0; j0 < count; j0 += 64) {
+      const int jj = j0 + lane;
+      const int j = (jj < count) ? (sites ? sites[base + jj] : jj) : a.N;
+      bool keep = false, inter = false;
+      int t = 0;
+      if (j < a.N && (a.full || j < i) && j != i) {
+        bool excl = false;
+        for (int k = 0; k < en; k++) excl |= (a.ex_j[eo + k] == j);
+        if (!excl) {
+          double dx, dy, dz;
+          double xmj = 0.0, ymj = 0.0, zmj = 0.0;
+          if (a.pimd) {
+            xmj = a.xm[j];
+            ymj = a.ym[j];
+            zmj = a.zm[j];
+          }
+          const double r2 = min_image_sep(a.cell, a.pimd, xi, yi, zi, a.bx[j], a.by[j], a.bz[j], xmi, ymi, zmi, xmj,
+                                          ymj, zmj, dx, dy, dz);
+          t = pair_index(a.pt.ntype, ti, a.type[j]);
+          const bool in = r2 < a.pt.cutskin2[t];
+          keep = in && j < i;
+          inter = in && (a.pt.lj[t] || (qi * a.q[j] != 0.0));
         }
-        const double r2 = min_image_sep(a.cell, a.pimd, xi, yi, zi, a.bx[j], a.by[j], a.bz[j], xmi, ymi, zmi, xmj,
-                                        ymj, zmj, dx, dy, dz);
-        t = pair_index(a.pt.ntype, ti, a.type[j]);
-        const bool in = r2 < a.pt.cutskin2[t];
-        keep = in && j < i;
-        inter = in && (a.pt.lj[t] || (qi * a.q[j] != 0.0));
       }
+      const unsigned long long mk = __ballot(keep), mi = __ballot(inter);
+      const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+      if (keep) {
+        const int p = nfull + __popcll(mk & below);
+        if (p < a.cap) rowf[p] = j;
+      }
+      if (inter) {
+        const int p = nforce + __popcll(mi & below);
+        if (p < a.cap) rowi[p] = j | (t << ENTRY_SHIFT);
+      }
+      nfull += __popcll(mk);
+      nforce += __popcll(mi);
     }
-    const unsigned long long mk = __ballot(keep), mi = __ballot(inter);
-    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    if (keep) {
-      const int p = nfull + __popcll(mk & below);
-      if (p < a.cap) rowf[p] = j;
-    }
-    if (inter) {
-      const int p = nforce + __popcll(mi & below);
-      if (p < a.cap) rowi[p] = j | (t << ENTRY_SHIFT);
-    }
-    nfull += __popcll(mk);
-    nforce += __popcll(mi);
   }
   if (lane == 0) {
     a.nfull[i] = nfull;
@@ -253,6 +278,92 @@ __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
     const int mx = nfull > nforce ? nfull : nforce;
     if (mx > a.cap) atomicMax(a.overflow, mx);
     atomicAdd(a.overflow + 1, nfull);  // pair count of the reference list
+  }
+}
+
+// Link cells (make_lnk_lst.c:37-355 bins atoms into cells of fractional
+// space): the bead-1 position's cell, a counting sort of the sites by cell.
+__global__ void __launch_bounds__(256) k_cell_count(ListArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  const Cell &c = a.cell;
+  const double x = a.bx[i], y = a.by[i], z = a.bz[i];
+  int idx[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double f = x * c.hi[d] + y * c.hi[d + 3] + z * c.hi[d + 6];
+    f -= floor(f);
+    int k = (int)(f * (double)a.nc[d]);
+    idx[d] = k < 0 ? 0 : (k >= a.nc[d] ? a.nc[d] - 1 : k);
+  }
+  const int cell = (idx[2] * a.nc[1] + idx[1]) * a.nc[0] + idx[0];
+  a.site_cell[i] = cell;
+  a.site_pos[i] = atomicAdd(a.cell_cnt + cell, 1);
+}
+
+// exclusive scan of the cell counts (one workgroup)
+__global__ void __launch_bounds__(256) k_cell_scan(int ncell, const int *cnt, int *start) {
+  __shared__ int sums[256];
+  const int tid = threadIdx.x;
+  const int per = (ncell + 255) / 256;
+  const int lo = min(ncell, tid * per), hi = min(ncell, lo + per);
+  int s = 0;
+  for (int t = lo; t < hi; t++) s += cnt[t];
+  sums[tid] = s;
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int k = 0; k < 256; k++) {
+      const int v = sums[k];
+      sums[k] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  int run = sums[tid];
+  for (int t = lo; t < hi; t++) {
+    start[t] = run;
+    run += cnt[t];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_cell_scatter(ListArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N) return;
+  a.cell_sites[a.cell_start[a.site_cell[i]] + a.site_pos[i]] = i;
+}
+
+// Sorts each list row ascending by site index (one workgroup per row,
+// bitonic sort in LDS over n2 >= cap keys): the link-cell build then yields
+// the same rows, in the same order, as the ascending scan.
+__global__ void __launch_bounds__(256) k_sort_rows(int cap, int n2, const int *nfull, const int *nforce, int *jfull,
+                                                   int *jforce) {
+  extern __shared__ int keys[];
+  const int i = blockIdx.x, tid = threadIdx.x;
+  for (int which = 0; which < 2; which++) {
+    int *row = (which == 0 ? jfull : jforce) + (size_t)i * cap;
+    const int cnt = which == 0 ? nfull[i] : nforce[i];
+    for (int k = tid; k < n2; k += blockDim.x) keys[k] = k < cnt ? row[k] : ENTRY_MASK | 0x7f000000;
+    __syncthreads();
+    for (int len = 2; len <= n2; len <<= 1) {
+      for (int half = len >> 1; half > 0; half >>= 1) {
+        for (int k = tid; k < n2; k += blockDim.x) {
+          const int p = k ^ half;
+          if (p > k) {
+            const int a = keys[k], b = keys[p];
+            const bool up = (k & len) == 0;
+            const bool swap = up ? ((a & ENTRY_MASK) > (b & ENTRY_MASK)) : ((a & ENTRY_MASK) < (b & ENTRY_MASK));
+            if (swap) {
+              keys[k] = b;
+              keys[p] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    for (int k = tid; k < cnt; k += blockDim.x) row[k] = keys[k];
+    __syncthreads();
   }
 }
 

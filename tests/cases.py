@@ -44,6 +44,11 @@ CASES = {
     "cfg1_k40": lambda: w.config1(_st(kmax_ewd=14, kmax_pme=17)),
     "tip128_p4_k48": lambda: w.build_config(w.Q_TIP4P_F, 128, w.density_box(128), 4, 17,
                                             _st(cutoff=6.0, alpha=0.55, kmax_ewd=16, kmax_pme=22, n_interp=8)),
+    # boxes with at least 3 link cells per axis
+    "tip512_p4": lambda: w.build_config(w.Q_TIP4P_F, 512, w.density_box(512), 4, 19,
+                                        _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
+    "spc512": lambda: w.build_config(w.SPC_FW, 512, w.density_box(512), 1, 20,
+                                     _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
     # 256 q-TIP4P/F at P = 32, K = 32
     "tip256_p32": lambda: w.build_config(w.Q_TIP4P_F, 256, w.density_box(256), 32, 14,
                                          _st(cutoff=9.0, alpha=0.35, kmax_ewd=14, kmax_pme=14)),

@@ -106,7 +106,22 @@ def test_half_list_with_atomics(case, mode, monkeypatch):
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("case", ["spc64", "tip64_p4", "cfg1", "cfg2_p8"])
+@pytest.mark.parametrize("case", ["spc512", "tip512_p4"])
+def test_link_cell_list(case, monkeypatch):
+    # the link-cell build (forced on; these boxes have 3 cells per axis)
+    # gives the reference's rows in the reference's order
+    monkeypatch.setenv("PMX_LIST", "cells")
+    c, o, g, eo, fo, eg, fg = _run(case)
+    for a, b in zip(g.verlet_list(), o.verlet_list()):
+        assert np.array_equal(a, b)
+    scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
+    for k in abi.ENERGY_KEYS:
+        assert abs(eg[k] - eo[k]) <= ETOL * max(abs(eo[k]), scale), (k, eg[k], eo[k])
+    rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
+    assert rms < FTOL
+
+
+@pytest.mark.parametrize("case", ["spc64", "tip64_p4", "cfg1", "cfg2_p8", "spc512"])
 def test_verlet_list_bit_exact(case):
     c, o, g, eo, fo, eg, fg = _run(case)
     for a, b in zip(g.verlet_list(), o.verlet_list()):
