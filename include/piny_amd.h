@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PMX_ABI_VERSION 2
+#define PMX_ABI_VERSION 3
 
 /* return codes */
 enum {
@@ -264,6 +264,45 @@ int pmx_get_positions(pmx_ctx *ctx, double *x, double *y, double *z);
  * kinetic (if not NULL) 1/2 sum m v^2 at the end (get_tvten). */
 int pmx_md_nve(pmx_ctx *ctx, int nsteps, double dt, pmx_energies *out,
                double *kinetic);
+
+/* New force-field parameters for the context's topology: sys must describe
+ * the same sites, types, bonds, bends and ghosts as at pmx_create, with the
+ * same charged sites and LJ type pairs; its charges q, LJ inter_sig and
+ * inter_eps, bond_c, bond_eq, bend_c and bend_s replace the old ones (the LJ
+ * tables, the LJ tail and the self/background sums are rebuilt, as parse()
+ * would for an input with these values: control_inter_params.c:643-772,
+ * get_clong.c:27-114, ewald3d_self_bgr.c:29-83).  PINY sets these only at
+ * parse time; the entry point lets a caller differentiate the energy with
+ * respect to them (python/piny/piny.pyx:126-198 builds a new instance per
+ * parameter set instead). */
+int pmx_set_force_field(pmx_ctx *ctx, const pmx_system *sys);
+
+/* Derivatives of each bead's potential energy
+ *     E_b = vreal_b + vrecip_b + vecor_b + vbond_b + vbend_b
+ *           + vlong + vself + vbgr
+ * (the energy of bead b as a classical configuration; with PMX_EVAL_PIMD
+ * the force call's vintert + vintrat is the mean of E_b over beads, and
+ * -dE_b/dx_b is P times its bead forces) with respect to the force-field
+ * parameters of pmx_system, at the positions and parameters of the last
+ * pmx_energy_control (which must have included the k-space term).  This is
+ * the backward of the force call; the forces are its position gradient.
+ * Each output may be NULL; sizes per bead, bead-major:
+ *   energy   [P]                 E_b
+ *   dq       [P*natm]            dE_b/dq_i (0 at sites without charge: the
+ *                                set of charged sites is fixed)
+ *   deps     [P*npair]           dE_b/d inter_eps[t] (LJ tail included)
+ *   dsig     [P*npair]           dE_b/d inter_sig[t]
+ *   dbond_c  [P*nbond_type*7]    dE_b/d bond_c[ty*7 + m]
+ *   dbond_eq [P*nbond_type]      dE_b/d bond_eq[ty]
+ *   dbend_c  [P*nbend_type*7]    dE_b/d bend_c[ty*7 + m]
+ *   dbend_s  [P*nbend_type*7]    dE_b/d bend_s[ty*7 + m]
+ * LJ gradients are formed for up to 4 LJ type pairs.  The reference
+ * computes no parameter derivatives; its debug driver checks forces by
+ * finite differences (test_energy.c:489-710). */
+typedef struct pmx_param_grad {
+  double *energy, *dq, *deps, *dsig, *dbond_c, *dbond_eq, *dbend_c, *dbend_s;
+} pmx_param_grad;
+int pmx_get_param_grad(pmx_ctx *ctx, pmx_param_grad *out);
 
 /* Coefficient helpers restating intra_coefs.c (units: Hartree, Bohr). */
 int pmx_bond_coef_harm(double fk, double *c7);

@@ -15,6 +15,8 @@ PMX_PREC_FP64 = 0
 PMX_PREC_MIXED = 1
 PMX_PAIR_TABLE = 0
 PMX_PAIR_DIRECT = 1
+PMX_INTER_LJ = 2
+PMX_INTER_NULL = 4
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -63,6 +65,10 @@ class PmxEnergies(C.Structure):
         d["pvten"] = np.array(self.pvten[:])
         d["pvten_tot"] = np.array(self.pvten_tot[:])
         return d
+
+
+class PmxParamGrad(C.Structure):
+    _fields_ = [(k, _dp) for k in ("energy", "dq", "deps", "dsig", "dbond_c", "dbond_eq", "dbend_c", "dbend_s")]
 
 
 ENERGY_KEYS = ("vintert", "vintrat", "vreal", "vvdw", "vcoul", "vlong", "vrecip",

@@ -83,7 +83,16 @@ struct pmx_ctx {
   int nver = 0;
   PairTypes pt{};
   Cell cell{};
-  int nbond = 0, nbend = 0;
+  int nbond = 0, nbend = 0, nbond_type = 0, nbend_type = 0;
+  // parameter gradients: LJ slot per type pair, ecor partners per site,
+  // buffers (allocated by the first pmx_get_param_grad), and whether the
+  // device state is that of a finished force call at the current positions
+  // and parameters (with the k-space grid)
+  std::vector<int> lj_slot;
+  std::vector<double> bc_h;  // bond coefficients [nbond_type*7]
+  int *ecs_off = nullptr, *ecs_num = nullptr, *ecs_j = nullptr;
+  double *phi = nullptr, *gpart = nullptr, *gacc = nullptr, *gbond = nullptr, *gbend = nullptr;
+  bool grad_ready = false;
 };
 
 namespace {
@@ -256,6 +265,41 @@ int setup_pme_tiles(pmx_ctx *c) {
   return PMX_OK;
 }
 
+// Per type pair constants of the force kernels (PP_* entries).  LJ type
+// pairs get gradient slots in order, up to MAX_LJ_GRAD.
+std::vector<double> pair_constants(pmx_ctx *c) {
+  const Setup &s = c->s;
+  std::vector<double> pp((size_t)s.npair * PP_NUM, 0.0);
+  c->lj_slot.assign(s.npair, -1);
+  int nslot = 0;
+  for (int t = 0; t < s.npair; t++) {
+    double *e = &pp[(size_t)t * PP_NUM];
+    const bool lj = (s.label[t] == PMX_INTER_LJ);
+    e[PP_CUT] = s.cutoff[t];
+    e[PP_RMIN] = s.rmin[t];
+    e[PP_DRI] = s.dri[t];
+    e[PP_DR] = s.dr[t];
+    e[PP_EPS] = s.lj_eps[t];
+    e[PP_SIG6] = s.lj_sig6[t];
+    e[PP_VSLJ] = s.vshift_lj[t];
+    e[PP_VSC] = s.vshift_c[t];
+    e[PP_LJ] = lj ? 1.0 : 0.0;
+    if (lj && c->par.ishift == PMX_SHIFT_SHIFT) {
+      // the shift is the table's value at node nsplin - 2 (spline_fit.c:28-113)
+      const double rs = s.dr[t] * (double)(s.nsplin - 5) + s.rmin[t];
+      const double r2 = rs * rs, s6 = s.lj_sig6[t] / (r2 * r2 * r2);
+      e[PP_VSLJ1] = 4.0 * s6 * (s6 - 1.0);
+      e[PP_DSLJ1] = 4.0 * (12.0 * s6 * s6 - 6.0 * s6);
+    }
+    e[PP_GSLOT] = -1.0;
+    if (lj && nslot < MAX_LJ_GRAD) {
+      c->lj_slot[t] = nslot;
+      e[PP_GSLOT] = (double)nslot++;
+    }
+  }
+  return pp;
+}
+
 int build_device(pmx_ctx *c, const pmx_system *sys) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
@@ -324,19 +368,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(upload(c, &c->ecor_tab, et.data(), et.size()));
   }
   {
-    std::vector<double> pp((size_t)s.npair * PP_NUM, 0.0);
-    for (int t = 0; t < s.npair; t++) {
-      double *e = &pp[(size_t)t * PP_NUM];
-      e[PP_CUT] = s.cutoff[t];
-      e[PP_RMIN] = s.rmin[t];
-      e[PP_DRI] = s.dri[t];
-      e[PP_DR] = s.dr[t];
-      e[PP_EPS] = s.lj_eps[t];
-      e[PP_SIG6] = s.lj_sig6[t];
-      e[PP_VSLJ] = s.vshift_lj[t];
-      e[PP_VSC] = s.vshift_c[t];
-      e[PP_LJ] = (s.label[t] == PMX_INTER_LJ) ? 1.0 : 0.0;
-    }
+    const std::vector<double> pp = pair_constants(c);
     TRY(upload(c, &c->pair_pp, pp.data(), pp.size()));
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
@@ -353,6 +385,9 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   // bonded
   c->nbond = sys->nbond;
   c->nbend = sys->nbend;
+  c->nbond_type = sys->nbond_type;
+  c->nbend_type = sys->nbend_type;
+  c->bc_h.assign(sys->bond_c, sys->bond_c + 7 * (size_t)sys->nbond_type);
   TRY(upload(c, &c->b1, sys->bond_j1, sys->nbond));
   TRY(upload(c, &c->b2, sys->bond_j2, sys->nbond));
   TRY(upload(c, &c->btyp, sys->bond_type, sys->nbond));
@@ -366,6 +401,23 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(upload(c, &c->as, sys->bend_s, (size_t)7 * sys->nbend_type));
   TRY(upload(c, &c->e1, s.e1.data(), s.e1.size()));
   TRY(upload(c, &c->e2, s.e2.data(), s.e2.size()));
+  {  // the Ewald-corrected partners of every site, both directions
+    std::vector<int> off(N, 0), num(N, 0), js(2 * s.e1.size());
+    for (size_t k = 0; k < s.e1.size(); k++) {
+      num[s.e1[k]]++;
+      num[s.e2[k]]++;
+    }
+    for (int i = 1; i < N; i++) off[i] = off[i - 1] + num[i - 1];
+    std::vector<int> fill(N, 0);
+    for (size_t k = 0; k < s.e1.size(); k++) {
+      const int i = s.e1[k], j = s.e2[k];
+      js[off[i] + fill[i]++] = j;
+      js[off[j] + fill[j]++] = i;
+    }
+    TRY(upload(c, &c->ecs_off, off.data(), N));
+    TRY(upload(c, &c->ecs_num, num.data(), N));
+    TRY(upload(c, &c->ecs_j, js.data(), js.size()));
+  }
   c->nghost = sys->nghost;
   TRY(upload(c, &c->gatom, sys->ghost_atom, sys->nghost));
   TRY(upload(c, &c->gncomp, sys->ghost_ncomp, sys->nghost));
@@ -702,6 +754,7 @@ static int set_pos_common(pmx_ctx *c, const double *x, const double *y, const do
   }
   HIPCHK(hipStreamSynchronize(c->st));
   c->have_pos = true;
+  c->grad_ready = false;
   return PMX_OK;
 }
 
@@ -719,6 +772,7 @@ int pmx_set_xmod(pmx_ctx *c, const double *x, const double *y, const double *z) 
   if (!c) return PMX_ERR_ARG;
   if (!x) {
     c->xmod_given = false;
+    c->grad_ready = false;
     return PMX_OK;
   }
   const size_t n = (size_t)c->s.N * sizeof(double);
@@ -726,6 +780,7 @@ int pmx_set_xmod(pmx_ctx *c, const double *x, const double *y, const double *z) 
   HIPCHK(hipMemcpy(c->ym, y, n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->zm, z, n, hipMemcpyHostToDevice));
   c->xmod_given = true;
+  c->grad_ready = false;
   return PMX_OK;
 }
 
@@ -745,6 +800,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   const size_t NP = (size_t)N * P;
   HIPCHK(hipSetDevice(c->device));
   memset(out, 0, sizeof(*out));
+  c->grad_ready = false;
   hipStream_t st = c->st;
   RedArgs red{};
   HIPCHK(hipEventRecord(c->ev[0], st));
@@ -943,6 +999,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   }
   c->last_modes = modes;
   c->forces_valid = true;
+  const bool grad_state = !(c->pme && (flags & PMX_EVAL_NO_RECIP));
   if (modes) {
     ModeArgs a{};
     a.N = N;
@@ -961,6 +1018,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     HIPCHK(hipGetLastError());
   }
   red.P = P;
+  red.nslot = ACC_NUM;
   red.part = c->part;
   red.acc = c->acc;
   hipLaunchKernelGGL(k_reduce_partials, dim3(blocks((size_t)P * ACC_NUM * 64)), dim3(256), 0, st, red);
@@ -1044,6 +1102,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     out->kin_harm = ake;
     out->pi_ke_prim = 1.5 * ((double)(N - c->nghost)) / tau - ake;
   }
+  c->grad_ready = grad_state;
   return PMX_OK;
 }
 
@@ -1177,6 +1236,249 @@ int pmx_md_nve(pmx_ctx *c, int nsteps, double dt, pmx_energies *out, double *kin
     HIPCHK(hipMemcpyAsync(kinetic, c->ke, sizeof(double), hipMemcpyDeviceToHost, c->st));
   }
   HIPCHK(hipStreamSynchronize(c->st));
+  return PMX_OK;
+}
+
+// New force-field parameters for the same topology: charges (the set of
+// charged sites fixed), LJ sigma and eps, bond and bend coefficients.  The
+// parameter-dependent setup (LJ tables, tail constant, charge sums) is
+// rebuilt from the same input; the excluded-pair table keeps its range.
+int pmx_set_force_field(pmx_ctx *c, const pmx_system *sys) {
+  if (!c || !sys) return PMX_ERR_ARG;
+  Setup &s = c->s;
+  if (sys->natm != s.N || sys->ntype != s.ntype || sys->nbond != c->nbond || sys->nbend != c->nbend ||
+      sys->nbond_type != c->nbond_type || sys->nbend_type != c->nbend_type || sys->nghost != c->nghost) {
+    set_err(c, "pmx_set_force_field: the topology (sites, types, bonds, bends, ghosts) must not change");
+    return PMX_ERR_ARG;
+  }
+  Setup s2;
+  std::string m = build_setup(sys, &c->par, nullptr, nullptr, nullptr, s2);
+  if (!m.empty()) {
+    set_err(c, "pmx_set_force_field: %s", m.c_str());
+    return PMX_ERR_ARG;
+  }
+  if (s2.ichrg != s.ichrg || s2.e1 != s.e1 || s2.e2 != s.e2 || s2.label != s.label) {
+    set_err(c, "pmx_set_force_field: the charged sites and the LJ type pairs must not change");
+    return PMX_ERR_ARG;
+  }
+  s.qsq = s2.qsq;
+  s.qsum = s2.qsum;
+  s.tab = s2.tab;
+  s.lj_eps = s2.lj_eps;
+  s.lj_sig6 = s2.lj_sig6;
+  s.lj_sig = s2.lj_sig;
+  s.vshift_lj = s2.vshift_lj;
+  s.vshift_c = s2.vshift_c;
+  s.clong = s2.clong;
+  s.clong_res = s2.clong_res;
+  s.clong_u = s2.clong_u;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->st));
+  const size_t N = (size_t)s.N;
+  HIPCHK(hipMemcpy(c->q, sys->q, N * sizeof(double), hipMemcpyHostToDevice));
+  {
+    const size_t nt = (size_t)s.npair * s.nsplin;
+    std::vector<double2> lj(nt), cc(nt);
+    for (size_t k = 0; k < nt; k++) {
+      lj[k] = make_double2(s.tab[4 * k + 0], s.tab[4 * k + 1]);
+      cc[k] = make_double2(s.tab[4 * k + 2], s.tab[4 * k + 3]);
+    }
+    HIPCHK(hipMemcpy(c->tab_lj, lj.data(), nt * sizeof(double2), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->tab_c, cc.data(), nt * sizeof(double2), hipMemcpyHostToDevice));
+  }
+  const std::vector<double> pp = pair_constants(c);
+  HIPCHK(hipMemcpy(c->pair_pp, pp.data(), pp.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->bc_h.assign(sys->bond_c, sys->bond_c + 7 * (size_t)c->nbond_type);
+  if (c->nbond_type) {
+    HIPCHK(hipMemcpy(c->bc, sys->bond_c, 7 * (size_t)c->nbond_type * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->beq, sys->bond_eq, (size_t)c->nbond_type * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (c->nbend_type) {
+    HIPCHK(hipMemcpy(c->ac, sys->bend_c, 7 * (size_t)c->nbend_type * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->as, sys->bend_s, 7 * (size_t)c->nbend_type * sizeof(double), hipMemcpyHostToDevice));
+  }
+  c->forces_valid = false;
+  c->grad_ready = false;
+  return PMX_OK;
+}
+
+// Derivatives of every bead's potential energy with respect to the force
+// field, at the state of the last force call (the backward of
+// energy_control): site potentials from the real-space, k-space,
+// excluded-pair, self and background terms; LJ sums incl. the tail; bond
+// and bend moments.
+int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
+  if (!c || !out) return PMX_ERR_ARG;
+  if (!c->grad_ready) {
+    set_err(c, "pmx_get_param_grad: needs a force call (with the k-space term) at the current positions and "
+               "parameters first");
+    return PMX_ERR_STATE;
+  }
+  if (!c->full_list) {
+    set_err(c, "pmx_get_param_grad: needs the full force list");
+    return PMX_ERR_NOTIMPL;
+  }
+  Setup &s = c->s;
+  const int N = s.N, P = s.P;
+  const size_t NP = (size_t)N * P;
+  int nlj = 0;
+  for (int t = 0; t < s.npair; t++) nlj += (s.label[t] == PMX_INTER_LJ);
+  if (nlj > MAX_LJ_GRAD) {
+    set_err(c, "pmx_get_param_grad: %d LJ type pairs (at most %d)", nlj, MAX_LJ_GRAD);
+    return PMX_ERR_NOTIMPL;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  hipStream_t st = c->st;
+  int rc = PMX_OK;
+  const unsigned gx = pair_grid_x(N);
+  const bool kernels = out->dq || out->deps || out->dsig || out->dbond_c || out->dbond_eq || out->dbend_c ||
+                       out->dbend_s;
+  if (kernels && !c->phi) {
+    if ((rc = dalloc(c, &c->phi, NP))) return rc;
+    if ((rc = dalloc(c, &c->gpart, (size_t)gx * P * PAIRG_NV))) return rc;
+    if ((rc = dalloc(c, &c->gacc, (size_t)P * PAIRG_NV))) return rc;
+    if ((rc = dalloc(c, &c->gbond, (size_t)P * std::max(c->nbond_type, 1) * 7))) return rc;
+    if ((rc = dalloc(c, &c->gbend, (size_t)P * std::max(c->nbend_type, 1) * BEND_GRAD_NV))) return rc;
+  }
+  if (kernels) {  // real space: phi (stored for every site) and the LJ sums
+    PairArgs a{};
+    a.N = N;
+    a.P = P;
+    a.cap = c->cap;
+    a.nsplin = s.nsplin;
+    a.pimd = (P > 1);
+    a.npair = s.npair;
+    a.alp = s.alp_ewd;
+    a.palp = kErfP * s.alp_ewd;
+    a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
+    a.rheal = c->par.rheal;
+    a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
+    a.swit = (c->par.ishift == PMX_SHIFT_SWIT);
+    a.pp = c->pair_pp;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.site = c->site;
+    a.type = c->type;
+    a.jlist = c->jforce;
+    a.cnt = c->nforce;
+    a.tab_lj = c->tab_lj;
+    a.tab_c = c->tab_c;
+    a.cell = c->cell;
+    a.part = c->gpart;
+    a.phi = c->phi;
+    HIPCHK(launch_pair_grad(a, c->pair_mode == 0 ? 0 : 1, st));
+    RedArgs red{};
+    red.P = P;
+    red.nslot = PAIRG_NV;
+    red.nstage = 1;
+    red.st[0].off = 0;
+    red.st[0].G = (int)gx;
+    red.st[0].nv = PAIRG_NV;
+    for (int v = 0; v < PAIRG_NV; v++) red.st[0].slot[v] = v;
+    red.part = c->gpart;
+    red.acc = c->gacc;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(blocks((size_t)P * PAIRG_NV * 64)), dim3(256), 0, st, red);
+    HIPCHK(hipGetLastError());
+  }
+  const int nchrg = (int)s.ichrg.size();
+  if (kernels && c->pme && nchrg > 0) {  // k space, on the grid the force call left
+    PmeArgs a = c->pa;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.cell = c->cell;
+    a.phi = c->phi;
+    HIPCHK(launch_pme_phi(a, st));
+  }
+  IntraArgs ia = intra_args(c, P > 1);
+  ia.nchrg = nchrg;
+  ia.ichrg = c->ichrg;
+  ia.ecs_off = c->ecs_off;
+  ia.ecs_num = c->ecs_num;
+  ia.ecs_j = c->ecs_j;
+  ia.cself = -2.0 * s.alp_ewd * s.self_erf / sqrt(M_PI);
+  ia.cbgr = -M_PI * s.qsum / (s.alp_ewd * s.alp_ewd * s.vol);
+  ia.phi = c->phi;
+  ia.nbond_type = c->nbond_type;
+  ia.nbend_type = c->nbend_type;
+  ia.gbond = c->gbond;
+  ia.gbend = c->gbend;
+  if (kernels && nchrg > 0) {
+    hipLaunchKernelGGL(k_ecor_phi, dim3(blocks((size_t)nchrg * P)), dim3(256), 0, st, ia);
+    HIPCHK(hipGetLastError());
+  }
+  if (kernels && c->nbond && c->nbond_type) {
+    hipLaunchKernelGGL(k_bond_moments, dim3(c->nbond_type, P), dim3(256), 0, st, ia);
+    HIPCHK(hipGetLastError());
+  }
+  if (kernels && c->nbend && c->nbend_type) {
+    hipLaunchKernelGGL(k_bend_moments, dim3(c->nbend_type, P), dim3(256), 0, st, ia);
+    HIPCHK(hipGetLastError());
+  }
+  std::vector<double> gacc((size_t)P * PAIRG_NV, 0.0), gb((size_t)P * c->nbond_type * 7, 0.0),
+      ga((size_t)P * c->nbend_type * BEND_GRAD_NV, 0.0);
+  if (kernels) {
+    HIPCHK(hipMemcpyAsync(gacc.data(), c->gacc, gacc.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (!gb.empty())
+      HIPCHK(hipMemcpyAsync(gb.data(), c->gbond, gb.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (!ga.empty())
+      HIPCHK(hipMemcpyAsync(ga.data(), c->gbend, ga.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  if (out->dq) {
+    hipLaunchKernelGGL(k_atom_to_bead, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->phi, c->stage);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out->dq, c->stage, NP * sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  const double vlong = s.clong / s.vol;
+  double vself = 0.0, vbgr = 0.0;
+  if (nchrg > 0) {
+    vself = -s.qsq * s.alp_ewd * s.self_erf / sqrt(M_PI);
+    vbgr = -0.5 * s.qsum * s.qsum * M_PI / (s.alp_ewd * s.alp_ewd * s.vol);
+  }
+  for (int b = 0; b < P; b++) {
+    const double *e = &c->acc_h[(size_t)b * ACC_NUM];
+    if (out->energy)
+      out->energy[b] = e[ACC_VVDW] + e[ACC_VCOUL] + e[ACC_VRECIP] + e[ACC_VECOR] + e[ACC_VBOND] + e[ACC_VBEND] +
+                       vlong + vself + vbgr;
+    for (int t = 0; t < s.npair; t++) {
+      double de = 0.0, ds = 0.0;
+      const int g = c->lj_slot[t];
+      if (g >= 0) {
+        const double U = gacc[(size_t)b * PAIRG_NV + g], S = gacc[(size_t)b * PAIRG_NV + MAX_LJ_GRAD + g];
+        const double eps = s.lj_eps[t], sig = s.lj_sig[t];
+        const double tail = s.clong_u[t] / s.vol;
+        de = U + tail;
+        ds = (eps * S + 6.0 * eps * tail) / sig;
+      }
+      if (out->deps) out->deps[(size_t)b * s.npair + t] = de;
+      if (out->dsig) out->dsig[(size_t)b * s.npair + t] = ds;
+    }
+    const std::vector<double> &bc = c->bc_h;
+    for (int ty = 0; ty < c->nbond_type; ty++) {
+      const double *M = &gb[((size_t)b * c->nbond_type + ty) * 7];
+      if (out->dbond_c)
+        for (int k = 0; k < 7; k++) out->dbond_c[((size_t)b * c->nbond_type + ty) * 7 + k] = M[k];
+      if (out->dbond_eq) {
+        double d = 0.0;
+        for (int k = 1; k < 7; k++) d += (double)k * bc[7 * ty + k] * M[k - 1];
+        out->dbond_eq[(size_t)b * c->nbond_type + ty] = -d;
+      }
+    }
+    for (int ty = 0; ty < c->nbend_type; ty++) {
+      const double *M = &ga[((size_t)b * c->nbend_type + ty) * BEND_GRAD_NV];
+      double *dc = out->dbend_c ? out->dbend_c + ((size_t)b * c->nbend_type + ty) * 7 : nullptr;
+      double *ds = out->dbend_s ? out->dbend_s + ((size_t)b * c->nbend_type + ty) * 7 : nullptr;
+      if (dc)
+        for (int k = 0; k < 7; k++) dc[k] = M[k];
+      if (ds) {
+        ds[0] = 0.0;
+        ds[1] = M[7];
+        for (int k = 2; k < 7; k++) ds[k] = M[6 + k];
+      }
+    }
+  }
   return PMX_OK;
 }
 

@@ -43,8 +43,10 @@ struct Setup {
   std::vector<double> tab;
   // the tabulated functions' parameters, for direct evaluation: LJ eps and
   // sigma^6, and the constant removed from V by shift_inter_pe = shift
-  std::vector<double> lj_eps, lj_sig6, vshift_lj, vshift_c;
+  std::vector<double> lj_eps, lj_sig, lj_sig6, vshift_lj, vshift_c;
   double cutoff_max = 0.0, clong = 0.0, clong_res = 0.0;
+  // each type pair's share of clong per unit LJ eps (clong = sum eps*clong_u)
+  std::vector<double> clong_u;
   // charges and the sums of ewald3d_selfbgr (ewald3d_self_bgr.c:55-61)
   std::vector<int> ichrg;
   double qsq = 0.0, qsum = 0.0;

@@ -31,7 +31,15 @@ struct ListArgs {
 // direct evaluation (spline_fit.c:28-398): LJ 4 eps (s^6/r^6)((s^6/r^6) - 1)
 // and the PINY erfc polynomial, both times the switch of width rheal below
 // the cutoff when swit is set, minus vshift (shift_inter_pe = shift).
-enum { PP_CUT = 0, PP_RMIN, PP_DRI, PP_DR, PP_EPS, PP_SIG6, PP_VSLJ, PP_VSC, PP_LJ, PP_NUM };
+// For the parameter gradients: the LJ shift per unit eps (VSLJ1) and
+// sigma d/dsigma of it (DSLJ1), and the type pair's LJ gradient slot (GSLOT,
+// -1 when the pair has no LJ term).
+enum {
+  PP_CUT = 0, PP_RMIN, PP_DRI, PP_DR, PP_EPS, PP_SIG6, PP_VSLJ, PP_VSC, PP_LJ,
+  PP_VSLJ1, PP_DSLJ1, PP_GSLOT, PP_NUM
+};
+// LJ type pairs whose eps/sigma gradients one context forms
+constexpr int MAX_LJ_GRAD = 4;
 
 // List entries of the force kernels carry the type pair in the top byte.
 constexpr int ENTRY_SHIFT = 24;
@@ -50,6 +58,7 @@ struct PairArgs {
   Cell cell;
   double *fx, *fy, *fz;
   double *part;  // per-block partial sums [gridDim.x][P][PAIR_NV]
+  double *phi;   // gradient kernel: dE_b/dq_i, real-space part [N*P]
 };
 
 // PME on a grid cut into tiles of T[0] x T[1] x T[2] points.  Charge (c, b)
@@ -77,6 +86,7 @@ struct PmeArgs {
   void *hbuf;          // [P*ntile*hvol] halo boxes of spread charge (grid type)
   void *grid;          // [P*gstride] (grid type)
   double *fx, *fy, *fz;
+  double *phi;         // potential gather: dE_b/dq_i += k-space part [N*P]
 };
 
 constexpr int FFT_MAX_PASSES = 12;
@@ -118,7 +128,20 @@ struct IntraArgs {
   const int *e1, *e2;
   const double2 *ecor_tab;
   double e_rmin, e_dr, e_dri;
+  // gradients: ecor partners of every charged site (both directions),
+  // d(vself + vbgr)/dq_i = cself*q_i + cbgr, outputs
+  int nchrg;
+  const int *ichrg, *ecs_off, *ecs_num, *ecs_j;
+  double cself, cbgr;
+  double *phi;   // [N*P] += excluded-pair and self/background parts
+  int nbond_type, nbend_type;
+  double *gbond;  // [P][nbond_type][7] sum over bonds of (r - eq)^m
+  double *gbend;  // [P][nbend_type][BEND_GRAD_NV]
 };
+
+// Bend moments per (bead, bend type): cos^m (m = 0..6), sin, cos sin^(m-1)
+// (m = 2..6): the derivatives of V with respect to c_m and s_m.
+constexpr int BEND_GRAD_NV = 13;
 
 struct FinishArgs {
   int N, P;
@@ -141,6 +164,7 @@ constexpr int PAIR_NV = 9;   // vvdw, vcoul, 6 virial, pair count
 constexpr int FFT_NV = 7;    // vrecip, 6 virial
 constexpr int INTRA_NV = 7;  // energy, 6 virial
 constexpr int FIN_NV = 10;   // 9 virial corrections, akvir
+constexpr int PAIRG_NV = 2 * MAX_LJ_GRAD;  // per LJ slot: energy per unit eps, sigma dV/dsigma per unit eps
 
 // One stage of partial sums: part[off + (g*P + b)*nv + v] for g < G holds
 // the contribution of block g to accumulator slot[v] of bead b.
@@ -152,9 +176,10 @@ struct RedStage {
 constexpr int MAX_RED_STAGES = 8;
 struct RedArgs {
   int P, nstage;
+  int nslot;  // accumulator slots per bead (ACC_NUM for the energies)
   RedStage st[MAX_RED_STAGES];
   const double *part;
-  double *acc;  // [P][ACC_NUM], every entry written
+  double *acc;  // [P][nslot], every entry written
 };
 
 __global__ void k_bead_to_atom(int N, int P, const double *src, double *dst);
@@ -178,6 +203,9 @@ __global__ void k_pme_scan(PmeArgs a);
 __global__ void k_pme_scatter(PmeArgs a);
 __global__ void k_fill_int(int *p, int v, size_t n);
 __global__ void k_reduce_partials(RedArgs a);
+__global__ void k_ecor_phi(IntraArgs a);
+__global__ void k_bond_moments(IntraArgs a);
+__global__ void k_bend_moments(IntraArgs a);
 __global__ void k_ecor(IntraArgs a);
 __global__ void k_bond(IntraArgs a);
 __global__ void k_bend(IntraArgs a);
@@ -196,11 +224,13 @@ __global__ void k_kinetic(int n, const double *mass, const double *vx, const dou
 // full: the list holds both directions and each wave stores its sites'
 // forces (no atomics); otherwise j forces are scattered with atomics
 hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st);
+hipError_t launch_pair_grad(const PairArgs &a, int mode, hipStream_t st);
 int pair_grid_x(int N);
 // LDS bytes of the spread and gather workgroups for a given tiling
 size_t pme_lds_bytes(const PmeArgs &a, bool gather);
 hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st);
 hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st);
+hipError_t launch_pme_phi(const PmeArgs &a, hipStream_t st);
 hipError_t launch_pme_fft(const FftArgs &f, hipStream_t st);
 // block size for grid-stride kernels whose threads keep one bead: the
 // largest multiple of P not above 256

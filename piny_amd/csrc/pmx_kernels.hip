@@ -24,8 +24,8 @@ __global__ void k_fill_int(int *p, int v, size_t n) {
 // wavefront per (bead, slot); blocks are summed in a fixed order.
 __global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {
   const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
-  if (w >= a.P * ACC_NUM) return;  // uniform over the wavefront
-  const int b = w / ACC_NUM, slot = w % ACC_NUM;
+  if (w >= a.P * a.nslot) return;  // uniform over the wavefront
+  const int b = w / a.nslot, slot = w % a.nslot;
   double s = 0.0;
   for (int si = 0; si < a.nstage; si++) {
     const RedStage &r = a.st[si];
@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {
     }
   }
   for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane == 0) a.acc[(size_t)b * ACC_NUM + slot] = s;
+  if (lane == 0) a.acc[(size_t)b * a.nslot + slot] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -610,6 +610,98 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Parameter gradients of the real-space term (the backward of force_npol).
+// Per (site, bead): phi_i = sum_j q_j vc(r_ij), the derivative of the bead's
+// energy with respect to q_i (vc: the erfc-Coulomb function per unit charge
+// product, switched and shifted as tabulated; from the tables when MODE is 0,
+// directly otherwise).  Per LJ type pair slot: the LJ energy per unit eps and
+// sigma dV/dsigma per unit eps, from the LJ form itself.  Sites without
+// charge get phi = 0: the set of charged sites is fixed.  Full list only (each
+// pair is seen from both ends, so the pair sums are halved).
+template <int B, int MODE>
+__global__ void __launch_bounds__(256) k_pair_grad(PairArgs a) {
+  __shared__ double spp[MAX_PAIRS * PP_NUM];
+  for (int k = threadIdx.x; k < a.npair * PP_NUM; k += blockDim.x) spp[k] = a.pp[k];
+  __syncthreads();
+  constexpr int S = 64 / B;
+  const int lane = threadIdx.x & 63;
+  const int slot = lane / B;
+  const int bead = (lane % B) + blockIdx.y * B;
+  const bool bact = bead < a.P;
+  const int P = a.P;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
+  const Cell &c = a.cell;
+  double ge[MAX_LJ_GRAD] = {0, 0, 0, 0}, gs[MAX_LJ_GRAD] = {0, 0, 0, 0};
+  for (int i = wave; i < a.N; i += nwave) {
+    const size_t ib = (size_t)i * P + (bact ? bead : 0);
+    const double xi = a.x[ib], yi = a.y[ib], zi = a.z[ib];
+    const double4 si = a.site[i];
+    const int cnt = a.cnt[i];
+    const int *row = a.jlist + (size_t)i * a.cap;
+    double phi = 0.0;
+    for (int k = slot; k < cnt; k += S) {
+      const int e = row[k];
+      const int j = e & ENTRY_MASK, t = e >> ENTRY_SHIFT;
+      const double *pp = spp + t * PP_NUM;
+      const size_t jb = (size_t)j * P + (bact ? bead : 0);
+      const double4 sj = a.site[j];
+      double dx, dy, dz;
+      const double r2d = min_image_sep(c, a.pimd, xi, yi, zi, a.x[jb], a.y[jb], a.z[jb], si.x, si.y, si.z, sj.x,
+                                       sj.y, sj.z, dx, dy, dz);
+      const double rd = sqrt(r2d);
+      if (!bact || !(rd < pp[PP_CUT])) continue;
+      const double rinv = 1.0 / rd;
+      double sw = 1.0;
+      if (a.swit) {
+        double rp = (rd - pp[PP_CUT] + a.rheal) * a.irheal;
+        rp = rp < 1.0 ? rp : 1.0;
+        rp = rp > 0.0 ? rp : 0.0;
+        sw = 1.0 + rp * rp * (2.0 * rp - 3.0);
+      }
+      if (sj.w != 0.0) {
+        double vc;
+        if constexpr (MODE == 0) {
+          const double rmin = pp[PP_RMIN], dri = pp[PP_DRI];
+          int kk = (int)(((rd - rmin) * dri) + 0.5) + 3;
+          kk = min(kk, a.nsplin - 2);
+          kk = max(kk, 3);
+          const double rsp = ((double)(kk - 3)) * pp[PP_DR] + rmin;
+          vc = spline4(a.tab_c + (size_t)t * a.nsplin, kk, (rd - rsp) * dri, false);
+        } else {
+          double vlj_u, vc_u;
+          pair_direct<double>(a, pp, 1.0, rd, rinv, vlj_u, vc_u);
+          vc = vc_u;
+        }
+        phi += sj.w * vc;
+      }
+      const int g = (int)pp[PP_GSLOT];
+      if (g >= 0) {
+        const double r2i = rinv * rinv;
+        const double s6 = pp[PP_SIG6] * (r2i * r2i * r2i);
+        const double u = 4.0 * s6 * (s6 - 1.0) * sw - pp[PP_VSLJ1];
+        const double d = 4.0 * (12.0 * s6 * s6 - 6.0 * s6) * sw - pp[PP_DSLJ1];
+#pragma unroll
+        for (int q = 0; q < MAX_LJ_GRAD; q++)
+          if (q == g) {
+            ge[q] += u;
+            gs[q] += d;
+          }
+      }
+    }
+    for (int off = 32; off >= B; off >>= 1) phi += __shfl_xor(phi, off, 64);
+    if (slot == 0 && bact) a.phi[ib] = (si.w != 0.0) ? phi : 0.0;
+  }
+  double vals[PAIRG_NV];
+#pragma unroll
+  for (int q = 0; q < MAX_LJ_GRAD; q++) {
+    vals[q] = 0.5 * ge[q];
+    vals[MAX_LJ_GRAD + q] = 0.5 * gs[q];
+  }
+  block_partials<PAIRG_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
 // bonded terms and excluded-pair corrections
 
 __device__ __forceinline__ void intra_sep(const IntraArgs &a, int i, int j, int b, double &dx, double &dy,
@@ -761,6 +853,116 @@ __global__ void __launch_bounds__(256) k_bend(IntraArgs a) {
     acc[6] += dy12 * f1z + dy23 * f3z;
   }
   block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Parameter gradients of the excluded-pair, self and background terms and of
+// the bonded terms.
+
+// One thread per (charged site, bead): phi_i += sum over the site's
+// Ewald-corrected excluded partners j of q_j ecor(r_ij) (the table and
+// interpolation of k_ecor), plus d(vself + vbgr)/dq_i
+// (ewald3d_self_bgr.c:55-61).
+__global__ void __launch_bounds__(256) k_ecor_phi(IntraArgs a) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)a.nchrg * a.P) return;
+  const int i = a.ichrg[t / a.P], b = (int)(t % a.P);
+  double phi = a.cself * a.q[i] + a.cbgr;
+  const int o = a.ecs_off[i], n = a.ecs_num[i];
+  for (int k = 0; k < n; k++) {
+    const int j = a.ecs_j[o + k];
+    double dx, dy, dz;
+    intra_sep(a, i, j, b, dx, dy, dz);
+    const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
+    int kk = (int)(((r12 - a.e_rmin) * a.e_dri) + 0.5) + 3;
+    kk = min(kk, a.nsplin - 2);
+    kk = max(kk, 3);
+    const double rsp = ((double)(kk - 3)) * a.e_dr + a.e_rmin;
+    phi += a.q[j] * spline4(a.ecor_tab, kk, (r12 - rsp) * a.e_dri, false);
+  }
+  a.phi[(size_t)i * a.P + b] += phi;
+}
+
+// Sums of a workgroup's threads in a fixed order (tree in LDS); every
+// thread of the block calls it, thread 0 gets the result.
+__device__ __forceinline__ double block_sum(double v, double *red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int h = blockDim.x >> 1; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// One workgroup of 256 per (bond type, bead): sum over that type's bonds of
+// (r - eq)^m, m = 0..6, the derivatives of V with respect to c_m
+// (bond.c:150-166).
+__global__ void __launch_bounds__(256) k_bond_moments(IntraArgs a) {
+  __shared__ double red[256];
+  const int ty = blockIdx.x, b = blockIdx.y;
+  double m[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int k = threadIdx.x; k < a.nbond; k += blockDim.x) {
+    if (a.btyp[k] != ty) continue;
+    double dx, dy, dz;
+    intra_sep(a, a.b1[k], a.b2[k], b, dx, dy, dz);
+    const double rr0 = sqrt(dx * dx + dy * dy + dz * dz) - a.beq[ty];
+    double p = 1.0;
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+      m[q] += p;
+      p *= rr0;
+    }
+  }
+  for (int q = 0; q < 7; q++) {
+    const double v = block_sum(m[q], red);
+    if (threadIdx.x == 0) a.gbond[((size_t)b * a.nbond_type + ty) * 7 + q] = v;
+  }
+}
+
+// One workgroup of 256 per (bend type, bead): cos^m (m = 0..6), sin and
+// cos sin^(m-1) (m = 2..6) summed over the type's bends, with the cosine and
+// sine clamped as k_bend clamps them (bend.c:253-270).
+__global__ void __launch_bounds__(256) k_bend_moments(IntraArgs a) {
+  __shared__ double red[256];
+  const int ty = blockIdx.x, b = blockIdx.y;
+  double m[BEND_GRAD_NV];
+#pragma unroll
+  for (int q = 0; q < BEND_GRAD_NV; q++) m[q] = 0.0;
+  for (int k = threadIdx.x; k < a.nbend; k += blockDim.x) {
+    if (a.atyp[k] != ty) continue;
+    double dx12, dy12, dz12, dx23, dy23, dz23;
+    intra_sep(a, a.a1[k], a.a2[k], b, dx12, dy12, dz12);
+    intra_sep(a, a.a3[k], a.a2[k], b, dx23, dy23, dz23);
+    const double r122i = 1.0 / (dx12 * dx12 + dy12 * dy12 + dz12 * dz12);
+    const double r322i = 1.0 / (dx23 * dx23 + dy23 * dy23 + dz23 * dz23);
+    const double rpmag = sqrt(r122i) * sqrt(r322i);
+    double cost = (dx12 * dx23 + dy12 * dy23 + dz12 * dz23) * rpmag;
+    cost = (cost < 1.0 ? cost : 1.0);
+    cost = (cost > -1.0 ? cost : -1.0);
+    double sint = sqrt(1.0 - cost * cost);
+    sint = (sint > 1.0e-8 ? sint : 1.0e-8);
+    double p = 1.0;
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+      m[q] += p;
+      p *= cost;
+    }
+    m[7] += sint;
+    double sp = sint;
+#pragma unroll
+    for (int q = 8; q < BEND_GRAD_NV; q++) {
+      m[q] += cost * sp;
+      sp *= sint;
+    }
+  }
+  for (int q = 0; q < BEND_GRAD_NV; q++) {
+    const double v = block_sum(m[q], red);
+    if (threadIdx.x == 0) a.gbend[((size_t)b * a.nbend_type + ty) * BEND_GRAD_NV + q] = v;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -935,6 +1137,31 @@ hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st) {
     case 16: launch_pair_b<16>(a, mode, full, grid, st); break;
     case 32: launch_pair_b<32>(a, mode, full, grid, st); break;
     default: launch_pair_b<64>(a, mode, full, grid, st); break;
+  }
+  return hipGetLastError();
+}
+
+template <int B>
+void launch_pair_grad_b(const PairArgs &a, int mode, dim3 grid, hipStream_t st) {
+  if (mode == 0)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_grad<B, 0>), grid, dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_grad<B, 1>), grid, dim3(256), 0, st, a);
+}
+
+hipError_t launch_pair_grad(const PairArgs &a, int mode, hipStream_t st) {
+  int B = 1;
+  while (B < a.P && B < 64) B <<= 1;
+  const int nbg = (a.P + B - 1) / B;
+  const dim3 grid(pair_grid_x(a.N), nbg);
+  switch (B) {
+    case 1: launch_pair_grad_b<1>(a, mode, grid, st); break;
+    case 2: launch_pair_grad_b<2>(a, mode, grid, st); break;
+    case 4: launch_pair_grad_b<4>(a, mode, grid, st); break;
+    case 8: launch_pair_grad_b<8>(a, mode, grid, st); break;
+    case 16: launch_pair_grad_b<16>(a, mode, grid, st); break;
+    case 32: launch_pair_grad_b<32>(a, mode, grid, st); break;
+    default: launch_pair_grad_b<64>(a, mode, grid, st); break;
   }
   return hipGetLastError();
 }

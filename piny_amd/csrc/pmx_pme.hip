@@ -260,24 +260,15 @@ __global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
 
 // --- gather ---------------------------------------------------------------------
 
-// one workgroup per (tile, bead): the potential on the halo box into LDS,
-// then one lane per charge (ewald3d_pme.c:534-560)
-template <int NI, typename T>
-__global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
-  extern __shared__ __align__(16) unsigned char lds_raw[];
-  T *box = (T *)lds_raw;
+// The potential on the halo box of tile origin org, bead b, into LDS, with
+// batches of 8 loads in flight per lane.
+template <typename T>
+__device__ __forceinline__ void load_halo(const PmeArgs &a, int b, const int (&org)[3], T *box) {
   const int hvol = (int)a.hvol;
-  const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
-  const size_t bin = (size_t)b * a.ntile + tl;
-  const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
-  if (cnt == 0) return;  // uniform over the workgroup
-  int org[3];
-  tile_origin(a, tl, org[0], org[1], org[2]);
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2];
   const int H0 = a.H[0], H1 = a.H[1], H01 = H0 * H1;
   const T *g = (const T *)a.grid + (size_t)b * a.gstride;
-  // batches of 8 loads in flight per lane
-  for (int k0 = tid; k0 < hvol; k0 += 8 * blockDim.x) {
+  for (int k0 = threadIdx.x; k0 < hvol; k0 += 8 * blockDim.x) {
     T v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -298,6 +289,23 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
       if (k < hvol) box[k] = v[u];
     }
   }
+}
+
+// one workgroup per (tile, bead): the potential on the halo box into LDS,
+// then one lane per charge (ewald3d_pme.c:534-560)
+template <int NI, typename T>
+__global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *box = (T *)lds_raw;
+  const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const size_t bin = (size_t)b * a.ntile + tl;
+  const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
+  if (cnt == 0) return;  // uniform over the workgroup
+  int org[3];
+  tile_origin(a, tl, org[0], org[1], org[2]);
+  const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2];
+  const int H0 = a.H[0], H01 = H0 * a.H[1];
+  load_halo<T>(a, b, org, box);
   __syncthreads();
   const double *hi = a.cell.hi;
   for (int c = tid; c < cnt; c += blockDim.x) {
@@ -333,6 +341,45 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
     a.fx[ib] -= da * hi[0] + db * hi[1] + dc * hi[2];
     a.fy[ib] -= da * hi[3] + db * hi[4] + dc * hi[5];
     a.fz[ib] -= da * hi[6] + db * hi[7] + dc * hi[8];
+  }
+}
+
+// The k-space part of dE_b/dq_i: the convolved grid (the derivative of the
+// k-space energy with respect to the grid charges) interpolated at the charge
+// with the B-spline weights themselves (the gather's weights without the
+// derivative).  One workgroup per (tile, bead), one lane per charge.
+template <int NI, typename T>
+__global__ void __launch_bounds__(PME_BLOCK) k_pme_phi(PmeArgs a) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  T *box = (T *)lds_raw;
+  const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const size_t bin = (size_t)b * a.ntile + tl;
+  const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
+  if (cnt == 0) return;  // uniform over the workgroup
+  int org[3];
+  tile_origin(a, tl, org[0], org[1], org[2]);
+  const int H0 = a.H[0], H01 = H0 * a.H[1];
+  load_halo<T>(a, b, org, box);
+  __syncthreads();
+  for (int c = tid; c < cnt; c += blockDim.x) {
+    const int it = a.sorted[lo + c];
+    const int ic = it % a.nchrg, bead = it / a.nchrg;
+    const int i = a.ichrg[ic];
+    T w[3][NI], dw[3][NI];
+    int l[3];
+    item_weights<NI, false, T>(a, it, org, w, dw, l);
+    double v = 0.0;
+    for (int mc = 0; mc < NI; mc++) {
+#pragma unroll
+      for (int mb = 0; mb < NI; mb++) {
+        const T *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
+        T r = (T)0;
+#pragma unroll
+        for (int ma = 0; ma < NI; ma++) r += w[0][ma] * row[ma];
+        v += (double)(r * (w[1][mb] * w[2][mc]));
+      }
+    }
+    a.phi[(size_t)i * a.P + bead] += v;
   }
 }
 
@@ -375,6 +422,24 @@ hipError_t launch_gather_t(const PmeArgs &a, hipStream_t st) {
 // spread into halo boxes, then the fold into the grid
 hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
   return a.f32 ? launch_spread_t<float>(a, st) : launch_spread_t<double>(a, st);
+}
+
+template <typename T>
+hipError_t launch_phi_t(const PmeArgs &a, hipStream_t st) {
+  const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
+  const size_t lds = pme_lds_bytes(a, true);
+  switch (a.n) {
+#define X(NI) \
+  case NI: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_phi<NI, T>), grid, block, lds, st, a); break;
+    PMX_PME_CASES(X)
+#undef X
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_pme_phi(const PmeArgs &a, hipStream_t st) {
+  return a.f32 ? launch_phi_t<float>(a, st) : launch_phi_t<double>(a, st);
 }
 
 hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {

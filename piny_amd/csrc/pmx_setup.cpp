@@ -393,6 +393,7 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
   s.tab.assign((size_t)s.npair * s.nsplin * 4, 0.0);
   s.lj_eps.assign(s.npair, 0.0);
   s.lj_sig6.assign(s.npair, 0.0);
+  s.lj_sig.assign(s.npair, 0.0);
   s.vshift_lj.assign(s.npair, 0.0);
   s.vshift_c.assign(s.npair, 0.0);
   s.cutoff_max = 0.0;
@@ -418,6 +419,7 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
     const double sg = sys->inter_sig[t];
     s.lj_eps[t] = sys->inter_eps[t];
     s.lj_sig6[t] = sg * sg * sg * sg * sg * sg;
+    s.lj_sig[t] = sg;
     for (int i = 0; i < s.nsplin; i++) {
       double *e = &s.tab[((size_t)t * s.nsplin + i) * 4];
       e[0] = v[i];
@@ -439,15 +441,18 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
     const int iswit = (par->ishift == PMX_SHIFT_SWIT);
     s.clong = 0.0;
     s.clong_res = 0.0;
+    s.clong_u.assign(s.npair, 0.0);
     for (int t = 0; t < s.npair; t++) {
-      double c6m = 0.0;
+      double c6m = 0.0, c6u = 0.0;
       if (s.label[t] == PMX_INTER_LJ) {
         const double sg = sys->inter_sig[t];
         c6m = (4.0 * sys->inter_eps[t]) * (sg * sg * sg * sg * sg * sg);
+        c6u = 4.0 * (sg * sg * sg * sg * sg * sg);
       }
       const double rc = s.cutoff[t], rr = s.cutoff_res[t];
       const double rmin = (rc < rr ? rc : rr);
       if (iswit == 0) {
+        s.clong_u[t] = -fpi3 * c6u * clongm[t] / pow(rc, 3.0);
         s.clong -= fpi3 * c6m * clongm[t] / pow(rc, 3.0);
         s.clong_res -= fpi * c6m * clongm[t] *
                        ((1.0 / (rheal * rheal)) * (1.0 / rmin + 1.0 / (rmin - rheal)) +
@@ -455,6 +460,7 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
       } else if ((c6m > 0) && (rc - rheal > 0)) {
         const double fact = (1.0 / (rheal * rheal)) * (1.0 / rc + 1.0 / (rc - rheal)) +
                             (2.0 / (rheal * rheal * rheal)) * log((rc - rheal) / rc);
+        s.clong_u[t] = -(fpi * c6u * clongm[t] * fact);
         s.clong -= (fpi * c6m * clongm[t] * fact);
         const double fres = (1.0 / (rheal * rheal)) * (1.0 / rmin + 1.0 / (rmin - rheal)) +
                             (2.0 / (rheal * rheal * rheal)) * log((rmin - rheal) / rmin);

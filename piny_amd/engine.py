@@ -64,7 +64,9 @@ def lib():
     L.pmx_bond_coef_harm.argtypes = [C.c_double, dp]
     L.pmx_bond_coef_morse.argtypes = [C.c_double, C.c_double, dp]
     L.pmx_bend_coef_harm.argtypes = [C.c_double, C.c_double, dp, dp]
-    if L.pmx_abi_version() != 2:
+    L.pmx_set_force_field.argtypes = [C.c_void_p, C.POINTER(abi.PmxSystem)]
+    L.pmx_get_param_grad.argtypes = [C.c_void_p, C.POINTER(abi.PmxParamGrad)]
+    if L.pmx_abi_version() != 3:
         raise PinyAmdError("libpiny_amd.so ABI version mismatch")
     _lib = L
     return L
@@ -77,7 +79,7 @@ EXPORTED = ("pmx_abi_version", "pmx_device_count", "pmx_create", "pmx_destroy", 
             "pmx_get_verlet_list", "pmx_get_nktot", "pmx_get_kvectors", "pmx_get_spline",
             "pmx_get_setup_scalars", "pmx_get_timings", "pmx_get_pair_count", "pmx_set_velocities",
             "pmx_get_velocities", "pmx_get_positions", "pmx_md_nve", "pmx_bond_coef_harm", "pmx_bond_coef_morse",
-            "pmx_bend_coef_harm")
+            "pmx_bend_coef_harm", "pmx_set_force_field", "pmx_get_param_grad")
 
 
 class Engine:
@@ -173,6 +175,33 @@ class Engine:
         ke = C.c_double(0.0)
         self._chk(lib().pmx_md_nve(self.h, int(nsteps), float(dt), C.byref(e), C.byref(ke)), "pmx_md_nve")
         return e.as_dict(), ke.value
+
+    def set_force_field(self, system):
+        """New charges, LJ sigma/eps and bond/bend coefficients (same topology)."""
+        st = abi.SystemStruct(system, self.st.par.precision, self.st.par.pair_eval)
+        self._chk(lib().pmx_set_force_field(self.h, C.byref(st.sys)), "pmx_set_force_field")
+        self.system = system
+        self.st = st
+
+    def param_grad(self):
+        """dE_b/d(parameters) per bead at the last force call (pmx_get_param_grad):
+        dict of arrays energy [P], dq [P, N], deps/dsig [P, npair],
+        dbond_c [P, nbt, 7], dbond_eq [P, nbt], dbend_c/dbend_s [P, nat, 7]."""
+        P, sy = self.P, self.st.sys
+        nbt, nat, npair = sy.nbond_type, sy.nbend_type, self.system.ntype * (self.system.ntype + 1) // 2
+        out = {"energy": np.zeros(P), "dq": np.zeros((P, self.N)), "deps": np.zeros((P, npair)),
+               "dsig": np.zeros((P, npair)), "dbond_c": np.zeros((P, nbt, 7)), "dbond_eq": np.zeros((P, nbt)),
+               "dbend_c": np.zeros((P, nat, 7)), "dbend_s": np.zeros((P, nat, 7))}
+        g = abi.PmxParamGrad(**{k: abi.dptr(v) for k, v in out.items()})
+        self._chk(lib().pmx_get_param_grad(self.h, C.byref(g)), "pmx_get_param_grad")
+        return out
+
+    def bead_potential(self):
+        """E_b [P] of the last force call (pmx_get_param_grad, energies only)."""
+        e = np.zeros(self.P)
+        g = abi.PmxParamGrad(energy=abi.dptr(e))
+        self._chk(lib().pmx_get_param_grad(self.h, C.byref(g)), "pmx_get_param_grad")
+        return e
 
     def bead_energies(self):
         out = np.zeros(4 * self.P)

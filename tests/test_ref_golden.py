@@ -34,9 +34,16 @@ def load(name):
     return d, ref
 
 
-def compare(name, e, f, d, ref, etol, ftol, vtol=None):
+def compare(name, e, f, d, ref, etol, ftol, vtol=None, term_scale=False):
+    """Energies within etol of max(|value|, scale): scale is the larger of
+    vintert and vintrat, or with term_scale the largest of the terms whose sum
+    vintert is (real space, k space incl. self/background/ecor), the bar of a
+    reduced-precision path whose errors scale with the terms, not with the
+    cancelled total."""
     vtol = etol if vtol is None else vtol
     scale = max(abs(ref["vintert"]), abs(ref["vintrat"]), 1e-3)
+    if term_scale:
+        scale = max(scale, abs(ref["vreal_implied"]), abs(ref["vrecip"]))
     got = dict(e)
     got["vrecip_total"] = e["vrecip"] + e["vself"] + e["vbgr"] + e["vecor"]
     got["vreal_implied"] = e["vreal"]
