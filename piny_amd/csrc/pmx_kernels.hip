@@ -343,11 +343,13 @@ __global__ void __launch_bounds__(256) k_sort_rows(int cap, int n2, const int *n
   for (int which = 0; which < 2; which++) {
     int *row = (which == 0 ? jfull : jforce) + (size_t)i * cap;
     const int cnt = which == 0 ? nfull[i] : nforce[i];
-    for (int k = tid; k < n2; k += blockDim.x) keys[k] = k < cnt ? row[k] : ENTRY_MASK | 0x7f000000;
+    int m = 64;  // this row's sort width: a power of two >= cnt, <= n2
+    while (m < cnt && m < n2) m <<= 1;
+    for (int k = tid; k < m; k += blockDim.x) keys[k] = k < cnt ? row[k] : ENTRY_MASK | 0x7f000000;
     __syncthreads();
-    for (int len = 2; len <= n2; len <<= 1) {
+    for (int len = 2; len <= m; len <<= 1) {
       for (int half = len >> 1; half > 0; half >>= 1) {
-        for (int k = tid; k < n2; k += blockDim.x) {
+        for (int k = tid; k < m; k += blockDim.x) {
           const int p = k ^ half;
           if (p > k) {
             const int a = keys[k], b = keys[p];
