@@ -46,6 +46,7 @@ struct pmx_ctx {
   double2 *tab_lj = nullptr, *tab_c = nullptr, *ecor_tab = nullptr;
   double *pair_pp = nullptr;   // [npair][PP_NUM]
   double4 *site = nullptr;     // [N] (xmod, q)
+  float4 *disp = nullptr;      // mixed precision with beads: x - xmod in FP32 [N*P]
   int pair_mode = 0;       // 0 FP64 tables, 1 FP64 direct, 2 FP32 direct
   bool full_list = true;   // force list with both directions (no atomics)
   int cap = 0, *jfull = nullptr, *jforce = nullptr, *nfull = nullptr, *nforce = nullptr, *overflow = nullptr;
@@ -372,6 +373,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(upload(c, &c->pair_pp, pp.data(), pp.size()));
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
+    if (c->pair_mode == 2 && P > 1) TRY(dalloc(c, &c->disp, NP));
   }
   TRY(dalloc(c, &c->site_cell, N));
   TRY(dalloc(c, &c->site_pos, N));
@@ -859,6 +861,11 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
                        c->q, c->site);
     HIPCHK(hipGetLastError());
+    if (c->disp) {
+      hipLaunchKernelGGL(k_pack_disp, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm,
+                         c->disp);
+      HIPCHK(hipGetLastError());
+    }
     PairArgs a{};
     a.N = N;
     a.P = P;
@@ -877,6 +884,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     a.y = c->y;
     a.z = c->z;
     a.site = c->site;
+    a.u = c->disp;
     a.type = c->type;
     a.jlist = c->jforce;
     a.cnt = c->nforce;

@@ -103,6 +103,19 @@ __global__ void k_pack_sites(int N, int pimd, const double *__restrict__ xm, con
   site[i] = pimd ? make_double4(xm[i], ym[i], zm[i], q[i]) : make_double4(0.0, 0.0, 0.0, q[i]);
 }
 
+// Bead displacements from the centroid reference xmod in FP32, for the
+// mixed-precision pair kernel: the separation of a pair is then
+// (u_i - u_j) + (xmod_i - xmod_j imaged), the same quantity the FP64 kernel
+// forms as x_i - x_j plus the xmod image shift.
+__global__ void k_pack_disp(int N, int P, const double *__restrict__ x, const double *__restrict__ y,
+                            const double *__restrict__ z, const double *__restrict__ xm,
+                            const double *__restrict__ ym, const double *__restrict__ zm, float4 *__restrict__ u) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * P) return;
+  const int i = (int)(t / P);
+  u[t] = make_float4((float)(x[t] - xm[i]), (float)(y[t] - ym[i]), (float)(z[t] - zm[i]), 0.0f);
+}
+
 // Bead-1 copy, largest displacement since the last list build
 // (check_ver_list, nbr_list_control.c:182-239) and bead spread
 // (get_pimd_spread, pimd_estimators.c:121-222).  Maxima of squares are
@@ -420,8 +433,8 @@ __device__ __forceinline__ float exp_r<float>(float x) {
 // energy (ev) and -(1/r) dV/dr (returned) of one pair at distance r, for
 // the LJ term (lj) and the erfc term with charge product q2; pp holds the
 // type pair's constants
-template <typename R>
-__device__ __forceinline__ R pair_direct(const PairArgs &a, const double *pp, R q2, R r, R rinv, R &vlj, R &vc) {
+template <typename R, typename PT = double>
+__device__ __forceinline__ R pair_direct(const PairArgs &a, const PT *pp, R q2, R r, R rinv, R &vlj, R &vc) {
   R sw = (R)1, dsw = (R)0;
   if (a.swit) {
     const R rheal = (R)a.rheal, irheal = (R)a.irheal;
@@ -435,7 +448,7 @@ __device__ __forceinline__ R pair_direct(const PairArgs &a, const double *pp, R 
   R dvt = (R)0;
   vlj = (R)0;
   vc = (R)0;
-  if (pp[PP_LJ] != 0.0) {
+  if (pp[PP_LJ] != (PT)0) {
     const R eps = (R)pp[PP_EPS];
     const R s6 = (R)pp[PP_SIG6] * (r2i * r2i * r2i);
     const R v = (R)4 * eps * s6 * (s6 - (R)1);
@@ -608,6 +621,139 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
   const double h = FULL ? 0.5 : 1.0;
   const double vals[PAIR_NV] = {s.vlj * h, s.vc * h, s.p11 * h, s.p22 * h, s.p33 * h,
                                 s.p12 * h, s.p13 * h, s.p23 * h, s.npr * h};
+  block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x);
+}
+
+// Orders the LDS writes of a wavefront before its other lanes read them
+// (and the reads before the next writes).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Mixed-precision pair kernel over the full list.  A wavefront owns site i;
+// its row is taken in chunks of 64 entries.  First every lane stages one
+// entry: the bead-independent part of the separation, the imaged centroid
+// separation D_ij (FP64: xmod_i - xmod_j - h nint(hi (xmod_i - xmod_j)),
+// period.c:121-206; without PIMD the imaged x_i - x_j itself), rounded to
+// FP32, and q_i q_j, into the wavefront's LDS slice.  Then the (slot, bead)
+// lanes evaluate the chunk's pairs in FP32: r = u_i - u_j + D_ij with the
+// bead displacements u = x - xmod, the LJ and erfc terms of pair_direct.
+// Energies, virial and forces are summed in FP32 within a chunk and in FP64
+// across chunks and lanes.
+template <int B>
+__global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
+  __shared__ float spp[MAX_PAIRS * PP_NUM];
+  __shared__ float4 sdq[4][64];
+  __shared__ int sent[4][64];
+  for (int k = threadIdx.x; k < a.npair * PP_NUM; k += blockDim.x) spp[k] = (float)a.pp[k];
+  __syncthreads();
+  constexpr int S = 64 / B;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int slot = lane / B;
+  const int bead = (lane % B) + blockIdx.y * B;
+  const bool bact = bead < a.P;
+  const int P = a.P;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
+  const Cell &c = a.cell;
+  double vlj = 0, vc = 0, p11 = 0, p22 = 0, p33 = 0, p12 = 0, p13 = 0, p23 = 0, npr = 0;
+  for (int i = wave; i < a.N; i += nwave) {
+    const size_t ib = (size_t)i * P + (bact ? bead : 0);
+    const double4 si = a.site[i];
+    const float4 ui = a.pimd ? a.u[ib] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int cnt = a.cnt[i];
+    const int *row = a.jlist + (size_t)i * a.cap;
+    double fxi = 0.0, fyi = 0.0, fzi = 0.0;
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const int ks = k0 + lane;
+      if (ks < cnt) {
+        const int e = row[ks];
+        const int j = e & ENTRY_MASK;
+        const double4 sj = a.site[j];
+        double dx, dy, dz;
+        if (a.pimd) {
+          min_image_sep(c, 1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, si.x, si.y, si.z, sj.x, sj.y, sj.z, dx, dy, dz);
+          dx += si.x - sj.x;
+          dy += si.y - sj.y;
+          dz += si.z - sj.z;
+        } else {
+          min_image_sep(c, 0, a.x[i], a.y[i], a.z[i], a.x[j], a.y[j], a.z[j], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dx,
+                        dy, dz);
+        }
+        sdq[w][lane] = make_float4((float)dx, (float)dy, (float)dz, (float)(si.w * sj.w));
+        sent[w][lane] = e;
+      }
+      wave_sync();
+      const int nk = min(64, cnt - k0);
+      float cvlj = 0.f, cvc = 0.f, c11 = 0.f, c22 = 0.f, c33 = 0.f, c12 = 0.f, c13 = 0.f, c23 = 0.f;
+      float cfx = 0.f, cfy = 0.f, cfz = 0.f;
+      int cn = 0;
+      const int bsel = bact ? bead : 0;
+      // the next pair's displacement is loaded while this pair is evaluated
+      float4 un = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (a.pimd && slot < nk) un = a.u[(size_t)(sent[w][slot] & ENTRY_MASK) * P + bsel];
+      for (int p = slot; p < nk; p += S) {
+        const float4 d = sdq[w][p];
+        const int e = sent[w][p];
+        const int t = e >> ENTRY_SHIFT;
+        const float *pp = spp + t * PP_NUM;
+        float dx = d.x, dy = d.y, dz = d.z;
+        if (a.pimd) {
+          const float4 uj = un;
+          if (p + S < nk) un = a.u[(size_t)(sent[w][p + S] & ENTRY_MASK) * P + bsel];
+          dx += ui.x - uj.x;
+          dy += ui.y - uj.y;
+          dz += ui.z - uj.z;
+        }
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const float cut = pp[PP_CUT];
+        if (!bact || !(r2 < cut * cut)) continue;
+        const float rinv = __frsqrt_rn(r2);
+        float ev_lj, ev_c;
+        const float dvd = pair_direct<float, float>(a, pp, d.w, r2 * rinv, rinv, ev_lj, ev_c);
+        const float fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
+        cvlj += ev_lj;
+        cvc += ev_c;
+        cfx += fx1;
+        cfy += fy1;
+        cfz += fz1;
+        c11 += dx * fx1;
+        c22 += dy * fy1;
+        c33 += dz * fz1;
+        c12 += dx * fy1;
+        c13 += dx * fz1;
+        c23 += dy * fz1;
+        cn++;
+      }
+      vlj += (double)cvlj;
+      vc += (double)cvc;
+      p11 += (double)c11;
+      p22 += (double)c22;
+      p33 += (double)c33;
+      p12 += (double)c12;
+      p13 += (double)c13;
+      p23 += (double)c23;
+      npr += (double)cn;
+      fxi += (double)cfx;
+      fyi += (double)cfy;
+      fzi += (double)cfz;
+      wave_sync();
+    }
+    for (int off = 32; off >= B; off >>= 1) {
+      fxi += __shfl_xor(fxi, off, 64);
+      fyi += __shfl_xor(fyi, off, 64);
+      fzi += __shfl_xor(fzi, off, 64);
+    }
+    if (slot == 0 && bact) {  // the first writer of this step's forces
+      a.fx[ib] = fxi;
+      a.fy[ib] = fyi;
+      a.fz[ib] = fzi;
+    }
+  }
+  const double vals[PAIR_NV] = {vlj * 0.5, vc * 0.5, p11 * 0.5, p22 * 0.5, p33 * 0.5,
+                                p12 * 0.5, p13 * 0.5, p23 * 0.5, npr * 0.5};
   block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x);
 }
 
@@ -1116,7 +1262,7 @@ void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_
   if (full) {
     if (mode == 0) PMX_PAIR_LAUNCH(0, true);
     else if (mode == 1) PMX_PAIR_LAUNCH(1, true);
-    else PMX_PAIR_LAUNCH(2, true);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B>), grid, block, 0, st, a);
   } else {
     if (mode == 0) PMX_PAIR_LAUNCH(0, false);
     else if (mode == 1) PMX_PAIR_LAUNCH(1, false);
