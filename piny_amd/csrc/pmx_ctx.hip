@@ -50,6 +50,7 @@ struct pmx_ctx {
   int pair_mode = 0;       // 0 FP64 tables, 1 FP64 direct, 2 FP32 direct
   bool full_list = true;   // force list with both directions (no atomics)
   int cap = 0, *jfull = nullptr, *jforce = nullptr, *nfull = nullptr, *nforce = nullptr, *overflow = nullptr;
+  int *jsplit = nullptr;  // the force list's second buffer (rows split by the cutoff)
   // link cells of the list build
   int *site_cell = nullptr, *site_pos = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_sites = nullptr;
   bool list_cells = false;  // the last build used link cells
@@ -155,14 +156,15 @@ unsigned blocks(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / 
 int alloc_lists(pmx_ctx *c, int cap) {
   const size_t n = (size_t)c->s.N * cap;
   if (c->jfull) {
-    (void)hipFree(c->jfull);
-    (void)hipFree(c->jforce);
-    c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->jfull), c->allocs.end());
-    c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->jforce), c->allocs.end());
+    for (int *p : {c->jfull, c->jforce, c->jsplit}) {
+      (void)hipFree(p);
+      c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)p), c->allocs.end());
+    }
   }
   c->cap = cap;
   int rc = dalloc(c, &c->jfull, n);
   if (!rc) rc = dalloc(c, &c->jforce, n);
+  if (!rc) rc = dalloc(c, &c->jsplit, n);
   return rc;
 }
 
@@ -226,6 +228,11 @@ int setup_pme_tiles(pmx_ctx *c) {
       a.H[d] = T[d] + n - 1;
     }
     a.hvol = (size_t)a.H[0] * a.H[1] * a.H[2];
+    auto magic = [](int d) { return (unsigned)((1ull << 32) / (unsigned)d + 1ull); };
+    a.mH0 = magic(a.H[0]);
+    a.mH1 = magic(a.H[1]);
+    a.mT0 = magic(a.T[0]);
+    a.mT1 = magic(a.T[1]);
     bool ok = true;
     for (int d = 0; d < 3; d++) a.nbr[d] = 1 + (n - 1 + a.T[d] - 1) / a.T[d];
     ok = ok && (size_t)a.nt[0] * a.nt[1] * a.nt[2] * sizeof(int) <= 65536;  // LDS histogram of k_pme_bin
@@ -612,6 +619,10 @@ int build_list(pmx_ctx *c) {
                            c->nfull, c->nforce, c->jfull, c->jforce);
         HIPCHK(hipGetLastError());
       }
+      // force-list rows: pairs inside the cutoff first
+      hipLaunchKernelGGL(k_split_rows, dim3(blocks((size_t)s.N * 64)), dim3(256), 0, c->st, a, c->jsplit);
+      HIPCHK(hipGetLastError());
+      std::swap(c->jforce, c->jsplit);
       c->list_cells = cells;
       c->nver = ov[1];
       return PMX_OK;

@@ -75,6 +75,9 @@ struct PmeArgs {
   int ntile;           // tiles per bead
   size_t gstride;      // doubles per bead grid
   size_t hvol;         // H[0]*H[1]*H[2]
+  // k / d as __umulhi(k, m) with m = 2^32 / d + 1 (exact for k * d < 2^32):
+  // for H[0], H[1], T[0], T[1]
+  unsigned mH0, mH1, mT0, mT1;
   const double *x, *y, *z;
   const double *q;
   const int *ichrg;
@@ -206,6 +209,7 @@ __global__ void k_pme_scan(PmeArgs a);
 __global__ void k_pme_scatter(PmeArgs a);
 __global__ void k_fill_int(int *p, int v, size_t n);
 __global__ void k_reduce_partials(RedArgs a);
+__global__ void k_split_rows(ListArgs a, int *out);
 __global__ void k_ecor_phi(IntraArgs a);
 __global__ void k_bond_moments(IntraArgs a);
 __global__ void k_bend_moments(IntraArgs a);

@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {
     const RedStage &r = a.st[si];
     for (int v = 0; v < r.nv; v++) {
       if (r.slot[v] != slot) continue;
-      for (int g = lane; g < r.G; g += 64) s += a.part[r.off + ((size_t)g * a.P + b) * r.nv + v];
+      for (int g = lane; g < r.G; g += 64) s += a.part[r.off + ((size_t)v * a.P + b) * r.G + g];
     }
   }
   for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -382,6 +382,51 @@ __global__ void __launch_bounds__(256) k_sort_rows(int cap, int n2, const int *n
   }
 }
 
+// Splits every force-list row, stably, into the entries whose pair lies
+// inside its cutoff at the build positions (the centroid reference xmod with
+// beads, bead 1 without), then the rest of the list shell, into out.  The
+// pair kernels evaluate a row's entries in order, several per wavefront at a
+// time; grouped like this, the pairs of one iteration mostly agree on the
+// cutoff test and the lanes of a wavefront seldom idle.  One wavefront per
+// row.
+__global__ void __launch_bounds__(256) k_split_rows(ListArgs a, int *out) {
+  const int lane = threadIdx.x & 63;
+  const int i = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  if (i >= a.N) return;  // uniform over the wavefront
+  const int cnt = a.nforce[i];
+  const int *row = a.jforce + (size_t)i * a.cap;
+  int *dst = out + (size_t)i * a.cap;
+  const double xi = a.pimd ? a.xm[i] : a.bx[i], yi = a.pimd ? a.ym[i] : a.by[i], zi = a.pimd ? a.zm[i] : a.bz[i];
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int ninner = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    int pin = 0, pout = ninner;
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const int k = k0 + lane;
+      const bool valid = k < cnt;
+      bool inner = false;
+      int e = 0;
+      if (valid) {
+        e = row[k];
+        const int j = e & ENTRY_MASK, t = e >> ENTRY_SHIFT;
+        const double xj = a.pimd ? a.xm[j] : a.bx[j], yj = a.pimd ? a.ym[j] : a.by[j],
+                     zj = a.pimd ? a.zm[j] : a.bz[j];
+        double dx, dy, dz;
+        const double r2 = min_image_sep(a.cell, 0, xi, yi, zi, xj, yj, zj, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dx, dy, dz);
+        inner = r2 < a.pt.cut[t] * a.pt.cut[t];
+      }
+      const uint64_t mi = __ballot(inner), mv = __ballot(valid);
+      if (pass == 0) {
+        ninner += __popcll(mi);
+      } else if (valid) {
+        dst[inner ? pin + __popcll(mi & below) : pout + __popcll(mv & ~mi & below)] = e;
+      }
+      pin += __popcll(mi);
+      pout += __popcll(mv & ~mi);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Real-space pair forces over the interacting list: force_npol
 // (force_npol.c:28-328) with shave_skin_opt on and RESPA off.  A wavefront
@@ -621,7 +666,7 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
   const double h = FULL ? 0.5 : 1.0;
   const double vals[PAIR_NV] = {s.vlj * h, s.vc * h, s.p11 * h, s.p22 * h, s.p33 * h,
                                 s.p12 * h, s.p13 * h, s.p23 * h, s.npr * h};
-  block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x);
+  block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
 }
 
 // Orders the LDS writes of a wavefront before its other lanes read them
@@ -754,7 +799,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   }
   const double vals[PAIR_NV] = {vlj * 0.5, vc * 0.5, p11 * 0.5, p22 * 0.5, p33 * 0.5,
                                 p12 * 0.5, p13 * 0.5, p23 * 0.5, npr * 0.5};
-  block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x);
+  block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -846,7 +891,7 @@ __global__ void __launch_bounds__(256) k_pair_grad(PairArgs a) {
     vals[q] = 0.5 * ge[q];
     vals[MAX_LJ_GRAD + q] = 0.5 * gs[q];
   }
-  block_partials<PAIRG_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x);
+  block_partials<PAIRG_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -928,7 +973,7 @@ __global__ void __launch_bounds__(256) k_ecor(IntraArgs a) {
     add_force(a, j, b, -fx, -fy, -fz);
     add_virial(acc, e, dx, dy, dz, fx, fy, fz);
   }
-  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x);
+  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);
 }
 
 // bond, bond.c:29-312
@@ -952,7 +997,7 @@ __global__ void __launch_bounds__(256) k_bond(IntraArgs a) {
     add_force(a, j, b, -fx, -fy, -fz);
     add_virial(acc, e, dx, dy, dz, fx, fy, fz);
   }
-  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x);
+  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);
 }
 
 // bend, bend.c:28-434
@@ -1000,7 +1045,7 @@ __global__ void __launch_bounds__(256) k_bend(IntraArgs a) {
     acc[5] += dx12 * f1z + dx23 * f3z;
     acc[6] += dy12 * f1z + dy23 * f3z;
   }
-  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x);
+  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1140,7 +1185,7 @@ __global__ void __launch_bounds__(256) k_pimd_finish(FinishArgs a) {
       c[9] -= dx * fx + dy * fy + dz * fz;
     }
   }
-  block_partials<FIN_NV>(c, a.P, 0, a.P, a.part, blockIdx.x);
+  block_partials<FIN_NV>(c, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);
 }
 
 // distrib_ghost_force, ghost_control.c:71-125

@@ -192,7 +192,9 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double *box = (double *)lds_raw;
   const int hvol = (int)a.hvol;
-  const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  int tl, b;
+  if (!xcd_unit_bead(a.ntile, a.P, tl, b)) return;  // uniform over the workgroup
+  const int tid = threadIdx.x;
   for (int k = tid; k < hvol; k += blockDim.x) box[k] = 0.0;
   const size_t bin = (size_t)b * a.ntile + tl;
   const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
@@ -230,7 +232,8 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
 // across the periodic boundary.
 template <typename T>
 __global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
-  const int tl = blockIdx.x, b = blockIdx.y;
+  int tl, b;
+  if (!xcd_unit_bead(a.ntile, a.P, tl, b)) return;
   const int tcoord[3] = {tl % a.nt[0], (tl / a.nt[0]) % a.nt[1], tl / (a.nt[0] * a.nt[1])};
   const int T0 = a.T[0], T1 = a.T[1], T01 = T0 * T1, vol = T01 * a.T[2];
   const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
@@ -238,20 +241,24 @@ __global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
   T *g = (T *)a.grid + (size_t)b * a.gstride;
   const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
   for (int k = threadIdx.x; k < vol; k += blockDim.x) {
-    const int l[3] = {k % T0, (k / T0) % T1, k / T01};
+    const int q0 = (int)__umulhi((unsigned)k, a.mT0), l2 = (int)__umulhi((unsigned)q0, a.mT1);
+    const int l[3] = {k - q0 * T0, q0 - l2 * T1, l2};
     T s = (T)0;
     for (int oc = 0; oc < a.nbr[2]; oc++) {
       const int lc = l[2] + oc * a.T[2];
       if (lc >= a.H[2]) break;
-      const int sc = ((tcoord[2] - oc) % a.nt[2] + a.nt[2]) % a.nt[2];
+      int sc = tcoord[2] - oc;
+      while (sc < 0) sc += a.nt[2];
       for (int ob = 0; ob < a.nbr[1]; ob++) {
         const int lb = l[1] + ob * T1;
         if (lb >= a.H[1]) break;
-        const int sb = ((tcoord[1] - ob) % a.nt[1] + a.nt[1]) % a.nt[1];
+        int sb = tcoord[1] - ob;
+        while (sb < 0) sb += a.nt[1];
         for (int oa = 0; oa < a.nbr[0]; oa++) {
           const int la = l[0] + oa * T0;
           if (la >= H0) break;
-          const int sa = ((tcoord[0] - oa) % a.nt[0] + a.nt[0]) % a.nt[0];
+          int sa = tcoord[0] - oa;
+          while (sa < 0) sa += a.nt[0];
           const size_t src = (size_t)(sc * a.nt[1] + sb) * a.nt[0] + sa;
           s += hb[src * a.hvol + (size_t)lc * H01 + lb * H0 + la];
         }
@@ -278,7 +285,8 @@ __device__ __forceinline__ void load_halo(const PmeArgs &a, int b, const int (&o
       const int k = k0 + u * blockDim.x;
       v[u] = (T)0;
       if (k < hvol) {
-        const int la = k % H0, lb = (k / H0) % H1, lc = k / H01;
+        const int q0 = (int)__umulhi((unsigned)k, a.mH0), lc = (int)__umulhi((unsigned)q0, a.mH1);
+        const int la = k - q0 * H0, lb = q0 - lc * H1;
         int ga = org[0] + la, gb = org[1] + lb, gc = org[2] + lc;
         while (ga >= Ka) ga -= Ka;
         while (gb >= Kb) gb -= Kb;
@@ -300,7 +308,9 @@ template <int NI, typename T>
 __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *box = (T *)lds_raw;
-  const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  int tl, b;
+  if (!xcd_unit_bead(a.ntile, a.P, tl, b)) return;
+  const int tid = threadIdx.x;
   const size_t bin = (size_t)b * a.ntile + tl;
   const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
   if (cnt == 0) return;  // uniform over the workgroup
@@ -355,7 +365,9 @@ template <int NI, typename T>
 __global__ void __launch_bounds__(PME_BLOCK) k_pme_phi(PmeArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *box = (T *)lds_raw;
-  const int tl = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  int tl, b;
+  if (!xcd_unit_bead(a.ntile, a.P, tl, b)) return;
+  const int tid = threadIdx.x;
   const size_t bin = (size_t)b * a.ntile + tl;
   const int lo = a.bin_off[bin], cnt = a.bin_cnt[bin];
   if (cnt == 0) return;  // uniform over the workgroup
@@ -396,7 +408,7 @@ size_t pme_lds_bytes(const PmeArgs &a, bool gather) {
 
 template <typename T>
 hipError_t launch_spread_t(const PmeArgs &a, hipStream_t st) {
-  const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
+  const dim3 grid(xcd_grid(a.ntile, a.P)), block(PME_BLOCK);
   const size_t lds = pme_lds_bytes(a, false);
   switch (a.n) {
 #define X(NI) \
@@ -405,13 +417,13 @@ hipError_t launch_spread_t(const PmeArgs &a, hipStream_t st) {
 #undef X
     default: return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_fold<T>), dim3(a.ntile, a.P), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_fold<T>), dim3(xcd_grid(a.ntile, a.P)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_gather_t(const PmeArgs &a, hipStream_t st) {
-  const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
+  const dim3 grid(xcd_grid(a.ntile, a.P)), block(PME_BLOCK);
   const size_t lds = pme_lds_bytes(a, true);
   switch (a.n) {
 #define X(NI) \
@@ -430,7 +442,7 @@ hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
 
 template <typename T>
 hipError_t launch_phi_t(const PmeArgs &a, hipStream_t st) {
-  const dim3 grid(a.ntile, a.P), block(PME_BLOCK);
+  const dim3 grid(xcd_grid(a.ntile, a.P)), block(PME_BLOCK);
   const size_t lds = pme_lds_bytes(a, true);
   switch (a.n) {
 #define X(NI) \
@@ -582,7 +594,8 @@ __global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
   using C = typename Cx<T>::C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *sm = (C *)lds_raw;
-  const int c = blockIdx.x, b = blockIdx.y;
+  int c, b;
+  if (!xcd_unit_bead(a.K[2], a.P, c, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
   const T *g = (const T *)a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
   for (int w0 = threadIdx.x; w0 < Ka * Kb; w0 += 8 * blockDim.x) {  // 8 loads in flight per lane
@@ -617,7 +630,8 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
   using C = typename Cx<T>::C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *sm = (C *)lds_raw;
-  const int bb = blockIdx.x, b = blockIdx.y;
+  int bb, b;
+  if (!xcd_unit_bead(a.K[1], a.P, bb, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
   C *sp = (C *)a.spec + (size_t)b * a.sstride;
   for (int w0 = threadIdx.x; w0 < Kc * Ha; w0 += 8 * blockDim.x) {
@@ -673,7 +687,7 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
   // the k-space pressure also carries the energy on its diagonal
   // (ewald3d_pme.c:412-414)
   const double vals[FFT_NV] = {e, v11 + e, v22 + e, v33 + e, v12, v13, v23};
-  block_partials<FFT_NV>(vals, 1, b, a.P, a.part, bb);
+  block_partials<FFT_NV>(vals, 1, b, a.P, a.part, bb, a.K[1]);
 }
 
 // inverse transform of one c-plane: along b, then complex-to-real along a.
@@ -682,7 +696,8 @@ __global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
   using C = typename Cx<T>::C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *sm = (C *)lds_raw;
-  const int c = blockIdx.x, b = blockIdx.y;
+  int c, b;
+  if (!xcd_unit_bead(a.K[2], a.P, c, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
   const C *sp = (const C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
   for (int w0 = threadIdx.x; w0 < Kb * Ka; w0 += 8 * blockDim.x) {
@@ -719,11 +734,11 @@ template <typename T>
 hipError_t launch_fft_t(const FftArgs &f, hipStream_t st) {
   using C = typename Cx<T>::C;
   const int Ka = f.K[0], Kb = f.K[1], Kc = f.K[2], Ha = Ka / 2 + 1;
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_fwd_ab<T>), dim3(Kc, f.P), dim3(256), (size_t)Ka * Kb * sizeof(C), st,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_fwd_ab<T>), dim3(xcd_grid(Kc, f.P)), dim3(256), (size_t)Ka * Kb * sizeof(C), st,
                      f);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_c_conv<T>), dim3(Kb, f.P), dim3(256), (size_t)Kc * Ha * sizeof(C), st,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_c_conv<T>), dim3(xcd_grid(Kb, f.P)), dim3(256), (size_t)Kc * Ha * sizeof(C), st,
                      f);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_inv_ba<T>), dim3(Kc, f.P), dim3(256), (size_t)Ka * Kb * sizeof(C), st,
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_inv_ba<T>), dim3(xcd_grid(Kc, f.P)), dim3(256), (size_t)Ka * Kb * sizeof(C), st,
                      f);
   return hipGetLastError();
 }
