@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PMX_ABI_VERSION 3
+#define PMX_ABI_VERSION 4
 
 /* return codes */
 enum {
@@ -303,6 +303,25 @@ typedef struct pmx_param_grad {
   double *energy, *dq, *deps, *dsig, *dbond_c, *dbond_eq, *dbend_c, *dbend_s;
 } pmx_param_grad;
 int pmx_get_param_grad(pmx_ctx *ctx, pmx_param_grad *out);
+
+/* A batch of contexts (independent boxes, each with its own beads) on one
+ * device, evaluated together: the list stage of every context, then one
+ * launch of a graph that holds every context's force sequence as a
+ * parallel branch, then one readback.  PINY evaluates replicas one instance
+ * at a time, each on its own host thread (python/piny/piny.pyx:426-436,
+ * prange over energy_control); here one call covers them all.  Each
+ * context keeps its own state and results (pmx_get_forces etc. work after
+ * the call).  flags as for pmx_energy_control; PMX_EVAL_PIMD is implied for
+ * contexts with P > 1 and PMX_EVAL_MODE_FORCES is ignored for P = 1.  out
+ * receives one pmx_energies per context, in the order given.  The graph is
+ * captured on the first call and again when flags change or a context's
+ * list grows.  A context must not be used by another call while its batch
+ * runs. */
+typedef struct pmx_batch pmx_batch;
+pmx_batch *pmx_batch_create(pmx_ctx *const *ctx, int n, char *err, size_t errlen);
+int pmx_batch_energy_control(pmx_batch *batch, unsigned flags, pmx_energies *out);
+const char *pmx_batch_last_error(const pmx_batch *batch);
+void pmx_batch_destroy(pmx_batch *batch);
 
 /* Coefficient helpers restating intra_coefs.c (units: Hartree, Bohr). */
 int pmx_bond_coef_harm(double fk, double *c7);

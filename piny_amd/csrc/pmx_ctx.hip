@@ -77,6 +77,10 @@ struct pmx_ctx {
   int g_intra = 1024;
   double *acc = nullptr, *ake = nullptr;
   std::vector<double> acc_h;
+  // pinned host staging of the asynchronous readbacks: list-check maxima,
+  // per-bead sums [P*ACC_NUM] followed by the spring energy
+  unsigned long long *mx_pin = nullptr;
+  double *acc_pin = nullptr;
   // springs
   double *prekf = nullptr, *veig = nullptr, *cs = nullptr;
   // state
@@ -95,6 +99,23 @@ struct pmx_ctx {
   int *ecs_off = nullptr, *ecs_num = nullptr, *ecs_j = nullptr;
   double *phi = nullptr, *gpart = nullptr, *gacc = nullptr, *gbond = nullptr, *gbend = nullptr;
   bool grad_ready = false;
+  // bumped whenever an argument captured by a force-sequence graph changes
+  // (list width and buffers, launch shapes)
+  unsigned epoch = 0;
+  hipEvent_t ready = nullptr;  // list stage done (batched calls)
+};
+
+struct pmx_batch {
+  std::string err;
+  int device = 0;
+  std::vector<pmx_ctx *> ctx;
+  hipStream_t cs = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> join;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  unsigned flags = 0;
+  std::vector<unsigned> epochs;
 };
 
 namespace {
@@ -162,6 +183,7 @@ int alloc_lists(pmx_ctx *c, int cap) {
     }
   }
   c->cap = cap;
+  c->epoch++;
   int rc = dalloc(c, &c->jfull, n);
   if (!rc) rc = dalloc(c, &c->jforce, n);
   if (!rc) rc = dalloc(c, &c->jsplit, n);
@@ -502,6 +524,8 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(dalloc(c, &c->acc, (size_t)P * ACC_NUM));
   TRY(dalloc(c, &c->ake, 1));
   c->acc_h.assign((size_t)P * ACC_NUM, 0.0);
+  HIPCHK(hipHostMalloc((void **)&c->mx_pin, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&c->acc_pin, ((size_t)P * ACC_NUM + 1) * sizeof(double), hipHostMallocDefault));
   TRY(upload(c, &c->prekf, s.prekf.data(), N));
   TRY(upload(c, &c->veig, s.veig.data(), s.veig.size()));
   {
@@ -622,7 +646,10 @@ int build_list(pmx_ctx *c) {
       // force-list rows: pairs inside the cutoff first
       hipLaunchKernelGGL(k_split_rows, dim3(blocks((size_t)s.N * 64)), dim3(256), 0, c->st, a, c->jsplit);
       HIPCHK(hipGetLastError());
-      std::swap(c->jforce, c->jsplit);
+      // copied back rather than swapped: captured force sequences keep
+      // pointing at jforce
+      HIPCHK(hipMemcpyAsync(c->jforce, c->jsplit, (size_t)s.N * c->cap * sizeof(int), hipMemcpyDeviceToDevice,
+                            c->st));
       c->list_cells = cells;
       c->nver = ov[1];
       return PMX_OK;
@@ -686,164 +713,40 @@ IntraArgs intra_args(pmx_ctx *c, bool pimd) {
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int pmx_abi_version(void) { return PMX_ABI_VERSION; }
-
-int pmx_device_count(void) {
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
-  return n;
-}
-
-pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *ix, const double *iy,
-                    const double *iz, int device, char *err, size_t errlen) {
-  auto fail = [&](const std::string &m) -> pmx_ctx * {
-    if (err && errlen) snprintf(err, errlen, "%s", m.c_str());
-    return nullptr;
-  };
-  if (!sys || !par) return fail("pmx_create: null argument");
-  if (sys->natm < 2 || sys->ntype < 1 || par->pi_beads < 1 || par->nsplin < 10)
-    return fail("pmx_create: invalid sizes");
-  if (par->pi_beads > 64) return fail("pmx_create: more than 64 beads per context is not supported");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("pmx_create: no HIP device");
-  if (device < 0 || device >= ndev) return fail("pmx_create: device index out of range");
-  pmx_ctx *c = new pmx_ctx();
-  c->device = device;
-  c->par = *par;
-  if (hipSetDevice(device) != hipSuccess) {
-    delete c;
-    return fail("pmx_create: hipSetDevice failed");
-  }
-  std::string m = build_setup(sys, par, ix, iy, iz, c->s);
-  if (!m.empty()) {
-    delete c;
-    return fail("pmx_create: " + m);
-  }
-  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    return fail("pmx_create: stream creation failed");
-  }
-  for (auto &e : c->ev) {
-    if (hipEventCreate(&e) != hipSuccess) {
-      pmx_destroy(c);
-      return fail("pmx_create: event creation failed");
-    }
-  }
-  if (build_device(c, sys) != PMX_OK) {
-    std::string e = c->err;
-    pmx_destroy(c);
-    return fail("pmx_create: " + e);
-  }
-  return c;
-}
-
-void pmx_destroy(pmx_ctx *c) {
-  if (!c) return;
-  (void)hipSetDevice(c->device);
-  if (c->st) (void)hipStreamSynchronize(c->st);
-  for (void *p : c->allocs) (void)hipFree(p);
-  for (auto &e : c->ev)
-    if (e) (void)hipEventDestroy(e);
-  if (c->st) (void)hipStreamDestroy(c->st);
-  delete c;
-}
-
-const char *pmx_last_error(const pmx_ctx *c) { return c ? c->err.c_str() : "null context"; }
-
-static int set_pos_common(pmx_ctx *c, const double *x, const double *y, const double *z, hipMemcpyKind kind) {
-  Setup &s = c->s;
-  const size_t NP = (size_t)s.N * s.P;
-  HIPCHK(hipSetDevice(c->device));
-  const double *src[3] = {x, y, z};
-  double *dst[3] = {c->x, c->y, c->z};
-  for (int d = 0; d < 3; d++) {
-    HIPCHK(hipMemcpyAsync(c->stage, src[d], NP * sizeof(double), kind, c->st));
-    hipLaunchKernelGGL(k_bead_to_atom, dim3(blocks(NP)), dim3(256), 0, c->st, s.N, s.P, c->stage, dst[d]);
-    HIPCHK(hipGetLastError());
-  }
-  if (c->nghost) {
-    hipLaunchKernelGGL(k_ghost_pos, dim3(blocks((size_t)c->nghost * s.P)), dim3(256), 0, c->st, c->nghost, s.P,
-                       c->gatom, c->gncomp, c->gcomp, c->gcoef, c->x, c->y, c->z);
-    HIPCHK(hipGetLastError());
-  }
-  HIPCHK(hipStreamSynchronize(c->st));
-  c->have_pos = true;
-  c->grad_ready = false;
-  return PMX_OK;
-}
-
-int pmx_set_positions(pmx_ctx *c, const double *x, const double *y, const double *z) {
-  if (!c || !x || !y || !z) return PMX_ERR_ARG;
-  return set_pos_common(c, x, y, z, hipMemcpyHostToDevice);
-}
-
-int pmx_set_positions_device(pmx_ctx *c, const double *x, const double *y, const double *z) {
-  if (!c || !x || !y || !z) return PMX_ERR_ARG;
-  return set_pos_common(c, x, y, z, hipMemcpyDeviceToDevice);
-}
-
-int pmx_set_xmod(pmx_ctx *c, const double *x, const double *y, const double *z) {
-  if (!c) return PMX_ERR_ARG;
-  if (!x) {
-    c->xmod_given = false;
-    c->grad_ready = false;
-    return PMX_OK;
-  }
-  const size_t n = (size_t)c->s.N * sizeof(double);
-  HIPCHK(hipMemcpy(c->xm, x, n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->ym, y, n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->zm, z, n, hipMemcpyHostToDevice));
-  c->xmod_given = true;
-  c->grad_ready = false;
-  return PMX_OK;
-}
-
-int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
-  if (!c || !out) return PMX_ERR_ARG;
-  if (!c->have_pos) {
-    set_err(c, "pmx_energy_control: no positions set");
-    return PMX_ERR_STATE;
-  }
+// The list stage of a force call, first half: centroid reference and list
+// check (nbr_list_control.c:32-118) launched on st, with the two maxima
+// copied to pinned host memory.  The caller waits for st before the second
+// half.
+int list_stage_launch(pmx_ctx *c, hipStream_t st, bool timing) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
-  const bool pimd = (flags & PMX_EVAL_PIMD) != 0;
-  if (P > 1 && !pimd) {
-    set_err(c, "pmx_energy_control: more than one bead needs PMX_EVAL_PIMD");
-    return PMX_ERR_ARG;
-  }
-  const size_t NP = (size_t)N * P;
-  HIPCHK(hipSetDevice(c->device));
-  memset(out, 0, sizeof(*out));
-  c->grad_ready = false;
-  hipStream_t st = c->st;
-  RedArgs red{};
-  HIPCHK(hipEventRecord(c->ev[0], st));
-  // --- initial (energy_control_initial.c:168-208)
-  if (!c->full_list) {  // with the full list the pair kernel stores every force first
-    HIPCHK(hipMemsetAsync(c->fx, 0, NP * sizeof(double), st));
-    HIPCHK(hipMemsetAsync(c->fy, 0, NP * sizeof(double), st));
-    HIPCHK(hipMemsetAsync(c->fz, 0, NP * sizeof(double), st));
-  }
-  HIPCHK(hipMemsetAsync(c->ake, 0, sizeof(double), st));
+  if (timing) HIPCHK(hipEventRecord(c->ev[0], st));
   if (P > 1 && !c->xmod_given) {
     hipLaunchKernelGGL(k_centroid, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm);
     HIPCHK(hipGetLastError());
   }
-  // --- neighbour list (nbr_list_control.c:32-118)
   HIPCHK(hipMemsetAsync(c->maxes, 0, 2 * sizeof(unsigned long long), st));
   hipLaunchKernelGGL(k_list_check, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->x0, c->y0, c->z0,
                      c->b0x, c->b0y, c->b0z, c->maxes);
   HIPCHK(hipGetLastError());
-  unsigned long long mx[2];
-  HIPCHK(hipMemcpyAsync(mx, c->maxes, sizeof mx, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpyAsync(c->mx_pin, c->maxes, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  return PMX_OK;
+}
+
+// Second half: the bead spread and cut+skin (get_cut_skin,
+// nbr_list_control.c:249-283), the skin criterion, and the list build when
+// it is due.
+int list_stage_finish(pmx_ctx *c, unsigned flags, pmx_energies *out) {
+  Setup &s = c->s;
+  const int N = s.N, P = s.P;
+  hipStream_t st = c->st;
   double dmax, smax;
-  memcpy(&dmax, &mx[0], sizeof(double));
-  memcpy(&smax, &mx[1], sizeof(double));
+  memcpy(&dmax, &c->mx_pin[0], sizeof(double));
+  memcpy(&smax, &c->mx_pin[1], sizeof(double));
   const double spread_now = (P > 1) ? sqrt(smax) : 0.0;
-  if (P > 1) {  // get_cut_skin with the current spread (nbr_list_control.c:249-283)
+  if (P > 1) {
     for (int t = 0; t < s.npair; t++) {
       const double cs = s.cutoff[t] + c->par.skin + spread_now;
       c->pt.cutskin2[t] = cs * cs;
@@ -866,7 +769,29 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   out->list_rebuilt = rebuild;
   out->nver = c->nver;
   out->spread = c->spread;
-  HIPCHK(hipEventRecord(c->ev[1], st));
+  return PMX_OK;
+}
+
+// The device work of a force call after the list stage, launched on st:
+// real space, k space, intramolecular terms, PIMD finish, ghosts, modes and
+// the reduction of the partial sums.  It makes no host synchronization and
+// no memset, so it can be captured into a graph.
+int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
+  Setup &s = c->s;
+  const int N = s.N, P = s.P;
+  const size_t NP = (size_t)N * P;
+  const bool pimd = (flags & PMX_EVAL_PIMD) != 0;
+  RedArgs red{};
+  if (timing) HIPCHK(hipEventRecord(c->ev[1], st));
+  // --- initial (energy_control_initial.c:168-208)
+  if (!c->full_list) {  // with the full list the pair kernel stores every force first
+    for (double *f : {c->fx, c->fy, c->fz}) {
+      hipLaunchKernelGGL(k_fill_double, dim3(blocks(NP)), dim3(256), 0, st, f, 0.0, NP);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, st, c->ake, 0.0, (size_t)1);
+  HIPCHK(hipGetLastError());
   // --- real space (force_control -> force_verlst -> force_npol)
   {
     hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
@@ -910,7 +835,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     add_stage(red, c->off_pair, pair_grid_x(N), PAIR_NV,
               {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR});
   }
-  HIPCHK(hipEventRecord(c->ev[2], st));
+  if (timing) HIPCHK(hipEventRecord(c->ev[2], st));
   // --- reciprocal space (energy_control_inter_recip.c:119-355)
   const int nchrg = (int)s.ichrg.size();
   if (c->pme && !(flags & PMX_EVAL_NO_RECIP)) {
@@ -971,7 +896,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     HIPCHK(hipGetLastError());
     intra_stage(c->off_ecor, G, ACC_VECOR);
   }
-  HIPCHK(hipEventRecord(c->ev[3], st));
+  if (timing) HIPCHK(hipEventRecord(c->ev[3], st));
   // --- intramolecular (energy_control_intra.c:33-331)
   if (c->nbond) {
     const unsigned G = igrid((size_t)c->nbond * P);
@@ -1019,9 +944,6 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     HIPCHK(hipMemcpyAsync(c->fyt, c->fy, NP * sizeof(double), hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(c->fzt, c->fz, NP * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
-  c->last_modes = modes;
-  c->forces_valid = true;
-  const bool grad_state = !(c->pme && (flags & PMX_EVAL_NO_RECIP));
   if (modes) {
     ModeArgs a{};
     a.N = N;
@@ -1045,12 +967,28 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   red.acc = c->acc;
   hipLaunchKernelGGL(k_reduce_partials, dim3(blocks((size_t)P * ACC_NUM * 64)), dim3(256), 0, st, red);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c->ev[4], st));
-  HIPCHK(hipMemcpyAsync(c->acc_h.data(), c->acc, c->acc_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  double ake = 0.0;
-  HIPCHK(hipMemcpyAsync(&ake, c->ake, sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  {
+  if (timing) HIPCHK(hipEventRecord(c->ev[4], st));
+  return PMX_OK;
+}
+
+// Readback of the per-bead sums into pinned host memory, launched on st.
+int readback_launch(pmx_ctx *c, hipStream_t st) {
+  HIPCHK(hipMemcpyAsync(c->acc_pin, c->acc, c->acc_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(c->acc_pin + c->acc_h.size(), c->ake, sizeof(double), hipMemcpyDeviceToHost, st));
+  return PMX_OK;
+}
+
+// Energies and pressure tensors from the per-bead sums, after the readback
+// has completed.
+void assemble(pmx_ctx *c, unsigned flags, pmx_energies *out, bool timing) {
+  Setup &s = c->s;
+  const int N = s.N, P = s.P;
+  const bool pimd = (flags & PMX_EVAL_PIMD) != 0;
+  const bool modes = pimd && (flags & PMX_EVAL_MODE_FORCES) && P > 1;
+  const int nchrg = (int)s.ichrg.size();
+  memcpy(c->acc_h.data(), c->acc_pin, c->acc_h.size() * sizeof(double));
+  const double ake = c->acc_pin[c->acc_h.size()];
+  if (timing) {
     float t;
     for (int k = 0; k < 4; k++) {
       (void)hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]);
@@ -1124,7 +1062,312 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
     out->kin_harm = ake;
     out->pi_ke_prim = 1.5 * ((double)(N - c->nghost)) / tau - ake;
   }
-  c->grad_ready = grad_state;
+  c->last_modes = modes;
+  c->forces_valid = true;
+  c->grad_ready = !(c->pme && (flags & PMX_EVAL_NO_RECIP));
+}
+
+int check_call(pmx_ctx *c, unsigned flags, pmx_energies *out, const char *who) {
+  if (!c || !out) return PMX_ERR_ARG;
+  if (!c->have_pos) {
+    set_err(c, "%s: no positions set", who);
+    return PMX_ERR_STATE;
+  }
+  if (c->s.P > 1 && !(flags & PMX_EVAL_PIMD)) {
+    set_err(c, "%s: more than one bead needs PMX_EVAL_PIMD", who);
+    return PMX_ERR_ARG;
+  }
+  return PMX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmx_abi_version(void) { return PMX_ABI_VERSION; }
+
+int pmx_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *ix, const double *iy,
+                    const double *iz, int device, char *err, size_t errlen) {
+  auto fail = [&](const std::string &m) -> pmx_ctx * {
+    if (err && errlen) snprintf(err, errlen, "%s", m.c_str());
+    return nullptr;
+  };
+  if (!sys || !par) return fail("pmx_create: null argument");
+  if (sys->natm < 2 || sys->ntype < 1 || par->pi_beads < 1 || par->nsplin < 10)
+    return fail("pmx_create: invalid sizes");
+  if (par->pi_beads > 64) return fail("pmx_create: more than 64 beads per context is not supported");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail("pmx_create: no HIP device");
+  if (device < 0 || device >= ndev) return fail("pmx_create: device index out of range");
+  pmx_ctx *c = new pmx_ctx();
+  c->device = device;
+  c->par = *par;
+  if (hipSetDevice(device) != hipSuccess) {
+    delete c;
+    return fail("pmx_create: hipSetDevice failed");
+  }
+  std::string m = build_setup(sys, par, ix, iy, iz, c->s);
+  if (!m.empty()) {
+    delete c;
+    return fail("pmx_create: " + m);
+  }
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail("pmx_create: stream creation failed");
+  }
+  for (auto &e : c->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      pmx_destroy(c);
+      return fail("pmx_create: event creation failed");
+    }
+  }
+  if (hipEventCreateWithFlags(&c->ready, hipEventDisableTiming) != hipSuccess) {
+    pmx_destroy(c);
+    return fail("pmx_create: event creation failed");
+  }
+  if (build_device(c, sys) != PMX_OK) {
+    std::string e = c->err;
+    pmx_destroy(c);
+    return fail("pmx_create: " + e);
+  }
+  return c;
+}
+
+void pmx_destroy(pmx_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->mx_pin) (void)hipHostFree(c->mx_pin);
+  if (c->acc_pin) (void)hipHostFree(c->acc_pin);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->ready) (void)hipEventDestroy(c->ready);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char *pmx_last_error(const pmx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+static int set_pos_common(pmx_ctx *c, const double *x, const double *y, const double *z, hipMemcpyKind kind) {
+  Setup &s = c->s;
+  const size_t NP = (size_t)s.N * s.P;
+  HIPCHK(hipSetDevice(c->device));
+  const double *src[3] = {x, y, z};
+  double *dst[3] = {c->x, c->y, c->z};
+  for (int d = 0; d < 3; d++) {
+    HIPCHK(hipMemcpyAsync(c->stage, src[d], NP * sizeof(double), kind, c->st));
+    hipLaunchKernelGGL(k_bead_to_atom, dim3(blocks(NP)), dim3(256), 0, c->st, s.N, s.P, c->stage, dst[d]);
+    HIPCHK(hipGetLastError());
+  }
+  if (c->nghost) {
+    hipLaunchKernelGGL(k_ghost_pos, dim3(blocks((size_t)c->nghost * s.P)), dim3(256), 0, c->st, c->nghost, s.P,
+                       c->gatom, c->gncomp, c->gcomp, c->gcoef, c->x, c->y, c->z);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  c->have_pos = true;
+  c->grad_ready = false;
+  return PMX_OK;
+}
+
+int pmx_set_positions(pmx_ctx *c, const double *x, const double *y, const double *z) {
+  if (!c || !x || !y || !z) return PMX_ERR_ARG;
+  return set_pos_common(c, x, y, z, hipMemcpyHostToDevice);
+}
+
+int pmx_set_positions_device(pmx_ctx *c, const double *x, const double *y, const double *z) {
+  if (!c || !x || !y || !z) return PMX_ERR_ARG;
+  return set_pos_common(c, x, y, z, hipMemcpyDeviceToDevice);
+}
+
+int pmx_set_xmod(pmx_ctx *c, const double *x, const double *y, const double *z) {
+  if (!c) return PMX_ERR_ARG;
+  if (!x) {
+    c->xmod_given = false;
+    c->grad_ready = false;
+    return PMX_OK;
+  }
+  const size_t n = (size_t)c->s.N * sizeof(double);
+  HIPCHK(hipMemcpy(c->xm, x, n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->ym, y, n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->zm, z, n, hipMemcpyHostToDevice));
+  c->xmod_given = true;
+  c->grad_ready = false;
+  return PMX_OK;
+}
+
+int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
+  int rc = check_call(c, flags, out, "pmx_energy_control");
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  memset(out, 0, sizeof(*out));
+  c->grad_ready = false;
+  hipStream_t st = c->st;
+  if ((rc = list_stage_launch(c, st, true))) return rc;
+  HIPCHK(hipStreamSynchronize(st));
+  if ((rc = list_stage_finish(c, flags, out))) return rc;
+  if ((rc = force_sequence(c, flags, st, true))) return rc;
+  if ((rc = readback_launch(c, st))) return rc;
+  HIPCHK(hipStreamSynchronize(st));
+  assemble(c, flags, out, true);
+  return PMX_OK;
+}
+
+// --- batches of contexts --------------------------------------------------------
+
+static void batch_release_graph(pmx_batch *b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+pmx_batch *pmx_batch_create(pmx_ctx *const *ctx, int n, char *err, size_t errlen) {
+  auto fail = [&](const char *m, pmx_batch *b) -> pmx_batch * {
+    if (err && errlen) snprintf(err, errlen, "pmx_batch_create: %s", m);
+    if (b) pmx_batch_destroy(b);
+    return nullptr;
+  };
+  if (!ctx || n < 1) return fail("no contexts", nullptr);
+  for (int i = 0; i < n; i++) {
+    if (!ctx[i]) return fail("null context", nullptr);
+    if (ctx[i]->device != ctx[0]->device) return fail("the contexts of a batch share one device", nullptr);
+    for (int k = 0; k < i; k++)
+      if (ctx[k] == ctx[i]) return fail("a context appears twice", nullptr);
+  }
+  pmx_batch *b = new pmx_batch();
+  b->device = ctx[0]->device;
+  b->ctx.assign(ctx, ctx + n);
+  if (hipSetDevice(b->device) != hipSuccess) return fail("hipSetDevice failed", b);
+  if (hipStreamCreateWithFlags(&b->cs, hipStreamNonBlocking) != hipSuccess) return fail("stream creation failed", b);
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess) return fail("event creation failed", b);
+  b->join.assign(n, nullptr);
+  for (auto &e : b->join)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return fail("event creation failed", b);
+  return b;
+}
+
+void pmx_batch_destroy(pmx_batch *b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->cs) (void)hipStreamSynchronize(b->cs);
+  batch_release_graph(b);
+  for (auto e : b->join)
+    if (e) (void)hipEventDestroy(e);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->cs) (void)hipStreamDestroy(b->cs);
+  delete b;
+}
+
+const char *pmx_batch_last_error(const pmx_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+static unsigned batch_flags(const pmx_ctx *c, unsigned flags) {
+  return c->s.P > 1 ? (flags | PMX_EVAL_PIMD) : (flags & ~(unsigned)PMX_EVAL_MODE_FORCES);
+}
+
+// Captures every context's force sequence as a parallel branch of one graph.
+static int batch_capture(pmx_batch *b, unsigned flags) {
+  batch_release_graph(b);
+  if (hipStreamBeginCapture(b->cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    b->err = "graph capture could not begin";
+    return PMX_ERR_DEVICE;
+  }
+  int rc = PMX_OK;
+  hipError_t e = hipEventRecord(b->fork, b->cs);
+  for (size_t i = 0; i < b->ctx.size() && e == hipSuccess && rc == PMX_OK; i++) {
+    pmx_ctx *c = b->ctx[i];
+    e = hipStreamWaitEvent(c->st, b->fork, 0);
+    if (e != hipSuccess) break;
+    rc = force_sequence(c, batch_flags(c, flags), c->st, false);
+    if (rc) {
+      b->err = "context " + std::to_string(i) + ": " + c->err;
+      break;
+    }
+    e = hipEventRecord(b->join[i], c->st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->cs, b->join[i], 0);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t ee = hipStreamEndCapture(b->cs, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess || ee != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    b->err = std::string("graph capture failed: ") + hipGetErrorString(e != hipSuccess ? e : ee);
+    return PMX_ERR_DEVICE;
+  }
+  b->graph = g;
+  e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    b->exec = nullptr;
+    b->err = std::string("graph instantiation failed: ") + hipGetErrorString(e);
+    return PMX_ERR_DEVICE;
+  }
+  b->flags = flags;
+  b->epochs.resize(b->ctx.size());
+  for (size_t i = 0; i < b->ctx.size(); i++) b->epochs[i] = b->ctx[i]->epoch;
+  return PMX_OK;
+}
+
+int pmx_batch_energy_control(pmx_batch *b, unsigned flags, pmx_energies *out) {
+  if (!b || !out) return PMX_ERR_ARG;
+  const size_t n = b->ctx.size();
+  for (size_t i = 0; i < n; i++) {
+    pmx_ctx *c = b->ctx[i];
+    if (!c->have_pos) {
+      b->err = "context " + std::to_string(i) + " has no positions";
+      return PMX_ERR_STATE;
+    }
+  }
+  if (hipSetDevice(b->device) != hipSuccess) return PMX_ERR_DEVICE;
+  auto dev_fail = [&](hipError_t e, const char *what) {
+    b->err = std::string(what) + ": " + hipGetErrorString(e);
+    return PMX_ERR_DEVICE;
+  };
+  // list stages: all checks in flight, then the decisions and rebuilds
+  for (size_t i = 0; i < n; i++) {
+    pmx_ctx *c = b->ctx[i];
+    memset(&out[i], 0, sizeof(pmx_energies));
+    c->grad_ready = false;
+    if (int rc = list_stage_launch(c, c->st, false)) {
+      b->err = "context " + std::to_string(i) + ": " + c->err;
+      return rc;
+    }
+  }
+  for (size_t i = 0; i < n; i++) {
+    pmx_ctx *c = b->ctx[i];
+    hipError_t e = hipStreamSynchronize(c->st);
+    if (e != hipSuccess) return dev_fail(e, "list check");
+    if (int rc = list_stage_finish(c, batch_flags(c, flags), &out[i])) {
+      b->err = "context " + std::to_string(i) + ": " + c->err;
+      return rc;
+    }
+    e = hipEventRecord(c->ready, c->st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b->cs, c->ready, 0);
+    if (e != hipSuccess) return dev_fail(e, "list stage");
+  }
+  bool recapture = !b->exec || b->flags != flags;
+  for (size_t i = 0; i < n && !recapture; i++) recapture = b->epochs[i] != b->ctx[i]->epoch;
+  if (recapture)
+    if (int rc = batch_capture(b, flags)) return rc;
+  hipError_t e = hipGraphLaunch(b->exec, b->cs);
+  if (e != hipSuccess) return dev_fail(e, "graph launch");
+  for (size_t i = 0; i < n; i++)
+    if (int rc = readback_launch(b->ctx[i], b->cs)) {
+      b->err = "context " + std::to_string(i) + ": " + b->ctx[i]->err;
+      return rc;
+    }
+  e = hipStreamSynchronize(b->cs);
+  if (e != hipSuccess) return dev_fail(e, "batched force call");
+  for (size_t i = 0; i < n; i++) assemble(b->ctx[i], batch_flags(b->ctx[i], flags), &out[i], false);
   return PMX_OK;
 }
 

@@ -208,6 +208,7 @@ __global__ void k_pme_bin(PmeArgs a);
 __global__ void k_pme_scan(PmeArgs a);
 __global__ void k_pme_scatter(PmeArgs a);
 __global__ void k_fill_int(int *p, int v, size_t n);
+__global__ void k_fill_double(double *p, double v, size_t n);
 __global__ void k_reduce_partials(RedArgs a);
 __global__ void k_split_rows(ListArgs a, int *out);
 __global__ void k_ecor_phi(IntraArgs a);

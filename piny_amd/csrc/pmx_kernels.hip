@@ -20,6 +20,13 @@ __global__ void k_fill_int(int *p, int v, size_t n) {
   if (t < n) p[t] = v;
 }
 
+// a fill kernel instead of hipMemsetAsync, which must not be recorded into
+// graphs
+__global__ void k_fill_double(double *p, double v, size_t n) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) p[t] = v;
+}
+
 // Sums the partial rows of every stage into the per-bead accumulators, one
 // wavefront per (bead, slot); blocks are summed in a fixed order.
 __global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {

@@ -66,7 +66,13 @@ def lib():
     L.pmx_bend_coef_harm.argtypes = [C.c_double, C.c_double, dp, dp]
     L.pmx_set_force_field.argtypes = [C.c_void_p, C.POINTER(abi.PmxSystem)]
     L.pmx_get_param_grad.argtypes = [C.c_void_p, C.POINTER(abi.PmxParamGrad)]
-    if L.pmx_abi_version() != 3:
+    L.pmx_batch_create.restype = C.c_void_p
+    L.pmx_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_char_p, C.c_size_t]
+    L.pmx_batch_energy_control.argtypes = [C.c_void_p, C.c_uint, C.POINTER(abi.PmxEnergies)]
+    L.pmx_batch_last_error.restype = C.c_char_p
+    L.pmx_batch_last_error.argtypes = [C.c_void_p]
+    L.pmx_batch_destroy.argtypes = [C.c_void_p]
+    if L.pmx_abi_version() != 4:
         raise PinyAmdError("libpiny_amd.so ABI version mismatch")
     _lib = L
     return L
@@ -79,7 +85,8 @@ EXPORTED = ("pmx_abi_version", "pmx_device_count", "pmx_create", "pmx_destroy", 
             "pmx_get_verlet_list", "pmx_get_nktot", "pmx_get_kvectors", "pmx_get_spline",
             "pmx_get_setup_scalars", "pmx_get_timings", "pmx_get_pair_count", "pmx_set_velocities",
             "pmx_get_velocities", "pmx_get_positions", "pmx_md_nve", "pmx_bond_coef_harm", "pmx_bond_coef_morse",
-            "pmx_bend_coef_harm", "pmx_set_force_field", "pmx_get_param_grad")
+            "pmx_bend_coef_harm", "pmx_set_force_field", "pmx_get_param_grad", "pmx_batch_create",
+            "pmx_batch_energy_control", "pmx_batch_last_error", "pmx_batch_destroy")
 
 
 class Engine:
@@ -250,3 +257,33 @@ class Engine:
         out = np.zeros(5)
         self._chk(lib().pmx_get_timings(self.h, abi.dptr(out)), "pmx_get_timings")
         return dict(zip(("list", "real", "recip", "intra", "total"), out))
+
+
+class Batch:
+    """Several engines (boxes) on one device evaluated by one graph launch per
+    call (pmx_batch_energy_control)."""
+
+    def __init__(self, engines):
+        L = lib()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        err = C.create_string_buffer(1024)
+        self.h = L.pmx_batch_create(arr, len(self.engines), err, 1024)
+        if not self.h:
+            raise PinyAmdError(err.value.decode())
+
+    def energy_control(self, flags=0):
+        out = (abi.PmxEnergies * len(self.engines))()
+        rc = lib().pmx_batch_energy_control(self.h, flags, out)
+        if rc != 0:
+            msg = lib().pmx_batch_last_error(self.h)
+            raise PinyAmdError(f"pmx_batch_energy_control failed ({rc}): {msg.decode() if msg else ''}")
+        return [e.as_dict() for e in out]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pmx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -384,6 +384,31 @@ def config4(settings: Settings | None = None, P: int = 8, sample: int = 0) -> Co
     return build_config(Q_TIP4P_F, 1024, 31.28, P, SEED_BASE + 4 + 100 * sample, st)
 
 
+# radixme's table of FFT-friendly sizes (set_cp_ewald.c:898-947)
+RADIX_SIZES = (4, 8, 12, 16, 20, 24, 32, 36, 40, 48, 60, 64, 72, 80, 96, 100, 108, 120, 128, 144, 160, 180,
+               192, 200, 216, 240, 256)
+
+
+def config5(nbox: int = 64, seed: int = SEED_BASE + 5, max_mol: int = 4096) -> list:
+    """64 SPC/Fw boxes at 0.997 g/cm^3: molecule counts log-uniform in
+    [64, max_mol], bead counts from {1, 4, 8, 16, 32, 64}, per-box cutoff
+    min(9 A, L/2 - 0.05 A), PME grid about L / 1 A snapped to the radix
+    table (kmax_pme chosen so that PINY's sizing gives that grid), order 6,
+    alpha = 3.5 / r_c.  Seeded; box k uses seed + 1000 k."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nmol = np.exp(rng.uniform(np.log(64), np.log(max_mol), nbox)).astype(int)
+    beads = rng.choice(np.array([1, 4, 8, 16, 32, 64]), nbox)
+    out = []
+    for k in range(nbox):
+        L = density_box(int(nmol[k]))
+        rc = min(9.0, 0.5 * L - 0.05)
+        K = min(RADIX_SIZES, key=lambda r: abs(r - L))
+        kp = max(2, 2 * (K // 4 - 1))
+        st = Settings(cutoff=rc, alpha=3.5 / rc, kmax_ewd=min(kp, K // 2 - 2), kmax_pme=kp)
+        out.append(build_config(SPC_FW, int(nmol[k]), L, int(beads[k]), seed + 1000 * k, st))
+    return out
+
+
 def small_config(model: WaterModel = SPC_FW, nmol: int = 27, P: int = 1, seed: int = 7,
                  box: float | None = None, cutoff: float | None = None,
                  kmax: int | None = None) -> Config:
