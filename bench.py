@@ -119,13 +119,19 @@ def run_batch(steps, warmup, nbox=64):
     cfgs = w.config5(nbox=nbox)
     evals = sum(c.system.pi_beads for c in cfgs)
     res = {}
-    for name, prec, batched in (("sequential_fp64", abi.PMX_PREC_FP64, False),
-                                ("batched_fp64", abi.PMX_PREC_FP64, True),
-                                ("batched_mixed", abi.PMX_PREC_MIXED, True)):
+    tuned = {}
+    for name, prec, batched, tune in (("sequential_fp64", abi.PMX_PREC_FP64, False, False),
+                                      ("batched_fp64", abi.PMX_PREC_FP64, True, False),
+                                      ("batched_mixed", abi.PMX_PREC_MIXED, True, False),
+                                      ("batched_mixed_tuned", abi.PMX_PREC_MIXED, True, True)):
         engs = [Engine(c.system, c.x, c.y, c.z, precision=prec) for c in cfgs]
         for e, c in zip(engs, cfgs):
             e.set_positions(c.x, c.y, c.z)
         b = Batch(engs) if batched else None
+        if tune:  # the batch autotuner picks the launch shapes first
+            t0 = time.perf_counter()
+            tuned = b.autotune(reps=2)
+            tuned["seconds"] = time.perf_counter() - t0
 
         def step():
             if b is not None:
@@ -144,14 +150,16 @@ def run_batch(steps, warmup, nbox=64):
             b.close()
         for e in engs:
             e.close()
+    head = max(("batched_mixed", "batched_mixed_tuned"), key=lambda k: res[k]["value"])
     return {"metric": "bead-force evals/s, 64 uneven SPC/Fw boxes (64-4096 molecules, 1-64 beads), one batched "
-                      "graph launch per step", "value": res["batched_mixed"]["value"], "unit": "bead-force evals/s",
-            "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": res["batched_mixed"]["ms_per_step"],
+                      "graph launch per step", "value": res[head]["value"], "unit": "bead-force evals/s",
+            "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": res[head]["ms_per_step"],
+            "autotune": tuned,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
             "data": "synthetic",
             "config": {"workload": f"{nbox} SPC/Fw boxes at 0.997 g/cm^3, seeded", "boxes": nbox,
                        "bead_evals_per_step": evals, "sites_x_beads": int(sum(c.x.size for c in cfgs))},
-            "paths": res,
+            "paths": res, "path": head,
             "batched_vs_sequential_fp64": res["batched_fp64"]["value"] / res["sequential_fp64"]["value"]}
 
 

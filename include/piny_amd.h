@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PMX_ABI_VERSION 4
+#define PMX_ABI_VERSION 6
 
 /* return codes */
 enum {
@@ -322,6 +322,21 @@ pmx_batch *pmx_batch_create(pmx_ctx *const *ctx, int n, char *err, size_t errlen
 int pmx_batch_energy_control(pmx_batch *batch, unsigned flags, pmx_energies *out);
 const char *pmx_batch_last_error(const pmx_batch *batch);
 void pmx_batch_destroy(pmx_batch *batch);
+
+/* Autotuner of the launch shapes (no counterpart in the reference, which
+ * has no device): times reps force calls at the current positions for each
+ * candidate of the pair kernels' wavefront count, the PME tile sizes and the
+ * list build (plain scan or link cells), keeps the fastest of each, and
+ * ends with a force call using them.  choice (may be NULL) receives
+ * [tile a, tile b, tile c, pair wavefronts, list build 1 plain / 2 cells].
+ * Results are unchanged by the shapes up to the order of sums. */
+int pmx_autotune(pmx_ctx *ctx, unsigned flags, int reps, int *choice);
+
+/* The autotuner for a batch: the boxes share the device, so candidates are
+ * timed as whole batched calls, each applied to every context: the pair
+ * kernels' wavefront cap, then the PME tile caps.  choice (may be NULL)
+ * receives [pair wavefronts, tile cap a, b, c].  Ends with a batched call. */
+int pmx_batch_autotune(pmx_batch *batch, unsigned flags, int reps, int *choice);
 
 /* Coefficient helpers restating intra_coefs.c (units: Hartree, Bohr). */
 int pmx_bond_coef_harm(double fk, double *c7);

@@ -17,6 +17,8 @@
 #include <string>
 #include <vector>
 
+#include <chrono>
+
 #include "pmx_internal.h"
 #include "pmx_kernels.h"
 
@@ -102,6 +104,12 @@ struct pmx_ctx {
   // bumped whenever an argument captured by a force-sequence graph changes
   // (list width and buffers, launch shapes)
   unsigned epoch = 0;
+  // launch shapes (set at creation, changed by pmx_autotune): PME tile
+  // caps, wavefronts of the pair kernels, list build (0 by size, 1 plain
+  // scan, 2 link cells)
+  int tile_cap[3] = {16, 16, 8};
+  int pair_waves = 8192;
+  int list_mode = 0;
   hipEvent_t ready = nullptr;  // list stage done (batched calls)
 };
 
@@ -172,6 +180,14 @@ int fft_factor(int k, int *radix) {
   return k == 1 ? n : -1;
 }
 
+template <typename T>
+void dfree(pmx_ctx *c, T *&p) {
+  if (!p) return;
+  (void)hipFree((void *)p);
+  c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)p), c->allocs.end());
+  p = nullptr;
+}
+
 unsigned blocks(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 int alloc_lists(pmx_ctx *c, int cap) {
@@ -232,9 +248,7 @@ int setup_pme_tiles(pmx_ctx *c) {
   a.f32 = c->pme_f32;
   // tiles divide the grid, so every tile is whole and a support of n points
   // starting in a tile reaches at most ceil((n-1)/T) tiles beyond it
-  int cap[3] = {16, 16, 8};
-  const char *env = getenv("PMX_PME_TILE");  // tuning override "Ta,Tb,Tc" (upper bounds)
-  if (env) sscanf(env, "%d,%d,%d", &cap[0], &cap[1], &cap[2]);
+  int cap[3] = {c->tile_cap[0], c->tile_cap[1], c->tile_cap[2]};
   auto divisor_below = [](int K, int m) {
     int t = std::max(1, std::min(m, K));
     while (K % t) t--;
@@ -273,11 +287,16 @@ int setup_pme_tiles(pmx_ctx *c) {
   a.ntile = a.nt[0] * a.nt[1] * a.nt[2];
   a.gstride = c->gstride;
   const size_t nitem = (size_t)s.P * a.nchrg;
-  TRY_RC(dalloc(c, &c->item_s, 3 * nitem));
-  TRY_RC(dalloc(c, &c->item_f, 3 * nitem));
-  TRY_RC(dalloc(c, &c->item_tile, nitem));
-  TRY_RC(dalloc(c, &c->item_pos, nitem));
-  TRY_RC(dalloc(c, &c->sorted, nitem));
+  if (!c->item_s) {
+    TRY_RC(dalloc(c, &c->item_s, 3 * nitem));
+    TRY_RC(dalloc(c, &c->item_f, 3 * nitem));
+    TRY_RC(dalloc(c, &c->item_tile, nitem));
+    TRY_RC(dalloc(c, &c->item_pos, nitem));
+    TRY_RC(dalloc(c, &c->sorted, nitem));
+  }
+  dfree(c, c->bin_cnt);  // a second tiling (autotuner) replaces the tile-sized buffers
+  dfree(c, c->bin_off);
+  dfree(c, c->hbuf);
   TRY_RC(dalloc(c, &c->bin_cnt, (size_t)s.P * a.ntile));
   TRY_RC(dalloc(c, &c->bin_off, (size_t)s.P * a.ntile));
   TRY_RC(dalloc(c, &c->hbuf, (size_t)s.P * a.ntile * a.hvol));
@@ -509,7 +528,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(setup_pme_tiles(c));
   }
   {
-    const size_t pair = (size_t)pair_grid_x(N) * P * PAIR_NV;
+    const size_t pair = (size_t)pair_grid_x(N, PAIR_MAX_WAVES) * P * PAIR_NV;
     const size_t fft = (size_t)(c->pme ? s.ngrid[1] : 0) * P * FFT_NV;
     const size_t intra = (size_t)c->g_intra * P * INTRA_NV;
     const size_t fin = (size_t)c->g_intra * P * FIN_NV;
@@ -563,7 +582,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
 // the rows would not fit the sort), and the ascending O(N^2) scan is used.
 bool choose_cells(pmx_ctx *c, int nc[3], int *n2) {
   const char *env = getenv("PMX_LIST");  // tuning override: "brute" or "cells"
-  if (env && strcmp(env, "brute") == 0) return false;
+  if ((env && strcmp(env, "brute") == 0) || c->list_mode == 1) return false;
   double rl = 0.0;
   for (int t = 0; t < c->s.npair; t++) rl = std::max(rl, sqrt(c->pt.cutskin2[t]));
   int ncell = 1;
@@ -583,7 +602,7 @@ bool choose_cells(pmx_ctx *c, int nc[3], int *n2) {
   // (q-TIP4P/F at 0.997 g/cm^3, r_c + skin = 10 A): 16384 sites 1.2 ms both
   // ways, 32768 sites 4.0 ms plain vs 1.65 ms with cells, 65536 sites
   // 13.8 ms vs 3.4 ms.
-  return (ncell >= 64 && c->s.N > 16384) || (env && strcmp(env, "cells") == 0);
+  return (ncell >= 64 && c->s.N > 16384) || (env && strcmp(env, "cells") == 0) || c->list_mode == 2;
 }
 
 // (Re)build the Verlet list from the bead-1 copy; grows the ELL width when a
@@ -809,6 +828,7 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.nsplin = s.nsplin;
     a.pimd = (P > 1);
     a.npair = s.npair;
+    a.gx = pair_grid_x(N, c->pair_waves);
     a.alp = s.alp_ewd;
     a.palp = kErfP * s.alp_ewd;
     a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
@@ -832,7 +852,7 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.fz = c->fz;
     a.part = c->part + c->off_pair;
     HIPCHK(launch_pair(a, c->pair_mode, c->full_list, st));
-    add_stage(red, c->off_pair, pair_grid_x(N), PAIR_NV,
+    add_stage(red, c->off_pair, pair_grid_x(N, c->pair_waves), PAIR_NV,
               {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR});
   }
   if (timing) HIPCHK(hipEventRecord(c->ev[2], st));
@@ -1131,6 +1151,8 @@ pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *
     pmx_destroy(c);
     return fail("pmx_create: event creation failed");
   }
+  if (const char *env = getenv("PMX_PME_TILE"))  // tuning override "Ta,Tb,Tc" (upper bounds)
+    sscanf(env, "%d,%d,%d", &c->tile_cap[0], &c->tile_cap[1], &c->tile_cap[2]);
   if (build_device(c, sys) != PMX_OK) {
     std::string e = c->err;
     pmx_destroy(c);
@@ -1371,6 +1393,164 @@ int pmx_batch_energy_control(pmx_batch *b, unsigned flags, pmx_energies *out) {
   return PMX_OK;
 }
 
+// Launch shapes picked by measurement (the batching path's autotuner): the
+// pair kernels' wavefront count, the PME tile caps and the list build, each
+// in turn, timed with the stage events of reps force calls at the current
+// positions (one untimed call first).  The force call's results at the end
+// are those of the chosen shapes.
+int pmx_autotune(pmx_ctx *c, unsigned flags, int reps, int *choice) {
+  if (!c) return PMX_ERR_ARG;
+  if (!c->have_pos) {
+    set_err(c, "pmx_autotune: no positions set");
+    return PMX_ERR_STATE;
+  }
+  if (reps < 1) reps = 1;
+  pmx_energies e;
+  auto stage_ms = [&](int stage, unsigned f, double *ms) -> int {
+    int rc = pmx_energy_control(c, f, &e);
+    if (rc) return rc;
+    double t = 0.0;
+    for (int r = 0; r < reps; r++) {
+      if ((rc = pmx_energy_control(c, f, &e))) return rc;
+      t += c->ms[stage];
+    }
+    *ms = t / reps;
+    return PMX_OK;
+  };
+  int rc = PMX_OK;
+  // pair kernels: wavefronts (stage 1, real space)
+  {
+    const int cand[] = {1024, 2048, 4096, 8192, 16384};
+    int best = c->pair_waves;
+    double tbest = 1e300;
+    for (int w : cand) {
+      if (w > PAIR_MAX_WAVES || (w > c->s.N && w != cand[0])) continue;
+      c->pair_waves = w;
+      double t;
+      if ((rc = stage_ms(1, flags, &t))) return rc;
+      if (t < tbest) {
+        tbest = t;
+        best = w;
+      }
+    }
+    c->pair_waves = best;
+  }
+  // PME tiles (stage 2, k space)
+  if (c->pme) {
+    const int cand[][3] = {{16, 16, 8}, {8, 8, 8}, {16, 8, 8}, {16, 16, 4}, {32, 16, 8}, {32, 8, 8}, {8, 8, 4}};
+    int best[3] = {c->tile_cap[0], c->tile_cap[1], c->tile_cap[2]};
+    double tbest = 1e300;
+    for (const auto &cap : cand) {
+      for (int d = 0; d < 3; d++) c->tile_cap[d] = cap[d];
+      if (setup_pme_tiles(c) != PMX_OK) continue;
+      double t;
+      if ((rc = stage_ms(2, flags, &t))) return rc;
+      if (t < tbest) {
+        tbest = t;
+        for (int d = 0; d < 3; d++) best[d] = cap[d];
+      }
+    }
+    for (int d = 0; d < 3; d++) c->tile_cap[d] = best[d];
+    if ((rc = setup_pme_tiles(c))) return rc;
+  }
+  // list build (stage 0, with the list rebuilt every call)
+  {
+    int nc[3], n2;
+    c->list_mode = 2;
+    const bool cells_ok = choose_cells(c, nc, &n2);
+    double tb = 0.0, tc = 1e300;
+    c->list_mode = 1;
+    if ((rc = stage_ms(0, flags | PMX_EVAL_FORCE_LIST, &tb))) return rc;
+    if (cells_ok) {
+      c->list_mode = 2;
+      if ((rc = stage_ms(0, flags | PMX_EVAL_FORCE_LIST, &tc))) return rc;
+    }
+    c->list_mode = tc < tb ? 2 : 1;
+  }
+  c->epoch++;
+  if ((rc = pmx_energy_control(c, flags | PMX_EVAL_FORCE_LIST, &e))) return rc;
+  if (choice) {
+    choice[0] = c->pa.T[0];
+    choice[1] = c->pa.T[1];
+    choice[2] = c->pa.T[2];
+    choice[3] = c->pair_waves;
+    choice[4] = c->list_mode;
+  }
+  return PMX_OK;
+}
+
+// Launch shapes of a batch picked by timing whole batched calls: the boxes
+// share the device, so the shape that is fastest for a box alone is not the
+// one that is fastest for the batch.  Each candidate (the pair kernels'
+// wavefront cap, then the PME tile caps) is applied to every context and
+// timed over reps calls after one untimed call (which recaptures the graph).
+int pmx_batch_autotune(pmx_batch *b, unsigned flags, int reps, int *choice) {
+  if (!b) return PMX_ERR_ARG;
+  if (reps < 1) reps = 1;
+  const size_t n = b->ctx.size();
+  std::vector<pmx_energies> out(n);
+  auto timed = [&](double *sec) -> int {
+    int rc = pmx_batch_energy_control(b, flags, out.data());
+    if (rc) return rc;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; r++)
+      if ((rc = pmx_batch_energy_control(b, flags, out.data()))) return rc;
+    *sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() / reps;
+    return PMX_OK;
+  };
+  int rc = PMX_OK;
+  {
+    const int cand[] = {512, 1024, 2048, 4096, 8192, 16384};
+    int best = b->ctx[0]->pair_waves;
+    double tbest = 1e300;
+    for (int w : cand) {
+      for (pmx_ctx *c : b->ctx) {
+        c->pair_waves = w;
+        c->epoch++;
+      }
+      double t;
+      if ((rc = timed(&t))) return rc;
+      if (t < tbest) {
+        tbest = t;
+        best = w;
+      }
+    }
+    for (pmx_ctx *c : b->ctx) {
+      c->pair_waves = best;
+      c->epoch++;
+    }
+    if (choice) choice[0] = best;
+  }
+  {
+    const int cand[][3] = {{16, 16, 8}, {8, 8, 8}, {16, 8, 8}, {8, 8, 4}, {32, 16, 8}, {16, 16, 16}};
+    int best = 0;
+    double tbest = 1e300;
+    for (int k = 0; k < (int)(sizeof cand / sizeof cand[0]); k++) {
+      bool ok = true;
+      for (pmx_ctx *c : b->ctx) {
+        for (int d = 0; d < 3; d++) c->tile_cap[d] = cand[k][d];
+        if (c->pme && setup_pme_tiles(c) != PMX_OK) ok = false;
+        c->epoch++;
+      }
+      if (!ok) continue;
+      double t;
+      if ((rc = timed(&t))) return rc;
+      if (t < tbest) {
+        tbest = t;
+        best = k;
+      }
+    }
+    for (pmx_ctx *c : b->ctx) {
+      for (int d = 0; d < 3; d++) c->tile_cap[d] = cand[best][d];
+      if (c->pme && (rc = setup_pme_tiles(c))) return rc;
+      c->epoch++;
+    }
+    if (choice)
+      for (int d = 0; d < 3; d++) choice[1 + d] = cand[best][d];
+  }
+  return pmx_batch_energy_control(b, flags, out.data());
+}
+
 int pmx_get_forces(pmx_ctx *c, double *fx, double *fy, double *fz) {
   if (!c || !fx || !fy || !fz) return PMX_ERR_ARG;
   Setup &s = c->s;
@@ -1595,12 +1775,12 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
   HIPCHK(hipSetDevice(c->device));
   hipStream_t st = c->st;
   int rc = PMX_OK;
-  const unsigned gx = pair_grid_x(N);
+  const unsigned gx = pair_grid_x(N, c->pair_waves);
   const bool kernels = out->dq || out->deps || out->dsig || out->dbond_c || out->dbond_eq || out->dbend_c ||
                        out->dbend_s;
   if (kernels && !c->phi) {
     if ((rc = dalloc(c, &c->phi, NP))) return rc;
-    if ((rc = dalloc(c, &c->gpart, (size_t)gx * P * PAIRG_NV))) return rc;
+    if ((rc = dalloc(c, &c->gpart, (size_t)pair_grid_x(N, PAIR_MAX_WAVES) * P * PAIRG_NV))) return rc;
     if ((rc = dalloc(c, &c->gacc, (size_t)P * PAIRG_NV))) return rc;
     if ((rc = dalloc(c, &c->gbond, (size_t)P * std::max(c->nbond_type, 1) * 7))) return rc;
     if ((rc = dalloc(c, &c->gbend, (size_t)P * std::max(c->nbend_type, 1) * BEND_GRAD_NV))) return rc;
@@ -1613,6 +1793,7 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
     a.nsplin = s.nsplin;
     a.pimd = (P > 1);
     a.npair = s.npair;
+    a.gx = pair_grid_x(N, c->pair_waves);
     a.alp = s.alp_ewd;
     a.palp = kErfP * s.alp_ewd;
     a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;

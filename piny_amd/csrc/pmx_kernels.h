@@ -47,6 +47,7 @@ constexpr int ENTRY_MASK = (1 << ENTRY_SHIFT) - 1;
 
 struct PairArgs {
   int N, P, cap, nsplin, pimd, npair;
+  int gx;  // workgroups along x (pair_grid_x)
   double alp, palp, talp2, rheal, irheal;  // erfc and switch constants
   int swit;
   const double *pp;            // [npair][PP_NUM]
@@ -233,7 +234,8 @@ __global__ void k_kinetic(int n, const double *mass, const double *vx, const dou
 // forces (no atomics); otherwise j forces are scattered with atomics
 hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st);
 hipError_t launch_pair_grad(const PairArgs &a, int mode, hipStream_t st);
-int pair_grid_x(int N);
+int pair_grid_x(int N, int max_waves);
+constexpr int PAIR_MAX_WAVES = 16384;  // the largest pair grid the autotuner tries
 // LDS bytes of the spread and gather workgroups for a given tiling
 size_t pme_lds_bytes(const PmeArgs &a, bool gather);
 hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st);

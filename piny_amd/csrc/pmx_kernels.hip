@@ -1302,8 +1302,10 @@ __global__ void k_kinetic(int n, const double *__restrict__ mass, const double *
 // ---------------------------------------------------------------------------
 // launch wrappers
 
-int pair_grid_x(int N) {
-  const int waves = N < 8192 ? N : 8192;
+// Workgroups (4 wavefronts, one site per wavefront at a time) of the pair
+// kernels for N sites and at most max_waves wavefronts.
+int pair_grid_x(int N, int max_waves) {
+  const int waves = N < max_waves ? N : max_waves;
   return (waves + 3) / 4;
 }
 
@@ -1328,7 +1330,7 @@ hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st) {
   int B = 1;
   while (B < a.P && B < 64) B <<= 1;
   const int nbg = (a.P + B - 1) / B;
-  const dim3 grid(pair_grid_x(a.N), nbg);
+  const dim3 grid(a.gx, nbg);
   switch (B) {
     case 1: launch_pair_b<1>(a, mode, full, grid, st); break;
     case 2: launch_pair_b<2>(a, mode, full, grid, st); break;
@@ -1353,7 +1355,7 @@ hipError_t launch_pair_grad(const PairArgs &a, int mode, hipStream_t st) {
   int B = 1;
   while (B < a.P && B < 64) B <<= 1;
   const int nbg = (a.P + B - 1) / B;
-  const dim3 grid(pair_grid_x(a.N), nbg);
+  const dim3 grid(a.gx, nbg);
   switch (B) {
     case 1: launch_pair_grad_b<1>(a, mode, grid, st); break;
     case 2: launch_pair_grad_b<2>(a, mode, grid, st); break;

@@ -72,7 +72,9 @@ def lib():
     L.pmx_batch_last_error.restype = C.c_char_p
     L.pmx_batch_last_error.argtypes = [C.c_void_p]
     L.pmx_batch_destroy.argtypes = [C.c_void_p]
-    if L.pmx_abi_version() != 4:
+    L.pmx_autotune.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_int)]
+    L.pmx_batch_autotune.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_int)]
+    if L.pmx_abi_version() != 6:
         raise PinyAmdError("libpiny_amd.so ABI version mismatch")
     _lib = L
     return L
@@ -86,7 +88,8 @@ EXPORTED = ("pmx_abi_version", "pmx_device_count", "pmx_create", "pmx_destroy", 
             "pmx_get_setup_scalars", "pmx_get_timings", "pmx_get_pair_count", "pmx_set_velocities",
             "pmx_get_velocities", "pmx_get_positions", "pmx_md_nve", "pmx_bond_coef_harm", "pmx_bond_coef_morse",
             "pmx_bend_coef_harm", "pmx_set_force_field", "pmx_get_param_grad", "pmx_batch_create",
-            "pmx_batch_energy_control", "pmx_batch_last_error", "pmx_batch_destroy")
+            "pmx_batch_energy_control", "pmx_batch_last_error", "pmx_batch_destroy", "pmx_autotune",
+            "pmx_batch_autotune")
 
 
 class Engine:
@@ -203,6 +206,14 @@ class Engine:
         self._chk(lib().pmx_get_param_grad(self.h, C.byref(g)), "pmx_get_param_grad")
         return out
 
+    def autotune(self, reps=3, flags=None):
+        """Picks launch shapes by timing (pmx_autotune); returns them."""
+        if flags is None:
+            flags = abi.PMX_EVAL_PIMD if self.P > 1 else 0
+        ch = (C.c_int * 5)()
+        self._chk(lib().pmx_autotune(self.h, flags, int(reps), ch), "pmx_autotune")
+        return {"pme_tile": tuple(ch[:3]), "pair_waves": ch[3], "list": ("plain", "cells")[ch[4] - 1]}
+
     def bead_potential(self):
         """E_b [P] of the last force call (pmx_get_param_grad, energies only)."""
         e = np.zeros(self.P)
@@ -279,6 +290,15 @@ class Batch:
             msg = lib().pmx_batch_last_error(self.h)
             raise PinyAmdError(f"pmx_batch_energy_control failed ({rc}): {msg.decode() if msg else ''}")
         return [e.as_dict() for e in out]
+
+    def autotune(self, reps=2, flags=0):
+        """Launch shapes for the whole batch, by timing batched calls."""
+        ch = (C.c_int * 4)()
+        rc = lib().pmx_batch_autotune(self.h, flags, int(reps), ch)
+        if rc != 0:
+            msg = lib().pmx_batch_last_error(self.h)
+            raise PinyAmdError(f"pmx_batch_autotune failed ({rc}): {msg.decode() if msg else ''}")
+        return {"pair_waves": ch[0], "pme_tile_cap": tuple(ch[1:4])}
 
     def close(self):
         if getattr(self, "h", None):

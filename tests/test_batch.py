@@ -82,3 +82,41 @@ def test_batch_box_matches_oracle():
             assert abs(out[k][key] - eo[key]) <= 1e-9 * max(abs(eo[key]), scale), (k, key)
         fg = eb[k].forces()
         assert np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum()) < 1e-9
+
+
+def test_autotune_keeps_results():
+    # the tuned launch shapes change only the order of sums
+    cfgs = w.config5(nbox=3, max_mol=600)
+    for c in cfgs:
+        s = c.system
+        print(f"[autotune] N={s.natm} P={s.pi_beads}", flush=True)
+        a = Engine(s, c.x, c.y, c.z)
+        b = Engine(s, c.x, c.y, c.z)
+        for e in (a, b):
+            e.set_positions(c.x, c.y, c.z)
+        ref = b.energy_control()
+        ch = a.autotune(reps=1)
+        print("  choice", ch, flush=True)
+        assert ch["pair_waves"] in (1024, 2048, 4096, 8192, 16384)
+        got = a.energy_control()
+        for key in abi.ENERGY_KEYS:
+            assert abs(got[key] - ref[key]) <= 1e-10 * max(abs(ref[key]), 1e-3), key
+        df = float(np.max(np.abs(a.forces() - b.forces())) / np.max(np.abs(b.forces())))
+        assert df < 1e-10
+
+
+def test_batch_autotune_keeps_results():
+    cfgs = _boxes()
+    eb = [Engine(c.system, c.x, c.y, c.z) for c in cfgs]
+    es = [Engine(c.system, c.x, c.y, c.z) for c in cfgs]
+    for e1, e2, c in zip(eb, es, cfgs):
+        e1.set_positions(c.x, c.y, c.z)
+        e2.set_positions(c.x, c.y, c.z)
+    b = Batch(eb)
+    ch = b.autotune(reps=1)
+    print("[batch autotune] choice", ch, flush=True)
+    out = b.energy_control()
+    for k, e2 in enumerate(es):
+        ref = e2.energy_control()
+        for key in abi.ENERGY_KEYS:
+            assert abs(out[k][key] - ref[key]) <= 1e-10 * max(abs(ref[key]), 1e-3), (k, key)
