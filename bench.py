@@ -12,7 +12,9 @@ Three paths are timed on the same box:
   fp64        PINY's arithmetic: FP64 with cubic spline-table interpolation
               (the drop-in path, parity with the reference to ~1e-12);
   fp64_direct FP64 with the tabulated pair functions evaluated directly;
-  mixed       FP32 pair terms with FP64 sums (PME, bonded, springs FP64).
+  mixed       FP32 pair terms and PME grid/FFT with FP32/FP64 split sums
+              (separations' image part, energies, virial, bonded terms,
+              springs in FP64).
 `value` is the mixed path's throughput, reported with the fp64 path's and
 the mixed path's force RMS error relative to fp64.
 
@@ -34,7 +36,7 @@ PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X vector TFLOP/s (FP64 vector = FP64
 # HBM bytes per launch of the pair kernel, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (separate runs) scaled by a 512 MiB copy probe (FETCH x 2.00,
 # WRITE x 1.00); profiles/kernels_q-tip4p-f_4096_p32_{mixed,fp64}.md
-PAIR_TRAFFIC = {"mixed": (124078 * 2.0 + 16904) * 1024, "fp64": (134511 * 2.0 + 16897) * 1024}
+PAIR_TRAFFIC = {"mixed": (91967 * 2.0 + 31848) * 1024, "fp64": (234576 * 2.0 + 33699) * 1024}
 FLOP_PER_PAIR = 100.0                 # nominal algorithmic flops per pair term inside the cutoff
 DT_FS = 0.25
 
