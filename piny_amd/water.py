@@ -162,16 +162,29 @@ class System:
     ghost_ncomp: np.ndarray = None
     ghost_comp: np.ndarray = None
     ghost_coef: np.ndarray = None
+    # a general cell (column-major like PINY's hmat[1..9], cell vectors as
+    # columns, Angstrom) instead of the cube of edge `box`
+    cell: np.ndarray = None
 
     @property
     def hmat(self):
+        """The cell in Bohr, column-major (PINY's hmat[1..9])."""
+        if self.cell is not None:
+            return np.asarray(self.cell, dtype=np.float64) / BOHR
         h = np.zeros(9)
         h[0] = h[4] = h[8] = self.box / BOHR
         return h
 
     @property
+    def side(self):
+        """Cube root of the cell volume, Angstrom."""
+        if self.cell is None:
+            return self.box
+        return float(abs(np.linalg.det(np.asarray(self.cell, dtype=np.float64).reshape(3, 3).T))) ** (1.0 / 3.0)
+
+    @property
     def ewald_alpha(self):
-        return self.settings.alpha * self.box
+        return self.settings.alpha * self.side
 
 
 def make_system(model: WaterModel, nmol: int, box: float, pi_beads: int = 1,
@@ -407,6 +420,15 @@ def config5(nbox: int = 64, seed: int = SEED_BASE + 5, max_mol: int = 4096) -> l
         st = Settings(cutoff=rc, alpha=3.5 / rc, kmax_ewd=min(kp, K // 2 - 2), kmax_pme=kp)
         out.append(build_config(SPC_FW, int(nmol[k]), L, int(beads[k]), seed + 1000 * k, st))
     return out
+
+
+def with_cell(cfg: Config, cell_rows) -> Config:
+    """The configuration in another cell: cell_rows are the rows of the
+    cell matrix in Angstrom (cell vectors as columns, the layout of PINY's
+    initial file), positions unchanged."""
+    H = np.asarray(cell_rows, dtype=np.float64).reshape(3, 3)
+    sysd = dataclasses.replace(cfg.system, cell=H.T.reshape(-1).copy())
+    return Config(sysd, cfg.x, cfg.y, cfg.z)
 
 
 def small_config(model: WaterModel = SPC_FW, nmol: int = 27, P: int = 1, seed: int = 7,

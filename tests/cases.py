@@ -44,6 +44,14 @@ CASES = {
     "cfg1_k40": lambda: w.config1(_st(kmax_ewd=14, kmax_pme=17)),
     "tip128_p4_k48": lambda: w.build_config(w.Q_TIP4P_F, 128, w.density_box(128), 4, 17,
                                             _st(cutoff=6.0, alpha=0.55, kmax_ewd=16, kmax_pme=22, n_interp=8)),
+    # non-cubic cells: 64-molecule boxes in an orthorhombic cell (PME grid
+    # 16 x 20 x 24) and, with beads, a triclinic one (20 x 24 x 24)
+    "spc64_ortho": lambda: w.with_cell(
+        w.build_config(w.SPC_FW, 64, 12.4, 1, 12, _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=8)),
+        [[12.4, 0.0, 0.0], [0.0, 14.26, 0.0], [0.0, 0.0, 16.12]]),
+    "tip64_p4_tric": lambda: w.with_cell(
+        w.build_config(w.Q_TIP4P_F, 64, 12.4, 4, 13, _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=10)),
+        [[12.4, 2.48, 1.24], [0.0, 13.64, 1.86], [0.0, 0.0, 14.88]]),
     # boxes with at least 3 link cells per axis
     "tip512_p4": lambda: w.build_config(w.Q_TIP4P_F, 512, w.density_box(512), 4, 19,
                                         _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),

@@ -31,7 +31,8 @@ from piny_amd import water as w  # noqa: E402
 from tests import cases  # noqa: E402
 
 EXE = os.path.join(ROOT, "oracle", "_ref", "piny_ref")
-CASES = ["spc64", "spc64_shift", "spc64_none", "tip64", "cfg1", "spc64_k20", "tip64_p4", "cfg2_p4"]
+CASES = ["spc64", "spc64_shift", "spc64_none", "tip64", "cfg1", "spc64_k20", "tip64_p4", "cfg2_p4", "spc64_ortho",
+         "tip64_p4_tric"]
 SHIFT = {0: "off", 1: "on", 2: "swit"}  # shift_inter_pe values (set_sim_dict.c:736)
 
 
@@ -131,9 +132,10 @@ def write_inputs(d, c):
     lines = ["%d 1 %d" % (N, P)]
     for i in range(N * P):
         lines.append("%.15f %.15f %.15f" % (c.x[i] * w.BOHR, c.y[i] * w.BOHR, c.z[i] * w.BOHR))
+    # cell rows (read_hmat.c:255-301 reads line i as hmat[1+i], hmat[4+i], hmat[7+i])
+    h = s.hmat * w.BOHR
     for r in range(3):
-        row = [s.box if k == r else 0.0 for k in range(3)]
-        lines.append("%.15f %.15f %.15f" % tuple(row))
+        lines.append("%.15f %.15f %.15f" % (h[r], h[3 + r], h[6 + r]))
     open(os.path.join(d, "water.initial"), "w").write("\n".join(lines) + "\n")
 
     pos = np.stack([c.x, c.y, c.z], 1).reshape(P, N, 3)
