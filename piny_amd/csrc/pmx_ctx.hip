@@ -49,6 +49,7 @@ struct pmx_ctx {
   double *pair_pp = nullptr;   // [npair][PP_NUM]
   double4 *site = nullptr;     // [N] (xmod, q)
   float4 *disp = nullptr;      // mixed precision with beads: x - xmod in FP32 [N*P]
+  double *umax = nullptr;      // the largest |x - xmod| of every site over its beads [N]
   int pair_mode = 0;       // 0 FP64 tables, 1 FP64 direct, 2 FP32 direct
   bool full_list = true;   // force list with both directions (no atomics)
   int cap = 0, *jfull = nullptr, *jforce = nullptr, *nfull = nullptr, *nforce = nullptr, *overflow = nullptr;
@@ -421,7 +422,10 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(upload(c, &c->pair_pp, pp.data(), pp.size()));
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
-    if (c->pair_mode == 2 && P > 1) TRY(dalloc(c, &c->disp, NP));
+    if (c->pair_mode == 2 && P > 1) {
+      TRY(dalloc(c, &c->disp, NP));
+      TRY(dalloc(c, &c->umax, (size_t)N));
+    }
   }
   TRY(dalloc(c, &c->site_cell, N));
   TRY(dalloc(c, &c->site_pos, N));
@@ -819,6 +823,7 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     if (c->disp) {
       hipLaunchKernelGGL(k_pack_disp, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm,
                          c->disp);
+      hipLaunchKernelGGL(k_site_umax, dim3(blocks(N)), dim3(256), 0, st, N, P, c->disp, c->umax);
       HIPCHK(hipGetLastError());
     }
     PairArgs a{};
@@ -841,6 +846,7 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.z = c->z;
     a.site = c->site;
     a.u = c->disp;
+    a.umax = c->umax;
     a.type = c->type;
     a.jlist = c->jforce;
     a.cnt = c->nforce;

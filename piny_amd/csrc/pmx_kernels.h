@@ -54,6 +54,7 @@ struct PairArgs {
   const double *x, *y, *z;     // [N*P]
   const double4 *site;         // [N] (xmod, ymod, zmod, q)
   const float4 *u;             // mixed precision, PIMD: (x - xmod) per (site, bead) [N*P]
+  const double *umax;          // mixed precision, PIMD: the largest |u| of every site over its beads [N]
   const int *type;
   const int *jlist, *cnt;      // entries j | t << ENTRY_SHIFT
   const double2 *tab_lj, *tab_c;  // [npair*nsplin] (V, -V'/r)
@@ -197,6 +198,7 @@ __global__ void k_pack_sites(int N, int pimd, const double *xm, const double *ym
                              const double *q, double4 *site);
 __global__ void k_pack_disp(int N, int P, const double *x, const double *y, const double *z, const double *xm,
                             const double *ym, const double *zm, float4 *u);
+__global__ void k_site_umax(int N, int P, const float4 *u, double *umax);
 __global__ void k_list_check(int N, int P, const double *x, const double *y, const double *z,
                              const double *x0, const double *y0, const double *z0, double *b0x, double *b0y,
                              double *b0z, unsigned long long *maxes);

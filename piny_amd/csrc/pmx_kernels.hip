@@ -123,6 +123,18 @@ __global__ void k_pack_disp(int N, int P, const double *__restrict__ x, const do
   u[t] = make_float4((float)(x[t] - xm[i]), (float)(y[t] - ym[i]), (float)(z[t] - zm[i]), 0.0f);
 }
 
+// The largest |u| of every site over its beads (one thread per site).
+__global__ void k_site_umax(int N, int P, const float4 *__restrict__ u, double *__restrict__ umax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  float m = 0.0f;
+  for (int b = 0; b < P; b++) {
+    const float4 v = u[(size_t)i * P + b];
+    m = fmaxf(m, v.x * v.x + v.y * v.y + v.z * v.z);
+  }
+  umax[i] = sqrt((double)m);
+}
+
 // Bead-1 copy, largest displacement since the last list build
 // (check_ver_list, nbr_list_control.c:182-239) and bead spread
 // (get_pimd_spread, pimd_estimators.c:121-222).  Maxima of squares are
@@ -710,6 +722,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
   const Cell &c = a.cell;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   double vlj = 0, vc = 0, p11 = 0, p22 = 0, p33 = 0, p12 = 0, p13 = 0, p23 = 0, npr = 0;
   for (int i = wave; i < a.N; i += nwave) {
     const size_t ib = (size_t)i * P + (bact ? bead : 0);
@@ -720,11 +733,16 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
     double fxi = 0.0, fyi = 0.0, fzi = 0.0;
     for (int k0 = 0; k0 < cnt; k0 += 64) {
       const int ks = k0 + lane;
+      // entries that no bead can bring inside the cutoff
+      // (|D| - max|u_i| - max|u_j| >= r_c, with room for the FP32
+      // rounding) are dropped here and the chunk is compacted
+      bool keep = false;
+      int e = 0;
+      double dx = 0.0, dy = 0.0, dz = 0.0, q2 = 0.0;
       if (ks < cnt) {
-        const int e = row[ks];
+        e = row[ks];
         const int j = e & ENTRY_MASK;
         const double4 sj = a.site[j];
-        double dx, dy, dz;
         if (a.pimd) {
           min_image_sep(c, 1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, si.x, si.y, si.z, sj.x, sj.y, sj.z, dx, dy, dz);
           dx += si.x - sj.x;
@@ -734,11 +752,19 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           min_image_sep(c, 0, a.x[i], a.y[i], a.z[i], a.x[j], a.y[j], a.z[j], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dx,
                         dy, dz);
         }
-        sdq[w][lane] = make_float4((float)dx, (float)dy, (float)dz, (float)(si.w * sj.w));
-        sent[w][lane] = e;
+        const double lim =
+            ((double)spp[(e >> ENTRY_SHIFT) * PP_NUM + PP_CUT] + (a.pimd ? a.umax[i] + a.umax[j] : 0.0)) * (1.0 + 1e-5);
+        keep = dx * dx + dy * dy + dz * dz < lim * lim;
+        q2 = si.w * sj.w;
+      }
+      const uint64_t mk = __ballot(keep);
+      if (keep) {
+        const int pos = __popcll(mk & below);
+        sdq[w][pos] = make_float4((float)dx, (float)dy, (float)dz, (float)q2);
+        sent[w][pos] = e;
       }
       wave_sync();
-      const int nk = min(64, cnt - k0);
+      const int nk = __popcll(mk);
       float cvlj = 0.f, cvc = 0.f, c11 = 0.f, c22 = 0.f, c33 = 0.f, c12 = 0.f, c13 = 0.f, c23 = 0.f;
       float cfx = 0.f, cfy = 0.f, cfz = 0.f;
       int cn = 0;
