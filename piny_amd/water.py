@@ -431,6 +431,12 @@ def with_cell(cfg: Config, cell_rows) -> Config:
     return Config(sysd, cfg.x, cfg.y, cfg.z)
 
 
+def without_charges(cfg: Config) -> Config:
+    """The configuration with every charge zero (LJ and bonded terms only)."""
+    sysd = dataclasses.replace(cfg.system, q=np.zeros_like(cfg.system.q))
+    return Config(sysd, cfg.x, cfg.y, cfg.z)
+
+
 def small_config(model: WaterModel = SPC_FW, nmol: int = 27, P: int = 1, seed: int = 7,
                  box: float | None = None, cutoff: float | None = None,
                  kmax: int | None = None) -> Config:

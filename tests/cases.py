@@ -52,6 +52,14 @@ CASES = {
     "tip64_p4_tric": lambda: w.with_cell(
         w.build_config(w.Q_TIP4P_F, 64, 12.4, 4, 13, _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=10)),
         [[12.4, 2.48, 1.24], [0.0, 13.64, 1.86], [0.0, 0.0, 14.88]]),
+    # a bead count that is not a power of two, the largest bead count, and a
+    # box without charges (LJ and bonded terms only: no k space, no ecor)
+    "tip64_p6": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 6, 21,
+                                       _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
+    "spc27_p64": lambda: w.build_config(w.SPC_FW, 27, w.density_box(27), 64, 22,
+                                        _st(cutoff=4.0, alpha=0.85, kmax_ewd=6, kmax_pme=6)),
+    "spc64_noq": lambda: w.without_charges(w.build_config(w.SPC_FW, 64, w.density_box(64), 2, 23,
+                                                          _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6))),
     # boxes with at least 3 link cells per axis
     "tip512_p4": lambda: w.build_config(w.Q_TIP4P_F, 512, w.density_box(512), 4, 19,
                                         _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),

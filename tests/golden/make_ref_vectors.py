@@ -83,7 +83,10 @@ def write_inputs(d, c):
     ghost = dict()
     if m.ghost is not None:
         ghost[m.ghost[0]] = m.ghost[1]
-    for k, (name, q, mass) in enumerate(zip(m.sites, m.charges, m.masses_amu)):
+    # the system's charges (the same in every molecule)
+    qs = np.asarray(s.q).reshape(s.nmol, len(m.sites))
+    assert np.all(qs == qs[0])
+    for k, (name, q, mass) in enumerate(zip(m.sites, qs[0], m.masses_amu)):
         items = [("atom_typ", name), ("atom_ind", k + 1), ("mass", repr(mass)), ("charge", repr(q))]
         for g, (comp, coef) in enumerate(ghost.get(k, ())):
             items.append(("def_ghost%d" % (g + 1), "%d, %r" % (comp + 1, coef)))
