@@ -16,7 +16,9 @@ Three paths are timed on the same box:
               (separations' image part, energies, virial, bonded terms,
               springs in FP64).
 `value` is the mixed path's throughput, reported with the fp64 path's and
-the mixed path's force RMS error relative to fp64.
+the mixed path's force RMS error relative to fp64.  `fp64_per_bead` times
+the plain FP64 form beside them: each bead as its own classical force call,
+one after another.
 
 Prints one JSON line.
 """
@@ -165,6 +167,35 @@ def run_batch(steps, warmup, nbox=64):
             "batched_vs_sequential_fp64": res["batched_fp64"]["value"] / res["sequential_fp64"]["value"]}
 
 
+def run_per_bead(cfg, steps, warmup):
+    """The plain FP64 form: every bead of the workload box evaluated by its own
+    classical force call, one after another (one context per bead, each with
+    its own list), as PINYEmpirical evaluates replicas
+    (python/piny/piny.pyx:426-436).  One step = one call per bead."""
+    from piny_amd import water as w
+    from piny_amd.engine import Engine
+    s = cfg.system
+    P, n = s.pi_beads, s.natm
+    s1 = w.make_system(s.model, s.nmol, s.box, 1, s.settings)
+    engs = []
+    for b in range(P):
+        x, y, z = (v[b * n:(b + 1) * n].copy() for v in (cfg.x, cfg.y, cfg.z))
+        e = Engine(s1, x, y, z)
+        e.set_positions(x, y, z)
+        engs.append(e)
+    for _ in range(warmup):
+        for e in engs:
+            e.energy_control(0)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for e in engs:
+            e.energy_control(0)
+    el = time.perf_counter() - t0
+    for e in engs:
+        e.close()
+    return {"value": steps * P / el, "ms_per_step": el / steps * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,6 +236,7 @@ def main():
           "mixed": {"precision": abi.PMX_PREC_MIXED}}
     paths = [p for p in args.paths.split(",") if p]
     res = {p: run_path(cfg, args.steps, args.warmup, **kw[p]) for p in paths}
+    per_bead = run_per_bead(cfg, max(2, args.steps // 4), 1) if "fp64" in paths else None
     head = res["mixed"] if "mixed" in res else res[paths[-1]]
     head_name = "mixed" if "mixed" in res else paths[-1]
     ref = res.get("fp64")
@@ -240,6 +272,8 @@ def main():
                                                   / abs(ref["energies"]["vintert"])) if ref else None}
                   for p, r in res.items()},
         "speedup_vs_fp64": head["value"] / ref["value"] if ref else None,
+        "fp64_per_bead": per_bead,
+        "speedup_vs_fp64_per_bead": head["value"] / per_bead["value"] if per_bead else None,
         "force_rms_err": rms(head["forces"]),
         "pairs_in_cutoff_all_beads": npairs,
         "verlet_pairs": head["energies"]["nver"],
