@@ -69,6 +69,10 @@ struct pmx_ctx {
   double2 *spec = nullptr, *tw[3] = {nullptr, nullptr, nullptr};
   float2 *twf[3] = {nullptr, nullptr, nullptr};
   int pme_f32 = 0;  // grids, B-splines and FFT in FP32 (the mixed precision)
+  // standard Ewald (ewald_pme_opt off): k list, structure factors
+  int ewald = 0;
+  double4 *ew_kv = nullptr, *ew_kg = nullptr;
+  double2 *ew_sk = nullptr;
   // binned PME: tiling, per-item data, bins, halo boxes
   PmeArgs pa{};
   int *item_s = nullptr, *item_tile = nullptr, *item_pos = nullptr, *bin_cnt = nullptr, *bin_off = nullptr,
@@ -480,9 +484,25 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   // PME grids, twiddles and the influence function on the sphere list
   c->pme = (!s.ichrg.empty() && c->par.pme_on);
   c->pme_f32 = (c->par.precision == PMX_PREC_MIXED) ? 1 : 0;
-  if (!s.ichrg.empty() && !c->par.pme_on) {
-    set_err(c, "standard Ewald (ewald_pme_opt off) is not implemented on the device; use PME");
-    return PMX_ERR_NOTIMPL;
+  c->ewald = (!s.ichrg.empty() && !c->par.pme_on);
+  if (c->ewald) {  // ewald3d_recip's k terms (ewald3d.c:27-373)
+    std::vector<double4> kv(s.nktot), kg(s.nktot);
+    const double *hi = s.hmati;
+    const double falp2 = 4.0 * s.alp_ewd * s.alp_ewd, tpi = 2.0 * M_PI, pivol = s.vol / (4.0 * M_PI);
+    for (int k = 0; k < s.nktot; k++) {
+      const double aka = tpi * ((double)s.ka[k]), akb = tpi * ((double)s.kb[k]), akc = tpi * ((double)s.kc[k]);
+      const double xk = (aka * hi[1] + akb * hi[2] + akc * hi[3]);
+      const double yk = (aka * hi[4] + akb * hi[5] + akc * hi[6]);
+      const double zk = (aka * hi[7] + akb * hi[8] + akc * hi[9]);
+      const double g2 = xk * xk + yk * yk + zk * zk;
+      const double preg = exp((-g2 / falp2)) / (g2 * pivol);
+      const double prep = -2.0 * preg * ((g2 / falp2) + 1.0) / g2;
+      kv[k] = make_double4(aka, akb, akc, preg);
+      kg[k] = make_double4(xk, yk, zk, prep);
+    }
+    TRY(upload(c, &c->ew_kv, kv.data(), kv.size()));
+    TRY(upload(c, &c->ew_kg, kg.data(), kg.size()));
+    TRY(dalloc(c, &c->ew_sk, (size_t)s.nktot * P));
   }
   if (c->pme) {
     const int Ka = s.ngrid[0], Kb = s.ngrid[1], Kc = s.ngrid[2], Ha = Ka / 2 + 1;
@@ -533,7 +553,9 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   }
   {
     const size_t pair = (size_t)pair_grid_x(N, PAIR_MAX_WAVES) * P * PAIR_NV;
-    const size_t fft = (size_t)(c->pme ? s.ngrid[1] : 0) * P * FFT_NV;
+    const size_t fft = (size_t)std::max(c->pme ? s.ngrid[1] : 0,
+                                        c->ewald ? (s.nktot + EWALD_BLOCK - 1) / EWALD_BLOCK : 0) *
+                       P * FFT_NV;
     const size_t intra = (size_t)c->g_intra * P * INTRA_NV;
     const size_t fin = (size_t)c->g_intra * P * FIN_NV;
     c->off_pair = 0;
@@ -905,6 +927,29 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     HIPCHK(launch_pme_gather(a, st));
     const int Kb = s.ngrid[1];
     add_stage(red, c->off_fft, Kb, FFT_NV, {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
+  }
+  if (c->ewald && !(flags & PMX_EVAL_NO_RECIP)) {
+    EwaldArgs a{};
+    a.N = N;
+    a.P = P;
+    a.nchrg = nchrg;
+    a.nk = s.nktot;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.q = c->q;
+    a.ichrg = c->ichrg;
+    a.cell = c->cell;
+    a.kv = c->ew_kv;
+    a.kg = c->ew_kg;
+    a.sk = c->ew_sk;
+    a.fx = c->fx;
+    a.fy = c->fy;
+    a.fz = c->fz;
+    a.part = c->part + c->off_fft;
+    HIPCHK(launch_ewald(a, st));
+    add_stage(red, c->off_fft, (s.nktot + EWALD_BLOCK - 1) / EWALD_BLOCK, FFT_NV,
+              {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
   }
   IntraArgs ia = intra_args(c, pimd);
   const int bsP = bead_block(P);
@@ -1767,6 +1812,10 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
   }
   if (!c->full_list) {
     set_err(c, "pmx_get_param_grad: needs the full force list");
+    return PMX_ERR_NOTIMPL;
+  }
+  if (c->ewald) {
+    set_err(c, "pmx_get_param_grad: the charge gradients are formed for PME, not for standard Ewald");
     return PMX_ERR_NOTIMPL;
   }
   Setup &s = c->s;

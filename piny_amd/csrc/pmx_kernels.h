@@ -95,6 +95,21 @@ struct PmeArgs {
   double *phi;         // potential gather: dE_b/dq_i += k-space part [N*P]
 };
 
+// Standard Ewald (ewald_pme_opt off): the sum over the half-space k list
+// with direct phases.
+struct EwaldArgs {
+  int N, P, nchrg, nk;
+  const double *x, *y, *z, *q;
+  const int *ichrg;
+  Cell cell;
+  const double4 *kv;  // [nk] (2 pi ka, 2 pi kb, 2 pi kc, preg)
+  const double4 *kg;  // [nk] (kx, ky, kz, prep / |S|^2 factor)
+  double2 *sk;        // [P][nk] 2 preg (Re S, Im S)
+  double *fx, *fy, *fz;
+  double *part;       // partial rows, FFT_NV values
+};
+constexpr int EWALD_BLOCK = 256;
+
 constexpr int FFT_MAX_PASSES = 12;
 constexpr int FFT_MAX_PLANE = 4096;  // complex values per LDS plane (64 KB)
 
@@ -243,6 +258,7 @@ size_t pme_lds_bytes(const PmeArgs &a, bool gather);
 hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st);
 hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st);
 hipError_t launch_pme_phi(const PmeArgs &a, hipStream_t st);
+hipError_t launch_ewald(const EwaldArgs &a, hipStream_t st);
 hipError_t launch_pme_fft(const FftArgs &f, hipStream_t st);
 // block size for grid-stride kernels whose threads keep one bead: the
 // largest multiple of P not above 256

@@ -458,6 +458,112 @@ hipError_t launch_pme_phi(const PmeArgs &a, hipStream_t st) {
   return a.f32 ? launch_phi_t<float>(a, st) : launch_phi_t<double>(a, st);
 }
 
+// --- standard Ewald ---------------------------------------------------------------
+
+// Structure factors, ewald3d_recip (ewald3d.c:27-373) with direct phases:
+// one thread per (k, bead) sums q cos and q sin of 2 pi k.s over LDS tiles
+// of the charges' fractional coordinates s = h^-1 r, then the k term's
+// energy preg |S|^2 and pressure tensor go into block partial rows (the
+// energy also on the diagonal, as the PME convolution writes them).
+__global__ void __launch_bounds__(EWALD_BLOCK) k_ewald_sk(EwaldArgs a) {
+  __shared__ double4 sq[EWALD_BLOCK];
+  const int tid = threadIdx.x, b = blockIdx.y;
+  const int k = blockIdx.x * EWALD_BLOCK + tid;
+  const bool on = k < a.nk;
+  const double4 kv = on ? a.kv[k] : make_double4(0, 0, 0, 0);
+  const double *hi = a.cell.hi;
+  double sr = 0.0, si = 0.0;
+  for (int c0 = 0; c0 < a.nchrg; c0 += EWALD_BLOCK) {
+    __syncthreads();
+    if (c0 + tid < a.nchrg) {
+      const int i = a.ichrg[c0 + tid];
+      const size_t ib = (size_t)i * a.P + b;
+      const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
+      sq[tid] = make_double4(x * hi[0] + y * hi[3] + z * hi[6], x * hi[1] + y * hi[4] + z * hi[7],
+                             x * hi[2] + y * hi[5] + z * hi[8], a.q[i]);
+    }
+    __syncthreads();
+    const int n = min(EWALD_BLOCK, a.nchrg - c0);
+    for (int m = 0; m < n; m++) {
+      const double4 c = sq[m];
+      double sn, cs;
+      sincos(kv.x * c.x + kv.y * c.y + kv.z * c.z, &sn, &cs);
+      sr += cs * c.w;
+      si += sn * c.w;
+    }
+  }
+  double vals[FFT_NV] = {0, 0, 0, 0, 0, 0, 0};
+  if (on) {
+    const double4 kg = a.kg[k];
+    const double smag = sr * sr + si * si;
+    const double e = smag * kv.w, pr = kg.w * smag;
+    vals[0] = e;
+    vals[1] = pr * kg.x * kg.x + e;
+    vals[2] = pr * kg.y * kg.y + e;
+    vals[3] = pr * kg.z * kg.z + e;
+    vals[4] = pr * kg.x * kg.y;
+    vals[5] = pr * kg.x * kg.z;
+    vals[6] = pr * kg.y * kg.z;
+    a.sk[(size_t)b * a.nk + k] = make_double2(2.0 * kv.w * sr, 2.0 * kv.w * si);
+  }
+  block_partials<FFT_NV>(vals, 1, b, a.P, a.part, blockIdx.x, gridDim.x);
+}
+
+// Forces: one thread per (charge, bead) sums, over LDS tiles of the k list
+// in ascending order, k (sin(k.r) 2 preg Re S - cos(k.r) 2 preg Im S), and
+// adds q times it to the force (each (charge, bead) is one thread's).
+__global__ void __launch_bounds__(EWALD_BLOCK) k_ewald_force(EwaldArgs a) {
+  __shared__ double4 skv[EWALD_BLOCK], skg[EWALD_BLOCK];
+  __shared__ double2 ssk[EWALD_BLOCK];
+  const int tid = threadIdx.x, b = blockIdx.y;
+  const int c = blockIdx.x * EWALD_BLOCK + tid;
+  const bool on = c < a.nchrg;
+  const double *hi = a.cell.hi;
+  int i = 0;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (on) {
+    i = a.ichrg[c];
+    const size_t ib = (size_t)i * a.P + b;
+    const double x = a.x[ib], y = a.y[ib], z = a.z[ib];
+    s0 = x * hi[0] + y * hi[3] + z * hi[6];
+    s1 = x * hi[1] + y * hi[4] + z * hi[7];
+    s2 = x * hi[2] + y * hi[5] + z * hi[8];
+  }
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int k0 = 0; k0 < a.nk; k0 += EWALD_BLOCK) {
+    __syncthreads();
+    if (k0 + tid < a.nk) {
+      skv[tid] = a.kv[k0 + tid];
+      skg[tid] = a.kg[k0 + tid];
+      ssk[tid] = a.sk[(size_t)b * a.nk + k0 + tid];
+    }
+    __syncthreads();
+    const int n = min(EWALD_BLOCK, a.nk - k0);
+    for (int m = 0; m < n; m++) {
+      double sn, cs;
+      sincos(skv[m].x * s0 + skv[m].y * s1 + skv[m].z * s2, &sn, &cs);
+      const double fd = sn * ssk[m].x - cs * ssk[m].y;
+      fx += skg[m].x * fd;
+      fy += skg[m].y * fd;
+      fz += skg[m].z * fd;
+    }
+  }
+  if (on) {
+    const size_t ib = (size_t)i * a.P + b;
+    const double q = a.q[i];
+    a.fx[ib] += fx * q;
+    a.fy[ib] += fy * q;
+    a.fz[ib] += fz * q;
+  }
+}
+
+hipError_t launch_ewald(const EwaldArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_ewald_sk, dim3((a.nk + EWALD_BLOCK - 1) / EWALD_BLOCK, a.P), dim3(EWALD_BLOCK), 0, st, a);
+  hipLaunchKernelGGL(k_ewald_force, dim3((a.nchrg + EWALD_BLOCK - 1) / EWALD_BLOCK, a.P), dim3(EWALD_BLOCK), 0, st,
+                     a);
+  return hipGetLastError();
+}
+
 hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {
   return a.f32 ? launch_gather_t<float>(a, st) : launch_gather_t<double>(a, st);
 }

@@ -60,6 +60,11 @@ CASES = {
                                         _st(cutoff=4.0, alpha=0.85, kmax_ewd=6, kmax_pme=6)),
     "spc64_noq": lambda: w.without_charges(w.build_config(w.SPC_FW, 64, w.density_box(64), 2, 23,
                                                           _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6))),
+    # standard Ewald (ewald_pme_opt off)
+    "spc64_ewald": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
+                                          _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, pme_on=0)),
+    "tip64_p4_ewald": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
+                                             _st(cutoff=5.5, alpha=0.62, kmax_ewd=7, kmax_pme=6, pme_on=0)),
     # boxes with at least 3 link cells per axis
     "tip512_p4": lambda: w.build_config(w.Q_TIP4P_F, 512, w.density_box(512), 4, 19,
                                         _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),

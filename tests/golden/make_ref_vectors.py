@@ -32,7 +32,7 @@ from tests import cases  # noqa: E402
 
 EXE = os.path.join(ROOT, "oracle", "_ref", "piny_ref")
 CASES = ["spc64", "spc64_shift", "spc64_none", "tip64", "cfg1", "spc64_k20", "tip64_p4", "cfg2_p4", "spc64_ortho",
-         "tip64_p4_tric"]
+         "tip64_p4_tric", "tip64_p6", "spc27_p64"]
 SHIFT = {0: "off", 1: "on", 2: "swit"}  # shift_inter_pe values (set_sim_dict.c:736)
 
 

@@ -39,7 +39,8 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("case", ["spc27", "spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg1", "cfg2_p8",
                                   "tip256_p32", "spc64_k20", "tip64_p4_k24", "cfg1_k40", "tip128_p4_k48",
-                                  "spc64_ortho", "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq"])
+                                  "spc64_ortho", "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq",
+                                  "spc64_ewald", "tip64_p4_ewald"])
 def test_energy_control_parity(case):
     c, o, g, eo, fo, eg, fg = _run(case)
     scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
@@ -72,7 +73,7 @@ MIXED_ETOL, MIXED_VTOL, MIXED_FTOL = 1e-5, 1e-4, 1e-4
 
 @pytest.mark.parametrize("mode", ["direct", "mixed"])
 @pytest.mark.parametrize("case", ["spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg2_p8", "tip256_p32",
-                                  "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq"])
+                                  "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq", "tip64_p4_ewald"])
 def test_pair_direct_and_mixed(case, mode):
     kw = {"pair_eval": abi.PMX_PAIR_DIRECT} if mode == "direct" else {"precision": abi.PMX_PREC_MIXED}
     c, o, g, eo, fo, eg, fg = _run(case, **kw)
@@ -264,3 +265,23 @@ def test_mixed_energy_conservation_within_2x_of_fp64():
         spread[mode] = np.max(np.abs(tot - tot[0]))
         print(f"  {mode}: max |E(t) - E(0)| = {spread[mode]:.3e} Ha over 1000 steps", flush=True)
     assert spread["mixed"] <= 2.0 * spread["fp64"] + 1e-6
+
+
+def test_ewald_matches_pme_limit():
+    # standard Ewald against PME on a fine grid of high order: the k-space
+    # totals agree to PME's interpolation error
+    import dataclasses
+    c = cases.get("spc64_ewald")
+    s = c.system
+    print("[ewald vs pme]", flush=True)
+    ge = Engine(s, c.x, c.y, c.z)
+    ge.set_positions(c.x, c.y, c.z)
+    ee = ge.energy_control()
+    st = dataclasses.replace(s.settings, pme_on=1, kmax_pme=14, n_interp=10)
+    sp = dataclasses.replace(s, settings=st)
+    gp = Engine(sp, c.x, c.y, c.z)
+    gp.set_positions(c.x, c.y, c.z)
+    ep = gp.energy_control()
+    assert abs(ep["vrecip"] - ee["vrecip"]) < 1e-6 * abs(ee["vrecip"]), (ep["vrecip"], ee["vrecip"])
+    fe, fp = ge.forces(), gp.forces()
+    assert np.sqrt(((fe - fp) ** 2).sum() / (fe ** 2).sum()) < 1e-4
