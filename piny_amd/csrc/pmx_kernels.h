@@ -111,7 +111,8 @@ struct EwaldArgs {
 constexpr int EWALD_BLOCK = 256;
 
 constexpr int FFT_MAX_PASSES = 12;
-constexpr int FFT_MAX_PLANE = 4096;  // complex values per LDS plane (64 KB)
+constexpr int FFT_MAX_PLANE = 4096;       // complex values per plane of the smaller class
+constexpr size_t FFT_LDS_MAX = 163840;    // bytes of LDS one workgroup may hold on MI355X
 
 struct FftArgs {
   int P, f32;

@@ -577,7 +577,8 @@ hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {
 // pass reads every butterfly's inputs into registers, waits, then writes the
 // outputs in place, so no permutation pass is needed.  Lanes run along the
 // contiguous direction (lines when stride > 1), which keeps LDS accesses
-// free of bank conflicts.  nlines * K <= FFT_MAX_PLANE and blockDim.x == 256.
+// free of bank conflicts.  nlines * K <= MAXP and blockDim.x == NT (the
+// plane classes of launch_fft_t).
 template <int R, typename T>
 __device__ __forceinline__ void dft_small(typename Cx<T>::C (&v)[R], int sign) {
   using C = typename Cx<T>::C;
@@ -623,11 +624,11 @@ __device__ __forceinline__ void dft_small(typename Cx<T>::C (&v)[R], int sign) {
   }
 }
 
-template <int R, typename T>
+template <int R, typename T, int NT, int MAXP>
 __device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int stride, int dist, int Ns, int sign,
                               const typename Cx<T>::C *__restrict__ tw) {
   using C = typename Cx<T>::C;
-  constexpr int MAXB = (FFT_MAX_PLANE / R + 255) / 256;
+  constexpr int MAXB = (MAXP / R + NT - 1) / NT;
   const int tid = threadIdx.x, nt = blockDim.x;
   const int KR = K / R, nb = nlines * KR;
   const bool lines_fast = stride != 1;
@@ -669,16 +670,16 @@ __device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int str
   __syncthreads();
 }
 
-template <typename T>
+template <typename T, int NT, int MAXP>
 __device__ void lds_fft(typename Cx<T>::C *buf, int K, int nlines, int stride, int dist, int sign,
                         const typename Cx<T>::C *tw, int npass, const int *radix) {
   int Ns = 1;
   for (int s = 0; s < npass; s++) {
     switch (radix[s]) {
-      case 2: stockham_pass<2, T>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
-      case 3: stockham_pass<3, T>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
-      case 4: stockham_pass<4, T>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
-      default: stockham_pass<5, T>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      case 2: stockham_pass<2, T, NT, MAXP>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      case 3: stockham_pass<3, T, NT, MAXP>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      case 4: stockham_pass<4, T, NT, MAXP>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
+      default: stockham_pass<5, T, NT, MAXP>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
     }
     Ns *= radix[s];
   }
@@ -695,8 +696,8 @@ __device__ __forceinline__ const typename Cx<T>::C *twiddles(const FftArgs &a, i
 // forward transform of one c-plane of one bead: real along a, then b.
 // LDS holds the plane as Kb lines of Ka complex values; only a' = 0..Ka/2
 // is kept (Hermitian half).  spec layout: [bead][c][b][a'] complex.
-template <typename T>
-__global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
+template <typename T, int NT, int MAXP>
+__global__ void __launch_bounds__(NT) k_fft_fwd_ab(FftArgs a) {
   using C = typename Cx<T>::C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *sm = (C *)lds_raw;
@@ -718,8 +719,8 @@ __global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T>(sm, Ka, Kb, 1, Ka, -1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
-  lds_fft<T>(sm, Kb, Ha, Ka, 1, -1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
+  lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, -1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
+  lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, -1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
   C *sp = (C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
   for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
     const int bb = w / Ha, aa = w % Ha;
@@ -731,8 +732,8 @@ __global__ void __launch_bounds__(256) k_fft_fwd_ab(FftArgs a) {
 // half-space sphere list (ewald3d_pme.c:367-430): energy, virial, and the
 // product with preg*bweight, then the inverse FFT.  Energy and virial are
 // summed in FP64 into one partial row per (b row, bead).
-template <typename T>
-__global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
+template <typename T, int NT, int MAXP>
+__global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
   using C = typename Cx<T>::C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *sm = (C *)lds_raw;
@@ -754,7 +755,7 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T>(sm, Kc, Ha, Ha, 1, -1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
+  lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, -1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
   const double *hi = a.cell.hi;
   const double tpi = 2.0 * M_PI, falp2 = a.falp2;
   const int kb = bb <= Kb / 2 ? bb : bb - Kb;
@@ -785,7 +786,7 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
     sm[w] = Cx<T>::make((T)(s.x * W), (T)(s.y * W));
   }
   __syncthreads();
-  lds_fft<T>(sm, Kc, Ha, Ha, 1, +1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
+  lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, +1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
     const int cc = w / Ha, aa = w % Ha;
     sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
@@ -797,8 +798,8 @@ __global__ void __launch_bounds__(256) k_fft_c_conv(FftArgs a) {
 }
 
 // inverse transform of one c-plane: along b, then complex-to-real along a.
-template <typename T>
-__global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
+template <typename T, int NT, int MAXP>
+__global__ void __launch_bounds__(NT) k_fft_inv_ba(FftArgs a) {
   using C = typename Cx<T>::C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *sm = (C *)lds_raw;
@@ -821,7 +822,7 @@ __global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T>(sm, Kb, Ha, Ka, 1, +1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
+  lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, +1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
   // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
   for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
     const int bb = w / Ka, aa = w % Ka;
@@ -831,22 +832,45 @@ __global__ void __launch_bounds__(256) k_fft_inv_ba(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T>(sm, Ka, Kb, 1, Ka, +1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
+  lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, +1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
   T *g = (T *)a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
   for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) g[w] = sm[w].x;
 }
 
+template <typename T, int NT, int MAXP>
+hipError_t launch_fft_class(const FftArgs &f, hipStream_t st) {
+  using C = typename Cx<T>::C;
+  const int Ka = f.K[0], Kb = f.K[1], Kc = f.K[2], Ha = Ka / 2 + 1;
+  const size_t lab = (size_t)Ka * Kb * sizeof(C), lch = (size_t)Kc * Ha * sizeof(C);
+  if (lab > 65536 || lch > 65536) {  // beyond 64 KB the kernels ask for the larger LDS
+    hipError_t e;
+    if ((e = hipFuncSetAttribute((const void *)k_fft_fwd_ab<T, NT, MAXP>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lab)) != hipSuccess)
+      return e;
+    if ((e = hipFuncSetAttribute((const void *)k_fft_c_conv<T, NT, MAXP>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lch)) != hipSuccess)
+      return e;
+    if ((e = hipFuncSetAttribute((const void *)k_fft_inv_ba<T, NT, MAXP>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lab)) != hipSuccess)
+      return e;
+  }
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_fwd_ab<T, NT, MAXP>), dim3(xcd_grid(Kc, f.P)), dim3(NT), lab, st, f);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_c_conv<T, NT, MAXP>), dim3(xcd_grid(Kb, f.P)), dim3(NT), lch, st, f);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_inv_ba<T, NT, MAXP>), dim3(xcd_grid(Kc, f.P)), dim3(NT), lab, st, f);
+  return hipGetLastError();
+}
+
+// Two plane classes: planes of up to FFT_MAX_PLANE points with 256 lanes,
+// larger ones (up to FFT_LDS_MAX bytes, the LDS a workgroup may hold on
+// MI355X) with 1024 lanes, so the registers that stage a Stockham pass stay
+// few in both.
 template <typename T>
 hipError_t launch_fft_t(const FftArgs &f, hipStream_t st) {
   using C = typename Cx<T>::C;
   const int Ka = f.K[0], Kb = f.K[1], Kc = f.K[2], Ha = Ka / 2 + 1;
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_fwd_ab<T>), dim3(xcd_grid(Kc, f.P)), dim3(256), (size_t)Ka * Kb * sizeof(C), st,
-                     f);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_c_conv<T>), dim3(xcd_grid(Kb, f.P)), dim3(256), (size_t)Kc * Ha * sizeof(C), st,
-                     f);
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fft_inv_ba<T>), dim3(xcd_grid(Kc, f.P)), dim3(256), (size_t)Ka * Kb * sizeof(C), st,
-                     f);
-  return hipGetLastError();
+  if ((size_t)Ka * Kb <= FFT_MAX_PLANE && (size_t)Kc * Ha <= FFT_MAX_PLANE)
+    return launch_fft_class<T, 256, FFT_MAX_PLANE>(f, st);
+  return launch_fft_class<T, 1024, (int)(FFT_LDS_MAX / sizeof(C))>(f, st);
 }
 
 // forward 3-D FFT, convolution with energy and virial, inverse 3-D FFT

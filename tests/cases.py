@@ -60,6 +60,12 @@ CASES = {
                                         _st(cutoff=4.0, alpha=0.85, kmax_ewd=6, kmax_pme=6)),
     "spc64_noq": lambda: w.without_charges(w.build_config(w.SPC_FW, 64, w.density_box(64), 2, 23,
                                                           _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6))),
+    # PME grids whose planes exceed 4096 points (the larger FFT class):
+    # 80^3 and, with beads, 96^3 (a 96 x 49 complex plane is 75 KB in FP64)
+    "spc64_k80": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
+                                        _st(cutoff=5.5, alpha=0.62, kmax_ewd=38, kmax_pme=38)),
+    "tip64_p4_k96": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
+                                           _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=44)),
     # standard Ewald (ewald_pme_opt off)
     "spc64_ewald": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
                                           _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, pme_on=0)),

@@ -40,7 +40,7 @@ def _rel(a, b):
 @pytest.mark.parametrize("case", ["spc27", "spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg1", "cfg2_p8",
                                   "tip256_p32", "spc64_k20", "tip64_p4_k24", "cfg1_k40", "tip128_p4_k48",
                                   "spc64_ortho", "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq",
-                                  "spc64_ewald", "tip64_p4_ewald"])
+                                  "spc64_ewald", "tip64_p4_ewald", "spc64_k80", "tip64_p4_k96"])
 def test_energy_control_parity(case):
     c, o, g, eo, fo, eg, fg = _run(case)
     scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
@@ -73,7 +73,8 @@ MIXED_ETOL, MIXED_VTOL, MIXED_FTOL = 1e-5, 1e-4, 1e-4
 
 @pytest.mark.parametrize("mode", ["direct", "mixed"])
 @pytest.mark.parametrize("case", ["spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg2_p8", "tip256_p32",
-                                  "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq", "tip64_p4_ewald"])
+                                  "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq", "tip64_p4_ewald",
+                                  "spc64_k80", "tip64_p4_k96"])
 def test_pair_direct_and_mixed(case, mode):
     kw = {"pair_eval": abi.PMX_PAIR_DIRECT} if mode == "direct" else {"precision": abi.PMX_PREC_MIXED}
     c, o, g, eo, fo, eg, fg = _run(case, **kw)
