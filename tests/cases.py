@@ -12,6 +12,20 @@ def _st(**kw):
     return w.Settings(**kw)
 
 
+def _w512(pme_on):
+    """The reference's own example box (examples/q-TIP4P-F/W512-initial.xyz,
+    512 q-TIP4P/F, L = 24.832 A, ewald_alpha 10, kmax 13, PME 60^3 order 8)."""
+    import os
+    import numpy as np
+    d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "w512_ewald_kat.npz"))
+    L = float(d["box_A"])
+    st = w.Settings(cutoff=9.0, alpha=float(d["ewald_alpha"]) / L, kmax_ewd=13, kmax_pme=25, n_interp=8,
+                    pme_on=pme_on)
+    s = w.make_system(w.Q_TIP4P_F, 512, L, 1, st)
+    p = d["pos_A"] / w.BOHR
+    return w.Config(s, p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy())
+
+
 CASES = {
     # 27 SPC/Fw, P = 1, K = 16
     "spc27": lambda: w.build_config(w.SPC_FW, 27, w.density_box(27), 1, 11,
@@ -66,6 +80,9 @@ CASES = {
                                         _st(cutoff=5.5, alpha=0.62, kmax_ewd=38, kmax_pme=38)),
     "tip64_p4_k96": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
                                            _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=44)),
+    # the reference's own W512 example, PME and standard Ewald
+    "w512": lambda: _w512(1),
+    "w512_ewald": lambda: _w512(0),
     # standard Ewald (ewald_pme_opt off)
     "spc64_ewald": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
                                           _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, pme_on=0)),

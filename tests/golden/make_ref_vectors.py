@@ -32,7 +32,8 @@ from tests import cases  # noqa: E402
 
 EXE = os.path.join(ROOT, "oracle", "_ref", "piny_ref")
 CASES = ["spc64", "spc64_shift", "spc64_none", "tip64", "cfg1", "spc64_k20", "tip64_p4", "cfg2_p4", "spc64_ortho",
-         "tip64_p4_tric", "tip64_p6", "spc27_p64"]
+         "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq", "spc64_ewald",
+         "tip64_p4_ewald", "spc64_k80", "w512", "w512_ewald"]
 SHIFT = {0: "off", 1: "on", 2: "swit"}  # shift_inter_pe values (set_sim_dict.c:736)
 
 
@@ -87,7 +88,7 @@ def write_inputs(d, c):
     qs = np.asarray(s.q).reshape(s.nmol, len(m.sites))
     assert np.all(qs == qs[0])
     for k, (name, q, mass) in enumerate(zip(m.sites, qs[0], m.masses_amu)):
-        items = [("atom_typ", name), ("atom_ind", k + 1), ("mass", repr(mass)), ("charge", repr(q))]
+        items = [("atom_typ", name), ("atom_ind", k + 1), ("mass", repr(float(mass))), ("charge", repr(float(q)))]
         for g, (comp, coef) in enumerate(ghost.get(k, ())):
             items.append(("def_ghost%d" % (g + 1), "%d, %r" % (comp + 1, coef)))
         parm.append(section("atom_def", items))
