@@ -88,6 +88,48 @@ def cpu_baseline(cfg, seconds=12.0):
                       f"{el:.1f} s on one host core"}
 
 
+def cpu_baseline_all_cores(cfg, procs=None, repeat=10):
+    """The reference on all host cores the way PINYEmpirical runs replicas
+    (one energy_control per replica per host thread, piny.pyx:426-436): one
+    oracle/_ref/piny_ref process per core, each on its own bead of the
+    workload box as a classical configuration, all at once; the throughput
+    is the sum over processes.  None without the reference build."""
+    import subprocess
+    import tempfile
+    from piny_amd import water as w
+    exe = os.path.join(ROOT, "oracle", "_ref", "piny_ref")
+    if not os.path.exists(exe):
+        return None
+    from tests.golden.make_ref_vectors import write_inputs
+    s = cfg.system
+    n = s.natm
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    procs = procs or max(1, min(16, ncpu, s.pi_beads))
+    s1 = w.make_system(s.model, s.nmol, s.box, 1, s.settings)
+    with tempfile.TemporaryDirectory() as root:
+        runs = []
+        for p in range(procs):
+            d = os.path.join(root, str(p))
+            os.makedirs(d)
+            x, y, z = (v[p * n:(p + 1) * n].copy() for v in (cfg.x, cfg.y, cfg.z))
+            write_inputs(d, w.Config(system=s1, x=x, y=y, z=z))
+            runs.append(subprocess.Popen([exe, "water.input", "pos.bin", "out.bin", str(repeat)], cwd=d,
+                                         stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True))
+        total, secs = 0.0, []
+        for r in runs:
+            out, _ = r.communicate(timeout=900)
+            line = [ln for ln in out.splitlines() if ln.startswith("REPEAT")]
+            if r.returncode != 0 or not line:
+                return None
+            k, el = int(line[0].split()[1]), float(line[0].split()[2])
+            total += k / el
+            secs.append(el)
+    return {"value": total, "unit": "bead-force evals/s", "cores": procs, "kind": "reference",
+            "sample": f"{procs} concurrent processes of the reference (built from its sources), each {repeat} "
+                      f"energy_control calls on its own bead of the workload box (16384 sites) as a classical "
+                      f"configuration, list reused; {min(secs):.1f}-{max(secs):.1f} s each"}
+
+
 def run_path(cfg, steps, warmup, **engine_kw):
     from piny_amd import abi
     from piny_amd.engine import Engine
@@ -288,6 +330,7 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg)
+        out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(cfg)
     print(json.dumps(out))
 
 
