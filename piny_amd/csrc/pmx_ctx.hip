@@ -426,10 +426,8 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(upload(c, &c->pair_pp, pp.data(), pp.size()));
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
-    if (c->pair_mode == 2 && P > 1) {
-      TRY(dalloc(c, &c->disp, NP));
-      TRY(dalloc(c, &c->umax, (size_t)N));
-    }
+    if (c->pair_mode == 2 && P > 1) TRY(dalloc(c, &c->disp, NP));
+    if (P > 1) TRY(dalloc(c, &c->umax, (size_t)N));
   }
   TRY(dalloc(c, &c->site_cell, N));
   TRY(dalloc(c, &c->site_pos, N));
@@ -848,6 +846,10 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
       hipLaunchKernelGGL(k_pack_disp, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm,
                          c->disp);
       hipLaunchKernelGGL(k_site_umax, dim3(blocks(N)), dim3(256), 0, st, N, P, c->disp, c->umax);
+      HIPCHK(hipGetLastError());
+    } else if (c->umax && c->full_list) {
+      hipLaunchKernelGGL(k_site_umax64, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym,
+                         c->zm, c->umax);
       HIPCHK(hipGetLastError());
     }
     PairArgs a{};

@@ -835,6 +835,129 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
 }
 
+// FP64 pair kernel over the full list, staged like the mixed one: per
+// chunk of 64 entries one lane per entry forms, in FP64, the imaged
+// centroid separation plus the partner's centroid, F = D_ij + xmod_j, and
+// drops the entries no bead can bring inside the cutoff; the (slot, bead)
+// lanes then form r = (x_i - xmod_i) + F - x_j (the separation the
+// reference forms as x_i - x_j plus the xmod image shift, to rounding) and
+// evaluate the term with pair_term<MODE> (tables or direct).  Without beads
+// F is the imaged separation itself.
+template <int B, int MODE>
+__global__ void __launch_bounds__(256) k_pair_staged(PairArgs a) {
+  __shared__ double spp[MAX_PAIRS * PP_NUM];
+  __shared__ double4 sdf[4][64];
+  __shared__ int sent[4][64];
+  for (int k = threadIdx.x; k < a.npair * PP_NUM; k += blockDim.x) spp[k] = a.pp[k];
+  __syncthreads();
+  constexpr int S = 64 / B;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int slot = lane / B;
+  const int bead = (lane % B) + blockIdx.y * B;
+  const bool bact = bead < a.P;
+  const int P = a.P;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
+  const Cell &c = a.cell;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  PairAcc s{};
+  for (int i = wave; i < a.N; i += nwave) {
+    const size_t ib = (size_t)i * P + (bact ? bead : 0);
+    const double4 si = a.site[i];
+    double uix = 0.0, uiy = 0.0, uiz = 0.0;
+    if (a.pimd) {
+      uix = a.x[ib] - si.x;
+      uiy = a.y[ib] - si.y;
+      uiz = a.z[ib] - si.z;
+    }
+    const int cnt = a.cnt[i];
+    const int *row = a.jlist + (size_t)i * a.cap;
+    s.fx = s.fy = s.fz = 0.0;
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const int ks = k0 + lane;
+      bool keep = false;
+      int e = 0;
+      double dx = 0.0, dy = 0.0, dz = 0.0, q2 = 0.0, lim = 0.0;
+      if (ks < cnt) {
+        e = row[ks];
+        const int j = e & ENTRY_MASK;
+        const double4 sj = a.site[j];
+        if (a.pimd) {
+          min_image_sep(c, 1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, si.x, si.y, si.z, sj.x, sj.y, sj.z, dx, dy, dz);
+          dx += si.x - sj.x;
+          dy += si.y - sj.y;
+          dz += si.z - sj.z;
+          lim = (spp[(e >> ENTRY_SHIFT) * PP_NUM + PP_CUT] + a.umax[i] + a.umax[j]) * (1.0 + 1e-9);
+          keep = dx * dx + dy * dy + dz * dz < lim * lim;
+          dx += sj.x;  // F = D + xmod_j
+          dy += sj.y;
+          dz += sj.z;
+        } else {
+          min_image_sep(c, 0, a.x[i], a.y[i], a.z[i], a.x[j], a.y[j], a.z[j], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dx,
+                        dy, dz);
+          lim = spp[(e >> ENTRY_SHIFT) * PP_NUM + PP_CUT] * (1.0 + 1e-9);
+          keep = dx * dx + dy * dy + dz * dz < lim * lim;
+        }
+        q2 = si.w * sj.w;
+      }
+      const uint64_t mk = __ballot(keep);
+      if (keep) {
+        const int pos = __popcll(mk & below);
+        sdf[w][pos] = make_double4(dx, dy, dz, q2);
+        sent[w][pos] = e;
+      }
+      wave_sync();
+      const int nk = __popcll(mk);
+      for (int p = slot; p < nk; p += S) {
+        const double4 f = sdf[w][p];
+        const int en = sent[w][p];
+        const int j = en & ENTRY_MASK, t = en >> ENTRY_SHIFT;
+        double rx = f.x, ry = f.y, rz = f.z;
+        if (a.pimd) {
+          const size_t jb = (size_t)j * P + (bact ? bead : 0);
+          rx = uix + (f.x - a.x[jb]);
+          ry = uiy + (f.y - a.y[jb]);
+          rz = uiz + (f.z - a.z[jb]);
+        }
+        double dvd;
+        if (bact) pair_term<MODE>(a, spp + t * PP_NUM, t, f.w, rx, ry, rz, rx * rx + ry * ry + rz * rz, s, dvd);
+      }
+      wave_sync();
+    }
+    double fxi = s.fx, fyi = s.fy, fzi = s.fz;
+    for (int off = 32; off >= B; off >>= 1) {
+      fxi += __shfl_xor(fxi, off, 64);
+      fyi += __shfl_xor(fyi, off, 64);
+      fzi += __shfl_xor(fzi, off, 64);
+    }
+    if (slot == 0 && bact) {  // the first writer of this step's forces
+      a.fx[ib] = fxi;
+      a.fy[ib] = fyi;
+      a.fz[ib] = fzi;
+    }
+  }
+  const double vals[PAIR_NV] = {s.vlj * 0.5, s.vc * 0.5, s.p11 * 0.5, s.p22 * 0.5, s.p33 * 0.5,
+                                s.p12 * 0.5, s.p13 * 0.5, s.p23 * 0.5, s.npr * 0.5};
+  block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
+}
+
+// The largest |x - xmod| of every site over its beads, in FP64 (one thread
+// per site), for the FP64 staged kernel's culling.
+__global__ void k_site_umax64(int N, int P, const double *__restrict__ x, const double *__restrict__ y,
+                              const double *__restrict__ z, const double *__restrict__ xm,
+                              const double *__restrict__ ym, const double *__restrict__ zm,
+                              double *__restrict__ umax) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  double m = 0.0;
+  for (int b = 0; b < P; b++) {
+    const size_t t = (size_t)i * P + b;
+    const double dx = x[t] - xm[i], dy = y[t] - ym[i], dz = z[t] - zm[i];
+    m = fmax(m, dx * dx + dy * dy + dz * dz);
+  }
+  umax[i] = sqrt(m);
+}
+
 // ---------------------------------------------------------------------------
 // Parameter gradients of the real-space term (the backward of force_npol).
 // Per (site, bead): phi_i = sum_j q_j vc(r_ij), the derivative of the bead's
@@ -1340,8 +1463,8 @@ void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_
   const dim3 block(256);
 #define PMX_PAIR_LAUNCH(M, F) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<B, M, F>), grid, block, 0, st, a)
   if (full) {
-    if (mode == 0) PMX_PAIR_LAUNCH(0, true);
-    else if (mode == 1) PMX_PAIR_LAUNCH(1, true);
+    if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged<B, 0>), grid, block, 0, st, a);
+    else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged<B, 1>), grid, block, 0, st, a);
     else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B>), grid, block, 0, st, a);
   } else {
     if (mode == 0) PMX_PAIR_LAUNCH(0, false);
