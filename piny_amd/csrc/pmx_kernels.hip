@@ -452,10 +452,11 @@ __global__ void __launch_bounds__(256) k_split_rows(ListArgs a, int *out) {
 // owns one site i at a time; its 64 lanes are (slot, bead) with B beads per
 // slot, so the B lanes of a slot evaluate the same pair for B beads and
 // touch contiguous memory.  The i force is reduced over slots with shuffles.
-// With the full list every pair is met from both sites: each wavefront
-// stores its own sites' forces and the energies and virial carry a factor
-// 1/2, so the kernel has no atomics.  With the half list the j forces are
-// scattered with FP64 atomics.
+// With the full list (the default: k_pair_staged, k_pair_mixed) every pair
+// is met from both sites: each wavefront stores its own sites' forces and
+// the energies and virial carry a factor 1/2, so there are no atomics.  With
+// the half list (k_pair_half, a tuning switch) the j forces are scattered
+// with FP64 atomics.
 //
 // MODE 0 interpolates PINY's spline tables in FP64 (npol_vspl_fetch); MODE 1
 // evaluates the tabulated functions directly in FP64; MODE 2 does that in
@@ -603,8 +604,8 @@ __device__ __forceinline__ bool pair_term(const PairArgs &a, const double *pp, i
   return true;
 }
 
-template <int B, int MODE, bool FULL>
-__global__ void __launch_bounds__(256) k_pair(PairArgs a) {
+template <int B, int MODE>
+__global__ void __launch_bounds__(256) k_pair_half(PairArgs a) {
   __shared__ double spp[MAX_PAIRS * PP_NUM];
   for (int k = threadIdx.x; k < a.npair * PP_NUM; k += blockDim.x) spp[k] = a.pp[k];
   __syncthreads();
@@ -655,12 +656,10 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
                                          sj[u].y, sj[u].z, dx, dy, dz);
         double dvd;
         if (bact && pair_term<MODE>(a, spp + t * PP_NUM, t, si.w * sj[u].w, dx, dy, dz, r2d, s, dvd)) {
-          if constexpr (!FULL) {
-            const size_t jb = (size_t)j * P + bead;
-            atomic_add(a.fx + jb, -(dx * dvd));
-            atomic_add(a.fy + jb, -(dy * dvd));
-            atomic_add(a.fz + jb, -(dz * dvd));
-          }
+          const size_t jb = (size_t)j * P + bead;
+          atomic_add(a.fx + jb, -(dx * dvd));
+          atomic_add(a.fy + jb, -(dy * dvd));
+          atomic_add(a.fz + jb, -(dz * dvd));
         }
       }
     }
@@ -670,21 +669,13 @@ __global__ void __launch_bounds__(256) k_pair(PairArgs a) {
       fyi += __shfl_xor(fyi, off, 64);
       fzi += __shfl_xor(fzi, off, 64);
     }
-    if (slot == 0 && bact) {
-      if constexpr (FULL) {  // the first writer of this step's forces
-        a.fx[ib] = fxi;
-        a.fy[ib] = fyi;
-        a.fz[ib] = fzi;
-      } else if (cnt > 0) {
-        atomic_add(a.fx + ib, fxi);
-        atomic_add(a.fy + ib, fyi);
-        atomic_add(a.fz + ib, fzi);
-      }
+    if (slot == 0 && bact && cnt > 0) {
+      atomic_add(a.fx + ib, fxi);
+      atomic_add(a.fy + ib, fyi);
+      atomic_add(a.fz + ib, fzi);
     }
   }
-  const double h = FULL ? 0.5 : 1.0;
-  const double vals[PAIR_NV] = {s.vlj * h, s.vc * h, s.p11 * h, s.p22 * h, s.p33 * h,
-                                s.p12 * h, s.p13 * h, s.p23 * h, s.npr * h};
+  const double vals[PAIR_NV] = {s.vlj, s.vc, s.p11, s.p22, s.p33, s.p12, s.p13, s.p23, s.npr};
   block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
 }
 
@@ -1461,17 +1452,15 @@ int pair_grid_x(int N, int max_waves) {
 template <int B>
 void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_t st) {
   const dim3 block(256);
-#define PMX_PAIR_LAUNCH(M, F) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair<B, M, F>), grid, block, 0, st, a)
   if (full) {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged<B, 0>), grid, block, 0, st, a);
     else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged<B, 1>), grid, block, 0, st, a);
     else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B>), grid, block, 0, st, a);
   } else {
-    if (mode == 0) PMX_PAIR_LAUNCH(0, false);
-    else if (mode == 1) PMX_PAIR_LAUNCH(1, false);
-    else PMX_PAIR_LAUNCH(2, false);
+    if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 0>), grid, block, 0, st, a);
+    else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 1>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 2>), grid, block, 0, st, a);
   }
-#undef PMX_PAIR_LAUNCH
 }
 
 hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st) {
