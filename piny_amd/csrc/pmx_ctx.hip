@@ -229,8 +229,9 @@ constexpr int MAX_LIST_CELLS = 1 << 18;
     if (rc_) return rc_; \
   } while (0)
 
-// Chooses the PME tile (largest of 16 x 16 x 8 and its halvings whose halo
-// box fits 64 KB of LDS), allocates the binning buffers and
+// Chooses the PME tile (the largest divisors of the grid below the caps
+// whose halo box fits the LDS of one workgroup), allocates the binning
+// buffers and
 // fills the constant part of the PME argument block.
 int setup_pme_tiles(pmx_ctx *c) {
   Setup &s = c->s;
@@ -277,7 +278,7 @@ int setup_pme_tiles(pmx_ctx *c) {
     bool ok = true;
     for (int d = 0; d < 3; d++) a.nbr[d] = 1 + (n - 1 + a.T[d] - 1) / a.T[d];
     ok = ok && (size_t)a.nt[0] * a.nt[1] * a.nt[2] * sizeof(int) <= 65536;  // LDS histogram of k_pme_bin
-    ok = ok && pme_lds_bytes(a, true) <= 65536 && pme_lds_bytes(a, false) <= 65536;
+    ok = ok && pme_lds_bytes(a, true) <= FFT_LDS_MAX && pme_lds_bytes(a, false) <= FFT_LDS_MAX;
     if (ok) break;
     int dm = 0;
     for (int d = 1; d < 3; d++)

@@ -412,7 +412,11 @@ hipError_t launch_spread_t(const PmeArgs &a, hipStream_t st) {
   const size_t lds = pme_lds_bytes(a, false);
   switch (a.n) {
 #define X(NI) \
-  case NI: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_spread<NI, T>), grid, block, lds, st, a); break;
+  case NI:                                                                                                  \
+    if (lds > 65536)                                                                                          \
+      (void)hipFuncSetAttribute((const void *)k_pme_spread<NI, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_spread<NI, T>), grid, block, lds, st, a);                                \
+    break;
     PMX_PME_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -427,7 +431,11 @@ hipError_t launch_gather_t(const PmeArgs &a, hipStream_t st) {
   const size_t lds = pme_lds_bytes(a, true);
   switch (a.n) {
 #define X(NI) \
-  case NI: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_gather<NI, T>), grid, block, lds, st, a); break;
+  case NI:                                                                                                  \
+    if (lds > 65536)                                                                                          \
+      (void)hipFuncSetAttribute((const void *)k_pme_gather<NI, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_gather<NI, T>), grid, block, lds, st, a);                                \
+    break;
     PMX_PME_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
@@ -446,7 +454,11 @@ hipError_t launch_phi_t(const PmeArgs &a, hipStream_t st) {
   const size_t lds = pme_lds_bytes(a, true);
   switch (a.n) {
 #define X(NI) \
-  case NI: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_phi<NI, T>), grid, block, lds, st, a); break;
+  case NI:                                                                                                  \
+    if (lds > 65536)                                                                                          \
+      (void)hipFuncSetAttribute((const void *)k_pme_phi<NI, T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); \
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_phi<NI, T>), grid, block, lds, st, a);                                \
+    break;
     PMX_PME_CASES(X)
 #undef X
     default: return hipErrorInvalidValue;
