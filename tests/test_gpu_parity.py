@@ -286,3 +286,18 @@ def test_ewald_matches_pme_limit():
     assert abs(ep["vrecip"] - ee["vrecip"]) < 1e-6 * abs(ee["vrecip"]), (ep["vrecip"], ee["vrecip"])
     fe, fp = ge.forces(), gp.forces()
     assert np.sqrt(((fe - fp) ** 2).sum() / (fe ** 2).sum()) < 1e-4
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_large_pme_tiles(mode, monkeypatch):
+    # tiles whose halo box needs more than 64 KB of LDS (32 x 16 x 16 on a
+    # 32^3 grid: 37 x 21 x 21 points, 130 KB in FP64)
+    monkeypatch.setenv("PMX_PME_TILE", "32,16,16")
+    kw = {} if mode == "fp64" else {"precision": abi.PMX_PREC_MIXED}
+    c, o, g, eo, fo, eg, fg = _run("cfg1", **kw)
+    etol, ftol = (ETOL, FTOL) if mode == "fp64" else (MIXED_ETOL, MIXED_FTOL)
+    scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
+    for k in abi.ENERGY_KEYS:
+        assert abs(eg[k] - eo[k]) <= etol * max(abs(eo[k]), scale), (k, eg[k], eo[k])
+    rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
+    assert rms < ftol
