@@ -769,13 +769,9 @@ int list_stage_launch(pmx_ctx *c, hipStream_t st, bool timing) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
   if (timing) HIPCHK(hipEventRecord(c->ev[0], st));
-  if (P > 1 && !c->xmod_given) {
-    hipLaunchKernelGGL(k_centroid, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm);
-    HIPCHK(hipGetLastError());
-  }
   HIPCHK(hipMemsetAsync(c->maxes, 0, 2 * sizeof(unsigned long long), st));
   hipLaunchKernelGGL(k_list_check, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->x0, c->y0, c->z0,
-                     c->b0x, c->b0y, c->b0z, c->maxes);
+                     c->b0x, c->b0y, c->b0z, c->maxes, (P > 1 && !c->xmod_given) ? 1 : 0, c->xm, c->ym, c->zm);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->mx_pin, c->maxes, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
   return PMX_OK;

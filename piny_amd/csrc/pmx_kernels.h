@@ -208,8 +208,6 @@ __global__ void k_bead_to_atom(int N, int P, const double *src, double *dst);
 __global__ void k_atom_to_bead(int N, int P, const double *src, double *dst);
 __global__ void k_ghost_pos(int ng, int P, const int *gatom, const int *gncomp, const int *gcomp,
                             const double *gcoef, double *x, double *y, double *z);
-__global__ void k_centroid(int N, int P, const double *x, const double *y, const double *z, double *xm,
-                           double *ym, double *zm);
 __global__ void k_pack_sites(int N, int pimd, const double *xm, const double *ym, const double *zm,
                              const double *q, double4 *site);
 __global__ void k_pack_disp(int N, int P, const double *x, const double *y, const double *z, const double *xm,
@@ -217,9 +215,9 @@ __global__ void k_pack_disp(int N, int P, const double *x, const double *y, cons
 __global__ void k_site_umax(int N, int P, const float4 *u, double *umax);
 __global__ void k_site_umax64(int N, int P, const double *x, const double *y, const double *z, const double *xm,
                               const double *ym, const double *zm, double *umax);
-__global__ void k_list_check(int N, int P, const double *x, const double *y, const double *z,
-                             const double *x0, const double *y0, const double *z0, double *b0x, double *b0y,
-                             double *b0z, unsigned long long *maxes);
+__global__ void k_list_check(int N, int P, const double *x, const double *y, const double *z, const double *x0,
+                             const double *y0, const double *z0, double *b0x, double *b0y, double *b0z,
+                             unsigned long long *maxes, int centroid, double *xm, double *ym, double *zm);
 __global__ void k_list_build(ListArgs a);
 __global__ void k_cell_count(ListArgs a);
 __global__ void k_cell_scan(int ncell, const int *cnt, int *start);

@@ -33,14 +33,24 @@ __global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {
   const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
   if (w >= a.P * a.nslot) return;  // uniform over the wavefront
   const int b = w / a.nslot, slot = w % a.nslot;
-  double s = 0.0;
+  // four independent sums per lane keep four loads in flight
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
   for (int si = 0; si < a.nstage; si++) {
     const RedStage &r = a.st[si];
     for (int v = 0; v < r.nv; v++) {
       if (r.slot[v] != slot) continue;
-      for (int g = lane; g < r.G; g += 64) s += a.part[r.off + ((size_t)v * a.P + b) * r.G + g];
+      const double *row = a.part + r.off + ((size_t)v * a.P + b) * r.G;
+      int g = lane;
+      for (; g + 192 < r.G; g += 256) {
+        s0 += row[g];
+        s1 += row[g + 64];
+        s2 += row[g + 128];
+        s3 += row[g + 192];
+      }
+      for (; g < r.G; g += 64) s0 += row[g];
     }
   }
+  double s = (s0 + s1) + (s2 + s3);
   for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) a.acc[(size_t)b * a.nslot + slot] = s;
 }
@@ -84,24 +94,6 @@ __global__ void k_ghost_pos(int ng, int P, const int *__restrict__ gatom, const 
   z[ig] = sz;
 }
 
-// bead centroid as the mode-1 coordinate of the normal-mode transform
-// (control_pimd_trans.c:175-181 takes xmod from mode 1)
-__global__ void k_centroid(int N, int P, const double *__restrict__ x, const double *__restrict__ y,
-                           const double *__restrict__ z, double *xm, double *ym, double *zm) {
-#pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  for (int b = 0; b < P; b++) {
-    sx += x[(size_t)i * P + b];
-    sy += y[(size_t)i * P + b];
-    sz += z[(size_t)i * P + b];
-  }
-  xm[i] = sx / P;
-  ym[i] = sy / P;
-  zm[i] = sz / P;
-}
-
 // (xmod, q) of every site in one 32-byte record for the force kernels
 __global__ void k_pack_sites(int N, int pimd, const double *__restrict__ xm, const double *__restrict__ ym,
                              const double *__restrict__ zm, const double *__restrict__ q, double4 *site) {
@@ -136,13 +128,16 @@ __global__ void k_site_umax(int N, int P, const float4 *__restrict__ u, double *
 }
 
 // Bead-1 copy, largest displacement since the last list build
-// (check_ver_list, nbr_list_control.c:182-239) and bead spread
-// (get_pimd_spread, pimd_estimators.c:121-222).  Maxima of squares are
+// (check_ver_list, nbr_list_control.c:182-239), bead spread
+// (get_pimd_spread, pimd_estimators.c:121-222) and, when asked, the bead
+// centroid as xmod, the mode-1 coordinate of the normal-mode transform
+// (control_pimd_trans.c:175-181 takes xmod from mode 1).  Maxima of squares are
 // reduced as the bit patterns of non-negative doubles.
 __global__ void k_list_check(int N, int P, const double *__restrict__ x, const double *__restrict__ y,
                              const double *__restrict__ z, const double *__restrict__ x0,
                              const double *__restrict__ y0, const double *__restrict__ z0, double *b0x,
-                             double *b0y, double *b0z, unsigned long long *maxes) {
+                             double *b0y, double *b0z, unsigned long long *maxes, int centroid, double *xm,
+                             double *ym, double *zm) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double dmax = 0.0, smax = 0.0;
@@ -153,10 +148,21 @@ __global__ void k_list_check(int N, int P, const double *__restrict__ x, const d
     b0z[i] = zi;
     const double dx = xi - x0[i], dy = yi - y0[i], dz = zi - z0[i];
     dmax = dx * dx + dy * dy + dz * dz;
+    // the centroid in the same pass (same order of sums as bead 0, 1, ...)
+    double sx = xi, sy = yi, sz = zi;
     for (int b = 1; b < P; b++) {
-      const double ex = x[(size_t)i * P + b] - xi, ey = y[(size_t)i * P + b] - yi, ez = z[(size_t)i * P + b] - zi;
+      const double xb = x[(size_t)i * P + b], yb = y[(size_t)i * P + b], zb = z[(size_t)i * P + b];
+      const double ex = xb - xi, ey = yb - yi, ez = zb - zi;
       const double r2 = ex * ex + ey * ey + ez * ez;
       smax = fmax(smax, r2);
+      sx += xb;
+      sy += yb;
+      sz += zb;
+    }
+    if (centroid) {
+      xm[i] = sx / P;
+      ym[i] = sy / P;
+      zm[i] = sz / P;
     }
   }
   for (int off = 32; off >= 1; off >>= 1) {
