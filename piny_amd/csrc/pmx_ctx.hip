@@ -279,6 +279,8 @@ int setup_pme_tiles(pmx_ctx *c) {
     for (int d = 0; d < 3; d++) a.nbr[d] = 1 + (n - 1 + a.T[d] - 1) / a.T[d];
     ok = ok && (size_t)a.nt[0] * a.nt[1] * a.nt[2] * sizeof(int) <= 65536;  // LDS histogram of k_pme_bin
     ok = ok && pme_lds_bytes(a, true) <= FFT_LDS_MAX && pme_lds_bytes(a, false) <= FFT_LDS_MAX;
+    // 32-bit element offsets into one bead's halo boxes
+    ok = ok && (size_t)a.nt[0] * a.nt[1] * a.nt[2] * a.hvol < (1ull << 31);
     if (ok) break;
     int dm = 0;
     for (int d = 1; d < 3; d++)
@@ -924,6 +926,18 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     f.falp2 = 4.0 * s.alp_ewd * s.alp_ewd;
     f.cell = c->cell;
     f.part = c->part + c->off_fft;
+    f.hbuf = a.hbuf;
+    for (int d = 0; d < 3; d++) {
+      f.T[d] = a.T[d];
+      f.nt[d] = a.nt[d];
+      f.H[d] = a.H[d];
+      f.nbr[d] = a.nbr[d];
+    }
+    f.ntile = a.ntile;
+    f.hvol = a.hvol;
+    f.mKa = (unsigned)((1ull << 32) / (unsigned)s.ngrid[0] + 1ull);
+    f.mT0 = a.mT0;
+    f.mT1 = a.mT1;
     HIPCHK(launch_pme_fft(f, st));
     HIPCHK(launch_pme_gather(a, st));
     const int Kb = s.ngrid[1];

@@ -127,6 +127,12 @@ struct FftArgs {
   double falp2;
   Cell cell;
   double *part;  // [Kb][P][FFT_NV]
+  // The forward transform reads the spread charge from the halo boxes of the
+  // PME tiling (PmeArgs) and sums their overlaps as it loads each plane.
+  const void *hbuf;  // [P*ntile*hvol] (grid type)
+  int T[3], nt[3], H[3], nbr[3], ntile;
+  size_t hvol;
+  unsigned mKa, mT0, mT1;  // magic divisors of K[0], T[0], T[1]
 };
 
 struct IntraArgs {

@@ -10,8 +10,9 @@
 //  - one workgroup per (tile, bead) spreads the charges of its bin, one lane
 //    per charge, into an LDS halo box of (T + n - 1)^3 points with LDS
 //    atomics and writes the box out once (no global atomics, no grid clear);
-//  - k_pme_fold sums, per tile, the tile's own box and the overhang of the
-//    boxes below it into the periodic grid;
+//  - the forward FFT sums, as it loads each grid plane, every point's own
+//    box and the overhang of the boxes below it (the fold into the periodic
+//    grid never goes through memory);
 //  - the gather loads the tile's halo box of (T + n - 1)^3 potential values
 //    into LDS and each lane reduces one charge's n^3 products in registers.
 #include <hip/hip_runtime.h>
@@ -225,49 +226,6 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
   for (int k = tid; k < hvol; k += blockDim.x) dst[k] = (T)box[k];
 }
 
-// Sums the halo boxes into the periodic grid, one workgroup per (tile, bead):
-// core point l of tile t collects l + o*T from the box of tile t - o (per
-// axis, periodically, for o < nbr), which holds that point when
-// l + o*T < T + n - 1.  Tiles divide the grid, so the offset is exact also
-// across the periodic boundary.
-template <typename T>
-__global__ void __launch_bounds__(256) k_pme_fold(PmeArgs a) {
-  int tl, b;
-  if (!xcd_unit_bead(a.ntile, a.P, tl, b)) return;
-  const int tcoord[3] = {tl % a.nt[0], (tl / a.nt[0]) % a.nt[1], tl / (a.nt[0] * a.nt[1])};
-  const int T0 = a.T[0], T1 = a.T[1], T01 = T0 * T1, vol = T01 * a.T[2];
-  const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
-  const T *hb = (const T *)a.hbuf + (size_t)b * a.ntile * a.hvol;
-  T *g = (T *)a.grid + (size_t)b * a.gstride;
-  const int Ka = a.K[0], Kab = a.K[0] * a.K[1];
-  for (int k = threadIdx.x; k < vol; k += blockDim.x) {
-    const int q0 = (int)__umulhi((unsigned)k, a.mT0), l2 = (int)__umulhi((unsigned)q0, a.mT1);
-    const int l[3] = {k - q0 * T0, q0 - l2 * T1, l2};
-    T s = (T)0;
-    for (int oc = 0; oc < a.nbr[2]; oc++) {
-      const int lc = l[2] + oc * a.T[2];
-      if (lc >= a.H[2]) break;
-      int sc = tcoord[2] - oc;
-      while (sc < 0) sc += a.nt[2];
-      for (int ob = 0; ob < a.nbr[1]; ob++) {
-        const int lb = l[1] + ob * T1;
-        if (lb >= a.H[1]) break;
-        int sb = tcoord[1] - ob;
-        while (sb < 0) sb += a.nt[1];
-        for (int oa = 0; oa < a.nbr[0]; oa++) {
-          const int la = l[0] + oa * T0;
-          if (la >= H0) break;
-          int sa = tcoord[0] - oa;
-          while (sa < 0) sa += a.nt[0];
-          const size_t src = (size_t)(sc * a.nt[1] + sb) * a.nt[0] + sa;
-          s += hb[src * a.hvol + (size_t)lc * H01 + lb * H0 + la];
-        }
-      }
-    }
-    g[(size_t)(tcoord[2] * a.T[2] + l[2]) * Kab + (size_t)(tcoord[1] * T1 + l[1]) * Ka + tcoord[0] * T0 + l[0]] = s;
-  }
-}
-
 // --- gather ---------------------------------------------------------------------
 
 // The potential on the halo box of tile origin org, bead b, into LDS, with
@@ -421,7 +379,6 @@ hipError_t launch_spread_t(const PmeArgs &a, hipStream_t st) {
 #undef X
     default: return hipErrorInvalidValue;
   }
-  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pme_fold<T>), dim3(xcd_grid(a.ntile, a.P)), dim3(256), 0, st, a);
   return hipGetLastError();
 }
 
@@ -443,7 +400,7 @@ hipError_t launch_gather_t(const PmeArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
-// spread into halo boxes, then the fold into the grid
+// spread into halo boxes (the forward FFT folds them into the grid)
 hipError_t launch_pme_spread(const PmeArgs &a, hipStream_t st) {
   return a.f32 ? launch_spread_t<float>(a, st) : launch_spread_t<double>(a, st);
 }
@@ -705,9 +662,88 @@ __device__ __forceinline__ const typename Cx<T>::C *twiddles(const FftArgs &a, i
     return a.twf[d];
 }
 
-// forward transform of one c-plane of one bead: real along a, then b.
-// LDS holds the plane as Kb lines of Ka complex values; only a' = 0..Ka/2
-// is kept (Hermitian half).  spec layout: [bead][c][b][a'] complex.
+// Plane c of bead b's spread charge into LDS as complex values, folded from
+// the halo boxes: along each axis, point l of tile t is held by the box of
+// tile t - o (periodically) at l + o*T for every o with l + o*T < H.  Tiles
+// divide the grid, so the offset is exact also across the periodic boundary.
+// With at most two boxes per axis (T >= n - 1) every point makes the same 8
+// loads, the absent overlaps re-reading its own element and adding zero, so
+// the loads of 4 points per lane are in flight together; smaller tiles take
+// the loop over all overlaps.  Sums run in the order (c, b, a) of the boxes.
+template <typename T>
+__device__ __forceinline__ void fold_plane(const FftArgs &a, int c, int b, typename Cx<T>::C *sm) {
+  const int Ka = a.K[0], npt = Ka * a.K[1];
+  const int T0 = a.T[0], T1 = a.T[1], T2 = a.T[2];
+  const int H0 = a.H[0], H1 = a.H[1], H2 = a.H[2], H01 = H0 * H1;
+  const int nt0 = a.nt[0], nt1 = a.nt[1], nt2 = a.nt[2];
+  const int hv = (int)a.hvol;
+  const T *hb = (const T *)a.hbuf + (size_t)b * a.ntile * a.hvol;
+  const int tc = c / T2, lc = c - tc * T2;
+  if (a.nbr[0] <= 2 && a.nbr[1] <= 2 && a.nbr[2] <= 2) {
+    const bool ec = lc + T2 < H2;
+    const int rc0 = tc * nt1, rc1 = ec ? (tc ? tc - 1 : nt2 - 1) * nt1 : rc0;
+    const int oc0 = lc * H01, oc1 = ec ? (lc + T2) * H01 : oc0;
+    for (int w0 = threadIdx.x; w0 < npt; w0 += 4 * blockDim.x) {
+      T v[4][8];
+      unsigned m[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int w = min(w0 + u * (int)blockDim.x, npt - 1);
+        const int gb = (int)__umulhi((unsigned)w, a.mKa), ga = w - gb * Ka;
+        const int ta = (int)__umulhi((unsigned)ga, a.mT0), la = ga - ta * T0;
+        const int tb = (int)__umulhi((unsigned)gb, a.mT1), lb = gb - tb * T1;
+        const bool ea = la + T0 < H0, eb = lb + T1 < H1;
+        const int sa1 = ea ? (ta ? ta - 1 : nt0 - 1) : ta, sb1 = eb ? (tb ? tb - 1 : nt1 - 1) : tb;
+        const int xa = ea ? T0 : 0, xb = eb ? T1 * H0 : 0, e = lb * H0 + la;
+        m[u] = (ea ? 1u : 0u) | (eb ? 2u : 0u);
+        v[u][0] = hb[((rc0 + tb) * nt0 + ta) * hv + oc0 + e];
+        v[u][1] = hb[((rc0 + tb) * nt0 + sa1) * hv + oc0 + e + xa];
+        v[u][2] = hb[((rc0 + sb1) * nt0 + ta) * hv + oc0 + e + xb];
+        v[u][3] = hb[((rc0 + sb1) * nt0 + sa1) * hv + oc0 + e + xa + xb];
+        v[u][4] = hb[((rc1 + tb) * nt0 + ta) * hv + oc1 + e];
+        v[u][5] = hb[((rc1 + tb) * nt0 + sa1) * hv + oc1 + e + xa];
+        v[u][6] = hb[((rc1 + sb1) * nt0 + ta) * hv + oc1 + e + xb];
+        v[u][7] = hb[((rc1 + sb1) * nt0 + sa1) * hv + oc1 + e + xa + xb];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int w = w0 + u * (int)blockDim.x;
+        const bool ea = m[u] & 1u, eb = m[u] & 2u;
+        T s = v[u][0];
+        s += ea ? v[u][1] : (T)0;
+        s += eb ? v[u][2] : (T)0;
+        s += (ea && eb) ? v[u][3] : (T)0;
+        s += ec ? v[u][4] : (T)0;
+        s += (ec && ea) ? v[u][5] : (T)0;
+        s += (ec && eb) ? v[u][6] : (T)0;
+        s += (ec && ea && eb) ? v[u][7] : (T)0;
+        if (w < npt) sm[w] = Cx<T>::make(s, (T)0);
+      }
+    }
+    return;
+  }
+  for (int w = threadIdx.x; w < npt; w += blockDim.x) {
+    const int gb = w / Ka, ga = w - gb * Ka;
+    const int ta = ga / T0, la = ga - ta * T0, tb = gb / T1, lb = gb - tb * T1;
+    T s = (T)0;
+    for (int oc = 0; oc < a.nbr[2] && lc + oc * T2 < H2; oc++) {
+      const int sc = (tc - oc % nt2 + nt2) % nt2;
+      for (int ob = 0; ob < a.nbr[1] && lb + ob * T1 < H1; ob++) {
+        const int sb = (tb - ob % nt1 + nt1) % nt1;
+        for (int oa = 0; oa < a.nbr[0] && la + oa * T0 < H0; oa++) {
+          const int sa = (ta - oa % nt0 + nt0) % nt0;
+          s += hb[((sc * nt1 + sb) * nt0 + sa) * hv + (lc + oc * T2) * H01 + (lb + ob * T1) * H0 + la + oa * T0];
+        }
+      }
+    }
+    sm[w] = Cx<T>::make(s, (T)0);
+  }
+}
+
+// forward transform of one c-plane of one bead: the folded charge plane,
+// real along a, then b.  LDS holds the plane as Kb lines of Ka complex
+// values; only a' = 0..Ka/2 is kept (Hermitian half).  spec layout:
+// [bead][c][b][a'] complex.
 template <typename T, int NT, int MAXP>
 __global__ void __launch_bounds__(NT) k_fft_fwd_ab(FftArgs a) {
   using C = typename Cx<T>::C;
@@ -716,20 +752,7 @@ __global__ void __launch_bounds__(NT) k_fft_fwd_ab(FftArgs a) {
   int c, b;
   if (!xcd_unit_bead(a.K[2], a.P, c, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
-  const T *g = (const T *)a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
-  for (int w0 = threadIdx.x; w0 < Ka * Kb; w0 += 8 * blockDim.x) {  // 8 loads in flight per lane
-    T v[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int w = w0 + u * blockDim.x;
-      v[u] = w < Ka * Kb ? g[w] : (T)0;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int w = w0 + u * blockDim.x;
-      if (w < Ka * Kb) sm[w] = Cx<T>::make(v[u], (T)0);
-    }
-  }
+  fold_plane<T>(a, c, b, sm);
   __syncthreads();
   lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, -1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
   lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, -1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);

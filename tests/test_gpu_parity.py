@@ -288,11 +288,15 @@ def test_ewald_matches_pme_limit():
     assert np.sqrt(((fe - fp) ** 2).sum() / (fe ** 2).sum()) < 1e-4
 
 
+@pytest.mark.parametrize("tiles", ["32,16,16", "4,4,2", "8,16,4"])
 @pytest.mark.parametrize("mode", ["fp64", "mixed"])
-def test_large_pme_tiles(mode, monkeypatch):
+def test_pme_tilings(mode, tiles, monkeypatch):
     # tiles whose halo box needs more than 64 KB of LDS (32 x 16 x 16 on a
-    # 32^3 grid: 37 x 21 x 21 points, 130 KB in FP64)
-    monkeypatch.setenv("PMX_PME_TILE", "32,16,16")
+    # 32^3 grid: 37 x 21 x 21 points, 130 KB in FP64), and tiles smaller
+    # than the order-6 support, whose points overlap up to 3 or 4 halo boxes
+    # per axis (4 x 4 x 2, 8 x 16 x 4)
+    monkeypatch.setenv("PMX_PME_TILE", tiles)
+    print(f"[pme tiles {tiles} {mode}]", flush=True)
     kw = {} if mode == "fp64" else {"precision": abi.PMX_PREC_MIXED}
     c, o, g, eo, fo, eg, fg = _run("cfg1", **kw)
     etol, ftol = (ETOL, FTOL) if mode == "fp64" else (MIXED_ETOL, MIXED_FTOL)
