@@ -66,6 +66,7 @@ struct pmx_ctx {
   int pme = 0, npass[3] = {0, 0, 0}, radix[3][FFT_MAX_PASSES] = {};
   size_t gstride = 0, sstride = 0;
   double *grid = nullptr, *infl = nullptr;
+  double *frec = nullptr;  // PME gather forces, bead-major [3][P][N] (zero for uncharged sites)
   double2 *spec = nullptr, *tw[3] = {nullptr, nullptr, nullptr};
   float2 *twf[3] = {nullptr, nullptr, nullptr};
   int pme_f32 = 0;  // grids, B-splines and FFT in FP32 (the mixed precision)
@@ -333,6 +334,7 @@ std::vector<double> pair_constants(pmx_ctx *c) {
     double *e = &pp[(size_t)t * PP_NUM];
     const bool lj = (s.label[t] == PMX_INTER_LJ);
     e[PP_CUT] = s.cutoff[t];
+    e[PP_CUT2] = s.cutoff[t] * s.cutoff[t];
     e[PP_RMIN] = s.rmin[t];
     e[PP_DRI] = s.dri[t];
     e[PP_DR] = s.dr[t];
@@ -341,6 +343,8 @@ std::vector<double> pair_constants(pmx_ctx *c) {
     e[PP_VSLJ] = s.vshift_lj[t];
     e[PP_VSC] = s.vshift_c[t];
     e[PP_LJ] = lj ? 1.0 : 0.0;
+    e[PP_C6] = 4.0 * s.lj_eps[t] * s.lj_sig6[t];
+    e[PP_C12] = e[PP_C6] * s.lj_sig6[t];
     if (lj && c->par.ishift == PMX_SHIFT_SHIFT) {
       // the shift is the table's value at node nsplin - 2 (spline_fit.c:28-113)
       const double rs = s.dr[t] * (double)(s.nsplin - 5) + s.rmin[t];
@@ -364,6 +368,10 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   int rc = 0;
   if (N > ENTRY_MASK) {  // force-list entries hold the site index in 24 bits
     set_err(c, "%d sites exceed the %d sites of one context", N, ENTRY_MASK);
+    return PMX_ERR_RANGE;
+  }
+  if (NP > (size_t)INT32_MAX) {  // kernels index (site, bead) values with 32-bit offsets
+    set_err(c, "%d sites x %d beads exceed 2^31 - 1 site-bead values", N, P);
     return PMX_ERR_RANGE;
   }
 #define TRY(x)        \
@@ -524,6 +532,10 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     c->sstride = (size_t)Kc * Kb * Ha;
     TRY(dalloc(c, &c->grid, c->gstride * P));
     TRY(dalloc(c, &c->spec, c->sstride * P));
+    // the gather stores the k-space force of every (charge, bead); the
+    // entries of the other sites stay zero
+    TRY(dalloc(c, &c->frec, 3 * (size_t)N * P));
+    HIPCHK(hipMemset(c->frec, 0, 3 * (size_t)N * P * sizeof(double)));
     for (int d = 0; d < 3; d++) {
       const int K = s.ngrid[d];
       std::vector<double2> t(K);
@@ -862,6 +874,7 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.alp = s.alp_ewd;
     a.palp = kErfP * s.alp_ewd;
     a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
+    a.spalp = 2.0 * s.alp_ewd / sqrt(M_PI);
     a.rheal = c->par.rheal;
     a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
     a.swit = (c->par.ishift == PMX_SHIFT_SWIT);
@@ -895,9 +908,9 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.y = c->y;
     a.z = c->z;
     a.cell = c->cell;
-    a.fx = c->fx;
-    a.fy = c->fy;
-    a.fz = c->fz;
+    a.rx = c->frec;
+    a.ry = c->frec + NP;
+    a.rz = c->frec + 2 * NP;
     const size_t nitem = (size_t)P * nchrg, nbin = (size_t)P * a.ntile;
     hipLaunchKernelGGL(k_fill_int, dim3(blocks(nbin)), dim3(256), 0, st, c->bin_cnt, 0, nbin);
     HIPCHK(hipGetLastError());
@@ -999,6 +1012,8 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     intra_stage(c->off_bend, G, ACC_VBEND);
   }
   // --- PIMD: f/P and virial correction, then ghosts, then modes
+  // the PME forces join the others here, with f/P for PIMD
+  const double *frec = (c->pme && !(flags & PMX_EVAL_NO_RECIP)) ? c->frec : nullptr;
   if (pimd) {
     FinishArgs a{};
     a.N = N;
@@ -1012,12 +1027,16 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.fx = c->fx;
     a.fy = c->fy;
     a.fz = c->fz;
+    a.rec = frec;
     a.part = c->part + c->off_fin;
     const unsigned G = igrid(NP);
     hipLaunchKernelGGL(k_pimd_finish, dim3(G), dim3(bsP), 0, st, a);
     HIPCHK(hipGetLastError());
     add_stage(red, c->off_fin, (int)G, FIN_NV,
               {ACC_C11, ACC_C22, ACC_C33, ACC_C12, ACC_C21, ACC_C13, ACC_C31, ACC_C23, ACC_C32, ACC_AKVIR});
+  } else if (frec) {
+    hipLaunchKernelGGL(k_add_rec, dim3(blocks(NP)), dim3(256), 0, st, N, P, frec, c->fx, c->fy, c->fz);
+    HIPCHK(hipGetLastError());
   }
   if (c->nghost) {
     hipLaunchKernelGGL(k_ghost_force, dim3(blocks((size_t)c->nghost * P)), dim3(256), 0, st, c->nghost, P,
@@ -1867,6 +1886,7 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
     a.alp = s.alp_ewd;
     a.palp = kErfP * s.alp_ewd;
     a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
+    a.spalp = 2.0 * s.alp_ewd / sqrt(M_PI);
     a.rheal = c->par.rheal;
     a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
     a.swit = (c->par.ishift == PMX_SHIFT_SWIT);

@@ -33,10 +33,12 @@ struct ListArgs {
 // the cutoff when swit is set, minus vshift (shift_inter_pe = shift).
 // For the parameter gradients: the LJ shift per unit eps (VSLJ1) and
 // sigma d/dsigma of it (DSLJ1), and the type pair's LJ gradient slot (GSLOT,
-// -1 when the pair has no LJ term).
+// -1 when the pair has no LJ term).  C12 = 4 eps s^12 and C6 = 4 eps s^6
+// give the FP32 kernels the LJ term as r^-6 (C12 r^-6 - C6); CUT2 is the
+// squared cutoff.
 enum {
   PP_CUT = 0, PP_RMIN, PP_DRI, PP_DR, PP_EPS, PP_SIG6, PP_VSLJ, PP_VSC, PP_LJ,
-  PP_VSLJ1, PP_DSLJ1, PP_GSLOT, PP_NUM
+  PP_VSLJ1, PP_DSLJ1, PP_GSLOT, PP_C12, PP_C6, PP_CUT2, PP_NUM
 };
 // LJ type pairs whose eps/sigma gradients one context forms
 constexpr int MAX_LJ_GRAD = 4;
@@ -49,6 +51,7 @@ struct PairArgs {
   int N, P, cap, nsplin, pimd, npair;
   int gx;  // workgroups along x (pair_grid_x)
   double alp, palp, talp2, rheal, irheal;  // erfc and switch constants
+  double spalp;                           // 2 alp / sqrt(pi)
   int swit;
   const double *pp;            // [npair][PP_NUM]
   const double *x, *y, *z;     // [N*P]
@@ -91,7 +94,7 @@ struct PmeArgs {
   int *sorted;         // [P*nchrg] item ids in bin order
   void *hbuf;          // [P*ntile*hvol] halo boxes of spread charge (grid type)
   void *grid;          // [P*gstride] (grid type)
-  double *fx, *fy, *fz;
+  double *rx, *ry, *rz;  // gather: the k-space force of every (charge, bead), bead-major [P][N]
   double *phi;         // potential gather: dE_b/dq_i += k-space part [N*P]
 };
 
@@ -175,6 +178,7 @@ struct FinishArgs {
   int N, P;
   const double *x, *y, *z, *xm, *ym, *zm;
   double *fx, *fy, *fz;
+  const double *rec;  // PME forces to add, bead-major [3][P][N], or null
   double *part;  // [gridDim.x][P][FIN_NV]
 };
 
@@ -243,6 +247,7 @@ __global__ void k_ecor(IntraArgs a);
 __global__ void k_bond(IntraArgs a);
 __global__ void k_bend(IntraArgs a);
 __global__ void k_pimd_finish(FinishArgs a);
+__global__ void k_add_rec(int N, int P, const double *rec, double *fx, double *fy, double *fz);
 __global__ void k_ghost_force(int ng, int P, const int *gatom, const int *gncomp, const int *gcomp,
                               const double *gcoef, double *fx, double *fy, double *fz);
 __global__ void k_mode_forces(ModeArgs a);

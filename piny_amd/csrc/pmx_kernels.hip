@@ -509,21 +509,31 @@ __device__ __forceinline__ R pair_direct(const PairArgs &a, const PT *pp, R q2, 
   R sw = (R)1, dsw = (R)0;
   if (a.swit) {
     const R rheal = (R)a.rheal, irheal = (R)a.irheal;
-    R rp = (r - (R)pp[PP_CUT] + rheal) * irheal;
-    rp = rp < (R)1 ? rp : (R)1;
-    rp = rp > (R)0 ? rp : (R)0;
-    sw = (R)1 + rp * rp * ((R)2 * rp - (R)3);
-    dsw = (R)(-6) * (rp * irheal) * (rp - (R)1) * rinv;
+    const R rs = r - (R)pp[PP_CUT] + rheal;
+    if (rs > (R)0) {  // inside the switching shell (below it sw = 1, dsw = 0)
+      R rp = rs * irheal;
+      rp = rp < (R)1 ? rp : (R)1;
+      sw = (R)1 + rp * rp * ((R)2 * rp - (R)3);
+      dsw = (R)(-6) * (rp * irheal) * (rp - (R)1) * rinv;
+    }
   }
   const R r2i = rinv * rinv;
+  constexpr bool f32 = sizeof(R) == 4;
   R dvt = (R)0;
   vlj = (R)0;
   vc = (R)0;
   if (pp[PP_LJ] != (PT)0) {
-    const R eps = (R)pp[PP_EPS];
-    const R s6 = (R)pp[PP_SIG6] * (r2i * r2i * r2i);
-    const R v = (R)4 * eps * s6 * (s6 - (R)1);
-    const R dv = (R)24 * eps * s6 * (s6 * (R)2 - (R)1) * r2i;
+    R v, dv;
+    if constexpr (f32) {  // r^-6 (C12 r^-6 - C6) and 6 r^-6 (2 C12 r^-6 - C6) / r^2
+      const R r6i = r2i * r2i * r2i, c6 = (R)pp[PP_C6], t12 = (R)pp[PP_C12] * r6i;
+      v = r6i * (t12 - c6);
+      dv = ((R)6 * r6i) * ((R)2 * t12 - c6) * r2i;
+    } else {
+      const R eps = (R)pp[PP_EPS];
+      const R s6 = (R)pp[PP_SIG6] * (r2i * r2i * r2i);
+      v = (R)4 * eps * s6 * (s6 - (R)1);
+      dv = (R)24 * eps * s6 * (s6 * (R)2 - (R)1) * r2i;
+    }
     vlj = v * sw - (R)pp[PP_VSLJ];
     dvt = dv * sw + v * dsw;
   }
@@ -531,14 +541,25 @@ __device__ __forceinline__ R pair_direct(const PairArgs &a, const PT *pp, R q2, 
     const R ralp = r * (R)a.alp;
     const R eee = exp_r<R>(-ralp * ralp);
     const R tt = rcp_r<R>((R)1 + (R)kErfP * ralp);
-    R g = (R)kErfE[9], dg = (R)(9.0 * kErfE[9]);
+    R g = (R)kErfE[9], dg;
+    if constexpr (f32) {
+      // -d erfc(alp r)/dr = 2 alp / sqrt(pi) exp(-alp^2 r^2); the polynomial's
+      // own derivative differs from it by at most 1.1e-7 relative (5e-9 in
+      // the force term), below FP32 rounding
 #pragma unroll
-    for (int k = 8; k >= 1; k--) {
-      g = g * tt + (R)kErfE[k];
-      dg = dg * tt + (R)(k * kErfE[k]);
+      for (int k = 8; k >= 1; k--) g = g * tt + (R)kErfE[k];
+      g = g * tt * eee;
+      dg = (R)a.spalp * eee;
+    } else {
+      dg = (R)(9.0 * kErfE[9]);
+#pragma unroll
+      for (int k = 8; k >= 1; k--) {
+        g = g * tt + (R)kErfE[k];
+        dg = dg * tt + (R)(k * kErfE[k]);
+      }
+      g = g * tt * eee;
+      dg = dg * tt * tt * eee * (R)a.palp + (R)a.talp2 * g * r;
     }
-    g = g * tt * eee;
-    dg = dg * tt * tt * eee * (R)a.palp + (R)a.talp2 * g * r;
     const R v = g * rinv;
     const R dv = (g * r2i + dg * rinv) * rinv;
     vc = q2 * (v * sw - (R)pp[PP_VSC]);
@@ -707,7 +728,7 @@ template <int B>
 __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   __shared__ float spp[MAX_PAIRS * PP_NUM];
   __shared__ float4 sdq[4][64];
-  __shared__ int sent[4][64];
+  __shared__ int2 sent[4][64];  // (j * P, type pair * PP_NUM) of the staged entries
   for (int k = threadIdx.x; k < a.npair * PP_NUM; k += blockDim.x) spp[k] = (float)a.pp[k];
   __syncthreads();
   constexpr int S = 64 / B;
@@ -758,7 +779,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
       if (keep) {
         const int pos = __popcll(mk & below);
         sdq[w][pos] = make_float4((float)dx, (float)dy, (float)dz, (float)q2);
-        sent[w][pos] = e;
+        sent[w][pos] = make_int2((e & ENTRY_MASK) * P, (e >> ENTRY_SHIFT) * PP_NUM);
       }
       wave_sync();
       const int nk = __popcll(mk);
@@ -768,23 +789,20 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
       const int bsel = bact ? bead : 0;
       // the next pair's displacement is loaded while this pair is evaluated
       float4 un = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (a.pimd && slot < nk) un = a.u[(size_t)(sent[w][slot] & ENTRY_MASK) * P + bsel];
+      if (a.pimd && slot < nk) un = a.u[sent[w][slot].x + bsel];
       for (int p = slot; p < nk; p += S) {
         const float4 d = sdq[w][p];
-        const int e = sent[w][p];
-        const int t = e >> ENTRY_SHIFT;
-        const float *pp = spp + t * PP_NUM;
+        const float *pp = spp + sent[w][p].y;
         float dx = d.x, dy = d.y, dz = d.z;
         if (a.pimd) {
           const float4 uj = un;
-          if (p + S < nk) un = a.u[(size_t)(sent[w][p + S] & ENTRY_MASK) * P + bsel];
+          if (p + S < nk) un = a.u[sent[w][p + S].x + bsel];
           dx += ui.x - uj.x;
           dy += ui.y - uj.y;
           dz += ui.z - uj.z;
         }
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const float cut = pp[PP_CUT];
-        if (!bact || !(r2 < cut * cut)) continue;
+        if (!bact || !(r2 < pp[PP_CUT2])) continue;
         const float rinv = __frsqrt_rn(r2);
         float ev_lj, ev_c;
         const float dvd = pair_direct<float, float>(a, pp, d.w, r2 * rinv, rinv, ev_lj, ev_c);
@@ -1312,15 +1330,26 @@ __global__ void __launch_bounds__(256) k_bend_moments(IntraArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// PIMD finish: f /= P (energy_control_pimd.c:116-126) and the virial
-// correction -sum (x - xmod) (x) f (pimd_estimators.c:223-323), grid-stride
-// over (site, bead) items.
+// PIMD finish: the PME forces added (rec, bead-major), f /= P
+// (energy_control_pimd.c:116-126) and the virial correction
+// -sum (x - xmod) (x) f (pimd_estimators.c:223-323), grid-stride over (site,
+// bead) items.
 __global__ void __launch_bounds__(256) k_pimd_finish(FinishArgs a) {
   double c[FIN_NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const double dP = (double)a.P;
+  const size_t NP = (size_t)a.N * a.P;
   PMX_TERM_LOOP(a.N) {
     const int i = (int)(t / a.P);
-    const double fx = a.fx[t] / dP, fy = a.fy[t] / dP, fz = a.fz[t] / dP;
+    double fx = a.fx[t], fy = a.fy[t], fz = a.fz[t];
+    if (a.rec) {
+      const size_t r = (t - (size_t)i * a.P) * a.N + i;
+      fx += a.rec[r];
+      fy += a.rec[NP + r];
+      fz += a.rec[2 * NP + r];
+    }
+    fx /= dP;
+    fy /= dP;
+    fz /= dP;
     a.fx[t] = fx;
     a.fy[t] = fy;
     a.fz[t] = fz;
@@ -1339,6 +1368,17 @@ __global__ void __launch_bounds__(256) k_pimd_finish(FinishArgs a) {
     }
   }
   block_partials<FIN_NV>(c, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);
+}
+
+// The PME forces (bead-major [3][P][N]) added to the site-major forces,
+// outside PIMD.
+__global__ void k_add_rec(int N, int P, const double *__restrict__ rec, double *fx, double *fy, double *fz) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, NP = (size_t)N * P;
+  if (t >= NP) return;
+  const size_t i = t / P, r = (t - i * P) * N + i;
+  fx[t] += rec[r];
+  fy[t] += rec[NP + r];
+  fz[t] += rec[2 * NP + r];
 }
 
 // distrib_ghost_force, ghost_control.c:71-125

@@ -308,10 +308,11 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
     da *= (double)Ka * q;
     db *= (double)Kb * q;
     dc *= (double)Kc * q;
-    const size_t ib = (size_t)i * a.P + bead;
-    a.fx[ib] -= da * hi[0] + db * hi[1] + dc * hi[2];
-    a.fy[ib] -= da * hi[3] + db * hi[4] + dc * hi[5];
-    a.fz[ib] -= da * hi[6] + db * hi[7] + dc * hi[8];
+    // bead-major, so the lines of one bead are written from its own XCD
+    const size_t ib = (size_t)bead * a.N + i;
+    a.rx[ib] = -(da * hi[0] + db * hi[1] + dc * hi[2]);
+    a.ry[ib] = -(da * hi[3] + db * hi[4] + dc * hi[5]);
+    a.rz[ib] = -(da * hi[6] + db * hi[7] + dc * hi[8]);
   }
 }
 
