@@ -238,7 +238,11 @@ int pmx_get_spline(pmx_ctx *ctx, int which, int ipair, double *out, int cap);
 int pmx_get_setup_scalars(pmx_ctx *ctx, double *out); /* see setup_scalars */
 
 /* Wall time (ms) of the last pmx_energy_control per stage:
- * [0] list, [1] real, [2] recip, [3] intra, [4] total. */
+ * [0] list, [1] real, [2] recip, [3] intra, [4] total.  With PME on its own
+ * stream, concurrent with the pair kernel (chosen by pmx_autotune when it
+ * is faster), [2] is the PME
+ * chain from the start of the device sequence and [3] everything on the
+ * main stream after the pair kernel. */
 int pmx_get_timings(pmx_ctx *ctx, double *ms5);
 
 /* Real-space pairs inside the cutoff evaluated by the last force call,
@@ -326,8 +330,9 @@ void pmx_batch_destroy(pmx_batch *batch);
 /* Autotuner of the launch shapes (no counterpart in the reference, which
  * has no device): times reps force calls at the current positions for each
  * candidate of the pair kernels' wavefront count, the PME tile sizes and the
- * list build (plain scan or link cells), keeps the fastest of each, and
- * ends with a force call using them.  choice (may be NULL) receives
+ * list build (plain scan or link cells), keeps the fastest of each, then
+ * times whole calls with PME on a second stream beside the pair kernel and
+ * without, keeps the faster, and ends with a force call using them.  choice (may be NULL) receives
  * [tile a, tile b, tile c, pair wavefronts, list build 1 plain / 2 cells].
  * Results are unchanged by the shapes up to the order of sums. */
 int pmx_autotune(pmx_ctx *ctx, unsigned flags, int reps, int *choice);

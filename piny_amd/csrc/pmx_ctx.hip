@@ -114,9 +114,15 @@ struct pmx_ctx {
   // caps, wavefronts of the pair kernels, list build (0 by size, 1 plain
   // scan, 2 link cells)
   int tile_cap[3] = {16, 16, 8};
-  int pair_waves = 8192;
+  int pair_waves = 4096;
   int list_mode = 0;
   hipEvent_t ready = nullptr;  // list stage done (batched calls)
+  // PME on a second stream, concurrent with the pair kernel (the pair
+  // kernel is bound by vector ALU issue, PME by memory and LDS)
+  int overlap = 0;
+  bool last_overlap = false;  // the last force sequence ran PME on sk
+  hipStream_t sk = nullptr;
+  hipEvent_t kfork = nullptr, kjoin = nullptr;
 };
 
 struct pmx_batch {
@@ -831,8 +837,12 @@ int list_stage_finish(pmx_ctx *c, unsigned flags, pmx_energies *out) {
 // The device work of a force call after the list stage, launched on st:
 // real space, k space, intramolecular terms, PIMD finish, ghosts, modes and
 // the reduction of the partial sums.  It makes no host synchronization and
-// no memset, so it can be captured into a graph.
-int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
+// no memset, so it can be captured into a graph.  With fork (direct calls)
+// PME runs on the context's second stream; a captured sequence (batched
+// calls, whose boxes are already parallel branches of the graph) stays on
+// st: with a second fork inside the capture the batched call ended in a
+// host-side segmentation fault on MI355X (ROCm 7).
+int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool fork) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
   const size_t NP = (size_t)N * P;
@@ -848,6 +858,73 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
   }
   hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, st, c->ake, 0.0, (size_t)1);
   HIPCHK(hipGetLastError());
+  // --- reciprocal space (energy_control_inter_recip.c:119-355), PME on stream sk
+  const int nchrg = (int)s.ichrg.size();
+  const bool pme = c->pme && !(flags & PMX_EVAL_NO_RECIP);
+  auto launch_pme = [&](hipStream_t sk) -> int {
+    PmeArgs a = c->pa;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.cell = c->cell;
+    a.rx = c->frec;
+    a.ry = c->frec + NP;
+    a.rz = c->frec + 2 * NP;
+    const size_t nitem = (size_t)P * nchrg, nbin = (size_t)P * a.ntile;
+    hipLaunchKernelGGL(k_fill_int, dim3(blocks(nbin)), dim3(256), 0, sk, c->bin_cnt, 0, nbin);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pme_bin, dim3(blocks(nchrg), P), dim3(256), (size_t)a.ntile * sizeof(int), sk, a);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pme_scan, dim3(P), dim3(256), 0, sk, a);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_pme_scatter, dim3(blocks(nitem)), dim3(256), 0, sk, a);
+    HIPCHK(hipGetLastError());
+    HIPCHK(launch_pme_spread(a, sk));
+    FftArgs f{};
+    f.P = P;
+    f.f32 = c->pme_f32;
+    for (int d = 0; d < 3; d++) {
+      f.K[d] = s.ngrid[d];
+      f.npass[d] = c->npass[d];
+      for (int k = 0; k < c->npass[d]; k++) f.radix[d][k] = c->radix[d][k];
+      f.tw[d] = c->tw[d];
+      f.twf[d] = c->twf[d];
+    }
+    f.gstride = c->gstride;
+    f.sstride = c->sstride;
+    f.grid = c->grid;
+    f.spec = c->spec;
+    f.infl = c->infl;
+    f.falp2 = 4.0 * s.alp_ewd * s.alp_ewd;
+    f.cell = c->cell;
+    f.part = c->part + c->off_fft;
+    f.hbuf = a.hbuf;
+    for (int d = 0; d < 3; d++) {
+      f.T[d] = a.T[d];
+      f.nt[d] = a.nt[d];
+      f.H[d] = a.H[d];
+      f.nbr[d] = a.nbr[d];
+    }
+    f.ntile = a.ntile;
+    f.hvol = a.hvol;
+    f.mKa = (unsigned)((1ull << 32) / (unsigned)s.ngrid[0] + 1ull);
+    f.mT0 = a.mT0;
+    f.mT1 = a.mT1;
+    HIPCHK(launch_pme_fft(f, sk));
+    HIPCHK(launch_pme_gather(a, sk));
+    return PMX_OK;
+  };
+  // with overlap the k-space chain runs on c->sk from here and joins st
+  // before its forces and partial sums are used
+  const bool overlap = pme && c->overlap && fork;
+  c->last_overlap = overlap;
+  if (overlap) {
+    HIPCHK(hipEventRecord(c->kfork, st));
+    HIPCHK(hipStreamWaitEvent(c->sk, c->kfork, 0));
+    if (int rc = launch_pme(c->sk)) return rc;
+    if (timing) HIPCHK(hipEventRecord(c->ev[5], c->sk));
+    HIPCHK(hipEventRecord(c->kjoin, c->sk));
+  }
   // --- real space (force_control -> force_verlst -> force_npol)
   {
     hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
@@ -900,61 +977,10 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
               {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR});
   }
   if (timing) HIPCHK(hipEventRecord(c->ev[2], st));
-  // --- reciprocal space (energy_control_inter_recip.c:119-355)
-  const int nchrg = (int)s.ichrg.size();
-  if (c->pme && !(flags & PMX_EVAL_NO_RECIP)) {
-    PmeArgs a = c->pa;
-    a.x = c->x;
-    a.y = c->y;
-    a.z = c->z;
-    a.cell = c->cell;
-    a.rx = c->frec;
-    a.ry = c->frec + NP;
-    a.rz = c->frec + 2 * NP;
-    const size_t nitem = (size_t)P * nchrg, nbin = (size_t)P * a.ntile;
-    hipLaunchKernelGGL(k_fill_int, dim3(blocks(nbin)), dim3(256), 0, st, c->bin_cnt, 0, nbin);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_pme_bin, dim3(blocks(nchrg), P), dim3(256), (size_t)a.ntile * sizeof(int), st, a);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_pme_scan, dim3(P), dim3(256), 0, st, a);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_pme_scatter, dim3(blocks(nitem)), dim3(256), 0, st, a);
-    HIPCHK(hipGetLastError());
-    HIPCHK(launch_pme_spread(a, st));
-    FftArgs f{};
-    f.P = P;
-    f.f32 = c->pme_f32;
-    for (int d = 0; d < 3; d++) {
-      f.K[d] = s.ngrid[d];
-      f.npass[d] = c->npass[d];
-      for (int k = 0; k < c->npass[d]; k++) f.radix[d][k] = c->radix[d][k];
-      f.tw[d] = c->tw[d];
-      f.twf[d] = c->twf[d];
-    }
-    f.gstride = c->gstride;
-    f.sstride = c->sstride;
-    f.grid = c->grid;
-    f.spec = c->spec;
-    f.infl = c->infl;
-    f.falp2 = 4.0 * s.alp_ewd * s.alp_ewd;
-    f.cell = c->cell;
-    f.part = c->part + c->off_fft;
-    f.hbuf = a.hbuf;
-    for (int d = 0; d < 3; d++) {
-      f.T[d] = a.T[d];
-      f.nt[d] = a.nt[d];
-      f.H[d] = a.H[d];
-      f.nbr[d] = a.nbr[d];
-    }
-    f.ntile = a.ntile;
-    f.hvol = a.hvol;
-    f.mKa = (unsigned)((1ull << 32) / (unsigned)s.ngrid[0] + 1ull);
-    f.mT0 = a.mT0;
-    f.mT1 = a.mT1;
-    HIPCHK(launch_pme_fft(f, st));
-    HIPCHK(launch_pme_gather(a, st));
-    const int Kb = s.ngrid[1];
-    add_stage(red, c->off_fft, Kb, FFT_NV, {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
+  if (pme) {
+    if (!overlap)
+      if (int rc = launch_pme(st)) return rc;
+    add_stage(red, c->off_fft, s.ngrid[1], FFT_NV, {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
   }
   if (c->ewald && !(flags & PMX_EVAL_NO_RECIP)) {
     EwaldArgs a{};
@@ -1013,7 +1039,8 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
   }
   // --- PIMD: f/P and virial correction, then ghosts, then modes
   // the PME forces join the others here, with f/P for PIMD
-  const double *frec = (c->pme && !(flags & PMX_EVAL_NO_RECIP)) ? c->frec : nullptr;
+  if (overlap) HIPCHK(hipStreamWaitEvent(st, c->kjoin, 0));
+  const double *frec = pme ? c->frec : nullptr;
   if (pimd) {
     FinishArgs a{};
     a.N = N;
@@ -1101,6 +1128,12 @@ void assemble(pmx_ctx *c, unsigned flags, pmx_energies *out, bool timing) {
     }
     (void)hipEventElapsedTime(&t, c->ev[0], c->ev[4]);
     c->ms[4] = t;
+    if (c->last_overlap) {  // k space on its own stream; the rest after the pair kernel
+      (void)hipEventElapsedTime(&t, c->ev[1], c->ev[5]);
+      c->ms[2] = t;
+      (void)hipEventElapsedTime(&t, c->ev[2], c->ev[4]);
+      c->ms[3] = t;
+    }
   }
   // --- assemble (energy_control_inter_real.c:188-197, _recip.c:340-347, _intra.c)
   const double dP = (double)P;
@@ -1232,10 +1265,19 @@ pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *
       return fail("pmx_create: event creation failed");
     }
   }
-  if (hipEventCreateWithFlags(&c->ready, hipEventDisableTiming) != hipSuccess) {
+  if (hipEventCreateWithFlags(&c->ready, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->kfork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->kjoin, hipEventDisableTiming) != hipSuccess) {
     pmx_destroy(c);
     return fail("pmx_create: event creation failed");
   }
+  if (hipStreamCreateWithFlags(&c->sk, hipStreamNonBlocking) != hipSuccess) {
+    pmx_destroy(c);
+    return fail("pmx_create: stream creation failed");
+  }
+  if (const char *env = getenv("PMX_OVERLAP")) c->overlap = atoi(env) != 0;  // tuning override 0 / 1
+  if (const char *env = getenv("PMX_PAIR_WAVES"))  // tuning override: wavefronts of the pair kernels
+    c->pair_waves = std::max(64, std::min(PAIR_MAX_WAVES, atoi(env)));
   if (const char *env = getenv("PMX_PME_TILE"))  // tuning override "Ta,Tb,Tc" (upper bounds)
     sscanf(env, "%d,%d,%d", &c->tile_cap[0], &c->tile_cap[1], &c->tile_cap[2]);
   if (build_device(c, sys) != PMX_OK) {
@@ -1250,13 +1292,17 @@ void pmx_destroy(pmx_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->st) (void)hipStreamSynchronize(c->st);
+  if (c->sk) (void)hipStreamSynchronize(c->sk);
   for (void *p : c->allocs) (void)hipFree(p);
   if (c->mx_pin) (void)hipHostFree(c->mx_pin);
   if (c->acc_pin) (void)hipHostFree(c->acc_pin);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->ready) (void)hipEventDestroy(c->ready);
+  if (c->kfork) (void)hipEventDestroy(c->kfork);
+  if (c->kjoin) (void)hipEventDestroy(c->kjoin);
   if (c->st) (void)hipStreamDestroy(c->st);
+  if (c->sk) (void)hipStreamDestroy(c->sk);
   delete c;
 }
 
@@ -1320,7 +1366,7 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   if ((rc = list_stage_launch(c, st, true))) return rc;
   HIPCHK(hipStreamSynchronize(st));
   if ((rc = list_stage_finish(c, flags, out))) return rc;
-  if ((rc = force_sequence(c, flags, st, true))) return rc;
+  if ((rc = force_sequence(c, flags, st, true, true))) return rc;
   if ((rc = readback_launch(c, st))) return rc;
   HIPCHK(hipStreamSynchronize(st));
   assemble(c, flags, out, true);
@@ -1392,7 +1438,7 @@ static int batch_capture(pmx_batch *b, unsigned flags) {
     pmx_ctx *c = b->ctx[i];
     e = hipStreamWaitEvent(c->st, b->fork, 0);
     if (e != hipSuccess) break;
-    rc = force_sequence(c, batch_flags(c, flags), c->st, false);
+    rc = force_sequence(c, batch_flags(c, flags), c->st, false, false);
     if (rc) {
       b->err = "context " + std::to_string(i) + ": " + c->err;
       break;
@@ -1503,6 +1549,9 @@ int pmx_autotune(pmx_ctx *c, unsigned flags, int reps, int *choice) {
     return PMX_OK;
   };
   int rc = PMX_OK;
+  // the stages are timed one after the other, then PME beside the pair
+  // kernel is tried last on the whole call
+  c->overlap = 0;
   // pair kernels: wavefronts (stage 1, real space)
   {
     const int cand[] = {1024, 2048, 4096, 8192, 16384};
@@ -1551,6 +1600,14 @@ int pmx_autotune(pmx_ctx *c, unsigned flags, int reps, int *choice) {
       if ((rc = stage_ms(0, flags | PMX_EVAL_FORCE_LIST, &tc))) return rc;
     }
     c->list_mode = tc < tb ? 2 : 1;
+  }
+  // PME on the second stream (stage 4, the whole call)
+  if (c->pme) {
+    double t0, t1;
+    if ((rc = stage_ms(4, flags, &t0))) return rc;
+    c->overlap = 1;
+    if ((rc = stage_ms(4, flags, &t1))) return rc;
+    c->overlap = t1 < t0 ? 1 : 0;
   }
   c->epoch++;
   if ((rc = pmx_energy_control(c, flags | PMX_EVAL_FORCE_LIST, &e))) return rc;

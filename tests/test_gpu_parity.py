@@ -305,3 +305,33 @@ def test_pme_tilings(mode, tiles, monkeypatch):
         assert abs(eg[k] - eo[k]) <= etol * max(abs(eo[k]), scale), (k, eg[k], eo[k])
     rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
     assert rms < ftol
+
+
+@pytest.mark.parametrize("case", ["cfg1", "tip64_p4", "tip256_p32"])
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_pme_beside_pair_kernel(case, mode, monkeypatch):
+    # PME on the context's second stream, concurrent with the pair kernel:
+    # the same results as the serial sequence (forces up to the order of
+    # the bonded kernels' atomic additions), checked over two calls with
+    # the positions moved in between
+    kw = {} if mode == "fp64" else {"precision": abi.PMX_PREC_MIXED}
+    c = cases.get(case)
+    s = c.system
+    flags = abi.PMX_EVAL_PIMD if s.pi_beads > 1 else 0
+    print(f"[pme stream {case} {mode}]", flush=True)
+    res = []
+    for ov in ("0", "1"):
+        monkeypatch.setenv("PMX_OVERLAP", ov)
+        g = Engine(s, c.x, c.y, c.z, **kw)
+        out = []
+        for call in range(2):
+            d = 0.01 * call
+            g.set_positions(c.x + d, c.y - d, c.z + 0.5 * d)
+            e = g.energy_control(flags)
+            out.append((e, g.forces()))
+        res.append(out)
+    tol = 1e-12 if mode == "fp64" else 1e-6
+    for (e0, f0), (e1, f1) in zip(*res):
+        for k in abi.ENERGY_KEYS:
+            assert abs(e1[k] - e0[k]) <= tol * max(abs(e0[k]), 1e-3), (k, e1[k], e0[k])
+        assert np.max(np.abs(f1 - f0)) <= tol * np.max(np.abs(f0))
