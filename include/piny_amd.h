@@ -237,12 +237,12 @@ int pmx_get_kvectors(pmx_ctx *ctx, int *ka, int *kb, int *kc,
 int pmx_get_spline(pmx_ctx *ctx, int which, int ipair, double *out, int cap);
 int pmx_get_setup_scalars(pmx_ctx *ctx, double *out); /* see setup_scalars */
 
-/* Wall time (ms) of the last pmx_energy_control per stage:
- * [0] list, [1] real, [2] recip, [3] intra, [4] total.  With PME on its own
- * stream, concurrent with the pair kernel (chosen by pmx_autotune when it
- * is faster), [2] is the PME
- * chain from the start of the device sequence and [3] everything on the
- * main stream after the pair kernel. */
+/* Wall time (ms) of the last pmx_energy_control per stage: [0] list (check,
+ * site packing and, when due, the build), [1] real (the pair kernel),
+ * [2] recip (the k-space chain, launched before the list decision, on its
+ * own stream beside the pair kernel when pmx_autotune finds that faster),
+ * [3] intra (everything after the pair kernel: excluded-pair corrections,
+ * bonded terms, PIMD finish, ghosts, modes, sums), [4] total. */
 int pmx_get_timings(pmx_ctx *ctx, double *ms5);
 
 /* Real-space pairs inside the cutoff evaluated by the last force call,

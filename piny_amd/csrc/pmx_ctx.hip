@@ -66,7 +66,7 @@ struct pmx_ctx {
   int pme = 0, npass[3] = {0, 0, 0}, radix[3][FFT_MAX_PASSES] = {};
   size_t gstride = 0, sstride = 0;
   double *grid = nullptr, *infl = nullptr;
-  double *frec = nullptr;  // PME gather forces, bead-major [3][P][N] (zero for uncharged sites)
+  double *frec = nullptr;  // k-space forces, bead-major [3][P][N] (zero for uncharged sites)
   double2 *spec = nullptr, *tw[3] = {nullptr, nullptr, nullptr};
   float2 *twf[3] = {nullptr, nullptr, nullptr};
   int pme_f32 = 0;  // grids, B-splines and FFT in FP32 (the mixed precision)
@@ -123,6 +123,7 @@ struct pmx_ctx {
   bool last_overlap = false;  // the last force sequence ran PME on sk
   hipStream_t sk = nullptr;
   hipEvent_t kfork = nullptr, kjoin = nullptr;
+  hipEvent_t mxev = nullptr;  // the list check's maxima are in mx_pin
 };
 
 struct pmx_batch {
@@ -538,10 +539,6 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     c->sstride = (size_t)Kc * Kb * Ha;
     TRY(dalloc(c, &c->grid, c->gstride * P));
     TRY(dalloc(c, &c->spec, c->sstride * P));
-    // the gather stores the k-space force of every (charge, bead); the
-    // entries of the other sites stay zero
-    TRY(dalloc(c, &c->frec, 3 * (size_t)N * P));
-    HIPCHK(hipMemset(c->frec, 0, 3 * (size_t)N * P * sizeof(double)));
     for (int d = 0; d < 3; d++) {
       const int K = s.ngrid[d];
       std::vector<double2> t(K);
@@ -586,6 +583,12 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     c->off_bend = c->off_bond + intra;
     c->off_fin = c->off_bend + intra;
     TRY(dalloc(c, &c->part, c->off_fin + fin));
+  }
+  if (c->pme || c->ewald) {
+    // the k-space kernels store the force of every (charge, bead); the
+    // entries of the other sites stay zero
+    TRY(dalloc(c, &c->frec, 3 * NP));
+    HIPCHK(hipMemset(c->frec, 0, 3 * NP * sizeof(double)));
   }
   TRY(dalloc(c, &c->acc, (size_t)P * ACC_NUM));
   TRY(dalloc(c, &c->ake, 1));
@@ -783,8 +786,8 @@ namespace {
 
 // The list stage of a force call, first half: centroid reference and list
 // check (nbr_list_control.c:32-118) launched on st, with the two maxima
-// copied to pinned host memory.  The caller waits for st before the second
-// half.
+// copied to pinned host memory (event mxev).  The caller waits for mxev
+// before the second half.
 int list_stage_launch(pmx_ctx *c, hipStream_t st, bool timing) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
@@ -794,6 +797,7 @@ int list_stage_launch(pmx_ctx *c, hipStream_t st, bool timing) {
                      c->b0x, c->b0y, c->b0z, c->maxes, (P > 1 && !c->xmod_given) ? 1 : 0, c->xm, c->ym, c->zm);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->mx_pin, c->maxes, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(c->mxev, st));
   return PMX_OK;
 }
 
@@ -834,21 +838,21 @@ int list_stage_finish(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   return PMX_OK;
 }
 
-// The device work of a force call after the list stage, launched on st:
-// real space, k space, intramolecular terms, PIMD finish, ghosts, modes and
-// the reduction of the partial sums.  It makes no host synchronization and
-// no memset, so it can be captured into a graph.  With fork (direct calls)
-// PME runs on the context's second stream; a captured sequence (batched
-// calls, whose boxes are already parallel branches of the graph) stays on
-// st: with a second fork inside the capture the batched call ended in a
-// host-side segmentation fault on MI355X (ROCm 7).
-int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool fork) {
+// The device work of a force call that does not depend on the Verlet list,
+// launched on st before the host reads the list check, so the device keeps
+// working meanwhile: the force and kinetic clears, the site packing and
+// bead displacements of the pair kernels, and the k space (PME or standard
+// Ewald, energy_control_inter_recip.c:119-355), whose forces go to frec.
+// With fork and c->overlap PME runs on the context's second stream, joined
+// in main_sequence.  A captured sequence (batched calls, whose boxes are
+// already parallel branches of the graph) stays on st: with a second fork
+// inside the capture the batched call ended in a host-side segmentation
+// fault on MI355X (ROCm 7).  No host synchronization and no memset, so it
+// can be captured into a graph.
+int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool fork) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
   const size_t NP = (size_t)N * P;
-  const bool pimd = (flags & PMX_EVAL_PIMD) != 0;
-  RedArgs red{};
-  if (timing) HIPCHK(hipEventRecord(c->ev[1], st));
   // --- initial (energy_control_initial.c:168-208)
   if (!c->full_list) {  // with the full list the pair kernel stores every force first
     for (double *f : {c->fx, c->fy, c->fz}) {
@@ -858,7 +862,20 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool
   }
   hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, st, c->ake, 0.0, (size_t)1);
   HIPCHK(hipGetLastError());
-  // --- reciprocal space (energy_control_inter_recip.c:119-355), PME on stream sk
+  hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
+                     c->q, c->site);
+  HIPCHK(hipGetLastError());
+  if (c->disp) {
+    hipLaunchKernelGGL(k_pack_disp, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm,
+                       c->disp);
+    hipLaunchKernelGGL(k_site_umax, dim3(blocks(N)), dim3(256), 0, st, N, P, c->disp, c->umax);
+    HIPCHK(hipGetLastError());
+  } else if (c->umax && c->full_list) {
+    hipLaunchKernelGGL(k_site_umax64, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym,
+                       c->zm, c->umax);
+    HIPCHK(hipGetLastError());
+  }
+  if (timing) HIPCHK(hipEventRecord(c->ev[1], st));
   const int nchrg = (int)s.ichrg.size();
   const bool pme = c->pme && !(flags & PMX_EVAL_NO_RECIP);
   auto launch_pme = [&](hipStream_t sk) -> int {
@@ -914,32 +931,57 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool
     HIPCHK(launch_pme_gather(a, sk));
     return PMX_OK;
   };
-  // with overlap the k-space chain runs on c->sk from here and joins st
-  // before its forces and partial sums are used
   const bool overlap = pme && c->overlap && fork;
   c->last_overlap = overlap;
   if (overlap) {
     HIPCHK(hipEventRecord(c->kfork, st));
     HIPCHK(hipStreamWaitEvent(c->sk, c->kfork, 0));
     if (int rc = launch_pme(c->sk)) return rc;
-    if (timing) HIPCHK(hipEventRecord(c->ev[5], c->sk));
+    if (timing) HIPCHK(hipEventRecord(c->ev[2], c->sk));
     HIPCHK(hipEventRecord(c->kjoin, c->sk));
+    return PMX_OK;
   }
+  if (pme)
+    if (int rc = launch_pme(st)) return rc;
+  if (c->ewald && !(flags & PMX_EVAL_NO_RECIP)) {
+    EwaldArgs a{};
+    a.N = N;
+    a.P = P;
+    a.nchrg = nchrg;
+    a.nk = s.nktot;
+    a.x = c->x;
+    a.y = c->y;
+    a.z = c->z;
+    a.q = c->q;
+    a.ichrg = c->ichrg;
+    a.cell = c->cell;
+    a.kv = c->ew_kv;
+    a.kg = c->ew_kg;
+    a.sk = c->ew_sk;
+    a.rx = c->frec;
+    a.ry = c->frec + NP;
+    a.rz = c->frec + 2 * NP;
+    a.part = c->part + c->off_fft;
+    HIPCHK(launch_ewald(a, st));
+  }
+  if (timing) HIPCHK(hipEventRecord(c->ev[2], st));
+  return PMX_OK;
+}
+
+// The device work after the list stage, launched on st: real space,
+// excluded-pair corrections, intramolecular terms, the k-space forces
+// added, PIMD finish, ghosts, modes and the reduction of the partial sums.
+int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
+  Setup &s = c->s;
+  const int N = s.N, P = s.P;
+  const size_t NP = (size_t)N * P;
+  const bool pimd = (flags & PMX_EVAL_PIMD) != 0;
+  const int nchrg = (int)s.ichrg.size();
+  const bool kspace = (c->pme || c->ewald) && !(flags & PMX_EVAL_NO_RECIP);
+  RedArgs red{};
+  if (timing) HIPCHK(hipEventRecord(c->ev[3], st));
   // --- real space (force_control -> force_verlst -> force_npol)
   {
-    hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
-                       c->q, c->site);
-    HIPCHK(hipGetLastError());
-    if (c->disp) {
-      hipLaunchKernelGGL(k_pack_disp, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm,
-                         c->disp);
-      hipLaunchKernelGGL(k_site_umax, dim3(blocks(N)), dim3(256), 0, st, N, P, c->disp, c->umax);
-      HIPCHK(hipGetLastError());
-    } else if (c->umax && c->full_list) {
-      hipLaunchKernelGGL(k_site_umax64, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym,
-                         c->zm, c->umax);
-      HIPCHK(hipGetLastError());
-    }
     PairArgs a{};
     a.N = N;
     a.P = P;
@@ -976,34 +1018,10 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool
     add_stage(red, c->off_pair, pair_grid_x(N, c->pair_waves), PAIR_NV,
               {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR});
   }
-  if (timing) HIPCHK(hipEventRecord(c->ev[2], st));
-  if (pme) {
-    if (!overlap)
-      if (int rc = launch_pme(st)) return rc;
-    add_stage(red, c->off_fft, s.ngrid[1], FFT_NV, {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
-  }
-  if (c->ewald && !(flags & PMX_EVAL_NO_RECIP)) {
-    EwaldArgs a{};
-    a.N = N;
-    a.P = P;
-    a.nchrg = nchrg;
-    a.nk = s.nktot;
-    a.x = c->x;
-    a.y = c->y;
-    a.z = c->z;
-    a.q = c->q;
-    a.ichrg = c->ichrg;
-    a.cell = c->cell;
-    a.kv = c->ew_kv;
-    a.kg = c->ew_kg;
-    a.sk = c->ew_sk;
-    a.fx = c->fx;
-    a.fy = c->fy;
-    a.fz = c->fz;
-    a.part = c->part + c->off_fft;
-    HIPCHK(launch_ewald(a, st));
-    add_stage(red, c->off_fft, (s.nktot + EWALD_BLOCK - 1) / EWALD_BLOCK, FFT_NV,
-              {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
+  if (timing) HIPCHK(hipEventRecord(c->ev[4], st));
+  if (kspace) {
+    const int G = c->pme ? s.ngrid[1] : (s.nktot + EWALD_BLOCK - 1) / EWALD_BLOCK;
+    add_stage(red, c->off_fft, G, FFT_NV, {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
   }
   IntraArgs ia = intra_args(c, pimd);
   const int bsP = bead_block(P);
@@ -1021,7 +1039,6 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool
     HIPCHK(hipGetLastError());
     intra_stage(c->off_ecor, G, ACC_VECOR);
   }
-  if (timing) HIPCHK(hipEventRecord(c->ev[3], st));
   // --- intramolecular (energy_control_intra.c:33-331)
   if (c->nbond) {
     const unsigned G = igrid((size_t)c->nbond * P);
@@ -1038,9 +1055,9 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool
     intra_stage(c->off_bend, G, ACC_VBEND);
   }
   // --- PIMD: f/P and virial correction, then ghosts, then modes
-  // the PME forces join the others here, with f/P for PIMD
-  if (overlap) HIPCHK(hipStreamWaitEvent(st, c->kjoin, 0));
-  const double *frec = pme ? c->frec : nullptr;
+  // the k-space forces join the others here, with f/P for PIMD
+  if (c->last_overlap) HIPCHK(hipStreamWaitEvent(st, c->kjoin, 0));
+  const double *frec = kspace ? c->frec : nullptr;
   if (pimd) {
     FinishArgs a{};
     a.N = N;
@@ -1099,8 +1116,15 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool
   red.acc = c->acc;
   hipLaunchKernelGGL(k_reduce_partials, dim3(blocks((size_t)P * ACC_NUM * 64)), dim3(256), 0, st, red);
   HIPCHK(hipGetLastError());
-  if (timing) HIPCHK(hipEventRecord(c->ev[4], st));
+  if (timing) HIPCHK(hipEventRecord(c->ev[5], st));
   return PMX_OK;
+}
+
+
+// The whole device sequence of a force call on st (batched calls capture it).
+int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st) {
+  if (int rc = pre_sequence(c, flags, st, false, false)) return rc;
+  return main_sequence(c, flags, st, false);
 }
 
 // Readback of the per-bead sums into pinned host memory, launched on st.
@@ -1121,19 +1145,19 @@ void assemble(pmx_ctx *c, unsigned flags, pmx_energies *out, bool timing) {
   memcpy(c->acc_h.data(), c->acc_pin, c->acc_h.size() * sizeof(double));
   const double ake = c->acc_pin[c->acc_h.size()];
   if (timing) {
-    float t;
-    for (int k = 0; k < 4; k++) {
-      (void)hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]);
-      c->ms[k] = t;
-    }
-    (void)hipEventElapsedTime(&t, c->ev[0], c->ev[4]);
-    c->ms[4] = t;
-    if (c->last_overlap) {  // k space on its own stream; the rest after the pair kernel
-      (void)hipEventElapsedTime(&t, c->ev[1], c->ev[5]);
-      c->ms[2] = t;
-      (void)hipEventElapsedTime(&t, c->ev[2], c->ev[4]);
-      c->ms[3] = t;
-    }
+    // events: 0 list check, 1 k space starts, 2 k space ends (on the second
+    // stream with overlap), 3 after the list decision, 4 after the pair
+    // kernel, 5 end
+    auto el = [&](int i, int j) {
+      float t = 0.f;
+      (void)hipEventElapsedTime(&t, c->ev[i], c->ev[j]);
+      return (double)t;
+    };
+    c->ms[0] = c->last_overlap ? el(0, 3) : el(0, 1) + el(2, 3);
+    c->ms[1] = el(3, 4);
+    c->ms[2] = el(1, 2);
+    c->ms[3] = el(4, 5);
+    c->ms[4] = el(0, 5);
   }
   // --- assemble (energy_control_inter_real.c:188-197, _recip.c:340-347, _intra.c)
   const double dP = (double)P;
@@ -1267,7 +1291,8 @@ pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *
   }
   if (hipEventCreateWithFlags(&c->ready, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->kfork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->kjoin, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->kjoin, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->mxev, hipEventDisableTiming) != hipSuccess) {
     pmx_destroy(c);
     return fail("pmx_create: event creation failed");
   }
@@ -1301,6 +1326,7 @@ void pmx_destroy(pmx_ctx *c) {
   if (c->ready) (void)hipEventDestroy(c->ready);
   if (c->kfork) (void)hipEventDestroy(c->kfork);
   if (c->kjoin) (void)hipEventDestroy(c->kjoin);
+  if (c->mxev) (void)hipEventDestroy(c->mxev);
   if (c->st) (void)hipStreamDestroy(c->st);
   if (c->sk) (void)hipStreamDestroy(c->sk);
   delete c;
@@ -1364,9 +1390,10 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   c->grad_ready = false;
   hipStream_t st = c->st;
   if ((rc = list_stage_launch(c, st, true))) return rc;
-  HIPCHK(hipStreamSynchronize(st));
+  if ((rc = pre_sequence(c, flags, st, true, true))) return rc;
+  HIPCHK(hipEventSynchronize(c->mxev));
   if ((rc = list_stage_finish(c, flags, out))) return rc;
-  if ((rc = force_sequence(c, flags, st, true, true))) return rc;
+  if ((rc = main_sequence(c, flags, st, true))) return rc;
   if ((rc = readback_launch(c, st))) return rc;
   HIPCHK(hipStreamSynchronize(st));
   assemble(c, flags, out, true);
@@ -1438,7 +1465,7 @@ static int batch_capture(pmx_batch *b, unsigned flags) {
     pmx_ctx *c = b->ctx[i];
     e = hipStreamWaitEvent(c->st, b->fork, 0);
     if (e != hipSuccess) break;
-    rc = force_sequence(c, batch_flags(c, flags), c->st, false, false);
+    rc = force_sequence(c, batch_flags(c, flags), c->st);
     if (rc) {
       b->err = "context " + std::to_string(i) + ": " + c->err;
       break;

@@ -108,7 +108,7 @@ struct EwaldArgs {
   const double4 *kv;  // [nk] (2 pi ka, 2 pi kb, 2 pi kc, preg)
   const double4 *kg;  // [nk] (kx, ky, kz, prep / |S|^2 factor)
   double2 *sk;        // [P][nk] 2 preg (Re S, Im S)
-  double *fx, *fy, *fz;
+  double *rx, *ry, *rz;  // the force of every (charge, bead), bead-major [P][N]
   double *part;       // partial rows, FFT_NV values
 };
 constexpr int EWALD_BLOCK = 256;

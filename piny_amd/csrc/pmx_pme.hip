@@ -518,12 +518,12 @@ __global__ void __launch_bounds__(EWALD_BLOCK) k_ewald_force(EwaldArgs a) {
       fz += skg[m].z * fd;
     }
   }
-  if (on) {
-    const size_t ib = (size_t)i * a.P + b;
+  if (on) {  // bead-major, added to the other forces after the pair kernel
+    const size_t ib = (size_t)b * a.N + i;
     const double q = a.q[i];
-    a.fx[ib] += fx * q;
-    a.fy[ib] += fy * q;
-    a.fz[ib] += fz * q;
+    a.rx[ib] = fx * q;
+    a.ry[ib] = fy * q;
+    a.rz[ib] = fz * q;
   }
 }
 
