@@ -793,7 +793,12 @@ int list_stage_launch(pmx_ctx *c, hipStream_t st, bool timing) {
   const int N = s.N, P = s.P;
   if (timing) HIPCHK(hipEventRecord(c->ev[0], st));
   HIPCHK(hipMemsetAsync(c->maxes, 0, 2 * sizeof(unsigned long long), st));
-  hipLaunchKernelGGL(k_list_check, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->x0, c->y0, c->z0,
+  const int sites_per_wave = P <= 64 ? 64 / P : 1;
+  const size_t check_waves = ((size_t)N + sites_per_wave - 1) / sites_per_wave;
+  // at most 512 workgroups, each making one atomic per maximum
+  const unsigned check_blocks = (unsigned)std::min<size_t>(512, blocks(check_waves * 64));
+  hipLaunchKernelGGL(k_list_check, dim3(check_blocks), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->x0,
+                     c->y0, c->z0,
                      c->b0x, c->b0y, c->b0z, c->maxes, (P > 1 && !c->xmod_given) ? 1 : 0, c->xm, c->ym, c->zm);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->mx_pin, c->maxes, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));

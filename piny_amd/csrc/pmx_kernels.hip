@@ -132,44 +132,89 @@ __global__ void k_site_umax(int N, int P, const float4 *__restrict__ u, double *
 // (get_pimd_spread, pimd_estimators.c:121-222) and, when asked, the bead
 // centroid as xmod, the mode-1 coordinate of the normal-mode transform
 // (control_pimd_trans.c:175-181 takes xmod from mode 1).  Maxima of squares are
-// reduced as the bit patterns of non-negative doubles.
-__global__ void k_list_check(int N, int P, const double *__restrict__ x, const double *__restrict__ y,
-                             const double *__restrict__ z, const double *__restrict__ x0,
-                             const double *__restrict__ y0, const double *__restrict__ z0, double *b0x,
-                             double *b0y, double *b0z, unsigned long long *maxes, int centroid, double *xm,
-                             double *ym, double *zm) {
+// reduced as the bit patterns of non-negative doubles.  A group of P lanes
+// holds one site's beads (64 / P sites per wavefront; with P > 64 one site
+// per wavefront in chunks of 64 beads), so the loads are coalesced; the
+// centroid sums run bead 0, 1, ... in order through lane shuffles.
+__device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
+
+__global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *__restrict__ x,
+                                                    const double *__restrict__ y, const double *__restrict__ z,
+                                                    const double *__restrict__ x0, const double *__restrict__ y0,
+                                                    const double *__restrict__ z0, double *b0x, double *b0y,
+                                                    double *b0z, unsigned long long *maxes, int centroid,
+                                                    double *xm, double *ym, double *zm) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double red[2][4];
+  const int lane = threadIdx.x & 63;
+  const int G = P <= 64 ? P : 64;           // lanes per site
+  const int S = 64 / G;                     // sites per wavefront
+  const int g = lane / G, k = lane - g * G;  // site group, bead within the chunk
+  const int lead = g * G;                   // lane of the site's bead 0 (of each chunk)
+  const int nunit = (N + S - 1) / S;
   double dmax = 0.0, smax = 0.0;
-  if (i < N) {
-    const double xi = x[(size_t)i * P], yi = y[(size_t)i * P], zi = z[(size_t)i * P];
-    b0x[i] = xi;
-    b0y[i] = yi;
-    b0z[i] = zi;
-    const double dx = xi - x0[i], dy = yi - y0[i], dz = zi - z0[i];
-    dmax = dx * dx + dy * dy + dz * dz;
-    // the centroid in the same pass (same order of sums as bead 0, 1, ...)
-    double sx = xi, sy = yi, sz = zi;
-    for (int b = 1; b < P; b++) {
-      const double xb = x[(size_t)i * P + b], yb = y[(size_t)i * P + b], zb = z[(size_t)i * P + b];
-      const double ex = xb - xi, ey = yb - yi, ez = zb - zi;
-      const double r2 = ex * ex + ey * ey + ez * ez;
-      smax = fmax(smax, r2);
-      sx += xb;
-      sy += yb;
-      sz += zb;
+  // grid-stride over wavefront units, so few workgroups make the global atomics
+  for (int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); wave < nunit;
+       wave += (int)((gridDim.x * blockDim.x) >> 6)) {
+    const int i = wave * S + g;
+    const bool act = g < S && i < N;
+    const size_t base = (size_t)(act ? i : 0) * P;
+    double sx = 0.0, sy = 0.0, sz = 0.0, xi = 0.0, yi = 0.0, zi = 0.0;
+    for (int c0 = 0; c0 < P; c0 += G) {
+      const bool on = act && c0 + k < P;
+      const double xb = on ? x[base + c0 + k] : 0.0, yb = on ? y[base + c0 + k] : 0.0,
+                   zb = on ? z[base + c0 + k] : 0.0;
+      if (c0 == 0) {
+        xi = shfl_d(xb, lead);
+        yi = shfl_d(yb, lead);
+        zi = shfl_d(zb, lead);
+      }
+      if (on) {
+        const double ex = xb - xi, ey = yb - yi, ez = zb - zi;
+        smax = fmax(smax, ex * ex + ey * ey + ez * ez);
+      }
+      const int n = min(G, P - c0);
+      for (int m = 0; m < n; m++) {  // every lane of the group forms the same ordered sum
+        const double vx = shfl_d(xb, lead + m), vy = shfl_d(yb, lead + m), vz = shfl_d(zb, lead + m);
+        if (c0 + m == 0) {
+          sx = vx;
+          sy = vy;
+          sz = vz;
+        } else {
+          sx += vx;
+          sy += vy;
+          sz += vz;
+        }
+      }
     }
-    if (centroid) {
-      xm[i] = sx / P;
-      ym[i] = sy / P;
-      zm[i] = sz / P;
+    if (act && k == 0) {
+      b0x[i] = xi;
+      b0y[i] = yi;
+      b0z[i] = zi;
+      const double dx = xi - x0[i], dy = yi - y0[i], dz = zi - z0[i];
+      dmax = fmax(dmax, dx * dx + dy * dy + dz * dz);
+      if (centroid) {
+        xm[i] = sx / P;
+        ym[i] = sy / P;
+        zm[i] = sz / P;
+      }
     }
   }
   for (int off = 32; off >= 1; off >>= 1) {
     dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
     smax = fmax(smax, __shfl_xor(smax, off, 64));
   }
-  if ((threadIdx.x & 63) == 0) {
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (lane == 0) {
+    red[0][w] = dmax;
+    red[1][w] = smax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int v = 1; v < nw; v++) {
+      dmax = fmax(dmax, red[0][v]);
+      smax = fmax(smax, red[1][v]);
+    }
     atomicMax(&maxes[0], (unsigned long long)__double_as_longlong(dmax));
     atomicMax(&maxes[1], (unsigned long long)__double_as_longlong(smax));
   }
