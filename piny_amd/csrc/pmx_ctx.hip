@@ -61,6 +61,12 @@ struct pmx_ctx {
       *atyp = nullptr, *e1 = nullptr, *e2 = nullptr;
   double *bc = nullptr, *beq = nullptr, *ac = nullptr, *as = nullptr;
   int nghost = 0, *gatom = nullptr, *gncomp = nullptr, *gcomp = nullptr;
+  // molecules of k_intra_mol (nmol = 0: the topology does not split into
+  // molecules of at most MOL_MAX consecutive sites; per-term kernels then)
+  int nmol = 0;
+  int *mol_lo = nullptr, *mol_n = nullptr, *mol_ecor_off = nullptr, *mol_ecor = nullptr, *mol_bond_off = nullptr,
+      *mol_bond = nullptr, *mol_bend_off = nullptr, *mol_bend = nullptr, *mol_ghost_off = nullptr,
+      *mol_ghost = nullptr;
   double *gcoef = nullptr;
   // PME
   int pme = 0, npass[3] = {0, 0, 0}, radix[3][FFT_MAX_PASSES] = {};
@@ -368,6 +374,69 @@ std::vector<double> pair_constants(pmx_ctx *c) {
   return pp;
 }
 
+// Molecules for k_intra_mol: the connected components of the bonds, bends,
+// excluded pairs and ghost constructions.  Used when every component is a
+// run of at most MOL_MAX consecutive sites (water and other small
+// molecules); each term is listed under the molecule of its sites.
+int setup_molecules(pmx_ctx *c, const pmx_system *sys) {
+  if (const char *env = getenv("PMX_INTRA"))  // tuning override: "terms" keeps the per-term kernels
+    if (!strcmp(env, "terms")) return PMX_OK;
+  const Setup &s = c->s;
+  const int N = s.N;
+  std::vector<int> par(N);
+  for (int i = 0; i < N; i++) par[i] = i;
+  auto find = [&](int i) {
+    while (par[i] != i) i = par[i] = par[par[i]];
+    return i;
+  };
+  auto unite = [&](int i, int j) {
+    i = find(i);
+    j = find(j);
+    if (i != j) par[std::max(i, j)] = std::min(i, j);
+  };
+  for (int k = 0; k < sys->nbond; k++) unite(sys->bond_j1[k], sys->bond_j2[k]);
+  for (int k = 0; k < sys->nbend; k++) {
+    unite(sys->bend_j1[k], sys->bend_j2[k]);
+    unite(sys->bend_j3[k], sys->bend_j2[k]);
+  }
+  for (size_t k = 0; k < s.e1.size(); k++) unite(s.e1[k], s.e2[k]);
+  for (int g = 0; g < sys->nghost; g++)
+    for (int k = 0; k < sys->ghost_ncomp[g]; k++) unite(sys->ghost_atom[g], sys->ghost_comp[4 * g + k]);
+  // components in order of their first site; each must be a run of sites
+  std::vector<int> mol(N), lo, n;
+  for (int i = 0; i < N; i++) {
+    const int r = find(i);
+    if (r == i) {
+      mol[i] = (int)lo.size();
+      lo.push_back(i);
+      n.push_back(1);
+    } else {
+      mol[i] = mol[r];
+      if (i != lo[mol[i]] + n[mol[i]]) return PMX_OK;  // not consecutive
+      if (++n[mol[i]] > MOL_MAX) return PMX_OK;
+    }
+  }
+  const int M = (int)lo.size();
+  auto csr = [&](int nterm, auto site_of, int **d_off, int **d_list) -> int {
+    std::vector<int> off(M + 1, 0), list(nterm);
+    for (int k = 0; k < nterm; k++) off[mol[site_of(k)] + 1]++;
+    for (int m = 0; m < M; m++) off[m + 1] += off[m];
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int k = 0; k < nterm; k++) list[pos[mol[site_of(k)]]++] = k;
+    int rc = upload(c, d_off, off.data(), off.size());
+    return rc ? rc : upload(c, d_list, list.data(), list.size());
+  };
+  int rc;
+  if ((rc = upload(c, &c->mol_lo, lo.data(), M)) || (rc = upload(c, &c->mol_n, n.data(), M)) ||
+      (rc = csr((int)s.e1.size(), [&](int k) { return s.e1[k]; }, &c->mol_ecor_off, &c->mol_ecor)) ||
+      (rc = csr(sys->nbond, [&](int k) { return sys->bond_j1[k]; }, &c->mol_bond_off, &c->mol_bond)) ||
+      (rc = csr(sys->nbend, [&](int k) { return sys->bend_j2[k]; }, &c->mol_bend_off, &c->mol_bend)) ||
+      (rc = csr(sys->nghost, [&](int k) { return sys->ghost_atom[k]; }, &c->mol_ghost_off, &c->mol_ghost)))
+    return rc;
+  c->nmol = M;
+  return PMX_OK;
+}
+
 int build_device(pmx_ctx *c, const pmx_system *sys) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
@@ -497,6 +566,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(upload(c, &c->gncomp, sys->ghost_ncomp, sys->nghost));
   TRY(upload(c, &c->gcomp, sys->ghost_comp, (size_t)4 * sys->nghost));
   TRY(upload(c, &c->gcoef, sys->ghost_coef, (size_t)4 * sys->nghost));
+  TRY(setup_molecules(c, sys));
   // PME grids, twiddles and the influence function on the sphere list
   c->pme = (!s.ichrg.empty() && c->par.pme_on);
   c->pme_f32 = (c->par.precision == PMX_PREC_MIXED) ? 1 : 0;
@@ -1037,60 +1107,96 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     sl[0] = eslot;
     add_stage(red, off, (int)G, INTRA_NV, sl);
   };
-  if (nchrg > 0 && ia.necor > 0) {
-    const unsigned G = igrid((size_t)ia.necor * P);
-    ia.part = c->part + c->off_ecor;
-    hipLaunchKernelGGL(k_ecor, dim3(G), dim3(bsP), 0, st, ia);
+  const double *frec = kspace ? c->frec : nullptr;
+  if (c->nmol > 0) {
+    // --- intramolecular (energy_control_intra.c:33-331), PIMD finish and
+    // ghosts in one pass over the molecules
+    if (c->last_overlap) HIPCHK(hipStreamWaitEvent(st, c->kjoin, 0));
+    const unsigned G = igrid((size_t)c->nmol * P);
+    ia.necor = nchrg > 0 ? ia.necor : 0;
+    ia.nmol = c->nmol;
+    ia.mol_lo = c->mol_lo;
+    ia.mol_n = c->mol_n;
+    ia.mol_ecor_off = c->mol_ecor_off;
+    ia.mol_ecor = c->mol_ecor;
+    ia.mol_bond_off = c->mol_bond_off;
+    ia.mol_bond = c->mol_bond;
+    ia.mol_bend_off = c->mol_bend_off;
+    ia.mol_bend = c->mol_bend;
+    ia.mol_ghost_off = c->mol_ghost_off;
+    ia.mol_ghost = c->mol_ghost;
+    ia.gatom = c->gatom;
+    ia.gncomp = c->gncomp;
+    ia.gcomp = c->gcomp;
+    ia.gcoef = c->gcoef;
+    ia.rec = frec;
+    ia.part_ecor = c->part + c->off_ecor;
+    ia.part_bond = c->part + c->off_bond;
+    ia.part_bend = c->part + c->off_bend;
+    ia.part_fin = c->part + c->off_fin;
+    hipLaunchKernelGGL(k_intra_mol, dim3(G), dim3(bsP), 0, st, ia);
     HIPCHK(hipGetLastError());
     intra_stage(c->off_ecor, G, ACC_VECOR);
-  }
-  // --- intramolecular (energy_control_intra.c:33-331)
-  if (c->nbond) {
-    const unsigned G = igrid((size_t)c->nbond * P);
-    ia.part = c->part + c->off_bond;
-    hipLaunchKernelGGL(k_bond, dim3(G), dim3(bsP), 0, st, ia);
-    HIPCHK(hipGetLastError());
     intra_stage(c->off_bond, G, ACC_VBOND);
-  }
-  if (c->nbend) {
-    const unsigned G = igrid((size_t)c->nbend * P);
-    ia.part = c->part + c->off_bend;
-    hipLaunchKernelGGL(k_bend, dim3(G), dim3(bsP), 0, st, ia);
-    HIPCHK(hipGetLastError());
     intra_stage(c->off_bend, G, ACC_VBEND);
-  }
-  // --- PIMD: f/P and virial correction, then ghosts, then modes
-  // the k-space forces join the others here, with f/P for PIMD
-  if (c->last_overlap) HIPCHK(hipStreamWaitEvent(st, c->kjoin, 0));
-  const double *frec = kspace ? c->frec : nullptr;
-  if (pimd) {
-    FinishArgs a{};
-    a.N = N;
-    a.P = P;
-    a.x = c->x;
-    a.y = c->y;
-    a.z = c->z;
-    a.xm = c->xm;
-    a.ym = c->ym;
-    a.zm = c->zm;
-    a.fx = c->fx;
-    a.fy = c->fy;
-    a.fz = c->fz;
-    a.rec = frec;
-    a.part = c->part + c->off_fin;
-    const unsigned G = igrid(NP);
-    hipLaunchKernelGGL(k_pimd_finish, dim3(G), dim3(bsP), 0, st, a);
-    HIPCHK(hipGetLastError());
-    add_stage(red, c->off_fin, (int)G, FIN_NV,
-              {ACC_C11, ACC_C22, ACC_C33, ACC_C12, ACC_C21, ACC_C13, ACC_C31, ACC_C23, ACC_C32, ACC_AKVIR});
-  } else if (frec) {
-    hipLaunchKernelGGL(k_add_rec, dim3(blocks(NP)), dim3(256), 0, st, N, P, frec, c->fx, c->fy, c->fz);
-    HIPCHK(hipGetLastError());
-  }
-  if (c->nghost) {
-    hipLaunchKernelGGL(k_ghost_force, dim3(blocks((size_t)c->nghost * P)), dim3(256), 0, st, c->nghost, P,
-                       c->gatom, c->gncomp, c->gcomp, c->gcoef, c->fx, c->fy, c->fz);
-    HIPCHK(hipGetLastError());
+    if (pimd)
+      add_stage(red, c->off_fin, (int)G, FIN_NV,
+                {ACC_C11, ACC_C22, ACC_C33, ACC_C12, ACC_C21, ACC_C13, ACC_C31, ACC_C23, ACC_C32, ACC_AKVIR});
+  } else {
+    if (nchrg > 0 && ia.necor > 0) {
+      const unsigned G = igrid((size_t)ia.necor * P);
+      ia.part = c->part + c->off_ecor;
+      hipLaunchKernelGGL(k_ecor, dim3(G), dim3(bsP), 0, st, ia);
+      HIPCHK(hipGetLastError());
+      intra_stage(c->off_ecor, G, ACC_VECOR);
+    }
+    // --- intramolecular (energy_control_intra.c:33-331)
+    if (c->nbond) {
+      const unsigned G = igrid((size_t)c->nbond * P);
+      ia.part = c->part + c->off_bond;
+      hipLaunchKernelGGL(k_bond, dim3(G), dim3(bsP), 0, st, ia);
+      HIPCHK(hipGetLastError());
+      intra_stage(c->off_bond, G, ACC_VBOND);
+    }
+    if (c->nbend) {
+      const unsigned G = igrid((size_t)c->nbend * P);
+      ia.part = c->part + c->off_bend;
+      hipLaunchKernelGGL(k_bend, dim3(G), dim3(bsP), 0, st, ia);
+      HIPCHK(hipGetLastError());
+      intra_stage(c->off_bend, G, ACC_VBEND);
+    }
+    // --- PIMD: f/P and virial correction, then ghosts, then modes
+    // the k-space forces join the others here, with f/P for PIMD
+    if (c->last_overlap) HIPCHK(hipStreamWaitEvent(st, c->kjoin, 0));
+    if (pimd) {
+      FinishArgs a{};
+      a.N = N;
+      a.P = P;
+      a.x = c->x;
+      a.y = c->y;
+      a.z = c->z;
+      a.xm = c->xm;
+      a.ym = c->ym;
+      a.zm = c->zm;
+      a.fx = c->fx;
+      a.fy = c->fy;
+      a.fz = c->fz;
+      a.rec = frec;
+      a.part = c->part + c->off_fin;
+      const unsigned G = igrid(NP);
+      hipLaunchKernelGGL(k_pimd_finish, dim3(G), dim3(bsP), 0, st, a);
+      HIPCHK(hipGetLastError());
+      add_stage(red, c->off_fin, (int)G, FIN_NV,
+                {ACC_C11, ACC_C22, ACC_C33, ACC_C12, ACC_C21, ACC_C13, ACC_C31, ACC_C23, ACC_C32, ACC_AKVIR});
+    } else if (frec) {
+      hipLaunchKernelGGL(k_add_rec, dim3(blocks(NP)), dim3(256), 0, st, N, P, frec, c->fx, c->fy, c->fz);
+      HIPCHK(hipGetLastError());
+    }
+    if (c->nghost) {
+      hipLaunchKernelGGL(k_ghost_force, dim3(blocks((size_t)c->nghost * P)), dim3(256), 0, st, c->nghost, P,
+                         c->gatom, c->gncomp, c->gcomp, c->gcoef, c->fx, c->fy, c->fz);
+      HIPCHK(hipGetLastError());
+    }
   }
   const bool modes = pimd && (flags & PMX_EVAL_MODE_FORCES) && P > 1;
   if (modes) {  // fxt: the bead forces before the transform (energy_control_pimd.c:116-144)

@@ -168,7 +168,18 @@ struct IntraArgs {
   int nbond_type, nbend_type;
   double *gbond;  // [P][nbond_type][7] sum over bonds of (r - eq)^m
   double *gbend;  // [P][nbend_type][BEND_GRAD_NV]
+  // k_intra_mol: molecules of consecutive sites (first site, size) with the
+  // CSR lists of their terms, the ghosts, the k-space forces to add
+  // (bead-major [3][P][N], or null) and the partial rows it fills
+  int nmol;
+  const int *mol_lo, *mol_n, *mol_ecor_off, *mol_ecor, *mol_bond_off, *mol_bond, *mol_bend_off, *mol_bend,
+      *mol_ghost_off, *mol_ghost;
+  const int *gatom, *gncomp, *gcomp;
+  const double *gcoef;
+  const double *rec;
+  double *part_ecor, *part_bond, *part_bend, *part_fin;
 };
+constexpr int MOL_MAX = 8;  // sites of a molecule of k_intra_mol
 
 // Bend moments per (bead, bend type): cos^m (m = 0..6), sin, cos sin^(m-1)
 // (m = 2..6): the derivatives of V with respect to c_m and s_m.
@@ -246,6 +257,7 @@ __global__ void k_bend_moments(IntraArgs a);
 __global__ void k_ecor(IntraArgs a);
 __global__ void k_bond(IntraArgs a);
 __global__ void k_bend(IntraArgs a);
+__global__ void k_intra_mol(IntraArgs a);
 __global__ void k_pimd_finish(FinishArgs a);
 __global__ void k_add_rec(int N, int P, const double *rec, double *fx, double *fy, double *fz);
 __global__ void k_ghost_force(int ng, int P, const int *gatom, const int *gncomp, const int *gcomp,

@@ -1167,101 +1167,199 @@ __device__ __forceinline__ void add_virial(double (&acc)[INTRA_NV], double e, do
   acc[6] += dy * fz;
 }
 
+// One term of each intramolecular kind for bead b: energy and virial into
+// acc, forces through add(site, fx, fy, fz) (global atomics in the per-term
+// kernels, per-thread LDS rows in k_intra_mol).
+
 // ecor, ecorr.c:28-265 (no switching on the table)
-__global__ void __launch_bounds__(256) k_ecor(IntraArgs a) {
-  double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
-  PMX_TERM_LOOP(a.necor) {
-    const int k = (int)(t / a.P), b = (int)(t % a.P);
-    const int i = a.e1[k], j = a.e2[k];
-    double dx, dy, dz;
-    intra_sep(a, i, j, b, dx, dy, dz);
-    const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
-    int kk = (int)(((r12 - a.e_rmin) * a.e_dri) + 0.5) + 3;
-    kk = min(kk, a.nsplin - 2);
-    kk = max(kk, 3);
-    const double rsp = ((double)(kk - 3)) * a.e_dr + a.e_rmin;
-    const double del = (r12 - rsp) * a.e_dri;
-    const double q12 = a.q[i] * a.q[j];
-    const double e = spline4(a.ecor_tab, kk, del, false) * q12;
-    const double dvt = spline4(a.ecor_tab, kk, del, true) * q12;
-    const double fx = dx * dvt, fy = dy * dvt, fz = dz * dvt;
-    add_force(a, i, b, fx, fy, fz);
-    add_force(a, j, b, -fx, -fy, -fz);
-    add_virial(acc, e, dx, dy, dz, fx, fy, fz);
-  }
-  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);
+template <class Add>
+__device__ __forceinline__ void ecor_term(const IntraArgs &a, int k, int b, double (&acc)[INTRA_NV], Add add) {
+  const int i = a.e1[k], j = a.e2[k];
+  double dx, dy, dz;
+  intra_sep(a, i, j, b, dx, dy, dz);
+  const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
+  int kk = (int)(((r12 - a.e_rmin) * a.e_dri) + 0.5) + 3;
+  kk = min(kk, a.nsplin - 2);
+  kk = max(kk, 3);
+  const double rsp = ((double)(kk - 3)) * a.e_dr + a.e_rmin;
+  const double del = (r12 - rsp) * a.e_dri;
+  const double q12 = a.q[i] * a.q[j];
+  const double e = spline4(a.ecor_tab, kk, del, false) * q12;
+  const double dvt = spline4(a.ecor_tab, kk, del, true) * q12;
+  const double fx = dx * dvt, fy = dy * dvt, fz = dz * dvt;
+  add(i, fx, fy, fz);
+  add(j, -fx, -fy, -fz);
+  add_virial(acc, e, dx, dy, dz, fx, fy, fz);
 }
 
 // bond, bond.c:29-312
-__global__ void __launch_bounds__(256) k_bond(IntraArgs a) {
-  double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
-  PMX_TERM_LOOP(a.nbond) {
-    const int k = (int)(t / a.P), b = (int)(t % a.P);
-    const int i = a.b1[k], j = a.b2[k], ty = a.btyp[k];
-    const double *cc = a.bc + 7 * ty;
-    double dx, dy, dz;
-    intra_sep(a, i, j, b, dx, dy, dz);
-    const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
-    const double rr0 = r12 - a.beq[ty];
-    const double e =
-        (((((cc[6] * rr0 + cc[5]) * rr0 + cc[4]) * rr0 + cc[3]) * rr0 + cc[2]) * rr0 + cc[1]) * rr0 + cc[0];
-    const double dvb = (((((6.0 * cc[6]) * rr0 + 5.0 * cc[5]) * rr0 + 4.0 * cc[4]) * rr0 + 3.0 * cc[3]) * rr0 +
-                        2.0 * cc[2]) * rr0 + cc[1];
-    const double pre = -dvb / r12;
-    const double fx = dx * pre, fy = dy * pre, fz = dz * pre;
-    add_force(a, i, b, fx, fy, fz);
-    add_force(a, j, b, -fx, -fy, -fz);
-    add_virial(acc, e, dx, dy, dz, fx, fy, fz);
-  }
-  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);
+template <class Add>
+__device__ __forceinline__ void bond_term(const IntraArgs &a, int k, int b, double (&acc)[INTRA_NV], Add add) {
+  const int i = a.b1[k], j = a.b2[k], ty = a.btyp[k];
+  const double *cc = a.bc + 7 * ty;
+  double dx, dy, dz;
+  intra_sep(a, i, j, b, dx, dy, dz);
+  const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
+  const double rr0 = r12 - a.beq[ty];
+  const double e =
+      (((((cc[6] * rr0 + cc[5]) * rr0 + cc[4]) * rr0 + cc[3]) * rr0 + cc[2]) * rr0 + cc[1]) * rr0 + cc[0];
+  const double dvb = (((((6.0 * cc[6]) * rr0 + 5.0 * cc[5]) * rr0 + 4.0 * cc[4]) * rr0 + 3.0 * cc[3]) * rr0 +
+                      2.0 * cc[2]) * rr0 + cc[1];
+  const double pre = -dvb / r12;
+  const double fx = dx * pre, fy = dy * pre, fz = dz * pre;
+  add(i, fx, fy, fz);
+  add(j, -fx, -fy, -fz);
+  add_virial(acc, e, dx, dy, dz, fx, fy, fz);
 }
 
 // bend, bend.c:28-434
-__global__ void __launch_bounds__(256) k_bend(IntraArgs a) {
-  double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
-  PMX_TERM_LOOP(a.nbend) {
-    const int k = (int)(t / a.P), b = (int)(t % a.P);
-    const int i1 = a.a1[k], i2 = a.a2[k], i3 = a.a3[k], ty = a.atyp[k];
-    const double *cc = a.ac + 7 * ty, *ss = a.as + 7 * ty;
-    double dx12, dy12, dz12, dx23, dy23, dz23;
-    intra_sep(a, i1, i2, b, dx12, dy12, dz12);
-    intra_sep(a, i3, i2, b, dx23, dy23, dz23);
-    const double r122i = 1.0 / (dx12 * dx12 + dy12 * dy12 + dz12 * dz12);
-    const double r322i = 1.0 / (dx23 * dx23 + dy23 * dy23 + dz23 * dz23);
-    const double rpmag = sqrt(r122i) * sqrt(r322i);
-    double cost = (dx12 * dx23 + dy12 * dy23 + dz12 * dz23) * rpmag;
-    cost = (cost < 1.0 ? cost : 1.0);
-    cost = (cost > -1.0 ? cost : -1.0);
-    double sint = sqrt(1.0 - cost * cost);
-    sint = (sint > 1.0e-8 ? sint : 1.0e-8);
-    const double cos122 = cost * r122i, cos322 = cost * r322i;
-    const double vbc = ((((((cc[6] * cost + cc[5]) * cost + cc[4]) * cost + cc[3]) * cost + cc[2]) * cost + cc[1]) *
-                            cost + cc[0]);
-    const double sisum = (((((ss[6] * sint + ss[5]) * sint + ss[4]) * sint + ss[3]) * sint + ss[2]) * sint);
-    const double e = vbc + sisum * cost + ss[1] * sint;
-    const double dvbc = (((((6.0 * cc[6]) * cost + 5.0 * cc[5]) * cost + 4.0 * cc[4]) * cost + 3.0 * cc[3]) * cost +
-                         2.0 * cc[2]) * cost + cc[1];
-    double dvbs = ((((5.0 * ss[6]) * sint + 4.0 * ss[5]) * sint + 3.0 * ss[4]) * sint + 2.0 * ss[3]) * sint + ss[2];
-    dvbs = dvbs * cost + ss[1];
-    const double pre = -(dvbc + sisum - dvbs * cost / sint);
-    const double f1x = (dx23 * rpmag - dx12 * cos122) * pre;
-    const double f1y = (dy23 * rpmag - dy12 * cos122) * pre;
-    const double f1z = (dz23 * rpmag - dz12 * cos122) * pre;
-    const double f3x = (dx12 * rpmag - dx23 * cos322) * pre;
-    const double f3y = (dy12 * rpmag - dy23 * cos322) * pre;
-    const double f3z = (dz12 * rpmag - dz23 * cos322) * pre;
-    add_force(a, i1, b, f1x, f1y, f1z);
-    add_force(a, i2, b, -(f1x + f3x), -(f1y + f3y), -(f1z + f3z));
-    add_force(a, i3, b, f3x, f3y, f3z);
-    acc[0] += e;
-    acc[1] += dx12 * f1x + dx23 * f3x;
-    acc[2] += dy12 * f1y + dy23 * f3y;
-    acc[3] += dz12 * f1z + dz23 * f3z;
-    acc[4] += dx12 * f1y + dx23 * f3y;
-    acc[5] += dx12 * f1z + dx23 * f3z;
-    acc[6] += dy12 * f1z + dy23 * f3z;
+template <class Add>
+__device__ __forceinline__ void bend_term(const IntraArgs &a, int k, int b, double (&acc)[INTRA_NV], Add add) {
+  const int i1 = a.a1[k], i2 = a.a2[k], i3 = a.a3[k], ty = a.atyp[k];
+  const double *cc = a.ac + 7 * ty, *ss = a.as + 7 * ty;
+  double dx12, dy12, dz12, dx23, dy23, dz23;
+  intra_sep(a, i1, i2, b, dx12, dy12, dz12);
+  intra_sep(a, i3, i2, b, dx23, dy23, dz23);
+  const double r122i = 1.0 / (dx12 * dx12 + dy12 * dy12 + dz12 * dz12);
+  const double r322i = 1.0 / (dx23 * dx23 + dy23 * dy23 + dz23 * dz23);
+  const double rpmag = sqrt(r122i) * sqrt(r322i);
+  double cost = (dx12 * dx23 + dy12 * dy23 + dz12 * dz23) * rpmag;
+  cost = (cost < 1.0 ? cost : 1.0);
+  cost = (cost > -1.0 ? cost : -1.0);
+  double sint = sqrt(1.0 - cost * cost);
+  sint = (sint > 1.0e-8 ? sint : 1.0e-8);
+  const double cos122 = cost * r122i, cos322 = cost * r322i;
+  const double vbc = ((((((cc[6] * cost + cc[5]) * cost + cc[4]) * cost + cc[3]) * cost + cc[2]) * cost + cc[1]) *
+                          cost + cc[0]);
+  const double sisum = (((((ss[6] * sint + ss[5]) * sint + ss[4]) * sint + ss[3]) * sint + ss[2]) * sint);
+  const double e = vbc + sisum * cost + ss[1] * sint;
+  const double dvbc = (((((6.0 * cc[6]) * cost + 5.0 * cc[5]) * cost + 4.0 * cc[4]) * cost + 3.0 * cc[3]) * cost +
+                       2.0 * cc[2]) * cost + cc[1];
+  double dvbs = ((((5.0 * ss[6]) * sint + 4.0 * ss[5]) * sint + 3.0 * ss[4]) * sint + 2.0 * ss[3]) * sint + ss[2];
+  dvbs = dvbs * cost + ss[1];
+  const double pre = -(dvbc + sisum - dvbs * cost / sint);
+  const double f1x = (dx23 * rpmag - dx12 * cos122) * pre;
+  const double f1y = (dy23 * rpmag - dy12 * cos122) * pre;
+  const double f1z = (dz23 * rpmag - dz12 * cos122) * pre;
+  const double f3x = (dx12 * rpmag - dx23 * cos322) * pre;
+  const double f3y = (dy12 * rpmag - dy23 * cos322) * pre;
+  const double f3z = (dz12 * rpmag - dz23 * cos322) * pre;
+  add(i1, f1x, f1y, f1z);
+  add(i2, -(f1x + f3x), -(f1y + f3y), -(f1z + f3z));
+  add(i3, f3x, f3y, f3z);
+  acc[0] += e;
+  acc[1] += dx12 * f1x + dx23 * f3x;
+  acc[2] += dy12 * f1y + dy23 * f3y;
+  acc[3] += dz12 * f1z + dz23 * f3z;
+  acc[4] += dx12 * f1y + dx23 * f3y;
+  acc[5] += dx12 * f1z + dx23 * f3z;
+  acc[6] += dy12 * f1z + dy23 * f3z;
+}
+
+// Per-term kernels (topologies that do not split into small molecules):
+// one thread per (term, bead), forces by global atomics.
+#define PMX_TERM_KERNEL(name, fn, nterm)                                                      \
+  __global__ void __launch_bounds__(256) name(IntraArgs a) {                                   \
+    double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};                                              \
+    PMX_TERM_LOOP(nterm) {                                                                     \
+      const int b = (int)(t % a.P);                                                            \
+      fn(a, (int)(t / a.P), b, acc,                                                            \
+         [&](int i, double fx, double fy, double fz) { add_force(a, i, b, fx, fy, fz); });     \
+    }                                                                                          \
+    block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);                 \
   }
-  block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);
+PMX_TERM_KERNEL(k_ecor, ecor_term, a.necor)
+PMX_TERM_KERNEL(k_bond, bond_term, a.nbond)
+PMX_TERM_KERNEL(k_bend, bend_term, a.nbend)
+#undef PMX_TERM_KERNEL
+
+// Every intramolecular term of a force call in one pass, for topologies
+// whose terms (bonds, bends, excluded pairs, ghosts) split into molecules of
+// at most MOL_MAX consecutive sites: one thread per (molecule, bead) holds
+// the molecule's forces in an LDS row of its own, starting from the pair
+// forces, adds the excluded-pair, bond and bend terms, then does what
+// k_pimd_finish (or k_add_rec) and k_ghost_force do for these sites, in the
+// same order, and writes the forces once: no atomics.  Energies and virials
+// go to the partial rows of the ecor, bond, bend and finish stages.
+__global__ void __launch_bounds__(256) k_intra_mol(IntraArgs a) {
+  __shared__ double fl[MOL_MAX * 3][256];
+  double ae[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0}, ab[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
+  double aa[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
+  double cf[FIN_NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int tid = threadIdx.x;
+  const size_t NP = (size_t)a.N * a.P;
+  const double dP = (double)a.P;
+  PMX_TERM_LOOP(a.nmol) {
+    const int m = (int)(t / a.P), b = (int)(t % a.P);
+    const int lo = a.mol_lo[m], n = a.mol_n[m];
+    for (int s = 0; s < n; s++) {
+      const size_t ib = (size_t)(lo + s) * a.P + b;
+      fl[3 * s][tid] = a.fx[ib];
+      fl[3 * s + 1][tid] = a.fy[ib];
+      fl[3 * s + 2][tid] = a.fz[ib];
+    }
+    auto add = [&](int i, double fx, double fy, double fz) {
+      const int s = i - lo;
+      fl[3 * s][tid] += fx;
+      fl[3 * s + 1][tid] += fy;
+      fl[3 * s + 2][tid] += fz;
+    };
+    if (a.necor > 0)
+      for (int q = a.mol_ecor_off[m]; q < a.mol_ecor_off[m + 1]; q++) ecor_term(a, a.mol_ecor[q], b, ae, add);
+    for (int q = a.mol_bond_off[m]; q < a.mol_bond_off[m + 1]; q++) bond_term(a, a.mol_bond[q], b, ab, add);
+    for (int q = a.mol_bend_off[m]; q < a.mol_bend_off[m + 1]; q++) bend_term(a, a.mol_bend[q], b, aa, add);
+    for (int s = 0; s < n; s++) {
+      const int i = lo + s;
+      double fx = fl[3 * s][tid], fy = fl[3 * s + 1][tid], fz = fl[3 * s + 2][tid];
+      if (a.rec) {
+        const size_t r = (size_t)b * a.N + i;
+        fx += a.rec[r];
+        fy += a.rec[NP + r];
+        fz += a.rec[2 * NP + r];
+      }
+      if (a.pimd) {
+        fx /= dP;
+        fy /= dP;
+        fz /= dP;
+        if (a.P > 1) {
+          const size_t ib = (size_t)i * a.P + b;
+          const double dx = a.x[ib] - a.xm[i], dy = a.y[ib] - a.ym[i], dz = a.z[ib] - a.zm[i];
+          cf[0] -= dx * fx;
+          cf[1] -= dy * fy;
+          cf[2] -= dz * fz;
+          cf[3] -= dx * fy;
+          cf[4] -= dy * fx;
+          cf[5] -= dx * fz;
+          cf[6] -= dz * fx;
+          cf[7] -= dy * fz;
+          cf[8] -= dz * fy;
+          cf[9] -= dx * fx + dy * fy + dz * fz;
+        }
+      }
+      fl[3 * s][tid] = fx;
+      fl[3 * s + 1][tid] = fy;
+      fl[3 * s + 2][tid] = fz;
+    }
+    for (int q = a.mol_ghost_off[m]; q < a.mol_ghost_off[m + 1]; q++) {  // distrib_ghost_force
+      const int g = a.mol_ghost[q], sg = a.gatom[g] - lo;
+      const double gx = fl[3 * sg][tid], gy = fl[3 * sg + 1][tid], gz = fl[3 * sg + 2][tid];
+      for (int k = 0; k < a.gncomp[g]; k++) add(a.gcomp[4 * g + k], a.gcoef[4 * g + k] * gx,
+                                                a.gcoef[4 * g + k] * gy, a.gcoef[4 * g + k] * gz);
+      fl[3 * sg][tid] = 0.0;
+      fl[3 * sg + 1][tid] = 0.0;
+      fl[3 * sg + 2][tid] = 0.0;
+    }
+    for (int s = 0; s < n; s++) {
+      const size_t ib = (size_t)(lo + s) * a.P + b;
+      a.fx[ib] = fl[3 * s][tid];
+      a.fy[ib] = fl[3 * s + 1][tid];
+      a.fz[ib] = fl[3 * s + 2][tid];
+    }
+  }
+  block_partials<INTRA_NV>(ae, a.P, 0, a.P, a.part_ecor, blockIdx.x, gridDim.x);
+  block_partials<INTRA_NV>(ab, a.P, 0, a.P, a.part_bond, blockIdx.x, gridDim.x);
+  block_partials<INTRA_NV>(aa, a.P, 0, a.P, a.part_bend, blockIdx.x, gridDim.x);
+  block_partials<FIN_NV>(cf, a.P, 0, a.P, a.part_fin, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------

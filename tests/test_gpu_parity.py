@@ -335,3 +335,34 @@ def test_pme_beside_pair_kernel(case, mode, monkeypatch):
         for k in abi.ENERGY_KEYS:
             assert abs(e1[k] - e0[k]) <= tol * max(abs(e0[k]), 1e-3), (k, e1[k], e0[k])
         assert np.max(np.abs(f1 - f0)) <= tol * np.max(np.abs(f0))
+
+
+@pytest.mark.parametrize("case", ["spc64", "tip64_p4", "tip256_p32", "spc64_ewald", "spc64_noq"])
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_molecule_pass_matches_term_kernels(case, mode, monkeypatch):
+    # the one-pass molecule kernel (bonded terms, excluded pairs, PIMD
+    # finish and ghosts per (molecule, bead), no atomics) against the
+    # per-term kernels with global atomics, over two calls with the
+    # positions moved in between; both are also checked against the oracle
+    # by the other tests (the molecule pass is the default for water)
+    kw = {} if mode == "fp64" else {"precision": abi.PMX_PREC_MIXED}
+    c = cases.get(case)
+    s = c.system
+    flags = (abi.PMX_EVAL_PIMD | abi.PMX_EVAL_MODE_FORCES) if s.pi_beads > 1 else 0
+    print(f"[molecule pass {case} {mode}]", flush=True)
+    res = []
+    for how in ("terms", "molecules"):
+        monkeypatch.setenv("PMX_INTRA", how)
+        g = Engine(s, c.x, c.y, c.z, **kw)
+        out = []
+        for call in range(2):
+            d = 0.01 * call
+            g.set_positions(c.x + d, c.y - d, c.z + 0.5 * d)
+            e = g.energy_control(flags)
+            out.append((e, g.forces()))
+        res.append(out)
+    tol = 1e-12 if mode == "fp64" else 1e-6
+    for (e0, f0), (e1, f1) in zip(*res):
+        for k in abi.ENERGY_KEYS:
+            assert abs(e1[k] - e0[k]) <= tol * max(abs(e0[k]), 1e-3), (k, e1[k], e0[k])
+        assert np.max(np.abs(f1 - f0)) <= tol * np.max(np.abs(f0))
