@@ -42,7 +42,8 @@ struct pmx_ctx {
   bool last_modes = false, forces_valid = false;
   double *xm = nullptr, *ym = nullptr, *zm = nullptr;
   double *b0x = nullptr, *b0y = nullptr, *b0z = nullptr, *x0 = nullptr, *y0 = nullptr, *z0 = nullptr;
-  unsigned long long *maxes = nullptr;
+  double *maxes = nullptr;  // list check: per-workgroup maxima [2][LIST_CHECK_BLOCKS]
+  int check_blocks = 0;     // workgroups of the last list check
   int *type = nullptr, *ichrg = nullptr, *ex_off = nullptr, *ex_num = nullptr, *ex_j = nullptr;
   double *q = nullptr, *mass = nullptr;
   double2 *tab_lj = nullptr, *tab_c = nullptr, *ecor_tab = nullptr;
@@ -93,7 +94,7 @@ struct pmx_ctx {
   std::vector<double> acc_h;
   // pinned host staging of the asynchronous readbacks: list-check maxima,
   // per-bead sums [P*ACC_NUM] followed by the spring energy
-  unsigned long long *mx_pin = nullptr;
+  double *mx_pin = nullptr;
   double *acc_pin = nullptr;
   // springs
   double *prekf = nullptr, *veig = nullptr, *cs = nullptr;
@@ -481,7 +482,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(dalloc(c, &c->x0, N));
   TRY(dalloc(c, &c->y0, N));
   TRY(dalloc(c, &c->z0, N));
-  TRY(dalloc(c, &c->maxes, 2));
+  TRY(dalloc(c, &c->maxes, 2 * LIST_CHECK_BLOCKS));
   TRY(upload(c, &c->type, sys->atm_type, N));
   TRY(upload(c, &c->q, sys->q, N));
   TRY(upload(c, &c->mass, sys->mass, N));
@@ -663,7 +664,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(dalloc(c, &c->acc, (size_t)P * ACC_NUM));
   TRY(dalloc(c, &c->ake, 1));
   c->acc_h.assign((size_t)P * ACC_NUM, 0.0);
-  HIPCHK(hipHostMalloc((void **)&c->mx_pin, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&c->mx_pin, 2 * LIST_CHECK_BLOCKS * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void **)&c->acc_pin, ((size_t)P * ACC_NUM + 1) * sizeof(double), hipHostMallocDefault));
   TRY(upload(c, &c->prekf, s.prekf.data(), N));
   TRY(upload(c, &c->veig, s.veig.data(), s.veig.size()));
@@ -855,23 +856,22 @@ IntraArgs intra_args(pmx_ctx *c, bool pimd) {
 namespace {
 
 // The list stage of a force call, first half: centroid reference and list
-// check (nbr_list_control.c:32-118) launched on st, with the two maxima
-// copied to pinned host memory (event mxev).  The caller waits for mxev
-// before the second half.
+// check (nbr_list_control.c:32-118) launched on st, with the per-workgroup
+// maxima copied to pinned host memory (event mxev).  The caller waits for
+// mxev before the second half, which takes the maxima over the workgroups.
 int list_stage_launch(pmx_ctx *c, hipStream_t st, bool timing) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
   if (timing) HIPCHK(hipEventRecord(c->ev[0], st));
-  HIPCHK(hipMemsetAsync(c->maxes, 0, 2 * sizeof(unsigned long long), st));
   const int sites_per_wave = P <= 64 ? 64 / P : 1;
   const size_t check_waves = ((size_t)N + sites_per_wave - 1) / sites_per_wave;
-  // at most 512 workgroups, each making one atomic per maximum
-  const unsigned check_blocks = (unsigned)std::min<size_t>(512, blocks(check_waves * 64));
+  const unsigned check_blocks = (unsigned)std::min<size_t>(LIST_CHECK_BLOCKS, blocks(check_waves * 64));
+  c->check_blocks = (int)check_blocks;
   hipLaunchKernelGGL(k_list_check, dim3(check_blocks), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->x0,
                      c->y0, c->z0,
                      c->b0x, c->b0y, c->b0z, c->maxes, (P > 1 && !c->xmod_given) ? 1 : 0, c->xm, c->ym, c->zm);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(c->mx_pin, c->maxes, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(c->mx_pin, c->maxes, 2 * check_blocks * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(c->mxev, st));
   return PMX_OK;
 }
@@ -883,9 +883,11 @@ int list_stage_finish(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
   hipStream_t st = c->st;
-  double dmax, smax;
-  memcpy(&dmax, &c->mx_pin[0], sizeof(double));
-  memcpy(&smax, &c->mx_pin[1], sizeof(double));
+  double dmax = 0.0, smax = 0.0;
+  for (int k = 0; k < c->check_blocks; k++) {
+    dmax = std::max(dmax, c->mx_pin[k]);
+    smax = std::max(smax, c->mx_pin[c->check_blocks + k]);
+  }
   const double spread_now = (P > 1) ? sqrt(smax) : 0.0;
   if (P > 1) {
     for (int t = 0; t < s.npair; t++) {

@@ -238,7 +238,8 @@ __global__ void k_site_umax64(int N, int P, const double *x, const double *y, co
                               const double *ym, const double *zm, double *umax);
 __global__ void k_list_check(int N, int P, const double *x, const double *y, const double *z, const double *x0,
                              const double *y0, const double *z0, double *b0x, double *b0y, double *b0z,
-                             unsigned long long *maxes, int centroid, double *xm, double *ym, double *zm);
+                             double *maxes, int centroid, double *xm, double *ym, double *zm);
+constexpr int LIST_CHECK_BLOCKS = 512;  // workgroups of k_list_check, one pair of maxima each
 __global__ void k_list_build(ListArgs a);
 __global__ void k_cell_count(ListArgs a);
 __global__ void k_cell_scan(int ncell, const int *cnt, int *start);

@@ -131,8 +131,8 @@ __global__ void k_site_umax(int N, int P, const float4 *__restrict__ u, double *
 // (check_ver_list, nbr_list_control.c:182-239), bead spread
 // (get_pimd_spread, pimd_estimators.c:121-222) and, when asked, the bead
 // centroid as xmod, the mode-1 coordinate of the normal-mode transform
-// (control_pimd_trans.c:175-181 takes xmod from mode 1).  Maxima of squares are
-// reduced as the bit patterns of non-negative doubles.  A group of P lanes
+// (control_pimd_trans.c:175-181 takes xmod from mode 1).  Each workgroup
+// writes its maxima of the squares; a group of P lanes
 // holds one site's beads (64 / P sites per wavefront; with P > 64 one site
 // per wavefront in chunks of 64 beads), so the loads are coalesced; the
 // centroid sums run bead 0, 1, ... in order through lane shuffles.
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
                                                     const double *__restrict__ y, const double *__restrict__ z,
                                                     const double *__restrict__ x0, const double *__restrict__ y0,
                                                     const double *__restrict__ z0, double *b0x, double *b0y,
-                                                    double *b0z, unsigned long long *maxes, int centroid,
+                                                    double *b0z, double *maxes, int centroid,
                                                     double *xm, double *ym, double *zm) {
 #pragma clang fp contract(off)
   __shared__ double red[2][4];
@@ -210,13 +210,13 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
     red[1][w] = smax;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0) {  // the host takes the maxima over the workgroups
     for (int v = 1; v < nw; v++) {
       dmax = fmax(dmax, red[0][v]);
       smax = fmax(smax, red[1][v]);
     }
-    atomicMax(&maxes[0], (unsigned long long)__double_as_longlong(dmax));
-    atomicMax(&maxes[1], (unsigned long long)__double_as_longlong(smax));
+    maxes[blockIdx.x] = dmax;
+    maxes[gridDim.x + blockIdx.x] = smax;
   }
 }
 
