@@ -594,6 +594,18 @@ __device__ __forceinline__ void dft_small(typename Cx<T>::C (&v)[R], int sign) {
   }
 }
 
+// k / d for 0 <= k, k * d < 2^32 as a multiply-high by m = floor(2^32 / d) + 1,
+// m formed once per divisor (2^32 / d is at least 1/d from an integer unless
+// exact, so its double quotient floors exactly); d = 1, whose m does not fit
+// 32 bits, is taken apart.
+struct FastDiv {
+  unsigned m;
+  int d;
+  __device__ explicit FastDiv(int d_)
+      : m(d_ > 1 ? (unsigned)(unsigned long long)(4294967296.0 / (double)d_) + 1u : 0u), d(d_) {}
+  __device__ int q(int k) const { return d > 1 ? (int)__umulhi((unsigned)k, m) : k; }
+};
+
 template <int R, typename T, int NT, int MAXP>
 __device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int stride, int dist, int Ns, int sign,
                               const typename Cx<T>::C *__restrict__ tw) {
@@ -602,13 +614,20 @@ __device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int str
   const int tid = threadIdx.x, nt = blockDim.x;
   const int KR = K / R, nb = nlines * KR;
   const bool lines_fast = stride != 1;
+  // butterfly bi -> (line l, index j): lines run fastest when stride > 1
+  const FastDiv dl(lines_fast ? nlines : KR), dn(Ns);
+  auto line_index = [&](int bi, int &l, int &j) {
+    const int qt = dl.q(bi), rm = bi - qt * dl.d;
+    l = lines_fast ? rm : qt;
+    j = lines_fast ? qt : rm;
+  };
   C v[MAXB][R];
 #pragma unroll
   for (int q = 0; q < MAXB; q++) {
     const int bi = tid + q * nt;
     if (bi < nb) {
-      const int l = lines_fast ? bi % nlines : bi / KR;
-      const int j = lines_fast ? bi / nlines : bi % KR;
+      int l, j;
+      line_index(bi, l, j);
       const C *p = buf + (size_t)l * dist;
 #pragma unroll
       for (int k = 0; k < R; k++) v[q][k] = p[(j + k * KR) * stride];
@@ -620,9 +639,9 @@ __device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int str
   for (int q = 0; q < MAXB; q++) {
     const int bi = tid + q * nt;
     if (bi < nb) {
-      const int l = lines_fast ? bi % nlines : bi / KR;
-      const int j = lines_fast ? bi / nlines : bi % KR;
-      const int jm = j % Ns;
+      int l, j;
+      line_index(bi, l, j);
+      const int jq = dn.q(j), jm = j - jq * Ns;
 #pragma unroll
       for (int k = 1; k < R; k++) {  // twiddle exp(sign 2 pi i jm k / (Ns R))
         const C t = tw[jm * k * tstep];  // (cos, -sin) of 2 pi m / K
@@ -632,7 +651,7 @@ __device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int str
       }
       dft_small<R, T>(v[q], sign);
       C *p = buf + (size_t)l * dist;
-      const int base = (j / Ns) * Ns * R + jm;
+      const int base = jq * Ns * R + jm;
 #pragma unroll
       for (int k = 0; k < R; k++) p[(base + k * Ns) * stride] = v[q][k];
     }
