@@ -246,6 +246,8 @@ def main():
     ap.add_argument("--beads", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--paths", default="fp64,fp64_direct,mixed")
+    ap.add_argument("--no-per-bead", action="store_true",
+                    help="skip the one-bead-per-call FP64 baseline (kernel profiles of the batched paths)")
     ap.add_argument("--batch", action="store_true",
                     help="the uneven-box workload (64 boxes, batched graph launch vs sequential calls)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -278,7 +280,8 @@ def main():
           "mixed": {"precision": abi.PMX_PREC_MIXED}}
     paths = [p for p in args.paths.split(",") if p]
     res = {p: run_path(cfg, args.steps, args.warmup, **kw[p]) for p in paths}
-    per_bead = run_per_bead(cfg, max(2, args.steps // 4), 1) if "fp64" in paths else None
+    per_bead = (run_per_bead(cfg, max(2, args.steps // 4), 1)
+                if "fp64" in paths and not args.no_per_bead else None)
     head = res["mixed"] if "mixed" in res else res[paths[-1]]
     head_name = "mixed" if "mixed" in res else paths[-1]
     ref = res.get("fp64")

@@ -64,10 +64,9 @@ struct pmx_ctx {
   int nghost = 0, *gatom = nullptr, *gncomp = nullptr, *gcomp = nullptr;
   // molecules of k_intra_mol (nmol = 0: the topology does not split into
   // molecules of at most MOL_MAX consecutive sites; per-term kernels then)
-  int nmol = 0;
-  int *mol_lo = nullptr, *mol_n = nullptr, *mol_ecor_off = nullptr, *mol_ecor = nullptr, *mol_bond_off = nullptr,
-      *mol_bond = nullptr, *mol_bend_off = nullptr, *mol_bend = nullptr, *mol_ghost_off = nullptr,
-      *mol_ghost = nullptr;
+  int nmol = 0, mol_max = 0, ntpl = 0;
+  int2 *mol_dev = nullptr;     // [nmol] (first site, template)
+  MolTpl *tpl_dev = nullptr;   // [TPL_MAX]
   double *gcoef = nullptr;
   // PME
   int pme = 0, npass[3] = {0, 0, 0}, radix[3][FFT_MAX_PASSES] = {};
@@ -377,9 +376,13 @@ std::vector<double> pair_constants(pmx_ctx *c) {
 
 // Molecules for k_intra_mol: the connected components of the bonds, bends,
 // excluded pairs and ghost constructions.  Used when every component is a
-// run of at most MOL_MAX consecutive sites (water and other small
-// molecules); each term is listed under the molecule of its sites.
+// run of at most MOL_MAX consecutive sites and the components follow at most
+// TPL_MAX templates (terms by local site index, charge products, ghost
+// coefficients), as water and other small molecules do; otherwise nmol
+// stays 0 and the per-term kernels run.  Called again when the charges
+// change (the templates hold charge products).
 int setup_molecules(pmx_ctx *c, const pmx_system *sys) {
+  c->nmol = 0;
   if (const char *env = getenv("PMX_INTRA"))  // tuning override: "terms" keeps the per-term kernels
     if (!strcmp(env, "terms")) return PMX_OK;
   const Setup &s = c->s;
@@ -403,37 +406,87 @@ int setup_molecules(pmx_ctx *c, const pmx_system *sys) {
   for (size_t k = 0; k < s.e1.size(); k++) unite(s.e1[k], s.e2[k]);
   for (int g = 0; g < sys->nghost; g++)
     for (int k = 0; k < sys->ghost_ncomp[g]; k++) unite(sys->ghost_atom[g], sys->ghost_comp[4 * g + k]);
-  // components in order of their first site; each must be a run of sites
-  std::vector<int> mol(N), lo, n;
+  // components in order of their first site (the root); each must be a run
+  std::vector<int> mol(N), lo;
+  std::vector<MolTpl> mt;
   for (int i = 0; i < N; i++) {
     const int r = find(i);
     if (r == i) {
       mol[i] = (int)lo.size();
       lo.push_back(i);
-      n.push_back(1);
+      MolTpl t;
+      memset(&t, 0, sizeof t);
+      t.n = 1;
+      mt.push_back(t);
     } else {
-      mol[i] = mol[r];
-      if (i != lo[mol[i]] + n[mol[i]]) return PMX_OK;  // not consecutive
-      if (++n[mol[i]] > MOL_MAX) return PMX_OK;
+      const int m = mol[i] = mol[r];
+      if (i != lo[m] + mt[m].n || mt[m].n == MOL_MAX) return PMX_OK;  // not consecutive, or too large
+      mt[m].n++;
     }
   }
+  // the terms of each molecule, in term order, by local index
+  for (size_t k = 0; k < s.e1.size(); k++) {
+    const int i = s.e1[k], j = s.e2[k], m = mol[i];
+    MolTpl &t = mt[m];
+    if (t.ne == TPL_MAXT) return PMX_OK;
+    t.e[t.ne][0] = i - lo[m];
+    t.e[t.ne][1] = j - lo[m];
+    t.eq[t.ne++] = sys->q[i] * sys->q[j];
+  }
+  for (int k = 0; k < sys->nbond; k++) {
+    const int i = sys->bond_j1[k], m = mol[i];
+    MolTpl &t = mt[m];
+    if (t.nb == TPL_MAXT) return PMX_OK;
+    t.bd[t.nb][0] = i - lo[m];
+    t.bd[t.nb][1] = sys->bond_j2[k] - lo[m];
+    t.bd[t.nb++][2] = sys->bond_type[k];
+  }
+  for (int k = 0; k < sys->nbend; k++) {
+    const int m = mol[sys->bend_j2[k]];
+    MolTpl &t = mt[m];
+    if (t.na == TPL_MAXT) return PMX_OK;
+    t.an[t.na][0] = sys->bend_j1[k] - lo[m];
+    t.an[t.na][1] = sys->bend_j2[k] - lo[m];
+    t.an[t.na][2] = sys->bend_j3[k] - lo[m];
+    t.an[t.na++][3] = sys->bend_type[k];
+  }
+  for (int g = 0; g < sys->nghost; g++) {
+    const int m = mol[sys->ghost_atom[g]];
+    MolTpl &t = mt[m];
+    if (t.ng == MOL_MAX) return PMX_OK;
+    t.gh[t.ng][0] = sys->ghost_atom[g] - lo[m];
+    t.gh[t.ng][1] = sys->ghost_ncomp[g];
+    for (int k = 0; k < sys->ghost_ncomp[g]; k++) {
+      t.gh[t.ng][2 + k] = sys->ghost_comp[4 * g + k] - lo[m];
+      t.gc[t.ng][k] = sys->ghost_coef[4 * g + k];
+    }
+    t.ng++;
+  }
+  // distinct templates (byte-wise; the structs are zero-filled)
   const int M = (int)lo.size();
-  auto csr = [&](int nterm, auto site_of, int **d_off, int **d_list) -> int {
-    std::vector<int> off(M + 1, 0), list(nterm);
-    for (int k = 0; k < nterm; k++) off[mol[site_of(k)] + 1]++;
-    for (int m = 0; m < M; m++) off[m + 1] += off[m];
-    std::vector<int> pos(off.begin(), off.end() - 1);
-    for (int k = 0; k < nterm; k++) list[pos[mol[site_of(k)]]++] = k;
-    int rc = upload(c, d_off, off.data(), off.size());
-    return rc ? rc : upload(c, d_list, list.data(), list.size());
-  };
-  int rc;
-  if ((rc = upload(c, &c->mol_lo, lo.data(), M)) || (rc = upload(c, &c->mol_n, n.data(), M)) ||
-      (rc = csr((int)s.e1.size(), [&](int k) { return s.e1[k]; }, &c->mol_ecor_off, &c->mol_ecor)) ||
-      (rc = csr(sys->nbond, [&](int k) { return sys->bond_j1[k]; }, &c->mol_bond_off, &c->mol_bond)) ||
-      (rc = csr(sys->nbend, [&](int k) { return sys->bend_j2[k]; }, &c->mol_bend_off, &c->mol_bend)) ||
-      (rc = csr(sys->nghost, [&](int k) { return sys->ghost_atom[k]; }, &c->mol_ghost_off, &c->mol_ghost)))
-    return rc;
+  std::vector<MolTpl> tpl;
+  std::vector<int2> md(M);
+  int mmax = 0;
+  for (int m = 0; m < M; m++) {
+    int id = -1;
+    for (size_t u = 0; u < tpl.size() && id < 0; u++)
+      if (!memcmp(&tpl[u], &mt[m], sizeof(MolTpl))) id = (int)u;
+    if (id < 0) {
+      if ((int)tpl.size() == TPL_MAX) return PMX_OK;
+      id = (int)tpl.size();
+      tpl.push_back(mt[m]);
+    }
+    md[m] = make_int2(lo[m], id);
+    mmax = std::max(mmax, mt[m].n);
+  }
+  if (!c->mol_dev) {
+    int rc;
+    if ((rc = dalloc(c, &c->mol_dev, (size_t)N)) || (rc = dalloc(c, &c->tpl_dev, (size_t)TPL_MAX))) return rc;
+  }
+  HIPCHK(hipMemcpy(c->mol_dev, md.data(), M * sizeof(int2), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->tpl_dev, tpl.data(), tpl.size() * sizeof(MolTpl), hipMemcpyHostToDevice));
+  c->ntpl = (int)tpl.size();
+  c->mol_max = mmax;
   c->nmol = M;
   return PMX_OK;
 }
@@ -1117,26 +1170,19 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     const unsigned G = igrid((size_t)c->nmol * P);
     ia.necor = nchrg > 0 ? ia.necor : 0;
     ia.nmol = c->nmol;
-    ia.mol_lo = c->mol_lo;
-    ia.mol_n = c->mol_n;
-    ia.mol_ecor_off = c->mol_ecor_off;
-    ia.mol_ecor = c->mol_ecor;
-    ia.mol_bond_off = c->mol_bond_off;
-    ia.mol_bond = c->mol_bond;
-    ia.mol_bend_off = c->mol_bend_off;
-    ia.mol_bend = c->mol_bend;
-    ia.mol_ghost_off = c->mol_ghost_off;
-    ia.mol_ghost = c->mol_ghost;
-    ia.gatom = c->gatom;
-    ia.gncomp = c->gncomp;
-    ia.gcomp = c->gcomp;
-    ia.gcoef = c->gcoef;
+    ia.mol_max = c->mol_max;
+    ia.ntpl = c->ntpl;
+    ia.mol = c->mol_dev;
+    ia.tpl = c->tpl_dev;
     ia.rec = frec;
     ia.part_ecor = c->part + c->off_ecor;
     ia.part_bond = c->part + c->off_bond;
     ia.part_bend = c->part + c->off_bend;
     ia.part_fin = c->part + c->off_fin;
-    hipLaunchKernelGGL(k_intra_mol, dim3(G), dim3(bsP), 0, st, ia);
+    const size_t lds = (size_t)6 * c->mol_max * bsP * sizeof(double);  // at most 96 KB (MOL_MAX 8)
+    if (lds > 65536)
+      HIPCHK(hipFuncSetAttribute((const void *)k_intra_mol, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_intra_mol, dim3(G), dim3(bsP), lds, st, ia);
     HIPCHK(hipGetLastError());
     intra_stage(c->off_ecor, G, ACC_VECOR);
     intra_stage(c->off_bond, G, ACC_VBOND);
@@ -2003,6 +2049,7 @@ int pmx_set_force_field(pmx_ctx *c, const pmx_system *sys) {
   HIPCHK(hipStreamSynchronize(c->st));
   const size_t N = (size_t)s.N;
   HIPCHK(hipMemcpy(c->q, sys->q, N * sizeof(double), hipMemcpyHostToDevice));
+  if (int rc = setup_molecules(c, sys)) return rc;  // the templates hold charge products
   {
     const size_t nt = (size_t)s.npair * s.nsplin;
     std::vector<double2> lj(nt), cc(nt);
@@ -2025,6 +2072,7 @@ int pmx_set_force_field(pmx_ctx *c, const pmx_system *sys) {
     HIPCHK(hipMemcpy(c->as, sys->bend_s, 7 * (size_t)c->nbend_type * sizeof(double), hipMemcpyHostToDevice));
   }
   c->forces_valid = false;
+  c->epoch++;  // captured sequences hold the molecule templates and counts
   c->grad_ready = false;
   return PMX_OK;
 }

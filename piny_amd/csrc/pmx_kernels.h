@@ -168,18 +168,32 @@ struct IntraArgs {
   int nbond_type, nbend_type;
   double *gbond;  // [P][nbond_type][7] sum over bonds of (r - eq)^m
   double *gbend;  // [P][nbend_type][BEND_GRAD_NV]
-  // k_intra_mol: molecules of consecutive sites (first site, size) with the
-  // CSR lists of their terms, the ghosts, the k-space forces to add
-  // (bead-major [3][P][N], or null) and the partial rows it fills
-  int nmol;
-  const int *mol_lo, *mol_n, *mol_ecor_off, *mol_ecor, *mol_bond_off, *mol_bond, *mol_bend_off, *mol_bend,
-      *mol_ghost_off, *mol_ghost;
-  const int *gatom, *gncomp, *gcomp;
-  const double *gcoef;
+  // k_intra_mol: molecules of consecutive sites (first site, template),
+  // their templates, the k-space forces to add (bead-major [3][P][N], or
+  // null) and the partial rows it fills
+  int nmol, mol_max, ntpl;  // molecules, sites of the largest, templates
+  const int2 *mol;
+  const struct MolTpl *tpl;
   const double *rec;
   double *part_ecor, *part_bond, *part_bend, *part_fin;
 };
-constexpr int MOL_MAX = 8;  // sites of a molecule of k_intra_mol
+constexpr int MOL_MAX = 8;   // sites of a molecule of k_intra_mol
+constexpr int TPL_MAXT = 16;  // terms of one kind in a molecule template
+constexpr int TPL_MAX = 4;   // templates of one context
+
+// The intramolecular terms of one kind of molecule, by local site index:
+// excluded pairs (sites, q_i q_j), bonds (sites, type), bends (sites with
+// the apex second, type), ghosts (site, components, their sites and
+// coefficients).
+struct MolTpl {
+  int n, ne, nb, na, ng, pad;
+  int e[TPL_MAXT][2];
+  int bd[TPL_MAXT][3];
+  int an[TPL_MAXT][4];
+  int gh[MOL_MAX][6];
+  double eq[TPL_MAXT];
+  double gc[MOL_MAX][4];
+};
 
 // Bend moments per (bead, bend type): cos^m (m = 0..6), sin, cos sin^(m-1)
 // (m = 2..6): the derivatives of V with respect to c_m and s_m.

@@ -1167,23 +1167,23 @@ __device__ __forceinline__ void add_virial(double (&acc)[INTRA_NV], double e, do
   acc[6] += dy * fz;
 }
 
-// One term of each intramolecular kind for bead b: energy and virial into
-// acc, forces through add(site, fx, fy, fz) (global atomics in the per-term
-// kernels, per-thread LDS rows in k_intra_mol).
+// One term of each intramolecular kind for bead b: separations from
+// sep(i, j, dx, dy, dz) (intra_sep, or positions held in LDS), energy and
+// virial into acc, forces through add(site, fx, fy, fz) (global atomics in
+// the per-term kernels, per-thread LDS rows in k_intra_mol).
 
 // ecor, ecorr.c:28-265 (no switching on the table)
-template <class Add>
-__device__ __forceinline__ void ecor_term(const IntraArgs &a, int k, int b, double (&acc)[INTRA_NV], Add add) {
-  const int i = a.e1[k], j = a.e2[k];
+template <class Sep, class Add>
+__device__ __forceinline__ void ecor_term(const IntraArgs &a, int i, int j, double q12, double (&acc)[INTRA_NV],
+                                          Sep sep, Add add) {
   double dx, dy, dz;
-  intra_sep(a, i, j, b, dx, dy, dz);
+  sep(i, j, dx, dy, dz);
   const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
   int kk = (int)(((r12 - a.e_rmin) * a.e_dri) + 0.5) + 3;
   kk = min(kk, a.nsplin - 2);
   kk = max(kk, 3);
   const double rsp = ((double)(kk - 3)) * a.e_dr + a.e_rmin;
   const double del = (r12 - rsp) * a.e_dri;
-  const double q12 = a.q[i] * a.q[j];
   const double e = spline4(a.ecor_tab, kk, del, false) * q12;
   const double dvt = spline4(a.ecor_tab, kk, del, true) * q12;
   const double fx = dx * dvt, fy = dy * dvt, fz = dz * dvt;
@@ -1193,12 +1193,12 @@ __device__ __forceinline__ void ecor_term(const IntraArgs &a, int k, int b, doub
 }
 
 // bond, bond.c:29-312
-template <class Add>
-__device__ __forceinline__ void bond_term(const IntraArgs &a, int k, int b, double (&acc)[INTRA_NV], Add add) {
-  const int i = a.b1[k], j = a.b2[k], ty = a.btyp[k];
+template <class Sep, class Add>
+__device__ __forceinline__ void bond_term(const IntraArgs &a, int i, int j, int ty, double (&acc)[INTRA_NV], Sep sep,
+                                          Add add) {
   const double *cc = a.bc + 7 * ty;
   double dx, dy, dz;
-  intra_sep(a, i, j, b, dx, dy, dz);
+  sep(i, j, dx, dy, dz);
   const double r12 = sqrt(dx * dx + dy * dy + dz * dz);
   const double rr0 = r12 - a.beq[ty];
   const double e =
@@ -1213,13 +1213,13 @@ __device__ __forceinline__ void bond_term(const IntraArgs &a, int k, int b, doub
 }
 
 // bend, bend.c:28-434
-template <class Add>
-__device__ __forceinline__ void bend_term(const IntraArgs &a, int k, int b, double (&acc)[INTRA_NV], Add add) {
-  const int i1 = a.a1[k], i2 = a.a2[k], i3 = a.a3[k], ty = a.atyp[k];
+template <class Sep, class Add>
+__device__ __forceinline__ void bend_term(const IntraArgs &a, int i1, int i2, int i3, int ty,
+                                          double (&acc)[INTRA_NV], Sep sep, Add add) {
   const double *cc = a.ac + 7 * ty, *ss = a.as + 7 * ty;
   double dx12, dy12, dz12, dx23, dy23, dz23;
-  intra_sep(a, i1, i2, b, dx12, dy12, dz12);
-  intra_sep(a, i3, i2, b, dx23, dy23, dz23);
+  sep(i1, i2, dx12, dy12, dz12);
+  sep(i3, i2, dx23, dy23, dz23);
   const double r122i = 1.0 / (dx12 * dx12 + dy12 * dy12 + dz12 * dz12);
   const double r322i = 1.0 / (dx23 * dx23 + dy23 * dy23 + dz23 * dz23);
   const double rpmag = sqrt(r122i) * sqrt(r322i);
@@ -1258,59 +1258,92 @@ __device__ __forceinline__ void bend_term(const IntraArgs &a, int k, int b, doub
 
 // Per-term kernels (topologies that do not split into small molecules):
 // one thread per (term, bead), forces by global atomics.
-#define PMX_TERM_KERNEL(name, fn, nterm)                                                      \
+#define PMX_TERM_KERNEL(name, nterm, CALL)                                                    \
   __global__ void __launch_bounds__(256) name(IntraArgs a) {                                   \
     double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};                                              \
     PMX_TERM_LOOP(nterm) {                                                                     \
-      const int b = (int)(t % a.P);                                                            \
-      fn(a, (int)(t / a.P), b, acc,                                                            \
-         [&](int i, double fx, double fy, double fz) { add_force(a, i, b, fx, fy, fz); });     \
+      const int k = (int)(t / a.P), b = (int)(t % a.P);                                        \
+      auto sep = [&](int i, int j, double &dx, double &dy, double &dz) { intra_sep(a, i, j, b, dx, dy, dz); }; \
+      auto add = [&](int i, double fx, double fy, double fz) { add_force(a, i, b, fx, fy, fz); }; \
+      CALL;                                                                                    \
     }                                                                                          \
     block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);                 \
   }
-PMX_TERM_KERNEL(k_ecor, ecor_term, a.necor)
-PMX_TERM_KERNEL(k_bond, bond_term, a.nbond)
-PMX_TERM_KERNEL(k_bend, bend_term, a.nbend)
+PMX_TERM_KERNEL(k_ecor, a.necor, ecor_term(a, a.e1[k], a.e2[k], a.q[a.e1[k]] * a.q[a.e2[k]], acc, sep, add))
+PMX_TERM_KERNEL(k_bond, a.nbond, bond_term(a, a.b1[k], a.b2[k], a.btyp[k], acc, sep, add))
+PMX_TERM_KERNEL(k_bend, a.nbend, bend_term(a, a.a1[k], a.a2[k], a.a3[k], a.atyp[k], acc, sep, add))
 #undef PMX_TERM_KERNEL
 
 // Every intramolecular term of a force call in one pass, for topologies
 // whose terms (bonds, bends, excluded pairs, ghosts) split into molecules of
-// at most MOL_MAX consecutive sites: one thread per (molecule, bead) holds
-// the molecule's forces in an LDS row of its own, starting from the pair
-// forces, adds the excluded-pair, bond and bend terms, then does what
+// at most MOL_MAX consecutive sites that follow a few templates (local site
+// indices, types, charge products and ghost coefficients; one for a water
+// box).  The templates sit in LDS, so a thread's only global reads are its
+// molecule's template id, positions and forces, the excluded-pair table
+// and the k-space forces.  One thread per (molecule, bead) holds the
+// molecule's positions and forces in LDS rows of its own (the forces
+// starting from the pair forces; 6 * mol_max rows of blockDim doubles,
+// dynamic LDS), adds the excluded-pair, bond and bend terms, then does what
 // k_pimd_finish (or k_add_rec) and k_ghost_force do for these sites, in the
 // same order, and writes the forces once: no atomics.  Energies and virials
 // go to the partial rows of the ecor, bond, bend and finish stages.
 __global__ void __launch_bounds__(256) k_intra_mol(IntraArgs a) {
-  __shared__ double fl[MOL_MAX * 3][256];
+  extern __shared__ double lds_mol[];
+  __shared__ MolTpl tpl[TPL_MAX];
+  const int tid = threadIdx.x, bs = blockDim.x, R = 3 * a.mol_max;
+  {
+    const int *src = (const int *)a.tpl;
+    int *dst = (int *)tpl;
+    const int nw = a.ntpl * (int)(sizeof(MolTpl) / sizeof(int));
+    for (int k = tid; k < nw; k += bs) dst[k] = src[k];
+  }
+  __syncthreads();
+  double *fl = lds_mol, *pl = lds_mol + (size_t)R * bs;  // [row][thread]
   double ae[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0}, ab[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
   double aa[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};
   double cf[FIN_NV] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  const int tid = threadIdx.x;
   const size_t NP = (size_t)a.N * a.P;
   const double dP = (double)a.P;
   PMX_TERM_LOOP(a.nmol) {
     const int m = (int)(t / a.P), b = (int)(t % a.P);
-    const int lo = a.mol_lo[m], n = a.mol_n[m];
+    const int2 mt = a.mol[m];  // (first site, template)
+    const int lo = mt.x;
+    const MolTpl &T = tpl[mt.y];
+    const int n = T.n;
     for (int s = 0; s < n; s++) {
       const size_t ib = (size_t)(lo + s) * a.P + b;
-      fl[3 * s][tid] = a.fx[ib];
-      fl[3 * s + 1][tid] = a.fy[ib];
-      fl[3 * s + 2][tid] = a.fz[ib];
+      fl[(3 * s) * bs + tid] = a.fx[ib];
+      fl[(3 * s + 1) * bs + tid] = a.fy[ib];
+      fl[(3 * s + 2) * bs + tid] = a.fz[ib];
+      pl[(3 * s) * bs + tid] = a.x[ib];
+      pl[(3 * s + 1) * bs + tid] = a.y[ib];
+      pl[(3 * s + 2) * bs + tid] = a.z[ib];
     }
+    // terms address sites by global index (lo + local), as the per-term kernels
     auto add = [&](int i, double fx, double fy, double fz) {
       const int s = i - lo;
-      fl[3 * s][tid] += fx;
-      fl[3 * s + 1][tid] += fy;
-      fl[3 * s + 2][tid] += fz;
+      fl[(3 * s) * bs + tid] += fx;
+      fl[(3 * s + 1) * bs + tid] += fy;
+      fl[(3 * s + 2) * bs + tid] += fz;
+    };
+    auto sep = [&](int i, int j, double &dx, double &dy, double &dz) {
+      if (a.intra_perds) {
+        intra_sep(a, i, j, b, dx, dy, dz);
+        return;
+      }
+      const int si = i - lo, sj = j - lo;
+      dx = pl[(3 * si) * bs + tid] - pl[(3 * sj) * bs + tid];
+      dy = pl[(3 * si + 1) * bs + tid] - pl[(3 * sj + 1) * bs + tid];
+      dz = pl[(3 * si + 2) * bs + tid] - pl[(3 * sj + 2) * bs + tid];
     };
     if (a.necor > 0)
-      for (int q = a.mol_ecor_off[m]; q < a.mol_ecor_off[m + 1]; q++) ecor_term(a, a.mol_ecor[q], b, ae, add);
-    for (int q = a.mol_bond_off[m]; q < a.mol_bond_off[m + 1]; q++) bond_term(a, a.mol_bond[q], b, ab, add);
-    for (int q = a.mol_bend_off[m]; q < a.mol_bend_off[m + 1]; q++) bend_term(a, a.mol_bend[q], b, aa, add);
+      for (int q = 0; q < T.ne; q++) ecor_term(a, lo + T.e[q][0], lo + T.e[q][1], T.eq[q], ae, sep, add);
+    for (int q = 0; q < T.nb; q++) bond_term(a, lo + T.bd[q][0], lo + T.bd[q][1], T.bd[q][2], ab, sep, add);
+    for (int q = 0; q < T.na; q++)
+      bend_term(a, lo + T.an[q][0], lo + T.an[q][1], lo + T.an[q][2], T.an[q][3], aa, sep, add);
     for (int s = 0; s < n; s++) {
       const int i = lo + s;
-      double fx = fl[3 * s][tid], fy = fl[3 * s + 1][tid], fz = fl[3 * s + 2][tid];
+      double fx = fl[(3 * s) * bs + tid], fy = fl[(3 * s + 1) * bs + tid], fz = fl[(3 * s + 2) * bs + tid];
       if (a.rec) {
         const size_t r = (size_t)b * a.N + i;
         fx += a.rec[r];
@@ -1322,8 +1355,8 @@ __global__ void __launch_bounds__(256) k_intra_mol(IntraArgs a) {
         fy /= dP;
         fz /= dP;
         if (a.P > 1) {
-          const size_t ib = (size_t)i * a.P + b;
-          const double dx = a.x[ib] - a.xm[i], dy = a.y[ib] - a.ym[i], dz = a.z[ib] - a.zm[i];
+          const double dx = pl[(3 * s) * bs + tid] - a.xm[i], dy = pl[(3 * s + 1) * bs + tid] - a.ym[i],
+                       dz = pl[(3 * s + 2) * bs + tid] - a.zm[i];
           cf[0] -= dx * fx;
           cf[1] -= dy * fy;
           cf[2] -= dz * fz;
@@ -1336,24 +1369,26 @@ __global__ void __launch_bounds__(256) k_intra_mol(IntraArgs a) {
           cf[9] -= dx * fx + dy * fy + dz * fz;
         }
       }
-      fl[3 * s][tid] = fx;
-      fl[3 * s + 1][tid] = fy;
-      fl[3 * s + 2][tid] = fz;
+      fl[(3 * s) * bs + tid] = fx;
+      fl[(3 * s + 1) * bs + tid] = fy;
+      fl[(3 * s + 2) * bs + tid] = fz;
     }
-    for (int q = a.mol_ghost_off[m]; q < a.mol_ghost_off[m + 1]; q++) {  // distrib_ghost_force
-      const int g = a.mol_ghost[q], sg = a.gatom[g] - lo;
-      const double gx = fl[3 * sg][tid], gy = fl[3 * sg + 1][tid], gz = fl[3 * sg + 2][tid];
-      for (int k = 0; k < a.gncomp[g]; k++) add(a.gcomp[4 * g + k], a.gcoef[4 * g + k] * gx,
-                                                a.gcoef[4 * g + k] * gy, a.gcoef[4 * g + k] * gz);
-      fl[3 * sg][tid] = 0.0;
-      fl[3 * sg + 1][tid] = 0.0;
-      fl[3 * sg + 2][tid] = 0.0;
+    for (int q = 0; q < T.ng; q++) {  // distrib_ghost_force
+      const int sg = T.gh[q][0];
+      const double gx = fl[(3 * sg) * bs + tid], gy = fl[(3 * sg + 1) * bs + tid], gz = fl[(3 * sg + 2) * bs + tid];
+      for (int k = 0; k < T.gh[q][1]; k++) {
+        const double cg = T.gc[q][k];
+        add(lo + T.gh[q][2 + k], cg * gx, cg * gy, cg * gz);
+      }
+      fl[(3 * sg) * bs + tid] = 0.0;
+      fl[(3 * sg + 1) * bs + tid] = 0.0;
+      fl[(3 * sg + 2) * bs + tid] = 0.0;
     }
     for (int s = 0; s < n; s++) {
       const size_t ib = (size_t)(lo + s) * a.P + b;
-      a.fx[ib] = fl[3 * s][tid];
-      a.fy[ib] = fl[3 * s + 1][tid];
-      a.fz[ib] = fl[3 * s + 2][tid];
+      a.fx[ib] = fl[(3 * s) * bs + tid];
+      a.fy[ib] = fl[(3 * s + 1) * bs + tid];
+      a.fz[ib] = fl[(3 * s + 2) * bs + tid];
     }
   }
   block_partials<INTRA_NV>(ae, a.P, 0, a.P, a.part_ecor, blockIdx.x, gridDim.x);
