@@ -751,6 +751,53 @@ __global__ void __launch_bounds__(256) k_pair_half(PairArgs a) {
   block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
 }
 
+// The FP32 pair term of pair_direct with the type pair's constants in
+// registers: A = (cut^2, cut, C12, C6), B = (LJ shift, Coulomb shift, LJ
+// flag, 0), read from LDS as two 16-byte loads per pair.
+__device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4 &A, const float4 &B, float q2,
+                                                 float r, float rinv, float &vlj, float &vc) {
+  float sw = 1.f, dsw = 0.f;
+  if (a.swit) {
+    const float rheal = (float)a.rheal, irheal = (float)a.irheal;
+    const float rs = r - A.y + rheal;
+    if (rs > 0.f) {  // inside the switching shell (below it sw = 1, dsw = 0)
+      float rp = rs * irheal;
+      rp = rp < 1.f ? rp : 1.f;
+      sw = 1.f + rp * rp * (2.f * rp - 3.f);
+      dsw = -6.f * (rp * irheal) * (rp - 1.f) * rinv;
+    }
+  }
+  const float r2i = rinv * rinv;
+  float dvt = 0.f;
+  vlj = 0.f;
+  vc = 0.f;
+  if (B.z != 0.f) {  // r^-6 (C12 r^-6 - C6) and 6 r^-6 (2 C12 r^-6 - C6) / r^2
+    const float r6i = r2i * r2i * r2i, t12 = A.z * r6i;
+    const float v = r6i * (t12 - A.w);
+    const float dv = (6.f * r6i) * (2.f * t12 - A.w) * r2i;
+    vlj = v * sw - B.x;
+    dvt = dv * sw + v * dsw;
+  }
+  if (q2 != 0.f) {
+    // PINY's erfc polynomial; -d erfc(alp r)/dr = 2 alp / sqrt(pi)
+    // exp(-alp^2 r^2) (the polynomial's own derivative differs by at most
+    // 1.1e-7 relative, 5e-9 in the force term, below FP32 rounding)
+    const float ralp = r * (float)a.alp;
+    const float eee = exp_r<float>(-ralp * ralp);
+    const float tt = rcp_r<float>(1.f + (float)kErfP * ralp);
+    float g = (float)kErfE[9];
+#pragma unroll
+    for (int k = 8; k >= 1; k--) g = g * tt + (float)kErfE[k];
+    g = g * tt * eee;
+    const float dg = (float)a.spalp * eee;
+    const float v = g * rinv;
+    const float dv = (g * r2i + dg * rinv) * rinv;
+    vc = q2 * (v * sw - B.y);
+    dvt += q2 * (dv * sw + v * dsw);
+  }
+  return dvt;
+}
+
 // Orders the LDS writes of a wavefront before its other lanes read them
 // (and the reads before the next writes).
 __device__ __forceinline__ void wave_sync() {
@@ -771,10 +818,14 @@ __device__ __forceinline__ void wave_sync() {
 // across chunks and lanes.
 template <int B>
 __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
-  __shared__ float spp[MAX_PAIRS * PP_NUM];
+  __shared__ float4 sppA[MAX_PAIRS], sppB[MAX_PAIRS];  // pair_direct_f32's constants per type pair
   __shared__ float4 sdq[4][64];
-  __shared__ int2 sent[4][64];  // (j * P, type pair * PP_NUM) of the staged entries
-  for (int k = threadIdx.x; k < a.npair * PP_NUM; k += blockDim.x) spp[k] = (float)a.pp[k];
+  __shared__ int2 sent[4][64];  // (j * P, type pair) of the staged entries
+  for (int t = threadIdx.x; t < a.npair; t += blockDim.x) {
+    const double *pp = a.pp + (size_t)t * PP_NUM;
+    sppA[t] = make_float4((float)pp[PP_CUT2], (float)pp[PP_CUT], (float)pp[PP_C12], (float)pp[PP_C6]);
+    sppB[t] = make_float4((float)pp[PP_VSLJ], (float)pp[PP_VSC], (float)pp[PP_LJ], 0.f);
+  }
   __syncthreads();
   constexpr int S = 64 / B;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -816,7 +867,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
                         dy, dz);
         }
         const double lim =
-            ((double)spp[(e >> ENTRY_SHIFT) * PP_NUM + PP_CUT] + (a.pimd ? a.umax[i] + a.umax[j] : 0.0)) * (1.0 + 1e-5);
+            ((double)sppA[e >> ENTRY_SHIFT].y + (a.pimd ? a.umax[i] + a.umax[j] : 0.0)) * (1.0 + 1e-5);
         keep = dx * dx + dy * dy + dz * dz < lim * lim;
         q2 = si.w * sj.w;
       }
@@ -824,7 +875,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
       if (keep) {
         const int pos = __popcll(mk & below);
         sdq[w][pos] = make_float4((float)dx, (float)dy, (float)dz, (float)q2);
-        sent[w][pos] = make_int2((e & ENTRY_MASK) * P, (e >> ENTRY_SHIFT) * PP_NUM);
+        sent[w][pos] = make_int2((e & ENTRY_MASK) * P, e >> ENTRY_SHIFT);
       }
       wave_sync();
       const int nk = __popcll(mk);
@@ -837,7 +888,8 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
       if (a.pimd && slot < nk) un = a.u[sent[w][slot].x + bsel];
       for (int p = slot; p < nk; p += S) {
         const float4 d = sdq[w][p];
-        const float *pp = spp + sent[w][p].y;
+        const int t = sent[w][p].y;
+        const float4 A = sppA[t], B2 = sppB[t];
         float dx = d.x, dy = d.y, dz = d.z;
         if (a.pimd) {
           const float4 uj = un;
@@ -847,10 +899,10 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           dz += ui.z - uj.z;
         }
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!bact || !(r2 < pp[PP_CUT2])) continue;
+        if (!bact || !(r2 < A.x)) continue;
         const float rinv = __frsqrt_rn(r2);
         float ev_lj, ev_c;
-        const float dvd = pair_direct<float, float>(a, pp, d.w, r2 * rinv, rinv, ev_lj, ev_c);
+        const float dvd = pair_direct_f32(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c);
         const float fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
         cvlj += ev_lj;
         cvc += ev_c;
