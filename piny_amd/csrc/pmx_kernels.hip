@@ -883,27 +883,20 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
       float cfx = 0.f, cfy = 0.f, cfz = 0.f;
       int cn = 0;
       const int bsel = bact ? bead : 0;
-      // the displacements of the next two pairs are in flight while this
-      // pair is evaluated (a third raises the kernel above 128 VGPRs, to 3
-      // wavefronts per SIMD: 0.67 instead of 0.44 ms)
-      float4 un = make_float4(0.f, 0.f, 0.f, 0.f), un2 = un;
-      if (a.pimd && slot < nk) un = a.u[sent[w][slot].x + bsel];
-      if (a.pimd && slot + S < nk) un2 = a.u[sent[w][slot + S].x + bsel];
-      for (int p = slot; p < nk; p += S) {
+      // one pair of the chunk for this lane's bead, with the displacement uj
+      // of site j
+      auto pair = [&](int p, const float4 &uj) {
         const float4 d = sdq[w][p];
         const int t = sent[w][p].y;
         const float4 A = sppA[t], B2 = sppB[t];
         float dx = d.x, dy = d.y, dz = d.z;
         if (a.pimd) {
-          const float4 uj = un;
-          un = un2;
-          if (p + 2 * S < nk) un2 = a.u[sent[w][p + 2 * S].x + bsel];
           dx += ui.x - uj.x;
           dy += ui.y - uj.y;
           dz += ui.z - uj.z;
         }
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!bact || !(r2 < A.x)) continue;
+        if (!bact || !(r2 < A.x)) return;
         const float rinv = __frsqrt_rn(r2);
         float ev_lj, ev_c;
         const float dvd = pair_direct_f32(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c);
@@ -920,7 +913,30 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         c13 += dx * fz1;
         c23 += dy * fz1;
         cn++;
+      };
+      auto disp = [&](int p) {
+        return (a.pimd && p < nk) ? a.u[sent[w][p].x + bsel] : make_float4(0.f, 0.f, 0.f, 0.f);
+      };
+      // four pairs per iteration, each displacement loaded a whole iteration
+      // ahead into a register of its own: no register copies, which would
+      // wait for the load (one pair ahead through copies: 0.46 ms; two: 0.42
+      // ms, and three raised the kernel above 128 VGPRs, to 3 wavefronts per
+      // SIMD: 0.67 ms; four ahead, unrolled: 0.41 ms at 120 VGPRs)
+      float4 uA = disp(slot), uB = disp(slot + S), uC = disp(slot + 2 * S), uD = disp(slot + 3 * S);
+      int p = slot;
+      for (; p + 3 * S < nk; p += 4 * S) {
+        pair(p, uA);
+        uA = disp(p + 4 * S);
+        pair(p + S, uB);
+        uB = disp(p + 5 * S);
+        pair(p + 2 * S, uC);
+        uC = disp(p + 6 * S);
+        pair(p + 3 * S, uD);
+        uD = disp(p + 7 * S);
       }
+      if (p < nk) pair(p, uA);
+      if (p + S < nk) pair(p + S, uB);
+      if (p + 2 * S < nk) pair(p + 2 * S, uC);
       vlj += (double)cvlj;
       vc += (double)cvc;
       p11 += (double)c11;
