@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(256) k_split_rows(ListArgs a, int *out) {
 // owns one site i at a time; its 64 lanes are (slot, bead) with B beads per
 // slot, so the B lanes of a slot evaluate the same pair for B beads and
 // touch contiguous memory.  The i force is reduced over slots with shuffles.
-// With the full list (the default: k_pair_staged, k_pair_mixed) every pair
+// With the full list (the default: k_pair_staged_tab/_dir, k_pair_mixed) every pair
 // is met from both sites: each wavefront stores its own sites' forces and
 // the energies and virial carry a factor 1/2, so there are no atomics.  With
 // the half list (k_pair_half, a tuning switch) the j forces are scattered
@@ -976,7 +976,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
 // evaluate the term with pair_term<MODE> (tables or direct).  Without beads
 // F is the imaged separation itself.
 template <int B, int MODE>
-__global__ void __launch_bounds__(256) k_pair_staged(PairArgs a) {
+__device__ __forceinline__ void pair_staged(PairArgs a) {
   __shared__ double spp[MAX_PAIRS * PP_NUM];
   __shared__ double4 sdf[4][64];
   __shared__ int sent[4][64];
@@ -1040,35 +1040,39 @@ __global__ void __launch_bounds__(256) k_pair_staged(PairArgs a) {
       }
       wave_sync();
       const int nk = __popcll(mk);
-      // the next pair's bead position is loaded while this pair is evaluated
+      // two pairs per iteration, each bead position of site j loaded a whole
+      // iteration ahead into registers of its own (no copies that would
+      // wait for the load)
       const int bsel = bact ? bead : 0;
-      double xn = 0.0, yn = 0.0, zn = 0.0;
-      if (a.pimd && slot < nk) {
-        const size_t jb = (size_t)(sent[w][slot] & ENTRY_MASK) * P + bsel;
-        xn = a.x[jb];
-        yn = a.y[jb];
-        zn = a.z[jb];
-      }
-      for (int p = slot; p < nk; p += S) {
+      auto pos = [&](int p) {
+        double3 r = make_double3(0.0, 0.0, 0.0);
+        if (a.pimd && p < nk) {
+          const size_t jb = (size_t)(sent[w][p] & ENTRY_MASK) * P + bsel;
+          r = make_double3(a.x[jb], a.y[jb], a.z[jb]);
+        }
+        return r;
+      };
+      auto pair = [&](int p, const double3 &xj) {
         const double4 f = sdf[w][p];
-        const int en = sent[w][p];
-        const int t = en >> ENTRY_SHIFT;
+        const int t = sent[w][p] >> ENTRY_SHIFT;
         double rx = f.x, ry = f.y, rz = f.z;
         if (a.pimd) {
-          const double xj = xn, yj = yn, zj = zn;
-          if (p + S < nk) {
-            const size_t jb = (size_t)(sent[w][p + S] & ENTRY_MASK) * P + bsel;
-            xn = a.x[jb];
-            yn = a.y[jb];
-            zn = a.z[jb];
-          }
-          rx = uix + (f.x - xj);
-          ry = uiy + (f.y - yj);
-          rz = uiz + (f.z - zj);
+          rx = uix + (f.x - xj.x);
+          ry = uiy + (f.y - xj.y);
+          rz = uiz + (f.z - xj.z);
         }
         double dvd;
         if (bact) pair_term<MODE>(a, spp + t * PP_NUM, t, f.w, rx, ry, rz, rx * rx + ry * ry + rz * rz, s, dvd);
+      };
+      double3 xA = pos(slot), xB = pos(slot + S);
+      int p = slot;
+      for (; p + S < nk; p += 2 * S) {
+        pair(p, xA);
+        xA = pos(p + 2 * S);
+        pair(p + S, xB);
+        xB = pos(p + 3 * S);
       }
+      if (p < nk) pair(p, xA);
       wave_sync();
     }
     double fxi = s.fx, fyi = s.fy, fzi = s.fz;
@@ -1086,6 +1090,19 @@ __global__ void __launch_bounds__(256) k_pair_staged(PairArgs a) {
   const double vals[PAIR_NV] = {s.vlj * 0.5, s.vc * 0.5, s.p11 * 0.5, s.p22 * 0.5, s.p33 * 0.5,
                                 s.p12 * 0.5, s.p13 * 0.5, s.p23 * 0.5, s.npr * 0.5};
   block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
+}
+
+// The table form holds 4 wavefronts per SIMD (at most 128 VGPRs, no
+// spills: 0.89 instead of 1.08 ms when the compiler chose 129); the direct
+// form, whose polynomial needs more registers, would spill under that
+// bound and runs at what the compiler picks.
+template <int B>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_pair_staged_tab(PairArgs a) {
+  pair_staged<B, 0>(a);
+}
+template <int B>
+__global__ void __launch_bounds__(256) k_pair_staged_dir(PairArgs a) {
+  pair_staged<B, 1>(a);
 }
 
 // The largest |x - xmod| of every site over its beads, in FP64 (one thread
@@ -1764,8 +1781,8 @@ template <int B>
 void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_t st) {
   const dim3 block(256);
   if (full) {
-    if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged<B, 0>), grid, block, 0, st, a);
-    else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged<B, 1>), grid, block, 0, st, a);
+    if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged_tab<B>), grid, block, 0, st, a);
+    else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged_dir<B>), grid, block, 0, st, a);
     else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B>), grid, block, 0, st, a);
   } else {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 0>), grid, block, 0, st, a);
