@@ -186,12 +186,18 @@ int upload(pmx_ctx *c, T **p, const T *src, size_t n) {
   return PMX_OK;
 }
 
-// Radices 4, 2, 3, 5 (in that order of preference) whose product is k; the
+// Radices 8 (FP32 grids only: in FP64 its butterflies need more than 128
+// VGPRs), 4, 2, 3, 5, in that order of preference, whose product is k; the
 // number of passes, or -1 when k has another prime factor.
-int fft_factor(int k, int *radix) {
+int fft_factor(int k, int *radix, bool radix8) {
   int n = 0;
   while (k > 1 && n < FFT_MAX_PASSES) {
-    int r = (k % 4 == 0) ? 4 : (k % 2 == 0) ? 2 : (k % 3 == 0) ? 3 : (k % 5 == 0) ? 5 : 0;
+    int r = (radix8 && k % 8 == 0) ? 8
+            : (k % 4 == 0)         ? 4
+            : (k % 2 == 0)         ? 2
+            : (k % 3 == 0)         ? 3
+            : (k % 5 == 0)         ? 5
+                                   : 0;
     if (r == 0) return -1;
     radix[n++] = r;
     k /= r;
@@ -647,7 +653,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   if (c->pme) {
     const int Ka = s.ngrid[0], Kb = s.ngrid[1], Kc = s.ngrid[2], Ha = Ka / 2 + 1;
     for (int d = 0; d < 3; d++) {
-      c->npass[d] = fft_factor(s.ngrid[d], c->radix[d]);
+      c->npass[d] = fft_factor(s.ngrid[d], c->radix[d], c->pme_f32 != 0);
       if (c->npass[d] < 0 || (s.ngrid[d] & 1)) {
         set_err(c, "PME grid %dx%dx%d: the device FFT takes even sizes with prime factors 2, 3, 5", Ka, Kb, Kc);
         return PMX_ERR_NOTIMPL;

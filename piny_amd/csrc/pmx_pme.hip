@@ -541,8 +541,8 @@ hipError_t launch_pme_gather(const PmeArgs &a, hipStream_t st) {
 // --- FFT --------------------------------------------------------------------------
 
 // Batched 1D complex FFTs held in LDS, cooperative over the workgroup:
-// mixed-radix Stockham passes (radix 2, 3, 4 or 5; the radices of FFTPACK,
-// fft_generic.f:14-84), unnormalised, sign -1 for exp(-i) (DCFFTF) and +1 for
+// mixed-radix Stockham passes (radix 2, 3, 4, 5 — the radices of FFTPACK,
+// fft_generic.f:14-84 — and 8), unnormalised, sign -1 for exp(-i) (DCFFTF) and +1 for
 // exp(+i) (DCFFTB).  Line l element e lives at buf[l*dist + e*stride].  A
 // pass reads every butterfly's inputs into registers, waits, then writes the
 // outputs in place, so no permutation pass is needed.  Lanes run along the
@@ -567,6 +567,25 @@ __device__ __forceinline__ void dft_small(typename Cx<T>::C (&v)[R], int sign) {
     v[2] = Cx<T>::make(a0.x - b0.x, a0.y - b0.y);
     v[1] = Cx<T>::make(a1.x + c1.x, a1.y + c1.y);
     v[3] = Cx<T>::make(a1.x - c1.x, a1.y - c1.y);
+  } else if constexpr (R == 8) {
+    // two 4-point DFTs of the even and odd inputs, combined with the
+    // eighth roots w^k = exp(sign 2 pi i k / 8): X[k] = E[k] + w^k O[k],
+    // X[k + 4] = E[k] - w^k O[k]
+    C e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<4, T>(e, sign);
+    dft_small<4, T>(o, sign);
+    const T h = (T)0.70710678118654752440, sg = (T)sign;
+    const C w1 = Cx<T>::make(h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x));    // (1 + sign i) h
+    const C w2 = Cx<T>::make(-sg * o[2].y, sg * o[2].x);                                // sign i
+    const C w3 = Cx<T>::make(-h * (o[3].x + sg * o[3].y), h * (sg * o[3].x - o[3].y));  // (-1 + sign i) h
+    v[0] = Cx<T>::make(e[0].x + o[0].x, e[0].y + o[0].y);
+    v[4] = Cx<T>::make(e[0].x - o[0].x, e[0].y - o[0].y);
+    v[1] = Cx<T>::make(e[1].x + w1.x, e[1].y + w1.y);
+    v[5] = Cx<T>::make(e[1].x - w1.x, e[1].y - w1.y);
+    v[2] = Cx<T>::make(e[2].x + w2.x, e[2].y + w2.y);
+    v[6] = Cx<T>::make(e[2].x - w2.x, e[2].y - w2.y);
+    v[3] = Cx<T>::make(e[3].x + w3.x, e[3].y + w3.y);
+    v[7] = Cx<T>::make(e[3].x - w3.x, e[3].y - w3.y);
   } else {
     // direct DFT: out[k] = sum_m v[m] exp(sign 2 pi i k m / R)
     constexpr double c3[3] = {1.0, -0.5, -0.5};
@@ -665,6 +684,9 @@ __device__ void lds_fft(typename Cx<T>::C *buf, int K, int nlines, int stride, i
   int Ns = 1;
   for (int s = 0; s < npass; s++) {
     switch (radix[s]) {
+      case 8:  // FP32 only (the host plans radix 8 for FP32 grids)
+        if constexpr (sizeof(T) == 4) stockham_pass<8, T, NT, MAXP>(buf, K, nlines, stride, dist, Ns, sign, tw);
+        break;
       case 2: stockham_pass<2, T, NT, MAXP>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
       case 3: stockham_pass<3, T, NT, MAXP>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
       case 4: stockham_pass<4, T, NT, MAXP>(buf, K, nlines, stride, dist, Ns, sign, tw); break;
