@@ -640,16 +640,23 @@ __device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int str
     l = lines_fast ? rm : qt;
     j = lines_fast ? qt : rm;
   };
+  // element offsets: the k-th input of a butterfly sits k * KR * stride
+  // after its first, the k-th output k * Ns * stride (uniform steps, so the
+  // per-lane index work is one multiply-add per butterfly and side)
+  const int kin = KR * stride, kout = Ns * stride;
   C v[MAXB][R];
+  int lq[MAXB], jqv[MAXB];
 #pragma unroll
   for (int q = 0; q < MAXB; q++) {
     const int bi = tid + q * nt;
     if (bi < nb) {
       int l, j;
       line_index(bi, l, j);
-      const C *p = buf + (size_t)l * dist;
+      lq[q] = l;
+      jqv[q] = j;
+      const C *p = buf + (l * dist + j * stride);
 #pragma unroll
-      for (int k = 0; k < R; k++) v[q][k] = p[(j + k * KR) * stride];
+      for (int k = 0; k < R; k++) v[q][k] = p[k * kin];
     }
   }
   __syncthreads();
@@ -658,21 +665,22 @@ __device__ void stockham_pass(typename Cx<T>::C *buf, int K, int nlines, int str
   for (int q = 0; q < MAXB; q++) {
     const int bi = tid + q * nt;
     if (bi < nb) {
-      int l, j;
-      line_index(bi, l, j);
+      const int l = lq[q], j = jqv[q];
       const int jq = dn.q(j), jm = j - jq * Ns;
+      const int jt = jm * tstep;
+      int ti = 0;
 #pragma unroll
       for (int k = 1; k < R; k++) {  // twiddle exp(sign 2 pi i jm k / (Ns R))
-        const C t = tw[jm * k * tstep];  // (cos, -sin) of 2 pi m / K
+        ti += jt;
+        const C t = tw[ti];  // (cos, -sin) of 2 pi m / K, m = jm k tstep
         const T wr = t.x, wi = sign < 0 ? t.y : -t.y;
         const C a = v[q][k];
         v[q][k] = Cx<T>::make(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
       }
       dft_small<R, T>(v[q], sign);
-      C *p = buf + (size_t)l * dist;
-      const int base = jq * Ns * R + jm;
+      C *p = buf + (l * dist + (jq * Ns * R + jm) * stride);
 #pragma unroll
-      for (int k = 0; k < R; k++) p[(base + k * Ns) * stride] = v[q][k];
+      for (int k = 0; k < R; k++) p[k * kout] = v[q][k];
     }
   }
   __syncthreads();
@@ -799,8 +807,9 @@ __global__ void __launch_bounds__(NT) k_fft_fwd_ab(FftArgs a) {
   lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, -1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
   lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, -1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
   C *sp = (C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
+  const FastDiv dh(Ha);
   for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
-    const int bb = w / Ha, aa = w % Ha;
+    const int bb = dh.q(w), aa = w - bb * Ha;
     sp[w] = sm[bb * Ka + aa];
   }
 }
@@ -818,12 +827,14 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
   if (!xcd_unit_bead(a.K[1], a.P, bb, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
   C *sp = (C *)a.spec + (size_t)b * a.sstride;
+  const FastDiv dh(Ha);
   for (int w0 = threadIdx.x; w0 < Kc * Ha; w0 += 8 * blockDim.x) {
     C v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int w = w0 + u * blockDim.x;
-      if (w < Kc * Ha) v[u] = sp[((size_t)(w / Ha) * Kb + bb) * Ha + (w % Ha)];
+      const int cc = dh.q(w);
+      if (w < Kc * Ha) v[u] = sp[((size_t)cc * Kb + bb) * Ha + (w - cc * Ha)];
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -838,7 +849,7 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
   const int kb = bb <= Kb / 2 ? bb : bb - Kb;
   double e = 0.0, v11 = 0.0, v22 = 0.0, v33 = 0.0, v12 = 0.0, v13 = 0.0, v23 = 0.0;
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
-    const int cc = w / Ha, aa = w % Ha;
+    const int cc = dh.q(w), aa = w - cc * Ha;
     const double W = a.infl[((size_t)cc * Kb + bb) * Ha + aa];
     const C s = sm[w];
     if (W != 0.0) {
@@ -865,7 +876,7 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
   __syncthreads();
   lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, +1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
-    const int cc = w / Ha, aa = w % Ha;
+    const int cc = dh.q(w), aa = w - cc * Ha;
     sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
   }
   // the k-space pressure also carries the energy on its diagonal
@@ -884,12 +895,13 @@ __global__ void __launch_bounds__(NT) k_fft_inv_ba(FftArgs a) {
   if (!xcd_unit_bead(a.K[2], a.P, c, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
   const C *sp = (const C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
+  const FastDiv dka(Ka);
   for (int w0 = threadIdx.x; w0 < Kb * Ka; w0 += 8 * blockDim.x) {
     C v[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const int w = w0 + u * blockDim.x;
-      const int bb = w / Ka, aa = w % Ka;
+      const int bb = dka.q(w), aa = w - bb * Ka;
       v[u] = (w < Kb * Ka && aa < Ha) ? sp[bb * Ha + aa] : Cx<T>::make((T)0, (T)0);
     }
 #pragma unroll
@@ -902,7 +914,7 @@ __global__ void __launch_bounds__(NT) k_fft_inv_ba(FftArgs a) {
   lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, +1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
   // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
   for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
-    const int bb = w / Ka, aa = w % Ka;
+    const int bb = dka.q(w), aa = w - bb * Ka;
     if (aa >= Ha) {
       const C v = sm[bb * Ka + (Ka - aa)];
       sm[w] = Cx<T>::make(v.x, -v.y);
