@@ -48,6 +48,8 @@ struct pmx_ctx {
   double *q = nullptr, *mass = nullptr;
   double2 *tab_lj = nullptr, *tab_c = nullptr, *ecor_tab = nullptr;
   double *pair_pp = nullptr;   // [npair][PP_NUM]
+  float4 *ctab = nullptr;      // mixed precision: the Coulomb table [CT_N]
+  float ct_r0 = 0.f, ct_idr = 0.f;
   double4 *site = nullptr;     // [N] (xmod, q)
   float4 *disp = nullptr;      // mixed precision with beads: x - xmod in FP32 [N*P]
   double *umax = nullptr;      // the largest |x - xmod| of every site over its beads [N]
@@ -497,6 +499,49 @@ int setup_molecules(pmx_ctx *c, const pmx_system *sys) {
   return PMX_OK;
 }
 
+// The mixed-precision Coulomb table: erfc(alp r)/r with PINY's polynomial
+// (spline_fit.c:327-339) and its r derivative, in FP64, at CT_N + 1 nodes
+// from 0.5 Bohr to the largest cutoff, as the cubic (Hermite) piece of each
+// interval in FP32.  Its interpolation error is below 5e-9 relative over
+// 1-17 Bohr, under the FP32 rounding of the polynomial itself.
+int build_coulomb_table(pmx_ctx *c) {
+  const Setup &s = c->s;
+  double rend = 0.0;
+  for (int t = 0; t < s.npair; t++) rend = std::max(rend, s.cutoff[t]);
+  const double r0 = 0.5, dr = (rend * (1.0 + 1e-6) - r0) / CT_N, alp = s.alp_ewd;
+  auto vd = [&](double r, double &v, double &d) {
+    const double x = alp * r, tt = 1.0 / (1.0 + kErfP * x), ee = exp(-x * x);
+    double g = kErfE[9], dg = 9.0 * kErfE[9];
+    for (int k = 8; k >= 1; k--) {
+      g = g * tt + kErfE[k];
+      dg = dg * tt + k * kErfE[k];
+    }
+    g = g * tt * ee;
+    dg = dg * tt * tt * ee * kErfP * alp + 2.0 * alp * alp * g * r;  // -d g / dr
+    v = g / r;
+    d = -(g / (r * r) + dg / r);  // dV/dr
+  };
+  std::vector<float4> tab(CT_N);
+  double v0, d0;
+  vd(r0, v0, d0);
+  for (int k = 0; k < CT_N; k++) {
+    double v1, d1;
+    vd(r0 + dr * (k + 1), v1, d1);
+    tab[k] = make_float4((float)v0, (float)(dr * d0), (float)(3.0 * (v1 - v0) - dr * (2.0 * d0 + d1)),
+                         (float)(2.0 * (v0 - v1) + dr * (d0 + d1)));
+    v0 = v1;
+    d0 = d1;
+  }
+  if (!c->ctab) {
+    int rc = dalloc(c, &c->ctab, (size_t)CT_N);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpy(c->ctab, tab.data(), CT_N * sizeof(float4), hipMemcpyHostToDevice));
+  c->ct_r0 = (float)r0;
+  c->ct_idr = (float)(1.0 / dr);
+  return PMX_OK;
+}
+
 int build_device(pmx_ctx *c, const pmx_system *sys) {
   Setup &s = c->s;
   const int N = s.N, P = s.P;
@@ -574,6 +619,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
     if (c->pair_mode == 2 && P > 1) TRY(dalloc(c, &c->disp, NP));
+    if (c->pair_mode == 2) TRY(build_coulomb_table(c));
     if (P > 1) TRY(dalloc(c, &c->umax, (size_t)N));
   }
   TRY(dalloc(c, &c->site_cell, N));
@@ -1130,6 +1176,9 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.palp = kErfP * s.alp_ewd;
     a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
     a.spalp = 2.0 * s.alp_ewd / sqrt(M_PI);
+    a.ctab = c->ctab;
+    a.ct_r0 = c->ct_r0;
+    a.ct_idr = c->ct_idr;
     a.rheal = c->par.rheal;
     a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
     a.swit = (c->par.ishift == PMX_SHIFT_SWIT);
@@ -2068,6 +2117,8 @@ int pmx_set_force_field(pmx_ctx *c, const pmx_system *sys) {
   }
   const std::vector<double> pp = pair_constants(c);
   HIPCHK(hipMemcpy(c->pair_pp, pp.data(), pp.size() * sizeof(double), hipMemcpyHostToDevice));
+  if (c->pair_mode == 2)
+    if (int rc = build_coulomb_table(c)) return rc;
   c->bc_h.assign(sys->bond_c, sys->bond_c + 7 * (size_t)c->nbond_type);
   if (c->nbond_type) {
     HIPCHK(hipMemcpy(c->bc, sys->bond_c, 7 * (size_t)c->nbond_type * sizeof(double), hipMemcpyHostToDevice));
@@ -2138,6 +2189,9 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
     a.palp = kErfP * s.alp_ewd;
     a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
     a.spalp = 2.0 * s.alp_ewd / sqrt(M_PI);
+    a.ctab = c->ctab;
+    a.ct_r0 = c->ct_r0;
+    a.ct_idr = c->ct_idr;
     a.rheal = c->par.rheal;
     a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
     a.swit = (c->par.ishift == PMX_SHIFT_SWIT);

@@ -42,6 +42,7 @@ enum {
 };
 // LJ type pairs whose eps/sigma gradients one context forms
 constexpr int MAX_LJ_GRAD = 4;
+constexpr int CT_N = 1024;  // intervals of the mixed kernel's Coulomb table (16 KB of LDS)
 
 // List entries of the force kernels carry the type pair in the top byte.
 constexpr int ENTRY_SHIFT = 24;
@@ -52,6 +53,11 @@ struct PairArgs {
   int gx;  // workgroups along x (pair_grid_x)
   double alp, palp, talp2, rheal, irheal;  // erfc and switch constants
   double spalp;                           // 2 alp / sqrt(pi)
+  // mixed precision: erfc(alp r)/r (PINY's polynomial) as cubic pieces in r,
+  // CT_N intervals from ct_r0 with 1/width ct_idr: (c0, c1, c2, c3) of
+  // c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1] across the interval
+  const float4 *ctab;
+  float ct_r0, ct_idr;
   int swit;
   const double *pp;            // [npair][PP_NUM]
   const double *x, *y, *z;     // [N*P]

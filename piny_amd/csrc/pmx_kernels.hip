@@ -755,7 +755,7 @@ __global__ void __launch_bounds__(256) k_pair_half(PairArgs a) {
 // registers: A = (cut^2, cut, C12, C6), B = (LJ shift, Coulomb shift, LJ
 // flag, 0), read from LDS as two 16-byte loads per pair.
 __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4 &A, const float4 &B, float q2,
-                                                 float r, float rinv, float &vlj, float &vc) {
+                                                 float r, float rinv, float &vlj, float &vc, const float4 *ct) {
   float sw = 1.f, dsw = 0.f;
   if (a.swit) {
     const float rheal = (float)a.rheal, irheal = (float)a.irheal;
@@ -779,19 +779,13 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
     dvt = dv * sw + v * dsw;
   }
   if (q2 != 0.f) {
-    // PINY's erfc polynomial; -d erfc(alp r)/dr = 2 alp / sqrt(pi)
-    // exp(-alp^2 r^2) (the polynomial's own derivative differs by at most
-    // 1.1e-7 relative, 5e-9 in the force term, below FP32 rounding)
-    const float ralp = r * (float)a.alp;
-    const float eee = exp_r<float>(-ralp * ralp);
-    const float tt = rcp_r<float>(1.f + (float)kErfP * ralp);
-    float g = (float)kErfE[9];
-#pragma unroll
-    for (int k = 8; k >= 1; k--) g = g * tt + (float)kErfE[k];
-    g = g * tt * eee;
-    const float dg = (float)a.spalp * eee;
-    const float v = g * rinv;
-    const float dv = (g * r2i + dg * rinv) * rinv;
+    // erfc(alp r)/r and its r derivative from the cubic piece of the table
+    const float u = fmaxf(r - a.ct_r0, 0.f) * a.ct_idr;
+    const int k = min((int)u, CT_N - 1);
+    const float t = u - (float)k;
+    const float4 cc = ct[k];
+    const float v = cc.x + t * (cc.y + t * (cc.z + t * cc.w));
+    const float dv = -(cc.y + t * (2.f * cc.z + 3.f * t * cc.w)) * a.ct_idr * rinv;  // -(1/r) dV/dr
     vc = q2 * (v * sw - B.y);
     dvt += q2 * (dv * sw + v * dsw);
   }
@@ -813,12 +807,14 @@ __device__ __forceinline__ void wave_sync() {
 // period.c:121-206; without PIMD the imaged x_i - x_j itself), rounded to
 // FP32, and q_i q_j, into the wavefront's LDS slice.  Then the (slot, bead)
 // lanes evaluate the chunk's pairs in FP32: r = u_i - u_j + D_ij with the
-// bead displacements u = x - xmod, the LJ and erfc terms of pair_direct.
+// bead displacements u = x - xmod, the LJ and erfc terms of pair_direct
+// (erfc(alp r)/r from an LDS table of cubic pieces of PINY's polynomial).
 // Energies, virial and forces are summed in FP32 within a chunk and in FP64
 // across chunks and lanes.
 template <int B>
 __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   __shared__ float4 sppA[MAX_PAIRS], sppB[MAX_PAIRS];  // pair_direct_f32's constants per type pair
+  __shared__ float4 sct[CT_N];                          // the Coulomb table
   __shared__ float4 sdq[4][64];
   __shared__ int2 sent[4][64];  // (j * P, type pair) of the staged entries
   for (int t = threadIdx.x; t < a.npair; t += blockDim.x) {
@@ -826,6 +822,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
     sppA[t] = make_float4((float)pp[PP_CUT2], (float)pp[PP_CUT], (float)pp[PP_C12], (float)pp[PP_C6]);
     sppB[t] = make_float4((float)pp[PP_VSLJ], (float)pp[PP_VSC], (float)pp[PP_LJ], 0.f);
   }
+  for (int k = threadIdx.x; k < CT_N; k += blockDim.x) sct[k] = a.ctab[k];
   __syncthreads();
   constexpr int S = 64 / B;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -899,7 +896,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         if (!bact || !(r2 < A.x)) return;
         const float rinv = __frsqrt_rn(r2);
         float ev_lj, ev_c;
-        const float dvd = pair_direct_f32(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c);
+        const float dvd = pair_direct_f32(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c, sct);
         const float fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
         cvlj += ev_lj;
         cvc += ev_c;
