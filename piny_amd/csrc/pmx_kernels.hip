@@ -780,8 +780,9 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
   }
   if (q2 != 0.f) {
     // erfc(alp r)/r and its r derivative from the cubic piece of the table
-    const float u = fmaxf(r - a.ct_r0, 0.f) * a.ct_idr;
-    const int k = min((int)u, CT_N - 1);
+    // u < 0 (r below the table) converts to k = 0 and extrapolates the first piece
+    const float u = r * a.ct_idr - a.ct_r0 * a.ct_idr;
+    const unsigned k = min(__float2uint_rz(u), (unsigned)(CT_N - 1));
     const float t = u - (float)k;
     const float4 cc = ct[k];
     const float v = cc.x + t * (cc.y + t * (cc.z + t * cc.w));
@@ -811,7 +812,7 @@ __device__ __forceinline__ void wave_sync() {
 // (erfc(alp r)/r from an LDS table of cubic pieces of PINY's polynomial).
 // Energies, virial and forces are summed in FP32 within a chunk and in FP64
 // across chunks and lanes.
-template <int B>
+template <int B, bool PIMD>
 __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   __shared__ float4 sppA[MAX_PAIRS], sppB[MAX_PAIRS];  // pair_direct_f32's constants per type pair
   __shared__ float4 sct[CT_N];                          // the Coulomb table
@@ -838,7 +839,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   for (int i = wave; i < a.N; i += nwave) {
     const size_t ib = (size_t)i * P + (bact ? bead : 0);
     const double4 si = a.site[i];
-    const float4 ui = a.pimd ? a.u[ib] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 ui = PIMD ? a.u[ib] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int cnt = a.cnt[i];
     const int *row = a.jlist + (size_t)i * a.cap;
     double fxi = 0.0, fyi = 0.0, fzi = 0.0;
@@ -854,7 +855,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         e = row[ks];
         const int j = e & ENTRY_MASK;
         const double4 sj = a.site[j];
-        if (a.pimd) {
+        if (PIMD) {
           min_image_sep(c, 1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, si.x, si.y, si.z, sj.x, sj.y, sj.z, dx, dy, dz);
           dx += si.x - sj.x;
           dy += si.y - sj.y;
@@ -864,7 +865,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
                         dy, dz);
         }
         const double lim =
-            ((double)sppA[e >> ENTRY_SHIFT].y + (a.pimd ? a.umax[i] + a.umax[j] : 0.0)) * (1.0 + 1e-5);
+            ((double)sppA[e >> ENTRY_SHIFT].y + (PIMD ? a.umax[i] + a.umax[j] : 0.0)) * (1.0 + 1e-5);
         keep = dx * dx + dy * dy + dz * dz < lim * lim;
         q2 = si.w * sj.w;
       }
@@ -887,7 +888,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         const int t = sent[w][p].y;
         const float4 A = sppA[t], B2 = sppB[t];
         float dx = d.x, dy = d.y, dz = d.z;
-        if (a.pimd) {
+        if (PIMD) {
           dx += ui.x - uj.x;
           dy += ui.y - uj.y;
           dz += ui.z - uj.z;
@@ -912,7 +913,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         cn++;
       };
       auto disp = [&](int p) {
-        return (a.pimd && p < nk) ? a.u[sent[w][p].x + bsel] : make_float4(0.f, 0.f, 0.f, 0.f);
+        return (PIMD && p < nk) ? a.u[(unsigned)(sent[w][p].x + bsel)] : make_float4(0.f, 0.f, 0.f, 0.f);
       };
       // four pairs per iteration, each displacement loaded a whole iteration
       // ahead into a register of its own: no register copies, which would
@@ -1780,7 +1781,8 @@ void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_
   if (full) {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged_tab<B>), grid, block, 0, st, a);
     else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged_dir<B>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B>), grid, block, 0, st, a);
+    else if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false>), grid, block, 0, st, a);
   } else {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 0>), grid, block, 0, st, a);
     else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 1>), grid, block, 0, st, a);
