@@ -912,25 +912,31 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         c23 += dy * fz1;
         cn++;
       };
-      auto disp = [&](int p) {
-        return (PIMD && p < nk) ? a.u[(unsigned)(sent[w][p].x + bsel)] : make_float4(0.f, 0.f, 0.f, 0.f);
+      // the displacement of pair p into u (left as it is past the chunk's
+      // end, where it is not used)
+      auto fetch = [&](float4 &u, int p) {
+        if (PIMD && p < nk) u = a.u[(unsigned)(sent[w][p].x + bsel)];
       };
       // four pairs per iteration, each displacement loaded a whole iteration
       // ahead into a register of its own: no register copies, which would
       // wait for the load (one pair ahead through copies: 0.46 ms; two: 0.42
       // ms, and three raised the kernel above 128 VGPRs, to 3 wavefronts per
       // SIMD: 0.67 ms; four ahead, unrolled: 0.41 ms at 120 VGPRs)
-      float4 uA = disp(slot), uB = disp(slot + S), uC = disp(slot + 2 * S), uD = disp(slot + 3 * S);
+      float4 uA = make_float4(0.f, 0.f, 0.f, 0.f), uB = uA, uC = uA, uD = uA;
+      fetch(uA, slot);
+      fetch(uB, slot + S);
+      fetch(uC, slot + 2 * S);
+      fetch(uD, slot + 3 * S);
       int p = slot;
       for (; p + 3 * S < nk; p += 4 * S) {
         pair(p, uA);
-        uA = disp(p + 4 * S);
+        fetch(uA, p + 4 * S);
         pair(p + S, uB);
-        uB = disp(p + 5 * S);
+        fetch(uB, p + 5 * S);
         pair(p + 2 * S, uC);
-        uC = disp(p + 6 * S);
+        fetch(uC, p + 6 * S);
         pair(p + 3 * S, uD);
-        uD = disp(p + 7 * S);
+        fetch(uD, p + 7 * S);
       }
       if (p < nk) pair(p, uA);
       if (p + S < nk) pair(p + S, uB);
