@@ -24,6 +24,13 @@ namespace pmx {
 namespace {
 
 constexpr int PME_BLOCK = 128;  // lanes per (tile, bead) workgroup: about one per charge
+// lanes per charge in the spread (measured: with FP32 weights 90, 80 and 74
+// us for 1, 2 and 4 lanes; with FP64 weights, whose registers allow fewer
+// wavefronts, more lanes did not help)
+template <typename T>
+constexpr int spread_lanes() {
+  return sizeof(T) == 4 ? 4 : 1;
+}
 
 // Complex type of a grid scalar T (double2 for FP64, float2 for the mixed
 // precision's FP32 grids).
@@ -189,7 +196,7 @@ __device__ __forceinline__ void item_weights(const PmeArgs &a, int it, const int
 // the FP32 path rounds the box once when it is stored (FP32 LDS atomics
 // measured 5x slower than FP64 ones on MI355X for this access pattern).
 template <int NI, typename T>
-__global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
+__global__ void __launch_bounds__(spread_lanes<T>() * PME_BLOCK) k_pme_spread(PmeArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   double *box = (double *)lds_raw;
   const int hvol = (int)a.hvol;
@@ -203,7 +210,11 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
   tile_origin(a, tl, org[0], org[1], org[2]);
   const int H0 = a.H[0], H01 = a.H[0] * a.H[1];
   __syncthreads();
-  for (int c = tid; c < cnt; c += blockDim.x) {
+  // L lanes per charge, each adding every L-th row (mc, mb) of the support
+  // (more wavefronts in flight for the LDS atomics)
+  constexpr int L = spread_lanes<T>();
+  const int part = tid % L;
+  for (int c = tid / L; c < cnt; c += blockDim.x / L) {
     const int it = a.sorted[lo + c];
     const T q = (T)a.q[a.ichrg[it % a.nchrg]];
     T w[3][NI], dw[3][NI];
@@ -211,14 +222,12 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_spread(PmeArgs a) {
     item_weights<NI, false, T>(a, it, org, w, dw, l);
 #pragma unroll
     for (int m = 0; m < NI; m++) w[0][m] *= q;
-    for (int mc = 0; mc < NI; mc++) {
+    for (int rw = part; rw < NI * NI; rw += L) {
+      const int mc = rw / NI, mb = rw - mc * NI;
+      const T wbc = w[1][mb] * w[2][mc];
+      double *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
 #pragma unroll
-      for (int mb = 0; mb < NI; mb++) {
-        const T wbc = w[1][mb] * w[2][mc];
-        double *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
-#pragma unroll
-        for (int ma = 0; ma < NI; ma++) atomicAdd(row + ma, (double)(w[0][ma] * wbc));
-      }
+      for (int ma = 0; ma < NI; ma++) atomicAdd(row + ma, (double)(w[0][ma] * wbc));
     }
   }
   __syncthreads();
@@ -367,7 +376,7 @@ size_t pme_lds_bytes(const PmeArgs &a, bool gather) {
 
 template <typename T>
 hipError_t launch_spread_t(const PmeArgs &a, hipStream_t st) {
-  const dim3 grid(xcd_grid(a.ntile, a.P)), block(PME_BLOCK);
+  const dim3 grid(xcd_grid(a.ntile, a.P)), block(spread_lanes<T>() * PME_BLOCK);
   const size_t lds = pme_lds_bytes(a, false);
   switch (a.n) {
 #define X(NI) \
