@@ -1048,9 +1048,15 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
                      c->q, c->site);
   HIPCHK(hipGetLastError());
   if (c->disp) {
-    hipLaunchKernelGGL(k_pack_disp, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm,
-                       c->disp);
-    hipLaunchKernelGGL(k_site_umax, dim3(blocks(N)), dim3(256), 0, st, N, P, c->disp, c->umax);
+    if (P <= 64) {
+      const size_t waves = ((size_t)N + 64 / P - 1) / (64 / P);
+      hipLaunchKernelGGL(k_pack_disp_umax, dim3(blocks(waves * 64)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm,
+                         c->ym, c->zm, c->disp, c->umax);
+    } else {
+      hipLaunchKernelGGL(k_pack_disp, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym, c->zm,
+                         c->disp);
+      hipLaunchKernelGGL(k_site_umax, dim3(blocks(N)), dim3(256), 0, st, N, P, c->disp, c->umax);
+    }
     HIPCHK(hipGetLastError());
   } else if (c->umax && c->full_list) {
     hipLaunchKernelGGL(k_site_umax64, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym,

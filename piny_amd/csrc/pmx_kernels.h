@@ -254,6 +254,8 @@ __global__ void k_pack_sites(int N, int pimd, const double *xm, const double *ym
 __global__ void k_pack_disp(int N, int P, const double *x, const double *y, const double *z, const double *xm,
                             const double *ym, const double *zm, float4 *u);
 __global__ void k_site_umax(int N, int P, const float4 *u, double *umax);
+__global__ void k_pack_disp_umax(int N, int P, const double *x, const double *y, const double *z, const double *xm,
+                                 const double *ym, const double *zm, float4 *u, double *umax);
 __global__ void k_site_umax64(int N, int P, const double *x, const double *y, const double *z, const double *xm,
                               const double *ym, const double *zm, double *umax);
 __global__ void k_list_check(int N, int P, const double *x, const double *y, const double *z, const double *x0,

@@ -115,6 +115,35 @@ __global__ void k_pack_disp(int N, int P, const double *__restrict__ x, const do
   u[t] = make_float4((float)(x[t] - xm[i]), (float)(y[t] - ym[i]), (float)(z[t] - zm[i]), 0.0f);
 }
 
+// k_pack_disp and k_site_umax in one pass for P <= 64: a group of P lanes
+// holds one site's beads (64 / P sites per wavefront); each lane stores its
+// bead's displacement and the group's largest |u|^2 meets by shuffles.
+__global__ void __launch_bounds__(256) k_pack_disp_umax(int N, int P, const double *__restrict__ x,
+                                                        const double *__restrict__ y, const double *__restrict__ z,
+                                                        const double *__restrict__ xm, const double *__restrict__ ym,
+                                                        const double *__restrict__ zm, float4 *__restrict__ u,
+                                                        double *__restrict__ umax) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int S = 64 / P, g = lane / P, k = lane - g * P, lead = g * P;
+  const int i = wave * S + g;
+  const bool act = g < S && i < N;
+  float m = 0.0f;
+  if (act) {
+    const size_t t = (size_t)i * P + k;
+    const float4 v = make_float4((float)(x[t] - xm[i]), (float)(y[t] - ym[i]), (float)(z[t] - zm[i]), 0.0f);
+    u[t] = v;
+    m = v.x * v.x + v.y * v.y + v.z * v.z;
+  }
+  float mx = m;
+  if ((P & (P - 1)) == 0) {  // groups of 2^k lanes: butterfly
+    for (int off = 1; off < P; off <<= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  } else {
+    for (int o = 1; o < P; o++) mx = fmaxf(mx, __shfl(m, lead + (k + o) % P, 64));
+  }
+  if (act && k == 0) umax[i] = sqrt((double)mx);
+}
+
 // The largest |u| of every site over its beads (one thread per site).
 __global__ void k_site_umax(int N, int P, const float4 *__restrict__ u, double *__restrict__ umax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
