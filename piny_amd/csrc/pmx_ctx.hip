@@ -706,7 +706,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
       }
     }
     const size_t cx = c->pme_f32 ? 2 * sizeof(float) : 2 * sizeof(double);
-    if ((size_t)Ka * Kb * cx > FFT_LDS_MAX || (size_t)Kc * Ha * cx > FFT_LDS_MAX) {
+    if (((size_t)Ka * Kb + Ka + Kb) * cx > FFT_LDS_MAX || ((size_t)Kc * Ha + Kc) * cx > FFT_LDS_MAX) {
       set_err(c, "PME grid %dx%dx%d: a plane exceeds the %zu bytes of LDS of one workgroup", Ka, Kb, Kc,
               (size_t)FFT_LDS_MAX);
       return PMX_ERR_NOTIMPL;

@@ -721,6 +721,14 @@ __device__ __forceinline__ const typename Cx<T>::C *twiddles(const FftArgs &a, i
     return a.twf[d];
 }
 
+// The K[d] twiddles of axis d copied into LDS at dst: the passes read them at
+// lane-varying indices, which LDS serves without a vector-memory round trip.
+template <typename T>
+__device__ __forceinline__ void stage_twiddles(const FftArgs &a, int d, typename Cx<T>::C *dst) {
+  const typename Cx<T>::C *src = twiddles<T>(a, d);
+  for (int k = threadIdx.x; k < a.K[d]; k += blockDim.x) dst[k] = src[k];
+}
+
 // Plane c of bead b's spread charge into LDS as complex values, folded from
 // the halo boxes: along each axis, point l of tile t is held by the box of
 // tile t - o (periodically) at l + o*T for every o with l + o*T < H.  Tiles
@@ -811,10 +819,13 @@ __global__ void __launch_bounds__(NT) k_fft_fwd_ab(FftArgs a) {
   int c, b;
   if (!xcd_unit_bead(a.K[2], a.P, c, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
+  C *twa = sm + Ka * Kb, *twb = twa + Ka;  // twiddles after the plane
+  stage_twiddles<T>(a, 0, twa);
+  stage_twiddles<T>(a, 1, twb);
   fold_plane<T>(a, c, b, sm);
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, -1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
-  lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, -1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
+  lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, -1, twa, a.npass[0], a.radix[0]);
+  lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, -1, twb, a.npass[1], a.radix[1]);
   C *sp = (C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
   const FastDiv dh(Ha);
   for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
@@ -835,6 +846,8 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
   int bb, b;
   if (!xcd_unit_bead(a.K[1], a.P, bb, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Kc = a.K[2], Ha = Ka / 2 + 1;
+  C *twc = sm + Kc * Ha;  // twiddles after the lines
+  stage_twiddles<T>(a, 2, twc);
   C *sp = (C *)a.spec + (size_t)b * a.sstride;
   const FastDiv dh(Ha);
   for (int w0 = threadIdx.x; w0 < Kc * Ha; w0 += 8 * blockDim.x) {
@@ -852,7 +865,7 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, -1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
+  lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, -1, twc, a.npass[2], a.radix[2]);
   const double *hi = a.cell.hi;
   const double tpi = 2.0 * M_PI, falp2 = a.falp2;
   const int kb = bb <= Kb / 2 ? bb : bb - Kb;
@@ -883,7 +896,7 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
     sm[w] = Cx<T>::make((T)(s.x * W), (T)(s.y * W));
   }
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, +1, twiddles<T>(a, 2), a.npass[2], a.radix[2]);
+  lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, +1, twc, a.npass[2], a.radix[2]);
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
     const int cc = dh.q(w), aa = w - cc * Ha;
     sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
@@ -903,6 +916,9 @@ __global__ void __launch_bounds__(NT) k_fft_inv_ba(FftArgs a) {
   int c, b;
   if (!xcd_unit_bead(a.K[2], a.P, c, b)) return;
   const int Ka = a.K[0], Kb = a.K[1], Ha = Ka / 2 + 1;
+  C *twa = sm + Ka * Kb, *twb = twa + Ka;  // twiddles after the plane
+  stage_twiddles<T>(a, 0, twa);
+  stage_twiddles<T>(a, 1, twb);
   const C *sp = (const C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
   const FastDiv dka(Ka);
   for (int w0 = threadIdx.x; w0 < Kb * Ka; w0 += 8 * blockDim.x) {
@@ -920,7 +936,7 @@ __global__ void __launch_bounds__(NT) k_fft_inv_ba(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, +1, twiddles<T>(a, 1), a.npass[1], a.radix[1]);
+  lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, +1, twb, a.npass[1], a.radix[1]);
   // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
   for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
     const int bb = dka.q(w), aa = w - bb * Ka;
@@ -930,7 +946,7 @@ __global__ void __launch_bounds__(NT) k_fft_inv_ba(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, +1, twiddles<T>(a, 0), a.npass[0], a.radix[0]);
+  lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, +1, twa, a.npass[0], a.radix[0]);
   T *g = (T *)a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
   for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) g[w] = sm[w].x;
 }
@@ -939,7 +955,8 @@ template <typename T, int NT, int MAXP>
 hipError_t launch_fft_class(const FftArgs &f, hipStream_t st) {
   using C = typename Cx<T>::C;
   const int Ka = f.K[0], Kb = f.K[1], Kc = f.K[2], Ha = Ka / 2 + 1;
-  const size_t lab = (size_t)Ka * Kb * sizeof(C), lch = (size_t)Kc * Ha * sizeof(C);
+  // the plane (or the lines) and the twiddles of the axes it transforms
+  const size_t lab = ((size_t)Ka * Kb + Ka + Kb) * sizeof(C), lch = ((size_t)Kc * Ha + Kc) * sizeof(C);
   if (lab > 65536 || lch > 65536) {  // beyond 64 KB the kernels ask for the larger LDS
     hipError_t e;
     if ((e = hipFuncSetAttribute((const void *)k_fft_fwd_ab<T, NT, MAXP>,
