@@ -856,7 +856,8 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
     for (int u = 0; u < 8; u++) {
       const int w = w0 + u * blockDim.x;
       const int cc = dh.q(w);
-      if (w < Kc * Ha) v[u] = sp[((size_t)cc * Kb + bb) * Ha + (w - cc * Ha)];
+      // a select, not a conditional store: the latter put v on the stack in FP64
+      v[u] = w < Kc * Ha ? sp[((size_t)cc * Kb + bb) * Ha + (w - cc * Ha)] : Cx<T>::make((T)0, (T)0);
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
@@ -895,16 +896,19 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
     }
     sm[w] = Cx<T>::make((T)(s.x * W), (T)(s.y * W));
   }
+  {
+    // the k-space pressure also carries the energy on its diagonal
+    // (ewald3d_pme.c:412-414); reduced before the inverse transform so the
+    // sums are not held in registers through it
+    const double vals[FFT_NV] = {e, v11 + e, v22 + e, v33 + e, v12, v13, v23};
+    block_partials<FFT_NV>(vals, 1, b, a.P, a.part, bb, a.K[1]);
+  }
   __syncthreads();
   lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, +1, twc, a.npass[2], a.radix[2]);
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
     const int cc = dh.q(w), aa = w - cc * Ha;
     sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
   }
-  // the k-space pressure also carries the energy on its diagonal
-  // (ewald3d_pme.c:412-414)
-  const double vals[FFT_NV] = {e, v11 + e, v22 + e, v33 + e, v12, v13, v23};
-  block_partials<FFT_NV>(vals, 1, b, a.P, a.part, bb, a.K[1]);
 }
 
 // inverse transform of one c-plane: along b, then complex-to-real along a.
