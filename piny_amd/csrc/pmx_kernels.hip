@@ -175,6 +175,7 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
                                                     double *xm, double *ym, double *zm) {
 #pragma clang fp contract(off)
   __shared__ double red[2][4];
+  __shared__ double stage[4][3 * 64];  // one chunk of bead positions per wavefront
   const int lane = threadIdx.x & 63;
   const int G = P <= 64 ? P : 64;           // lanes per site
   const int S = 64 / G;                     // sites per wavefront
@@ -202,19 +203,34 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
         const double ex = xb - xi, ey = yb - yi, ez = zb - zi;
         smax = fmax(smax, ex * ex + ey * ey + ez * ez);
       }
+      // the group's lead lane forms the ordered sum over the chunk from LDS
+      // (the wavefront's own rows; lanes of one wavefront need no barrier
+      // beyond the ordering of their LDS accesses)
       const int n = min(G, P - c0);
-      for (int m = 0; m < n; m++) {  // every lane of the group forms the same ordered sum
-        const double vx = shfl_d(xb, lead + m), vy = shfl_d(yb, lead + m), vz = shfl_d(zb, lead + m);
-        if (c0 + m == 0) {
-          sx = vx;
-          sy = vy;
-          sz = vz;
-        } else {
-          sx += vx;
-          sy += vy;
-          sz += vz;
+      double *sw = stage[threadIdx.x >> 6];
+      sw[lane] = xb;
+      sw[64 + lane] = yb;
+      sw[128 + lane] = zb;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (k == 0) {
+        for (int m = 0; m < n; m++) {
+          const double vx = sw[lead + m], vy = sw[64 + lead + m], vz = sw[128 + lead + m];
+          if (c0 + m == 0) {
+            sx = vx;
+            sy = vy;
+            sz = vz;
+          } else {
+            sx += vx;
+            sy += vy;
+            sz += vz;
+          }
         }
       }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // reads done before the next chunk's stores
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (act && k == 0) {
       b0x[i] = xi;
