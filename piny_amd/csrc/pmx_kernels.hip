@@ -215,6 +215,7 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       if (k == 0) {
+#pragma unroll 8
         for (int m = 0; m < n; m++) {
           const double vx = sw[lead + m], vy = sw[64 + lead + m], vz = sw[128 + lead + m];
           if (c0 + m == 0) {
