@@ -1606,6 +1606,50 @@ void or_nm_force_to_mode(int P, const double *f, double *fm) {
 }
 
 /* ------------------------------------------------------------------------
+ * staging transforms of transform_stg.c for one site (path_int_md_typ =
+ * staging), with rat1[ip] = (ip-1)/ip and rat2[ip] = 1/ip for 1-based beads
+ * ip = 2..P (transform_cnt.c:206-218).  Mode 1 is bead 1 itself.
+ */
+static double or_rat1(int ip) { return ((double)(ip - 1)) / ((double)(ip)); }
+static double or_rat2(int ip) { return 1.0 / ((double)(ip)); }
+
+/* convert_pimd_mode_stag, transform_stg.c:94-160: for ip = 2..P-1,
+ * u_ip = x_ip - (rat1 x_{ip+1} + rat2 x_1) with the untransformed x_{ip+1};
+ * u_P = x_P - (rat1 x_1 + rat2 x_1). */
+void or_stag_pos_to_mode(int P, const double *x, double *u) {
+  u[0] = x[0];
+  for (int ip = 2; ip <= P - 1; ip++) {
+    double xstar = or_rat1(ip) * x[ip] + or_rat2(ip) * x[0];
+    u[ip - 1] = x[ip - 1] - xstar;
+  }
+  if (P > 1) {
+    double xstar = or_rat1(P) * x[0] + or_rat2(P) * x[0];
+    u[P - 1] = x[P - 1] - xstar;
+  }
+}
+
+/* convert_pimd_pos_stag, transform_stg.c:168-226: x_P = u_P + x_1, then for
+ * ip = P-1 down to 2, x_ip = u_ip + (rat1 x_{ip+1} + x_1 rat2). */
+void or_stag_mode_to_pos(int P, const double *u, double *x) {
+  for (int k = 0; k < P; k++) x[k] = u[k];
+  if (P == 1) return;
+  x[P - 1] += x[0];
+  for (int ip = P - 1; ip >= 2; ip--) {
+    double xadd = or_rat1(ip) * x[ip] + x[0] * or_rat2(ip);
+    x[ip - 1] += xadd;
+  }
+}
+
+/* convert_pimd_force_stag, transform_stg.c:31-88: f_1 += f_ip for
+ * ip = 2..P in order, then f_{ip+1} += rat1[ip] f_ip for ip = 2..P-1 (the
+ * updated f_ip: a recurrence). */
+void or_stag_force_to_mode(int P, const double *f, double *fm) {
+  for (int k = 0; k < P; k++) fm[k] = f[k];
+  for (int ip = 2; ip <= P; ip++) fm[0] += fm[ip - 1];
+  for (int ip = 2; ip <= P - 1; ip++) fm[ip] += (or_rat1(ip) * fm[ip - 1]);
+}
+
+/* ------------------------------------------------------------------------
  * energy_control (energy_control.c:33-77) / energy_control_pimd
  * (energy_control_pimd.c:37-278) with RESPA off.
  */
@@ -1617,12 +1661,18 @@ int or_energy_control(or_ctx *c, double *x, double *y, double *z, const double *
   memset(out, 0, sizeof(*out));
   /* ghosts of every bead */
   for (int ip = 0; ip < P; ip++) or_ghost_pos(c, x + (size_t)ip * N, y + (size_t)ip * N, z + (size_t)ip * N);
-  /* xmod: given, else the bead centroid */
+  /* xmod: given, else mode 1 of the bead transform (control_pimd_trans.c:
+   * 175-181 copies it from mode 1 before the back transform): bead 1 with
+   * staging, the bead centroid with normal modes */
   double *xm = calloc(N, sizeof(double)), *ym = calloc(N, sizeof(double)), *zm = calloc(N, sizeof(double));
   if (xmi) {
     memcpy(xm, xmi, sizeof(double) * N);
     memcpy(ym, ymi, sizeof(double) * N);
     memcpy(zm, zmi, sizeof(double) * N);
+  } else if (c->par.pimd_typ == PMX_PIMD_STAGING) {
+    memcpy(xm, x, sizeof(double) * N);
+    memcpy(ym, y, sizeof(double) * N);
+    memcpy(zm, z, sizeof(double) * N);
   } else {
     for (int i = 0; i < N; i++) {
       double sx = 0, sy = 0, sz = 0;
@@ -1769,8 +1819,10 @@ int or_energy_control(or_ctx *c, double *x, double *y, double *z, const double *
   for (int ip = 0; ip < P; ip++)
     or_ghost_force(c, fx + (size_t)ip * N, fy + (size_t)ip * N, fz + (size_t)ip * N);
   /* PIMD mode forces and springs (energy_control_pimd.c:200-218,
-   * pimd_estimators.c:31-113, transform_cnt.c:415-489) */
+   * pimd_estimators.c:31-113, transform_cnt.c:415-489 for normal modes,
+   * transform_stg.c:31-160 for staging) */
   if ((flags & PMX_EVAL_PIMD) && (flags & PMX_EVAL_MODE_FORCES) && P > 1) {
+    const int stag = c->par.pimd_typ == PMX_PIMD_STAGING;
     double *a = malloc(sizeof(double) * P), *b = malloc(sizeof(double) * P);
     double ake = 0.0;
     for (int d = 0; d < 3; d++) {
@@ -1778,10 +1830,16 @@ int or_energy_control(or_ctx *c, double *x, double *y, double *z, const double *
       double *frc = d == 0 ? fx : (d == 1 ? fy : fz);
       for (int i = 0; i < N; i++) {
         for (int ip = 0; ip < P; ip++) a[ip] = pos[(size_t)ip * N + i];
-        or_nm_pos_to_mode(P, a, b); /* b = modes */
+        if (stag)
+          or_stag_pos_to_mode(P, a, b); /* b = modes */
+        else
+          or_nm_pos_to_mode(P, a, b);
         for (int ip = 0; ip < P; ip++) a[ip] = frc[(size_t)ip * N + i];
         double *fm = malloc(sizeof(double) * P);
-        or_nm_force_to_mode(P, a, fm);
+        if (stag)
+          or_stag_force_to_mode(P, a, fm);
+        else
+          or_nm_force_to_mode(P, a, fm);
         for (int ip = 0; ip < P; ip++) {
           double fk = c->prekf[i] * c->veig[ip + 1];
           ake += fk * b[ip] * b[ip];

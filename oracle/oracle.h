@@ -62,6 +62,12 @@ void or_nm_pos_to_mode(int P, const double *x, double *u);
 void or_nm_mode_to_pos(int P, const double *u, double *x);
 void or_nm_force_to_mode(int P, const double *f, double *fm);
 
+/* Staging transforms of transform_stg.c for one site (path_int_md_typ
+ * staging): positions to modes, modes to positions, forces to modes. */
+void or_stag_pos_to_mode(int P, const double *x, double *u);
+void or_stag_mode_to_pos(int P, const double *u, double *x);
+void or_stag_force_to_mode(int P, const double *f, double *fm);
+
 /* Gauss-Legendre nodes and weights on [-1,1] (n points). */
 void or_gauss_legendre(int n, double *node, double *weight);
 

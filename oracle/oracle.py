@@ -46,7 +46,8 @@ def lib():
         L.or_get_bead_energies.argtypes = [C.c_void_p, dp]
         L.or_recip_energy.restype = C.c_double
         L.or_recip_energy.argtypes = [C.c_void_p, dp, dp, dp, C.c_int, dp, dp, dp, dp]
-        for nm in ("or_nm_pos_to_mode", "or_nm_mode_to_pos", "or_nm_force_to_mode"):
+        for nm in ("or_nm_pos_to_mode", "or_nm_mode_to_pos", "or_nm_force_to_mode", "or_stag_pos_to_mode",
+                   "or_stag_mode_to_pos", "or_stag_force_to_mode"):
             getattr(L, nm).argtypes = [C.c_int, dp, dp]
         L.or_gauss_legendre.argtypes = [C.c_int, dp, dp]
         L.or_bond_coef_morse.argtypes = [C.c_double, C.c_double, dp]
@@ -170,6 +171,25 @@ def nm_force_to_mode(f):
     fm = np.zeros(P)
     lib().or_nm_force_to_mode(P, abi.dptr(np.ascontiguousarray(f, dtype=np.float64)), abi.dptr(fm))
     return fm
+
+
+def _stag(fn, v):
+    P = len(v)
+    out = np.zeros(P)
+    getattr(lib(), fn)(P, abi.dptr(np.ascontiguousarray(v, dtype=np.float64)), abi.dptr(out))
+    return out
+
+
+def stag_pos_to_mode(xb):
+    return _stag("or_stag_pos_to_mode", xb)
+
+
+def stag_mode_to_pos(u):
+    return _stag("or_stag_mode_to_pos", u)
+
+
+def stag_force_to_mode(f):
+    return _stag("or_stag_force_to_mode", f)
 
 
 def gauss_legendre(n):

@@ -80,6 +80,10 @@ CASES = {
                                         _st(cutoff=5.5, alpha=0.62, kmax_ewd=38, kmax_pme=38)),
     "tip64_p4_k96": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
                                            _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=44)),
+    # a 64^3 grid: its FP32 plan is two radix-8 passes, the second with
+    # twiddles other than 1 (the benchmark's grid)
+    "tip64_p4_k64": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
+                                           _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=30)),
     # the reference's own W512 example, PME and standard Ewald
     "w512": lambda: _w512(1),
     "w512_ewald": lambda: _w512(0),
@@ -93,6 +97,11 @@ CASES = {
                                         _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
     "spc512": lambda: w.build_config(w.SPC_FW, 512, w.density_box(512), 1, 20,
                                      _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
+    # the staging transform (path_int_md_typ staging, transform_stg.c): xmod
+    # is bead 1, mode forces by the staging recurrence
+    "tip64_p4_stag": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
+                                            _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, pimd_typ=1)),
+    "cfg2_p4_stag": lambda: w.config2(P=4, settings=_st(kmax_ewd=14, kmax_pme=14, pimd_typ=1)),
     # 256 q-TIP4P/F at P = 32, K = 32
     "tip256_p32": lambda: w.build_config(w.Q_TIP4P_F, 256, w.density_box(256), 32, 14,
                                          _st(cutoff=9.0, alpha=0.35, kmax_ewd=14, kmax_pme=14)),

@@ -33,8 +33,9 @@ from tests import cases  # noqa: E402
 EXE = os.path.join(ROOT, "oracle", "_ref", "piny_ref")
 CASES = ["spc64", "spc64_shift", "spc64_none", "tip64", "cfg1", "spc64_k20", "tip64_p4", "cfg2_p4", "spc64_ortho",
          "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq", "spc64_ewald",
-         "tip64_p4_ewald", "spc64_k80", "w512", "w512_ewald"]
+         "tip64_p4_ewald", "spc64_k80", "w512", "w512_ewald", "tip64_p4_stag", "cfg2_p4_stag"]
 SHIFT = {0: "off", 1: "on", 2: "swit"}  # shift_inter_pe values (set_sim_dict.c:736)
+PIMD_TYP = {1: "staging", 2: "centroid"}  # path_int_md_typ values (transform_cnt.c:104,190)
 
 
 def section(name, items):
@@ -55,7 +56,7 @@ def write_inputs(d, c):
            ("generic_fft_opt", "on"), ("num_proc_tot", 1), ("num_proc_beads", 1), ("num_proc_class_forc", 1)]
     parts = [section("sim_gen_def", gen)]
     if P > 1:
-        parts.append(section("sim_pimd_def", [("path_int_beads", P), ("path_int_md_typ", "centroid"),
+        parts.append(section("sim_pimd_def", [("path_int_beads", P), ("path_int_md_typ", PIMD_TYP[st.pimd_typ]),
                                               ("initial_spread_opt", "off")]))
     if P > 1:
         parts.append(section("sim_nhc_def", [("atm_nhc_tau_def", 20.0), ("atm_nhc_len", 4), ("respa_steps_nhc", 4),

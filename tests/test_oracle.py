@@ -6,8 +6,10 @@ import math
 import numpy as np
 import pytest
 
-from oracle.oracle import Oracle, gauss_legendre, nm_force_to_mode, nm_mode_to_pos, nm_pos_to_mode
+from oracle.oracle import (Oracle, gauss_legendre, nm_force_to_mode, nm_mode_to_pos, nm_pos_to_mode,
+                           stag_force_to_mode, stag_mode_to_pos, stag_pos_to_mode)
 from oracle import oracle as orc
+from piny_amd import abi
 from piny_amd import water as w
 from tests import cases
 
@@ -186,6 +188,81 @@ def test_nm_transforms():
         J = np.stack([nm_mode_to_pos(np.eye(P)[m]) for m in range(P)], 1)
         f = rng.normal(size=P)
         assert np.allclose(nm_force_to_mode(f), J.T @ f, atol=1e-12)
+
+
+def _veig(P, staging):
+    """path_integral_init's eigenvalues (transform_cnt.c:175-233), 0-based
+    mode order: staging k/(k-1); normal modes 4P(1 - cos(2 pi k/P))."""
+    v = np.zeros(P)
+    if staging:
+        for ip in range(2, P + 1):
+            v[ip - 1] = ip / (ip - 1)
+    else:
+        v[P - 1] = 4.0 * P
+        for ip in range(2, P // 2 + 1):
+            v[2 * ip - 3] = v[2 * ip - 2] = (1.0 - np.cos(2 * np.pi * (ip - 1) / P)) * 4.0 * P
+    return v
+
+
+@pytest.mark.parametrize("staging", [False, True])
+def test_mode_springs_equal_the_ring_polymer(staging):
+    # both transforms diagonalise the ring polymer's springs: with PINY's
+    # eigenvalues, sum_k veig_k u_k^2 = sum_b (x_b - x_(b+1))^2 (cyclic)
+    rng = np.random.default_rng(3)
+    to_mode = stag_pos_to_mode if staging else nm_pos_to_mode
+    for P in (4, 6, 8, 32):
+        if not staging and P % 2:
+            continue
+        x = rng.normal(size=P)
+        u = to_mode(x)
+        ring = np.sum((x - np.roll(x, -1)) ** 2)
+        assert abs(np.sum(_veig(P, staging) * u * u) - ring) < 1e-12 * ring
+
+
+def test_staging_transforms():
+    # the closed form of the staging modes (Tuckerman's u_1 = x_1,
+    # u_k = x_k - ((k-1) x_(k+1) + x_1)/k), the round trip, and the force
+    # transform as the chain rule f_u = J^T f with J = dx/du
+    rng = np.random.default_rng(4)
+    for P in (2, 4, 6, 8, 32, 64):
+        x = rng.normal(size=P)
+        u = stag_pos_to_mode(x)
+        ref = np.empty(P)
+        ref[0] = x[0]
+        for k in range(2, P + 1):
+            nxt = x[k] if k < P else x[0]
+            ref[k - 1] = x[k - 1] - ((k - 1) * nxt + x[0]) / k
+        assert np.allclose(u, ref, atol=1e-13)
+        assert np.allclose(stag_mode_to_pos(u), x, atol=1e-13)
+        J = np.stack([stag_mode_to_pos(np.eye(P)[m]) for m in range(P)], 1)
+        f = rng.normal(size=P)
+        assert np.allclose(stag_force_to_mode(f), J.T @ f, atol=1e-12)
+
+
+def test_staging_and_normal_modes_agree_in_cartesian_forces():
+    # one configuration with both transforms: the spring energy is the same
+    # (both equal prekf sum_b |x_b - x_(b+1)|^2), and the mode forces, taken
+    # back to bead coordinates with each transform's inverse chain rule, are
+    # the same Cartesian forces (the physical-plus-spring force on every bead)
+    import dataclasses
+    c = cases.get("tip64_p4")
+    s = c.system
+    P, N = s.pi_beads, s.natm
+    out = {}
+    for typ in (1, 2):
+        st = dataclasses.replace(s.settings, pimd_typ=typ)
+        sy = dataclasses.replace(s, settings=st)
+        o = Oracle(sy, c.x, c.y, c.z)
+        e, f, _ = o.energy_control(c.x, c.y, c.z, abi.PMX_EVAL_PIMD | abi.PMX_EVAL_MODE_FORCES)
+        to_pos = stag_mode_to_pos if typ == 1 else nm_mode_to_pos
+        J = np.stack([to_pos(np.eye(P)[m]) for m in range(P)], 1)
+        Jit = np.linalg.inv(J).T  # f_x = J^-T f_u
+        fb = np.einsum("bm,mnd->bnd", Jit, f.reshape(P, N, 3))
+        out[typ] = (e, fb)
+    (e1, f1), (e2, f2) = out[1], out[2]
+    assert abs(e1["kin_harm"] - e2["kin_harm"]) < 1e-12 * abs(e2["kin_harm"])
+    assert abs(e1["pi_ke_prim"] - e2["pi_ke_prim"]) < 1e-12 * abs(e2["pi_ke_prim"])
+    assert np.max(np.abs(f1 - f2)) < 1e-10 * np.max(np.abs(f2))
 
 
 # --------------------------------------------------------------------------
