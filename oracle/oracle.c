@@ -598,29 +598,33 @@ static void or_setup_exclusions(or_ctx *c) {
 
 /* Gauss-Legendre on [-1,1] by Newton iteration on P_n */
 void or_gauss_legendre(int n, double *node, double *weight) {
+  /* Newton's iteration on P_n in quadruple precision (__float128), each
+   * node and weight rounded to double once: the correctly rounded values,
+   * which are also what 33-digit decimal tables of them parse to (the
+   * reference's weights_nodes_128.h) */
   for (int i = 0; i < n; i++) {
-    double x = cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 0.0;
-    for (int it = 0; it < 100; it++) {
-      double p0 = 1.0, p1 = x;
+    __float128 x = cos(M_PI * (i + 0.75) / (n + 0.5)), p0 = 1, p1 = x, dp = 0;
+    for (int it = 0; it < 8; it++) {  /* quadratic convergence from a double start */
+      p0 = 1;
+      p1 = x;
       for (int k = 2; k <= n; k++) {
-        double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        __float128 p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
         p0 = p1;
         p1 = p2;
       }
-      dp = n * (x * p1 - p0) / (x * x - 1.0);
-      double dx = p1 / dp;
-      x -= dx;
-      if (fabs(dx) < 1e-17) break;
+      dp = n * (x * p1 - p0) / (x * x - 1);
+      x -= p1 / dp;
     }
-    double p0 = 1.0, p1 = x;
+    p0 = 1;
+    p1 = x;
     for (int k = 2; k <= n; k++) {
-      double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+      __float128 p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
       p0 = p1;
       p1 = p2;
     }
-    dp = n * (x * p1 - p0) / (x * x - 1.0);
-    node[n - 1 - i] = x;
-    weight[n - 1 - i] = 2.0 / ((1.0 - x * x) * dp * dp);
+    dp = n * (x * p1 - p0) / (x * x - 1);
+    node[n - 1 - i] = (double)x;
+    weight[n - 1 - i] = (double)(2 / ((1 - x * x) * dp * dp));
   }
 }
 

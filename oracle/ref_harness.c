@@ -10,6 +10,7 @@
  *
  * usage: piny_ref <input file> <positions.bin> <out.bin> [repeat]
  *        piny_ref <input file> <positions.bin> <out.bin> md <nsteps> <velocities.bin>
+ *        piny_ref <input file> <positions.bin> <out.bin> dump <setup.bin>
  *   positions.bin: P*N*3 doubles, Bohr, [bead][site][x,y,z] (ghost sites
  *                  are recomputed by get_ghost_pos)
  *   out.bin:       header {N, P, nver} as doubles, then 24 energy/virial
@@ -22,6 +23,15 @@
  *                  same positions and the wall time of those calls
  *                  (CLOCK_MONOTONIC, as src/output/timer.c:55-64) is printed
  *                  as "REPEAT <calls> <seconds>"
+ *   dump:          after the force call, the setup products and the Verlet
+ *                  list of that call are written to setup.bin: int32
+ *                  {N, nver, ninter, nsplin, nktot, ecor_nsplin, 0, 0}, then
+ *                  int32 nter[N], jver_off[N], jter[nver] as the reference
+ *                  stores them (jter 1-based, jver_off the 0-based exclusive prefix), float64 cv0, cdv0, cv0_c, cdv0_c
+ *                  [ninter][nsplin] (each type pair's table through
+ *                  inter_map_index), int32 kastr, kbstr, kcstr [nktot],
+ *                  float64 bweight_tot [nktot] (PME only, else zeros),
+ *                  float64 ecor cv0, cdv0 [ecor_nsplin]
  *   md:            P = 1; after the first force call the velocities
  *                  (N*3 doubles, Bohr per atomic time unit) are set and
  *                  int_NVE (md/int_NVE.c) runs nsteps times with the
@@ -52,12 +62,14 @@ static ANALYSIS analysis;
 
 int main(int argc, char *argv[]) {
   const int md = argc == 7 && strcmp(argv[4], "md") == 0;
-  if (argc != 4 && argc != 5 && !md) {
+  const int dump = argc == 6 && strcmp(argv[4], "dump") == 0;
+  if (argc != 4 && argc != 5 && !md && !dump) {
     fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin> [repeat | md nsteps velocities.bin]\n",
             argv[0]);
     return 2;
   }
   const int repeat = argc == 5 ? atoi(argv[4]) : 0;
+  const char *dump_file = dump ? argv[5] : NULL;
   int zero = 0;
   char **noargs = NULL;
   Init(&zero, &noargs, &(cls.communicate.world));
@@ -147,6 +159,35 @@ int main(int argc, char *argv[]) {
       printf("REPEAT %d %.6f\n", repeat, (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec));
       fflush(stdout);
     }
+  }
+
+  if (dump_file) {
+    VERLIST *vl = &cls.nbr_list.verlist;
+    INTERACT *it = &cls.interact;
+    EWALD *ew = &gd.ewald;
+    ECOR *ec = &bonded.ecor;
+    const int nver = vl->nver_lst_now, ninter = (cls.atommaps.natm_typ * (cls.atommaps.natm_typ + 1)) / 2;
+    const int nsp = it->nsplin, nk = ew->nktot, nse = ec->nsplin;
+    int hdr8[8] = {N, nver, ninter, nsp, nk, nse, 0, 0};
+    FILE *fd = fopen(dump_file, "wb");
+    if (!fd) return 1;
+    fwrite(hdr8, sizeof(int), 8, fd);
+    fwrite(&vl->nter[1], sizeof(int), N, fd);
+    fwrite(&vl->jver_off[1], sizeof(int), N, fd);
+    fwrite(&vl->jter[1], sizeof(int), nver, fd);
+    double *tabs[4] = {it->cv0, it->cdv0, it->cv0_c, it->cdv0_c};
+    for (int w = 0; w < 4; w++)
+      for (int t = 1; t <= ninter; t++) fwrite(&tabs[w][(it->inter_map_index[t] - 1) * nsp + 1], sizeof(double), nsp, fd);
+    fwrite(&ew->kastr[1], sizeof(int), nk, fd);
+    fwrite(&ew->kbstr[1], sizeof(int), nk, fd);
+    fwrite(&ew->kcstr[1], sizeof(int), nk, fd);
+    for (int k = 1; k <= nk; k++) {
+      double b = cls.part_mesh.bweight_tot ? cls.part_mesh.bweight_tot[k] : 0.0;
+      fwrite(&b, sizeof(double), 1, fd);
+    }
+    fwrite(&ec->cv0[1], sizeof(double), nse, fd);
+    fwrite(&ec->cdv0[1], sizeof(double), nse, fd);
+    fclose(fd);
   }
 
   STAT_AVG *s = &gd.stat_avg;

@@ -108,5 +108,18 @@ CASES = {
 }
 
 
+def _ns500(name):
+    """A case with 500 spline points (inter_spline_pts): small fixtures of the
+    reference's tables."""
+    import dataclasses
+    c = CASES[name]()
+    st = dataclasses.replace(c.system.settings, nsplin=500)
+    return w.Config(dataclasses.replace(c.system, settings=st), c.x, c.y, c.z)
+
+
+for _n in ("spc64", "tip64_p4", "tip64_p4_tric", "cfg1"):
+    CASES[_n + "_ns500"] = (lambda n=_n: _ns500(n))
+
+
 def get(name):
     return CASES[name]()

@@ -174,6 +174,50 @@ def run_case(name):
     print(f"{name}: N={N} P={P} nver={nver} vintert={e[0]:.12g} vintrat={e[1]:.12g} -> {out}")
 
 
+# the reference's setup products and Verlet list (oracle/ref_harness.c dump
+# mode): tables with 500 spline points keep the fixtures small
+SETUP_CASES = ["spc64_ns500", "tip64_p4_ns500", "tip64_p4_tric_ns500", "cfg1_ns500"]
+
+
+def run_setup_case(name):
+    """The reference's Verlet list (nter, jver_off, jter), spline tables
+    (cv0, cdv0, cv0_c, cdv0_c per type pair), k-vectors with their B-spline
+    weights and the excluded-pair table, 0-based where they are indices."""
+    c = cases.get(name)
+    s = c.system
+    with tempfile.TemporaryDirectory() as d:
+        write_inputs(d, c)
+        with open(os.path.join(d, "stdout.txt"), "w") as log:
+            r = subprocess.run([EXE, "water.input", "pos.bin", "out.bin", "dump", "setup.bin"], cwd=d, stdout=log,
+                               stderr=subprocess.STDOUT, timeout=600)
+        if r.returncode != 0:
+            print(open(os.path.join(d, "stdout.txt")).read()[-3000:])
+            raise RuntimeError(f"{name}: reference run failed ({r.returncode})")
+        raw = open(os.path.join(d, "setup.bin"), "rb").read()
+    hdr = np.frombuffer(raw, np.int32, 8)
+    N, nver, ninter, nsp, nk, nse = (int(v) for v in hdr[:6])
+    assert N == s.natm
+    o = 32
+
+    def take(dtype, n):
+        nonlocal o
+        a = np.frombuffer(raw, dtype, n, o)
+        o += a.nbytes
+        return a.copy()
+    nter = take(np.int32, N)
+    jver_off = take(np.int32, N)
+    jter = take(np.int32, nver)
+    tabs = take(np.float64, 4 * ninter * nsp).reshape(4, ninter, nsp)
+    ka, kb, kc = take(np.int32, nk), take(np.int32, nk), take(np.int32, nk)
+    bweight = take(np.float64, nk)
+    ecor = take(np.float64, 2 * nse).reshape(2, nse)
+    assert o == len(raw)
+    out = os.path.join(HERE, f"setup_{name}.npz")
+    np.savez_compressed(out, case=name, nter=nter, jver_off=jver_off, jter=jter - 1, tables=tabs, ka=ka, kb=kb,
+                        kc=kc, bweight=bweight, ecor=ecor)
+    print(f"setup {name}: N={N} nver={nver} ninter={ninter} nsplin={nsp} nktot={nk} -> {out}")
+
+
 MD_CASES = {"cfg1": 100}   # configuration 1: 100 velocity-Verlet steps at 0.25 fs
 
 
@@ -218,11 +262,14 @@ def run_md_case(name, nsteps):
 
 
 def main():
-    names = sys.argv[1:] or CASES + ["md"]
+    names = sys.argv[1:] or CASES + ["md", "setup"]
     for n in names:
         if n == "md":
             for m, k in MD_CASES.items():
                 run_md_case(m, k)
+        elif n == "setup":
+            for m in SETUP_CASES:
+                run_setup_case(m)
         else:
             run_case(n)
 
