@@ -11,6 +11,7 @@
  * usage: piny_ref <input file> <positions.bin> <out.bin> [repeat]
  *        piny_ref <input file> <positions.bin> <out.bin> md <nsteps> <velocities.bin>
  *        piny_ref <input file> <positions.bin> <out.bin> dump <setup.bin>
+ *        piny_ref <input file> <positions.bin> <out.bin> recip
  *   positions.bin: P*N*3 doubles, Bohr, [bead][site][x,y,z] (ghost sites
  *                  are recomputed by get_ghost_pos)
  *   out.bin:       header {N, P, nver} as doubles, then 24 energy/virial
@@ -32,6 +33,9 @@
  *                  inter_map_index), int32 kastr, kbstr, kcstr [nktot],
  *                  float64 bweight_tot [nktot] (PME only, else zeros),
  *                  float64 ecor cv0, cdv0 [ecor_nsplin]
+ *   recip:         also the reference's debug driver's reciprocal-space
+ *                  check (test_force with inter = 3, test_energy.c:163-208,
+ *                  489-600) on bead 1, printed as "RECIP <energy>"
  *   md:            P = 1; after the first force call the velocities
  *                  (N*3 doubles, Bohr per atomic time unit) are set and
  *                  int_NVE (md/int_NVE.c) runs nsteps times with the
@@ -54,6 +58,9 @@
 #include "../proto_defs/proto_pimd_entry.h"
 #include "../proto_defs/proto_integrate_md_entry.h"
 
+/* the debug driver's per-term energy (src/energy/control/test_energy.c:489) */
+void test_force(CLASS *, BONDED *, GENERAL_DATA *, int, double *, double *, double *);
+
 static CLASS cls;
 static BONDED bonded;
 static GENERAL_DATA gd;
@@ -63,12 +70,13 @@ static ANALYSIS analysis;
 int main(int argc, char *argv[]) {
   const int md = argc == 7 && strcmp(argv[4], "md") == 0;
   const int dump = argc == 6 && strcmp(argv[4], "dump") == 0;
+  const int recip = argc == 5 && strcmp(argv[4], "recip") == 0;
   if (argc != 4 && argc != 5 && !md && !dump) {
     fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin> [repeat | md nsteps velocities.bin]\n",
             argv[0]);
     return 2;
   }
-  const int repeat = argc == 5 ? atoi(argv[4]) : 0;
+  const int repeat = (argc == 5 && !recip) ? atoi(argv[4]) : 0;
   const char *dump_file = dump ? argv[5] : NULL;
   int zero = 0;
   char **noargs = NULL;
@@ -161,6 +169,13 @@ int main(int argc, char *argv[]) {
     }
   }
 
+  if (recip) {
+    double vg = 0.0, vg2 = 0.0, vg3 = 0.0;
+    get_ghost_pos(&(cls.clatoms_info), &(cls.clatoms_pos[1]), &(cls.ghost_atoms));
+    test_force(&cls, &bonded, &gd, 3, &vg, &vg2, &vg3);
+    printf("RECIP %.13g\n", vg);
+    fflush(stdout);
+  }
   if (dump_file) {
     VERLIST *vl = &cls.nbr_list.verlist;
     INTERACT *it = &cls.interact;

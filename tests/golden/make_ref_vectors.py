@@ -218,6 +218,33 @@ def run_setup_case(name):
     print(f"setup {name}: N={N} nver={nver} ninter={ninter} nsplin={nsp} nktot={nk} -> {out}")
 
 
+def run_w512_debug():
+    """The reference's own debug check of the reciprocal-space sum
+    (test_force with inter = 3, test_energy.c:163-208, as the interactive
+    debug driver that wrote examples/q-TIP4P-F/Ewald-log.txt runs it) on the
+    shipped W512 coordinates, for the five settings of the log: standard
+    Ewald with kmax 13 and 26, PME of order 8 with kmax_pme 17, 22, 25."""
+    import dataclasses
+    kat = np.load(os.path.join(HERE, "w512_ewald_kat.npz"))
+    vals = []
+    for pme, kmax, kpme, order in zip(kat["use_pme"], kat["kmax_ewd"], kat["kmax_pme"], kat["order"]):
+        c = cases.get("w512" if pme else "w512_ewald")
+        st = dataclasses.replace(c.system.settings, kmax_ewd=int(kmax), kmax_pme=int(kpme) if pme else 25,
+                                 n_interp=int(order) if pme else 8)
+        s = w.make_system(w.Q_TIP4P_F, 512, float(kat["box_A"]), 1, st)
+        with tempfile.TemporaryDirectory() as d:
+            write_inputs(d, w.Config(s, c.x, c.y, c.z))
+            r = subprocess.run([EXE, "water.input", "pos.bin", "out.bin", "recip"], cwd=d, capture_output=True,
+                               text=True, timeout=600)
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("RECIP")]
+        if r.returncode != 0 or not line:
+            raise RuntimeError("w512 debug run failed")
+        vals.append(float(line[0].split()[1]))
+    out = os.path.join(HERE, "w512_debug_recip.npz")
+    np.savez(out, labels=kat["labels"], energy=np.array(vals))
+    print("w512 debug recip", vals, "->", out)
+
+
 MD_CASES = {"cfg1": 100}   # configuration 1: 100 velocity-Verlet steps at 0.25 fs
 
 
@@ -262,7 +289,7 @@ def run_md_case(name, nsteps):
 
 
 def main():
-    names = sys.argv[1:] or CASES + ["md", "setup"]
+    names = sys.argv[1:] or CASES + ["md", "setup", "w512_debug"]
     for n in names:
         if n == "md":
             for m, k in MD_CASES.items():
@@ -270,6 +297,8 @@ def main():
         elif n == "setup":
             for m in SETUP_CASES:
                 run_setup_case(m)
+        elif n == "w512_debug":
+            run_w512_debug()
         else:
             run_case(n)
 

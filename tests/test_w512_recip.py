@@ -4,12 +4,17 @@ molecules, L = 24.832 A, ewald_alpha = 10, the inputs of
 examples/q-TIP4P-F/Ewald-log.txt).
 
 The log's energies (8.7214 Ha) are not reproduced from these inputs: the
-direct Ewald sum of the shipped coordinates with the shipped charges is
-0.18534 Ha, by the oracle and by the independent numpy sum below alike.
-What the fixture pins is therefore (1) the oracle's standard-Ewald sum
-against an independent implementation of the same sphere sum, and (2) the
-PME sum's convergence to it with grid size at order 8, the settings the
-log varies (kmax_pme 17, 22, 25)."""
+reference's own debug check of the reciprocal sum (test_force with
+inter = 3, test_energy.c:163-208, the routine that prints the log's lines),
+run on the shipped coordinates with the five settings of the log, gives
+0.18534 Ha for each (tests/golden/w512_debug_recip.npz, written by
+tests/golden/make_ref_vectors.py w512_debug), and so do the oracle and the
+independent numpy sum below.  The log therefore comes from other inputs.
+Pinned here: (1) the oracle's k-space sum against the reference's debug
+values for all five settings; (2) the oracle's standard-Ewald sum against an
+independent implementation of the same sphere sum; (3) the PME sum's
+convergence to it with grid size at order 8, the settings the log varies
+(kmax_pme 17, 22, 25)."""
 import os
 
 import numpy as np
@@ -22,11 +27,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BOHR = 0.529177
 
 
-def _w512(kmax_ewd=13, kmax_pme=25, order=8):
+def _w512(kmax_ewd=13, kmax_pme=25, order=8, pme_on=1):
     d = np.load(os.path.join(HERE, "golden", "w512_ewald_kat.npz"))
     L = float(d["box_A"])
     st = w.Settings(cutoff=9.0, alpha=float(d["ewald_alpha"]) / L, kmax_ewd=kmax_ewd, kmax_pme=kmax_pme,
-                    n_interp=order)
+                    n_interp=order, pme_on=pme_on)
     s = w.make_system(w.Q_TIP4P_F, 512, L, 1, st)
     names = [str(n) for n in d["names"]]
     assert names[:4] == ["O", "H", "H", "M"] and len(names) == s.natm
@@ -69,3 +74,16 @@ def test_pme_converges_to_ewald(kmax_pme):
     assert [int(v) for v in o.scalars()[10:13]] == [{17: 40, 22: 48, 25: 60}[kmax_pme]] * 3
     bound = {17: 5e-6, 22: 6e-7, 25: 1e-7}[kmax_pme]
     assert e_pme < e_ew and rel < bound, rel
+
+
+def test_recip_matches_reference_debug_driver():
+    kat = np.load(os.path.join(HERE, "golden", "w512_ewald_kat.npz"))
+    ref = np.load(os.path.join(HERE, "golden", "w512_debug_recip.npz"))
+    for pme, kmax, kpme, order, e_ref in zip(kat["use_pme"], kat["kmax_ewd"], kat["kmax_pme"], kat["order"],
+                                             ref["energy"]):
+        s, x, y, z, names, L, alp = _w512(kmax_ewd=int(kmax), kmax_pme=int(kpme) if pme else 25,
+                                          order=int(order) if pme else 8, pme_on=int(pme))
+        o = Oracle(s, x, y, z)
+        _, _, (x, y, z) = o.energy_control(x, y, z, 0)  # M sites placed by the ghost rule, as get_ghost_pos
+        e = o.recip_energy(x, y, z, use_pme=bool(pme))[0]
+        assert abs(e - e_ref) <= 1e-12 * abs(e_ref), (int(pme), int(kmax), int(kpme), e, e_ref)
