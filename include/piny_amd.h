@@ -148,6 +148,20 @@ typedef struct pmx_params {
   int pair_eval;        /* PMX_PAIR_* (FP64 only)                          */
 } pmx_params;
 
+/* Settings of the PIMD NVT integrator (int_NVT_pimd with RESPA off; PINY
+ * input keywords in brackets).  The centroid mode (bead 1 with staging) is
+ * coupled to one Nose-Hoover chain over every real site (mol_opt_nhc
+ * global); every other mode's every degree of freedom to a chain of its own
+ * (set_atm_NHC.c:330-790). */
+typedef struct pmx_nhc_params {
+  double dt;            /* [time_step] in atomic time units (fs / TIME_CONV) */
+  double tau_nhc;       /* [atm_nhc_tau_def] in atomic time units           */
+  int len_nhc;          /* [atm_nhc_len] chain length, 1..7                 */
+  int nres_nhc;         /* [respa_steps_nhc]                                */
+  int nyosh_nhc;        /* [yosh_steps_nhc] 1, 3 or 5                       */
+  double gamma_adb;     /* [path_int_gamma_adb] adiabaticity (mass factor) */
+} pmx_nhc_params;
+
 /* Energies (Hartree) and pressure tensors (Hartree), stat_avg / ptens. */
 typedef struct pmx_energies {
   double vintert;       /* inter: vreal + vlong + vrecip+vself+vbgr+vecor  */

@@ -2001,3 +2001,247 @@ void or_bend_coef_harm(double fk, double th0, double *c7, double *s7) {
   s7[5] = 0.0;
   s7[6] = 0.5 * fk * (-cn9 * s6);
 }
+
+/* ------------------------------------------------------------------------
+ * PIMD in the NVT ensemble: int_NVT_pimd (pimd/int_NVT_pimd.c:36-380) with
+ * RESPA off, i.e. one pass of int_0_to_dt2_nvt_pimd (int_0_to_dt2_pimd.c:
+ * 34-196), energy_control_pimd and int_dt2_to_dt_nvt_pimd
+ * (int_dt2_to_dt_pimd.c:31-172) per step.  Velocities are mode velocities
+ * (bead-major [P*N]); positions stay bead Cartesian between steps.
+ * Thermostats as set_atm_NHC (set_atm_NHC.c:330-790) sets them for
+ * mol_opt_nhc = global and path-integral beads: mode 1 (the centroid, or
+ * bead 1 with staging) on one chain for every real site with the physical
+ * masses, gkt = 3 N_real (no constraints) times kT; modes 2..P each degree
+ * of freedom of each real site on its own chain with the mode masses.
+ * Ghost sites sit on the null thermostat.
+ *
+ * NHC state layout: class chain v[len], x[len]; then for mode ip = 1..P-1
+ * (0-based) and chain k = 3 i + d: v[len], x[len] at
+ * 2 len + ((ip - 1) 3N + k) 2 len.
+ */
+typedef struct {
+  int len, nres, nyosh;
+  double wdti2[10], wdti4[10], wdti8[10];
+  double cmass[8], cgkt[8];  /* class chain */
+  double bmass[8], bgkt[8];  /* bead chains (the same for every chain) */
+} or_nhc;
+
+/* set_yosh, int_utils.c:244-318 */
+static int or_set_yosh(int nyosh, double dti, or_nhc *h) {
+  double w[10] = {0};
+  if (nyosh == 1) {
+    w[1] = 1.0;
+  } else if (nyosh == 3) {
+    double onethird = 1.0 / 3.0, temp = pow(2.0, onethird);
+    w[1] = 1.0 / (2.0 - temp);
+    w[2] = -temp / (2.0 - temp);
+    w[3] = 1.0 / (2.0 - temp);
+  } else if (nyosh == 5) {
+    double onethird = 1.0 / 3.0, temp = pow(4.0, onethird), p2 = 1.0 / (4.0 - temp);
+    w[1] = p2;
+    w[2] = p2;
+    w[3] = 1.0 - 4.0 * p2;
+    w[4] = p2;
+    w[5] = p2;
+  } else {
+    return PMX_ERR_NOTIMPL;
+  }
+  for (int i = 1; i <= nyosh; i++) {
+    double wd = w[i] * dti;
+    h->wdti2[i] = wd / 2.0;
+    h->wdti4[i] = wd / 4.0;
+    h->wdti8[i] = wd / 8.0;
+  }
+  return PMX_OK;
+}
+
+/* one thermostat chain of apply_NHC_par (md/int_NVT.c:144-330): kin is the
+ * sum of m v^2 over the chain's degrees of freedom; returns the factor the
+ * velocities are scaled by */
+static double or_nhc_chain(const or_nhc *h, const double *mass, const double *gkt, double *v, double *x,
+                           double kin) {
+  int L = h->len;
+  double f[8];
+  double sc = 1.0;
+  f[1] = (kin - gkt[1]) / mass[1];
+  for (int ic = 1; ic <= L - 1; ic++) f[ic + 1] = (mass[ic] * v[ic] * v[ic] - gkt[ic + 1]) / mass[ic + 1];
+  for (int ir = 1; ir <= h->nres; ir++) {
+    for (int iy = 1; iy <= h->nyosh; iy++) {
+      v[L] += f[L] * h->wdti4[iy];
+      for (int ic = 1; ic <= L - 1; ic++) {
+        double aa = exp(-h->wdti8[iy] * v[L + 1 - ic]);
+        v[L - ic] = v[L - ic] * aa * aa + h->wdti4[iy] * f[L - ic] * aa;
+      }
+      double aa = exp(-h->wdti2[iy] * v[1]);
+      sc *= aa;
+      kin *= aa * aa;
+      for (int ic = 1; ic <= L; ic++) x[ic] += v[ic] * h->wdti2[iy];
+      f[1] = (kin - gkt[1]) / mass[1];
+      for (int ic = 1; ic <= L - 1; ic++) {
+        double a2 = exp(-h->wdti8[iy] * v[ic + 1]);
+        v[ic] = v[ic] * a2 * a2 + h->wdti4[iy] * f[ic] * a2;
+        f[ic + 1] = (mass[ic] * v[ic] * v[ic] - gkt[ic + 1]) / mass[ic + 1];
+      }
+      v[L] += f[L] * h->wdti4[iy];
+    }
+  }
+  return sc;
+}
+
+/* veig_ip of the mode masses of clatoms_pos[ip].mass (transform_cnt.c:
+ * 175-233), with veig_1 = 1 */
+static double or_mode_veig(const or_ctx *c, int ip) {
+  int P = c->P;
+  double dP = (double)P;
+  double v;
+  if (ip == 1) return 1.0;
+  if (c->par.pimd_typ == PMX_PIMD_STAGING) {
+    v = ((double)(ip)) / ((double)(ip - 1));
+  } else if (ip == P) {
+    v = 4.0 * dP;
+  } else {
+    int k = ip / 2 + 1; /* veig[2k-2] = veig[2k-1] for k = 2..P/2 */
+    v = (1.0 - cos(2.0 * M_PI / dP * (double)(k - 1))) * 4.0 * dP;
+  }
+  return v;
+}
+
+static void or_apply_nhc(const or_ctx *c, const or_nhc *h, const double *mm, double *vx, double *vy, double *vz,
+                         double *st, const int *ghost) {
+  int N = c->N, P = c->P, L = h->len;
+  /* mode 1: one chain over the real sites (x, then y, then z, in site order) */
+  double kin = 0.0;
+  double *vv[3] = {vx, vy, vz};
+  for (int d = 0; d < 3; d++)
+    for (int i = 0; i < N; i++)
+      if (!ghost[i]) kin += c->mass[i] * vv[d][i] * vv[d][i];
+  double sc = or_nhc_chain(h, h->cmass, h->cgkt, st - 1, st + L - 1, kin);
+  for (int d = 0; d < 3; d++)
+    for (int i = 0; i < N; i++)
+      if (!ghost[i]) vv[d][i] *= sc;
+  /* modes 2..P: one chain per degree of freedom */
+  for (int ip = 1; ip < P; ip++)
+    for (int d = 0; d < 3; d++)
+      for (int i = 0; i < N; i++) {
+        if (ghost[i]) continue;
+        size_t t = (size_t)ip * N + i;
+        double m = mm[t];
+        double *s = st + 2 * L + ((size_t)(ip - 1) * 3 * N + 3 * i + d) * 2 * L;
+        double sk = or_nhc_chain(h, h->bmass, h->bgkt, s - 1, s + L - 1, m * vv[d][t] * vv[d][t]);
+        vv[d][t] *= sk;
+      }
+}
+
+int or_pimd_nvt(or_ctx *c, const pmx_nhc_params *np, int nsteps, double *x, double *y, double *z, double *vx,
+                double *vy, double *vz, double *nhc, pmx_energies *out, double *econs) {
+  int N = c->N, P = c->P;
+  if (P < 2 || !np || np->len_nhc < 1 || np->len_nhc > 7 || np->nres_nhc < 1) return PMX_ERR_ARG;
+  double T = c->par.temperature > 0 ? c->par.temperature : 300.0;
+  or_nhc h;
+  memset(&h, 0, sizeof h);
+  h.len = np->len_nhc;
+  h.nres = np->nres_nhc;
+  h.nyosh = np->nyosh_nhc;
+  if (or_set_yosh(h.nyosh, np->dt / (double)h.nres, &h)) return PMX_ERR_NOTIMPL;
+  int *ghost = calloc(N, sizeof(int));
+  int nreal = N;
+  for (int g = 0; g < c->nghost; g++) {
+    ghost[c->gatom[g]] = 1;
+    nreal--;
+  }
+  /* set_atm_NHC.c:355-370 (global), 510-547 (NHC masses), 715-755 (beads) */
+  double text = T, tau = np->tau_nhc;
+  double m1 = tau * tau * text, g1 = 3.0 * (double)nreal;
+  for (int j = 2; j <= h.len; j++) {
+    h.cgkt[j] = text / OR_BOLTZ;
+    h.cmass[j] = m1 / OR_BOLTZ;
+  }
+  h.cmass[1] = m1 * g1 / OR_BOLTZ;
+  h.cgkt[1] = g1 * text / OR_BOLTZ;
+  double pre = (OR_BOLTZ * OR_BOLTZ * np->gamma_adb * np->gamma_adb) / ((double)P);
+  double bm1 = pre / (text * text);
+  for (int j = 2; j <= h.len; j++) {
+    h.bgkt[j] = text / OR_BOLTZ;
+    h.bmass[j] = bm1 * text / OR_BOLTZ;
+  }
+  h.bmass[1] = bm1 * text / OR_BOLTZ;
+  h.bgkt[1] = text / OR_BOLTZ;
+  /* mode masses */
+  size_t NP = (size_t)N * P;
+  double *mm = malloc(sizeof(double) * NP);
+  for (int ip = 0; ip < P; ip++)
+    for (int i = 0; i < N; i++) {
+      /* m veig for mode 1, m veig gamma^2 for the others */
+      double m = c->mass[i] * or_mode_veig(c, ip + 1);
+      mm[(size_t)ip * N + i] = ip == 0 ? m : m * np->gamma_adb * np->gamma_adb;
+    }
+  double *fx = calloc(NP, sizeof(double)), *fy = calloc(NP, sizeof(double)), *fz = calloc(NP, sizeof(double));
+  double *xm = malloc(sizeof(double) * N), *ym = malloc(sizeof(double) * N), *zm = malloc(sizeof(double) * N);
+  double *a = malloc(sizeof(double) * P), *b = malloc(sizeof(double) * P);
+  const int stag = c->par.pimd_typ == PMX_PIMD_STAGING;
+  unsigned flags = PMX_EVAL_PIMD | PMX_EVAL_MODE_FORCES;
+  int rc = or_energy_control(c, x, y, z, NULL, NULL, NULL, flags, fx, fy, fz, out);
+  double dt = np->dt, dt2 = dt / 2.0;
+  for (int step = 0; step < nsteps && !rc; step++) {
+    or_apply_nhc(c, &h, mm, vx, vy, vz, nhc, ghost);
+    for (size_t t = 0; t < NP; t++) {
+      vx[t] += fx[t] * dt2 / mm[t];
+      vy[t] += fy[t] * dt2 / mm[t];
+      vz[t] += fz[t] * dt2 / mm[t];
+    }
+    /* positions to modes, drift, back (mode 1 becomes xmod) */
+    double *pos[3] = {x, y, z}, *vel[3] = {vx, vy, vz}, *md[3] = {xm, ym, zm};
+    for (int d = 0; d < 3; d++)
+      for (int i = 0; i < N; i++) {
+        for (int ip = 0; ip < P; ip++) a[ip] = pos[d][(size_t)ip * N + i];
+        if (stag)
+          or_stag_pos_to_mode(P, a, b);
+        else
+          or_nm_pos_to_mode(P, a, b);
+        for (int ip = 0; ip < P; ip++) b[ip] += vel[d][(size_t)ip * N + i] * dt;
+        md[d][i] = b[0];
+        if (stag)
+          or_stag_mode_to_pos(P, b, a);
+        else
+          or_nm_mode_to_pos(P, b, a);
+        for (int ip = 0; ip < P; ip++) pos[d][(size_t)ip * N + i] = a[ip];
+      }
+    rc = or_energy_control(c, x, y, z, xm, ym, zm, flags, fx, fy, fz, out);
+    for (size_t t = 0; t < NP; t++) {
+      vx[t] += fx[t] * dt2 / mm[t];
+      vy[t] += fy[t] * dt2 / mm[t];
+      vz[t] += fz[t] * dt2 / mm[t];
+    }
+    or_apply_nhc(c, &h, mm, vx, vy, vz, nhc, ghost);
+  }
+  if (econs && !rc) {
+    /* get_tvten_pimd kinetic energy and the chains' energies */
+    double ke = 0.0, en = 0.0;
+    for (size_t t = 0; t < NP; t++) ke += 0.5 * mm[t] * (vx[t] * vx[t] + vy[t] * vy[t] + vz[t] * vz[t]);
+    int L = h.len;
+    for (int j = 1; j <= L; j++) en += 0.5 * h.cmass[j] * nhc[j - 1] * nhc[j - 1] + h.cgkt[j] * nhc[L + j - 1];
+    for (int ip = 1; ip < P; ip++)
+      for (int i = 0; i < N; i++) {
+        if (ghost[i]) continue;
+        for (int d = 0; d < 3; d++) {
+          const double *s = nhc + 2 * L + ((size_t)(ip - 1) * 3 * N + 3 * i + d) * 2 * L;
+          for (int j = 1; j <= L; j++) en += 0.5 * h.bmass[j] * s[j - 1] * s[j - 1] + h.bgkt[j] * s[L + j - 1];
+        }
+      }
+    econs[0] = ke;
+    econs[1] = en;
+    econs[2] = out->vintert + out->vintrat + out->kin_harm;
+    econs[3] = econs[0] + econs[1] + econs[2];
+  }
+  free(ghost);
+  free(mm);
+  free(fx);
+  free(fy);
+  free(fz);
+  free(xm);
+  free(ym);
+  free(zm);
+  free(a);
+  free(b);
+  return rc;
+}

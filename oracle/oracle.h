@@ -68,6 +68,16 @@ void or_stag_pos_to_mode(int P, const double *x, double *u);
 void or_stag_mode_to_pos(int P, const double *u, double *x);
 void or_stag_force_to_mode(int P, const double *f, double *fm);
 
+/* nsteps of PIMD in the NVT ensemble (int_NVT_pimd with RESPA off and
+ * global / per-degree-of-freedom Nose-Hoover chains), starting with a force
+ * call at the given positions.  x, y, z: bead Cartesian [P*N], in/out;
+ * vx, vy, vz: mode velocities [P*N], in/out; nhc: chain state, in/out
+ * (2 len for the centroid chain, then 2 len per mode 2..P and degree of
+ * freedom); econs[4] (may be NULL): kinetic, chain energy, potential with
+ * springs, their sum. */
+int or_pimd_nvt(or_ctx *c, const pmx_nhc_params *np, int nsteps, double *x, double *y, double *z, double *vx,
+                double *vy, double *vz, double *nhc, pmx_energies *out, double *econs);
+
 /* Gauss-Legendre nodes and weights on [-1,1] (n points). */
 void or_gauss_legendre(int n, double *node, double *weight);
 

@@ -50,6 +50,8 @@ def lib():
                    "or_stag_mode_to_pos", "or_stag_force_to_mode"):
             getattr(L, nm).argtypes = [C.c_int, dp, dp]
         L.or_gauss_legendre.argtypes = [C.c_int, dp, dp]
+        L.or_pimd_nvt.argtypes = [C.c_void_p, C.POINTER(abi.PmxNhcParams), C.c_int, dp, dp, dp, dp, dp, dp, dp,
+                                  C.POINTER(abi.PmxEnergies), dp]
         L.or_bond_coef_morse.argtypes = [C.c_double, C.c_double, dp]
         L.or_bond_coef_harm.argtypes = [C.c_double, dp]
         L.or_bend_coef_harm.argtypes = [C.c_double, C.c_double, dp, dp]
@@ -100,6 +102,28 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"or_energy_control failed: {rc}")
         return e.as_dict(), np.stack([fx, fy, fz], 1).reshape(P, N, 3), (x, y, z)
+
+    def pimd_nvt(self, nsteps, x, y, z, v, nhc=None, params=None):
+        """nsteps of PIMD NVT (or_pimd_nvt) from bead positions x, y, z [P*N]
+        and mode velocities v [P, N, 3]; returns (positions [P, N, 3], mode
+        velocities [P, N, 3], chain state, energies of the last force call,
+        econs = [kinetic, chain energy, potential with springs, total])."""
+        P, N = self.P, self.N
+        params = params or abi.nhc_params()
+        x, y, z = (np.array(a, dtype=np.float64) for a in (x, y, z))
+        v = np.asarray(v, dtype=np.float64).reshape(P, N, 3)
+        vx, vy, vz = (np.ascontiguousarray(v[:, :, d].reshape(-1)) for d in range(3))
+        st = (np.zeros(abi.nhc_state_size(N, P, params.len_nhc)) if nhc is None
+              else np.array(nhc, dtype=np.float64))
+        e = abi.PmxEnergies()
+        ec = np.zeros(4)
+        rc = lib().or_pimd_nvt(self.h, C.byref(params), int(nsteps), abi.dptr(x), abi.dptr(y), abi.dptr(z),
+                               abi.dptr(vx), abi.dptr(vy), abi.dptr(vz), abi.dptr(st), C.byref(e), abi.dptr(ec))
+        if rc != 0:
+            raise RuntimeError(f"or_pimd_nvt failed: {rc}")
+        pos = np.stack([x, y, z], 1).reshape(P, N, 3)
+        vel = np.stack([vx, vy, vz], 1).reshape(P, N, 3)
+        return pos, vel, st, e.as_dict(), ec
 
     def verlet_list(self):
         N = self.N

@@ -12,6 +12,7 @@
  *        piny_ref <input file> <positions.bin> <out.bin> md <nsteps> <velocities.bin>
  *        piny_ref <input file> <positions.bin> <out.bin> dump <setup.bin>
  *        piny_ref <input file> <positions.bin> <out.bin> recip
+ *        piny_ref <input file> <positions.bin> <out.bin> pimd <nsteps> <velocities.bin>
  *   positions.bin: P*N*3 doubles, Bohr, [bead][site][x,y,z] (ghost sites
  *                  are recomputed by get_ghost_pos)
  *   out.bin:       header {N, P, nver} as doubles, then 24 energy/virial
@@ -36,6 +37,17 @@
  *   recip:         also the reference's debug driver's reciprocal-space
  *                  check (test_force with inter = 3, test_energy.c:163-208,
  *                  489-600) on bead 1, printed as "RECIP <energy>"
+ *   pimd:          P > 1; after the first force call the mode velocities
+ *                  (P*N*3 doubles, [bead][site][3], Bohr per atomic time
+ *                  unit) are set, every Nose-Hoover chain position and
+ *                  velocity is zeroed, and int_NVT_pimd runs nsteps times;
+ *                  out.bin then holds {N, P, nver, nsteps, dt, len and
+ *                  number of chains of the centroid and of the bead
+ *                  thermostats}, {vintert, vintrat, kin_harm, pi_ke_prim,
+ *                  pi_ke_vir}, the bead positions and the mode velocities
+ *                  [bead][site][3], the centroid chain's v[len], x[len], and
+ *                  for beads 2..P, sites and x, y, z the chain's v[len],
+ *                  x[len] (zeros for ghost sites)
  *   md:            P = 1; after the first force call the velocities
  *                  (N*3 doubles, Bohr per atomic time unit) are set and
  *                  int_NVE (md/int_NVE.c) runs nsteps times with the
@@ -71,7 +83,8 @@ int main(int argc, char *argv[]) {
   const int md = argc == 7 && strcmp(argv[4], "md") == 0;
   const int dump = argc == 6 && strcmp(argv[4], "dump") == 0;
   const int recip = argc == 5 && strcmp(argv[4], "recip") == 0;
-  if (argc != 4 && argc != 5 && !md && !dump) {
+  const int pimd = argc == 7 && strcmp(argv[4], "pimd") == 0;
+  if (argc != 4 && argc != 5 && !md && !dump && !pimd) {
     fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin> [repeat | md nsteps velocities.bin]\n",
             argv[0]);
     return 2;
@@ -126,6 +139,68 @@ int main(int argc, char *argv[]) {
     control_pimd_trans_pos(&cls, &gd);
     for (int ip = 1; ip <= P; ip++) get_ghost_pos(&(cls.clatoms_info), &(cls.clatoms_pos[ip]), &(cls.ghost_atoms));
     energy_control_pimd(&cls, &bonded, &gd);
+    if (pimd) {
+      const int nsteps = atoi(argv[5]);
+      FILE *fv = fopen(argv[6], "rb");
+      if (!fv) return 1;
+      double *vel = (double *)malloc(sizeof(double) * 3 * (size_t)N * P);
+      if (fread(vel, sizeof(double), 3 * (size_t)N * P, fv) != 3 * (size_t)N * P) return 1;
+      fclose(fv);
+      for (int ip = 1; ip <= P; ip++)
+        for (int i = 1; i <= N; i++) {
+          const double *v = vel + 3 * ((size_t)(ip - 1) * N + (i - 1));
+          cls.clatoms_pos[ip].vx[i] = v[0];
+          cls.clatoms_pos[ip].vy[i] = v[1];
+          cls.clatoms_pos[ip].vz[i] = v[2];
+        }
+      free(vel);
+      THERM_INFO *tc = &cls.therm_info_class, *tb = &cls.therm_info_bead;
+      for (int j = 1; j <= tc->len_nhc; j++)
+        for (int k = 1; k <= tc->num_nhc; k++) cls.therm_class.v_nhc[j][k] = cls.therm_class.x_nhc[j][k] = 0.0;
+      for (int ip = 2; ip <= P; ip++)
+        for (int j = 1; j <= tb->len_nhc; j++)
+          for (int k = 1; k <= tb->num_nhc; k++) cls.therm_bead[ip].v_nhc[j][k] = cls.therm_bead[ip].x_nhc[j][k] = 0.0;
+      for (int st = 0; st < nsteps; st++) int_NVT_pimd(&cls, &bonded, &gd, &analysis);
+      STAT_AVG *sa = &gd.stat_avg;
+      double hdr[9] = {(double)N, (double)P, (double)cls.nbr_list.verlist.nver_lst_now, (double)nsteps,
+                       gd.timeinfo.dt, (double)tc->len_nhc, (double)tc->num_nhc, (double)tb->len_nhc,
+                       (double)tb->num_nhc};
+      double e[5] = {sa->vintert, sa->vintrat, sa->kin_harm, sa->pi_ke_prim, sa->pi_ke_vir};
+      FILE *fo = fopen(argv[3], "wb");
+      if (!fo) return 1;
+      fwrite(hdr, sizeof(double), 9, fo);
+      fwrite(e, sizeof(double), 5, fo);
+      for (int ip = 1; ip <= P; ip++)
+        for (int i = 1; i <= N; i++) {
+          double v[3] = {cls.clatoms_pos[ip].x[i], cls.clatoms_pos[ip].y[i], cls.clatoms_pos[ip].z[i]};
+          fwrite(v, sizeof(double), 3, fo);
+        }
+      for (int ip = 1; ip <= P; ip++)
+        for (int i = 1; i <= N; i++) {
+          double v[3] = {cls.clatoms_pos[ip].vx[i], cls.clatoms_pos[ip].vy[i], cls.clatoms_pos[ip].vz[i]};
+          fwrite(v, sizeof(double), 3, fo);
+        }
+      for (int j = 1; j <= tc->len_nhc; j++) fwrite(&cls.therm_class.v_nhc[j][1], sizeof(double), 1, fo);
+      for (int j = 1; j <= tc->len_nhc; j++) fwrite(&cls.therm_class.x_nhc[j][1], sizeof(double), 1, fo);
+      for (int ip = 2; ip <= P; ip++)
+        for (int i = 1; i <= N; i++) {
+          const int *maps[3] = {tb->inhc_x, tb->inhc_y, tb->inhc_z};
+          for (int d = 0; d < 3; d++) {
+            const int k = maps[d][i];
+            for (int j = 1; j <= tb->len_nhc; j++) {
+              double v = (k >= 1 && k <= tb->num_nhc) ? cls.therm_bead[ip].v_nhc[j][k] : 0.0;
+              fwrite(&v, sizeof(double), 1, fo);
+            }
+            for (int j = 1; j <= tb->len_nhc; j++) {
+              double v = (k >= 1 && k <= tb->num_nhc) ? cls.therm_bead[ip].x_nhc[j][k] : 0.0;
+              fwrite(&v, sizeof(double), 1, fo);
+            }
+          }
+        }
+      fclose(fo);
+      free(pos);
+      return 0;
+    }
   } else {
     energy_control(&cls, &bonded, &gd);
     if (md) {

@@ -48,6 +48,30 @@ class PmxParams(C.Structure):
     ]
 
 
+class PmxNhcParams(C.Structure):
+    """pmx_nhc_params: the PIMD NVT integrator's settings (atomic units)."""
+    _fields_ = [
+        ("dt", C.c_double), ("tau_nhc", C.c_double), ("len_nhc", C.c_int), ("nres_nhc", C.c_int),
+        ("nyosh_nhc", C.c_int), ("gamma_adb", C.c_double),
+    ]
+
+
+TIME_CONV = 0.0241888  # fs per atomic time unit (defines.h:85)
+
+
+def nhc_params(dt_fs=0.25, tau_fs=20.0, len_nhc=4, nres_nhc=4, nyosh_nhc=3, gamma_adb=1.0):
+    """The settings of PINY's PIMD inputs (time_step, atm_nhc_tau_def in fs,
+    atm_nhc_len, respa_steps_nhc, yosh_steps_nhc, path_int_gamma_adb)."""
+    return PmxNhcParams(dt=dt_fs / TIME_CONV, tau_nhc=tau_fs / TIME_CONV, len_nhc=len_nhc, nres_nhc=nres_nhc,
+                        nyosh_nhc=nyosh_nhc, gamma_adb=gamma_adb)
+
+
+def nhc_state_size(N, P, len_nhc):
+    """Doubles of the chain state: the centroid chain, then one chain per
+    degree of freedom of modes 2..P (v[len], x[len] each)."""
+    return 2 * len_nhc * (1 + 3 * N * (P - 1))
+
+
 class PmxEnergies(C.Structure):
     _fields_ = [
         ("vintert", C.c_double), ("vintrat", C.c_double),

@@ -245,6 +245,71 @@ def run_w512_debug():
     print("w512 debug recip", vals, "->", out)
 
 
+# PIMD NVT trajectories of the reference's own int_NVT_pimd (20 steps of
+# 0.25 fs, chains of length 4 with 4 RESPA steps and 3 Yoshida weights,
+# tau 20 fs, from zeroed chains), normal modes and staging
+PIMD_CASES = {"tip64_p4": 20, "tip64_p4_stag": 20, "cfg2_p4": 20}
+
+
+def pimd_velocities(c, seed):
+    """Seeded mode velocities at the case temperature: Gaussian with the mode
+    masses m veig (transform_cnt.c:175-233, gamma_adb = 1), zero on ghost
+    sites, [P, N, 3] in Bohr per atomic time unit."""
+    s = c.system
+    P, N = s.pi_beads, s.natm
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kt = s.settings.temperature / w.BOLTZ
+    m = np.asarray(s.mass, dtype=np.float64)
+    veig = np.ones(P)
+    if s.settings.pimd_typ == 1:
+        veig[1:] = [ip / (ip - 1.0) for ip in range(2, P + 1)]
+    else:
+        veig[P - 1] = 4.0 * P
+        for ip in range(2, P // 2 + 1):
+            veig[2 * ip - 3] = veig[2 * ip - 2] = (1.0 - np.cos(2 * np.pi / P * (ip - 1))) * 4.0 * P
+    v = rng.normal(size=(P, N, 3)) * np.sqrt(kt / (m[None, :] * veig[:, None]))[:, :, None]
+    v[:, np.asarray(s.ghost_atom, dtype=int)] = 0.0
+    return v
+
+
+def run_pimd_case(name, nsteps):
+    c = cases.get(name)
+    s = c.system
+    P, N = s.pi_beads, s.natm
+    v0 = pimd_velocities(c, w.SEED_BASE + 200)
+    with tempfile.TemporaryDirectory() as d:
+        write_inputs(d, c)
+        v0.astype(np.float64).tofile(os.path.join(d, "vel.bin"))
+        with open(os.path.join(d, "stdout.txt"), "w") as log:
+            r = subprocess.run([EXE, "water.input", "pos.bin", "out.bin", "pimd", str(nsteps), "vel.bin"], cwd=d,
+                               stdout=log, stderr=subprocess.STDOUT, timeout=1200)
+        if r.returncode != 0:
+            print(open(os.path.join(d, "stdout.txt")).read()[-3000:])
+            raise RuntimeError(f"pimd {name}: reference run failed ({r.returncode})")
+        raw = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64)
+    hdr = raw[:9]
+    assert int(hdr[0]) == N and int(hdr[1]) == P
+    lc, nc, lb, nb = (int(v) for v in hdr[5:9])
+    o = 9
+    e = raw[o:o + 5]
+    o += 5
+    pos = raw[o:o + 3 * N * P].reshape(P, N, 3)
+    o += 3 * N * P
+    vel = raw[o:o + 3 * N * P].reshape(P, N, 3)
+    o += 3 * N * P
+    cls = raw[o:o + 2 * lc]
+    o += 2 * lc
+    bead = raw[o:o + (P - 1) * N * 3 * 2 * lb].reshape(P - 1, N, 3, 2, lb)
+    o += (P - 1) * N * 3 * 2 * lb
+    assert o == len(raw), (o, len(raw))
+    out = os.path.join(HERE, f"ref_pimd_{name}.npz")
+    np.savez_compressed(out, case=name, nsteps=nsteps, dt=hdr[4], len_nhc=lc, nchain_class=nc, nchain_bead=nb,
+                        v0=v0, energies=e, energy_keys=np.array(["vintert", "vintrat", "kin_harm", "pi_ke_prim",
+                                                                 "pi_ke_vir"]),
+                        pos=pos, vel=vel, nhc_class=cls, nhc_bead=bead)
+    print(f"pimd {name}: {nsteps} steps dt={hdr[4]:.6g} au, chains {nc} + {nb}, vintert={e[0]:.12g} -> {out}")
+
+
 MD_CASES = {"cfg1": 100}   # configuration 1: 100 velocity-Verlet steps at 0.25 fs
 
 
@@ -289,7 +354,7 @@ def run_md_case(name, nsteps):
 
 
 def main():
-    names = sys.argv[1:] or CASES + ["md", "setup", "w512_debug"]
+    names = sys.argv[1:] or CASES + ["md", "setup", "w512_debug", "pimd"]
     for n in names:
         if n == "md":
             for m, k in MD_CASES.items():
@@ -299,6 +364,9 @@ def main():
                 run_setup_case(m)
         elif n == "w512_debug":
             run_w512_debug()
+        elif n == "pimd":
+            for m, k in PIMD_CASES.items():
+                run_pimd_case(m, k)
         else:
             run_case(n)
 
