@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PMX_ABI_VERSION 6
+#define PMX_ABI_VERSION 7
 
 /* return codes */
 enum {
@@ -282,6 +282,31 @@ int pmx_get_positions(pmx_ctx *ctx, double *x, double *y, double *z);
  * kinetic (if not NULL) 1/2 sum m v^2 at the end (get_tvten). */
 int pmx_md_nve(pmx_ctx *ctx, int nsteps, double dt, pmx_energies *out,
                double *kinetic);
+
+/* nsteps of path-integral MD in the NVT ensemble on the device, as
+ * int_NVT_pimd (integrate/pimd/int_NVT_pimd.c:36-380) with RESPA off: per
+ * step the Nose-Hoover chains and half a kick (int_0_to_dt2_nvt_pimd,
+ * int_0_to_dt2_pimd.c:34-196), the drift in mode coordinates with xmod set
+ * to mode 1 (control_pimd_trans_mode / _pos), ghost positions, the force
+ * call energy_control_pimd with mode forces and springs, half a kick and
+ * the chains (int_dt2_to_dt_nvt_pimd, int_dt2_to_dt_pimd.c:31-172).  The
+ * velocities of the context (pmx_set_velocities / pmx_get_velocities) are
+ * the mode velocities, bead-major [P*N]; positions stay bead Cartesian.  The
+ * chain state persists in the context between calls (zero at first;
+ * pmx_set_nhc_state / pmx_get_nhc_state).  A call starts with a force call
+ * at the current positions.  out receives the energies of the last force
+ * call; econs (may be NULL) [4]: mode kinetic energy, the chains' energy,
+ * vintert + vintrat + kin_harm, and their sum, the conserved quantity.  The
+ * host waits once per step, for the Verlet-list decision.  P > 1. */
+int pmx_pimd_nvt(pmx_ctx *ctx, const pmx_nhc_params *nhc, int nsteps, pmx_energies *out, double *econs);
+
+/* Chain state of pmx_pimd_nvt: 2 len doubles for the centroid chain
+ * (velocities, then positions), then 2 len per degree of freedom of modes
+ * 2..P: mode m, site i, direction d at 2 len + ((m - 2) 3N + 3i + d) 2 len
+ * (1-based m; ghost sites' entries unused), as therm_class.v_nhc/x_nhc and
+ * therm_bead[m].v_nhc/x_nhc hold them. */
+int pmx_set_nhc_state(pmx_ctx *ctx, int len_nhc, const double *state);
+int pmx_get_nhc_state(pmx_ctx *ctx, int len_nhc, double *state);
 
 /* New force-field parameters for the context's topology: sys must describe
  * the same sites, types, bonds, bends and ghosts as at pmx_create, with the

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -48,7 +49,10 @@ struct pmx_ctx {
   double *q = nullptr, *mass = nullptr;
   double2 *tab_lj = nullptr, *tab_c = nullptr, *ecor_tab = nullptr;
   double *pair_pp = nullptr;   // [npair][PP_NUM]
-  float4 *ctab = nullptr;      // mixed precision: the Coulomb table [CT_N]
+  float4 *ctab = nullptr;      // mixed precision: erfc(alp r)/r as cubic pieces [CT_N]
+  float4 *ftab = nullptr;      // mixed precision: the switched pair functions [nftab][CT_N] (null if too many)
+  int nftab = 0;
+  std::vector<int2> ftab_off;  // per type pair: (erfc table, LJ table or -1), offsets in pieces
   float ct_r0 = 0.f, ct_idr = 0.f;
   double4 *site = nullptr;     // [N] (xmod, q)
   float4 *disp = nullptr;      // mixed precision with beads: x - xmod in FP32 [N*P]
@@ -60,10 +64,16 @@ struct pmx_ctx {
   // link cells of the list build
   int *site_cell = nullptr, *site_pos = nullptr, *cell_cnt = nullptr, *cell_start = nullptr, *cell_sites = nullptr;
   bool list_cells = false;  // the last build used link cells
+  bool ref_valid = false;   // jfull / nfull hold the reference's half list of the last build
+  double *xm0 = nullptr, *ym0 = nullptr, *zm0 = nullptr;  // xmod at the last build
+  PairTypes pt_build{};     // cut+skin at the last build
+  double4 *cell_p = nullptr, *cell_m = nullptr;  // sites in cell order (link-cell build)
+  int *cell_i = nullptr;
   int *b1 = nullptr, *b2 = nullptr, *btyp = nullptr, *a1 = nullptr, *a2 = nullptr, *a3 = nullptr,
       *atyp = nullptr, *e1 = nullptr, *e2 = nullptr;
   double *bc = nullptr, *beq = nullptr, *ac = nullptr, *as = nullptr;
   int nghost = 0, *gatom = nullptr, *gncomp = nullptr, *gcomp = nullptr;
+  std::vector<int> ghost_h;  // the ghost sites (host copy)
   // molecules of k_intra_mol (nmol = 0: the topology does not split into
   // molecules of at most MOL_MAX consecutive sites; per-term kernels then)
   int nmol = 0, mol_max = 0, ntpl = 0;
@@ -89,7 +99,7 @@ struct pmx_ctx {
   double *item_f = nullptr, *hbuf = nullptr;
   // per-block partial sums of every stage, then accumulators [P][ACC_NUM] + ake
   double *part = nullptr;
-  size_t off_pair = 0, off_fft = 0, off_ecor = 0, off_bond = 0, off_bend = 0, off_fin = 0;
+  size_t off_pair = 0, off_fft = 0, off_ecor = 0, off_bond = 0, off_bend = 0, off_fin = 0, off_mode = 0;
   int g_intra = 1024;
   double *acc = nullptr, *ake = nullptr;
   std::vector<double> acc_h;
@@ -132,6 +142,12 @@ struct pmx_ctx {
   hipStream_t sk = nullptr;
   hipEvent_t kfork = nullptr, kjoin = nullptr;
   hipEvent_t mxev = nullptr;  // the list check's maxima are in mx_pin
+  // PIMD NVT integrator: chain state (device layout of NhcArgs), mode mass
+  // factors, ghost flags, partial sums and the energies' readback
+  int nhc_len = 0;
+  double *vnhc = nullptr, *xnhc = nullptr, *nhc_cls = nullptr, *mfac = nullptr, *nhc_part = nullptr;
+  double *nhc_out = nullptr;
+  int *ghostflag = nullptr;
 };
 
 struct pmx_batch {
@@ -499,17 +515,32 @@ int setup_molecules(pmx_ctx *c, const pmx_system *sys) {
   return PMX_OK;
 }
 
-// The mixed-precision Coulomb table: erfc(alp r)/r with PINY's polynomial
-// (spline_fit.c:327-339) and its r derivative, in FP64, at CT_N + 1 nodes
-// from 0.5 Bohr to the largest cutoff, as the cubic (Hermite) piece of each
-// interval in FP32.  Its interpolation error is below 5e-9 relative over
-// 1-17 Bohr, under the FP32 rounding of the polynomial itself.
-int build_coulomb_table(pmx_ctx *c) {
+// The mixed-precision pair functions as FP32 cubic pieces on one grid in r
+// (CT_N intervals of width dr from r0, built in FP64):
+// * ctab: erfc(alp r)/r with PINY's polynomial (spline_fit.c:327-339), for
+//   the direct FP32 evaluation (pair_direct_f32), which applies the switch
+//   and shift itself;
+// * ftab: every tabulated function whole, as control_inter_params.c:643-772
+//   tabulates it: the erfc term per unit charge product and the LJ term, each
+//   times the switch of width rheal below the type pair's cutoff (swit) and
+//   minus its shift (shift_inter_pe = shift).  Type pairs with the same
+//   function share one table; ftab_off[t] = (erfc table, LJ table or -1) in
+//   units of CT_N pieces.  The grid ends at the largest cutoff with
+//   cutoff - rheal a node, so the switch's kink (its second derivative jumps
+//   there) falls on a node when all type pairs share one cutoff.
+// Each piece is the cubic Hermite interpolant of the function and its r
+// derivative at the two nodes: c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1].
+// Interpolation error below 5e-9 relative for the erfc term over 1-17 Bohr,
+// under the FP32 rounding of the terms themselves.
+int build_pair_tables(pmx_ctx *c) {
   const Setup &s = c->s;
   double rend = 0.0;
   for (int t = 0; t < s.npair; t++) rend = std::max(rend, s.cutoff[t]);
-  const double r0 = 0.5, dr = (rend * (1.0 + 1e-6) - r0) / CT_N, alp = s.alp_ewd;
-  auto vd = [&](double r, double &v, double &d) {
+  const bool swit = (c->par.ishift == PMX_SHIFT_SWIT) && c->par.rheal > 0.0;
+  double dr = (rend - 0.5) / CT_N;
+  if (swit) dr = c->par.rheal / std::ceil(c->par.rheal / dr);  // rheal a whole number of intervals
+  const double r0 = rend - CT_N * dr, alp = s.alp_ewd;
+  auto erfc_r = [&](double r, double &v, double &d) {  // erfc(alp r)/r and its r derivative
     const double x = alp * r, tt = 1.0 / (1.0 + kErfP * x), ee = exp(-x * x);
     double g = kErfE[9], dg = 9.0 * kErfE[9];
     for (int k = 8; k >= 1; k--) {
@@ -519,26 +550,78 @@ int build_coulomb_table(pmx_ctx *c) {
     g = g * tt * ee;
     dg = dg * tt * tt * ee * kErfP * alp + 2.0 * alp * alp * g * r;  // -d g / dr
     v = g / r;
-    d = -(g / (r * r) + dg / r);  // dV/dr
+    d = -(g / (r * r) + dg / r);
   };
-  std::vector<float4> tab(CT_N);
-  double v0, d0;
-  vd(r0, v0, d0);
-  for (int k = 0; k < CT_N; k++) {
-    double v1, d1;
-    vd(r0 + dr * (k + 1), v1, d1);
-    tab[k] = make_float4((float)v0, (float)(dr * d0), (float)(3.0 * (v1 - v0) - dr * (2.0 * d0 + d1)),
-                         (float)(2.0 * (v0 - v1) + dr * (d0 + d1)));
-    v0 = v1;
-    d0 = d1;
+  // the switch of type pair t (spline_fit.c:65-73) and its r derivative
+  auto sw = [&](int t, double r, double &w, double &dw) {
+    w = 1.0;
+    dw = 0.0;
+    if (!swit) return;
+    const double rs = r - s.cutoff[t] + c->par.rheal;
+    if (rs <= 0.0) return;
+    const double rp = std::min(rs / c->par.rheal, 1.0);
+    w = 1.0 + rp * rp * (2.0 * rp - 3.0);
+    dw = rs < c->par.rheal ? 6.0 * rp * (rp - 1.0) / c->par.rheal : 0.0;
+  };
+  auto pieces = [&](auto fn, std::vector<float4> &out) {
+    double v0, d0;
+    fn(r0, v0, d0);
+    for (int k = 0; k < CT_N; k++) {
+      double v1, d1;
+      fn(r0 + dr * (k + 1), v1, d1);
+      out.push_back(make_float4((float)v0, (float)(dr * d0), (float)(3.0 * (v1 - v0) - dr * (2.0 * d0 + d1)),
+                                (float)(2.0 * (v0 - v1) + dr * (d0 + d1))));
+      v0 = v1;
+      d0 = d1;
+    }
+  };
+  std::vector<float4> ct;
+  pieces(erfc_r, ct);
+  // whole functions, deduplicated by their parameters
+  std::vector<float4> ft;
+  std::vector<std::array<double, 5>> keys;
+  auto table_of = [&](const std::array<double, 5> &key, auto fn) {
+    for (size_t k = 0; k < keys.size(); k++)
+      if (keys[k] == key) return (int)k;
+    keys.push_back(key);
+    pieces(fn, ft);
+    return (int)keys.size() - 1;
+  };
+  c->ftab_off.assign(s.npair, make_int2(-1, -1));
+  for (int t = 0; t < s.npair; t++) {
+    c->ftab_off[t].x = table_of({0.0, s.cutoff[t], s.vshift_c[t], 0.0, 0.0}, [&](double r, double &v, double &d) {
+      double e, de, w, dw;
+      erfc_r(r, e, de);
+      sw(t, r, w, dw);
+      v = e * w - s.vshift_c[t];
+      d = de * w + e * dw;
+    });
+    if (s.label[t] == PMX_INTER_LJ && s.lj_eps[t] != 0.0) {
+      c->ftab_off[t].y =
+          table_of({1.0, s.cutoff[t], s.vshift_lj[t], s.lj_eps[t], s.lj_sig6[t]}, [&](double r, double &v, double &d) {
+            const double r2i = 1.0 / (r * r), s6 = s.lj_sig6[t] * r2i * r2i * r2i;
+            const double e = 4.0 * s.lj_eps[t] * s6 * (s6 - 1.0);
+            const double de = -24.0 * s.lj_eps[t] * s6 * (2.0 * s6 - 1.0) / r;
+            double w, dw;
+            sw(t, r, w, dw);
+            v = e * w - s.vshift_lj[t];
+            d = de * w + e * dw;
+          });
+    }
   }
-  if (!c->ctab) {
-    int rc = dalloc(c, &c->ctab, (size_t)CT_N);
-    if (rc) return rc;
+  c->nftab = (int)keys.size();
+  for (int t = 0; t < s.npair; t++) {
+    c->ftab_off[t].x *= CT_N;
+    if (c->ftab_off[t].y >= 0) c->ftab_off[t].y *= CT_N;
   }
-  HIPCHK(hipMemcpy(c->ctab, tab.data(), CT_N * sizeof(float4), hipMemcpyHostToDevice));
+  dfree(c, c->ctab);
+  dfree(c, c->ftab);
+  int rc = upload(c, &c->ctab, ct.data(), ct.size());
+  if (!rc && c->nftab <= MAX_FTAB) rc = upload(c, &c->ftab, ft.data(), ft.size());
+  if (rc) return rc;
   c->ct_r0 = (float)r0;
   c->ct_idr = (float)(1.0 / dr);
+  c->epoch++;
   return PMX_OK;
 }
 
@@ -619,7 +702,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
     if (c->pair_mode == 2 && P > 1) TRY(dalloc(c, &c->disp, NP));
-    if (c->pair_mode == 2) TRY(build_coulomb_table(c));
+    if (c->pair_mode == 2) TRY(build_pair_tables(c));
     if (P > 1) TRY(dalloc(c, &c->umax, (size_t)N));
   }
   TRY(dalloc(c, &c->site_cell, N));
@@ -629,7 +712,13 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(dalloc(c, &c->cell_start, (size_t)MAX_LIST_CELLS));
   TRY(dalloc(c, &c->nfull, N));
   TRY(dalloc(c, &c->nforce, N));
-  TRY(dalloc(c, &c->overflow, 2));
+  TRY(dalloc(c, &c->overflow, 4));
+  TRY(dalloc(c, &c->cell_p, N));
+  TRY(dalloc(c, &c->cell_m, N));
+  TRY(dalloc(c, &c->cell_i, N));
+  TRY(dalloc(c, &c->xm0, N));
+  TRY(dalloc(c, &c->ym0, N));
+  TRY(dalloc(c, &c->zm0, N));
   TRY(alloc_lists(c, estimate_cap(s, s.cutoff_max + c->par.skin + 1.0 / BOHR * (P > 1 ? 1.0 : 0.0))));
   // bonded
   c->nbond = sys->nbond;
@@ -668,6 +757,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(upload(c, &c->ecs_j, js.data(), js.size()));
   }
   c->nghost = sys->nghost;
+  c->ghost_h.assign(sys->ghost_atom, sys->ghost_atom + sys->nghost);
   TRY(upload(c, &c->gatom, sys->ghost_atom, sys->nghost));
   TRY(upload(c, &c->gncomp, sys->ghost_ncomp, sys->nghost));
   TRY(upload(c, &c->gcomp, sys->ghost_comp, (size_t)4 * sys->nghost));
@@ -706,9 +796,9 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
       }
     }
     const size_t cx = c->pme_f32 ? 2 * sizeof(float) : 2 * sizeof(double);
-    if (((size_t)Ka * Kb + Ka + Kb) * cx > FFT_LDS_MAX || ((size_t)Kc * Ha + Kc) * cx > FFT_LDS_MAX) {
-      set_err(c, "PME grid %dx%dx%d: a plane exceeds the %zu bytes of LDS of one workgroup", Ka, Kb, Kc,
-              (size_t)FFT_LDS_MAX);
+    if (((size_t)Ka * Kb + Ka + Kb) * cx > FFT_LDS_DYN || ((size_t)Kc * Ha + Kc) * cx > FFT_LDS_DYN) {
+      set_err(c, "PME grid %dx%dx%d: a plane and its twiddles exceed the %zu bytes of LDS an FFT workgroup may use",
+              Ka, Kb, Kc, (size_t)FFT_LDS_DYN);
       return PMX_ERR_NOTIMPL;
     }
     c->gstride = (size_t)Ka * Kb * Kc;
@@ -758,7 +848,8 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     c->off_bond = c->off_ecor + intra;
     c->off_bend = c->off_bond + intra;
     c->off_fin = c->off_bend + intra;
-    TRY(dalloc(c, &c->part, c->off_fin + fin));
+    c->off_mode = c->off_fin + fin;
+    TRY(dalloc(c, &c->part, c->off_mode + (P > 1 ? mode_blocks(N, P) : 1)));
   }
   if (c->pme || c->ewald) {
     // the k-space kernels store the force of every (charge, bead); the
@@ -802,92 +893,104 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
 }
 
 // Link cells for the list build (make_lnk_lst.c / make_lnk_map.c): cells of
-// fractional space at least one list radius wide, at least 3 per axis, so
-// the 27 cells around a site hold every site with an image inside the list
-// radius, each once.  Returns false when the box is too small for that (or
-// the rows would not fit the sort), and the ascending O(N^2) scan is used.
-bool choose_cells(pmx_ctx *c, int nc[3], int *n2) {
+// fractional space at least r_list / H wide and at least 2H + 1 per axis,
+// so that the (2H + 1)^3 cells around a site's cell hold every site with an
+// image inside the list radius, each once.  H = 2 when the box allows it
+// (a 5^3 stencil of half-width cells: about 3.7 candidates per list pair),
+// else H = 1 (27 cells); 0 when the box is too small for either, or the
+// plain scan is asked for (PMX_LIST=brute, or the autotuner's choice), and
+// the ascending O(N^2) scan is used.
+int choose_cells(const pmx_ctx *c, int nc[3]) {
   const char *env = getenv("PMX_LIST");  // tuning override: "brute" or "cells"
-  if ((env && strcmp(env, "brute") == 0) || c->list_mode == 1) return false;
+  if ((env && strcmp(env, "brute") == 0) || c->list_mode == 1) return 0;
   double rl = 0.0;
   for (int t = 0; t < c->s.npair; t++) rl = std::max(rl, sqrt(c->pt.cutskin2[t]));
-  int ncell = 1;
-  for (int d = 0; d < 3; d++) {
-    const double *hi = c->cell.hi;
-    const double w = 1.0 / sqrt(hi[d] * hi[d] + hi[d + 3] * hi[d + 3] + hi[d + 6] * hi[d + 6]);
-    nc[d] = std::min(64, (int)floor(w / rl));
-    if (nc[d] < 3) return false;
-    ncell *= nc[d];
+  for (int H = 2; H >= 1; H--) {
+    size_t ncell = 1;
+    bool ok = true;
+    for (int d = 0; d < 3; d++) {
+      const double *hi = c->cell.hi;
+      const double w = 1.0 / sqrt(hi[d] * hi[d] + hi[d + 3] * hi[d + 3] + hi[d + 6] * hi[d + 6]);
+      nc[d] = std::min(64, (int)floor(w * H / rl));
+      ok = ok && nc[d] >= 2 * H + 1;
+      ncell *= (size_t)std::max(nc[d], 1);
+    }
+    if (ok && ncell <= (size_t)MAX_LIST_CELLS) return H;
   }
-  if (ncell > MAX_LIST_CELLS) return false;
-  int p2 = 64;
-  while (p2 < c->cap) p2 <<= 1;
-  if ((size_t)p2 * sizeof(int) > 65536) return false;
-  *n2 = p2;
-  // The plain scan is O(N^2) but needs no row sort.  Measured on MI355X
-  // (q-TIP4P/F at 0.997 g/cm^3, r_c + skin = 10 A): 16384 sites 1.2 ms both
-  // ways, 32768 sites 4.0 ms plain vs 1.65 ms with cells, 65536 sites
-  // 13.8 ms vs 3.4 ms.
-  return (ncell >= 64 && c->s.N > 16384) || (env && strcmp(env, "cells") == 0) || c->list_mode == 2;
+  return 0;
+}
+
+ListArgs list_args(pmx_ctx *c) {
+  Setup &s = c->s;
+  ListArgs a{};
+  a.N = s.N;
+  a.cap = c->cap;
+  a.pimd = (s.P > 1);
+  a.bx = c->b0x;
+  a.by = c->b0y;
+  a.bz = c->b0z;
+  a.xm = c->xm;
+  a.ym = c->ym;
+  a.zm = c->zm;
+  a.type = c->type;
+  a.q = c->q;
+  a.ex_off = c->ex_off;
+  a.ex_num = c->ex_num;
+  a.ex_j = c->ex_j;
+  a.pt = c->pt;
+  a.cell = c->cell;
+  a.full = c->full_list ? 1 : 0;
+  a.jfull = c->jfull;
+  a.jforce = c->jforce;
+  a.nfull = c->nfull;
+  a.nforce = c->nforce;
+  a.overflow = c->overflow;
+  a.site_cell = c->site_cell;
+  a.site_pos = c->site_pos;
+  a.cell_cnt = c->cell_cnt;
+  a.cell_start = c->cell_start;
+  a.cell_sites = c->cell_sites;
+  a.cell_p = c->cell_p;
+  a.cell_m = c->cell_m;
+  a.cell_i = c->cell_i;
+  return a;
 }
 
 // (Re)build the Verlet list from the bead-1 copy; grows the ELL width when a
-// row overflows.
+// row overflows.  The force list is built by link cells where the box
+// allows; the reference's half list in its own order (pmx_get_verlet_list)
+// then follows on demand from the build's saved state.
 int build_list(pmx_ctx *c) {
   Setup &s = c->s;
+  bool cells_failed = false;
   for (int attempt = 0; attempt < 8; attempt++) {
-    int nc[3] = {1, 1, 1}, n2 = 64;
-    const bool cells = choose_cells(c, nc, &n2);
-    HIPCHK(hipMemsetAsync(c->overflow, 0, 2 * sizeof(int), c->st));
-    ListArgs a{};
-    a.N = s.N;
-    a.cap = c->cap;
-    a.pimd = (s.P > 1);
-    a.bx = c->b0x;
-    a.by = c->b0y;
-    a.bz = c->b0z;
-    a.xm = c->xm;
-    a.ym = c->ym;
-    a.zm = c->zm;
-    a.type = c->type;
-    a.q = c->q;
-    a.ex_off = c->ex_off;
-    a.ex_num = c->ex_num;
-    a.ex_j = c->ex_j;
-    a.pt = c->pt;
-    a.cell = c->cell;
-    a.full = c->full_list ? 1 : 0;
-    a.jfull = c->jfull;
-    a.jforce = c->jforce;
-    a.nfull = c->nfull;
-    a.nforce = c->nforce;
-    a.overflow = c->overflow;
-    a.use_cells = cells ? 1 : 0;
+    int nc[3] = {1, 1, 1};
+    const int H = cells_failed ? 0 : choose_cells(c, nc);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->st, c->overflow, 0, (size_t)4);
+    ListArgs a = list_args(c);
+    a.use_cells = 0;
     for (int d = 0; d < 3; d++) a.nc[d] = nc[d];
-    a.site_cell = c->site_cell;
-    a.site_pos = c->site_pos;
-    a.cell_cnt = c->cell_cnt;
-    a.cell_start = c->cell_start;
-    a.cell_sites = c->cell_sites;
-    if (cells) {
+    if (H > 0) {
       const int ncell = nc[0] * nc[1] * nc[2];
       hipLaunchKernelGGL(k_fill_int, dim3(blocks(ncell)), dim3(256), 0, c->st, c->cell_cnt, 0, (size_t)ncell);
       hipLaunchKernelGGL(k_cell_count, dim3(blocks(s.N)), dim3(256), 0, c->st, a);
       hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(256), 0, c->st, ncell, c->cell_cnt, c->cell_start);
       hipLaunchKernelGGL(k_cell_scatter, dim3(blocks(s.N)), dim3(256), 0, c->st, a);
+      hipLaunchKernelGGL(k_cell_order, dim3(ncell), dim3(256), 0, c->st, a);
+      HIPCHK(hipGetLastError());
+      HIPCHK(launch_list_cells(a, H, ncell, c->st));
+    } else {
+      hipLaunchKernelGGL(k_list_build, dim3(blocks((size_t)s.N * 64)), dim3(256), 0, c->st, a);
       HIPCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_list_build, dim3(blocks((size_t)s.N * 64)), dim3(256), 0, c->st, a);
-    HIPCHK(hipGetLastError());
-    int ov[2];
+    int ov[4];
     HIPCHK(hipMemcpyAsync(ov, c->overflow, sizeof ov, hipMemcpyDeviceToHost, c->st));
     HIPCHK(hipStreamSynchronize(c->st));
+    if (ov[2]) {  // a cell too full for the cell kernels: the plain scan
+      cells_failed = true;
+      continue;
+    }
     if (ov[0] == 0) {
-      if (cells) {  // ascending rows, as the plain scan writes them
-        hipLaunchKernelGGL(k_sort_rows, dim3(s.N), dim3(256), (size_t)n2 * sizeof(int), c->st, c->cap, n2,
-                           c->nfull, c->nforce, c->jfull, c->jforce);
-        HIPCHK(hipGetLastError());
-      }
       // force-list rows: pairs inside the cutoff first
       hipLaunchKernelGGL(k_split_rows, dim3(blocks((size_t)s.N * 64)), dim3(256), 0, c->st, a, c->jsplit);
       HIPCHK(hipGetLastError());
@@ -895,7 +998,13 @@ int build_list(pmx_ctx *c) {
       // pointing at jforce
       HIPCHK(hipMemcpyAsync(c->jforce, c->jsplit, (size_t)s.N * c->cap * sizeof(int), hipMemcpyDeviceToDevice,
                             c->st));
-      c->list_cells = cells;
+      // the state the reference list is built from on demand
+      HIPCHK(hipMemcpyAsync(c->xm0, c->xm, s.N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->ym0, c->ym, s.N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+      HIPCHK(hipMemcpyAsync(c->zm0, c->zm, s.N * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+      c->pt_build = c->pt;
+      c->ref_valid = (H == 0);  // the plain scan also wrote the reference's rows
+      c->list_cells = H > 0;
       c->nver = ov[1];
       return PMX_OK;
     }
@@ -904,6 +1013,34 @@ int build_list(pmx_ctx *c) {
   }
   set_err(c, "Verlet list does not fit after growing its rows");
   return PMX_ERR_RANGE;
+}
+
+// The reference's half list in its order (verlist_create.c:373-503), from
+// the positions, xmod and cut+skin of the last build, by the ascending scan
+// over j < i.
+int build_ref_list(pmx_ctx *c) {
+  ListArgs a = list_args(c);
+  a.bx = c->x0;
+  a.by = c->y0;
+  a.bz = c->z0;
+  a.xm = c->xm0;
+  a.ym = c->ym0;
+  a.zm = c->zm0;
+  a.pt = c->pt_build;
+  a.full = 0;
+  a.ref_only = 1;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->st, c->overflow, 0, (size_t)4);
+  hipLaunchKernelGGL(k_list_build, dim3(blocks((size_t)c->s.N * 64)), dim3(256), 0, c->st, a);
+  HIPCHK(hipGetLastError());
+  int ov[4];
+  HIPCHK(hipMemcpyAsync(ov, c->overflow, sizeof ov, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(hipStreamSynchronize(c->st));
+  if (ov[0] != 0 || ov[1] != c->nver) {
+    set_err(c, "reference list rebuild: %d pairs against %d of the force list's build", ov[1], c->nver);
+    return PMX_ERR_STATE;
+  }
+  c->ref_valid = true;
+  return PMX_OK;
 }
 
 void add_stage(RedArgs &r, size_t off, int G, int nv, const std::vector<int> &slots) {
@@ -960,6 +1097,14 @@ IntraArgs intra_args(pmx_ctx *c, bool pimd) {
 
 namespace {
 
+// How the list check sets xmod: 0 given (or no beads), 1 the bead centroid
+// (normal modes), 2 bead 1 (staging): mode 1 of the transform, which
+// control_pimd_trans_pos copies into xmod (control_pimd_trans.c:175-181).
+int xmod_kind(const pmx_ctx *c) {
+  if (c->s.P == 1 || c->xmod_given) return 0;
+  return c->par.pimd_typ == PMX_PIMD_STAGING ? 2 : 1;
+}
+
 // The list stage of a force call, first half: centroid reference and list
 // check (nbr_list_control.c:32-118) launched on st, with the per-workgroup
 // maxima copied to pinned host memory (event mxev).  The caller waits for
@@ -974,7 +1119,7 @@ int list_stage_launch(pmx_ctx *c, hipStream_t st, bool timing) {
   c->check_blocks = (int)check_blocks;
   hipLaunchKernelGGL(k_list_check, dim3(check_blocks), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->x0,
                      c->y0, c->z0,
-                     c->b0x, c->b0y, c->b0z, c->maxes, (P > 1 && !c->xmod_given) ? 1 : 0, c->xm, c->ym, c->zm);
+                     c->b0x, c->b0y, c->b0z, c->maxes, xmod_kind(c), c->xm, c->ym, c->zm);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->mx_pin, c->maxes, 2 * check_blocks * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(c->mxev, st));
@@ -1042,8 +1187,6 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
       HIPCHK(hipGetLastError());
     }
   }
-  hipLaunchKernelGGL(k_fill_double, dim3(1), dim3(64), 0, st, c->ake, 0.0, (size_t)1);
-  HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
                      c->q, c->site);
   HIPCHK(hipGetLastError());
@@ -1185,6 +1328,9 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.ctab = c->ctab;
     a.ct_r0 = c->ct_r0;
     a.ct_idr = c->ct_idr;
+    a.ftab = c->ftab;
+    a.nftab = c->nftab;
+    for (size_t t = 0; t < c->ftab_off.size(); t++) a.ftoff[t] = c->ftab_off[t];
     a.rheal = c->par.rheal;
     a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
     a.swit = (c->par.ishift == PMX_SHIFT_SWIT);
@@ -1308,33 +1454,37 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     }
   }
   const bool modes = pimd && (flags & PMX_EVAL_MODE_FORCES) && P > 1;
-  if (modes) {  // fxt: the bead forces before the transform (energy_control_pimd.c:116-144)
-    HIPCHK(hipMemcpyAsync(c->fxt, c->fx, NP * sizeof(double), hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(c->fyt, c->fy, NP * sizeof(double), hipMemcpyDeviceToDevice, st));
-    HIPCHK(hipMemcpyAsync(c->fzt, c->fz, NP * sizeof(double), hipMemcpyDeviceToDevice, st));
-  }
   if (modes) {
+    // the bead forces kept as fxt (energy_control_pimd.c:116-144), then mode
+    // forces plus springs in place
     ModeArgs a{};
     a.N = N;
     a.P = P;
+    a.stag = (c->par.pimd_typ == PMX_PIMD_STAGING) ? 1 : 0;
     a.x = c->x;
     a.y = c->y;
     a.z = c->z;
     a.fx = c->fx;
     a.fy = c->fy;
     a.fz = c->fz;
+    a.fxt = c->fxt;
+    a.fyt = c->fyt;
+    a.fzt = c->fzt;
     a.prekf = c->prekf;
     a.veig = c->veig;
     a.cs = c->cs;
-    a.ake = c->ake;
-    hipLaunchKernelGGL(k_mode_forces, dim3(blocks((size_t)N * 3)), dim3(256), 0, st, a);
+    a.part = c->part + c->off_mode;
+    hipLaunchKernelGGL(k_mode_forces, dim3(mode_blocks(N, P)), dim3(256), 0, st, a);
     HIPCHK(hipGetLastError());
   }
   red.P = P;
   red.nslot = ACC_NUM;
   red.part = c->part;
   red.acc = c->acc;
-  hipLaunchKernelGGL(k_reduce_partials, dim3(blocks((size_t)P * ACC_NUM * 64)), dim3(256), 0, st, red);
+  red.spart = c->part + c->off_mode;  // the spring energy (0 without modes)
+  red.sG = modes ? mode_blocks(N, P) : 0;
+  red.sacc = c->ake;
+  hipLaunchKernelGGL(k_reduce_partials, dim3(blocks(((size_t)P * ACC_NUM + 1) * 64)), dim3(256), 0, st, red);
   HIPCHK(hipGetLastError());
   if (timing) HIPCHK(hipEventRecord(c->ev[5], st));
   return PMX_OK;
@@ -1836,9 +1986,9 @@ int pmx_autotune(pmx_ctx *c, unsigned flags, int reps, int *choice) {
   }
   // list build (stage 0, with the list rebuilt every call)
   {
-    int nc[3], n2;
+    int nc[3];
     c->list_mode = 2;
-    const bool cells_ok = choose_cells(c, nc, &n2);
+    const bool cells_ok = choose_cells(c, nc) > 0;
     double tb = 0.0, tc = 1e300;
     c->list_mode = 1;
     if ((rc = stage_ms(0, flags | PMX_EVAL_FORCE_LIST, &tb))) return rc;
@@ -2073,6 +2223,281 @@ int pmx_md_nve(pmx_ctx *c, int nsteps, double dt, pmx_energies *out, double *kin
   return PMX_OK;
 }
 
+namespace {
+
+// Allocates (zeroed) chain state for chains of length len.
+int nhc_alloc(pmx_ctx *c, int len) {
+  Setup &s = c->s;
+  const size_t NP = (size_t)s.N * s.P;
+  if (c->nhc_len == len) return PMX_OK;
+  dfree(c, c->vnhc);
+  dfree(c, c->xnhc);
+  dfree(c, c->nhc_cls);
+  c->nhc_len = 0;
+  TRY_RC(dalloc(c, &c->vnhc, (size_t)len * 3 * NP));
+  TRY_RC(dalloc(c, &c->xnhc, (size_t)len * 3 * NP));
+  TRY_RC(dalloc(c, &c->nhc_cls, (size_t)2 * len + 1));
+  HIPCHK(hipMemset(c->vnhc, 0, (size_t)len * 3 * NP * sizeof(double)));
+  HIPCHK(hipMemset(c->xnhc, 0, (size_t)len * 3 * NP * sizeof(double)));
+  HIPCHK(hipMemset(c->nhc_cls, 0, ((size_t)2 * len + 1) * sizeof(double)));
+  if (!c->mfac) TRY_RC(dalloc(c, &c->mfac, 64));
+  if (!c->nhc_part) TRY_RC(dalloc(c, &c->nhc_part, 2 * NHC_BLOCKS));
+  if (!c->nhc_out) TRY_RC(dalloc(c, &c->nhc_out, 2));
+  if (!c->ghostflag) {
+    std::vector<int> g(s.N, 0);
+    for (int k = 0; k < c->nghost; k++) g[c->ghost_h[k]] = 1;
+    TRY_RC(upload(c, &c->ghostflag, g.data(), g.size()));
+  }
+  c->nhc_len = len;
+  return PMX_OK;
+}
+
+// The integrator's constants: set_yosh (int_utils.c:244-318) for the chain
+// step dt / nres, and the chains' masses and gkt as set_atm_NHC sets them
+// (set_atm_NHC.c:355-370, 510-547 for the centroid chain on the global
+// thermostat, 715-755 for the bead chains), with the mode mass factors of
+// path_integral_init (transform_cnt.c:175-233).
+int nhc_args(pmx_ctx *c, const pmx_nhc_params *np, NhcArgs &a, std::vector<double> &mf) {
+  Setup &s = c->s;
+  const int P = s.P;
+  a = NhcArgs{};
+  a.N = s.N;
+  a.P = P;
+  a.len = np->len_nhc;
+  a.nres = np->nres_nhc;
+  a.nyosh = np->nyosh_nhc;
+  double w[10] = {0};
+  if (a.nyosh == 1) {
+    w[1] = 1.0;
+  } else if (a.nyosh == 3) {
+    const double onethird = 1.0 / 3.0, temp = pow(2.0, onethird);
+    w[1] = 1.0 / (2.0 - temp);
+    w[2] = -temp / (2.0 - temp);
+    w[3] = 1.0 / (2.0 - temp);
+  } else if (a.nyosh == 5) {
+    const double onethird = 1.0 / 3.0, temp = pow(4.0, onethird), p2 = 1.0 / (4.0 - temp);
+    w[1] = p2;
+    w[2] = p2;
+    w[3] = 1.0 - 4.0 * p2;
+    w[4] = p2;
+    w[5] = p2;
+  } else {
+    set_err(c, "pmx_pimd_nvt: yosh_steps_nhc %d (1, 3 or 5 are supported)", a.nyosh);
+    return PMX_ERR_NOTIMPL;
+  }
+  const double dti = np->dt / (double)a.nres;
+  for (int i = 1; i <= a.nyosh; i++) {
+    const double wd = w[i] * dti;
+    a.wdti2[i] = wd / 2.0;
+    a.wdti4[i] = wd / 4.0;
+    a.wdti8[i] = wd / 8.0;
+  }
+  const double T = c->par.temperature > 0 ? c->par.temperature : 300.0, tau = np->tau_nhc;
+  const int nreal = s.N - c->nghost;
+  const double m1 = tau * tau * T, g1 = 3.0 * (double)nreal;
+  for (int j = 2; j <= a.len; j++) {
+    a.cgkt[j] = T / BOLTZ;
+    a.cmass[j] = m1 / BOLTZ;
+  }
+  a.cmass[1] = m1 * g1 / BOLTZ;
+  a.cgkt[1] = g1 * T / BOLTZ;
+  const double pre = (BOLTZ * BOLTZ * np->gamma_adb * np->gamma_adb) / ((double)P);
+  const double bm1 = pre / (T * T);
+  for (int j = 1; j <= a.len; j++) {
+    a.bgkt[j] = T / BOLTZ;
+    a.bmass[j] = bm1 * T / BOLTZ;
+  }
+  mf.assign(P, 1.0);
+  const double dP = (double)P;
+  for (int ip = 2; ip <= P; ip++) {
+    double v;
+    if (c->par.pimd_typ == PMX_PIMD_STAGING) {
+      v = ((double)(ip)) / ((double)(ip - 1));
+    } else if (ip == P) {
+      v = 4.0 * dP;
+    } else {
+      const int k = ip / 2 + 1;  // veig[2k-2] = veig[2k-1] for k = 2..P/2
+      v = (1.0 - cos(2.0 * M_PI / dP * (double)(k - 1))) * 4.0 * dP;
+    }
+    mf[ip - 1] = v * np->gamma_adb * np->gamma_adb;
+  }
+  a.dt = np->dt;
+  a.dt2 = np->dt / 2.0;
+  a.mass = c->mass;
+  a.mfac = c->mfac;
+  a.ghost = c->ghostflag;
+  a.fx = c->fx;
+  a.fy = c->fy;
+  a.fz = c->fz;
+  a.vx = c->vx;
+  a.vy = c->vy;
+  a.vz = c->vz;
+  a.vnhc = c->vnhc;
+  a.xnhc = c->xnhc;
+  a.cls = c->nhc_cls;
+  a.part = c->nhc_part;
+  a.stag = c->par.pimd_typ == PMX_PIMD_STAGING ? 1 : 0;
+  a.cs = c->cs;
+  a.x = c->x;
+  a.y = c->y;
+  a.z = c->z;
+  a.xm = c->xm;
+  a.ym = c->ym;
+  a.zm = c->zm;
+  return PMX_OK;
+}
+
+int check_nhc_params(pmx_ctx *c, const pmx_nhc_params *np) {
+  if (!np || np->len_nhc < 1 || np->len_nhc > 7 || np->nres_nhc < 1 || !(np->dt > 0.0) || !(np->tau_nhc > 0.0) ||
+      !(np->gamma_adb > 0.0)) {
+    set_err(c, "pmx_pimd_nvt: invalid integrator settings");
+    return PMX_ERR_ARG;
+  }
+  return PMX_OK;
+}
+
+// One force call of energy_control_pimd with mode forces, launched on the
+// context's stream without the readback of the energies (the list decision
+// is the host's one wait).
+int pimd_force(pmx_ctx *c, unsigned flags) {
+  pmx_energies tmp;
+  memset(&tmp, 0, sizeof tmp);
+  c->grad_ready = false;
+  if (int rc = list_stage_launch(c, c->st, false)) return rc;
+  if (int rc = pre_sequence(c, flags, c->st, false, true)) return rc;
+  HIPCHK(hipEventSynchronize(c->mxev));
+  if (int rc = list_stage_finish(c, flags, &tmp)) return rc;
+  return main_sequence(c, flags, c->st, false);
+}
+
+}  // namespace
+
+int pmx_pimd_nvt(pmx_ctx *c, const pmx_nhc_params *np, int nsteps, pmx_energies *out, double *econs) {
+  if (!c || !out || nsteps < 0) return PMX_ERR_ARG;
+  Setup &s = c->s;
+  const int N = s.N, P = s.P;
+  const size_t NP = (size_t)N * P;
+  if (P < 2) {
+    set_err(c, "pmx_pimd_nvt: one bead: use pmx_md_nve");
+    return PMX_ERR_ARG;
+  }
+  if (!c->have_pos) {
+    set_err(c, "pmx_pimd_nvt: no positions set");
+    return PMX_ERR_STATE;
+  }
+  if (int rc = check_nhc_params(c, np)) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  if (int rc = nhc_alloc(c, np->len_nhc)) return rc;
+  NhcArgs a;
+  std::vector<double> mf;
+  if (int rc = nhc_args(c, np, a, mf)) return rc;
+  HIPCHK(hipMemcpyAsync(c->mfac, mf.data(), P * sizeof(double), hipMemcpyHostToDevice, c->st));
+  const unsigned flags = PMX_EVAL_PIMD | PMX_EVAL_MODE_FORCES;
+  memset(out, 0, sizeof(*out));
+  // forces at the starting positions, xmod as the context has it
+  if (int rc = pimd_force(c, flags)) return rc;
+  const bool xmod_given = c->xmod_given;
+  c->xmod_given = true;  // xmod is mode 1 of each drift from here on
+  const unsigned gN = blocks(N), gNP = blocks(NP);
+  int rc = PMX_OK;
+  for (int step = 0; step < nsteps && !rc; step++) {
+    hipLaunchKernelGGL(k_nhc_ckin, dim3(NHC_BLOCKS), dim3(256), 0, c->st, a);
+    hipLaunchKernelGGL(k_nhc_class, dim3(1), dim3(64), 0, c->st, a, NHC_BLOCKS);
+    hipLaunchKernelGGL(k_nhc_kick, dim3(gNP), dim3(256), 0, c->st, a, 0);
+    hipLaunchKernelGGL(k_mode_drift, dim3(mode_blocks(N, P)), dim3(256), 0, c->st, a);
+    if (c->nghost)
+      hipLaunchKernelGGL(k_ghost_pos, dim3(blocks((size_t)c->nghost * P)), dim3(256), 0, c->st, c->nghost, P,
+                         c->gatom, c->gncomp, c->gcomp, c->gcoef, c->x, c->y, c->z);
+    if (hipGetLastError() != hipSuccess) {
+      rc = PMX_ERR_DEVICE;
+      break;
+    }
+    if ((rc = pimd_force(c, flags))) break;
+    hipLaunchKernelGGL(k_nhc_kick, dim3(gNP), dim3(256), 0, c->st, a, 1);
+    hipLaunchKernelGGL(k_nhc_ckin, dim3(NHC_BLOCKS), dim3(256), 0, c->st, a);
+    hipLaunchKernelGGL(k_nhc_class, dim3(1), dim3(64), 0, c->st, a, NHC_BLOCKS);
+    hipLaunchKernelGGL(k_nhc_scale0, dim3(gN), dim3(256), 0, c->st, a);
+  }
+  c->xmod_given = xmod_given;
+  if (rc == PMX_OK && hipGetLastError() != hipSuccess) rc = PMX_ERR_DEVICE;
+  if (rc) {
+    if (rc == PMX_ERR_DEVICE) set_err(c, "pmx_pimd_nvt: kernel launch failed");
+    return rc;
+  }
+  TRY_RC(readback_launch(c, c->st));
+  double eo[2] = {0.0, 0.0};
+  if (econs) {
+    hipLaunchKernelGGL(k_pimd_energy, dim3(NHC_BLOCKS), dim3(256), 0, c->st, a);
+    hipLaunchKernelGGL(k_pimd_energy_sum, dim3(1), dim3(64), 0, c->st, a, NHC_BLOCKS, c->nhc_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(eo, c->nhc_out, sizeof eo, hipMemcpyDeviceToHost, c->st));
+  }
+  HIPCHK(hipStreamSynchronize(c->st));
+  assemble(c, flags, out, false);
+  if (econs) {
+    econs[0] = eo[0];
+    econs[1] = eo[1];
+    econs[2] = out->vintert + out->vintrat + out->kin_harm;
+    econs[3] = econs[0] + econs[1] + econs[2];
+  }
+  return PMX_OK;
+}
+
+int pmx_set_nhc_state(pmx_ctx *c, int len_nhc, const double *st) {
+  if (!c || !st || len_nhc < 1 || len_nhc > 7) return PMX_ERR_ARG;
+  Setup &s = c->s;
+  const int N = s.N, P = s.P, L = len_nhc;
+  if (P < 2) return PMX_ERR_ARG;
+  HIPCHK(hipSetDevice(c->device));
+  TRY_RC(nhc_alloc(c, L));
+  const size_t NP = (size_t)N * P;
+  std::vector<double> v((size_t)L * 3 * NP, 0.0), x((size_t)L * 3 * NP, 0.0), cls(2 * L + 1, 1.0);
+  for (int j = 0; j < 2 * L; j++) cls[j] = st[j];
+  for (int m = 1; m < P; m++)
+    for (int i = 0; i < N; i++)
+      for (int d = 0; d < 3; d++) {
+        const double *src = st + 2 * L + ((size_t)(m - 1) * 3 * N + 3 * i + d) * 2 * L;
+        for (int j = 0; j < L; j++) {
+          const size_t o = ((size_t)(j * 3 + d)) * NP + (size_t)i * P + m;
+          v[o] = src[j];
+          x[o] = src[L + j];
+        }
+      }
+  HIPCHK(hipMemcpy(c->vnhc, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->xnhc, x.data(), x.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->nhc_cls, cls.data(), cls.size() * sizeof(double), hipMemcpyHostToDevice));
+  return PMX_OK;
+}
+
+int pmx_get_nhc_state(pmx_ctx *c, int len_nhc, double *st) {
+  if (!c || !st) return PMX_ERR_ARG;
+  Setup &s = c->s;
+  const int N = s.N, P = s.P, L = len_nhc;
+  if (P < 2 || L != c->nhc_len) {
+    set_err(c, "pmx_get_nhc_state: no chain state of length %d", len_nhc);
+    return PMX_ERR_STATE;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  const size_t NP = (size_t)N * P;
+  std::vector<double> v((size_t)L * 3 * NP), x((size_t)L * 3 * NP), cls(2 * L + 1);
+  HIPCHK(hipStreamSynchronize(c->st));
+  HIPCHK(hipMemcpy(v.data(), c->vnhc, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(x.data(), c->xnhc, x.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(cls.data(), c->nhc_cls, cls.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int j = 0; j < 2 * L; j++) st[j] = cls[j];
+  for (int m = 1; m < P; m++)
+    for (int i = 0; i < N; i++)
+      for (int d = 0; d < 3; d++) {
+        double *dst = st + 2 * L + ((size_t)(m - 1) * 3 * N + 3 * i + d) * 2 * L;
+        for (int j = 0; j < L; j++) {
+          const size_t o = ((size_t)(j * 3 + d)) * NP + (size_t)i * P + m;
+          dst[j] = v[o];
+          dst[L + j] = x[o];
+        }
+      }
+  return PMX_OK;
+}
+
 // New force-field parameters for the same topology: charges (the set of
 // charged sites fixed), LJ sigma and eps, bond and bend coefficients.  The
 // parameter-dependent setup (LJ tables, tail constant, charge sums) is
@@ -2124,7 +2549,7 @@ int pmx_set_force_field(pmx_ctx *c, const pmx_system *sys) {
   const std::vector<double> pp = pair_constants(c);
   HIPCHK(hipMemcpy(c->pair_pp, pp.data(), pp.size() * sizeof(double), hipMemcpyHostToDevice));
   if (c->pair_mode == 2)
-    if (int rc = build_coulomb_table(c)) return rc;
+    if (int rc = build_pair_tables(c)) return rc;
   c->bc_h.assign(sys->bond_c, sys->bond_c + 7 * (size_t)c->nbond_type);
   if (c->nbond_type) {
     HIPCHK(hipMemcpy(c->bc, sys->bond_c, 7 * (size_t)c->nbond_type * sizeof(double), hipMemcpyHostToDevice));
@@ -2198,6 +2623,9 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
     a.ctab = c->ctab;
     a.ct_r0 = c->ct_r0;
     a.ct_idr = c->ct_idr;
+    a.ftab = c->ftab;
+    a.nftab = c->nftab;
+    for (size_t t = 0; t < c->ftab_off.size(); t++) a.ftoff[t] = c->ftab_off[t];
     a.rheal = c->par.rheal;
     a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
     a.swit = (c->par.ishift == PMX_SHIFT_SWIT);
@@ -2348,6 +2776,9 @@ int pmx_get_verlet_list(pmx_ctx *c, int *nter, int *jver_off, int *jter, int cap
     return PMX_ERR_STATE;
   }
   const int N = c->s.N;
+  HIPCHK(hipSetDevice(c->device));
+  if (!c->ref_valid)
+    if (int rc = build_ref_list(c)) return rc;
   HIPCHK(hipMemcpy(nter, c->nfull, N * sizeof(int), hipMemcpyDeviceToHost));
   long tot = 0;
   for (int i = 0; i < N; i++) {

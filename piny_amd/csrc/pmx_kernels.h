@@ -10,10 +10,15 @@ namespace pmx {
 struct ListArgs {
   int N, cap, pimd;
   int full;  // 1: jforce holds every interacting j != i (both directions)
+  int ref_only;  // k_list_build: write the reference's half list only (jfull, nfull)
   // link cells (when use_cells): nc[3] cells per axis of fractional space,
   // sites of cell k at cell_sites[cell_start[k] .. + cell_cnt[k]]
   int use_cells, nc[3];
   int *site_cell, *site_pos, *cell_cnt, *cell_start, *cell_sites;
+  // k_list_cells: the sites in cell order, (bead-1 x, y, z, q) and
+  // (xmod x, y, z, type as a double), and the site index
+  double4 *cell_p, *cell_m;
+  int *cell_i;
   const double *bx, *by, *bz;  // bead-1 positions [N]
   const double *xm, *ym, *zm;  // xmod [N]
   const int *type;
@@ -42,7 +47,8 @@ enum {
 };
 // LJ type pairs whose eps/sigma gradients one context forms
 constexpr int MAX_LJ_GRAD = 4;
-constexpr int CT_N = 1024;  // intervals of the mixed kernel's Coulomb table (16 KB of LDS)
+constexpr int CT_N = 1024;  // intervals of the mixed kernel's tables (16 KB of LDS each)
+constexpr int MAX_FTAB = 4;  // switched pair-function tables the mixed kernel holds in LDS
 
 // List entries of the force kernels carry the type pair in the top byte.
 constexpr int ENTRY_SHIFT = 24;
@@ -58,6 +64,13 @@ struct PairArgs {
   // c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1] across the interval
   const float4 *ctab;
   float ct_r0, ct_idr;
+  // the switched, shifted pair functions whole (erfc term per unit charge
+  // product, LJ term) on the same grid: nftab tables of CT_N pieces, and per
+  // type pair the offsets (erfc table, LJ table or -1) in pieces; ftab is
+  // null when the functions need more than MAX_FTAB tables
+  const float4 *ftab;
+  int nftab;
+  int2 ftoff[MAX_PAIRS];
   int swit;
   const double *pp;            // [npair][PP_NUM]
   const double *x, *y, *z;     // [N*P]
@@ -122,6 +135,9 @@ constexpr int EWALD_BLOCK = 256;
 constexpr int FFT_MAX_PASSES = 12;
 constexpr int FFT_MAX_PLANE = 4096;       // complex values per plane of the smaller class
 constexpr size_t FFT_LDS_MAX = 163840;    // bytes of LDS one workgroup may hold on MI355X
+// dynamic LDS the FFT kernels may ask for: the workgroup's LDS less the
+// static 256 doubles of block_partials (the convolution kernel's sums)
+constexpr size_t FFT_LDS_DYN = FFT_LDS_MAX - 256 * sizeof(double);
 
 struct FftArgs {
   int P, f32;
@@ -215,12 +231,52 @@ struct FinishArgs {
 
 struct ModeArgs {
   int N, P;
+  int stag;  // staging transform (path_int_md_typ staging) instead of normal modes
   const double *x, *y, *z;
   double *fx, *fy, *fz;
+  double *fxt, *fyt, *fzt;  // the bead forces before the transform (may be null)
   const double *prekf, *veig;
   const double *cs;  // [2P] cos, sin of 2 pi m/P
-  double *ake;
+  double *part;      // spring energy per workgroup
 };
+// sites of one k_mode_forces workgroup (four wavefronts of 64 / P sites)
+inline int mode_blocks(int N, int P) {
+  const int per = 4 * (64 / P);
+  return (N + per - 1) / per;
+}
+
+// PIMD NVT integrator (int_NVT_pimd with RESPA off).  Velocities are mode
+// velocities, atom-major like the positions: v[i*P + m] for mode m.  Chain
+// state on the device, chain-major: the chains of modes 2..P for every
+// (site, mode) with m >= 1 as vnhc[(j*3 + d)*N*P + i*P + m] and xnhc
+// likewise (j < len, d the direction; entries of mode 0 and of ghost sites
+// unused); the centroid chain as cls[j] (v) and cls[len + j] (x).
+struct NhcArgs {
+  int N, P, len, nres, nyosh;
+  double wdti2[10], wdti4[10], wdti8[10];  // set_yosh weights times the step, 1-based
+  double cmass[8], cgkt[8];                // the centroid chain (1-based)
+  double bmass[8], bgkt[8];                // every bead chain (1-based)
+  double dt, dt2;
+  const double *mass;   // [N] physical masses
+  const double *mfac;   // [P] mode mass factor: veig (1 for mode 1) times gamma^2 for modes 2..P
+  const int *ghost;     // [N] 1 for ghost sites (the null thermostat)
+  const double *fx, *fy, *fz;
+  double *vx, *vy, *vz;
+  double *vnhc, *xnhc;  // [len][3][N*P]
+  double *cls;          // [2 len] centroid chain; cls[2 len] holds the velocity scale of the last chain step
+  double *part;         // per-workgroup partial sums
+  int stag;
+  const double *cs;     // [2P] cos, sin of 2 pi m/P
+  double *x, *y, *z, *xm, *ym, *zm;
+};
+__global__ void k_nhc_ckin(NhcArgs a);
+__global__ void k_nhc_class(NhcArgs a, int G);
+__global__ void k_nhc_kick(NhcArgs a, int phase);
+__global__ void k_nhc_scale0(NhcArgs a);
+__global__ void k_mode_drift(NhcArgs a);
+__global__ void k_pimd_energy(NhcArgs a);
+__global__ void k_pimd_energy_sum(NhcArgs a, int G, double *out);
+constexpr int NHC_BLOCKS = 256;  // workgroups of the grid-stride kinetic and energy sums
 
 // Values per (block, bead) partial-sum row of each stage.
 constexpr int PAIR_NV = 9;   // vvdw, vcoul, 6 virial, pair count
@@ -243,6 +299,11 @@ struct RedArgs {
   RedStage st[MAX_RED_STAGES];
   const double *part;
   double *acc;  // [P][nslot], every entry written
+  // one scalar besides: *sacc = the sum of spart[0 .. sG) (0 when sG = 0),
+  // by the wavefront after the per-bead ones
+  const double *spart;
+  int sG;
+  double *sacc;
 };
 
 __global__ void k_bead_to_atom(int N, int P, const double *src, double *dst);
@@ -266,6 +327,11 @@ __global__ void k_list_build(ListArgs a);
 __global__ void k_cell_count(ListArgs a);
 __global__ void k_cell_scan(int ncell, const int *cnt, int *start);
 __global__ void k_cell_scatter(ListArgs a);
+constexpr int CELL_SORT_MAX = 1024;  // sites of one cell k_cell_order sorts in LDS
+constexpr int CELL_ROWS_MAX = 512;   // sites of one cell whose rows one k_list_cells workgroup keeps
+constexpr int CELL_TILE = 320;       // candidate sites one k_list_cells workgroup stages in LDS at a time
+__global__ void k_cell_order(ListArgs a);
+hipError_t launch_list_cells(const ListArgs &a, int halfwidth, int ncell, hipStream_t st);
 __global__ void k_sort_rows(int cap, int n2, const int *nfull, const int *nforce, int *jfull, int *jforce);
 __global__ void k_pme_bin(PmeArgs a);
 __global__ void k_pme_scan(PmeArgs a);

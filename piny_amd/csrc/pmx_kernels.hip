@@ -31,6 +31,14 @@ __global__ void k_fill_double(double *p, double v, size_t n) {
 // wavefront per (bead, slot); blocks are summed in a fixed order.
 __global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {
   const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (w == a.P * a.nslot) {  // the scalar
+    if (!a.sacc) return;
+    double s = 0.0;
+    for (int g = lane; g < a.sG; g += 64) s += a.spart[g];
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) *a.sacc = s;
+    return;
+  }
   if (w >= a.P * a.nslot) return;  // uniform over the wavefront
   const int b = w / a.nslot, slot = w % a.nslot;
   // four independent sums per lane keep four loads in flight
@@ -158,13 +166,14 @@ __global__ void k_site_umax(int N, int P, const float4 *__restrict__ u, double *
 
 // Bead-1 copy, largest displacement since the last list build
 // (check_ver_list, nbr_list_control.c:182-239), bead spread
-// (get_pimd_spread, pimd_estimators.c:121-222) and, when asked, the bead
-// centroid as xmod, the mode-1 coordinate of the normal-mode transform
-// (control_pimd_trans.c:175-181 takes xmod from mode 1).  Each workgroup
-// writes its maxima of the squares; a group of P lanes
-// holds one site's beads (64 / P sites per wavefront; with P > 64 one site
-// per wavefront in chunks of 64 beads), so the loads are coalesced; the
-// centroid sums run bead 0, 1, ... in order through lane shuffles.
+// (get_pimd_spread, pimd_estimators.c:121-222) and, when asked, xmod, the
+// mode-1 coordinate of the bead transform (control_pimd_trans.c:175-181
+// takes xmod from mode 1): the bead centroid for normal modes (centroid = 1)
+// or bead 1 for staging (centroid = 2).  Each workgroup writes its maxima
+// of the squares; a group of P lanes holds one site's beads (64 / P sites
+// per wavefront; with P > 64 one site per wavefront in chunks of 64 beads),
+// so the loads are coalesced; the group's lead lane forms the centroid sum
+// over beads 0, 1, ... in order from the wavefront's LDS rows.
 __device__ __forceinline__ double shfl_d(double v, int src) { return __shfl(v, src, 64); }
 
 __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *__restrict__ x,
@@ -183,7 +192,8 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
   const int lead = g * G;                   // lane of the site's bead 0 (of each chunk)
   const int nunit = (N + S - 1) / S;
   double dmax = 0.0, smax = 0.0;
-  // grid-stride over wavefront units, so few workgroups make the global atomics
+  // grid-stride over wavefront units: a bounded number of workgroups, each
+  // writing one pair of maxima that the host reduces
   for (int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); wave < nunit;
        wave += (int)((gridDim.x * blockDim.x) >> 6)) {
     const int i = wave * S + g;
@@ -239,10 +249,14 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
       b0z[i] = zi;
       const double dx = xi - x0[i], dy = yi - y0[i], dz = zi - z0[i];
       dmax = fmax(dmax, dx * dx + dy * dy + dz * dz);
-      if (centroid) {
+      if (centroid == 1) {
         xm[i] = sx / P;
         ym[i] = sy / P;
         zm[i] = sz / P;
+      } else if (centroid == 2) {
+        xm[i] = xi;
+        ym[i] = yi;
+        zm[i] = zi;
       }
     }
   }
@@ -387,6 +401,7 @@ __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
           inter = in && (a.pt.lj[t] || (qi * a.q[j] != 0.0));
         }
       }
+      if (a.ref_only) inter = false;
       const unsigned long long mk = __ballot(keep), mi = __ballot(inter);
       const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
       if (keep) {
@@ -403,7 +418,7 @@ __global__ void __launch_bounds__(256) k_list_build(ListArgs a) {
   }
   if (lane == 0) {
     a.nfull[i] = nfull;
-    a.nforce[i] = nforce;
+    if (!a.ref_only) a.nforce[i] = nforce;
     const int mx = nfull > nforce ? nfull : nforce;
     if (mx > a.cap) atomicMax(a.overflow, mx);
     atomicAdd(a.overflow + 1, nfull);  // pair count of the reference list
@@ -460,6 +475,161 @@ __global__ void __launch_bounds__(256) k_cell_scatter(ListArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.N) return;
   a.cell_sites[a.cell_start[a.site_cell[i]] + a.site_pos[i]] = i;
+}
+
+// Puts each cell's sites in ascending index order (one workgroup per cell;
+// the counting sort's atomics leave them in arbitrary order), so that the
+// cell order, and the force-list order that follows it, is the same on
+// every build; then writes the sites' records in that order.  A cell of
+// more than CELL_SORT_MAX sites sets overflow[2] and the host builds with
+// the plain scan instead.
+__global__ void __launch_bounds__(256) k_cell_order(ListArgs a) {
+  __shared__ int ids[CELL_SORT_MAX];
+  const int c = blockIdx.x, n = a.cell_cnt[c], o = a.cell_start[c];
+  if (n > CELL_SORT_MAX) {  // uniform over the workgroup
+    if (threadIdx.x == 0) a.overflow[2] = 1;
+    return;
+  }
+  for (int k = threadIdx.x; k < n; k += blockDim.x) ids[k] = a.cell_sites[o + k];
+  __syncthreads();
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int v = ids[k];
+    int rank = 0;
+    for (int m = 0; m < n; m++) rank += ids[m] < v;
+    const int d = o + rank;
+    a.cell_sites[d] = v;
+    a.cell_i[d] = v;
+    a.cell_p[d] = make_double4(a.bx[v], a.by[v], a.bz[v], a.q[v]);
+    a.cell_m[d] = a.pimd ? make_double4(a.xm[v], a.ym[v], a.zm[v], (double)a.type[v])
+                         : make_double4(0.0, 0.0, 0.0, (double)a.type[v]);
+  }
+}
+
+// Verlet list by link cells (make_lnk_lst.c:37-355, lnk_ver_gen_pad in
+// verlist_control.c:1738-2254): cells of fractional space at least
+// r_list / H wide, at least 2H + 1 per axis, so that the (2H + 1)^3 cells
+// around a site's cell hold every site within the list radius, each once.
+// One workgroup per cell.  For each row of 2H + 1 cells along a of the
+// stencil (contiguous in cell order, up to the wrap) the workgroup stages
+// the sites' records in LDS (CELL_TILE at a time); each wavefront takes
+// the cell's own sites in turn and tests 64 staged candidates per step:
+// the reference's image separation and cut+skin test (min_image_sep, the
+// same bits as the plain scan), then, for pairs inside, the exclusion
+// list.  A ballot appends the interacting pairs to the site's force-list
+// row (every j != i, or j < i for the half list), in stencil order; the
+// reference's half-list pairs (j < i, every type pair) are only counted
+// (nver).  Rows of the cell's sites are counted in LDS across the tiles.
+template <int H>
+__global__ void __launch_bounds__(256) k_list_cells(ListArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double4 tp[CELL_TILE], tm[CELL_TILE];
+  __shared__ int ti[CELL_TILE];
+  __shared__ int rowf[CELL_ROWS_MAX], rowr[CELL_ROWS_MAX];
+  __shared__ int red[4];
+  constexpr int W = 2 * H + 1;
+  const int c = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int ni = a.cell_cnt[c], oi = a.cell_start[c];
+  if (ni > CELL_ROWS_MAX) {  // uniform over the workgroup
+    if (threadIdx.x == 0) a.overflow[2] = 1;
+    return;
+  }
+  for (int k = threadIdx.x; k < ni; k += blockDim.x) {
+    rowf[k] = 0;
+    rowr[k] = 0;
+  }
+  const int ca = c % a.nc[0], cb = (c / a.nc[0]) % a.nc[1], cc = c / (a.nc[0] * a.nc[1]);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int st = 0; st < W * W; st++) {
+    const int nb = (cb + st % W - H + a.nc[1]) % a.nc[1], nz = (cc + st / W - H + a.nc[2]) % a.nc[2];
+    // the row's cells: their starts and the running count before each
+    int rs[W], pre[W + 1];
+    pre[0] = 0;
+#pragma unroll
+    for (int d = 0; d < W; d++) {
+      const int na = (ca + d - H + a.nc[0]) % a.nc[0];
+      const int cell = (nz * a.nc[1] + nb) * a.nc[0] + na;
+      rs[d] = a.cell_start[cell];
+      pre[d + 1] = pre[d] + a.cell_cnt[cell];
+    }
+    const int tot = pre[W];
+    for (int t0 = 0; t0 < tot; t0 += CELL_TILE) {
+      const int nt = min(CELL_TILE, tot - t0);
+      __syncthreads();  // the previous tile's readers are done (and the row counters are cleared)
+      for (int e = threadIdx.x; e < nt; e += blockDim.x) {
+        const int g = t0 + e;
+        int src = 0;
+#pragma unroll
+        for (int d = 0; d < W; d++)
+          if (g >= pre[d] && g < pre[d + 1]) src = rs[d] + (g - pre[d]);
+        tp[e] = a.cell_p[src];
+        tm[e] = a.cell_m[src];
+        ti[e] = a.cell_i[src];
+      }
+      __syncthreads();
+      for (int s = w; s < ni; s += 4) {
+        const double4 pi = a.cell_p[oi + s], mi = a.cell_m[oi + s];
+        const int i = a.cell_i[oi + s], typi = (int)mi.w;
+        const int eo = a.ex_off[i], en = a.ex_num[i];
+        int nf = rowf[s], nr = rowr[s];
+        int *row = a.jforce + (size_t)i * a.cap;
+        for (int b0 = 0; b0 < nt; b0 += 64) {
+          const int e = b0 + lane;
+          bool keep = false, inter = false;
+          int j = 0, t = 0;
+          if (e < nt) {
+            j = ti[e];
+            if (j != i) {
+              const double4 pj = tp[e], mj = tm[e];
+              double dx, dy, dz;
+              const double r2 =
+                  min_image_sep(a.cell, a.pimd, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, mi.x, mi.y, mi.z, mj.x, mj.y, mj.z,
+                                dx, dy, dz);
+              t = pair_index(a.pt.ntype, typi, (int)mj.w);
+              if (r2 < a.pt.cutskin2[t]) {
+                bool excl = false;
+                for (int k = 0; k < en; k++) excl |= (a.ex_j[eo + k] == j);
+                if (!excl) {
+                  keep = j < i;
+                  inter = (a.full || j < i) && (a.pt.lj[t] || (pi.w * pj.w != 0.0));
+                }
+              }
+            }
+          }
+          const uint64_t mi2 = __ballot(inter);
+          if (inter) {
+            const int p = nf + __popcll(mi2 & below);
+            if (p < a.cap) row[p] = j | (t << ENTRY_SHIFT);
+          }
+          nf += __popcll(mi2);
+          nr += __popcll(__ballot(keep));
+        }
+        if (lane == 0) {
+          rowf[s] = nf;
+          rowr[s] = nr;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  int sum = 0;
+  for (int s = threadIdx.x; s < ni; s += blockDim.x) {
+    const int i = a.cell_i[oi + s], nf = rowf[s];
+    a.nforce[i] = nf;
+    if (nf > a.cap) atomicMax(a.overflow, nf);
+    sum += rowr[s];
+  }
+  for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if (lane == 0) red[w] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(a.overflow + 1, red[0] + red[1] + red[2] + red[3]);
+}
+
+hipError_t launch_list_cells(const ListArgs &a, int halfwidth, int ncell, hipStream_t st) {
+  if (halfwidth == 2)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_list_cells<2>), dim3(ncell), dim3(256), 0, st, a);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_list_cells<1>), dim3(ncell), dim3(256), 0, st, a);
+  return hipGetLastError();
 }
 
 // Sorts each list row ascending by site index (one workgroup per row,
@@ -858,19 +1028,31 @@ __device__ __forceinline__ void wave_sync() {
 // (erfc(alp r)/r from an LDS table of cubic pieces of PINY's polynomial).
 // Energies, virial and forces are summed in FP32 within a chunk and in FP64
 // across chunks and lanes.
-template <int B, bool PIMD>
+//
+// With TAB the LJ and erfc terms come whole from the switched, shifted
+// function tables (build_pair_tables): per (pair, bead) one table index and
+// one cubic piece per term, no switch arithmetic and no branch on the
+// switching shell.  Without TAB (more distinct functions than MAX_FTAB
+// tables) pair_direct_f32 evaluates LJ and the switch directly with the
+// erfc table.
+template <int B, bool PIMD, bool TAB>
 __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   __shared__ float4 sppA[MAX_PAIRS], sppB[MAX_PAIRS];  // pair_direct_f32's constants per type pair
-  __shared__ float4 sct[CT_N];                          // the Coulomb table
+  __shared__ int4 stt[MAX_PAIRS];                       // TAB: (cut^2 as bits, erfc table, LJ table or -1, 0)
+  extern __shared__ float4 sct[];                       // the tables: TAB nftab * CT_N, else CT_N pieces
   __shared__ float4 sdq[4][64];
   __shared__ int2 sent[4][64];  // (j * P, type pair) of the staged entries
   for (int t = threadIdx.x; t < a.npair; t += blockDim.x) {
     const double *pp = a.pp + (size_t)t * PP_NUM;
     sppA[t] = make_float4((float)pp[PP_CUT2], (float)pp[PP_CUT], (float)pp[PP_C12], (float)pp[PP_C6]);
     sppB[t] = make_float4((float)pp[PP_VSLJ], (float)pp[PP_VSC], (float)pp[PP_LJ], 0.f);
+    stt[t] = make_int4(__float_as_int((float)pp[PP_CUT2]), a.ftoff[t].x, a.ftoff[t].y, 0);
   }
-  for (int k = threadIdx.x; k < CT_N; k += blockDim.x) sct[k] = a.ctab[k];
+  const int ntab = TAB ? a.nftab * CT_N : CT_N;
+  const float4 *src = TAB ? a.ftab : a.ctab;
+  for (int k = threadIdx.x; k < ntab; k += blockDim.x) sct[k] = src[k];
   __syncthreads();
+  const float r0i = a.ct_r0 * a.ct_idr;
   constexpr int S = 64 / B;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int slot = lane / B;
@@ -932,7 +1114,6 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
       auto pair = [&](int p, const float4 &uj) {
         const float4 d = sdq[w][p];
         const int t = sent[w][p].y;
-        const float4 A = sppA[t], B2 = sppB[t];
         float dx = d.x, dy = d.y, dz = d.z;
         if (PIMD) {
           dx += ui.x - uj.x;
@@ -940,10 +1121,35 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           dz += ui.z - uj.z;
         }
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (!bact || !(r2 < A.x)) return;
-        const float rinv = __frsqrt_rn(r2);
-        float ev_lj, ev_c;
-        const float dvd = pair_direct_f32(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c, sct);
+        float ev_lj, ev_c, dvd;
+        if constexpr (TAB) {
+          const int4 T = stt[t];
+          if (!bact || !(r2 < __int_as_float(T.x))) return;
+          const float rinv = __frsqrt_rn(r2);
+          // the piece of r = r2 rinv; below the grid the first piece extrapolates
+          const float u = (r2 * rinv) * a.ct_idr - r0i;
+          const unsigned k = min(__float2uint_rz(u), (unsigned)(CT_N - 1));
+          const float tt = u - (float)k;
+          float dv = 0.f;  // dV/dt of the pair
+          ev_c = 0.f;
+          ev_lj = 0.f;
+          if (d.w != 0.f) {
+            const float4 cc = sct[T.y + k];
+            ev_c = d.w * (cc.x + tt * (cc.y + tt * (cc.z + tt * cc.w)));
+            dv = d.w * (cc.y + tt * (2.f * cc.z + 3.f * tt * cc.w));
+          }
+          if (T.z >= 0) {
+            const float4 cl = sct[T.z + k];
+            ev_lj = cl.x + tt * (cl.y + tt * (cl.z + tt * cl.w));
+            dv += cl.y + tt * (2.f * cl.z + 3.f * tt * cl.w);
+          }
+          dvd = -(dv * a.ct_idr) * rinv;  // -(1/r) dV/dr
+        } else {
+          const float4 A = sppA[t], B2 = sppB[t];
+          if (!bact || !(r2 < A.x)) return;
+          const float rinv = __frsqrt_rn(r2);
+          dvd = pair_direct_f32(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c, sct);
+        }
         const float fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
         cvlj += ev_lj;
         cvc += ev_c;
@@ -1734,52 +1940,126 @@ __global__ void k_ghost_force(int ng, int P, const int *__restrict__ gatom, cons
   fz[ig] = 0.0;
 }
 
-// normal-mode forces and springs for one site and one Cartesian direction:
-// convert_pimd_force_cent (transform_cnt.c:415-489) of the bead forces,
-// mode coordinates of convert_pimd_mode_cent (:240-316) and the spring
-// force -2 prekf veig u of mode_energy_control (pimd_estimators.c:71-95).
-__global__ void k_mode_forces(ModeArgs a) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool act = t < a.N * 3;
+// Mode forces and springs of energy_control_pimd (energy_control_pimd.c:
+// 198-214): the bead forces transformed to mode coordinates plus the spring
+// force -2 prekf veig u of mode_energy_control (pimd_estimators.c:71-95),
+// with the modes u of the bead positions.  Normal modes (path_int_md_typ
+// centroid): convert_pimd_mode_cent / convert_pimd_force_cent
+// (transform_cnt.c:240-316, 415-489), a DFT over the beads in PINY's storage
+// order (0 centroid, P-1 the alternating mode, 2k-1 / 2k the real /
+// imaginary part of frequency k), mode positions divided by P and paired
+// mode forces doubled.  Staging: convert_pimd_mode_stag /
+// convert_pimd_force_stag (transform_stg.c:31-160), u_1 = x_1,
+// u_k = x_k - (rat1_k x_(k+1) + rat2_k x_1), the force of mode 1 the sum of
+// the bead forces and the others the recurrence F_k = f_k + rat1_(k-1) F_(k-1).
+//
+// A group of P lanes holds one site's beads (64 / P sites per wavefront,
+// P <= 64); each lane loads its bead, keeps the bead forces in fxt, stages
+// (x, y, z, fx, fy, fz) in the wavefront's LDS rows, then forms one mode
+// from the site's rows (reads of one address per group: LDS broadcasts).
+// The spring energy of a workgroup goes to part[blockIdx.x], summed in a
+// fixed order.
+__global__ void __launch_bounds__(256) k_mode_forces(ModeArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double3 spos[4][64], sfrc[4][64];
+  __shared__ double2 stw[64];
+  __shared__ double red[4];
+  const int P = a.P;
+  for (int m = threadIdx.x; m < P; m += blockDim.x) stw[m] = make_double2(a.cs[2 * m], a.cs[2 * m + 1]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int S = 64 / P, g = lane / P, m = lane - g * P, lead = g * P;
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * S + g;
+  const bool act = g < S && i < a.N;
+  const size_t t = (size_t)(act ? i : 0) * P + m;
+  double3 xb = make_double3(0.0, 0.0, 0.0), fb = xb;
+  if (act) {
+    xb = make_double3(a.x[t], a.y[t], a.z[t]);
+    fb = make_double3(a.fx[t], a.fy[t], a.fz[t]);
+    if (a.fxt) {
+      a.fxt[t] = fb.x;
+      a.fyt[t] = fb.y;
+      a.fzt[t] = fb.z;
+    }
+  }
+  spos[w][lane] = xb;
+  sfrc[w][lane] = fb;
+  __syncthreads();  // the twiddles and every wavefront's rows
   double ake = 0.0;
   if (act) {
-    const int i = t / 3, d = t % 3;
-    const double *x = d == 0 ? a.x : (d == 1 ? a.y : a.z);
-    double *f = d == 0 ? a.fx : (d == 1 ? a.fy : a.fz);
-    const int P = a.P;
-    double xr[64], fr[64];
-    for (int b = 0; b < P; b++) {
-      xr[b] = x[(size_t)i * P + b];
-      fr[b] = f[(size_t)i * P + b];
-    }
-    const double pk = a.prekf[i];
-    for (int m = 0; m < P; m++) {
-      // mode m in PINY's storage order: 0 centroid, P-1 alternating,
-      // 2k-1 real part and 2k imaginary part of frequency k
-      int k = 0, im = 0;
-      if (m == 0) k = 0;
-      else if (m == P - 1) k = P / 2;
-      else {
+    const double3 *px = spos[w] + lead, *pf = sfrc[w] + lead;
+    double3 u, F;
+    if (a.stag) {
+      const double3 x1 = px[0];
+      if (m == 0) {
+        u = x1;
+        F = pf[0];
+        for (int b = 1; b < P; b++) {
+          F.x += pf[b].x;
+          F.y += pf[b].y;
+          F.z += pf[b].z;
+        }
+      } else {
+        // 1-based bead ip = m + 1: rat1 = (ip-1)/ip, rat2 = 1/ip; the last
+        // bead pairs with bead 1
+        const double r1 = ((double)m) / ((double)(m + 1)), r2 = 1.0 / ((double)(m + 1));
+        const double3 nx = (m + 1 < P) ? px[m + 1] : x1;
+        u = make_double3(xb.x - (r1 * nx.x + r2 * x1.x), xb.y - (r1 * nx.y + r2 * x1.y),
+                         xb.z - (r1 * nx.z + r2 * x1.z));
+        F = pf[1];
+        for (int j = 2; j <= m; j++) {
+          const double r = ((double)(j - 1)) / ((double)j);
+          F.x = pf[j].x + (r * F.x);
+          F.y = pf[j].y + (r * F.y);
+          F.z = pf[j].z + (r * F.z);
+        }
+      }
+    } else {
+      int k = 0, im = 0;  // frequency and real (0) / imaginary (1) part of mode m
+      if (m == P - 1) k = P / 2;
+      else if (m > 0) {
         k = (m + 1) / 2;
         im = (m % 2 == 0);
       }
-      double su = 0.0, sf = 0.0;
+      u = make_double3(0.0, 0.0, 0.0);
+      F = u;
+      int ph = 0;  // k b mod P
       for (int b = 0; b < P; b++) {
-        const int ph = (k * b) % P;
-        const double cs = a.cs[2 * ph], sn = a.cs[2 * ph + 1];  // cos, sin of 2 pi ph/P
-        const double w = im ? sn : cs;                          // exp(+i) real / imaginary part
-        su += xr[b] * w;
-        sf += fr[b] * w;
+        const double2 cs = stw[ph];
+        const double wgt = im ? cs.y : cs.x;  // exp(+2 pi i k b / P): real or imaginary part
+        const double3 xv = px[b], fv = pf[b];
+        u.x += xv.x * wgt;
+        u.y += xv.y * wgt;
+        u.z += xv.z * wgt;
+        F.x += fv.x * wgt;
+        F.y += fv.y * wgt;
+        F.z += fv.z * wgt;
+        ph += k;
+        if (ph >= P) ph -= P;
       }
-      su /= (double)P;
-      if (m != 0 && m != P - 1) sf *= 2.0;
-      const double fk = pk * a.veig[m + 1];
-      ake += fk * su * su;
-      f[(size_t)i * P + m] = sf + (-2.0 * fk * su);
+      const double dP = (double)P;
+      u.x /= dP;
+      u.y /= dP;
+      u.z /= dP;
+      if (m != 0 && m != P - 1) {
+        F.x *= 2.0;
+        F.y *= 2.0;
+        F.z *= 2.0;
+      }
     }
+    const double fk = a.prekf[i] * a.veig[m + 1];
+    ake = fk * (u.x * u.x + u.y * u.y + u.z * u.z);
+    a.fx[t] = F.x + (-2.0 * fk * u.x);
+    a.fy[t] = F.y + (-2.0 * fk * u.y);
+    a.fz[t] = F.z + (-2.0 * fk * u.z);
   }
   for (int off = 32; off >= 1; off >>= 1) ake += __shfl_xor(ake, off, 64);
-  if ((threadIdx.x & 63) == 0 && ake != 0.0) atomic_add(a.ake, ake);
+  if (lane == 0) red[w] = ake;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int v = 0; v < (int)(blockDim.x >> 6); v++) s += red[v];
+    a.part[blockIdx.x] = s;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1833,8 +2113,22 @@ void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_
   if (full) {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged_tab<B>), grid, block, 0, st, a);
     else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged_dir<B>), grid, block, 0, st, a);
-    else if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false>), grid, block, 0, st, a);
+    else if (a.ftab) {
+      const size_t lds = (size_t)a.nftab * CT_N * sizeof(float4);
+      if (a.pimd) {
+        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, true, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, true>), grid, block, lds, st, a);
+      } else {
+        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, false, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, true>), grid, block, lds, st, a);
+      }
+    } else {
+      const size_t lds = (size_t)CT_N * sizeof(float4);
+      if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, false>), grid, block, lds, st, a);
+      else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, false>), grid, block, lds, st, a);
+    }
   } else {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 0>), grid, block, 0, st, a);
     else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 1>), grid, block, 0, st, a);
@@ -1883,6 +2177,308 @@ hipError_t launch_pair_grad(const PairArgs &a, int mode, hipStream_t st) {
     default: launch_pair_grad_b<64>(a, mode, grid, st); break;
   }
   return hipGetLastError();
+}
+
+}  // namespace pmx
+
+namespace pmx {
+
+// ---------------------------------------------------------------------------
+// PIMD NVT integrator: int_NVT_pimd (pimd/int_NVT_pimd.c:36-380) with RESPA
+// off.  A step: the chains and the first half kick (int_0_to_dt2_nvt_pimd,
+// int_0_to_dt2_pimd.c:34-196), the drift in mode coordinates, the force call
+// (energy_control_pimd, mode forces plus springs), the second half kick and
+// the chains (int_dt2_to_dt_nvt_pimd, int_dt2_to_dt_pimd.c:31-172).
+
+// One Nose-Hoover chain of apply_NHC_par (md/int_NVT.c:144-330) over its
+// degrees of freedom, whose sum of m v^2 is kin; v, x are the chain's
+// velocities and positions (1-based, len of them).  Returns the factor the
+// degrees of freedom's velocities are scaled by.
+__device__ __forceinline__ double nhc_chain(const NhcArgs &a, const double *mass, const double *gkt, double *v,
+                                            double *x, double kin) {
+#pragma clang fp contract(off)
+  const int L = a.len;
+  double f[8];
+  double sc = 1.0;
+  f[1] = (kin - gkt[1]) / mass[1];
+  for (int ic = 1; ic <= L - 1; ic++) f[ic + 1] = (mass[ic] * v[ic] * v[ic] - gkt[ic + 1]) / mass[ic + 1];
+  for (int ir = 1; ir <= a.nres; ir++) {
+    for (int iy = 1; iy <= a.nyosh; iy++) {
+      v[L] += f[L] * a.wdti4[iy];
+      for (int ic = 1; ic <= L - 1; ic++) {
+        const double aa = exp(-a.wdti8[iy] * v[L + 1 - ic]);
+        v[L - ic] = v[L - ic] * aa * aa + a.wdti4[iy] * f[L - ic] * aa;
+      }
+      const double aa = exp(-a.wdti2[iy] * v[1]);
+      sc *= aa;
+      kin *= aa * aa;
+      for (int ic = 1; ic <= L; ic++) x[ic] += v[ic] * a.wdti2[iy];
+      f[1] = (kin - gkt[1]) / mass[1];
+      for (int ic = 1; ic <= L - 1; ic++) {
+        const double a2 = exp(-a.wdti8[iy] * v[ic + 1]);
+        v[ic] = v[ic] * a2 * a2 + a.wdti4[iy] * f[ic] * a2;
+        f[ic + 1] = (mass[ic] * v[ic] * v[ic] - gkt[ic + 1]) / mass[ic + 1];
+      }
+      v[L] += f[L] * a.wdti4[iy];
+    }
+  }
+  return sc;
+}
+
+// Per-workgroup sums of m v^2 over the real sites' centroid-mode velocities
+// (the centroid chain's kinetic term, apply_NHC_par section I).
+__global__ void __launch_bounds__(256) k_nhc_ckin(NhcArgs a) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < a.N; i += gridDim.x * blockDim.x) {
+    if (a.ghost[i]) continue;
+    const size_t t = (size_t)i * a.P;
+    s += a.mass[i] * (a.vx[t] * a.vx[t] + a.vy[t] * a.vy[t] + a.vz[t] * a.vz[t]);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = blockDim.x >> 1; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.part[blockIdx.x] = red[0];
+}
+
+// The centroid chain (one thread): the kinetic sum in a fixed order, the
+// chain's evolution, and the scale factor in cls[2 len].
+__global__ void k_nhc_class(NhcArgs a, int G) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double kin = 0.0;
+  for (int g = 0; g < G; g++) kin += a.part[g];
+  const int L = a.len;
+  double v[8], x[8];
+  for (int j = 1; j <= L; j++) {
+    v[j] = a.cls[j - 1];
+    x[j] = a.cls[L + j - 1];
+  }
+  const double sc = nhc_chain(a, a.cmass, a.cgkt, v, x, kin);
+  for (int j = 1; j <= L; j++) {
+    a.cls[j - 1] = v[j];
+    a.cls[L + j - 1] = x[j];
+  }
+  a.cls[2 * L] = sc;
+}
+
+// One thread per (site, mode): phase 0 is the first half step (chains, then
+// the kick: int_0_to_dt2_nvt_pimd sections 1-2), phase 1 the second (the
+// kick, then the chains of modes 2..P; the centroid mode is scaled by
+// k_nhc_scale0 once its chain has run).  The centroid mode's velocities are
+// scaled by the centroid chain's factor in phase 0; every other mode's
+// degree of freedom runs its own chain with the mode mass.
+__global__ void __launch_bounds__(256) k_nhc_kick(NhcArgs a, int phase) {
+#pragma clang fp contract(off)
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, NP = (size_t)a.N * a.P;
+  if (t >= NP) return;
+  const int i = (int)(t / a.P), m = (int)(t - (size_t)i * a.P);
+  const double mm = a.mass[i] * a.mfac[m];
+  double *vv[3] = {a.vx, a.vy, a.vz};
+  const double *ff[3] = {a.fx, a.fy, a.fz};
+  const bool real = a.ghost[i] == 0;
+  auto chains = [&]() {
+    if (!real) return;
+    if (m == 0) {
+      if (phase == 0) {
+        const double sc = a.cls[2 * a.len];
+        for (int d = 0; d < 3; d++) vv[d][t] *= sc;
+      }
+      return;
+    }
+    for (int d = 0; d < 3; d++) {
+      double v[8], x[8];
+      for (int j = 1; j <= a.len; j++) {
+        const size_t o = ((size_t)((j - 1) * 3 + d)) * NP + t;
+        v[j] = a.vnhc[o];
+        x[j] = a.xnhc[o];
+      }
+      const double vd = vv[d][t];
+      const double sk = nhc_chain(a, a.bmass, a.bgkt, v, x, mm * vd * vd);
+      vv[d][t] = vd * sk;
+      for (int j = 1; j <= a.len; j++) {
+        const size_t o = ((size_t)((j - 1) * 3 + d)) * NP + t;
+        a.vnhc[o] = v[j];
+        a.xnhc[o] = x[j];
+      }
+    }
+  };
+  if (phase == 0) chains();
+  for (int d = 0; d < 3; d++) vv[d][t] += ff[d][t] * a.dt2 / mm;
+  if (phase == 1) chains();
+}
+
+// the centroid mode's velocities scaled by its chain's factor (phase 1)
+__global__ void __launch_bounds__(256) k_nhc_scale0(NhcArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N || a.ghost[i]) return;
+  const size_t t = (size_t)i * a.P;
+  const double sc = a.cls[2 * a.len];
+  a.vx[t] *= sc;
+  a.vy[t] *= sc;
+  a.vz[t] *= sc;
+}
+
+// The drift of int_0_to_dt2_nvt_pimd section 3: the bead positions of one
+// site to mode coordinates (control_pimd_trans_mode: transform_cnt.c:240-316
+// normal modes, transform_stg.c:94-160 staging), u += v dt, and back
+// (control_pimd_trans_pos: transform_cnt.c:320-410, transform_stg.c:168-226),
+// with mode 1 kept as xmod (control_pimd_trans.c:175-181).  A group of P
+// lanes holds one site's beads (64 / P sites per wavefront); the modes and
+// the positions go through the wavefront's LDS rows.
+__global__ void __launch_bounds__(256) k_mode_drift(NhcArgs a) {
+#pragma clang fp contract(off)
+  __shared__ double3 sp[4][64], su[4][64];
+  __shared__ double2 stw[64];
+  const int P = a.P;
+  for (int k = threadIdx.x; k < P; k += blockDim.x) stw[k] = make_double2(a.cs[2 * k], a.cs[2 * k + 1]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int S = 64 / P, g = lane / P, m = lane - g * P, lead = g * P;
+  const int i = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6) * S + g;
+  const bool act = g < S && i < a.N;
+  const size_t t = (size_t)(act ? i : 0) * P + m;
+  double3 xb = make_double3(0.0, 0.0, 0.0);
+  if (act) xb = make_double3(a.x[t], a.y[t], a.z[t]);
+  sp[w][lane] = xb;
+  __syncthreads();
+  const double3 *px = sp[w] + lead;
+  double3 u = make_double3(0.0, 0.0, 0.0);
+  if (act) {
+    if (a.stag) {
+      const double3 x1 = px[0];
+      if (m == 0) {
+        u = x1;
+      } else {
+        const double r1 = ((double)m) / ((double)(m + 1)), r2 = 1.0 / ((double)(m + 1));
+        const double3 nx = (m + 1 < P) ? px[m + 1] : x1;
+        u = make_double3(xb.x - (r1 * nx.x + r2 * x1.x), xb.y - (r1 * nx.y + r2 * x1.y),
+                         xb.z - (r1 * nx.z + r2 * x1.z));
+      }
+    } else {
+      int k = 0, im = 0;
+      if (m == P - 1) k = P / 2;
+      else if (m > 0) {
+        k = (m + 1) / 2;
+        im = (m % 2 == 0);
+      }
+      int ph = 0;
+      for (int b = 0; b < P; b++) {
+        const double2 cs = stw[ph];
+        const double wgt = im ? cs.y : cs.x;
+        const double3 xv = px[b];
+        u.x += xv.x * wgt;
+        u.y += xv.y * wgt;
+        u.z += xv.z * wgt;
+        ph += k;
+        if (ph >= P) ph -= P;
+      }
+      const double dP = (double)P;
+      u.x /= dP;
+      u.y /= dP;
+      u.z /= dP;
+    }
+    u.x += a.vx[t] * a.dt;
+    u.y += a.vy[t] * a.dt;
+    u.z += a.vz[t] * a.dt;
+  }
+  su[w][lane] = u;
+  __syncthreads();
+  const double3 *pu = su[w] + lead;
+  if (act) {
+    double3 xn;
+    if (a.stag) {
+      // x_P = u_P + x_1, then x_ip = u_ip + (rat1 x_(ip+1) + x_1 rat2) for
+      // ip = P-1 down to 2: each lane runs the recurrence down to its bead
+      const double3 x1 = pu[0];
+      if (m == 0) {
+        xn = x1;
+      } else {
+        double3 xr = make_double3(pu[P - 1].x + x1.x, pu[P - 1].y + x1.y, pu[P - 1].z + x1.z);
+        for (int j = P - 2; j >= m; j--) {
+          const double r1 = ((double)j) / ((double)(j + 1)), r2 = 1.0 / ((double)(j + 1));
+          const double3 uj = pu[j];
+          xr = make_double3(uj.x + (r1 * xr.x + x1.x * r2), uj.y + (r1 * xr.y + x1.y * r2),
+                            uj.z + (r1 * xr.z + x1.z * r2));
+        }
+        xn = xr;
+      }
+    } else {
+      // x_b = u_0 + u_(P-1) cos(pi b) + 2 sum_k (u_(2k-1) cos + u_(2k) sin)(2 pi k b / P)
+      xn = pu[0];
+      const double sgn = (m & 1) ? -1.0 : 1.0;
+      xn.x += pu[P - 1].x * sgn;
+      xn.y += pu[P - 1].y * sgn;
+      xn.z += pu[P - 1].z * sgn;
+      int ph = 0;
+      for (int k = 1; k < P / 2; k++) {
+        ph += m;
+        if (ph >= P) ph -= P;
+        const double2 cs = stw[ph];
+        const double3 re = pu[2 * k - 1], imv = pu[2 * k];
+        xn.x += 2.0 * (re.x * cs.x + imv.x * cs.y);
+        xn.y += 2.0 * (re.y * cs.x + imv.y * cs.y);
+        xn.z += 2.0 * (re.z * cs.x + imv.z * cs.y);
+      }
+    }
+    a.x[t] = xn.x;
+    a.y[t] = xn.y;
+    a.z[t] = xn.z;
+    if (m == 0) {
+      a.xm[i] = u.x;
+      a.ym[i] = u.y;
+      a.zm[i] = u.z;
+    }
+  }
+}
+
+// The extended-system energy's kinetic and chain parts, per workgroup:
+// part[2 blk] = sum 1/2 m v^2 over every mode (get_tvten_pimd), part[2 blk
+// + 1] = sum over the bead chains of 1/2 Q v^2 + gkt x.
+__global__ void __launch_bounds__(256) k_pimd_energy(NhcArgs a) {
+  __shared__ double red[2][256];
+  const size_t NP = (size_t)a.N * a.P;
+  double ke = 0.0, en = 0.0;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < NP; t += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(t / a.P), m = (int)(t - (size_t)i * a.P);
+    const double mm = a.mass[i] * a.mfac[m];
+    ke += 0.5 * mm * (a.vx[t] * a.vx[t] + a.vy[t] * a.vy[t] + a.vz[t] * a.vz[t]);
+    if (m > 0 && !a.ghost[i])
+      for (int d = 0; d < 3; d++)
+        for (int j = 1; j <= a.len; j++) {
+          const size_t o = ((size_t)((j - 1) * 3 + d)) * NP + t;
+          en += 0.5 * a.bmass[j] * a.vnhc[o] * a.vnhc[o] + a.bgkt[j] * a.xnhc[o];
+        }
+  }
+  red[0][threadIdx.x] = ke;
+  red[1][threadIdx.x] = en;
+  __syncthreads();
+  for (int h = blockDim.x >> 1; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + h];
+      red[1][threadIdx.x] += red[1][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a.part[2 * blockIdx.x] = red[0][0];
+    a.part[2 * blockIdx.x + 1] = red[1][0];
+  }
+}
+
+// out[0] kinetic, out[1] chain energy with the centroid chain (one thread)
+__global__ void k_pimd_energy_sum(NhcArgs a, int G, double *out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double ke = 0.0, en = 0.0;
+  for (int g = 0; g < G; g++) {
+    ke += a.part[2 * g];
+    en += a.part[2 * g + 1];
+  }
+  const int L = a.len;
+  for (int j = 1; j <= L; j++) en += 0.5 * a.cmass[j] * a.cls[j - 1] * a.cls[j - 1] + a.cgkt[j] * a.cls[L + j - 1];
+  out[0] = ke;
+  out[1] = en;
 }
 
 }  // namespace pmx

@@ -961,6 +961,7 @@ hipError_t launch_fft_class(const FftArgs &f, hipStream_t st) {
   const int Ka = f.K[0], Kb = f.K[1], Kc = f.K[2], Ha = Ka / 2 + 1;
   // the plane (or the lines) and the twiddles of the axes it transforms
   const size_t lab = ((size_t)Ka * Kb + Ka + Kb) * sizeof(C), lch = ((size_t)Kc * Ha + Kc) * sizeof(C);
+  if (lab > FFT_LDS_DYN || lch > FFT_LDS_DYN) return hipErrorInvalidValue;  // refused at setup already
   if (lab > 65536 || lch > 65536) {  // beyond 64 KB the kernels ask for the larger LDS
     hipError_t e;
     if ((e = hipFuncSetAttribute((const void *)k_fft_fwd_ab<T, NT, MAXP>,

@@ -61,6 +61,9 @@ def lib():
     L.pmx_get_velocities.argtypes = [C.c_void_p, dp, dp, dp]
     L.pmx_get_positions.argtypes = [C.c_void_p, dp, dp, dp]
     L.pmx_md_nve.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(abi.PmxEnergies), dp]
+    L.pmx_pimd_nvt.argtypes = [C.c_void_p, C.POINTER(abi.PmxNhcParams), C.c_int, C.POINTER(abi.PmxEnergies), dp]
+    L.pmx_set_nhc_state.argtypes = [C.c_void_p, C.c_int, dp]
+    L.pmx_get_nhc_state.argtypes = [C.c_void_p, C.c_int, dp]
     L.pmx_bond_coef_harm.argtypes = [C.c_double, dp]
     L.pmx_bond_coef_morse.argtypes = [C.c_double, C.c_double, dp]
     L.pmx_bend_coef_harm.argtypes = [C.c_double, C.c_double, dp, dp]
@@ -74,7 +77,7 @@ def lib():
     L.pmx_batch_destroy.argtypes = [C.c_void_p]
     L.pmx_autotune.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_int)]
     L.pmx_batch_autotune.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_int)]
-    if L.pmx_abi_version() != 6:
+    if L.pmx_abi_version() != 7:
         raise PinyAmdError("libpiny_amd.so ABI version mismatch")
     _lib = L
     return L
@@ -89,7 +92,7 @@ EXPORTED = ("pmx_abi_version", "pmx_device_count", "pmx_create", "pmx_destroy", 
             "pmx_get_velocities", "pmx_get_positions", "pmx_md_nve", "pmx_bond_coef_harm", "pmx_bond_coef_morse",
             "pmx_bend_coef_harm", "pmx_set_force_field", "pmx_get_param_grad", "pmx_batch_create",
             "pmx_batch_energy_control", "pmx_batch_last_error", "pmx_batch_destroy", "pmx_autotune",
-            "pmx_batch_autotune")
+            "pmx_batch_autotune", "pmx_pimd_nvt", "pmx_set_nhc_state", "pmx_get_nhc_state")
 
 
 class Engine:
@@ -185,6 +188,29 @@ class Engine:
         ke = C.c_double(0.0)
         self._chk(lib().pmx_md_nve(self.h, int(nsteps), float(dt), C.byref(e), C.byref(ke)), "pmx_md_nve")
         return e.as_dict(), ke.value
+
+    def pimd_nvt(self, nsteps, params=None):
+        """nsteps of PIMD NVT on the device (pmx_pimd_nvt); the velocities are
+        mode velocities.  Returns (energies of the last force call, econs =
+        [kinetic, chain energy, potential with springs, total])."""
+        params = params or abi.nhc_params()
+        self._nhc_len = params.len_nhc
+        e = abi.PmxEnergies()
+        ec = np.zeros(4)
+        self._chk(lib().pmx_pimd_nvt(self.h, C.byref(params), int(nsteps), C.byref(e), abi.dptr(ec)),
+                  "pmx_pimd_nvt")
+        return e.as_dict(), ec
+
+    def nhc_state(self, len_nhc=None):
+        L = len_nhc or getattr(self, "_nhc_len", 4)
+        st = np.zeros(abi.nhc_state_size(self.N, self.P, L))
+        self._chk(lib().pmx_get_nhc_state(self.h, L, abi.dptr(st)), "pmx_get_nhc_state")
+        return st
+
+    def set_nhc_state(self, st, len_nhc):
+        self._nhc_st = np.ascontiguousarray(st, dtype=np.float64)
+        self._nhc_len = len_nhc
+        self._chk(lib().pmx_set_nhc_state(self.h, int(len_nhc), abi.dptr(self._nhc_st)), "pmx_set_nhc_state")
 
     def set_force_field(self, system):
         """New charges, LJ sigma/eps and bond/bend coefficients (same topology)."""

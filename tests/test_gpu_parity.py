@@ -10,6 +10,7 @@ import pytest
 
 from oracle.oracle import Oracle
 from piny_amd import abi
+from piny_amd import water as w
 from piny_amd.engine import Engine
 from tests import cases
 
@@ -40,7 +41,8 @@ def _rel(a, b):
 @pytest.mark.parametrize("case", ["spc27", "spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg1", "cfg2_p8",
                                   "tip256_p32", "spc64_k20", "tip64_p4_k24", "cfg1_k40", "tip128_p4_k48",
                                   "spc64_ortho", "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq",
-                                  "spc64_ewald", "tip64_p4_ewald", "spc64_k80", "tip64_p4_k96"])
+                                  "spc64_ewald", "tip64_p4_ewald", "spc64_k80", "tip64_p4_k96", "tip64_p4_stag",
+                                  "tip64_p4_k64"])
 def test_energy_control_parity(case):
     c, o, g, eo, fo, eg, fg = _run(case)
     scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
@@ -74,7 +76,7 @@ MIXED_ETOL, MIXED_VTOL, MIXED_FTOL = 1e-5, 1e-4, 1e-4
 @pytest.mark.parametrize("mode", ["direct", "mixed"])
 @pytest.mark.parametrize("case", ["spc64", "spc64_shift", "spc64_none", "tip64_p4", "cfg2_p8", "tip256_p32",
                                   "tip64_p4_tric", "tip64_p6", "spc27_p64", "spc64_noq", "tip64_p4_ewald",
-                                  "spc64_k80", "tip64_p4_k96"])
+                                  "spc64_k80", "tip64_p4_k96", "tip64_p4_k64", "tip64_p4_stag"])
 def test_pair_direct_and_mixed(case, mode):
     kw = {"pair_eval": abi.PMX_PAIR_DIRECT} if mode == "direct" else {"precision": abi.PMX_PREC_MIXED}
     c, o, g, eo, fo, eg, fg = _run(case, **kw)
@@ -150,7 +152,8 @@ def test_setup_bit_exact(case):
         assert np.array_equal(g.spline(which), o.spline(which)), which
 
 
-@pytest.mark.parametrize("case", ["tip64_p4", "cfg2_p8"])
+@pytest.mark.parametrize("case", ["tip64_p4", "cfg2_p8", "tip64_p6", "tip256_p32", "spc27_p64", "tip64_p4_stag",
+                                  "cfg2_p4_stag"])
 def test_mode_forces_and_springs(case):
     flags = abi.PMX_EVAL_MODE_FORCES
     c, o, g, eo, fo, eg, fg = _run(case, flags)
@@ -186,6 +189,50 @@ def test_list_rebuild_after_motion():
         assert np.array_equal(a, b)
 
 
+def test_list_rebuild_decision_with_beads():
+    # the skin criterion of check_ver_list with beads: bead 1's displacement
+    # since the last build against skin/2 + max(0, spread_now - spread)
+    # (nbr_list_control.c:182-239), where the spread is the largest distance
+    # of a bead from bead 1 (get_pimd_spread), checked call by call against
+    # the oracle
+    c = cases.get("tip64_p4")
+    s = c.system
+    N, P = s.natm, s.pi_beads
+    skin = s.settings.skin / w.BOHR
+    print("[case tip64_p4 rebuild with beads]", flush=True)
+    o = Oracle(s, c.x, c.y, c.z)
+    g = Engine(s, c.x, c.y, c.z)
+    flags = abi.PMX_EVAL_PIMD
+    pos = [c.x.copy(), c.y.copy(), c.z.copy()]
+
+    def move(site, d, beads):
+        for b in beads:
+            pos[0][b * N + site] += d
+
+    def call(expect):
+        g.set_positions(*pos)
+        eg = g.energy_control(flags)
+        eo, fo, _ = o.energy_control(*[v.copy() for v in pos], flags)
+        print(f"  rebuilt {eg['list_rebuilt']} (oracle {eo['list_rebuilt']}), spread {eg['spread']:.4f}", flush=True)
+        assert eg["list_rebuilt"] == eo["list_rebuilt"] == expect
+        assert eg["nver"] == eo["nver"]
+        assert abs(eg["spread"] - eo["spread"]) <= 1e-12 * eo["spread"]
+        assert _rel(eg["vintert"], eo["vintert"]) < ETOL
+        fg = g.forces()
+        assert np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum()) < FTOL
+
+    call(1)                                   # first build
+    move(5, 0.7 * skin, range(P))             # the whole ring past skin/2: rebuild
+    call(1)
+    move(9, 0.5 * skin, range(1, P))          # spread grows by half the skin ...
+    move(12, 0.7 * skin, range(P))            # ... so 0.7 skin is inside skin/2 + dspread
+    call(0)
+    move(12, 0.5 * skin, range(P))            # 1.2 skin is not
+    call(1)
+    for a, b in zip(g.verlet_list(), o.verlet_list()):
+        assert np.array_equal(a, b)
+
+
 def test_repeat_runs_reproduce():
     c = cases.get("cfg1")
     print("[case cfg1 repeat]", flush=True)
@@ -200,6 +247,22 @@ def test_repeat_runs_reproduce():
 
 
 # --- the device against the reference itself (tests/golden/ref_*.npz) -------
+
+from tests.test_ref_setup import SETUP, compare_setup, load_setup  # noqa: E402
+
+
+@pytest.mark.parametrize("name", SETUP)
+def test_device_setup_matches_reference(name):
+    # the reference's own Verlet list, spline tables, k-vectors, B-spline
+    # weights and excluded-pair table, bit for bit
+    d = load_setup(name)
+    c = cases.get(name)
+    s = c.system
+    print(f"[reference setup {name}] N={s.natm} P={s.pi_beads}", flush=True)
+    g = Engine(s, c.x, c.y, c.z)
+    g.set_positions(c.x, c.y, c.z)
+    g.energy_control(abi.PMX_EVAL_PIMD if s.pi_beads > 1 else 0)
+    compare_setup(d, g.verlet_list(), g.spline, g.kvectors(), s.ntype * (s.ntype + 1) // 2)
 
 from tests.test_ref_golden import FIXTURES, compare, load  # noqa: E402
 

@@ -23,7 +23,7 @@ from tests import cases
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 FIXTURES = sorted(n for n in (os.path.basename(p)[4:-4] for p in glob.glob(os.path.join(HERE, "golden", "ref_*.npz")))
-                  if not n.startswith("md_"))
+                  if not n.startswith(("md_", "pimd_")))
 TOL = 1e-12
 
 
