@@ -772,14 +772,14 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     const double *hi = s.hmati;
     const double falp2 = 4.0 * s.alp_ewd * s.alp_ewd, tpi = 2.0 * M_PI, pivol = s.vol / (4.0 * M_PI);
     for (int k = 0; k < s.nktot; k++) {
-      const double aka = tpi * ((double)s.ka[k]), akb = tpi * ((double)s.kb[k]), akc = tpi * ((double)s.kc[k]);
-      const double xk = (aka * hi[1] + akb * hi[2] + akc * hi[3]);
-      const double yk = (aka * hi[4] + akb * hi[5] + akc * hi[6]);
-      const double zk = (aka * hi[7] + akb * hi[8] + akc * hi[9]);
+      const double ga = tpi * ((double)s.ka[k]), gb = tpi * ((double)s.kb[k]), gc = tpi * ((double)s.kc[k]);
+      const double xk = (ga * hi[1] + gb * hi[2] + gc * hi[3]);
+      const double yk = (ga * hi[4] + gb * hi[5] + gc * hi[6]);
+      const double zk = (ga * hi[7] + gb * hi[8] + gc * hi[9]);
       const double g2 = xk * xk + yk * yk + zk * zk;
       const double preg = exp((-g2 / falp2)) / (g2 * pivol);
       const double prep = -2.0 * preg * ((g2 / falp2) + 1.0) / g2;
-      kv[k] = make_double4(aka, akb, akc, preg);
+      kv[k] = make_double4(ga, gb, gc, preg);
       kg[k] = make_double4(xk, yk, zk, prep);
     }
     TRY(upload(c, &c->ew_kv, kv.data(), kv.size()));
@@ -818,10 +818,10 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     const double *hi = s.hmati;
     const double falp2 = 4.0 * s.alp_ewd * s.alp_ewd, tpi = 2.0 * M_PI, pivol = s.vol / (4.0 * M_PI);
     for (int k = 0; k < s.nktot; k++) {
-      const double aka = tpi * ((double)s.ka[k]), akb = tpi * ((double)s.kb[k]), akc = tpi * ((double)s.kc[k]);
-      const double xk = (aka * hi[1] + akb * hi[2] + akc * hi[3]);
-      const double yk = (aka * hi[4] + akb * hi[5] + akc * hi[6]);
-      const double zk = (aka * hi[7] + akb * hi[8] + akc * hi[9]);
+      const double ga = tpi * ((double)s.ka[k]), gb = tpi * ((double)s.kb[k]), gc = tpi * ((double)s.kc[k]);
+      const double xk = (ga * hi[1] + gb * hi[2] + gc * hi[3]);
+      const double yk = (ga * hi[4] + gb * hi[5] + gc * hi[6]);
+      const double zk = (ga * hi[7] + gb * hi[8] + gc * hi[9]);
       const double g2 = xk * xk + yk * yk + zk * zk;
       const double preg = exp((-g2 / falp2)) / (g2 * pivol);
       const double w = preg * s.bweight[k];
@@ -1322,8 +1322,8 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.npair = s.npair;
     a.gx = pair_grid_x(N, c->pair_waves);
     a.alp = s.alp_ewd;
-    a.palp = kErfP * s.alp_ewd;
-    a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
+    a.p_alpha = kErfP * s.alp_ewd;
+    a.two_alpha2 = 2.0 * s.alp_ewd * s.alp_ewd;
     a.spalp = 2.0 * s.alp_ewd / sqrt(M_PI);
     a.ctab = c->ctab;
     a.ct_r0 = c->ct_r0;
@@ -2617,8 +2617,8 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
     a.npair = s.npair;
     a.gx = pair_grid_x(N, c->pair_waves);
     a.alp = s.alp_ewd;
-    a.palp = kErfP * s.alp_ewd;
-    a.talp2 = 2.0 * s.alp_ewd * s.alp_ewd;
+    a.p_alpha = kErfP * s.alp_ewd;
+    a.two_alpha2 = 2.0 * s.alp_ewd * s.alp_ewd;
     a.spalp = 2.0 * s.alp_ewd / sqrt(M_PI);
     a.ctab = c->ctab;
     a.ct_r0 = c->ct_r0;

@@ -57,7 +57,7 @@ constexpr int ENTRY_MASK = (1 << ENTRY_SHIFT) - 1;
 struct PairArgs {
   int N, P, cap, nsplin, pimd, npair;
   int gx;  // workgroups along x (pair_grid_x)
-  double alp, palp, talp2, rheal, irheal;  // erfc and switch constants
+  double alp, p_alpha, two_alpha2, rheal, irheal;  // erfc and switch constants
   double spalp;                           // 2 alp / sqrt(pi)
   // mixed precision: erfc(alp r)/r (PINY's polynomial) as cubic pieces in r,
   // CT_N intervals from ct_r0 with 1/width ct_idr: (c0, c1, c2, c3) of

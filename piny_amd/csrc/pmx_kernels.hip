@@ -819,7 +819,7 @@ __device__ __forceinline__ R pair_direct(const PairArgs &a, const PT *pp, R q2, 
         dg = dg * tt + (R)(k * kErfE[k]);
       }
       g = g * tt * eee;
-      dg = dg * tt * tt * eee * (R)a.palp + (R)a.talp2 * g * r;
+      dg = dg * tt * tt * eee * (R)a.p_alpha + (R)a.two_alpha2 * g * r;
     }
     const R v = g * rinv;
     const R dv = (g * r2i + dg * rinv) * rinv;

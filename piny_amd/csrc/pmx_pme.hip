@@ -877,10 +877,10 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
     const C s = sm[w];
     if (W != 0.0) {
       const int kc = cc <= Kc / 2 ? cc : cc - Kc;
-      const double aka = tpi * (double)aa, akb = tpi * (double)kb, akc = tpi * (double)kc;
-      const double xk = (aka * hi[0] + akb * hi[1] + akc * hi[2]);
-      const double yk = (aka * hi[3] + akb * hi[4] + akc * hi[5]);
-      const double zk = (aka * hi[6] + akb * hi[7] + akc * hi[8]);
+      const double ga = tpi * (double)aa, gb = tpi * (double)kb, gc = tpi * (double)kc;
+      const double xk = (ga * hi[0] + gb * hi[1] + gc * hi[2]);
+      const double yk = (ga * hi[3] + gb * hi[4] + gc * hi[5]);
+      const double zk = (ga * hi[6] + gb * hi[7] + gc * hi[8]);
       const double g2 = xk * xk + yk * yk + zk * zk;
       const double wsum = (aa == 0) ? 0.5 : 1.0;  // a' = 0 holds both +k and -k
       const double sx = (double)s.x, sy = (double)s.y;

@@ -65,25 +65,25 @@ void spline_table(const Setup &s, const pmx_params &par, int itype, double rmin,
     for (int i = 0; i < n; i++) {
       const double r2 = r[i] * r[i];
       const double sig6r = sig6 / (r2 * r2 * r2);
-      const double vnow = (eps * 4. * sig6r * (sig6r - 1.));
-      const double dvnow = (eps * 24. * sig6r * (sig6r * 2. - 1.) / r2);
-      v[i] = vnow * sw[i];
-      dv[i] = dvnow * sw[i] + vnow * dsw[i];
+      const double e_pair = (eps * 4. * sig6r * (sig6r - 1.));
+      const double de_pair = (eps * 24. * sig6r * (sig6r * 2. - 1.) / r2);
+      v[i] = e_pair * sw[i];
+      dv[i] = de_pair * sw[i] + e_pair * dsw[i];
     }
   } else if (itype == 1) {
     const double alp = s.alp_ewd, qij = 1.0;
-    const double talp2 = 2.0 * alp * alp, palp = kErfP * alp;
+    const double two_alpha2 = 2.0 * alp * alp, p_alpha = kErfP * alp;
     for (int i = 0; i < n; i++) {
       const double r2 = r[i] * r[i];
       const double ralp = r[i] * alp;
       const double eee = exp(-ralp * ralp);
       const double tt = 1.0 / (1.0 + kErfP * ralp);
-      const double gerfc = erfc_poly(tt) * eee;
-      const double dgerfc = derfc_poly(tt) * eee * palp + talp2 * gerfc * r[i];
-      const double vnow = qij * gerfc / r[i];
-      const double dvnow = (gerfc / r2 + dgerfc / r[i]) * qij / r[i];
-      v[i] = vnow * sw[i];
-      dv[i] = dvnow * sw[i] + vnow * dsw[i];
+      const double screen = erfc_poly(tt) * eee;
+      const double dscreen = derfc_poly(tt) * eee * p_alpha + two_alpha2 * screen * r[i];
+      const double e_pair = qij * screen / r[i];
+      const double de_pair = (screen / r2 + dscreen / r[i]) * qij / r[i];
+      v[i] = e_pair * sw[i];
+      dv[i] = de_pair * sw[i] + e_pair * dsw[i];
     }
   }
   if (shift_out) *shift_out = 0.0;
@@ -120,11 +120,11 @@ double row_norm(const double *hi, int a, int b, int c) {
   return sqrt(d1 * d1 + d2 * d2 + d3 * d3);
 }
 
-double ktry(const double *hm, double aka, double akb, double akc) {
+double ktry(const double *hm, double ga, double gb, double gc) {
   const double tpi = 2.0 * M_PI;
-  const double xk = (aka * hm[1] + akb * hm[2] + akc * hm[3]) * tpi;
-  const double yk = (aka * hm[4] + akb * hm[5] + akc * hm[6]) * tpi;
-  const double zk = (aka * hm[7] + akb * hm[8] + akc * hm[9]) * tpi;
+  const double xk = (ga * hm[1] + gb * hm[2] + gc * hm[3]) * tpi;
+  const double yk = (ga * hm[4] + gb * hm[5] + gc * hm[6]) * tpi;
+  const double zk = (ga * hm[7] + gb * hm[8] + gc * hm[9]) * tpi;
   return (xk * xk + yk * yk + zk * zk) * .5;
 }
 
@@ -137,22 +137,22 @@ void kvectors(Setup &s) {
   s.kc.clear();
   int i;
   for (int ka = 0; ka <= s.kmaxv[1]; ++ka) {
-    const double aka = (double)ka;
+    const double ga = (double)ka;
     int kbmin = (ka == 0) ? 0 : -s.kmaxv[2];
     for (i = kbmin; i <= 0; ++i)
-      if (ktry(hm, aka, (double)i, 0.0) <= ec4) break;
+      if (ktry(hm, ga, (double)i, 0.0) <= ec4) break;
     kbmin = i;
     for (i = 1; i <= s.kmaxv[2]; ++i)
-      if (ktry(hm, aka, (double)i, 0.0) > ec4) break;
+      if (ktry(hm, ga, (double)i, 0.0) > ec4) break;
     const int kbmax = i - 1;
     for (int kb = kbmin; kb <= kbmax; ++kb) {
-      const double akb = (double)kb;
+      const double gb = (double)kb;
       int kcmin = (ka == 0 && kb == 0) ? 1 : -s.kmaxv[3];
       for (i = kcmin; i <= 0; ++i)
-        if (ktry(hm, aka, akb, (double)i) <= ec4) break;
+        if (ktry(hm, ga, gb, (double)i) <= ec4) break;
       kcmin = i;
       for (i = 1; i <= s.kmaxv[3]; ++i)
-        if (ktry(hm, aka, akb, (double)i) > ec4) break;
+        if (ktry(hm, ga, gb, (double)i) > ec4) break;
       const int kcmax = i - 1;
       for (int kc = kcmin; kc <= kcmax; ++kc) {
         s.ka.push_back(ka);
@@ -187,18 +187,18 @@ void bspline_weight1d(int n_interp, int ngrid, std::vector<double> &bw) {
     mn[1] = uk[1] * mn[1] * rn1[n];
   }
   std::vector<double> bre(ngrid + 1, 0.0), bim(ngrid + 1, 0.0);
-  double tpi_n = 2.0 * M_PI / grid;
+  double phase_k = 2.0 * M_PI / grid;
   for (int m = 1; m <= ngrid; m++) {
     for (int k = 1; k <= n_interp - 1; k++) {
-      const double arg = tpi_n * ((double)((k - 1) * (m - 1)));
+      const double arg = phase_k * ((double)((k - 1) * (m - 1)));
       bre[m] += cos(arg) * mn[k];
       bim[m] += sin(arg) * mn[k];
     }
   }
   bw.assign(ngrid + 1, 0.0);
-  tpi_n = 2.0 * M_PI * ((double)(n_interp - 1)) / grid;
+  phase_k = 2.0 * M_PI * ((double)(n_interp - 1)) / grid;
   for (int m = 1; m <= ngrid; m++) {
-    const double arg = tpi_n * ((double)(m - 1));
+    const double arg = phase_k * ((double)(m - 1));
     const double bnr = cos(arg), bni = sin(arg);
     const double bdr = bre[m], bdi = bim[m];
     const double denom = bdr * bdr + bdi * bdi;
@@ -551,7 +551,7 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
     s.e_dr = dr;
     s.e_dri = dri;
     s.ecor_tab.assign((size_t)n * 2, 0.0);
-    const double alp = s.alp_ewd, alp2 = alp * alp, talp2 = 2.0 * alp * alp, palp = kErfP * alp;
+    const double alp = s.alp_ewd, alp2 = alp * alp, two_alpha2 = 2.0 * alp * alp, p_alpha = kErfP * alp;
     for (int i = 1; i <= n; i++) {
       const double r12 = dr * (double)(i - 3) + rmin;
       const double r12i = 1.0 / r12;
@@ -560,7 +560,7 @@ std::string build_setup(const pmx_system *sys, const pmx_params *par, const doub
       const double tt = 1.0 / (1.0 + kErfP * ralp);
       const double temp = erfc_poly(tt) * eee;
       const double gerf = 1.0 - temp;
-      const double dgerf = derfc_poly(tt) * eee * palp + talp2 * temp * r12;
+      const double dgerf = derfc_poly(tt) * eee * p_alpha + two_alpha2 * temp * r12;
       s.ecor_tab[2 * (i - 1)] = -gerf * r12i;
       const double dvecor = -(dgerf * r12i - gerf / (r12 * r12));
       s.ecor_tab[2 * (i - 1) + 1] = -dvecor * r12i;
