@@ -530,15 +530,23 @@ int setup_molecules(pmx_ctx *c, const pmx_system *sys) {
 //   there) falls on a node when all type pairs share one cutoff.
 // Each piece is the cubic Hermite interpolant of the function and its r
 // derivative at the two nodes: c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1].
-// Interpolation error below 5e-9 relative for the erfc term over 1-17 Bohr,
-// under the FP32 rounding of the terms themselves.
+// Interpolation error of the erfc term: 3e-6 relative at the spline start
+// (min_dist 0.1 A), 6e-9 from 1 Bohr on (1024 pieces over 17 Bohr).  Below
+// the start the first piece extrapolates, as PINY's table does below its
+// rmin (force_npol.c:742-751).
 int build_pair_tables(pmx_ctx *c) {
   const Setup &s = c->s;
-  double rend = 0.0;
-  for (int t = 0; t < s.npair; t++) rend = std::max(rend, s.cutoff[t]);
+  double rend = 0.0, rlo = 1e300;
+  for (int t = 0; t < s.npair; t++) {
+    rend = std::max(rend, s.cutoff[t]);
+    if (s.rmin[t] > 0.0) rlo = std::min(rlo, s.rmin[t]);
+  }
+  if (!(rlo < rend)) rlo = 0.5;
   const bool swit = (c->par.ishift == PMX_SHIFT_SWIT) && c->par.rheal > 0.0;
-  double dr = (rend - 0.5) / CT_N;
-  if (swit) dr = c->par.rheal / std::ceil(c->par.rheal / dr);  // rheal a whole number of intervals
+  // the grid reaches down to PINY's spline start (the smallest min_dist)
+  double dr = (rend - rlo) / CT_N;
+  if (swit && c->par.rheal >= dr)  // rheal a whole number of intervals, the start kept at or below rlo
+    dr = c->par.rheal / std::floor(c->par.rheal / dr);
   const double r0 = rend - CT_N * dr, alp = s.alp_ewd;
   auto erfc_r = [&](double r, double &v, double &d) {  // erfc(alp r)/r and its r derivative
     const double x = alp * r, tt = 1.0 / (1.0 + kErfP * x), ee = exp(-x * x);

@@ -220,12 +220,18 @@ def test_list_rebuild_decision_with_beads():
         assert _rel(eg["vintert"], eo["vintert"]) < ETOL
         fg = g.forces()
         assert np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum()) < FTOL
+        return eg
 
     call(1)                                   # first build
     move(5, 0.7 * skin, range(P))             # the whole ring past skin/2: rebuild
-    call(1)
-    move(9, 0.5 * skin, range(1, P))          # spread grows by half the skin ...
-    move(12, 0.7 * skin, range(P))            # ... so 0.7 skin is inside skin/2 + dspread
+    spread = call(1)["spread"]                # the spread of this build (over all sites and beads)
+    # beads 2..P of site 9 at spread + skin/2 from its bead 1: the spread grows
+    # by half the skin, so 0.7 skin of bead-1 motion is inside skin/2 + dspread
+    for b in range(1, P):
+        pos[0][b * N + 9] = pos[0][9] + spread + 0.5 * skin
+        pos[1][b * N + 9] = pos[1][9]
+        pos[2][b * N + 9] = pos[2][9]
+    move(12, 0.7 * skin, range(P))
     call(0)
     move(12, 0.5 * skin, range(P))            # 1.2 skin is not
     call(1)
