@@ -223,6 +223,24 @@ int pmx_set_xmod(pmx_ctx *ctx, const double *x, const double *y,
 /* The force call. */
 int pmx_energy_control(pmx_ctx *ctx, unsigned flags, pmx_energies *out);
 
+/* The force call without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * inputs, flags and results as pmx_energy_control, with two differences:
+ *  - the per-workgroup partial sums of the call (energies, virial, k-space,
+ *    bonded and spring sums) live in the library's static device array of
+ *    PMX_NOALLOC_SCRATCH_BYTES per device instead of the context's own
+ *    buffer; a context whose partial sums need more returns PMX_ERR_RANGE;
+ *  - a Verlet list that outgrows its rows returns PMX_ERR_RANGE instead of
+ *    growing them (one pmx_energy_control call grows them).
+ * Every call on a device shares that device's array: calls that overlap in
+ * time on one device (from different host threads, or a batch running
+ * meanwhile with this context) race and must be serialized by the caller.
+ * The array keeps the last call's contents; each call writes every entry it
+ * reads before reading it. */
+#define PMX_NOALLOC_SCRATCH_BYTES ((size_t)32 << 20)
+int pmx_energy_control_noalloc(pmx_ctx *ctx, unsigned flags, pmx_energies *out);
+
 /* Forces of all beads [P*N] into host memory (ghost forces redistributed). */
 int pmx_get_forces(pmx_ctx *ctx, double *fx, double *fy, double *fz);
 

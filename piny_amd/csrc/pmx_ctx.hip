@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include <chrono>
@@ -99,6 +100,8 @@ struct pmx_ctx {
   double *item_f = nullptr, *hbuf = nullptr;
   // per-block partial sums of every stage, then accumulators [P][ACC_NUM] + ake
   double *part = nullptr;
+  size_t part_n = 0;       // doubles of part
+  bool no_alloc = false;   // inside pmx_energy_control_noalloc: lists may not grow
   size_t off_pair = 0, off_fft = 0, off_ecor = 0, off_bond = 0, off_bend = 0, off_fin = 0, off_mode = 0;
   int g_intra = 1024;
   double *acc = nullptr, *ake = nullptr;
@@ -163,7 +166,38 @@ struct pmx_batch {
   std::vector<unsigned> epochs;
 };
 
+// The static scratch of pmx_energy_control_noalloc: the per-workgroup
+// partial sums of one force call (energies, virial, the k-space and bonded
+// sums, the springs), which pmx_energy_control keeps in the context's own
+// allocation.  One copy per device, held until the process ends; every call
+// on a device shares it, and each call writes every entry it reads before
+// reading it.
+static __device__ __align__(256) double scratch[PMX_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
 namespace {
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The device address of the scratch array on the current device, taken once
+// per device (the host-side name of the array is not a device address).
+// Called by pmx_create with its device current.
+int scratch_setup(int device) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES) return PMX_ERR_RANGE;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_ptr[device]) return PMX_OK;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PMX_ERR_DEVICE;
+  scratch_ptr[device] = (double *)p;
+  return PMX_OK;
+}
+
+double *scratch_address(int device) {
+  if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  return scratch_ptr[device];
+}
 
 void set_err(pmx_ctx *c, const char *fmt, ...) {
   char buf[512];
@@ -235,6 +269,11 @@ unsigned blocks(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / 
 
 int alloc_lists(pmx_ctx *c, int cap) {
   const size_t n = (size_t)c->s.N * cap;
+  if (c->no_alloc) {
+    set_err(c, "the Verlet list needs rows of %d entries (now %d), which takes an allocation: "
+               "one call of pmx_energy_control grows them", cap, c->cap);
+    return PMX_ERR_RANGE;
+  }
   if (c->jfull) {
     for (int *p : {c->jfull, c->jforce, c->jsplit}) {
       (void)hipFree(p);
@@ -863,7 +902,8 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     c->off_bend = c->off_bond + intra;
     c->off_fin = c->off_bend + intra;
     c->off_mode = c->off_fin + fin;
-    TRY(dalloc(c, &c->part, c->off_mode + (P > 1 ? mode_blocks(N, P) : 1)));
+    c->part_n = c->off_mode + (P > 1 ? mode_blocks(N, P) : 1);
+    TRY(dalloc(c, &c->part, c->part_n));
   }
   if (c->pme || c->ewald) {
     // the k-space kernels store the force of every (charge, bead); the
@@ -1694,6 +1734,10 @@ pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *
     pmx_destroy(c);
     return fail("pmx_create: " + e);
   }
+  if (scratch_setup(device) != PMX_OK) {
+    pmx_destroy(c);
+    return fail("pmx_create: hipGetSymbolAddress of the static scratch failed");
+  }
   return c;
 }
 
@@ -1766,9 +1810,8 @@ int pmx_set_xmod(pmx_ctx *c, const double *x, const double *y, const double *z) 
   return PMX_OK;
 }
 
-int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
-  int rc = check_call(c, flags, out, "pmx_energy_control");
-  if (rc) return rc;
+static int force_call(pmx_ctx *c, unsigned flags, pmx_energies *out) {
+  int rc;
   HIPCHK(hipSetDevice(c->device));
   memset(out, 0, sizeof(*out));
   c->grad_ready = false;
@@ -1782,6 +1825,34 @@ int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   HIPCHK(hipStreamSynchronize(st));
   assemble(c, flags, out, true);
   return PMX_OK;
+}
+
+int pmx_energy_control(pmx_ctx *c, unsigned flags, pmx_energies *out) {
+  int rc = check_call(c, flags, out, "pmx_energy_control");
+  if (rc) return rc;
+  return force_call(c, flags, out);
+}
+
+int pmx_energy_control_noalloc(pmx_ctx *c, unsigned flags, pmx_energies *out) {
+  int rc = check_call(c, flags, out, "pmx_energy_control_noalloc");
+  if (rc) return rc;
+  if (c->part_n * sizeof(double) > PMX_NOALLOC_SCRATCH_BYTES) {
+    set_err(c, "pmx_energy_control_noalloc: the context's partial sums take %zu bytes, more than the %zu of "
+               "the static scratch", c->part_n * sizeof(double), (size_t)PMX_NOALLOC_SCRATCH_BYTES);
+    return PMX_ERR_RANGE;
+  }
+  double *ptr = scratch_address(c->device);
+  if (!ptr) {
+    set_err(c, "pmx_energy_control_noalloc: no scratch address for device %d", c->device);
+    return PMX_ERR_DEVICE;
+  }
+  double *own = c->part;
+  c->part = ptr;
+  c->no_alloc = true;
+  rc = force_call(c, flags, out);
+  c->part = own;
+  c->no_alloc = false;
+  return rc;
 }
 
 // --- batches of contexts --------------------------------------------------------

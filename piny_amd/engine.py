@@ -46,6 +46,7 @@ def lib():
     L.pmx_set_positions_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.pmx_set_xmod.argtypes = [C.c_void_p, dp, dp, dp]
     L.pmx_energy_control.argtypes = [C.c_void_p, C.c_uint, C.POINTER(abi.PmxEnergies)]
+    L.pmx_energy_control_noalloc.argtypes = [C.c_void_p, C.c_uint, C.POINTER(abi.PmxEnergies)]
     L.pmx_get_forces.argtypes = [C.c_void_p, dp, dp, dp]
     L.pmx_get_forces_device.argtypes = [C.c_void_p, cdp, cdp, cdp]
     L.pmx_get_fxt.argtypes = [C.c_void_p, dp, dp, dp]
@@ -86,6 +87,7 @@ def lib():
 # every symbol include/piny_amd.h declares
 EXPORTED = ("pmx_abi_version", "pmx_device_count", "pmx_create", "pmx_destroy", "pmx_last_error",
             "pmx_set_positions", "pmx_set_positions_device", "pmx_set_xmod", "pmx_energy_control",
+            "pmx_energy_control_noalloc",
             "pmx_get_forces", "pmx_get_forces_device", "pmx_get_fxt", "pmx_get_bead_energies",
             "pmx_get_verlet_list", "pmx_get_nktot", "pmx_get_kvectors", "pmx_get_spline",
             "pmx_get_setup_scalars", "pmx_get_timings", "pmx_get_pair_count", "pmx_set_velocities",
@@ -148,6 +150,15 @@ class Engine:
             flags = abi.PMX_EVAL_PIMD if self.P > 1 else 0
         e = abi.PmxEnergies()
         self._chk(lib().pmx_energy_control(self.h, flags, C.byref(e)), "pmx_energy_control")
+        return e.as_dict()
+
+    def energy_control_noalloc(self, flags=None):
+        """The force call through pmx_energy_control_noalloc (partial sums in
+        the library's static scratch, no allocation)."""
+        if flags is None:
+            flags = abi.PMX_EVAL_PIMD if self.P > 1 else 0
+        e = abi.PmxEnergies()
+        self._chk(lib().pmx_energy_control_noalloc(self.h, flags, C.byref(e)), "pmx_energy_control_noalloc")
         return e.as_dict()
 
     def forces(self):
