@@ -386,7 +386,9 @@ void pmx_batch_destroy(pmx_batch *batch);
 
 /* Autotuner of the launch shapes (no counterpart in the reference, which
  * has no device): times reps force calls at the current positions for each
- * candidate of the pair kernels' wavefront count, the PME tile sizes and the
+ * candidate of the pair kernels' wavefront count, then (mixed precision, a
+ * box of one molecule template) the molecule-cluster form of the pair
+ * kernel against the per-site one, the PME tile sizes and the
  * list build (plain scan or link cells), keeps the fastest of each, then
  * times whole calls with PME on a second stream beside the pair kernel and
  * without, keeps the faster, and ends with a force call using them.  choice (may be NULL) receives

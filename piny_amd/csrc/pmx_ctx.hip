@@ -59,6 +59,20 @@ struct pmx_ctx {
   float4 *disp = nullptr;      // mixed precision with beads: x - xmod in FP32 [N*P]
   double *umax = nullptr;      // the largest |x - xmod| of every site over its beads [N]
   int pair_mode = 0;       // 0 FP64 tables, 1 FP64 direct, 2 FP32 direct
+  // molecule-cluster form of the mixed pair kernel (k_pair_clu)
+  bool clu_ok = false;      // the box qualifies: one molecule template of <= 4 sites, one cutoff
+  bool clu_enabled = false; // chosen (PMX_CLUSTER=1 or the autotuner): rows are built with the list
+  bool clu_on = false;      // the current list has molecule rows and the image margin holds
+  bool clu_used = false;    // the last force sequence ran the cluster kernel
+  bool disp_packed = false; // this call's pre_sequence packed the bead displacements
+  int clu_ns = 0, clu_mcap = 0, clu_npieces = 0;
+  int2 *mrow = nullptr;
+  int *mcnt = nullptr, *clu_flag = nullptr, *clu_pin = nullptr;
+  float4 *clu_v = nullptr, *clu_tab = nullptr;
+  int clu_cmask = 0, clu_lmask = 0, clu_tcoul = 0, clu_tl[16];
+  float clu_qq[16];
+  double clu_cut = 0.0;
+  std::vector<float4> ft_host;  // the whole switched pair functions [nftab][CT_N]
   bool full_list = true;   // force list with both directions (no atomics)
   int cap = 0, *jfull = nullptr, *jforce = nullptr, *nfull = nullptr, *nforce = nullptr, *overflow = nullptr;
   int *jsplit = nullptr;  // the force list's second buffer (rows split by the cutoff)
@@ -78,6 +92,7 @@ struct pmx_ctx {
   // molecules of k_intra_mol (nmol = 0: the topology does not split into
   // molecules of at most MOL_MAX consecutive sites; per-term kernels then)
   int nmol = 0, mol_max = 0, ntpl = 0;
+  bool mol_regular = false;  // every molecule has mol_max sites, molecule m at sites m*mol_max ...
   int2 *mol_dev = nullptr;     // [nmol] (first site, template)
   MolTpl *tpl_dev = nullptr;   // [TPL_MAX]
   double *gcoef = nullptr;
@@ -446,6 +461,7 @@ std::vector<double> pair_constants(pmx_ctx *c) {
 // change (the templates hold charge products).
 int setup_molecules(pmx_ctx *c, const pmx_system *sys) {
   c->nmol = 0;
+  c->mol_regular = false;
   if (const char *env = getenv("PMX_INTRA"))  // tuning override: "terms" keeps the per-term kernels
     if (!strcmp(env, "terms")) return PMX_OK;
   const Setup &s = c->s;
@@ -550,6 +566,9 @@ int setup_molecules(pmx_ctx *c, const pmx_system *sys) {
   HIPCHK(hipMemcpy(c->tpl_dev, tpl.data(), tpl.size() * sizeof(MolTpl), hipMemcpyHostToDevice));
   c->ntpl = (int)tpl.size();
   c->mol_max = mmax;
+  c->mol_regular = true;
+  for (int m = 0; m < M; m++) c->mol_regular = c->mol_regular && md[m].x == m * mmax && mt[m].n == mmax;
+  c->mol_regular = c->mol_regular && (size_t)M * mmax == (size_t)N;
   c->nmol = M;
   return PMX_OK;
 }
@@ -657,6 +676,7 @@ int build_pair_tables(pmx_ctx *c) {
     }
   }
   c->nftab = (int)keys.size();
+  c->ft_host = ft;
   for (int t = 0; t < s.npair; t++) {
     c->ftab_off[t].x *= CT_N;
     if (c->ftab_off[t].y >= 0) c->ftab_off[t].y *= CT_N;
@@ -675,6 +695,175 @@ int build_pair_tables(pmx_ctx *c) {
   c->ct_r0 = (float)r0;
   c->ct_idr = (float)(1.0 / dr);
   c->epoch++;
+  return PMX_OK;
+}
+
+// The molecule-cluster pair kernel (mixed precision), when the box
+// qualifies: every molecule one template of ns <= CLU_MAX_NS consecutive
+// sites with the same types and charges, and every site pair with a pair
+// function at the grid's end as its cutoff (past the end the tables hold a
+// zero piece, which stands for the cutoff).  Tables: the whole switched
+// functions of build_pair_tables, CT_N + 1 pieces each.
+int setup_cluster(pmx_ctx *c, const pmx_system *sys) {
+  const bool was_on = c->clu_on;
+  c->clu_ok = false;
+  c->clu_on = false;
+  c->epoch++;
+  const Setup &s = c->s;
+  const int N = s.N, P = s.P, ns = c->mol_max;
+  if (c->pair_mode != 2 || !c->full_list || c->nmol <= 0 || c->ntpl != 1 || !c->mol_regular) return PMX_OK;
+  if (ns < 1 || ns > CLU_MAX_NS || P > 64) return PMX_OK;
+  if ((size_t)c->nftab * (CT_N + 1) * sizeof(float4) > 65 * 1024) return PMX_OK;
+  for (int a = 0; a < ns; a++)
+    for (int m = 1; m < c->nmol; m++)
+      if (sys->atm_type[m * ns + a] != sys->atm_type[a] || sys->q[m * ns + a] != sys->q[a]) return PMX_OK;
+  double rend = 0.0;
+  for (int t = 0; t < s.npair; t++) rend = std::max(rend, s.cutoff[t]);
+  c->clu_cmask = c->clu_lmask = 0;
+  c->clu_tcoul = -1;
+  c->clu_cut = rend;
+  for (int q = 0; q < 16; q++) {
+    c->clu_tl[q] = 0;
+    c->clu_qq[q] = 0.f;
+  }
+  for (int a = 0; a < ns; a++)
+    for (int b = 0; b < ns; b++) {
+      const int q = a * ns + b, t = pair_index(s.ntype, sys->atm_type[a], sys->atm_type[b]);
+      const double qq = sys->q[a] * sys->q[b];
+      const int2 o = c->ftab_off[t];
+      if (qq != 0.0) {
+        const int tc = (o.x / CT_N) * (CT_N + 1);
+        if (c->clu_tcoul >= 0 && tc != c->clu_tcoul) return PMX_OK;  // one erfc table for all
+        c->clu_tcoul = tc;
+        c->clu_cmask |= 1 << q;
+      }
+      if (o.y >= 0) {
+        c->clu_tl[q] = (o.y / CT_N) * (CT_N + 1);
+        c->clu_lmask |= 1 << q;
+      }
+      if (((c->clu_cmask | c->clu_lmask) >> q & 1) && s.cutoff[t] != rend) return PMX_OK;
+      c->clu_qq[q] = (float)qq;
+    }
+  if (c->clu_tcoul < 0) c->clu_tcoul = 0;
+  std::vector<float4> tab;
+  for (int k = 0; k < c->nftab; k++) {
+    tab.insert(tab.end(), c->ft_host.begin() + (size_t)k * CT_N, c->ft_host.begin() + (size_t)(k + 1) * CT_N);
+    tab.push_back(make_float4(0.f, 0.f, 0.f, 0.f));
+  }
+  dfree(c, c->clu_tab);
+  TRY_RC(upload(c, &c->clu_tab, tab.data(), tab.size()));
+  c->clu_npieces = (int)tab.size();
+  if (!c->mcnt) {
+    TRY_RC(dalloc(c, &c->mcnt, (size_t)c->nmol));
+    TRY_RC(dalloc(c, &c->clu_flag, 4));
+    TRY_RC(dalloc(c, &c->clu_v, (size_t)N * P));
+    if (P > 1 && !c->umax) TRY_RC(dalloc(c, &c->umax, (size_t)N));
+    HIPCHK(hipHostMalloc((void **)&c->clu_pin, 4 * sizeof(int), hipHostMallocDefault));
+    c->clu_pin[1] = 0;
+  }
+  c->clu_ns = ns;
+  c->clu_ok = true;
+  c->clu_on = was_on && c->mrow;  // a new force field keeps the force list, and so the rows
+  return PMX_OK;
+}
+
+CluArgs clu_args(pmx_ctx *c) {
+  const Setup &s = c->s;
+  CluArgs a{};
+  a.nmol = c->nmol;
+  a.ns = c->clu_ns;
+  a.N = s.N;
+  a.P = s.P;
+  a.pimd = s.P > 1;
+  a.mcap = c->clu_mcap;
+  a.mrow = c->mrow;
+  a.mcnt = c->mcnt;
+  a.jforce = c->jforce;
+  a.nforce = c->nforce;
+  a.cap = c->cap;
+  a.mrow_w = c->mrow;
+  a.mcnt_w = c->mcnt;
+  a.flag = c->clu_flag;
+  a.xr = s.P > 1 ? c->xm : c->x;
+  a.yr = s.P > 1 ? c->ym : c->y;
+  a.zr = s.P > 1 ? c->zm : c->z;
+  a.x = c->x;
+  a.y = c->y;
+  a.z = c->z;
+  a.vw = c->clu_v;
+  a.v = c->clu_v;
+  a.umax_w = c->umax;
+  a.umax = c->umax;
+  a.cell = c->cell;
+  a.ftab = c->clu_tab;
+  a.npieces = c->clu_npieces;
+  a.idr = c->ct_idr;
+  a.r0i = c->ct_r0 * c->ct_idr;
+  a.cmask = c->clu_cmask;
+  a.lmask = c->clu_lmask;
+  a.tcoul = c->clu_tcoul;
+  a.cut = c->clu_cut;
+  for (int q = 0; q < 16; q++) {
+    a.toff_l[q] = c->clu_tl[q];
+    a.qq[q] = c->clu_qq[q];
+  }
+  a.fx = c->fx;
+  a.fy = c->fy;
+  a.fz = c->fz;
+  a.part = c->part + c->off_pair;
+  return a;
+}
+
+// Molecule rows of the force list just built, and whether the cluster kernel
+// may use them.  A listed pair (a, b) lies within r_list at the build, so in
+// fractional coordinates its separation is within r_list / w of an integer
+// (w the cell's smallest width); the site-0 pair's differs from it by the
+// two sites' offsets from their site 0, at most 2 ext / w, so both round to
+// the same image when r_list + 2 ext < w / 2 (ext: the largest distance of
+// a site from its molecule's site 0, reference positions).  Motion after the
+// build is covered by the kernel's own check of every listed pair.
+int build_clu_rows(pmx_ctx *c) {
+  const bool was = c->clu_on;
+  c->clu_on = false;
+  for (int attempt = 0; attempt < 4; attempt++) {
+    if (!c->mrow) {
+      c->clu_mcap = std::max(64, ((c->cap + 63) / 64) * 64 / 2);
+      TRY_RC(dalloc(c, &c->mrow, (size_t)c->nmol * c->clu_mcap));
+    }
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->st, c->clu_flag, 0, (size_t)4);
+    CluArgs a = clu_args(c);
+    HIPCHK(launch_clu_rows(a, c->st));
+    int fl[4];
+    HIPCHK(hipMemcpyAsync(fl, c->clu_flag, sizeof fl, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (fl[0] > CLU_HASH) break;  // a molecule with too many partners: the per-site kernel
+    if (fl[0] > c->clu_mcap) {
+      if (c->no_alloc) {
+        set_err(c, "the molecule rows need %d entries (now %d), which takes an allocation: "
+                   "one call of pmx_energy_control grows them", fl[0], c->clu_mcap);
+        return PMX_ERR_RANGE;
+      }
+      (void)hipFree(c->mrow);
+      c->allocs.erase(std::remove(c->allocs.begin(), c->allocs.end(), (void *)c->mrow), c->allocs.end());
+      c->mrow = nullptr;
+      const int want = ((fl[0] * 5 / 4 + 63) / 64) * 64;
+      c->clu_mcap = want;
+      TRY_RC(dalloc(c, &c->mrow, (size_t)c->nmol * c->clu_mcap));
+      c->epoch++;
+      continue;
+    }
+    float ext;
+    memcpy(&ext, &fl[2], sizeof ext);
+    double rl = 0.0, wmin = 1e300;
+    for (int t = 0; t < c->s.npair; t++) rl = std::max(rl, sqrt(c->pt.cutskin2[t]));
+    for (int d = 0; d < 3; d++) {
+      const double *hi = c->cell.hi;
+      wmin = std::min(wmin, 1.0 / sqrt(hi[d] * hi[d] + hi[d + 3] * hi[d + 3] + hi[d + 6] * hi[d + 6]));
+    }
+    c->clu_on = rl + 2.0 * (double)ext < 0.5 * wmin * (1.0 - 1e-9);
+    break;
+  }
+  if (c->clu_on != was) c->epoch++;
   return PMX_OK;
 }
 
@@ -816,6 +1005,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   TRY(upload(c, &c->gcomp, sys->ghost_comp, (size_t)4 * sys->nghost));
   TRY(upload(c, &c->gcoef, sys->ghost_coef, (size_t)4 * sys->nghost));
   TRY(setup_molecules(c, sys));
+  TRY(setup_cluster(c, sys));
   // PME grids, twiddles and the influence function on the sphere list
   c->pme = (!s.ichrg.empty() && c->par.pme_on);
   c->pme_f32 = (c->par.precision == PMX_PREC_MIXED) ? 1 : 0;
@@ -1060,6 +1250,12 @@ int build_list(pmx_ctx *c) {
       c->ref_valid = (H == 0);  // the plain scan also wrote the reference's rows
       c->list_cells = H > 0;
       c->nver = ov[1];
+      if (c->clu_ok && c->clu_enabled) {
+        if (int rc = build_clu_rows(c)) return rc;
+      } else if (c->clu_on) {
+        c->clu_on = false;
+        c->epoch++;
+      }
       return PMX_OK;
     }
     int rc = alloc_lists(c, ((ov[0] * 5 / 4 + 63) / 64) * 64);
@@ -1244,7 +1440,9 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
   hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
                      c->q, c->site);
   HIPCHK(hipGetLastError());
-  if (c->disp) {
+  c->disp_packed = false;
+  if (c->disp && !c->clu_on) {
+    c->disp_packed = true;
     if (P <= 64) {
       const size_t waves = ((size_t)N + 64 / P - 1) / (64 / P);
       hipLaunchKernelGGL(k_pack_disp_umax, dim3(blocks(waves * 64)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm,
@@ -1255,7 +1453,7 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
       hipLaunchKernelGGL(k_site_umax, dim3(blocks(N)), dim3(256), 0, st, N, P, c->disp, c->umax);
     }
     HIPCHK(hipGetLastError());
-  } else if (c->umax && c->full_list) {
+  } else if (!c->disp && c->umax && c->full_list) {
     hipLaunchKernelGGL(k_site_umax64, dim3(blocks(N)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym,
                        c->zm, c->umax);
     HIPCHK(hipGetLastError());
@@ -1366,7 +1564,28 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
   RedArgs red{};
   if (timing) HIPCHK(hipEventRecord(c->ev[3], st));
   // --- real space (force_control -> force_verlst -> force_npol)
-  {
+  c->clu_used = c->clu_on;
+  if (c->clu_on) {
+    // the molecule-cluster form of the mixed pair kernel
+    CluArgs a = clu_args(c);
+    HIPCHK(launch_clu_pack(a, st));
+    const int gx = pair_grid_x(c->nmol, c->pair_waves);
+    HIPCHK(launch_pair_clu(a, gx, st));
+    add_stage(red, c->off_pair, gx, PAIR_NV,
+              {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR});
+  } else {
+    if (c->disp && !c->disp_packed) {  // packing skipped in pre_sequence for the cluster kernel
+      if (P <= 64) {
+        const size_t waves = ((size_t)N + 64 / P - 1) / (64 / P);
+        hipLaunchKernelGGL(k_pack_disp_umax, dim3(blocks(waves * 64)), dim3(256), 0, st, N, P, c->x, c->y, c->z,
+                           c->xm, c->ym, c->zm, c->disp, c->umax);
+      } else {
+        hipLaunchKernelGGL(k_pack_disp, dim3(blocks(NP)), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->xm, c->ym,
+                           c->zm, c->disp);
+        hipLaunchKernelGGL(k_site_umax, dim3(blocks(N)), dim3(256), 0, st, N, P, c->disp, c->umax);
+      }
+      HIPCHK(hipGetLastError());
+    }
     PairArgs a{};
     a.N = N;
     a.P = P;
@@ -1528,7 +1747,7 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.veig = c->veig;
     a.cs = c->cs;
     a.part = c->part + c->off_mode;
-    hipLaunchKernelGGL(k_mode_forces, dim3(mode_blocks(N, P)), dim3(256), 0, st, a);
+    HIPCHK(launch_mode_forces(a, st));
     HIPCHK(hipGetLastError());
   }
   red.P = P;
@@ -1553,6 +1772,7 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st) {
 
 // Readback of the per-bead sums into pinned host memory, launched on st.
 int readback_launch(pmx_ctx *c, hipStream_t st) {
+  if (c->clu_used) HIPCHK(hipMemcpyAsync(c->clu_pin + 1, c->clu_flag + 1, sizeof(int), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(c->acc_pin, c->acc, c->acc_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(c->acc_pin + c->acc_h.size(), c->ake, sizeof(double), hipMemcpyDeviceToHost, st));
   return PMX_OK;
@@ -1653,6 +1873,19 @@ void assemble(pmx_ctx *c, unsigned flags, pmx_energies *out, bool timing) {
   c->grad_ready = !(c->pme && (flags & PMX_EVAL_NO_RECIP));
 }
 
+// After a force call that ran the cluster kernel: true when a listed site
+// pair's image differed from its molecule pair's (the margin of
+// build_clu_rows makes this a safety net); the cluster kernel is then off
+// until the next list build and the caller repeats the call.
+bool clu_recheck(pmx_ctx *c) {
+  if (!c->clu_used || c->clu_pin[1] == 0) return false;
+  c->clu_pin[1] = 0;
+  c->clu_on = false;
+  c->clu_used = false;
+  c->epoch++;
+  return true;
+}
+
 int check_call(pmx_ctx *c, unsigned flags, pmx_energies *out, const char *who) {
   if (!c || !out) return PMX_ERR_ARG;
   if (!c->have_pos) {
@@ -1725,6 +1958,9 @@ pmx_ctx *pmx_create(const pmx_system *sys, const pmx_params *par, const double *
     return fail("pmx_create: stream creation failed");
   }
   if (const char *env = getenv("PMX_OVERLAP")) c->overlap = atoi(env) != 0;  // tuning override 0 / 1
+  // tuning override: 1 runs the molecule-cluster form of the mixed pair
+  // kernel where the box qualifies (the per-site form is the default)
+  if (const char *env = getenv("PMX_CLUSTER")) c->clu_enabled = atoi(env) == 1;
   if (const char *env = getenv("PMX_PAIR_WAVES"))  // tuning override: wavefronts of the pair kernels
     c->pair_waves = std::max(64, std::min(PAIR_MAX_WAVES, atoi(env)));
   if (const char *env = getenv("PMX_PME_TILE"))  // tuning override "Ta,Tb,Tc" (upper bounds)
@@ -1820,10 +2056,17 @@ static int force_call(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   if ((rc = pre_sequence(c, flags, st, true, true))) return rc;
   HIPCHK(hipEventSynchronize(c->mxev));
   if ((rc = list_stage_finish(c, flags, out))) return rc;
+  if (c->clu_on) hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, c->clu_flag + 1, 0, (size_t)1);
   if ((rc = main_sequence(c, flags, st, true))) return rc;
   if ((rc = readback_launch(c, st))) return rc;
   HIPCHK(hipStreamSynchronize(st));
   assemble(c, flags, out, true);
+  if (clu_recheck(c)) {  // the call again with the per-site kernel
+    const int rebuilt = out->list_rebuilt;
+    rc = force_call(c, flags, out);
+    out->list_rebuilt |= rebuilt;
+    return rc;
+  }
   return PMX_OK;
 }
 
@@ -1985,6 +2228,7 @@ int pmx_batch_energy_control(pmx_batch *b, unsigned flags, pmx_energies *out) {
       b->err = "context " + std::to_string(i) + ": " + c->err;
       return rc;
     }
+    if (c->clu_on) hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->st, c->clu_flag + 1, 0, (size_t)1);
     e = hipEventRecord(c->ready, c->st);
     if (e == hipSuccess) e = hipStreamWaitEvent(b->cs, c->ready, 0);
     if (e != hipSuccess) return dev_fail(e, "list stage");
@@ -2003,6 +2247,15 @@ int pmx_batch_energy_control(pmx_batch *b, unsigned flags, pmx_energies *out) {
   e = hipStreamSynchronize(b->cs);
   if (e != hipSuccess) return dev_fail(e, "batched force call");
   for (size_t i = 0; i < n; i++) assemble(b->ctx[i], batch_flags(b->ctx[i], flags), &out[i], false);
+  bool again = false;
+  for (size_t i = 0; i < n; i++) again = clu_recheck(b->ctx[i]) || again;
+  if (again) {  // the call again, those boxes with the per-site kernel
+    std::vector<int> rebuilt(n);
+    for (size_t i = 0; i < n; i++) rebuilt[i] = out[i].list_rebuilt;
+    const int rc = pmx_batch_energy_control(b, flags, out);
+    for (size_t i = 0; i < n; i++) out[i].list_rebuilt |= rebuilt[i];
+    return rc;
+  }
   return PMX_OK;
 }
 
@@ -2050,6 +2303,19 @@ int pmx_autotune(pmx_ctx *c, unsigned flags, int reps, int *choice) {
       }
     }
     c->pair_waves = best;
+    // the molecule-cluster form against the per-site one, where the box
+    // qualifies (the rows are built with a forced list build)
+    if (c->clu_ok) {
+      const bool was = c->clu_enabled;
+      c->clu_enabled = !was;
+      double t;
+      if ((rc = pmx_energy_control(c, flags | PMX_EVAL_FORCE_LIST, &e))) return rc;
+      if ((rc = stage_ms(1, flags, &t))) return rc;
+      if (!(t < tbest)) {
+        c->clu_enabled = was;
+        if ((rc = pmx_energy_control(c, flags | PMX_EVAL_FORCE_LIST, &e))) return rc;
+      }
+    }
   }
   // PME tiles (stage 2, k space)
   if (c->pme) {
@@ -2480,6 +2746,7 @@ int pmx_pimd_nvt(pmx_ctx *c, const pmx_nhc_params *np, int nsteps, pmx_energies 
   const unsigned flags = PMX_EVAL_PIMD | PMX_EVAL_MODE_FORCES;
   memset(out, 0, sizeof(*out));
   // forces at the starting positions, xmod as the context has it
+  if (c->clu_ok) hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->st, c->clu_flag + 1, 0, (size_t)1);
   if (int rc = pimd_force(c, flags)) return rc;
   const bool xmod_given = c->xmod_given;
   c->xmod_given = true;  // xmod is mode 1 of each drift from here on
@@ -2519,6 +2786,11 @@ int pmx_pimd_nvt(pmx_ctx *c, const pmx_nhc_params *np, int nsteps, pmx_energies 
   }
   HIPCHK(hipStreamSynchronize(c->st));
   assemble(c, flags, out, false);
+  if (clu_recheck(c)) {
+    set_err(c, "pmx_pimd_nvt: a listed site pair's periodic image differed from its molecule pair's; the "
+               "molecule-cluster kernel is off until the next list build: repeat the call");
+    return PMX_ERR_STATE;
+  }
   if (econs) {
     econs[0] = eo[0];
     econs[1] = eo[1];
@@ -2635,6 +2907,7 @@ int pmx_set_force_field(pmx_ctx *c, const pmx_system *sys) {
   HIPCHK(hipMemcpy(c->pair_pp, pp.data(), pp.size() * sizeof(double), hipMemcpyHostToDevice));
   if (c->pair_mode == 2)
     if (int rc = build_pair_tables(c)) return rc;
+  if (int rc = setup_cluster(c, sys)) return rc;  // same charged sites: the list and its rows stay
   c->bc_h.assign(sys->bond_c, sys->bond_c + 7 * (size_t)c->nbond_type);
   if (c->nbond_type) {
     HIPCHK(hipMemcpy(c->bc, sys->bond_c, 7 * (size_t)c->nbond_type * sizeof(double), hipMemcpyHostToDevice));
@@ -2947,6 +3220,21 @@ int pmx_get_timings(pmx_ctx *c, double *ms5) {
 
 int pmx_get_pair_count(pmx_ctx *c, double *npair) {
   if (!c || !npair) return PMX_ERR_ARG;
+  if (c->clu_used) {
+    // the cluster kernel does not count: a counting pass over the same rows
+    // and positions (the state of the last force call)
+    HIPCHK(hipSetDevice(c->device));
+    CluArgs a = clu_args(c);
+    unsigned long long *cnt = reinterpret_cast<unsigned long long *>(c->clu_flag + 2);
+    HIPCHK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->st));
+    a.count = cnt;
+    HIPCHK(launch_pair_clu(a, pair_grid_x(c->nmol, c->pair_waves), c->st));
+    unsigned long long n = 0;
+    HIPCHK(hipMemcpyAsync(&n, cnt, sizeof n, hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *npair = 0.5 * (double)n;  // the full list holds every pair from both ends
+    return PMX_OK;
+  }
   double s = 0.0;
   for (int b = 0; b < c->s.P; b++) s += c->acc_h[(size_t)b * ACC_NUM + ACC_NPAIR];
   *npair = s;

@@ -351,7 +351,7 @@ __global__ void k_pimd_finish(FinishArgs a);
 __global__ void k_add_rec(int N, int P, const double *rec, double *fx, double *fy, double *fz);
 __global__ void k_ghost_force(int ng, int P, const int *gatom, const int *gncomp, const int *gcomp,
                               const double *gcoef, double *fx, double *fy, double *fz);
-__global__ void k_mode_forces(ModeArgs a);
+hipError_t launch_mode_forces(const ModeArgs &a, hipStream_t st);
 __global__ void k_vv_kick(int n, double dt2, const double *mass, const double *fx, const double *fy,
                           const double *fz, double *vx, double *vy, double *vz);
 __global__ void k_vv_drift(int n, double dt, const double *vx, const double *vy, const double *vz, double *x,
@@ -363,6 +363,58 @@ __global__ void k_kinetic(int n, const double *mass, const double *vx, const dou
 // full: the list holds both directions and each wave stores its sites'
 // forces (no atomics); otherwise j forces are scattered with atomics
 hipError_t launch_pair(const PairArgs &a, int mode, bool full, hipStream_t st);
+// The mixed-precision pair kernel in molecule-cluster form, for boxes whose
+// sites are nmol molecules of one template of ns <= CLU_MAX_NS consecutive
+// sites.  Row I lists the molecules J with at least one pair (I a, J b) in
+// the force list, and the site pairs the force list holds as bits
+// a*ns + b (the list's own pairs, exclusions already removed).  The site
+// pair separation is D_00 + v_Ia - v_Jb: D_00 the imaged separation of the
+// molecules' site-0 reference positions (FP64, rounded), v the FP32
+// position of a bead relative to its molecule's site-0 reference.  It equals
+// the reference's separation whenever the site pair's image is the site-0
+// pair's, which the staging checks for every listed pair (flag[1] counts
+// the pairs where it is not; the host then repeats the call with the
+// per-site kernel).
+constexpr int CLU_MAX_NS = 4;
+constexpr int CLU_HASH = 2048;  // LDS hash slots of the row build (unique partners of one molecule)
+
+struct CluArgs {
+  int nmol, ns, N, P, pimd;
+  int mcap;                     // row width
+  const int2 *mrow;             // [nmol][mcap] (J, site-pair bits)
+  const int *mcnt;              // [nmol]
+  // row build: the force list's site rows, the rows written, and
+  // flag[0] the longest row, flag[2] the largest distance of a site from its
+  // molecule's site 0 (float bits); flag[1] image mismatches (pair kernel)
+  const int *jforce, *nforce;
+  int cap;
+  int2 *mrow_w;
+  int *mcnt_w;
+  int *flag;
+  const double *xr, *yr, *zr;   // reference positions: xmod (PIMD) or bead positions (P = 1) [N]
+  const double *x, *y, *z;      // bead positions [N*P] (packing)
+  float4 *vw;                   // packing output
+  const float4 *v;              // x - xr of the molecule's site 0, per (site, bead) [N*P]
+  double *umax_w;               // packing output (PIMD)
+  const double *umax;           // PIMD: the largest |x - xmod| of every site over its beads [N]
+  Cell cell;
+  const float4 *ftab;           // the whole pair functions, CT_N + 1 pieces each (the last zero)
+  int npieces;                  // pieces held in LDS
+  float r0i, idr;               // piece coordinate u = r idr - r0i
+  int cmask, lmask;             // site pairs (bit a*ns + b) with an erfc term, with an LJ term
+  int tcoul;                    // the erfc table (all site pairs share it), offset in pieces
+  int toff_l[16];               // per site pair: LJ table offset in pieces
+  float qq[16];                 // per site pair: q_a q_b
+  double cut;                   // the cutoff of every site pair (Bohr)
+  double *fx, *fy, *fz;
+  double *part;                 // per-block partial sums [gridDim.x][P][PAIR_NV]
+  unsigned long long *count;    // set: count the site pairs inside the cutoff instead (nothing else written)
+};
+
+hipError_t launch_clu_pack(const CluArgs &a, hipStream_t st);
+hipError_t launch_clu_rows(const CluArgs &a, hipStream_t st);
+hipError_t launch_pair_clu(const CluArgs &a, int gx, hipStream_t st);
+
 hipError_t launch_pair_grad(const PairArgs &a, int mode, hipStream_t st);
 int pair_grid_x(int N, int max_waves);
 constexpr int PAIR_MAX_WAVES = 16384;  // the largest pair grid the autotuner tries

@@ -1166,8 +1166,15 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
       };
       // the displacement of pair p into u (left as it is past the chunk's
       // end, where it is not used)
+      // The load is issued on every path (past the chunk's end it reads this
+      // bead of site 0): with the load under a branch the compiler waits for
+      // every outstanding load before each pair (vmcnt(0)), including the
+      // prefetches, and the pair loop runs at the memory latency.
       auto fetch = [&](float4 &u, int p) {
-        if (PIMD && p < nk) u = a.u[(unsigned)(sent[w][p].x + bsel)];
+        if (PIMD) {
+          const int e = sent[w][min(p, 63)].x;
+          u = a.u[(unsigned)((p < nk ? e : 0) + bsel)];
+        }
       };
       // four pairs per iteration, each displacement loaded a whole iteration
       // ahead into a register of its own: no register copies, which would
@@ -1958,13 +1965,15 @@ __global__ void k_ghost_force(int ng, int P, const int *__restrict__ gatom, cons
 // (x, y, z, fx, fy, fz) in the wavefront's LDS rows, then forms one mode
 // from the site's rows (reads of one address per group: LDS broadcasts).
 // The spring energy of a workgroup goes to part[blockIdx.x], summed in a
-// fixed order.
+// fixed order.  PT > 0 is the bead count at compile time (powers of two up
+// to 64): the transform's sums unroll and their LDS reads issue together;
+// PT = 0 takes P at run time.
+template <int PT>
 __global__ void __launch_bounds__(256) k_mode_forces(ModeArgs a) {
-#pragma clang fp contract(off)
   __shared__ double3 spos[4][64], sfrc[4][64];
   __shared__ double2 stw[64];
   __shared__ double red[4];
-  const int P = a.P;
+  const int P = PT > 0 ? PT : a.P;
   for (int m = threadIdx.x; m < P; m += blockDim.x) stw[m] = make_double2(a.cs[2 * m], a.cs[2 * m + 1]);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int S = 64 / P, g = lane / P, m = lane - g * P, lead = g * P;
@@ -2022,10 +2031,13 @@ __global__ void __launch_bounds__(256) k_mode_forces(ModeArgs a) {
       }
       u = make_double3(0.0, 0.0, 0.0);
       F = u;
-      int ph = 0;  // k b mod P
-      for (int b = 0; b < P; b++) {
+#pragma unroll
+      for (int b = 0; b < (PT > 0 ? PT : 64); b++) {
+        if (PT == 0 && b >= P) break;
+        // exp(+2 pi i k b / P): its real or imaginary part
+        const int ph = PT > 0 ? ((k * b) & (PT - 1)) : (k * b) % P;
         const double2 cs = stw[ph];
-        const double wgt = im ? cs.y : cs.x;  // exp(+2 pi i k b / P): real or imaginary part
+        const double wgt = im ? cs.y : cs.x;
         const double3 xv = px[b], fv = pf[b];
         u.x += xv.x * wgt;
         u.y += xv.y * wgt;
@@ -2033,8 +2045,6 @@ __global__ void __launch_bounds__(256) k_mode_forces(ModeArgs a) {
         F.x += fv.x * wgt;
         F.y += fv.y * wgt;
         F.z += fv.z * wgt;
-        ph += k;
-        if (ph >= P) ph -= P;
       }
       const double dP = (double)P;
       u.x /= dP;
@@ -2102,6 +2112,450 @@ __global__ void k_kinetic(int n, const double *__restrict__ mass, const double *
 
 // Workgroups (4 wavefronts, one site per wavefront at a time) of the pair
 // kernels for N sites and at most max_waves wavefronts.
+// ---------------------------------------------------------------------------
+// Molecule-cluster form of the mixed-precision pair kernel (CluArgs).
+
+// The shift s of the minimum image of a separation d: s = -nint(h^-1 d), as
+// period and period_pimd form it (period.c:27-206), and d + h s.
+__device__ __forceinline__ void clu_image(const Cell &c, double dx, double dy, double dz, double (&s)[3],
+                                          double &ix, double &iy, double &iz) {
+#pragma clang fp contract(off)
+  if (c.cubic) {
+    s[0] = -nint_d(dx * c.hi[0]);
+    s[1] = -nint_d(dy * c.hi[4]);
+    s[2] = -nint_d(dz * c.hi[8]);
+    ix = dx + s[0] * c.h[0];
+    iy = dy + s[1] * c.h[4];
+    iz = dz + s[2] * c.h[8];
+  } else {
+    const double fa = dx * c.hi[0] + dy * c.hi[3] + dz * c.hi[6];
+    const double fb = dx * c.hi[1] + dy * c.hi[4] + dz * c.hi[7];
+    const double fc = dx * c.hi[2] + dy * c.hi[5] + dz * c.hi[8];
+    s[0] = -nint_d(fa);
+    s[1] = -nint_d(fb);
+    s[2] = -nint_d(fc);
+    ix = dx + (s[0] * c.h[0] + s[1] * c.h[3] + s[2] * c.h[6]);
+    iy = dy + (s[0] * c.h[1] + s[1] * c.h[4] + s[2] * c.h[7]);
+    iz = dz + (s[0] * c.h[2] + s[1] * c.h[5] + s[2] * c.h[8]);
+  }
+}
+
+// Positions relative to the molecule's site-0 reference, in FP32, and for
+// beads the largest displacement of every site from its xmod: a group of P
+// lanes holds one site's beads (64 / P sites per wavefront, P <= 64).
+__global__ void __launch_bounds__(256) k_clu_pack(CluArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int P = a.P, S = 64 / P, g = lane / P, k = lane - g * P, lead = g * P;
+  const int i = wave * S + g;
+  const bool act = g < S && i < a.N;
+  float m = 0.0f;
+  if (act) {
+    const int i0 = (i / a.ns) * a.ns;
+    const size_t t = (size_t)i * P + k;
+    const double x = a.x[t], y = a.y[t], z = a.z[t];
+    a.vw[t] = make_float4((float)(x - a.xr[i0]), (float)(y - a.yr[i0]), (float)(z - a.zr[i0]), 0.0f);
+    if (a.pimd) {
+      const float ux = (float)(x - a.xr[i]), uy = (float)(y - a.yr[i]), uz = (float)(z - a.zr[i]);
+      m = ux * ux + uy * uy + uz * uz;
+    }
+  }
+  if (!a.pimd) return;
+  float mx = m;
+  if ((P & (P - 1)) == 0) {
+    for (int off = 1; off < P; off <<= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+  } else {
+    for (int o = 1; o < P; o++) mx = fmaxf(mx, __shfl(m, lead + (k + o) % P, 64));
+  }
+  if (act && k == 0) a.umax_w[i] = sqrt((double)mx);
+}
+
+// Molecule rows from the force list's site rows: one wavefront per molecule
+// I gathers the partner sites of its ns sites into an LDS hash on the
+// partner molecule, OR-ing the site-pair bits, then writes the partners in
+// ascending order of their site-0 distance at the build (a bitonic sort of
+// the compacted keys, ties by partner index), so every build gives the same
+// rows and neighbouring entries, which the pair kernel's slots take side by
+// side, have alike site-pair sets.  flag[0] receives the longest row (the host grows
+// the rows and builds again when it exceeds mcap); a molecule with more
+// than CLU_HASH / 2 partners sets flag[0] to CLU_HASH + 1, and the host then
+// keeps the per-site kernel.
+__global__ void __launch_bounds__(64) k_clu_rows(CluArgs a) {
+  __shared__ int hk[CLU_HASH];
+  __shared__ int hm[CLU_HASH];
+  const int lane = threadIdx.x;
+  const int I = blockIdx.x;
+  if (I >= a.nmol) return;
+  const int ns = a.ns;
+  for (int k = lane; k < CLU_HASH; k += 64) {
+    hk[k] = -1;
+    hm[k] = 0;
+  }
+  __syncthreads();
+  if (lane < ns) {  // the site's distance from site 0 (reference positions): the image margin
+    const int i = I * ns + lane, i0 = I * ns;
+    const double dx = a.xr[i] - a.xr[i0], dy = a.yr[i] - a.yr[i0], dz = a.zr[i] - a.zr[i0];
+    atomicMax(a.flag + 2, __float_as_int((float)sqrt(dx * dx + dy * dy + dz * dz) * 1.0001f));
+  }
+  int inserted = 0;
+  for (int s = 0; s < ns; s++) {
+    const int i = I * ns + s;
+    const int n = a.nforce[i];
+    const int *row = a.jforce + (size_t)i * a.cap;
+    for (int k = lane; k < n; k += 64) {
+      const int j = row[k] & ENTRY_MASK;
+      const int J = j / ns, b = j - J * ns;
+      unsigned h = ((unsigned)J * 2654435761u) >> (32 - 11);
+      for (int probe = 0; probe < CLU_HASH; probe++) {
+        const int old = atomicCAS(&hk[h], -1, J);
+        if (old == -1 || old == J) {
+          atomicOr(&hm[h], 1 << (s * ns + b));
+          inserted += (old == -1);
+          break;
+        }
+        h = (h + 1) & (CLU_HASH - 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int off = 32; off >= 1; off >>= 1) inserted += __shfl_xor(inserted, off, 64);
+  const int nu = inserted;
+  if (nu > CLU_HASH / 2) {
+    if (lane == 0) atomicMax(a.flag, CLU_HASH + 1);
+    return;
+  }
+  // compact the occupied slots into keys (J, bits) at the front
+  __shared__ int ck[CLU_HASH / 2], cm[CLU_HASH / 2], cj[CLU_HASH / 2];
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int pos = 0;
+  for (int b0 = 0; b0 < CLU_HASH; b0 += 64) {
+    const int J = hk[b0 + lane];
+    const bool occ = J >= 0;
+    const uint64_t msk = __ballot(occ);
+    if (occ) {
+      const int p = pos + __popcll(msk & below);
+      double s0[3], dx, dy, dz;
+      clu_image(a.cell, a.xr[I * ns] - a.xr[J * ns], a.yr[I * ns] - a.yr[J * ns], a.zr[I * ns] - a.zr[J * ns], s0,
+                dx, dy, dz);
+      // the key: the distance's float bits (ordered like the distances),
+      // low bits replaced by nothing; ties go by the partner index below
+      ck[p] = __float_as_int((float)sqrt(dx * dx + dy * dy + dz * dz));
+      cj[p] = J;
+      cm[p] = hm[b0 + lane];
+    }
+    pos += __popcll(msk);
+  }
+  int n2 = 1;
+  while (n2 < nu) n2 <<= 1;
+  for (int k = nu + lane; k < n2; k += 64) {
+    ck[k] = 0x7fffffff;
+    cj[k] = 0x7fffffff;
+  }
+  __syncthreads();
+  // bitonic sort of (ck, cm) by ck over n2 keys
+  for (int size = 2; size <= n2; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = lane; t < (n2 >> 1); t += 64) {
+        const int lo = 2 * stride * (t / stride) + (t % stride), hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        const int kl = ck[lo], kh = ck[hi], jl = cj[lo], jh = cj[hi];
+        if ((kl > kh || (kl == kh && jl > jh)) == up) {
+          ck[lo] = kh;
+          ck[hi] = kl;
+          cj[lo] = jh;
+          cj[hi] = jl;
+          const int ml = cm[lo];
+          cm[lo] = cm[hi];
+          cm[hi] = ml;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (nu <= a.mcap) {
+    int2 *out = a.mrow_w + (size_t)I * a.mcap;
+    for (int k = lane; k < nu; k += 64) out[k] = make_int2(cj[k], cm[k]);
+  }
+  if (lane == 0) {
+    a.mcnt_w[I] = nu;
+    atomicMax(a.flag, nu);
+  }
+}
+
+// A wavefront owns molecule I; its lanes are (slot, bead) with B beads per
+// slot.  The row is taken in chunks of 64 entries: each lane stages one, in
+// FP64: the site-0 separation and its image, then for every listed site pair
+// the check that its own image is the same and (beads) the cull of pairs
+// that no bead can bring inside the cutoff (|D_ab| - max|u_a| - max|u_b| >=
+// r_c); entries left without site pairs are dropped and the chunk
+// compacted.  The (slot, bead) lanes then evaluate the chunk's entries in
+// FP32: for each listed site pair one cubic piece of the whole switched,
+// shifted function (erfc term per unit charge product, LJ term) from LDS;
+// r beyond the cutoff reads the zero piece past the grid's end.  Forces of
+// I's sites are summed per lane in FP32 over the row (a few hundred terms
+// per site) and over the slots in FP64 at the end of the row; energies and
+// virial per lane in FP32 over the lane's rows, then in FP64 over lanes and
+// workgroups, halved for the full list's double count.
+template <int B, int NS, bool PIMD, bool COUNT>
+__global__ void __launch_bounds__(256) k_pair_clu(CluArgs a) {
+  extern __shared__ float4 sft[];
+  __shared__ float4 sD[4][64];
+  __shared__ int2 sE[4][64];
+  for (int k = threadIdx.x; k < a.npieces; k += blockDim.x) sft[k] = a.ftab[k];
+  __syncthreads();
+  constexpr int S = 64 / B;
+  constexpr int NP2 = NS * NS;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int slot = lane / B;
+  const int bead = (lane % B) + blockIdx.y * B;
+  const bool bact = bead < a.P;
+  const int bsel = bact ? bead : 0;
+  const int P = a.P;
+  // the molecule index is uniform over the wavefront: its loads are scalar
+  const int wave = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  const int nwave = (int)((gridDim.x * blockDim.x) >> 6);
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const Cell &c = a.cell;
+  const float idr = a.idr, r0i = a.r0i;
+  int mism = 0;
+  unsigned npair = 0;
+  // energies and virial in FP32 over the lane's rows (a few thousand terms),
+  // FP64 across lanes and workgroups
+  float cvl = 0.f, cvc = 0.f, c11 = 0.f, c22 = 0.f, c33 = 0.f, c12 = 0.f, c13 = 0.f, c23 = 0.f;
+  for (int I = wave; I < a.nmol; I += nwave) {
+    float vix[NS], viy[NS], viz[NS];
+    float fix[NS], fiy[NS], fiz[NS];
+    double xri[NS], yri[NS], zri[NS], umi[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const float4 v = a.v[(size_t)(I * NS + s) * P + bsel];
+      vix[s] = v.x;
+      viy[s] = v.y;
+      viz[s] = v.z;
+      fix[s] = fiy[s] = fiz[s] = 0.f;
+      xri[s] = a.xr[I * NS + s];
+      yri[s] = a.yr[I * NS + s];
+      zri[s] = a.zr[I * NS + s];
+      umi[s] = PIMD ? a.umax[I * NS + s] : 0.0;
+    }
+    const int cnt = a.mcnt[I];
+    const int2 *row = a.mrow + (size_t)I * a.mcap;
+    for (int k0 = 0; k0 < cnt; k0 += 64) {
+      const int ks = k0 + lane;
+      int J = 0, m = 0;
+      float4 dd = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (ks < cnt) {
+        const int2 e = row[ks];
+        J = e.x;
+        m = e.y;
+        double s0[3], dx0, dy0, dz0;
+        clu_image(c, xri[0] - a.xr[J * NS], yri[0] - a.yr[J * NS], zri[0] - a.zr[J * NS], s0, dx0, dy0, dz0);
+        dd = make_float4((float)dx0, (float)dy0, (float)dz0, 0.f);
+#pragma unroll
+        for (int sa = 0; sa < NS; sa++) {
+#pragma unroll
+          for (int sb = 0; sb < NS; sb++) {
+            const int bit = 1 << (sa * NS + sb);
+            if (!(m & bit)) continue;
+            const int j = J * NS + sb;
+            double sab[3], ex, ey, ez;
+            clu_image(c, xri[sa] - a.xr[j], yri[sa] - a.yr[j], zri[sa] - a.zr[j], sab, ex, ey, ez);
+            mism += (sab[0] != s0[0]) | (sab[1] != s0[1]) | (sab[2] != s0[2]);
+            const double lim = PIMD ? (a.cut + umi[sa] + a.umax[j]) * (1.0 + 1e-5) : a.cut;
+            if (!(ex * ex + ey * ey + ez * ez < lim * lim)) m &= ~bit;
+          }
+        }
+      }
+      const bool keep = m != 0;
+      const uint64_t mk = __ballot(keep);
+      if (keep) {
+        const int pos = __popcll(mk & below);
+        sD[w][pos] = dd;
+        sE[w][pos] = make_int2(J, m);
+      }
+      wave_sync();
+      const int nk = __popcll(mk);
+      float(&cfx)[NS] = fix, (&cfy)[NS] = fiy, (&cfz)[NS] = fiz;
+      // the partner's positions for entry p (past the chunk's end: molecule 0,
+      // unused), loaded on every path so the loads pipeline
+      auto fetch = [&](float (&jx)[NS], float (&jy)[NS], float (&jz)[NS], int p) {
+        const int e = sE[w][min(p, 63)].x;
+        const size_t base = (size_t)(p < nk ? e : 0) * NS;
+#pragma unroll
+        for (int sb = 0; sb < NS; sb++) {
+          const float4 v = a.v[(base + sb) * P + bsel];
+          jx[sb] = v.x;
+          jy[sb] = v.y;
+          jz[sb] = v.z;
+        }
+      };
+      auto entry = [&](int p, const float (&jx)[NS], const float (&jy)[NS], const float (&jz)[NS]) {
+        const float4 D = sD[w][p];
+        const int mm = sE[w][p].y;
+        float gx[NS], gy[NS], gz[NS];
+#pragma unroll
+        for (int sa = 0; sa < NS; sa++) {
+          gx[sa] = D.x + vix[sa];
+          gy[sa] = D.y + viy[sa];
+          gz[sa] = D.z + viz[sa];
+        }
+#pragma unroll
+        for (int sa = 0; sa < NS; sa++) {
+#pragma unroll
+          for (int sb = 0; sb < NS; sb++) {
+            const int q = sa * NS + sb;
+            const bool hc = (a.cmask >> q) & 1, hl = (a.lmask >> q) & 1;  // uniform
+            if (!hc && !hl) continue;  // no function for this site pair
+            if (!(mm & (1 << q))) continue;
+            const float dx = gx[sa] - jx[sb], dy = gy[sa] - jy[sb], dz = gz[sa] - jz[sb];
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (COUNT) {  // pairs inside the cutoff, for pmx_get_pair_count
+              const float ct = (float)a.cut;
+              npair += (bact && r2 < ct * ct) ? 1 : 0;
+              continue;
+            }
+            const float rinv = __frsqrt_rn(r2);
+            const float u = (r2 * rinv) * idr - r0i;
+            const unsigned k = min(__float2uint_rz(u), (unsigned)CT_N);
+            const float t = u - (float)k;
+            float dv = 0.f;
+            if (hc) {
+              const float4 cc = sft[a.tcoul + k];
+              const float qq = a.qq[q];
+              cvc += qq * (cc.x + t * (cc.y + t * (cc.z + t * cc.w)));
+              dv = qq * (cc.y + t * (2.f * cc.z + 3.f * t * cc.w));
+            }
+            if (hl) {
+              const float4 cl = sft[a.toff_l[q] + k];
+              cvl += cl.x + t * (cl.y + t * (cl.z + t * cl.w));
+              dv += cl.y + t * (2.f * cl.z + 3.f * t * cl.w);
+            }
+            const float dvd = -(dv * idr) * rinv;  // -(1/r) dV/dr
+            const float fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
+            cfx[sa] += fx1;
+            cfy[sa] += fy1;
+            cfz[sa] += fz1;
+            c11 += dx * fx1;
+            c22 += dy * fy1;
+            c33 += dz * fz1;
+            c12 += dx * fy1;
+            c13 += dx * fz1;
+            c23 += dy * fz1;
+          }
+        }
+      };
+      float jxA[NS], jyA[NS], jzA[NS], jxB[NS], jyB[NS], jzB[NS];
+      fetch(jxA, jyA, jzA, slot);
+      int p = slot;
+      for (; p + S < nk; p += 2 * S) {
+        fetch(jxB, jyB, jzB, p + S);
+        entry(p, jxA, jyA, jzA);
+        fetch(jxA, jyA, jzA, p + 2 * S);
+        entry(p + S, jxB, jyB, jzB);
+      }
+      if (p < nk) entry(p, jxA, jyA, jzA);
+      wave_sync();
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      double gx = fix[s], gy = fiy[s], gz = fiz[s];
+      for (int off = 32; off >= B; off >>= 1) {
+        gx += __shfl_xor(gx, off, 64);
+        gy += __shfl_xor(gy, off, 64);
+        gz += __shfl_xor(gz, off, 64);
+      }
+      if (!COUNT && slot == 0 && bact) {  // the first writer of this step's forces
+        const size_t ib = (size_t)(I * NS + s) * P + bead;
+        a.fx[ib] = gx;
+        a.fy[ib] = gy;
+        a.fz[ib] = gz;
+      }
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) mism += __shfl_xor(mism, off, 64);
+  if (lane == 0 && mism) atomicAdd(a.flag + 1, mism);
+  if (COUNT) {
+    unsigned long long n = npair;
+    for (int off = 32; off >= 1; off >>= 1) n += __shfl_xor(n, off, 64);
+    if (lane == 0) atomicAdd(a.count, n);
+    return;
+  }
+  const double vals[PAIR_NV] = {cvl * 0.5, cvc * 0.5, c11 * 0.5, c22 * 0.5, c33 * 0.5,
+                                c12 * 0.5, c13 * 0.5, c23 * 0.5, 0.0};
+  block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
+}
+
+hipError_t launch_clu_pack(const CluArgs &a, hipStream_t st) {
+  const int S = 64 / a.P;
+  const size_t waves = ((size_t)a.N + S - 1) / S;
+  hipLaunchKernelGGL(k_clu_pack, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_clu_rows(const CluArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_clu_rows, dim3(a.nmol), dim3(64), 0, st, a);
+  return hipGetLastError();
+}
+
+template <int B, int NS, bool PIMD, bool COUNT>
+static void launch_pair_clu_k(const CluArgs &a, dim3 grid, size_t lds, hipStream_t st) {
+  (void)hipFuncSetAttribute((const void *)k_pair_clu<B, NS, PIMD, COUNT>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_clu<B, NS, PIMD, COUNT>), grid, dim3(256), lds, st, a);
+}
+
+template <int B, int NS>
+static void launch_pair_clu_b(const CluArgs &a, dim3 grid, size_t lds, hipStream_t st) {
+  if (a.count) {
+    if (a.pimd) launch_pair_clu_k<B, NS, true, true>(a, grid, lds, st);
+    else launch_pair_clu_k<B, NS, false, true>(a, grid, lds, st);
+  } else {
+    if (a.pimd) launch_pair_clu_k<B, NS, true, false>(a, grid, lds, st);
+    else launch_pair_clu_k<B, NS, false, false>(a, grid, lds, st);
+  }
+}
+
+template <int B>
+static void launch_pair_clu_ns(const CluArgs &a, dim3 grid, size_t lds, hipStream_t st) {
+  switch (a.ns) {
+    case 1: launch_pair_clu_b<B, 1>(a, grid, lds, st); break;
+    case 2: launch_pair_clu_b<B, 2>(a, grid, lds, st); break;
+    case 3: launch_pair_clu_b<B, 3>(a, grid, lds, st); break;
+    default: launch_pair_clu_b<B, 4>(a, grid, lds, st); break;
+  }
+}
+
+hipError_t launch_pair_clu(const CluArgs &a, int gx, hipStream_t st) {
+  int B = 1;
+  while (B < a.P && B < 64) B <<= 1;
+  const dim3 grid(gx, (a.P + B - 1) / B);
+  const size_t lds = (size_t)a.npieces * sizeof(float4);
+  switch (B) {
+    case 1: launch_pair_clu_ns<1>(a, grid, lds, st); break;
+    case 2: launch_pair_clu_ns<2>(a, grid, lds, st); break;
+    case 4: launch_pair_clu_ns<4>(a, grid, lds, st); break;
+    case 8: launch_pair_clu_ns<8>(a, grid, lds, st); break;
+    case 16: launch_pair_clu_ns<16>(a, grid, lds, st); break;
+    case 32: launch_pair_clu_ns<32>(a, grid, lds, st); break;
+    default: launch_pair_clu_ns<64>(a, grid, lds, st); break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_mode_forces(const ModeArgs &a, hipStream_t st) {
+  const dim3 grid(mode_blocks(a.N, a.P)), block(256);
+  switch (a.P) {
+    case 2: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mode_forces<2>), grid, block, 0, st, a); break;
+    case 4: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mode_forces<4>), grid, block, 0, st, a); break;
+    case 8: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mode_forces<8>), grid, block, 0, st, a); break;
+    case 16: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mode_forces<16>), grid, block, 0, st, a); break;
+    case 32: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mode_forces<32>), grid, block, 0, st, a); break;
+    case 64: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mode_forces<64>), grid, block, 0, st, a); break;
+    default: hipLaunchKernelGGL(HIP_KERNEL_NAME(k_mode_forces<0>), grid, block, 0, st, a); break;
+  }
+  return hipGetLastError();
+}
+
 int pair_grid_x(int N, int max_waves) {
   const int waves = N < max_waves ? N : max_waves;
   return (waves + 3) / 4;

@@ -109,8 +109,12 @@ def bond_coef(model: WaterModel):
 
 def bend_coef(model: WaterModel):
     """Cosine/sine series of the harmonic bend, intra_coefs.c:236-277."""
-    fk = (model.bend[1] / KB_KCAL) / BOLTZ
-    th0 = model.bend[2] * (math.pi / 180.0)
+    return bend_series((model.bend[1] / KB_KCAL) / BOLTZ, model.bend[2] * (math.pi / 180.0))
+
+
+def bend_series(fk: float, th0: float):
+    """c_m, s_m of the harmonic bend 1/2 fk (theta - th0)^2 (fk in Hartree,
+    th0 in radians) as PINY expands it in cos and sin (intra_coefs.c:236-277)."""
     cn1, cn2, cn3, cn4 = 49.0 / 72.0, 3.0 / 4.0, 3.0 / 40.0, 1.0 / 180.0
     cn5, cn6, cn7, cn8 = 3.0 / 2.0, 3.0 / 5.0, 1.0 / 10.0, 4.0 / 15.0
     cn9, cn10, cn11 = 8.0 / 45.0, 3.0 / 10.0, 1.0 / 30.0
@@ -165,6 +169,9 @@ class System:
     # a general cell (column-major like PINY's hmat[1..9], cell vectors as
     # columns, Angstrom) instead of the cube of edge `box`
     cell: np.ndarray = None
+    # the ewald_alpha keyword as read from an input file (alpha times the
+    # cube root of the volume), kept to the bit; None: settings.alpha * side
+    ewald_alpha_in: float = None
 
     @property
     def hmat(self):
@@ -184,6 +191,8 @@ class System:
 
     @property
     def ewald_alpha(self):
+        if self.ewald_alpha_in is not None:
+            return self.ewald_alpha_in
         return self.settings.alpha * self.side
 
 

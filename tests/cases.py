@@ -97,6 +97,10 @@ CASES = {
                                         _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
     "spc512": lambda: w.build_config(w.SPC_FW, 512, w.density_box(512), 1, 20,
                                      _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6)),
+    # the 512-molecule box with beads in a triclinic cell
+    "tip512_p4_tric": lambda: w.with_cell(
+        w.build_config(w.Q_TIP4P_F, 512, 24.8, 4, 19, _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=8)),
+        [[24.8, 4.96, 2.48], [0.0, 27.28, 3.72], [0.0, 0.0, 29.76]]),
     # the staging transform (path_int_md_typ staging, transform_stg.c): xmod
     # is bead 1, mode forces by the staging recurrence
     "tip64_p4_stag": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,

@@ -4,7 +4,9 @@ pmx_energy_control, through the C ABI: two calls in a row, the second on a
 smaller system, in FP64 and in the mixed precision, each compared with the
 allocating entry point on its own context with the same input.  Both run
 the same kernels in the same order; only the order of the PME spread's
-atomic additions may differ, so the bar is 1e-12 relative."""
+atomic additions may differ: in FP64 that moves the last bits (bar 1e-12
+relative); the mixed precision rounds the accumulated grid to FP32, where
+a different order can flip a rounding (bar 1e-6, inside the path's 1e-4)."""
 import numpy as np
 import pytest
 
@@ -14,7 +16,7 @@ from tests import cases
 
 pytestmark = pytest.mark.gpu
 
-TOL = 1e-12
+TOLS = {abi.PMX_PREC_FP64: 1e-12, abi.PMX_PREC_MIXED: 1e-6}
 MODES = abi.PMX_EVAL_PIMD | abi.PMX_EVAL_MODE_FORCES
 
 
@@ -30,7 +32,8 @@ def _run(name, precision, noalloc):
     return e, f
 
 
-def _same(e, f, er, fr, label):
+def _same(e, f, er, fr, label, precision=abi.PMX_PREC_FP64):
+    TOL = TOLS[precision]
     for k in abi.ENERGY_KEYS:
         assert abs(e[k] - er[k]) <= TOL * max(abs(er[k]), 1e-6), (label, k, e[k], er[k])
     assert np.allclose(e["pvten"], er["pvten"], rtol=TOL, atol=TOL * np.max(np.abs(er["pvten"])))
@@ -47,7 +50,7 @@ def test_noalloc_matches_energy_control(precision):
         print(f"[noalloc {name} precision {precision}]", flush=True)
         e, f = _run(name, precision, True)
         er, fr = _run(name, precision, False)
-        _same(e, f, er, fr, name)
+        _same(e, f, er, fr, name, precision)
 
 
 def test_noalloc_twice_on_one_context():
