@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PMX_ABI_VERSION 7
+#define PMX_ABI_VERSION 8
 
 /* return codes */
 enum {
@@ -77,9 +77,25 @@ enum {
                                      the skin criterion                   */
   PMX_EVAL_NO_RECIP = 1u << 2,    /* skip the k-space sum (real, intra
                                      terms only)                          */
-  PMX_EVAL_MODE_FORCES = 1u << 3  /* PIMD: also transform the forces to
+  PMX_EVAL_MODE_FORCES = 1u << 3, /* PIMD: also transform the forces to
                                      mode coordinates and add the spring
                                      forces (energy_control_pimd.c:200-218) */
+  /* RESPA force calls (energy_ctrl.iget_res_inter / iget_full_inter, as
+   * int_NVT_pimd.c:174-187 and the RESPA integrators set them; the
+   * intramolecular flags are on).  Without flags a call is a full one. */
+  PMX_EVAL_RES_INTER = 1u << 4,   /* iget_res_inter = 1: the short-range
+                                     part of the pair forces, switched off
+                                     over rheal below each type pair's
+                                     inter_cutoff_res (force_npol.c:807-816,
+                                     1033-1060); needs respa_steps_lrf > 0 */
+  PMX_EVAL_NO_FULL_INTER = 1u << 5,/* iget_full_inter = 0: no full
+                                     intermolecular forces, no k space, no
+                                     excluded-pair correction and no
+                                     intermolecular energies */
+  PMX_EVAL_NO_FULL_INTRA = 1u << 6 /* iget_full_intra = 0 (an inner
+                                     intramolecular RESPA step): no self and
+                                     background terms, no energies kept
+                                     unless the inter forces are full */
 };
 
 /*
@@ -162,6 +178,28 @@ typedef struct pmx_nhc_params {
   double gamma_adb;     /* [path_int_gamma_adb] adiabaticity (mass factor) */
 } pmx_nhc_params;
 
+/* RESPA settings of the force call (PINY input keywords in brackets;
+ * set_sim_params.c:1585-1660).  A steps keyword left out of the input is 0
+ * here, which turns that split off (its step count is then 1).  The steps
+ * fix the force weights of energy_control_initial.c:212-243: with
+ * PMX_EVAL_PIMD (pimd_on) and n_x = max(respa_steps_x, 1)
+ *     wght_ter     = n_pimd n_intra n_torsion n_lrf   (full inter forces)
+ *     wght_ter_res = n_pimd n_intra n_torsion         (short-range part)
+ *     wght_tra_res = n_pimd                           (bonds, bends)
+ * and without it the same with n_pimd = 1 and wght_tra_res = 1.  pext,
+ * pten_inter_guess and nfree enter the RESPA terms of the pressure tensor
+ * (energy_control_inter_real.c:206-318). */
+typedef struct pmx_respa {
+  int respa_steps_lrf;     /* [respa_steps_lrf] > 0: the inter split on     */
+  int respa_steps_torsion; /* [respa_steps_torsion]                         */
+  int respa_steps_intra;   /* [respa_steps_intra]                           */
+  int respa_steps_pimd;    /* [respa_steps_pimd]                            */
+  int nfree;               /* classical degrees of freedom, clatoms_info.nfree
+                              (close_intra_params.c:222-247)                */
+  double pext;             /* [pressure] in Hartree/Bohr^3 (atm * PCONV)    */
+  double pten_inter_guess; /* [pten_inter_respa] in Hartree/Bohr^3         */
+} pmx_respa;
+
 /* Energies (Hartree) and pressure tensors (Hartree), stat_avg / ptens. */
 typedef struct pmx_energies {
   double vintert;       /* inter: vreal + vlong + vrecip+vself+vbgr+vecor  */
@@ -219,6 +257,17 @@ int pmx_set_positions_device(pmx_ctx *ctx, const double *x, const double *y,
  * (period.c:121-206).  If never set, the bead centroid is used. */
 int pmx_set_xmod(pmx_ctx *ctx, const double *x, const double *y,
                  const double *z);
+
+/* RESPA settings for the context's later force calls (NULL: off, all
+ * weights 1).  As parse() insists (set_sim_params.c:3346-3370), the inter
+ * split needs the intramolecular and torsion splits: respa_steps_lrf > 0
+ * with respa_steps_intra or respa_steps_torsion 0 returns PMX_ERR_ARG.
+ * Under weights other than 1 the call returns no virial estimator: the
+ * reference forms pi_ke_vir and pvten_tot from the unweighted forces fxt,
+ * which this library does not keep apart; both are NaN and pmx_get_fxt
+ * returns PMX_ERR_STATE.  The cluster pair kernel and the half list are
+ * not used under RESPA. */
+int pmx_set_respa(pmx_ctx *ctx, const pmx_respa *respa);
 
 /* The force call. */
 int pmx_energy_control(pmx_ctx *ctx, unsigned flags, pmx_energies *out);

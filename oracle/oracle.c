@@ -91,6 +91,8 @@ struct or_ctx {
   /* pimd */
   double *prekf, *veig;
   double *bead_e; /* [P*4] */
+  /* RESPA settings (zeros: off) */
+  pmx_respa respa;
 };
 
 /* ------------------------------------------------------------------------
@@ -1008,12 +1010,24 @@ static void or_spline_eval4(const double *tab, int k, double del, double swit, d
   *out = swit * (c0 + del * (c1 + del * (c2 + del * c3)));
 }
 
+/* One bead's real-space pair terms (force_npol.c:28-328).  full / res are
+ * iget_full_inter / iget_res_inter: the hard cutoff is the full one or,
+ * for the short-range part alone, inter_cutoff_res (npol_shave_skin,
+ * :1291-1395); the force of each term is weighted by wght_ter (full),
+ * by the RESPA switch times wght_ter_res (res), or by their blend (both)
+ * (npol_spldata_fetch :807-816, npol_force_calc :1029-1060); energies only
+ * with full (:243-256).  pvt gets the weighted virial (pten_sum_roll), pvtt
+ * (full only, may be NULL) the unweighted one, fxt (may be NULL) the
+ * unweighted forces (npol_vir_reduc, :1222-1287). */
 static void or_real_bead(const or_ctx *c, const double *x, const double *y, const double *z,
                          const double *xm, const double *ym, const double *zm, double *fx,
-                         double *fy, double *fz, double *vvdw, double *vcoul, double *pvt) {
+                         double *fy, double *fz, double *vvdw, double *vcoul, double *pvt, int full,
+                         int res, double wght_full, double wght_res, double *pvtt, double *fxt,
+                         double *fyt, double *fzt) {
   int N = c->N, n = c->nsplin;
   double vv = 0.0, vt = 0.0;
   int icoul = (c->nchrg > 0);
+  const double rheal_res = c->par.rheal, rheal_resi = 1.0 / c->par.rheal;
   for (int ip = 2; ip <= N; ip++) {
     int ti = c->type[ip - 1];
     double xi = x[ip - 1], yi = y[ip - 1], zi = z[ip - 1], qi = c->q[ip - 1];
@@ -1030,14 +1044,15 @@ static void or_real_bead(const or_ctx *c, const double *x, const double *y, cons
       double r2 = (dx * dx + dy * dy + dz * dz);
       double r = sqrt(r2);
       int t = or_pair(c, ti, c->type[jp - 1]);
-      if (!(r < c->cutoff[t])) continue; /* npol_shave_skin */
+      const double cut = full ? c->cutoff[t] : c->cutoff_res[t];
+      if (!(r < cut)) continue; /* npol_shave_skin */
       double rmin = c->rmin_spl[t], dr = c->dr_spl[t], dri = c->dri_spl[t];
       int kk = (int)(((r - rmin) * dri) + 0.5) + 3;
       kk = imin(kk, n - 2);
       kk = imax(kk, 3);
       double rsp = ((double)(kk - 3)) * dr + rmin;
       double del = (r - rsp) * dri;
-      double swit_hard = ((c->cutoff[t] - r) >= 0.0 ? 1.0 : 0.0);
+      double swit_hard = ((cut - r) >= 0.0 ? 1.0 : 0.0);
       const double *tv = c->cv0 + (size_t)t * n, *tdv = c->cdv0 + (size_t)t * n;
       const double *tvc = c->cv0_c + (size_t)t * n, *tdvc = c->cdv0_c + (size_t)t * n;
       double q2 = qi * c->q[jp - 1];
@@ -1052,21 +1067,48 @@ static void or_real_bead(const or_ctx *c, const double *x, const double *y, cons
       } else {
         dvtemp = dvlj;
       }
-      vv += vlj;
-      vt += vlj + q2 * vc;
+      if (full) {
+        vv += vlj;
+        vt += vlj + q2 * vc;
+      }
       double fx1 = dx * dvtemp, fy1 = dy * dvtemp, fz1 = dz * dvtemp;
-      fxi += fx1;
-      fyi += fy1;
-      fzi += fz1;
-      fx[jp - 1] -= fx1;
-      fy[jp - 1] -= fy1;
-      fz[jp - 1] -= fz1;
-      pvt[1] += dx * fx1;
-      pvt[5] += dy * fy1;
-      pvt[9] += dz * fz1;
-      pvt[2] += dx * fy1;
-      pvt[3] += dx * fz1;
-      pvt[6] += dy * fz1;
+      double fmult = wght_full;
+      if (res) {
+        double br = (r - c->cutoff_res[t] + rheal_res) * rheal_resi;
+        br = dmax(br, 0.0);
+        br = dmin(br, 1.0);
+        const double sw = br * br * (br * 2.0 - 3.0) + 1.0;
+        fmult = full ? (sw * wght_res + (1.0 - sw) * wght_full) : sw * wght_res;
+      }
+      double fx2 = fx1 * fmult, fy2 = fy1 * fmult, fz2 = fz1 * fmult;
+      fxi += fx2;
+      fyi += fy2;
+      fzi += fz2;
+      fx[jp - 1] -= fx2;
+      fy[jp - 1] -= fy2;
+      fz[jp - 1] -= fz2;
+      pvt[1] += dx * fx2;
+      pvt[5] += dy * fy2;
+      pvt[9] += dz * fz2;
+      pvt[2] += dx * fy2;
+      pvt[3] += dx * fz2;
+      pvt[6] += dy * fz2;
+      if (pvtt) {
+        pvtt[1] += dx * fx1;
+        pvtt[5] += dy * fy1;
+        pvtt[9] += dz * fz1;
+        pvtt[2] += dx * fy1;
+        pvtt[3] += dx * fz1;
+        pvtt[6] += dy * fz1;
+      }
+      if (fxt) {
+        fxt[ip - 1] += fx1;
+        fyt[ip - 1] += fy1;
+        fzt[ip - 1] += fz1;
+        fxt[jp - 1] -= fx1;
+        fyt[jp - 1] -= fy1;
+        fzt[jp - 1] -= fz1;
+      }
     }
     fx[ip - 1] += fxi;
     fy[ip - 1] += fyi;
@@ -1657,6 +1699,70 @@ void or_stag_force_to_mode(int P, const double *f, double *fm) {
  * energy_control (energy_control.c:33-77) / energy_control_pimd
  * (energy_control_pimd.c:37-278) with RESPA off.
  */
+/* RESPA on: any of the step keywords given (set_sim_params.c:1585-1660) */
+static int or_respa_on(const or_ctx *c) {
+  const pmx_respa *r = &c->respa;
+  return r->respa_steps_lrf > 0 || r->respa_steps_torsion > 0 || r->respa_steps_intra > 0 ||
+         r->respa_steps_pimd > 0;
+}
+
+/* energy_control_initial.c:212-243: wght_ter, wght_ter_res, wght_tra_res
+ * from the step counts (a keyword left out counts 1) */
+static void or_respa_weights(const or_ctx *c, int pimd_on, double *wt, double *wtr, double *wtrr) {
+  const pmx_respa *r = &c->respa;
+  const int nter = imax(r->respa_steps_lrf, 1), ntor = imax(r->respa_steps_torsion, 1);
+  const int ntra = imax(r->respa_steps_intra, 1), npimd = imax(r->respa_steps_pimd, 1);
+  if (pimd_on) {
+    *wt = (double)(npimd * ntra * ntor * nter);
+    *wtr = (double)(npimd * ntra * ntor);
+    *wtrr = (double)(npimd);
+  } else {
+    *wt = (double)(ntra * ntor * nter);
+    *wtr = (double)(ntra * ntor);
+    *wtrr = 1.0;
+  }
+}
+
+/* Adds one term's forces t and virial tp with weight w, as the reference
+ * scales each term's forces before adding them (ewald3d_pme.c:575-577,
+ * ecorr.c:229-231, bond.c:275-277, bend.c:386-394): f += w t, pvt += w tp;
+ * unweighted into pvtt (NULL outside full calls) and ft (NULL without the
+ * virial estimator). */
+static void or_add_term(int N, double w, const double *tx, const double *ty, const double *tz, const double *tp,
+                        double *fx, double *fy, double *fz, double *pvt, double *pvtt, double *ftx, double *fty,
+                        double *ftz) {
+  for (int i = 0; i < N; i++) {
+    fx[i] += tx[i] * w;
+    fy[i] += ty[i] * w;
+    fz[i] += tz[i] * w;
+  }
+  for (int k = 1; k <= 9; k++) pvt[k] += tp[k] * w;
+  if (pvtt)
+    for (int k = 1; k <= 9; k++) pvtt[k] += tp[k];
+  if (ftx)
+    for (int i = 0; i < N; i++) {
+      ftx[i] += tx[i];
+      fty[i] += ty[i];
+      ftz[i] += tz[i];
+    }
+}
+
+int or_set_respa(or_ctx *c, const pmx_respa *r) {
+  if (!c) return PMX_ERR_ARG;
+  if (!r) {
+    memset(&c->respa, 0, sizeof(c->respa));
+    return PMX_OK;
+  }
+  if (r->respa_steps_lrf < 0 || r->respa_steps_torsion < 0 || r->respa_steps_intra < 0 ||
+      r->respa_steps_pimd < 0 || r->nfree < 0)
+    return PMX_ERR_ARG;
+  /* set_sim_params.c:3346-3370 */
+  if (r->respa_steps_lrf > 0 && (r->respa_steps_intra == 0 || r->respa_steps_torsion == 0)) return PMX_ERR_ARG;
+  if (r->respa_steps_torsion > 0 && r->respa_steps_intra == 0) return PMX_ERR_ARG;
+  c->respa = *r;
+  return PMX_OK;
+}
+
 int or_energy_control(or_ctx *c, double *x, double *y, double *z, const double *xmi,
                       const double *ymi, const double *zmi, unsigned flags, double *fx,
                       double *fy, double *fz, pmx_energies *out) {
@@ -1710,90 +1816,197 @@ int or_energy_control(or_ctx *c, double *x, double *y, double *z, const double *
   out->list_rebuilt = rebuild;
   out->nver = c->nver;
   out->spread = c->spread;
-  /* real space per bead */
+  /* RESPA (energy_control_initial.c:212-243): the force weights from the
+   * step counts; the RESPA integrators make every force call with
+   * iget_res_intra = 1 (int_NVT_pimd.c:174-187, int_NVE_res.c:106-116),
+   * the others with 0 */
+  const int full = !(flags & PMX_EVAL_NO_FULL_INTER), res = (flags & PMX_EVAL_RES_INTER) != 0;
+  const int full_intra = !(flags & PMX_EVAL_NO_FULL_INTRA);
+  const int res_intra = or_respa_on(c);
+  const int pimd_on = (flags & PMX_EVAL_PIMD) != 0;
+  double wt, wtr, wtrr;
+  or_respa_weights(c, pimd_on, &wt, &wtr, &wtrr);
+  /* iver_get (energy_control_intra.c:111): the unweighted forces fxt of the
+   * virial estimator */
+  const int iver = pimd_on && full && P > 1;
+  double *fxt = NULL, *fyt = NULL, *fzt = NULL;
+  if (iver) {
+    fxt = calloc((size_t)P * N, sizeof(double));
+    fyt = calloc((size_t)P * N, sizeof(double));
+    fzt = calloc((size_t)P * N, sizeof(double));
+  }
+  double pvtt[10] = {0};
+  double *tx = calloc(N, sizeof(double)), *ty = calloc(N, sizeof(double)), *tz = calloc(N, sizeof(double));
+  /* real space per bead (energy_control_inter_real.c:112-146) */
   double vvdw = 0.0, vcoul = 0.0;
   for (int ip = 0; ip < P; ip++) {
     size_t o = (size_t)ip * N;
     double a = vvdw, b = vcoul;
-    or_real_bead(c, x + o, y + o, z + o, xm, ym, zm, fx + o, fy + o, fz + o, &vvdw, &vcoul, pvt);
+    if (full || res)
+      or_real_bead(c, x + o, y + o, z + o, xm, ym, zm, fx + o, fy + o, fz + o, &vvdw, &vcoul, pvt, full, res, wt,
+                   wtr, full ? pvtt : NULL, iver ? fxt + o : NULL, iver ? fyt + o : NULL, iver ? fzt + o : NULL);
     c->bead_e[4 * ip + 0] = (vvdw - a) + (vcoul - b);
   }
   out->vvdw = vvdw / dP;
   out->vcoul = vcoul / dP;
   out->vreal = out->vvdw + out->vcoul;
-  /* long range correction (energy_control_inter_real.c:206-318) */
+  /* long range correction with its RESPA pressure terms
+   * (energy_control_inter_real.c:148-157,206-318) */
   double vol = c->vol;
-  out->vlong = c->clong / vol;
+  out->vlong = 0.0;
   {
-    double amult = c->iswit ? 1.0 : 2.0;
-    double pd = (c->clong / vol) * amult * dP;
-    pvt[1] += pd;
-    pvt[5] += pd;
-    pvt[9] += pd;
+    const double amult = c->iswit ? 1.0 : 2.0, amult_bead = amult * dP;
+    const double T = c->par.temperature > 0 ? c->par.temperature : 300.0;
+    double pd_bead = 0.0, pd_tra, pd_ter, d;
+    if (P > 1 && wtrr != 1.0) {
+      const double pkin = (double)(c->respa.nfree / 3) * T / (OR_BOLTZ * vol);
+      pd_bead = (c->respa.pext - pkin) * vol * dP;
+      pvt[1] += pd_bead;
+      pvt[5] += pd_bead;
+      pvt[9] += pd_bead;
+    }
+    if (res_intra && wt != 1.0) {
+      pd_tra = c->respa.pten_inter_guess * vol * dP;
+      d = (pd_tra - pd_bead) * wtrr;
+      pvt[1] += d;
+      pvt[5] += d;
+      pvt[9] += d;
+    } else {
+      pd_tra = pd_bead;
+    }
+    if (res) {
+      pd_ter = (c->clong_res / vol) * amult_bead;
+      d = (pd_ter - pd_tra) * wtr;
+      pvt[1] += d;
+      pvt[5] += d;
+      pvt[9] += d;
+    } else {
+      pd_ter = pd_tra;
+    }
+    if (full) {
+      const double pd_full = (c->clong / vol) * amult_bead;
+      d = (pd_full - pd_ter) * wt;
+      pvt[1] += d;
+      pvt[5] += d;
+      pvt[9] += d;
+      out->vlong = c->clong / vol;
+      pvtt[1] += pd_full;
+      pvtt[5] += pd_full;
+      pvtt[9] += pd_full;
+    }
   }
-  /* reciprocal space (energy_control_inter_recip.c:119-355) */
+  /* reciprocal space (energy_control_inter_recip.c:119-355) with no RESPA
+   * k vectors (nktot_res = 0): the k-space sum and the excluded-pair
+   * correction in full calls, weighted by wght_ter */
   double vrecip = 0.0, vecor = 0.0;
   if (c->nchrg > 0) {
-    double qq = 0.0, qs = 0.0;
-    for (int i = 0; i < N; i++) {
-      qq += c->q[i] * c->q[i];
-      qs += c->q[i];
+    if (full_intra || !res_intra) {
+      double qq = 0.0, qs = 0.0;
+      for (int i = 0; i < N; i++) {
+        qq += c->q[i] * c->q[i];
+        qs += c->q[i];
+      }
+      out->vself = -qq * (c->alp_ewd) * (c->self_erf) / sqrt(M_PI);
+      double bgr = -0.5 * qs * qs * M_PI / (c->alp_ewd * c->alp_ewd * vol);
+      out->vbgr = bgr;
+      pvt[1] += bgr * wtrr * dP;
+      pvt[5] += bgr * wtrr * dP;
+      pvt[9] += bgr * wtrr * dP;
+      if (full) {
+        pvtt[1] += bgr * dP;
+        pvtt[5] += bgr * dP;
+        pvtt[9] += bgr * dP;
+      }
     }
-    out->vself = -qq * (c->alp_ewd) * (c->self_erf) / sqrt(M_PI);
-    double bgr = -0.5 * qs * qs * M_PI / (c->alp_ewd * c->alp_ewd * vol);
-    out->vbgr = bgr;
-    pvt[1] += bgr * dP;
-    pvt[5] += bgr * dP;
-    pvt[9] += bgr * dP;
-    if (!(flags & PMX_EVAL_NO_RECIP)) {
+    if (full && !(flags & PMX_EVAL_NO_RECIP)) {
       for (int ip = 0; ip < P; ip++) {
         size_t o = (size_t)ip * N;
-        double v = c->par.pme_on ? or_pme(c, x + o, y + o, z + o, fx + o, fy + o, fz + o, pvt)
-                                 : or_ewald(c, x + o, y + o, z + o, fx + o, fy + o, fz + o, pvt);
+        double tp[10] = {0};
+        memset(tx, 0, sizeof(double) * N);
+        memset(ty, 0, sizeof(double) * N);
+        memset(tz, 0, sizeof(double) * N);
+        double v = c->par.pme_on ? or_pme(c, x + o, y + o, z + o, tx, ty, tz, tp)
+                                 : or_ewald(c, x + o, y + o, z + o, tx, ty, tz, tp);
+        or_add_term(N, wt, tx, ty, tz, tp, fx + o, fy + o, fz + o, pvt, pvtt, iver ? fxt + o : NULL,
+                    iver ? fyt + o : NULL, iver ? fzt + o : NULL);
         vrecip += v;
         c->bead_e[4 * ip + 1] = v;
       }
     }
-    for (int ip = 0; ip < P; ip++) {
-      size_t o = (size_t)ip * N;
-      double v = or_ecor(c, x + o, y + o, z + o, xm, ym, zm, fx + o, fy + o, fz + o, pvt);
-      vecor += v;
-      c->bead_e[4 * ip + 2] = v;
+    if (full) {
+      for (int ip = 0; ip < P; ip++) {
+        size_t o = (size_t)ip * N;
+        double tp[10] = {0};
+        memset(tx, 0, sizeof(double) * N);
+        memset(ty, 0, sizeof(double) * N);
+        memset(tz, 0, sizeof(double) * N);
+        double v = or_ecor(c, x + o, y + o, z + o, xm, ym, zm, tx, ty, tz, tp);
+        or_add_term(N, wt, tx, ty, tz, tp, fx + o, fy + o, fz + o, pvt, pvtt, iver ? fxt + o : NULL,
+                    iver ? fyt + o : NULL, iver ? fzt + o : NULL);
+        vecor += v;
+        c->bead_e[4 * ip + 2] = v;
+      }
     }
   }
   out->vrecip = vrecip / dP;
   out->vecor = vecor / dP;
-  /* intramolecular (energy_control_intra.c:33-331) */
+  /* intramolecular (energy_control_intra.c:33-331): bonds and bends
+   * weighted by wght_tra_res (bond.c:118, bend.c:144; bond_both.c splits
+   * a bond at eq_res, which equals eq here) */
   double vbond = 0.0, vbend = 0.0;
-  for (int ip = 0; ip < P; ip++) {
-    size_t o = (size_t)ip * N;
-    double vb = or_bond(c, x + o, y + o, z + o, xm, ym, zm, fx + o, fy + o, fz + o, pvt);
-    double va = or_bend(c, x + o, y + o, z + o, xm, ym, zm, fx + o, fy + o, fz + o, pvt);
-    vbond += vb;
-    vbend += va;
-    c->bead_e[4 * ip + 3] = vb + va;
+  if (full_intra || res_intra) {
+    for (int ip = 0; ip < P; ip++) {
+      size_t o = (size_t)ip * N;
+      double tp[10] = {0};
+      memset(tx, 0, sizeof(double) * N);
+      memset(ty, 0, sizeof(double) * N);
+      memset(tz, 0, sizeof(double) * N);
+      double vb = or_bond(c, x + o, y + o, z + o, xm, ym, zm, tx, ty, tz, tp);
+      double va = or_bend(c, x + o, y + o, z + o, xm, ym, zm, tx, ty, tz, tp);
+      or_add_term(N, wtrr, tx, ty, tz, tp, fx + o, fy + o, fz + o, pvt, full ? pvtt : NULL,
+                  iver ? fxt + o : NULL, iver ? fyt + o : NULL, iver ? fzt + o : NULL);
+      vbond += vb;
+      vbend += va;
+      c->bead_e[4 * ip + 3] = vb + va;
+    }
   }
-  out->vbond = vbond / dP;
-  out->vbend = vbend / dP;
+  const int keep_e = full_intra || full; /* energies stored (energy_control_intra.c:284) */
+  out->vbond = keep_e ? vbond / dP : 0.0;
+  out->vbend = keep_e ? vbend / dP : 0.0;
   pvt[4] = pvt[2];
   pvt[7] = pvt[3];
   pvt[8] = pvt[6];
-  out->vintert = out->vreal + out->vlong + out->vrecip + out->vself + out->vbgr + out->vecor;
+  pvtt[4] = pvtt[2];
+  pvtt[7] = pvtt[3];
+  pvtt[8] = pvtt[6];
+  if (!keep_e) out->vself = out->vbgr = 0.0;
+  out->vintert = full ? out->vreal + out->vlong + out->vrecip + out->vself + out->vbgr + out->vecor : 0.0;
   out->vintrat = out->vbond + out->vbend;
   /* PIMD: f/P, pvten/P, virial correction (energy_control_pimd.c:116-144,
-   * pimd_estimators.c:223-323) */
+   * pimd_estimators.c:223-323): pvten with the (weighted) forces, the
+   * virial estimator and pvten_tot with the unweighted fxt, in full calls */
   if (flags & PMX_EVAL_PIMD) {
     for (size_t k = 0; k < (size_t)P * N; k++) {
       fx[k] /= dP;
       fy[k] /= dP;
       fz[k] /= dP;
     }
-    for (int i = 1; i <= 9; i++) pvt[i] /= dP;
+    if (iver)
+      for (size_t k = 0; k < (size_t)P * N; k++) {
+        fxt[k] /= dP;
+        fyt[k] /= dP;
+        fzt[k] /= dP;
+      }
+    for (int i = 1; i <= 9; i++) {
+      pvt[i] /= dP;
+      pvtt[i] /= dP;
+    }
     if (P > 1) {
       int nreal = N - c->nghost;
       double T = c->par.temperature > 0 ? c->par.temperature : 300.0;
-      double beta = OR_BOLTZ / T, akvirx = 3.0 * ((double)nreal) / beta;
-      double ptot[10] = {0};
+      double beta = OR_BOLTZ / T;
+      const int vir = full_intra && full;
+      double akvirx = vir ? 3.0 * ((double)nreal) / beta : 0.0;
       for (int ip = 0; ip < P; ip++) {
         size_t o = (size_t)ip * N;
         for (int i = 0; i < N; i++) {
@@ -1808,8 +2021,19 @@ int or_energy_control(or_ctx *c, double *x, double *y, double *z, const double *
           pvt[7] -= dz * gx;
           pvt[6] -= dy * gz;
           pvt[8] -= dz * gy;
-          akvirx -= (dx * gx + dy * gy + dz * gz);
-          (void)ptot;
+          if (vir) {
+            double hx = fxt[o + i], hy = fyt[o + i], hz = fzt[o + i];
+            akvirx -= (dx * hx + dy * hy + dz * hz);
+            pvtt[1] -= dx * hx;
+            pvtt[5] -= dy * hy;
+            pvtt[9] -= dz * hz;
+            pvtt[2] -= dx * hy;
+            pvtt[4] -= dy * hx;
+            pvtt[3] -= dx * hz;
+            pvtt[7] -= dz * hx;
+            pvtt[6] -= dy * hz;
+            pvtt[8] -= dz * hy;
+          }
         }
       }
       out->pi_ke_vir = 0.5 * akvirx;
@@ -1817,8 +2041,14 @@ int or_energy_control(or_ctx *c, double *x, double *y, double *z, const double *
   }
   for (int i = 1; i <= 9; i++) {
     out->pvten[i - 1] = pvt[i];
-    out->pvten_tot[i - 1] = pvt[i];
+    out->pvten_tot[i - 1] = pvtt[i];
   }
+  free(tx);
+  free(ty);
+  free(tz);
+  free(fxt);
+  free(fyt);
+  free(fzt);
   /* ghost forces (energy_control_final.c:174-181) */
   for (int ip = 0; ip < P; ip++)
     or_ghost_force(c, fx + (size_t)ip * N, fy + (size_t)ip * N, fz + (size_t)ip * N);

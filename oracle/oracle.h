@@ -34,6 +34,9 @@ int or_energy_control(or_ctx *c, double *x, double *y, double *z,
                       const double *zmod, unsigned flags, double *fx,
                       double *fy, double *fz, pmx_energies *out);
 
+/* RESPA settings (NULL: off), as pmx_set_respa. */
+int or_set_respa(or_ctx *c, const pmx_respa *r);
+
 int or_get_verlet_list(or_ctx *c, int *nter, int *jver_off, int *jter,
                        int cap, int *nver);
 int or_get_nktot(or_ctx *c, int *nktot, int *ngrid);

@@ -50,6 +50,7 @@ def lib():
                    "or_stag_mode_to_pos", "or_stag_force_to_mode"):
             getattr(L, nm).argtypes = [C.c_int, dp, dp]
         L.or_gauss_legendre.argtypes = [C.c_int, dp, dp]
+        L.or_set_respa.argtypes = [C.c_void_p, C.POINTER(abi.PmxRespa)]
         L.or_pimd_nvt.argtypes = [C.c_void_p, C.POINTER(abi.PmxNhcParams), C.c_int, dp, dp, dp, dp, dp, dp, dp,
                                   C.POINTER(abi.PmxEnergies), dp]
         L.or_bond_coef_morse.argtypes = [C.c_double, C.c_double, dp]
@@ -81,6 +82,13 @@ class Oracle:
         if getattr(self, "h", None):
             lib().or_destroy(self.h)
             self.h = None
+
+    def set_respa(self, respa):
+        """RESPA settings (an abi.PmxRespa, or None for off)."""
+        self._respa = respa
+        rc = lib().or_set_respa(self.h, C.byref(respa) if respa is not None else None)
+        if rc != 0:
+            raise RuntimeError(f"or_set_respa failed: {rc}")
 
     def energy_control(self, x, y, z, flags=None, xmod=None):
         P, N = self.P, self.N

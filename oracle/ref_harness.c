@@ -48,6 +48,15 @@
  *                  [bead][site][3], the centroid chain's v[len], x[len], and
  *                  for beads 2..P, sites and x, y, z the chain's v[len],
  *                  x[len] (zeros for ghost sites)
+ *   respa:         the force call with RESPA flags: argv[5], argv[6] and
+ *                  argv[7] are iget_full_inter, iget_res_inter and
+ *                  iget_full_intra; iget_res_intra is 1, as
+ *                  int_NVT_pimd.c:174-187 and the RESPA integrators of md/
+ *                  set it at the end of the innermost intramolecular step.  The weights follow from
+ *                  the input's respa_steps_* keywords
+ *                  (energy_control_initial.c:212-243); the real-space energy
+ *                  and pressure flags are left to the force call itself
+ *                  (force_npol.c:146-158).  out.bin as for a plain call.
  *   md:            P = 1; after the first force call the velocities
  *                  (N*3 doubles, Bohr per atomic time unit) are set and
  *                  int_NVE (md/int_NVE.c) runs nsteps times with the
@@ -84,7 +93,8 @@ int main(int argc, char *argv[]) {
   const int dump = argc == 6 && strcmp(argv[4], "dump") == 0;
   const int recip = argc == 5 && strcmp(argv[4], "recip") == 0;
   const int pimd = argc == 7 && strcmp(argv[4], "pimd") == 0;
-  if (argc != 4 && argc != 5 && !md && !dump && !pimd) {
+  const int respa = argc == 8 && strcmp(argv[4], "respa") == 0;
+  if (argc != 4 && argc != 5 && !md && !dump && !pimd && !respa) {
     fprintf(stderr, "usage: %s <input file> <positions.bin> <out.bin> [repeat | md nsteps velocities.bin]\n",
             argv[0]);
     return 2;
@@ -106,6 +116,13 @@ int main(int argc, char *argv[]) {
   /* real-space energy and pressure every call (test_energy.c:506-507) */
   cls.energy_ctrl.iget_pe_real_inter = 1;
   cls.energy_ctrl.iget_pv_real_inter = 1;
+  if (respa) {
+    cls.energy_ctrl.iget_full_inter = atoi(argv[5]);
+    cls.energy_ctrl.iget_res_inter = atoi(argv[6]);
+    cls.energy_ctrl.iget_full_intra = atoi(argv[7]);
+    cls.energy_ctrl.iget_res_intra = 1;
+    cls.energy_ctrl.itime = 1;
+  }
 
   const int N = cls.clatoms_info.natm_tot, P = cls.clatoms_info.pi_beads;
   FILE *f = fopen(argv[2], "rb");

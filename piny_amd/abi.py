@@ -11,6 +11,9 @@ PMX_EVAL_PIMD = 1 << 0
 PMX_EVAL_FORCE_LIST = 1 << 1
 PMX_EVAL_NO_RECIP = 1 << 2
 PMX_EVAL_MODE_FORCES = 1 << 3
+PMX_EVAL_RES_INTER = 1 << 4
+PMX_EVAL_NO_FULL_INTER = 1 << 5
+PMX_EVAL_NO_FULL_INTRA = 1 << 6
 PMX_PREC_FP64 = 0
 PMX_PREC_MIXED = 1
 PMX_PAIR_TABLE = 0
@@ -57,6 +60,33 @@ class PmxNhcParams(C.Structure):
 
 
 TIME_CONV = 0.0241888  # fs per atomic time unit (defines.h:85)
+PCONV = 3.3989242e-09  # atomic pressure units per atm (defines.h:83)
+
+
+class PmxRespa(C.Structure):
+    """pmx_respa: RESPA step counts (0: keyword left out) and the inputs of
+    the RESPA pressure-tensor terms (atomic units)."""
+    _fields_ = [
+        ("respa_steps_lrf", C.c_int), ("respa_steps_torsion", C.c_int), ("respa_steps_intra", C.c_int),
+        ("respa_steps_pimd", C.c_int), ("nfree", C.c_int), ("pext", C.c_double),
+        ("pten_inter_guess", C.c_double),
+    ]
+
+
+def respa_params(system, nfree=None):
+    """The pmx_respa of a system's settings (respa_steps_*, pressure and
+    pten_inter_respa in atm), or None when every RESPA keyword is left out.
+    nfree defaults to three per real (non-ghost) site, the count
+    close_intra_params.c:222-247 makes for molecules without constraints or
+    frozen atoms."""
+    st = system.settings
+    if not (st.respa_lrf or st.respa_torsion or st.respa_intra or st.respa_pimd):
+        return None
+    if nfree is None:
+        nfree = 3 * (system.natm - len(system.ghost_atom))
+    return PmxRespa(respa_steps_lrf=st.respa_lrf, respa_steps_torsion=st.respa_torsion,
+                    respa_steps_intra=st.respa_intra, respa_steps_pimd=st.respa_pimd, nfree=int(nfree),
+                    pext=st.pressure * PCONV, pten_inter_guess=st.pten_inter_respa * PCONV)
 
 
 def nhc_params(dt_fs=0.25, tau_fs=20.0, len_nhc=4, nres_nhc=4, nyosh_nhc=3, gamma_adb=1.0):

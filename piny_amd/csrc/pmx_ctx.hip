@@ -50,6 +50,9 @@ struct pmx_ctx {
   double *q = nullptr, *mass = nullptr;
   double2 *tab_lj = nullptr, *tab_c = nullptr, *ecor_tab = nullptr;
   double *pair_pp = nullptr;   // [npair][PP_NUM]
+  double *pair_pp_res = nullptr;  // the same with the RESPA cutoff as cutoff (short-range calls)
+  pmx_respa respa{};           // RESPA settings (zeros: off)
+  bool last_weighted = false;  // the last call's forces carried RESPA weights other than 1
   float4 *ctab = nullptr;      // mixed precision: erfc(alp r)/r as cubic pieces [CT_N]
   float4 *ftab = nullptr;      // mixed precision: the switched pair functions [nftab][CT_N] (null if too many)
   int nftab = 0;
@@ -426,6 +429,8 @@ std::vector<double> pair_constants(pmx_ctx *c) {
     const bool lj = (s.label[t] == PMX_INTER_LJ);
     e[PP_CUT] = s.cutoff[t];
     e[PP_CUT2] = s.cutoff[t] * s.cutoff[t];
+    e[PP_CUTRES] = s.cutoff_res[t];
+    e[PP_CULL] = s.cutoff[t];
     e[PP_RMIN] = s.rmin[t];
     e[PP_DRI] = s.dri[t];
     e[PP_DR] = s.dr[t];
@@ -450,6 +455,21 @@ std::vector<double> pair_constants(pmx_ctx *c) {
     }
   }
   return pp;
+}
+
+// The constants of the short-range RESPA calls: the RESPA cutoff as the
+// call's cutoff (CULL, CUT2), so that the pair kernels cull there, as
+// npol_shave_skin does for iget_full_inter = 0 (force_npol.c:1316-1327).
+// The functions keep their own cutoff (their switch); past the RESPA cutoff
+// the RESPA switch is 0, so whatever a kernel still evaluates there adds 0.
+std::vector<double> pair_constants_res(const pmx_ctx *c, const std::vector<double> &pp) {
+  std::vector<double> r = pp;
+  for (int t = 0; t < c->s.npair; t++) {
+    double *e = &r[(size_t)t * PP_NUM];
+    e[PP_CULL] = c->s.cutoff_res[t];
+    e[PP_CUT2] = c->s.cutoff_res[t] * c->s.cutoff_res[t];
+  }
+  return r;
 }
 
 // Molecules for k_intra_mol: the connected components of the bonds, bends,
@@ -941,6 +961,8 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
   {
     const std::vector<double> pp = pair_constants(c);
     TRY(upload(c, &c->pair_pp, pp.data(), pp.size()));
+    const std::vector<double> ppr = pair_constants_res(c, pp);
+    TRY(upload(c, &c->pair_pp_res, ppr.data(), ppr.size()));
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
     if (c->pair_mode == 2 && P > 1) TRY(dalloc(c, &c->disp, NP));
@@ -1298,6 +1320,7 @@ void add_stage(RedArgs &r, size_t off, int G, int nv, const std::vector<int> &sl
   st.off = off;
   st.G = G;
   st.nv = nv;
+  st.wv = 1.0;
   for (int v = 0; v < nv; v++) st.slot[v] = slots[v];
 }
 
@@ -1340,8 +1363,49 @@ IntraArgs intra_args(pmx_ctx *c, bool pimd) {
   a.e_rmin = s.e_rmin;
   a.e_dr = s.e_dr;
   a.e_dri = s.e_dri;
+  a.w_ecor = a.w_rec = a.w_intra = 1.0;
   return a;
 }
+
+bool respa_on(const pmx_ctx *c) {
+  const pmx_respa &r = c->respa;
+  return r.respa_steps_lrf > 0 || r.respa_steps_torsion > 0 || r.respa_steps_intra > 0 || r.respa_steps_pimd > 0;
+}
+
+// A call's RESPA reading: iget_full_inter, iget_res_inter and
+// iget_full_intra from the flags; iget_res_intra 1 when RESPA is on, as the
+// RESPA integrators make every call (int_NVT_pimd.c:174-187,
+// int_NVE_res.c:106-116), 0 otherwise; and the force weights of
+// energy_control_initial.c:212-243.
+struct RespaCall {
+  bool on, full, res, full_intra, res_intra, weighted;
+  double wt, wtr, wtrr;  // wght_ter, wght_ter_res, wght_tra_res
+};
+RespaCall respa_call(const pmx_ctx *c, unsigned flags) {
+  RespaCall r{};
+  r.on = respa_on(c);
+  r.full = !(flags & PMX_EVAL_NO_FULL_INTER);
+  r.res = (flags & PMX_EVAL_RES_INTER) != 0;
+  r.full_intra = !(flags & PMX_EVAL_NO_FULL_INTRA);
+  r.res_intra = r.on;
+  const pmx_respa &p = c->respa;
+  const int nter = std::max(p.respa_steps_lrf, 1), ntor = std::max(p.respa_steps_torsion, 1);
+  const int ntra = std::max(p.respa_steps_intra, 1), npimd = std::max(p.respa_steps_pimd, 1);
+  if (flags & PMX_EVAL_PIMD) {
+    r.wt = (double)(npimd * ntra * ntor * nter);
+    r.wtr = (double)(npimd * ntra * ntor);
+    r.wtrr = (double)npimd;
+  } else {
+    r.wt = (double)(ntra * ntor * nter);
+    r.wtr = (double)(ntra * ntor);
+    r.wtrr = 1.0;
+  }
+  r.weighted = r.wt != 1.0 || r.wtr != 1.0 || r.wtrr != 1.0;
+  return r;
+}
+
+// the molecule-cluster pair kernel runs for this call (not under RESPA)
+bool clu_active(const pmx_ctx *c) { return c->clu_on && !respa_on(c); }
 
 }  // namespace
 
@@ -1441,7 +1505,7 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
                      c->q, c->site);
   HIPCHK(hipGetLastError());
   c->disp_packed = false;
-  if (c->disp && !c->clu_on) {
+  if (c->disp && !clu_active(c)) {
     c->disp_packed = true;
     if (P <= 64) {
       const size_t waves = ((size_t)N + 64 / P - 1) / (64 / P);
@@ -1460,7 +1524,8 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
   }
   if (timing) HIPCHK(hipEventRecord(c->ev[1], st));
   const int nchrg = (int)s.ichrg.size();
-  const bool pme = c->pme && !(flags & PMX_EVAL_NO_RECIP);
+  const bool full = respa_call(c, flags).full;  // the k space only in full calls
+  const bool pme = c->pme && !(flags & PMX_EVAL_NO_RECIP) && full;
   auto launch_pme = [&](hipStream_t sk) -> int {
     PmeArgs a = c->pa;
     a.x = c->x;
@@ -1526,7 +1591,7 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
   }
   if (pme)
     if (int rc = launch_pme(st)) return rc;
-  if (c->ewald && !(flags & PMX_EVAL_NO_RECIP)) {
+  if (c->ewald && !(flags & PMX_EVAL_NO_RECIP) && full) {
     EwaldArgs a{};
     a.N = N;
     a.P = P;
@@ -1560,12 +1625,21 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
   const size_t NP = (size_t)N * P;
   const bool pimd = (flags & PMX_EVAL_PIMD) != 0;
   const int nchrg = (int)s.ichrg.size();
-  const bool kspace = (c->pme || c->ewald) && !(flags & PMX_EVAL_NO_RECIP);
+  const RespaCall rsp = respa_call(c, flags);
+  const bool kspace = (c->pme || c->ewald) && !(flags & PMX_EVAL_NO_RECIP) && rsp.full;
   RedArgs red{};
   if (timing) HIPCHK(hipEventRecord(c->ev[3], st));
-  // --- real space (force_control -> force_verlst -> force_npol)
-  c->clu_used = c->clu_on;
-  if (c->clu_on) {
+  // --- real space (force_control -> force_verlst -> force_npol); none
+  // when neither the full nor the short-range inter forces are asked for
+  // (energy_control_inter_real.c:112)
+  c->clu_used = clu_active(c);
+  if (!rsp.full && !rsp.res) {
+    if (c->full_list)  // the pair kernel would have been the forces' first writer
+      for (double *f : {c->fx, c->fy, c->fz}) {
+        hipLaunchKernelGGL(k_fill_double, dim3(blocks(NP)), dim3(256), 0, st, f, 0.0, NP);
+        HIPCHK(hipGetLastError());
+      }
+  } else if (c->clu_used) {
     // the molecule-cluster form of the mixed pair kernel
     CluArgs a = clu_args(c);
     HIPCHK(launch_clu_pack(a, st));
@@ -1608,6 +1682,12 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.irheal = c->par.rheal > 0.0 ? 1.0 / c->par.rheal : 0.0;
     a.swit = (c->par.ishift == PMX_SHIFT_SWIT);
     a.pp = c->pair_pp;
+    if (rsp.on) {
+      a.respa = rsp.res ? (rsp.full ? RESPA_BOTH : RESPA_RES) : RESPA_FULL;
+      a.w_full = rsp.wt;
+      a.w_res = rsp.wtr;
+      if (rsp.res && !rsp.full) a.pp = c->pair_pp_res;  // cut at the RESPA cutoff
+    }
     a.x = c->x;
     a.y = c->y;
     a.z = c->z;
@@ -1632,8 +1712,16 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
   if (kspace) {
     const int G = c->pme ? s.ngrid[1] : (s.nktot + EWALD_BLOCK - 1) / EWALD_BLOCK;
     add_stage(red, c->off_fft, G, FFT_NV, {ACC_VRECIP, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23});
+    red.st[red.nstage - 1].wv = rsp.wt;  // ewald3d_pme.c:420-422
   }
   IntraArgs ia = intra_args(c, pimd);
+  // RESPA weights: the excluded-pair terms and the k-space forces by
+  // wght_ter in full calls only (energy_control_inter_recip.c:278-312), the
+  // bonds and bends by wght_tra_res when the intramolecular terms are
+  // asked for (energy_control_intra.c:124-230)
+  ia.w_ecor = ia.w_rec = rsp.wt;
+  ia.w_intra = (rsp.full_intra || rsp.res_intra) ? rsp.wtrr : 0.0;
+  if (!rsp.full) ia.necor = 0;
   const int bsP = bead_block(P);
   auto igrid = [&](size_t items) { return (unsigned)std::min<size_t>(c->g_intra, (items + bsP - 1) / bsP); };
   const std::initializer_list<int> intra_slots = {0, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23};
@@ -1641,6 +1729,7 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     std::vector<int> sl(intra_slots);
     sl[0] = eslot;
     add_stage(red, off, (int)G, INTRA_NV, sl);
+    red.st[red.nstage - 1].wv = eslot == ACC_VECOR ? ia.w_ecor : ia.w_intra;
   };
   const double *frec = kspace ? c->frec : nullptr;
   if (c->nmol > 0) {
@@ -1710,6 +1799,7 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
       a.fy = c->fy;
       a.fz = c->fz;
       a.rec = frec;
+      a.w_rec = ia.w_rec;
       a.part = c->part + c->off_fin;
       const unsigned G = igrid(NP);
       hipLaunchKernelGGL(k_pimd_finish, dim3(G), dim3(bsP), 0, st, a);
@@ -1717,7 +1807,8 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
       add_stage(red, c->off_fin, (int)G, FIN_NV,
                 {ACC_C11, ACC_C22, ACC_C33, ACC_C12, ACC_C21, ACC_C13, ACC_C31, ACC_C23, ACC_C32, ACC_AKVIR});
     } else if (frec) {
-      hipLaunchKernelGGL(k_add_rec, dim3(blocks(NP)), dim3(256), 0, st, N, P, frec, c->fx, c->fy, c->fz);
+      hipLaunchKernelGGL(k_add_rec, dim3(blocks(NP)), dim3(256), 0, st, N, P, frec, ia.w_rec, c->fx, c->fy,
+                         c->fz);
       HIPCHK(hipGetLastError());
     }
     if (c->nghost) {
@@ -1805,17 +1896,23 @@ void assemble(pmx_ctx *c, unsigned flags, pmx_energies *out, bool timing) {
   }
   // --- assemble (energy_control_inter_real.c:188-197, _recip.c:340-347, _intra.c)
   const double dP = (double)P;
+  const RespaCall rsp = respa_call(c, flags);
   double sum[ACC_NUM] = {0};
   for (int b = 0; b < P; b++)
     for (int k = 0; k < ACC_NUM; k++) sum[k] += c->acc_h[(size_t)b * ACC_NUM + k];
-  out->vvdw = sum[ACC_VVDW] / dP;
-  out->vcoul = sum[ACC_VCOUL] / dP;
+  // energies the reference keeps: the intermolecular ones in full calls
+  // (force_npol.c:243-256), the intramolecular ones when the intra or the
+  // inter terms are full (energy_control_intra.c:284)
+  const bool keep_e = rsp.full_intra || rsp.full;
+  const bool intra = rsp.full_intra || rsp.res_intra;
+  out->vvdw = rsp.full ? sum[ACC_VVDW] / dP : 0.0;
+  out->vcoul = rsp.full ? sum[ACC_VCOUL] / dP : 0.0;
   out->vreal = out->vvdw + out->vcoul;
-  out->vlong = s.clong / s.vol;
+  out->vlong = rsp.full ? s.clong / s.vol : 0.0;
   out->vrecip = sum[ACC_VRECIP] / dP;
   out->vecor = sum[ACC_VECOR] / dP;
-  out->vbond = sum[ACC_VBOND] / dP;
-  out->vbend = sum[ACC_VBEND] / dP;
+  out->vbond = keep_e && intra ? sum[ACC_VBOND] / dP : 0.0;
+  out->vbend = keep_e && intra ? sum[ACC_VBEND] / dP : 0.0;
   double pv[10] = {0};
   pv[1] = sum[ACC_P11];
   pv[5] = sum[ACC_P22];
@@ -1824,22 +1921,50 @@ void assemble(pmx_ctx *c, unsigned flags, pmx_energies *out, bool timing) {
   pv[3] = pv[7] = sum[ACC_P13];
   pv[6] = pv[8] = sum[ACC_P23];
   {
-    const double amult = (c->par.ishift == PMX_SHIFT_SWIT) ? 1.0 : 2.0;
-    const double pd = (s.clong / s.vol) * amult * dP;
-    pv[1] += pd;
-    pv[5] += pd;
-    pv[9] += pd;
+    // long_range_corr with its RESPA terms (energy_control_inter_real.c:
+    // 206-318): the bead, intramolecular and short-range steps' guesses and
+    // the dispersion tail, each as the difference to the previous level
+    // times that level's weight
+    const double amult = (c->par.ishift == PMX_SHIFT_SWIT) ? 1.0 : 2.0, amult_bead = amult * dP;
+    const double T = c->par.temperature > 0 ? c->par.temperature : 300.0;
+    double pd_bead = 0.0, pd_tra, pd_ter, d = 0.0;
+    if (P > 1 && rsp.wtrr != 1.0) {
+      const double pkin = (double)(c->respa.nfree / 3) * T / (BOLTZ * s.vol);
+      pd_bead = (c->respa.pext - pkin) * s.vol * dP;
+      d += pd_bead;
+    }
+    if (rsp.res_intra && rsp.wt != 1.0) {
+      pd_tra = c->respa.pten_inter_guess * s.vol * dP;
+      d += (pd_tra - pd_bead) * rsp.wtrr;
+    } else {
+      pd_tra = pd_bead;
+    }
+    if (rsp.res) {
+      pd_ter = (s.clong_res / s.vol) * amult_bead;
+      d += (pd_ter - pd_tra) * rsp.wtr;
+    } else {
+      pd_ter = pd_tra;
+    }
+    if (rsp.full) d += ((s.clong / s.vol) * amult_bead - pd_ter) * rsp.wt;
+    pv[1] += d;
+    pv[5] += d;
+    pv[9] += d;
   }
-  if (nchrg > 0) {
-    // ewald3d_selfbgr (ewald3d_self_bgr.c:29-83) over all sites
+  out->vself = out->vbgr = 0.0;
+  if (nchrg > 0 && (rsp.full_intra || !rsp.res_intra)) {
+    // ewald3d_selfbgr (ewald3d_self_bgr.c:29-83) over all sites, its
+    // pressure times wght_tra_res
     const double qq = s.qsq, qs = s.qsum;
-    out->vself = -qq * (s.alp_ewd) * (s.self_erf) / sqrt(M_PI);
-    out->vbgr = -0.5 * qs * qs * M_PI / (s.alp_ewd * s.alp_ewd * s.vol);
-    pv[1] += out->vbgr * dP;
-    pv[5] += out->vbgr * dP;
-    pv[9] += out->vbgr * dP;
+    const double bgr = -0.5 * qs * qs * M_PI / (s.alp_ewd * s.alp_ewd * s.vol);
+    if (keep_e) {
+      out->vself = -qq * (s.alp_ewd) * (s.self_erf) / sqrt(M_PI);
+      out->vbgr = bgr;
+    }
+    pv[1] += bgr * rsp.wtrr * dP;
+    pv[5] += bgr * rsp.wtrr * dP;
+    pv[9] += bgr * rsp.wtrr * dP;
   }
-  out->vintert = out->vreal + out->vlong + out->vrecip + out->vself + out->vbgr + out->vecor;
+  out->vintert = rsp.full ? out->vreal + out->vlong + out->vrecip + out->vself + out->vbgr + out->vecor : 0.0;
   out->vintrat = out->vbond + out->vbend;
   if (pimd) {
     for (int k = 1; k <= 9; k++) pv[k] /= dP;
@@ -1855,13 +1980,20 @@ void assemble(pmx_ctx *c, unsigned flags, pmx_energies *out, bool timing) {
       pv[8] += sum[ACC_C32];
       const double T = c->par.temperature > 0 ? c->par.temperature : 300.0;
       const double beta = BOLTZ / T;
-      out->pi_ke_vir = 0.5 * (3.0 * ((double)(N - c->nghost)) / beta + sum[ACC_AKVIR]);
+      // the virial estimator needs the unweighted forces (fxt,
+      // pimd_estimators.c:259-303): under RESPA weights it is not formed
+      out->pi_ke_vir = !(rsp.full_intra && rsp.full) ? 0.0
+                       : rsp.weighted ? NAN
+                                      : 0.5 * (3.0 * ((double)(N - c->nghost)) / beta + sum[ACC_AKVIR]);
     }
   }
+  // pvten_tot: the unweighted tensor of full calls (zero in the others);
+  // without weights it is pvten
   for (int k = 0; k < 9; k++) {
     out->pvten[k] = pv[k + 1];
-    out->pvten_tot[k] = pv[k + 1];
+    out->pvten_tot[k] = !rsp.full ? 0.0 : rsp.weighted ? NAN : pv[k + 1];
   }
+  c->last_weighted = rsp.weighted || !rsp.full;
   if (modes) {
     const double T = c->par.temperature > 0 ? c->par.temperature : 300.0;
     const double tau = (BOLTZ / T) / dP;
@@ -2043,6 +2175,34 @@ int pmx_set_xmod(pmx_ctx *c, const double *x, const double *y, const double *z) 
   HIPCHK(hipMemcpy(c->zm, z, n, hipMemcpyHostToDevice));
   c->xmod_given = true;
   c->grad_ready = false;
+  return PMX_OK;
+}
+
+int pmx_set_respa(pmx_ctx *c, const pmx_respa *r) {
+  if (!c) return PMX_ERR_ARG;
+  if (!r) {
+    c->respa = pmx_respa{};
+    c->epoch++;  // captured batch sequences hold the weights
+    return PMX_OK;
+  }
+  if (r->respa_steps_lrf < 0 || r->respa_steps_torsion < 0 || r->respa_steps_intra < 0 || r->respa_steps_pimd < 0 ||
+      r->nfree < 0) {
+    set_err(c, "pmx_set_respa: negative step count or degrees of freedom");
+    return PMX_ERR_ARG;
+  }
+  // set_sim_params.c:3346-3370
+  if ((r->respa_steps_lrf > 0 && (r->respa_steps_intra == 0 || r->respa_steps_torsion == 0)) ||
+      (r->respa_steps_torsion > 0 && r->respa_steps_intra == 0)) {
+    set_err(c, "pmx_set_respa: the inter split needs the torsion and intramolecular splits, and the torsion "
+               "split the intramolecular one (respa_steps_intra, respa_steps_torsion > 0)");
+    return PMX_ERR_ARG;
+  }
+  if (r->respa_steps_lrf > 0 && !(c->par.rheal > 0.0)) {
+    set_err(c, "pmx_set_respa: the inter split needs respa_rheal > 0");
+    return PMX_ERR_ARG;
+  }
+  c->respa = *r;
+  c->epoch++;
   return PMX_OK;
 }
 
@@ -2485,6 +2645,11 @@ int pmx_get_fxt(pmx_ctx *c, double *fx, double *fy, double *fz) {
   // virial-estimator forces are the bead forces; after a call that returned
   // mode forces they are the bead forces kept before the transform.
   if (!c || !fx || !fy || !fz) return PMX_ERR_ARG;
+  if (c->last_weighted) {
+    set_err(c, "pmx_get_fxt: the last call was not a full call without RESPA weights; the reference's fxt "
+               "holds the unweighted forces, which are not kept apart");
+    return PMX_ERR_STATE;
+  }
   if (c->last_modes) {
     double *src[3] = {c->fxt, c->fyt, c->fzt};
     return get_bead_major(c, src, fx, fy, fz);
@@ -2534,6 +2699,10 @@ int pmx_md_nve(pmx_ctx *c, int nsteps, double dt, pmx_energies *out, double *kin
   if (s.P != 1) {
     set_err(c, "pmx_md_nve: the device integrator is classical (P = 1); use the force call for beads");
     return PMX_ERR_NOTIMPL;
+  }
+  if (respa_on(c)) {
+    set_err(c, "pmx_md_nve: int_NVE runs without RESPA; pmx_set_respa(ctx, NULL) first");
+    return PMX_ERR_STATE;
   }
   if (!c->have_pos) {
     set_err(c, "pmx_md_nve: no positions set");
@@ -2732,6 +2901,11 @@ int pmx_pimd_nvt(pmx_ctx *c, const pmx_nhc_params *np, int nsteps, pmx_energies 
     set_err(c, "pmx_pimd_nvt: one bead: use pmx_md_nve");
     return PMX_ERR_ARG;
   }
+  if (respa_on(c)) {
+    set_err(c, "pmx_pimd_nvt: the device integrator runs int_NVT_pimd with RESPA off; pmx_set_respa(ctx, NULL) "
+               "first");
+    return PMX_ERR_STATE;
+  }
   if (!c->have_pos) {
     set_err(c, "pmx_pimd_nvt: no positions set");
     return PMX_ERR_STATE;
@@ -2905,6 +3079,8 @@ int pmx_set_force_field(pmx_ctx *c, const pmx_system *sys) {
   }
   const std::vector<double> pp = pair_constants(c);
   HIPCHK(hipMemcpy(c->pair_pp, pp.data(), pp.size() * sizeof(double), hipMemcpyHostToDevice));
+  const std::vector<double> ppr = pair_constants_res(c, pp);
+  HIPCHK(hipMemcpy(c->pair_pp_res, ppr.data(), ppr.size() * sizeof(double), hipMemcpyHostToDevice));
   if (c->pair_mode == 2)
     if (int rc = build_pair_tables(c)) return rc;
   if (int rc = setup_cluster(c, sys)) return rc;  // same charged sites: the list and its rows stay
@@ -3008,6 +3184,7 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
     red.st[0].off = 0;
     red.st[0].G = (int)gx;
     red.st[0].nv = PAIRG_NV;
+    red.st[0].wv = 1.0;
     for (int v = 0; v < PAIRG_NV; v++) red.st[0].slot[v] = v;
     red.part = c->gpart;
     red.acc = c->gacc;

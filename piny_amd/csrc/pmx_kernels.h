@@ -39,12 +39,20 @@ struct ListArgs {
 // For the parameter gradients: the LJ shift per unit eps (VSLJ1) and
 // sigma d/dsigma of it (DSLJ1), and the type pair's LJ gradient slot (GSLOT,
 // -1 when the pair has no LJ term).  C12 = 4 eps s^12 and C6 = 4 eps s^6
-// give the FP32 kernels the LJ term as r^-6 (C12 r^-6 - C6); CUT2 is the
-// squared cutoff.
+// give the FP32 kernels the LJ term as r^-6 (C12 r^-6 - C6).  CULL and CUT2
+// are the cutoff of the call, where the kernels cull, and its square (the
+// RESPA cutoff in short-range RESPA calls, else CUT); CUT stays the
+// functions' cutoff, where their switch ends; CUTRES is the type pair's
+// RESPA cutoff (inter_cutoff_res).
 enum {
   PP_CUT = 0, PP_RMIN, PP_DRI, PP_DR, PP_EPS, PP_SIG6, PP_VSLJ, PP_VSC, PP_LJ,
-  PP_VSLJ1, PP_DSLJ1, PP_GSLOT, PP_C12, PP_C6, PP_CUT2, PP_NUM
+  PP_VSLJ1, PP_DSLJ1, PP_GSLOT, PP_C12, PP_C6, PP_CUT2, PP_CUTRES, PP_CULL, PP_NUM
 };
+// RESPA weighting of the pair forces (force_npol.c:807-816,1033-1060):
+// off; the short-range part alone (switch times w_res); full and short-range
+// (switch times w_res plus its complement times w_full); full alone, times
+// w_full.
+enum { RESPA_OFF = 0, RESPA_RES = 1, RESPA_BOTH = 2, RESPA_FULL = 3 };
 // LJ type pairs whose eps/sigma gradients one context forms
 constexpr int MAX_LJ_GRAD = 4;
 constexpr int CT_N = 1024;  // intervals of the mixed kernel's tables (16 KB of LDS each)
@@ -72,6 +80,10 @@ struct PairArgs {
   int nftab;
   int2 ftoff[MAX_PAIRS];
   int swit;
+  // RESPA: one of RESPA_*, the weights of the full and the short-range
+  // forces (the switch runs over rheal below PP_CUTRES)
+  int respa;
+  double w_full, w_res;
   const double *pp;            // [npair][PP_NUM]
   const double *x, *y, *z;     // [N*P]
   const double4 *site;         // [N] (xmod, ymod, zmod, q)
@@ -198,6 +210,10 @@ struct IntraArgs {
   const struct MolTpl *tpl;
   const double *rec;
   double *part_ecor, *part_bond, *part_bend, *part_fin;
+  // RESPA force weights (energy_control_initial.c:212-243) of the excluded-
+  // pair terms and the k-space forces (wght_ter) and of the bonds and bends
+  // (wght_tra_res); 1 without RESPA
+  double w_ecor, w_rec, w_intra;
 };
 constexpr int MOL_MAX = 8;   // sites of a molecule of k_intra_mol
 constexpr int TPL_MAXT = 16;  // terms of one kind in a molecule template
@@ -226,6 +242,7 @@ struct FinishArgs {
   const double *x, *y, *z, *xm, *ym, *zm;
   double *fx, *fy, *fz;
   const double *rec;  // PME forces to add, bead-major [3][P][N], or null
+  double w_rec;       // their weight (1 without RESPA)
   double *part;  // [gridDim.x][P][FIN_NV]
 };
 
@@ -291,6 +308,7 @@ struct RedStage {
   size_t off;
   int G, nv;
   int slot[12];
+  double wv;  // weight of values 1 .. nv-1 (the virial of a RESPA-weighted stage), 1 otherwise
 };
 constexpr int MAX_RED_STAGES = 8;
 struct RedArgs {
@@ -348,7 +366,7 @@ __global__ void k_bond(IntraArgs a);
 __global__ void k_bend(IntraArgs a);
 __global__ void k_intra_mol(IntraArgs a);
 __global__ void k_pimd_finish(FinishArgs a);
-__global__ void k_add_rec(int N, int P, const double *rec, double *fx, double *fy, double *fz);
+__global__ void k_add_rec(int N, int P, const double *rec, double w, double *fx, double *fy, double *fz);
 __global__ void k_ghost_force(int ng, int P, const int *gatom, const int *gncomp, const int *gcomp,
                               const double *gcoef, double *fx, double *fy, double *fz);
 hipError_t launch_mode_forces(const ModeArgs &a, hipStream_t st);

@@ -48,14 +48,16 @@ __global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {
     for (int v = 0; v < r.nv; v++) {
       if (r.slot[v] != slot) continue;
       const double *row = a.part + r.off + ((size_t)v * a.P + b) * r.G;
+      // a weight of 1 leaves every sum as it is (the products are exact)
+      const double w = v == 0 ? 1.0 : r.wv;
       int g = lane;
       for (; g + 192 < r.G; g += 256) {
-        s0 += row[g];
-        s1 += row[g + 64];
-        s2 += row[g + 128];
-        s3 += row[g + 192];
+        s0 += w * row[g];
+        s1 += w * row[g + 64];
+        s2 += w * row[g + 128];
+        s3 += w * row[g + 192];
       }
-      for (; g < r.G; g += 64) s0 += row[g];
+      for (; g < r.G; g += 64) s0 += w * row[g];
     }
   }
   double s = (s0 + s1) + (s2 + s3);
@@ -837,6 +839,19 @@ struct PairAcc {
 // One pair term for one bead: separation (dx, dy, dz) and squared distance
 // r2d (FP64), type pair constants pp; returns the scalar -(1/r) dV/dr and
 // adds the energies, the i force and the virial.
+// The RESPA weight of a pair force at distance r (npol_spldata_fetch and
+// npol_force_calc, force_npol.c:807-816,1029-1060): the switch
+// 1 + b^2 (2b - 3), b = (r - cutoff_res + rheal) / rheal clamped to [0, 1],
+// is 1 well inside the RESPA cutoff and 0 beyond it.
+__device__ __forceinline__ double respa_weight(const PairArgs &a, double cres, double r) {
+  if (a.respa == RESPA_FULL) return a.w_full;
+  double br = (r - cres + a.rheal) * a.irheal;
+  br = fmax(br, 0.0);
+  br = fmin(br, 1.0);
+  const double sw = br * br * (br * 2.0 - 3.0) + 1.0;
+  return a.respa == RESPA_RES ? sw * a.w_res : sw * a.w_res + (1.0 - sw) * a.w_full;
+}
+
 template <int MODE>
 __device__ __forceinline__ bool pair_term(const PairArgs &a, const double *pp, int t, double qq, double dx,
                                           double dy, double dz, double r2d, PairAcc &s, double &dvd) {
@@ -878,6 +893,7 @@ __device__ __forceinline__ bool pair_term(const PairArgs &a, const double *pp, i
       s.vc += ev_c;
     }
   }
+  if (a.respa) dvd *= respa_weight(a, pp[PP_CUTRES], sqrt(r2d));  // uniform over the launch
   const double fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
   s.fx += fx1;
   s.fy += fy1;
@@ -1035,18 +1051,19 @@ __device__ __forceinline__ void wave_sync() {
 // switching shell.  Without TAB (more distinct functions than MAX_FTAB
 // tables) pair_direct_f32 evaluates LJ and the switch directly with the
 // erfc table.
-template <int B, bool PIMD, bool TAB>
+template <int B, bool PIMD, bool TAB, bool RESPA>
 __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   __shared__ float4 sppA[MAX_PAIRS], sppB[MAX_PAIRS];  // pair_direct_f32's constants per type pair
-  __shared__ int4 stt[MAX_PAIRS];                       // TAB: (cut^2 as bits, erfc table, LJ table or -1, 0)
+  __shared__ int4 stt[MAX_PAIRS];  // (cut^2 as bits, TAB: erfc table, LJ table or -1, culling radius as bits)
   extern __shared__ float4 sct[];                       // the tables: TAB nftab * CT_N, else CT_N pieces
   __shared__ float4 sdq[4][64];
   __shared__ int2 sent[4][64];  // (j * P, type pair) of the staged entries
   for (int t = threadIdx.x; t < a.npair; t += blockDim.x) {
     const double *pp = a.pp + (size_t)t * PP_NUM;
     sppA[t] = make_float4((float)pp[PP_CUT2], (float)pp[PP_CUT], (float)pp[PP_C12], (float)pp[PP_C6]);
-    sppB[t] = make_float4((float)pp[PP_VSLJ], (float)pp[PP_VSC], (float)pp[PP_LJ], 0.f);
-    stt[t] = make_int4(__float_as_int((float)pp[PP_CUT2]), a.ftoff[t].x, a.ftoff[t].y, 0);
+    sppB[t] = make_float4((float)pp[PP_VSLJ], (float)pp[PP_VSC], (float)pp[PP_LJ], (float)pp[PP_CUTRES]);
+    stt[t] = make_int4(__float_as_int((float)pp[PP_CUT2]), a.ftoff[t].x, a.ftoff[t].y,
+                       __float_as_int((float)pp[PP_CULL]));
   }
   const int ntab = TAB ? a.nftab * CT_N : CT_N;
   const float4 *src = TAB ? a.ftab : a.ctab;
@@ -1093,7 +1110,8 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
                         dy, dz);
         }
         const double lim =
-            ((double)sppA[e >> ENTRY_SHIFT].y + (PIMD ? a.umax[i] + a.umax[j] : 0.0)) * (1.0 + 1e-5);
+            ((double)__int_as_float(stt[e >> ENTRY_SHIFT].w) + (PIMD ? a.umax[i] + a.umax[j] : 0.0)) *
+            (1.0 + 1e-5);
         keep = dx * dx + dy * dy + dz * dz < lim * lim;
         q2 = si.w * sj.w;
       }
@@ -1149,6 +1167,16 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           if (!bact || !(r2 < A.x)) return;
           const float rinv = __frsqrt_rn(r2);
           dvd = pair_direct_f32(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c, sct);
+          if constexpr (RESPA) {  // respa_weight in FP32; B2.w is the RESPA cutoff
+            float fm = (float)a.w_full;
+            if (a.respa != RESPA_FULL) {
+              float br = (r2 * rinv - B2.w + (float)a.rheal) * (float)a.irheal;
+              br = fminf(fmaxf(br, 0.f), 1.f);
+              const float sw = br * br * (br * 2.f - 3.f) + 1.f;
+              fm = a.respa == RESPA_RES ? sw * (float)a.w_res : sw * (float)a.w_res + (1.f - sw) * (float)a.w_full;
+            }
+            dvd *= fm;
+          }
         }
         const float fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
         cvlj += ev_lj;
@@ -1282,7 +1310,7 @@ __device__ __forceinline__ void pair_staged(PairArgs a) {
           dx += si.x - sj.x;
           dy += si.y - sj.y;
           dz += si.z - sj.z;
-          lim = (spp[(e >> ENTRY_SHIFT) * PP_NUM + PP_CUT] + a.umax[i] + a.umax[j]) * (1.0 + 1e-9);
+          lim = (spp[(e >> ENTRY_SHIFT) * PP_NUM + PP_CULL] + a.umax[i] + a.umax[j]) * (1.0 + 1e-9);
           keep = dx * dx + dy * dy + dz * dz < lim * lim;
           dx += sj.x;  // F = D + xmod_j
           dy += sj.y;
@@ -1290,7 +1318,7 @@ __device__ __forceinline__ void pair_staged(PairArgs a) {
         } else {
           min_image_sep(c, 0, a.x[i], a.y[i], a.z[i], a.x[j], a.y[j], a.z[j], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dx,
                         dy, dz);
-          lim = spp[(e >> ENTRY_SHIFT) * PP_NUM + PP_CUT] * (1.0 + 1e-9);
+          lim = spp[(e >> ENTRY_SHIFT) * PP_NUM + PP_CULL] * (1.0 + 1e-9);
           keep = dx * dx + dy * dy + dz * dz < lim * lim;
         }
         q2 = si.w * sj.w;
@@ -1625,20 +1653,23 @@ __device__ __forceinline__ void bend_term(const IntraArgs &a, int i1, int i2, in
 
 // Per-term kernels (topologies that do not split into small molecules):
 // one thread per (term, bead), forces by global atomics.
-#define PMX_TERM_KERNEL(name, nterm, CALL)                                                    \
+// The forces are added times the term's RESPA weight W (the virial rows
+// stay unweighted; the reduction weighs them).
+#define PMX_TERM_KERNEL(name, nterm, W, CALL)                                                 \
   __global__ void __launch_bounds__(256) name(IntraArgs a) {                                   \
     double acc[INTRA_NV] = {0, 0, 0, 0, 0, 0, 0};                                              \
+    const double w = W;                                                                        \
     PMX_TERM_LOOP(nterm) {                                                                     \
       const int k = (int)(t / a.P), b = (int)(t % a.P);                                        \
       auto sep = [&](int i, int j, double &dx, double &dy, double &dz) { intra_sep(a, i, j, b, dx, dy, dz); }; \
-      auto add = [&](int i, double fx, double fy, double fz) { add_force(a, i, b, fx, fy, fz); }; \
+      auto add = [&](int i, double fx, double fy, double fz) { add_force(a, i, b, fx * w, fy * w, fz * w); }; \
       CALL;                                                                                    \
     }                                                                                          \
     block_partials<INTRA_NV>(acc, a.P, 0, a.P, a.part, blockIdx.x, gridDim.x);                 \
   }
-PMX_TERM_KERNEL(k_ecor, a.necor, ecor_term(a, a.e1[k], a.e2[k], a.q[a.e1[k]] * a.q[a.e2[k]], acc, sep, add))
-PMX_TERM_KERNEL(k_bond, a.nbond, bond_term(a, a.b1[k], a.b2[k], a.btyp[k], acc, sep, add))
-PMX_TERM_KERNEL(k_bend, a.nbend, bend_term(a, a.a1[k], a.a2[k], a.a3[k], a.atyp[k], acc, sep, add))
+PMX_TERM_KERNEL(k_ecor, a.necor, a.w_ecor, ecor_term(a, a.e1[k], a.e2[k], a.q[a.e1[k]] * a.q[a.e2[k]], acc, sep, add))
+PMX_TERM_KERNEL(k_bond, a.nbond, a.w_intra, bond_term(a, a.b1[k], a.b2[k], a.btyp[k], acc, sep, add))
+PMX_TERM_KERNEL(k_bend, a.nbend, a.w_intra, bend_term(a, a.a1[k], a.a2[k], a.a3[k], a.atyp[k], acc, sep, add))
 #undef PMX_TERM_KERNEL
 
 // Every intramolecular term of a force call in one pass, for topologies
@@ -1703,19 +1734,24 @@ __global__ void __launch_bounds__(256) k_intra_mol(IntraArgs a) {
       dy = pl[(3 * si + 1) * bs + tid] - pl[(3 * sj + 1) * bs + tid];
       dz = pl[(3 * si + 2) * bs + tid] - pl[(3 * sj + 2) * bs + tid];
     };
+    // each term's forces times its RESPA weight (1 without RESPA: exact)
+    auto add_e = [&](int i, double fx, double fy, double fz) { add(i, fx * a.w_ecor, fy * a.w_ecor, fz * a.w_ecor); };
+    auto add_i = [&](int i, double fx, double fy, double fz) {
+      add(i, fx * a.w_intra, fy * a.w_intra, fz * a.w_intra);
+    };
     if (a.necor > 0)
-      for (int q = 0; q < T.ne; q++) ecor_term(a, lo + T.e[q][0], lo + T.e[q][1], T.eq[q], ae, sep, add);
-    for (int q = 0; q < T.nb; q++) bond_term(a, lo + T.bd[q][0], lo + T.bd[q][1], T.bd[q][2], ab, sep, add);
+      for (int q = 0; q < T.ne; q++) ecor_term(a, lo + T.e[q][0], lo + T.e[q][1], T.eq[q], ae, sep, add_e);
+    for (int q = 0; q < T.nb; q++) bond_term(a, lo + T.bd[q][0], lo + T.bd[q][1], T.bd[q][2], ab, sep, add_i);
     for (int q = 0; q < T.na; q++)
-      bend_term(a, lo + T.an[q][0], lo + T.an[q][1], lo + T.an[q][2], T.an[q][3], aa, sep, add);
+      bend_term(a, lo + T.an[q][0], lo + T.an[q][1], lo + T.an[q][2], T.an[q][3], aa, sep, add_i);
     for (int s = 0; s < n; s++) {
       const int i = lo + s;
       double fx = fl[(3 * s) * bs + tid], fy = fl[(3 * s + 1) * bs + tid], fz = fl[(3 * s + 2) * bs + tid];
       if (a.rec) {
         const size_t r = (size_t)b * a.N + i;
-        fx += a.rec[r];
-        fy += a.rec[NP + r];
-        fz += a.rec[2 * NP + r];
+        fx += a.rec[r] * a.w_rec;
+        fy += a.rec[NP + r] * a.w_rec;
+        fz += a.rec[2 * NP + r] * a.w_rec;
       }
       if (a.pimd) {
         fx /= dP;
@@ -1888,9 +1924,9 @@ __global__ void __launch_bounds__(256) k_pimd_finish(FinishArgs a) {
     double fx = a.fx[t], fy = a.fy[t], fz = a.fz[t];
     if (a.rec) {
       const size_t r = (t - (size_t)i * a.P) * a.N + i;
-      fx += a.rec[r];
-      fy += a.rec[NP + r];
-      fz += a.rec[2 * NP + r];
+      fx += a.rec[r] * a.w_rec;
+      fy += a.rec[NP + r] * a.w_rec;
+      fz += a.rec[2 * NP + r] * a.w_rec;
     }
     fx /= dP;
     fy /= dP;
@@ -1917,13 +1953,14 @@ __global__ void __launch_bounds__(256) k_pimd_finish(FinishArgs a) {
 
 // The PME forces (bead-major [3][P][N]) added to the site-major forces,
 // outside PIMD.
-__global__ void k_add_rec(int N, int P, const double *__restrict__ rec, double *fx, double *fy, double *fz) {
+__global__ void k_add_rec(int N, int P, const double *__restrict__ rec, double w, double *fx, double *fy,
+                          double *fz) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, NP = (size_t)N * P;
   if (t >= NP) return;
   const size_t i = t / P, r = (t - i * P) * N + i;
-  fx[t] += rec[r];
-  fy[t] += rec[NP + r];
-  fz[t] += rec[2 * NP + r];
+  fx[t] += rec[r] * w;
+  fy[t] += rec[NP + r] * w;
+  fz[t] += rec[2 * NP + r] * w;
 }
 
 // distrib_ghost_force, ghost_control.c:71-125
@@ -2567,21 +2604,25 @@ void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_
   if (full) {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged_tab<B>), grid, block, 0, st, a);
     else if (mode == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_staged_dir<B>), grid, block, 0, st, a);
-    else if (a.ftab) {
+    else if (a.ftab && !a.respa) {
       const size_t lds = (size_t)a.nftab * CT_N * sizeof(float4);
       if (a.pimd) {
-        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, true, true>,
+        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, true, true, false>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, true>), grid, block, lds, st, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, true, false>), grid, block, lds, st, a);
       } else {
-        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, false, true>,
+        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, false, true, false>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, true>), grid, block, lds, st, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, true, false>), grid, block, lds, st, a);
       }
+    } else if (a.respa) {
+      const size_t lds = (size_t)CT_N * sizeof(float4);
+      if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, false, true>), grid, block, lds, st, a);
+      else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, false, true>), grid, block, lds, st, a);
     } else {
       const size_t lds = (size_t)CT_N * sizeof(float4);
-      if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, false>), grid, block, lds, st, a);
-      else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, false>), grid, block, lds, st, a);
+      if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, false, false>), grid, block, lds, st, a);
+      else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, false, false>), grid, block, lds, st, a);
     }
   } else {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 0>), grid, block, 0, st, a);

@@ -78,7 +78,8 @@ def lib():
     L.pmx_batch_destroy.argtypes = [C.c_void_p]
     L.pmx_autotune.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_int)]
     L.pmx_batch_autotune.argtypes = [C.c_void_p, C.c_uint, C.c_int, C.POINTER(C.c_int)]
-    if L.pmx_abi_version() != 7:
+    L.pmx_set_respa.argtypes = [C.c_void_p, C.POINTER(abi.PmxRespa)]
+    if L.pmx_abi_version() != 8:
         raise PinyAmdError("libpiny_amd.so ABI version mismatch")
     _lib = L
     return L
@@ -94,7 +95,7 @@ EXPORTED = ("pmx_abi_version", "pmx_device_count", "pmx_create", "pmx_destroy", 
             "pmx_get_velocities", "pmx_get_positions", "pmx_md_nve", "pmx_bond_coef_harm", "pmx_bond_coef_morse",
             "pmx_bend_coef_harm", "pmx_set_force_field", "pmx_get_param_grad", "pmx_batch_create",
             "pmx_batch_energy_control", "pmx_batch_last_error", "pmx_batch_destroy", "pmx_autotune",
-            "pmx_batch_autotune", "pmx_pimd_nvt", "pmx_set_nhc_state", "pmx_get_nhc_state")
+            "pmx_batch_autotune", "pmx_pimd_nvt", "pmx_set_nhc_state", "pmx_get_nhc_state", "pmx_set_respa")
 
 
 class Engine:
@@ -144,6 +145,12 @@ class Engine:
             return
         self._xm = [np.ascontiguousarray(v, dtype=np.float64) for v in xm]
         self._chk(lib().pmx_set_xmod(self.h, *[abi.dptr(v) for v in self._xm]), "pmx_set_xmod")
+
+    def set_respa(self, respa):
+        """RESPA settings for the later force calls (an abi.PmxRespa, or None
+        for off)."""
+        self._respa = respa
+        self._chk(lib().pmx_set_respa(self.h, C.byref(respa) if respa is not None else None), "pmx_set_respa")
 
     def energy_control(self, flags=None):
         if flags is None:

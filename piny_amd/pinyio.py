@@ -342,7 +342,14 @@ def read_input_dir(directory: str, input_file: str = None, positions: bool = Tru
         pme_on=1 if _on(pe.get("ewald_pme_opt", "off")) else 0,
         intra_perds=1 if _on(vol.get("intra_perds", "off")) else 0,
         temperature=float(gen.get("temperature", "300")),
-        pimd_typ={"staging": 1, "centroid": 2, "cent": 2}[pimd.get("path_int_md_typ", "centroid").lower()])
+        pimd_typ={"staging": 1, "centroid": 2, "cent": 2}[pimd.get("path_int_md_typ", "centroid").lower()],
+        # RESPA (set_sim_params.c:1585-1660, set_sim_dict.c:749-756,921-940,1494-1498)
+        respa_lrf=int(float(run.get("respa_steps_lrf", "0"))),
+        respa_torsion=int(float(run.get("respa_steps_torsion", "0"))),
+        respa_intra=int(float(run.get("respa_steps_intra", "0"))),
+        respa_pimd=int(float(pimd.get("respa_steps_pimd", "0"))),
+        pressure=float(gen.get("pressure", "0")),
+        pten_inter_respa=float(pe.get("pten_inter_respa", "0")))
     P = int(pimd.get("path_int_beads", "1"))
     # molecules, in the set file's order of mol_index
     sets = read_dict_file(os.path.join(directory, wr["mol_set_file"]))
@@ -497,25 +504,32 @@ def write_input_dir(directory: str, cfg, name: str = "water", nstep: int = 0, dt
     kb = w.KB_KCAL
     pimd_name = {1: "staging", 2: "centroid"}
     shift_name = {0: "off", 1: "on", 2: "swit"}
-    inp = [("sim_gen_def", [("simulation_typ", "pimd" if P > 1 else "md"),
-                            ("ensemble_typ", "nvt" if P > 1 else "nve"), ("num_time_step", nstep),
-                            ("restart_type", "initial"), ("time_step", dt_fs), ("temperature", st.temperature),
-                            ("generic_fft_opt", "on"), ("num_proc_tot", 1), ("num_proc_beads", 1),
-                            ("num_proc_class_forc", 1)])]
+    gen = [("simulation_typ", "pimd" if P > 1 else "md"), ("ensemble_typ", "nvt" if P > 1 else "nve"),
+           ("num_time_step", nstep), ("restart_type", "initial"), ("time_step", dt_fs),
+           ("temperature", st.temperature), ("generic_fft_opt", "on"), ("num_proc_tot", 1), ("num_proc_beads", 1),
+           ("num_proc_class_forc", 1)]
+    if st.pressure:
+        gen.append(("pressure", repr(st.pressure)))
+    inp = [("sim_gen_def", gen)]
     if P > 1:
-        inp.append(("sim_pimd_def", [("path_int_beads", P), ("path_int_md_typ", pimd_name[st.pimd_typ]),
-                                     ("initial_spread_opt", "off")]))
+        pim = [("path_int_beads", P), ("path_int_md_typ", pimd_name[st.pimd_typ]), ("initial_spread_opt", "off")]
+        if st.respa_pimd:
+            pim.append(("respa_steps_pimd", st.respa_pimd))
+        inp.append(("sim_pimd_def", pim))
         inp.append(("sim_nhc_def", [("atm_nhc_tau_def", 20.0), ("atm_nhc_len", 4), ("respa_steps_nhc", 4),
                                     ("yosh_steps_nhc", 3), ("respa_xi_opt", 1)]))
     inp += [("sim_list_def", [("neighbor_list", "ver_list"), ("verlist_skin", repr(st.skin)),
                               ("update_type", "no_list")]),
-            ("sim_run_def", [("respa_rheal", repr(st.rheal))]),
+            ("sim_run_def", [("respa_rheal", repr(st.rheal))] +
+             [(k, v) for k, v in (("respa_steps_lrf", st.respa_lrf), ("respa_steps_torsion", st.respa_torsion),
+                                  ("respa_steps_intra", st.respa_intra)) if v]),
             ("sim_vol_def", [("periodicity", 3), ("intra_perds", "on" if st.intra_perds else "off")]),
             ("sim_class_PE_def", [("shift_inter_pe", shift_name[st.ishift]), ("inter_spline_pts", st.nsplin),
                                   ("ewald_alpha", repr(s.ewald_alpha)), ("ewald_kmax", st.kmax_ewd),
                                   ("ewald_pme_opt", "on" if st.pme_on else "off"), ("ewald_kmax_pme", st.kmax_pme),
                                   ("ewald_interp_pme", st.n_interp), ("ewald_respa_pme_opt", "off"),
-                                  ("inter_PE_calc_freq", 1)]),
+                                  ("inter_PE_calc_freq", 1)] +
+             ([("pten_inter_respa", repr(st.pten_inter_respa))] if st.pten_inter_respa else [])),
             ("sim_write_def", [("sim_name", name + ".input-all"), ("write_screen_freq", 1000000),
                                ("instant_file", name + ".iavg"), ("write_inst_freq", 1000000),
                                ("atm_pos_file", name + ".confp"), ("write_pos_freq", 1000000),

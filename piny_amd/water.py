@@ -80,6 +80,14 @@ class Settings:
     intra_perds: int = 0         # default off (set_sim_dict.c:1231-1234)
     temperature: float = 300.0
     pimd_typ: int = 2            # centroid (normal modes)
+    # RESPA (multiple time steps): 0 leaves a keyword out, which turns that
+    # split off (set_sim_params.c:1585-1640)
+    respa_lrf: int = 0           # respa_steps_lrf: the inter split (short range inside res_dist)
+    respa_torsion: int = 0       # respa_steps_torsion
+    respa_intra: int = 0         # respa_steps_intra
+    respa_pimd: int = 0          # respa_steps_pimd
+    pressure: float = 0.0        # pressure, atm (enters the RESPA pressure-tensor terms)
+    pten_inter_respa: float = 0.0  # pten_inter_respa, atm
 
 
 def bond_coef(model: WaterModel):

@@ -106,6 +106,20 @@ CASES = {
     "tip64_p4_stag": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
                                             _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, pimd_typ=1)),
     "cfg2_p4_stag": lambda: w.config2(P=4, settings=_st(kmax_ewd=14, kmax_pme=14, pimd_typ=1)),
+    # RESPA: the inter split at res_dist 4 A (cutoff 5.5 A, healing 1 A)
+    # with two long-range steps and two intramolecular steps, classical
+    # (PME and standard Ewald) and with beads (two bead steps, a pressure and
+    # a guessed inter pressure tensor, which enter the pressure tensor)
+    "spc64_respa": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
+                                          _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, res_dist=4.0,
+                                              respa_lrf=2, respa_intra=2, respa_torsion=1)),
+    "spc64_ewald_respa": lambda: w.build_config(w.SPC_FW, 64, w.density_box(64), 1, 12,
+                                                _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, pme_on=0,
+                                                    res_dist=4.0, respa_lrf=3, respa_intra=2, respa_torsion=1)),
+    "tip64_p4_respa": lambda: w.build_config(w.Q_TIP4P_F, 64, w.density_box(64), 4, 13,
+                                             _st(cutoff=5.5, alpha=0.62, kmax_ewd=6, kmax_pme=6, res_dist=4.0,
+                                                 respa_lrf=2, respa_intra=2, respa_torsion=1, respa_pimd=2,
+                                                 pressure=1.0, pten_inter_respa=500.0)),
     # 256 q-TIP4P/F at P = 32, K = 32
     "tip256_p32": lambda: w.build_config(w.Q_TIP4P_F, 256, w.density_box(256), 32, 14,
                                          _st(cutoff=9.0, alpha=0.35, kmax_ewd=14, kmax_pme=14)),

@@ -54,22 +54,32 @@ def write_inputs(d, c):
            ("num_time_step", 0),
            ("restart_type", "initial"), ("time_step", 0.25), ("temperature", st.temperature),
            ("generic_fft_opt", "on"), ("num_proc_tot", 1), ("num_proc_beads", 1), ("num_proc_class_forc", 1)]
+    if st.pressure:
+        gen.append(("pressure", repr(st.pressure)))
     parts = [section("sim_gen_def", gen)]
     if P > 1:
-        parts.append(section("sim_pimd_def", [("path_int_beads", P), ("path_int_md_typ", PIMD_TYP[st.pimd_typ]),
-                                              ("initial_spread_opt", "off")]))
+        pimd = [("path_int_beads", P), ("path_int_md_typ", PIMD_TYP[st.pimd_typ]), ("initial_spread_opt", "off")]
+        if st.respa_pimd:
+            pimd.append(("respa_steps_pimd", st.respa_pimd))
+        parts.append(section("sim_pimd_def", pimd))
     if P > 1:
         parts.append(section("sim_nhc_def", [("atm_nhc_tau_def", 20.0), ("atm_nhc_len", 4), ("respa_steps_nhc", 4),
                                              ("yosh_steps_nhc", 3), ("respa_xi_opt", 1)]))
     parts.append(section("sim_list_def", [("neighbor_list", "ver_list"), ("verlist_skin", repr(st.skin)),
                                           ("update_type", "no_list")]))
-    parts.append(section("sim_run_def", [("respa_rheal", repr(st.rheal))]))
+    run = [("respa_rheal", repr(st.rheal))]
+    for key, v in (("respa_steps_lrf", st.respa_lrf), ("respa_steps_torsion", st.respa_torsion),
+                   ("respa_steps_intra", st.respa_intra)):
+        if v:
+            run.append((key, v))
+    parts.append(section("sim_run_def", run))
     parts.append(section("sim_vol_def", [("periodicity", 3), ("intra_perds", "on" if st.intra_perds else "off")]))
     parts.append(section("sim_class_PE_def", [
         ("shift_inter_pe", SHIFT[st.ishift]), ("inter_spline_pts", st.nsplin),
         ("ewald_alpha", repr(s.ewald_alpha)), ("ewald_kmax", st.kmax_ewd),
         ("ewald_pme_opt", "on" if st.pme_on else "off"), ("ewald_kmax_pme", st.kmax_pme),
-        ("ewald_interp_pme", st.n_interp), ("ewald_respa_pme_opt", "off"), ("inter_PE_calc_freq", 1)]))
+        ("ewald_interp_pme", st.n_interp), ("ewald_respa_pme_opt", "off"), ("inter_PE_calc_freq", 1)]
+        + ([("pten_inter_respa", repr(st.pten_inter_respa))] if st.pten_inter_respa else [])))
     parts.append(section("sim_write_def", [
         ("sim_name", "water.input-all"), ("write_screen_freq", 1000000), ("instant_file", "water.iavg"),
         ("write_inst_freq", 1000000), ("atm_pos_file", "water.confp"), ("write_pos_freq", 1000000),
@@ -172,6 +182,51 @@ def run_case(name):
                         energy_keys=np.array(keys), energies=e[:12], pvten=e[12:21], pvten_tot=pvten_tot,
                         forces=f, fxt=fxt)
     print(f"{name}: N={N} P={P} nver={nver} vintert={e[0]:.12g} vintrat={e[1]:.12g} -> {out}")
+
+
+# RESPA: the force call with combinations of (iget_full_inter,
+# iget_res_inter, iget_full_intra), iget_res_intra on (oracle/ref_harness.c
+# respa mode)
+RESPA_CASES = ["spc64_respa", "spc64_ewald_respa", "tip64_p4_respa"]
+RESPA_FLAGS = [(1, 1, 1), (0, 1, 1), (0, 0, 1), (1, 0, 1), (0, 1, 0), (0, 0, 0)]
+
+
+def run_respa_case(name):
+    c = cases.get(name)
+    s = c.system
+    res = {k: [] for k in ("energies", "pvten", "pvten_tot", "forces", "fxt")}
+    nver = nfree = None
+    with tempfile.TemporaryDirectory() as d:
+        write_inputs(d, c)
+        for full, rs, fintra in RESPA_FLAGS:
+            # parse() refuses to overwrite the summary file of a previous run
+            if os.path.exists(os.path.join(d, "water.input-all")):
+                os.remove(os.path.join(d, "water.input-all"))
+            with open(os.path.join(d, "stdout.txt"), "w") as log:
+                r = subprocess.run([EXE, "water.input", "pos.bin", "out.bin", "respa", str(full), str(rs), str(fintra)], cwd=d,
+                                   stdout=log, stderr=subprocess.STDOUT, timeout=600)
+            text = open(os.path.join(d, "stdout.txt")).read()
+            if r.returncode != 0:
+                print(text[-3000:])
+                raise RuntimeError(f"{name}: reference run failed ({r.returncode})")
+            # the degrees of freedom the reference counted (close_intra_params.c:247,258)
+            nfree = int(text.split("number of atomic degrees of freedom:")[1].split()[0])
+            raw = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64)
+            N, P = int(raw[0]), int(raw[1])
+            nver = int(raw[2])
+            e = raw[3:27]
+            res["energies"].append(e[:12])
+            res["pvten"].append(e[12:21])
+            res["pvten_tot"].append(raw[27:36])
+            res["forces"].append(raw[36:36 + 3 * N * P].reshape(P, N, 3))
+            res["fxt"].append(raw[36 + 3 * N * P:36 + 6 * N * P].reshape(P, N, 3) if P > 1 else np.zeros((0, N, 3)))
+    keys = ["vintert", "vintrat", "vreal", "vrecip", "vvdw", "vcoul", "vlong", "vbond", "vbend",
+            "pi_ke_prim", "pi_ke_vir", "kin_harm"]
+    out = os.path.join(HERE, f"respa_{name}.npz")
+    np.savez_compressed(out, case=name, x=c.x, y=c.y, z=c.z, nver=nver, nfree=nfree, flags=np.array(RESPA_FLAGS),
+                        energy_keys=np.array(keys), **{k: np.array(v) for k, v in res.items()})
+    print(f"{name}: N={s.natm} P={s.pi_beads} nver={nver} nfree={nfree} vintert(full+res)="
+          f"{res['energies'][0][0]:.12g} -> {out}")
 
 
 # the reference's setup products and Verlet list (oracle/ref_harness.c dump
@@ -354,7 +409,7 @@ def run_md_case(name, nsteps):
 
 
 def main():
-    names = sys.argv[1:] or CASES + ["md", "setup", "w512_debug", "pimd"]
+    names = sys.argv[1:] or CASES + ["md", "setup", "w512_debug", "pimd", "respa"]
     for n in names:
         if n == "md":
             for m, k in MD_CASES.items():
@@ -367,6 +422,9 @@ def main():
         elif n == "pimd":
             for m, k in PIMD_CASES.items():
                 run_pimd_case(m, k)
+        elif n == "respa":
+            for m in RESPA_CASES:
+                run_respa_case(m)
         else:
             run_case(n)
 

@@ -28,6 +28,9 @@ ARRAYS = ("atm_type", "q", "mass", "inter_label", "inter_sig", "inter_eps", "int
 # q-TIP4P/F (a ghost site, Morse bonds), a triclinic cell, staging, a box
 # without charges
 CASES = ["spc64", "tip64_p4", "cfg1", "tip64_p4_tric", "tip64_p4_stag", "spc64_shift", "spc64_none"]
+SETTINGS = ("skin", "rheal", "kmax_ewd", "kmax_pme", "n_interp", "nsplin", "ishift", "pme_on", "intra_perds",
+            "temperature", "pimd_typ", "respa_lrf", "respa_torsion", "respa_intra", "respa_pimd", "pressure",
+            "pten_inter_respa")
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -44,8 +47,7 @@ def test_reader_rebuilds_what_the_reference_parsed(name):
         assert a.shape == b.shape and np.array_equal(a, b), (k, a, b)
     assert np.allclose(t.hmat, s.hmat, rtol=1e-14, atol=0)
     assert abs(t.ewald_alpha - s.ewald_alpha) <= 1e-15 * s.ewald_alpha
-    for k in ("skin", "rheal", "kmax_ewd", "kmax_pme", "n_interp", "nsplin", "ishift", "pme_on", "intra_perds",
-              "temperature", "pimd_typ"):
+    for k in SETTINGS:
         assert getattr(t.settings, k) == getattr(s.settings, k), k
     # the .initial positions (15 decimals in Angstrom) to rounding
     for u, v in ((r.x, c.x), (r.y, c.y), (r.z, c.z)):
@@ -99,3 +101,25 @@ def test_conf_file_round_trip_and_header():
     for (x, y, z, hm), dx in zip(frames, (0.0, 0.5)):
         assert np.max(np.abs(x - (c.x + dx))) < 1e-10 * np.max(np.abs(c.x))
         assert np.allclose(hm, s.hmat, rtol=1e-12)
+
+
+def test_respa_input_read_back():
+    """The RESPA keywords of an input the reference parsed for the RESPA
+    fixtures come back as the settings, and the pmx_respa they give matches
+    the reference's counted degrees of freedom."""
+    from tests.test_respa import load as load_respa
+    c = cases.get("tip64_p4_respa")
+    with tempfile.TemporaryDirectory() as d:
+        write_inputs(d, c)
+        r = pinyio.read_input_dir(d, "water.input")
+    for k in SETTINGS:
+        assert getattr(r.system.settings, k) == getattr(c.system.settings, k), k
+    assert np.array_equal(r.system.inter_cutoff_res, c.system.inter_cutoff_res)
+    resp = abi.respa_params(r.system)
+    assert resp.respa_steps_lrf == 2 and resp.respa_steps_pimd == 2
+    assert resp.nfree == int(load_respa("tip64_p4_respa")["nfree"])
+    with tempfile.TemporaryDirectory() as d:
+        pinyio.write_input_dir(d, c, name="box")
+        r2 = pinyio.read_input_dir(d)
+    for k in SETTINGS:
+        assert getattr(r2.system.settings, k) == getattr(c.system.settings, k), k
