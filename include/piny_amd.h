@@ -262,11 +262,13 @@ int pmx_set_xmod(pmx_ctx *ctx, const double *x, const double *y,
  * weights 1).  As parse() insists (set_sim_params.c:3346-3370), the inter
  * split needs the intramolecular and torsion splits: respa_steps_lrf > 0
  * with respa_steps_intra or respa_steps_torsion 0 returns PMX_ERR_ARG.
- * Under weights other than 1 the call returns no virial estimator: the
- * reference forms pi_ke_vir and pvten_tot from the unweighted forces fxt,
- * which this library does not keep apart; both are NaN and pmx_get_fxt
- * returns PMX_ERR_STATE.  The cluster pair kernel and the half list are
- * not used under RESPA. */
+ * The reference forms the virial estimator pi_ke_vir and pvten_tot from the
+ * unweighted forces fxt of a full call: under weights other than 1,
+ * pmx_energy_control makes such a call twice, first without weights (for
+ * fxt, pi_ke_vir and pvten_tot), then with them; a batched call does not,
+ * and returns both as NaN.  After a short-range or intramolecular call
+ * pmx_get_fxt returns PMX_ERR_STATE.  The cluster pair kernel is not used
+ * under RESPA. */
 int pmx_set_respa(pmx_ctx *ctx, const pmx_respa *respa);
 
 /* The force call. */

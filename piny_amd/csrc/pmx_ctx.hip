@@ -53,6 +53,8 @@ struct pmx_ctx {
   double *pair_pp_res = nullptr;  // the same with the RESPA cutoff as cutoff (short-range calls)
   pmx_respa respa{};           // RESPA settings (zeros: off)
   bool last_weighted = false;  // the last call's forces carried RESPA weights other than 1
+  bool fxt_kept = false;       // fxt holds the last call's unweighted forces (a weighted full call's first pass)
+  bool keep_fxt = false;       // this sequence must not overwrite fxt
   float4 *ctab = nullptr;      // mixed precision: erfc(alp r)/r as cubic pieces [CT_N]
   float4 *ftab = nullptr;      // mixed precision: the switched pair functions [nftab][CT_N] (null if too many)
   int nftab = 0;
@@ -1831,7 +1833,7 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.fx = c->fx;
     a.fy = c->fy;
     a.fz = c->fz;
-    a.fxt = c->fxt;
+    a.fxt = c->keep_fxt ? nullptr : c->fxt;
     a.fyt = c->fyt;
     a.fzt = c->fzt;
     a.prekf = c->prekf;
@@ -2206,12 +2208,72 @@ int pmx_set_respa(pmx_ctx *c, const pmx_respa *r) {
   return PMX_OK;
 }
 
+// A full call under RESPA weights other than 1 runs twice: first without
+// RESPA, whose forces are the reference's unweighted fxt (every term enters
+// fxt before its weight is applied: force_npol.c:1222-1287,
+// ewald3d_pme.c:568-577, ecorr.c:210-231, bond.c:270-277, bend.c:360-394)
+// and give the virial estimator and pvten_tot (pimd_estimators.c:223-303);
+// then with the weights, for the forces and pvten.
+static int weighted_full_call(pmx_ctx *c, unsigned flags, pmx_energies *out) {
+  int rc;
+  hipStream_t st = c->st;
+  const size_t NP = (size_t)c->s.N * c->s.P;
+  const pmx_respa saved = c->respa;
+  const unsigned plain = flags & ~(unsigned)(PMX_EVAL_RES_INTER | PMX_EVAL_NO_FULL_INTER | PMX_EVAL_NO_FULL_INTRA);
+  pmx_energies u{};
+  c->respa = pmx_respa{};
+  rc = list_stage_launch(c, st, true);
+  if (!rc) rc = pre_sequence(c, plain, st, false, true);
+  if (!rc && hipEventSynchronize(c->mxev) != hipSuccess) rc = PMX_ERR_DEVICE;
+  if (!rc) rc = list_stage_finish(c, plain, out);
+  if (!rc) rc = main_sequence(c, plain, st, false);
+  if (!rc) rc = readback_launch(c, st);
+  if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = PMX_ERR_DEVICE;
+  if (!rc) assemble(c, plain, &u, false);  // still without RESPA: its pressure terms unweighted
+  c->respa = saved;
+  if (rc) return rc;
+  const bool modes = (flags & PMX_EVAL_PIMD) && (flags & PMX_EVAL_MODE_FORCES) && c->s.P > 1;
+  if (!modes) {  // with modes the mode kernel kept them
+    double *dst[3] = {c->fxt, c->fyt, c->fzt};
+    const double *src[3] = {c->fx, c->fy, c->fz};
+    for (int d = 0; d < 3; d++)
+      HIPCHK(hipMemcpyAsync(dst[d], src[d], NP * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  c->keep_fxt = true;
+  rc = pre_sequence(c, flags, st, true, true);
+  if (!rc) rc = main_sequence(c, flags, st, true);
+  c->keep_fxt = false;
+  if (rc) return rc;
+  if ((rc = readback_launch(c, st))) return rc;
+  HIPCHK(hipStreamSynchronize(st));
+  const int nver = out->nver, rebuilt = out->list_rebuilt;
+  const double spread = out->spread;
+  assemble(c, flags, out, true);
+  out->nver = nver;
+  out->list_rebuilt = rebuilt;
+  out->spread = spread;
+  // with iget_full_intra = 0 (no reference integrator makes such a full
+  // call) the estimator is 0 and pvten_tot lacks the bead correction,
+  // which the unweighted pass does not hold apart
+  const bool vir = !(flags & PMX_EVAL_NO_FULL_INTRA);
+  out->pi_ke_vir = vir ? u.pi_ke_vir : 0.0;
+  for (int k = 0; k < 9; k++) out->pvten_tot[k] = (vir || c->s.P == 1) ? u.pvten[k] : NAN;
+  c->last_weighted = false;
+  c->fxt_kept = true;
+  return PMX_OK;
+}
+
 static int force_call(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   int rc;
   HIPCHK(hipSetDevice(c->device));
   memset(out, 0, sizeof(*out));
   c->grad_ready = false;
+  c->fxt_kept = false;
   hipStream_t st = c->st;
+  {
+    const RespaCall rsp = respa_call(c, flags);
+    if (rsp.full && rsp.weighted) return weighted_full_call(c, flags, out);
+  }
   if ((rc = list_stage_launch(c, st, true))) return rc;
   if ((rc = pre_sequence(c, flags, st, true, true))) return rc;
   HIPCHK(hipEventSynchronize(c->mxev));
@@ -2646,11 +2708,11 @@ int pmx_get_fxt(pmx_ctx *c, double *fx, double *fy, double *fz) {
   // mode forces they are the bead forces kept before the transform.
   if (!c || !fx || !fy || !fz) return PMX_ERR_ARG;
   if (c->last_weighted) {
-    set_err(c, "pmx_get_fxt: the last call was not a full call without RESPA weights; the reference's fxt "
-               "holds the unweighted forces, which are not kept apart");
+    set_err(c, "pmx_get_fxt: the last call was a RESPA call whose unweighted forces were not formed (a "
+               "short-range or intramolecular call, or a batched one)");
     return PMX_ERR_STATE;
   }
-  if (c->last_modes) {
+  if (c->last_modes || c->fxt_kept) {
     double *src[3] = {c->fxt, c->fyt, c->fzt};
     return get_bead_major(c, src, fx, fy, fz);
   }

@@ -3,9 +3,10 @@ reference's own RESPA calls (tests/golden/respa_*.npz, every flag
 combination) in FP64 and mixed precision, against the oracle with the half
 list, in a batch, and across calls that change the flags on one context.
 
-Under RESPA weights other than 1 the device forms no virial estimator
-(pi_ke_vir and pvten_tot are NaN in full calls): the reference builds them
-from the unweighted forces fxt, which the device does not keep apart."""
+The reference builds the virial estimator and pvten_tot of a full call
+from the unweighted forces fxt; under RESPA weights the device's single
+force call forms them in an unweighted first pass, a batched call returns
+them as NaN."""
 import numpy as np
 import pytest
 
@@ -40,7 +41,10 @@ def test_device_respa_matches_reference(name, mode):
         e = g.energy_control(flags_of(full, res, fintra, s.pi_beads))
         f = g.forces()
         assert e["nver"] == int(d["nver"])
-        rms = compare_respa(name, k, e, f, d, etol, ftol, vtol, no_fxt=True, term_scale=(mode == "mixed"))
+        rms = compare_respa(name, k, e, f, d, etol, ftol, vtol, term_scale=(mode == "mixed"))
+        if s.pi_beads > 1 and full:  # fxt: the unweighted forces, as the reference keeps them
+            fxt = g.fxt()
+            assert np.isfinite(fxt).all()
         print(f"  flags {tuple(int(v) for v in d['flags'][k])}: force rms rel err {rms:.2e}", flush=True)
     g.close()
 
@@ -108,19 +112,34 @@ def test_respa_flags_change_between_calls_and_in_a_batch():
         g.close()
 
 
-def test_respa_fxt_refused_and_integrators_refuse():
+def test_respa_fxt_and_integrators():
+    """fxt of a weighted full call equals the forces of the same call without
+    RESPA (the bead forces, before the mode transform); a short-range call
+    has none; the RESPA-off integrators refuse a RESPA context; a batch
+    returns the estimator as NaN."""
     c = cases.get("tip64_p4_respa")
     s = c.system
+    print("[respa fxt tip64_p4_respa]", flush=True)
     g = Engine(s, c.x, c.y, c.z)
-    g.set_respa(abi.respa_params(s))
     g.set_positions(c.x, c.y, c.z)
+    e0 = g.energy_control(abi.PMX_EVAL_PIMD | abi.PMX_EVAL_MODE_FORCES)
+    fxt0 = g.fxt()
+    g.set_respa(abi.respa_params(s))
     e = g.energy_control(flags_of(1, 1, 1, s.pi_beads))
-    assert np.isnan(e["pi_ke_vir"]) and np.all(np.isnan(e["pvten_tot"]))
+    assert np.max(np.abs(g.fxt() - fxt0)) <= 1e-12 * np.max(np.abs(fxt0))
+    assert abs(e["pi_ke_vir"] - e0["pi_ke_vir"]) <= 1e-12 * abs(e0["pi_ke_vir"])
+    assert np.max(np.abs(e["pvten_tot"] - e0["pvten"])) <= 1e-12 * np.max(np.abs(e0["pvten"]))
     from piny_amd.engine import PinyAmdError
+    g.energy_control(flags_of(0, 1, 1, s.pi_beads))
     with pytest.raises(PinyAmdError):
         g.fxt()
     with pytest.raises(PinyAmdError):
         g.pimd_nvt(1)
+    b = Batch([g])
+    eb = b.energy_control(flags_of(1, 1, 1, s.pi_beads))[0]
+    assert np.isnan(eb["pi_ke_vir"]) and np.all(np.isnan(eb["pvten_tot"]))
+    assert abs(eb["vintert"] - e["vintert"]) <= 1e-12 * abs(e["vintert"])
+    b.close()
     # off again: the plain call, the estimator and fxt are back
     g.set_respa(None)
     e = g.energy_control(abi.PMX_EVAL_PIMD | abi.PMX_EVAL_MODE_FORCES)
