@@ -22,7 +22,9 @@ Three paths are timed on the same box:
 `value` is the mixed path's throughput, reported with the fp64 path's and
 the mixed path's force RMS error relative to fp64.  `fp64_per_bead` times
 the plain FP64 form beside them: each bead as its own classical force call,
-one after another.
+one after another.  `respa` times the force calls of one outer RESPA step
+with the step counts of the reference's q-TIP4P/F example (intramolecular,
+short-range and full calls) against as many plain calls.
 
 Prints one JSON line.
 """
@@ -180,6 +182,52 @@ def run_path(cfg, steps, warmup, rebuild_every=20, tune=True, **engine_kw):
     return out
 
 
+def run_respa(cfg, reps, **engine_kw):
+    """The force calls of one outer step of int_NVT_pimd under the RESPA
+    steps of the reference's q-TIP4P/F example (respa_steps_lrf 4,
+    respa_steps_intra 2, respa_steps_torsion 1, respa_steps_pimd 5,
+    examples/q-TIP4P-F/build.py:54,66-68) on the benchmark box, the inter
+    split at res_dist: int_NVT_pimd.c:143-215 calls energy_control_pimd once
+    per intramolecular step (ir_pimd = nres_pimd), 4 x 2 = 8 calls per outer
+    step: 4 intramolecular only, 3 short-range, 1 full (which runs an
+    unweighted pass first for the virial estimator).  Reported: ms per call
+    of each kind, ms per outer step, and the same 8 calls as full plain
+    calls for comparison."""
+    import dataclasses
+    from piny_amd import abi
+    from piny_amd.engine import Engine
+    st = dataclasses.replace(cfg.system.settings, respa_lrf=4, respa_intra=2, respa_torsion=1, respa_pimd=5)
+    s = dataclasses.replace(cfg.system, settings=st)
+    eng = Engine(s, cfg.x, cfg.y, cfg.z, **engine_kw)
+    eng.set_positions(cfg.x, cfg.y, cfg.z)
+    base = abi.PMX_EVAL_PIMD | abi.PMX_EVAL_MODE_FORCES
+    kinds = {"intra_only": base | abi.PMX_EVAL_NO_FULL_INTER | abi.PMX_EVAL_NO_FULL_INTRA,
+             "short_range": base | abi.PMX_EVAL_NO_FULL_INTER | abi.PMX_EVAL_RES_INTER,
+             "full": base | abi.PMX_EVAL_RES_INTER}
+    seq = []
+    for ir_ter in range(1, 5):
+        for ir_tra in range(1, 3):
+            seq.append("full" if (ir_ter == 4 and ir_tra == 2) else ("short_range" if ir_tra == 2 else "intra_only"))
+
+    def timed(fn, n):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    ms_plain = timed(lambda: eng.energy_control(base), reps)
+    eng.set_respa(abi.respa_params(s))
+    ms = {k: timed(lambda f=f: eng.energy_control(f), reps) for k, f in kinds.items()}
+    ms_outer = timed(lambda: [eng.energy_control(kinds[k]) for k in seq], reps)
+    eng.close()
+    return {"steps": {"respa_steps_lrf": 4, "respa_steps_intra": 2, "respa_steps_torsion": 1,
+                      "respa_steps_pimd": 5},
+            "res_dist_A": st.res_dist, "calls_per_outer_step": seq, "ms_per_call": ms,
+            "ms_per_outer_step": ms_outer, "ms_plain_full_call": ms_plain,
+            "speedup_vs_plain_calls": len(seq) * ms_plain / ms_outer}
+
+
 def run_batch(steps, warmup, nbox=64):
     """The uneven-box workload: nbox SPC/Fw boxes (molecule counts
     log-uniform in [64, 4096], 1 to 64 beads).  One step evaluates every box:
@@ -274,6 +322,7 @@ def main():
     ap.add_argument("--paths", default="fp64,fp64_direct,mixed")
     ap.add_argument("--no-per-bead", action="store_true",
                     help="skip the one-bead-per-call FP64 baseline (kernel profiles of the batched paths)")
+    ap.add_argument("--no-respa", action="store_true", help="skip the RESPA call timings")
     ap.add_argument("--batch", action="store_true",
                     help="the uneven-box workload (64 boxes, batched graph launch vs sequential calls)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -308,6 +357,8 @@ def main():
           "mixed": {"precision": abi.PMX_PREC_MIXED}}
     paths = [p for p in args.paths.split(",") if p]
     res = {p: run_path(cfg, args.steps, args.warmup, tune=not args.no_tune, **kw[p]) for p in paths}
+    head_kw = kw["mixed" if "mixed" in res else paths[-1]]
+    respa = run_respa(cfg, max(5, args.steps // 4), **head_kw) if not args.no_respa else None
     per_bead = (run_per_bead(cfg, max(2, args.steps // 4), 1)
                 if "fp64" in paths and not args.no_per_bead else None)
     head = res["mixed"] if "mixed" in res else res[paths[-1]]
@@ -355,6 +406,7 @@ def main():
                   for p, r in res.items()},
         "speedup_vs_fp64": head["value"] / ref["value"] if ref else None,
         "fp64_per_bead": per_bead,
+        "respa": respa,
         "speedup_vs_fp64_per_bead": head["value"] / per_bead["value"] if per_bead else None,
         "force_rms_err": rms(head["forces"]),
         "pairs_in_cutoff_all_beads": npairs,
