@@ -47,7 +47,9 @@
  *                  pi_ke_vir}, the bead positions and the mode velocities
  *                  [bead][site][3], the centroid chain's v[len], x[len], and
  *                  for beads 2..P, sites and x, y, z the chain's v[len],
- *                  x[len] (zeros for ghost sites)
+ *                  x[len] (zeros for ghost sites); the reference's own
+ *                  restart writer (write_dump_file_pimd) then writes the
+ *                  final state to the input's out_restart_file
  *   respa:         the force call with RESPA flags: argv[5], argv[6] and
  *                  argv[7] are iget_full_inter, iget_res_inter and
  *                  iget_full_intra; iget_res_intra is 1, as
@@ -62,7 +64,9 @@
  *                  int_NVE (md/int_NVE.c) runs nsteps times with the
  *                  input's time step; out.bin then holds the header, the
  *                  energies of the last force call, kinet, and the final
- *                  positions, velocities and forces, [site][3] each
+ *                  positions, velocities and forces, [site][3] each; the
+ *                  reference's restart writer (write_dump_file_md) then
+ *                  writes the final state to the input's out_restart_file
  */
 #include <time.h>
 #include "standard_include.h"
@@ -81,6 +85,9 @@
 
 /* the debug driver's per-term energy (src/energy/control/test_energy.c:489) */
 void test_force(CLASS *, BONDED *, GENERAL_DATA *, int, double *, double *, double *);
+/* the restart writers (output/md/output_md.c:757, output/pimd/output_pimd.c:786) */
+void write_dump_file_md(CLASS *, BONDED *, GENERAL_DATA *);
+void write_dump_file_pimd(CLASS *, BONDED *, GENERAL_DATA *);
 
 static CLASS cls;
 static BONDED bonded;
@@ -178,6 +185,7 @@ int main(int argc, char *argv[]) {
         for (int j = 1; j <= tb->len_nhc; j++)
           for (int k = 1; k <= tb->num_nhc; k++) cls.therm_bead[ip].v_nhc[j][k] = cls.therm_bead[ip].x_nhc[j][k] = 0.0;
       for (int st = 0; st < nsteps; st++) int_NVT_pimd(&cls, &bonded, &gd, &analysis);
+      write_dump_file_pimd(&cls, &bonded, &gd);
       STAT_AVG *sa = &gd.stat_avg;
       double hdr[9] = {(double)N, (double)P, (double)cls.nbr_list.verlist.nver_lst_now, (double)nsteps,
                        gd.timeinfo.dt, (double)tc->len_nhc, (double)tc->num_nhc, (double)tb->len_nhc,
@@ -234,6 +242,7 @@ int main(int argc, char *argv[]) {
       }
       free(vel);
       for (int st = 0; st < nsteps; st++) int_NVE(&cls, &bonded, &gd);
+      write_dump_file_md(&cls, &bonded, &gd);
       STAT_AVG *s = &gd.stat_avg;
       double hdr[3] = {(double)N, (double)P, (double)cls.nbr_list.verlist.nver_lst_now};
       double e[4] = {s->vintert, s->vintrat, s->kinet, gd.timeinfo.dt};

@@ -24,9 +24,12 @@ starts from, and the coordinate files it reads and writes.
   number, atom type), then per frame P*natm lines of x y z (Bohr) and the
   cell's three rows.
 
-The restart dump (restart_type restart_all, output_md.c:757-900) holds
-thermostat and barostat state beyond the force path and is not read or
-written here.
+* Restart files (restart_type restart_all; write_dump_file_md,
+  output_md.c:757-880, and write_dump_file_pimd, output_pimd.c:786-930):
+  the bead positions with their type names, the cell, the velocities (mode
+  velocities with beads), the thermostat chains' velocities (per bead
+  chain with beads), the barostat's (zero in NVE and NVT), and the time
+  step, degrees of freedom, ensemble and state point.
 """
 from __future__ import annotations
 
@@ -236,6 +239,215 @@ def read_conf(path: str):
             hm[i], hm[3 + i], hm[6 + i] = blk[P * h.natm + i]
         frames.append((blk[:P * h.natm, 0].copy(), blk[:P * h.natm, 1].copy(), blk[:P * h.natm, 2].copy(), hm))
     return h, frames
+
+
+# ---------------------------------------------------------------------------
+# restart files
+
+@dataclasses.dataclass
+class Restart:
+    """A restart_all file's contents (PINY's 1-based indices made 0-based;
+    arrays bead-major [P][natm] like the library's)."""
+    natm: int
+    pi_beads: int
+    itime: int = 0
+    pos: np.ndarray = None          # [P, natm, 3] Bohr
+    labels: list = None             # per atom: (atm_typ, res_typ, mol_typ, mol_num[, res_num])
+    hmat: np.ndarray = None         # [9] column-major (PINY hmat[1..9])
+    hmat_ewd: np.ndarray = None
+    x_lnv: float = 0.0
+    vel: np.ndarray = None          # [P, natm, 3] (mode velocities with beads)
+    nhc_class_v: np.ndarray = None  # [len, num] therm_class.v_nhc
+    nhc_bead_v: np.ndarray = None   # [P-1, len, num] therm_bead[2..P].v_nhc (beads only)
+    vgmat: np.ndarray = None        # [9]
+    v_lnv: float = 0.0
+    v_vol_nhc: np.ndarray = None    # [len]
+    dt: float = 0.0
+    nfree: int = 0
+    ensemble: tuple = (0, 1, 0, 0, 0)  # nve nvt npt_i npt_f nst
+    t_ext: float = 300.0
+    pext: float = 0.0
+    stens_ext: float = 0.0
+    nfree_terms: tuple = (0, 0, 0)  # nbond_free nbend_free ntors_free
+
+
+def restart_labels(system, mol_name: str = "water", type_names=None):
+    """The per-atom names of a restart file for a system of molecules of one
+    template: atom type, residue type and molecule type (the molecule's name
+    for both, as PINY names the single residue of a molecule without
+    residues), the 1-based molecule number and residue number."""
+    ns = len(system.model.sites)
+    names = list(type_names) if type_names else []
+    if not names:
+        for n in system.model.sites:
+            if n not in names:
+                names.append(n)
+    return [(names[int(system.atm_type[i])], mol_name, mol_name, i // ns + 1, 1) for i in range(system.natm)]
+
+
+def restart_text(r: Restart) -> str:
+    """The text write_dump_file_pimd (pi_beads > 1) or write_dump_file_md
+    writes for r."""
+    g = lambda v: "%.13g" % v  # noqa: E731
+    P, N = r.pi_beads, r.natm
+    out = ["natm_tot restart_typ itime pi_beads", "%d restart_all %d %d" % (N, r.itime, P),
+           "atm pos, atm_typ, mol_typ mol_num"]
+    for ip in range(P):
+        for i in range(N):
+            lab = r.labels[i]
+            tail = "%s %s %s %d" % lab[:4] if P > 1 else "%s %s %s %d %d" % lab[:5]
+            out.append("%s %s %s %s" % (g(r.pos[ip, i, 0]), g(r.pos[ip, i, 1]), g(r.pos[ip, i, 2]), tail))
+    h, he = r.hmat, r.hmat_ewd
+    out.append("h matrix")
+    out += ["%s %s %s" % (g(h[k]), g(h[3 + k]), g(h[6 + k])) for k in range(3)]
+    out.append("h matrix for Ewald setup")
+    out += ["%s %s %s" % (g(he[k]), g(he[3 + k]), g(he[6 + k])) for k in range(3)]
+    out += ["1/3 log(Vol)", g(r.x_lnv), "atm vel"]
+    for ip in range(P):
+        for i in range(N):
+            out.append("%s %s %s" % (g(r.vel[ip, i, 0]), g(r.vel[ip, i, 1]), g(r.vel[ip, i, 2])))
+    cl = np.atleast_2d(np.asarray(r.nhc_class_v, dtype=np.float64))
+    lenc, numc = cl.shape if cl.size else (0, 0)
+    out += ["number of atm nhc, length of nhc", "%d %d" % (numc, lenc), "atm nhc velocities"]
+    out += [g(cl[j, k]) for j in range(lenc) for k in range(numc)]
+    if P > 1:
+        bd = np.asarray(r.nhc_bead_v)
+        lenb, numb = bd.shape[1], bd.shape[2]
+        out += ["number of bead nhc, length of nhc", "%d %d" % (numb, lenb), "bead nhc velocities"]
+        out += ["%.10g" % bd[ip, j, k] for ip in range(P - 1) for j in range(lenb) for k in range(numb)]
+    vg = r.vgmat
+    out.append("vol velocities")
+    out += ["%s %s %s" % (g(vg[k]), g(vg[3 + k]), g(vg[6 + k])) for k in range(3)]
+    out += ["log(vol) velocity", g(r.v_lnv), "vol nhc velocities"]
+    out += [g(v) for v in r.v_vol_nhc]
+    e = r.ensemble
+    out += ["dt=%s" % g(r.dt), "nfree=%d" % r.nfree,
+            "nve=%d nvt=%d npt_i=%d npt_f=%d nst=%d" % tuple(e),
+            "nbond_free=%d nbend_free=%d ntors_free=%d" % tuple(r.nfree_terms),
+            "t_ext=%s,pext=%s,stens_ext=%s" % (g(r.t_ext), g(r.pext), g(r.stens_ext))]
+    return "\n".join(out) + "\n"
+
+
+def restart_for(system, pos, vel, itime=0, nhc_class_v=None, nhc_bead_v=None, len_nhc=None, dt=0.0,
+                nfree=None, ensemble=None, mol_name="water") -> Restart:
+    """A Restart of a system's state: positions and velocities [P, natm, 3]
+    (mode velocities with beads), the chains' velocities (class [len, num],
+    beads [P-1, len, num]; without them the NVE layout of no chains), in an
+    NVT (beads) or NVE (classical) ensemble unless given."""
+    P, N = system.pi_beads, system.natm
+    h = np.asarray(system.hmat, dtype=np.float64).reshape(9)
+    hm = h.reshape(3, 3).T  # columns are the box vectors: hmat[3 c + r] is row r, column c
+    if ensemble is None:
+        ensemble = (0, 1, 0, 0, 0) if nhc_class_v is not None else (1, 0, 0, 0, 0)
+    if len_nhc is None:
+        len_nhc = np.shape(nhc_class_v)[0] if nhc_class_v is not None else 0
+    st = system.settings
+    return Restart(natm=N, pi_beads=P, itime=itime, pos=np.asarray(pos, dtype=np.float64).reshape(P, N, 3),
+                   labels=restart_labels(system, mol_name), hmat=h.copy(), hmat_ewd=h.copy(),
+                   x_lnv=math.log(abs(np.linalg.det(hm))) / 3.0,
+                   vel=np.asarray(vel, dtype=np.float64).reshape(P, N, 3),
+                   nhc_class_v=(np.zeros((0, 0)) if nhc_class_v is None else np.asarray(nhc_class_v)),
+                   nhc_bead_v=nhc_bead_v, vgmat=np.zeros(9), v_lnv=0.0, v_vol_nhc=np.zeros(len_nhc), dt=dt,
+                   nfree=3 * (N - len(system.ghost_atom)) if nfree is None else nfree, ensemble=tuple(ensemble),
+                   t_ext=st.temperature, pext=st.pressure * abi.PCONV)
+
+
+def write_restart(path: str, r: Restart) -> None:
+    with open(path, "w") as f:
+        f.write(restart_text(r))
+
+
+def read_restart(path: str) -> Restart:
+    """A restart_all file written by PINY (or by write_restart)."""
+    with open(path) as f:
+        lines = f.read().splitlines()
+    k = 0
+
+    def nxt():
+        nonlocal k
+        k += 1
+        return lines[k - 1]
+
+    nxt()
+    head = nxt().split()
+    N, itime, P = int(head[0]), int(head[2]), int(head[3])
+    if head[1] != "restart_all":
+        raise ValueError(f"{path}: restart type {head[1]}, not restart_all")
+    nxt()
+    pos = np.zeros((P, N, 3))
+    labels = []
+    for ip in range(P):
+        for i in range(N):
+            v = nxt().split()
+            pos[ip, i] = [float(u) for u in v[:3]]
+            if ip == 0:
+                labels.append(tuple(v[3:6]) + tuple(int(u) for u in v[6:]))
+
+    def mat():
+        m = np.zeros(9)
+        for r_ in range(3):
+            a, b, c = (float(u) for u in nxt().split())
+            m[r_], m[3 + r_], m[6 + r_] = a, b, c
+        return m
+
+    nxt()
+    hmat = mat()
+    nxt()
+    hmat_ewd = mat()
+    nxt()
+    x_lnv = float(nxt())
+    nxt()
+    vel = np.zeros((P, N, 3))
+    for ip in range(P):
+        for i in range(N):
+            vel[ip, i] = [float(u) for u in nxt().split()]
+    nxt()
+    numc, lenc = (int(u) for u in nxt().split())
+    nxt()
+    cl = np.array([float(nxt()) for _ in range(lenc * numc)]).reshape(lenc, numc)
+    bead = None
+    if P > 1:
+        nxt()
+        numb, lenb = (int(u) for u in nxt().split())
+        nxt()
+        bead = np.array([float(nxt()) for _ in range((P - 1) * lenb * numb)]).reshape(P - 1, lenb, numb)
+    nxt()
+    vg = mat()
+    nxt()
+    v_lnv = float(nxt())
+    nxt()
+    vvol = []
+    while not lines[k].startswith("dt="):
+        vvol.append(float(nxt()))
+    dt = float(nxt().split("=")[1])
+    nfree = int(nxt().split("=")[1])
+    ens = tuple(int(t.split("=")[1]) for t in nxt().split())
+    nft = tuple(int(t.split("=")[1]) for t in nxt().split())
+    sp = dict(t.split("=") for t in nxt().split(","))
+    return Restart(natm=N, pi_beads=P, itime=itime, pos=pos, labels=labels, hmat=hmat, hmat_ewd=hmat_ewd,
+                   x_lnv=x_lnv, vel=vel, nhc_class_v=cl, nhc_bead_v=bead, vgmat=vg, v_lnv=v_lnv,
+                   v_vol_nhc=np.array(vvol), dt=dt, nfree=nfree, ensemble=ens, t_ext=float(sp["t_ext"]),
+                   pext=float(sp["pext"]), stens_ext=float(sp["stens_ext"]), nfree_terms=nft)
+
+
+def nhc_state_to_restart(system, state, len_nhc):
+    """The chain velocities of a library chain state (pmx_get_nhc_state's
+    layout: the centroid chain, then per mode 2..P, site and direction) in a
+    restart file's order: the one global class chain [len, 1], and per bead
+    2..P the chains numbered 3 (real site) + direction, ghosts skipped
+    (set_atm_NHC.c:648-700), [P-1, len, 3 real sites]."""
+    P, N, L = system.pi_beads, system.natm, len_nhc
+    st = np.asarray(state, dtype=np.float64)
+    cls = st[:L].reshape(L, 1)
+    ghost = np.zeros(N, bool)
+    ghost[np.asarray(system.ghost_atom, dtype=int)] = True
+    real = np.nonzero(~ghost)[0]
+    bead = np.zeros((P - 1, L, 3 * len(real)))
+    blocks = st[2 * L:].reshape(P - 1, N, 3, 2, L)
+    for r_, i in enumerate(real):
+        for d in range(3):
+            bead[:, :, 3 * r_ + d] = blocks[:, i, d, 0, :]
+    return cls, bead
 
 
 # ---------------------------------------------------------------------------

@@ -342,6 +342,8 @@ def run_pimd_case(name, nsteps):
             print(open(os.path.join(d, "stdout.txt")).read()[-3000:])
             raise RuntimeError(f"pimd {name}: reference run failed ({r.returncode})")
         raw = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64)
+        # the reference's restart file of the final state (write_dump_file_pimd)
+        restart = open(os.path.join(d, "water.restart")).read()
     hdr = raw[:9]
     assert int(hdr[0]) == N and int(hdr[1]) == P
     lc, nc, lb, nb = (int(v) for v in hdr[5:9])
@@ -361,7 +363,7 @@ def run_pimd_case(name, nsteps):
     np.savez_compressed(out, case=name, nsteps=nsteps, dt=hdr[4], len_nhc=lc, nchain_class=nc, nchain_bead=nb,
                         v0=v0, energies=e, energy_keys=np.array(["vintert", "vintrat", "kin_harm", "pi_ke_prim",
                                                                  "pi_ke_vir"]),
-                        pos=pos, vel=vel, nhc_class=cls, nhc_bead=bead)
+                        pos=pos, vel=vel, nhc_class=cls, nhc_bead=bead, restart=np.array(restart))
     print(f"pimd {name}: {nsteps} steps dt={hdr[4]:.6g} au, chains {nc} + {nb}, vintert={e[0]:.12g} -> {out}")
 
 
@@ -399,12 +401,13 @@ def run_md_case(name, nsteps):
             print(open(os.path.join(d, "stdout.txt")).read()[-3000:])
             raise RuntimeError(f"md {name}: reference run failed ({r.returncode})")
         raw = np.fromfile(os.path.join(d, "out.bin"), dtype=np.float64)
+        restart = open(os.path.join(d, "water.restart")).read()  # write_dump_file_md
     N = int(raw[0])
     vintert, vintrat, kinet, dt = raw[3:7]
     xvf = raw[7:7 + 9 * N].reshape(N, 9)
     out = os.path.join(HERE, f"ref_md_{name}.npz")
     np.savez_compressed(out, case=name, nsteps=nsteps, dt=dt, v0=v0, vintert=vintert, vintrat=vintrat,
-                        kinet=kinet, x=xvf[:, 0:3], v=xvf[:, 3:6], f=xvf[:, 6:9])
+                        kinet=kinet, x=xvf[:, 0:3], v=xvf[:, 3:6], f=xvf[:, 6:9], restart=np.array(restart))
     print(f"md {name}: {nsteps} steps dt={dt:.6g} au, vintert={vintert:.12g} kinet={kinet:.12g} -> {out}")
 
 

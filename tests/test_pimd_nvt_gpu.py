@@ -7,11 +7,14 @@ Bars: FP64 trajectories (bead positions, mode velocities, every chain's
 state) 1e-9 relative to the largest value after 20 steps; the mixed path's
 extended-energy fluctuation over 50 fs within 2x of the FP64 path's (the
 north-star bound)."""
+import os
+import tempfile
+
 import numpy as np
 import pytest
 
 from oracle.oracle import Oracle
-from piny_amd import abi
+from piny_amd import abi, pinyio
 from piny_amd.engine import Engine
 from tests import cases
 from tests.test_pimd_nvt import PIMD, compare_trajectory, load_pimd
@@ -38,6 +41,22 @@ def test_device_pimd_nvt_matches_reference(name):
     pos, vel, st = g.positions(), g.velocities(), g.nhc_state(4)
     print(f"  pos {np.abs(pos - d['pos']).max():.2e} vel {np.abs(vel - d['vel']).max():.2e}", flush=True)
     compare_trajectory(name, d, pos, vel, st, e, 1e-9)
+    # the device's final state as a restart file (pinyio), against the one
+    # the reference's own writer made of its final state
+    s = cases.get(name).system
+    cls, bead = pinyio.nhc_state_to_restart(s, st, 4)
+    mine = pinyio.restart_for(s, pos, vel, nhc_class_v=cls, nhc_bead_v=bead, dt=float(d["dt"]))
+    with tempfile.TemporaryDirectory() as t:
+        p = os.path.join(t, "device.restart")
+        pinyio.write_restart(p, mine)
+        back = pinyio.read_restart(p)
+        q = os.path.join(t, "reference.restart")
+        open(q, "w").write(str(d["restart"]))
+        ref = pinyio.read_restart(q)
+    assert back.labels == ref.labels and back.nfree == ref.nfree and back.ensemble == ref.ensemble
+    for a, b in ((back.pos, ref.pos), (back.vel, ref.vel), (back.nhc_class_v, ref.nhc_class_v),
+                 (back.nhc_bead_v, ref.nhc_bead_v)):
+        assert np.max(np.abs(a - b)) <= 1e-9 * np.max(np.abs(b))
 
 
 def test_device_pimd_nvt_matches_oracle_over_calls():

@@ -123,3 +123,42 @@ def test_respa_input_read_back():
         r2 = pinyio.read_input_dir(d)
     for k in SETTINGS:
         assert getattr(r2.system.settings, k) == getattr(c.system.settings, k), k
+
+
+def _golden(name):
+    return np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name))
+
+
+def test_restart_text_is_the_reference_writers():
+    """The reference's own restart writers (write_dump_file_pimd after 20
+    PIMD NVT steps, write_dump_file_md after 100 NVE steps; oracle/ref_harness.c)
+    wrote the fixtures' restart files from the states the fixtures hold;
+    restart_text writes the same states byte for byte, and read_restart
+    reads the reference's files back to those states."""
+    # PIMD: bead positions, mode velocities, the global class chain and the
+    # per-degree-of-freedom bead chains
+    d = _golden("ref_pimd_tip64_p4.npz")
+    c = cases.get("tip64_p4")
+    s = c.system
+    L = int(d["len_nhc"])
+    state = np.concatenate([d["nhc_class"], d["nhc_bead"].reshape(-1)])
+    cls, bead = pinyio.nhc_state_to_restart(s, state, L)
+    r = pinyio.restart_for(s, d["pos"], d["vel"], nhc_class_v=cls, nhc_bead_v=bead, dt=float(d["dt"]))
+    ref_text = str(d["restart"])
+    assert pinyio.restart_text(r) == ref_text
+    with tempfile.TemporaryDirectory() as t:
+        p = os.path.join(t, "water.restart")
+        open(p, "w").write(ref_text)
+        back = pinyio.read_restart(p)
+    assert back.natm == s.natm and back.pi_beads == s.pi_beads
+    assert np.allclose(back.pos, d["pos"], rtol=1e-12, atol=0) and np.allclose(back.vel, d["vel"], rtol=1e-12, atol=0)
+    assert np.allclose(back.nhc_class_v, cls, rtol=1e-12, atol=0)
+    assert np.allclose(back.nhc_bead_v, bead, rtol=1e-9, atol=1e-300)  # the bead chains are written with 10 digits
+    assert back.labels[3] == ("M", "water", "water", 1) and back.nfree == 576 and back.ensemble == (0, 1, 0, 0, 0)
+    assert pinyio.restart_text(back) == ref_text
+    # classical NVE: Cartesian velocities, no chains
+    d = _golden("ref_md_cfg1.npz")
+    c = cases.get("cfg1")
+    s = c.system
+    r = pinyio.restart_for(s, d["x"][None], d["v"][None], dt=float(d["dt"]))
+    assert pinyio.restart_text(r) == str(d["restart"])
