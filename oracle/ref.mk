@@ -19,7 +19,22 @@ FOBJ   := $(addprefix $(OUT)/obj/,$(notdir $(FSRC:.f=.fo)))
 vpath %.c $(sort $(dir $(CSRC)))
 vpath %.f $(sort $(dir $(FSRC)))
 
-all: $(OUT)/piny_ref
+all: $(OUT)/piny_ref $(OUT)/piny_shim
+
+# The drop-in shim (piny_amd/shim) inside a PINY program: the reference's
+# objects, the shim and its test driver, linked with the library.
+SHIM   := ../piny_amd/shim
+AMDLIB := ../piny_amd
+
+$(OUT)/obj/pmx_piny.o: $(SHIM)/pmx_piny.c $(SHIM)/pmx_piny.h ../include/piny_amd.h | $(OUT)/obj
+	$(CC) $(CFLAGS) -I$(SHIM) -I../include -c -o $@ $<
+
+$(OUT)/obj/shim_driver.o: shim_driver.c $(SHIM)/pmx_piny.h | $(OUT)/obj
+	$(CC) $(CFLAGS) -I$(SHIM) -I../include -c -o $@ $<
+
+$(OUT)/piny_shim: $(COBJ) $(FOBJ) $(OUT)/obj/shim_driver.o $(OUT)/obj/pmx_piny.o $(AMDLIB)/libpiny_amd.so
+	$(FC) -O2 -o $@ $(COBJ) $(FOBJ) $(OUT)/obj/shim_driver.o $(OUT)/obj/pmx_piny.o \
+	    -L$(AMDLIB) -lpiny_amd -Wl,-rpath,'$$ORIGIN/../../piny_amd' -lm
 
 $(OUT)/obj:
 	mkdir -p $@
