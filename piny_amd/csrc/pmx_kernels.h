@@ -348,8 +348,9 @@ __global__ void k_cell_scatter(ListArgs a);
 constexpr int CELL_SORT_MAX = 1024;  // sites of one cell k_cell_order sorts in LDS
 constexpr int CELL_ROWS_MAX = 512;   // sites of one cell whose rows one k_list_cells workgroup keeps
 constexpr int CELL_TILE = 320;       // candidate sites one k_list_cells workgroup stages in LDS at a time
+constexpr int SPLIT_CHUNKS = 32;     // 64-entry chunks of a row whose inside flags k_split_rows keeps
 __global__ void k_cell_order(ListArgs a);
-hipError_t launch_list_cells(const ListArgs &a, int halfwidth, int ncell, hipStream_t st);
+hipError_t launch_list_cells(const ListArgs &a, int halfwidth, int ncell, int nw, hipStream_t st);
 __global__ void k_sort_rows(int cap, int n2, const int *nfull, const int *nforce, int *jfull, int *jforce);
 __global__ void k_pme_bin(PmeArgs a);
 __global__ void k_pme_scan(PmeArgs a);

@@ -521,13 +521,13 @@ __global__ void __launch_bounds__(256) k_cell_order(ListArgs a) {
 // row (every j != i, or j < i for the half list), in stencil order; the
 // reference's half-list pairs (j < i, every type pair) are only counted
 // (nver).  Rows of the cell's sites are counted in LDS across the tiles.
-template <int H>
-__global__ void __launch_bounds__(256) k_list_cells(ListArgs a) {
+template <int H, int NW>
+__global__ void __launch_bounds__(64 * NW) k_list_cells(ListArgs a) {
 #pragma clang fp contract(off)
   __shared__ double4 tp[CELL_TILE], tm[CELL_TILE];
   __shared__ int ti[CELL_TILE];
   __shared__ int rowf[CELL_ROWS_MAX], rowr[CELL_ROWS_MAX];
-  __shared__ int red[4];
+  __shared__ int red[NW];
   constexpr int W = 2 * H + 1;
   const int c = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int ni = a.cell_cnt[c], oi = a.cell_start[c];
@@ -568,7 +568,7 @@ __global__ void __launch_bounds__(256) k_list_cells(ListArgs a) {
         ti[e] = a.cell_i[src];
       }
       __syncthreads();
-      for (int s = w; s < ni; s += 4) {
+      for (int s = w; s < ni; s += NW) {
         const double4 pi = a.cell_p[oi + s], mi = a.cell_m[oi + s];
         const int i = a.cell_i[oi + s], typi = (int)mi.w;
         const int eo = a.ex_off[i], en = a.ex_num[i];
@@ -623,14 +623,28 @@ __global__ void __launch_bounds__(256) k_list_cells(ListArgs a) {
   for (int off = 32; off >= 1; off >>= 1) sum += __shfl_xor(sum, off, 64);
   if (lane == 0) red[w] = sum;
   __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(a.overflow + 1, red[0] + red[1] + red[2] + red[3]);
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int k = 0; k < NW; k++) t += red[k];
+    atomicAdd(a.overflow + 1, t);
+  }
 }
 
-hipError_t launch_list_cells(const ListArgs &a, int halfwidth, int ncell, hipStream_t st) {
+// nw wavefronts per cell's workgroup (4, 8 or 16): more of them keep more
+// loads in flight; each site's row is still written by one wavefront, in
+// stencil order.
+template <int NW>
+void launch_list_cells_w(const ListArgs &a, int halfwidth, int ncell, hipStream_t st) {
   if (halfwidth == 2)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_list_cells<2>), dim3(ncell), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_list_cells<2, NW>), dim3(ncell), dim3(64 * NW), 0, st, a);
   else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_list_cells<1>), dim3(ncell), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_list_cells<1, NW>), dim3(ncell), dim3(64 * NW), 0, st, a);
+}
+
+hipError_t launch_list_cells(const ListArgs &a, int halfwidth, int ncell, int nw, hipStream_t st) {
+  if (nw >= 16) launch_list_cells_w<16>(a, halfwidth, ncell, st);
+  else if (nw >= 8) launch_list_cells_w<8>(a, halfwidth, ncell, st);
+  else launch_list_cells_w<4>(a, halfwidth, ncell, st);
   return hipGetLastError();
 }
 
@@ -670,6 +684,14 @@ __global__ void __launch_bounds__(256) k_sort_rows(int cap, int n2, const int *n
   }
 }
 
+// Orders the LDS writes of a wavefront before its other lanes read them
+// (and the reads before the next writes).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Splits every force-list row, stably, into the entries whose pair lies
 // inside its cutoff at the build positions (the centroid reference xmod with
 // beads, bead 1 without), then the rest of the list shell, into out.  The
@@ -678,7 +700,10 @@ __global__ void __launch_bounds__(256) k_sort_rows(int cap, int n2, const int *n
 // cutoff test and the lanes of a wavefront seldom idle.  One wavefront per
 // row.
 __global__ void __launch_bounds__(256) k_split_rows(ListArgs a, int *out) {
-  const int lane = threadIdx.x & 63;
+  // the inside flags of a row's chunks, kept from the counting pass for the
+  // placing pass (rows of up to SPLIT_CHUNKS chunks; longer rows test again)
+  __shared__ uint64_t msk[4][SPLIT_CHUNKS];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int i = (int)(((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   if (i >= a.N) return;  // uniform over the wavefront
   const int cnt = a.nforce[i];
@@ -686,32 +711,33 @@ __global__ void __launch_bounds__(256) k_split_rows(ListArgs a, int *out) {
   int *dst = out + (size_t)i * a.cap;
   const double xi = a.pimd ? a.xm[i] : a.bx[i], yi = a.pimd ? a.ym[i] : a.by[i], zi = a.pimd ? a.zm[i] : a.bz[i];
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  auto inside = [&](int e) {
+    const int j = e & ENTRY_MASK, t = e >> ENTRY_SHIFT;
+    const double xj = a.pimd ? a.xm[j] : a.bx[j], yj = a.pimd ? a.ym[j] : a.by[j],
+                 zj = a.pimd ? a.zm[j] : a.bz[j];
+    double dx, dy, dz;
+    const double r2 = min_image_sep(a.cell, 0, xi, yi, zi, xj, yj, zj, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dx, dy, dz);
+    return r2 < a.pt.cut[t] * a.pt.cut[t];
+  };
   int ninner = 0;
-  for (int pass = 0; pass < 2; pass++) {
-    int pin = 0, pout = ninner;
-    for (int k0 = 0; k0 < cnt; k0 += 64) {
-      const int k = k0 + lane;
-      const bool valid = k < cnt;
-      bool inner = false;
-      int e = 0;
-      if (valid) {
-        e = row[k];
-        const int j = e & ENTRY_MASK, t = e >> ENTRY_SHIFT;
-        const double xj = a.pimd ? a.xm[j] : a.bx[j], yj = a.pimd ? a.ym[j] : a.by[j],
-                     zj = a.pimd ? a.zm[j] : a.bz[j];
-        double dx, dy, dz;
-        const double r2 = min_image_sep(a.cell, 0, xi, yi, zi, xj, yj, zj, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, dx, dy, dz);
-        inner = r2 < a.pt.cut[t] * a.pt.cut[t];
-      }
-      const uint64_t mi = __ballot(inner), mv = __ballot(valid);
-      if (pass == 0) {
-        ninner += __popcll(mi);
-      } else if (valid) {
-        dst[inner ? pin + __popcll(mi & below) : pout + __popcll(mv & ~mi & below)] = e;
-      }
-      pin += __popcll(mi);
-      pout += __popcll(mv & ~mi);
-    }
+  for (int k0 = 0, c = 0; k0 < cnt; k0 += 64, c++) {
+    const int k = k0 + lane;
+    const uint64_t mi = __ballot(k < cnt && inside(row[k]));
+    if (c < SPLIT_CHUNKS && lane == 0) msk[w][c] = mi;
+    ninner += __popcll(mi);
+  }
+  wave_sync();
+  int pin = 0, pout = ninner;
+  for (int k0 = 0, c = 0; k0 < cnt; k0 += 64, c++) {
+    const int k = k0 + lane;
+    const bool valid = k < cnt;
+    const int e = valid ? row[k] : 0;
+    const uint64_t mi = c < SPLIT_CHUNKS ? msk[w][c] : __ballot(valid && inside(e));
+    const uint64_t mv = __ballot(valid);
+    const bool inner = (mi >> lane) & 1ull;
+    if (valid) dst[inner ? pin + __popcll(mi & below) : pout + __popcll(mv & ~mi & below)] = e;
+    pin += __popcll(mi);
+    pout += __popcll(mv & ~mi);
   }
 }
 
@@ -1023,14 +1049,6 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
     dvt += q2 * (dv * sw + v * dsw);
   }
   return dvt;
-}
-
-// Orders the LDS writes of a wavefront before its other lanes read them
-// (and the reads before the next writes).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Mixed-precision pair kernel over the full list.  A wavefront owns site i;
