@@ -441,8 +441,9 @@ void pmx_batch_destroy(pmx_batch *batch);
  * box of one molecule template) the molecule-cluster form of the pair
  * kernel against the per-site one, the PME tile sizes and the
  * list build (plain scan or link cells), keeps the fastest of each, then
- * times whole calls with PME on a second stream beside the pair kernel and
- * without, keeps the faster, and ends with a force call using them.  choice (may be NULL) receives
+ * times whole calls with PME on a second stream beside the pair kernel, for
+ * several pair wavefront counts, against the best without, keeps the
+ * fastest, and ends with a force call using them.  choice (may be NULL) receives
  * [tile a, tile b, tile c, pair wavefronts, list build 1 plain / 2 cells].
  * Results are unchanged by the shapes up to the order of sums. */
 int pmx_autotune(pmx_ctx *ctx, unsigned flags, int reps, int *choice);

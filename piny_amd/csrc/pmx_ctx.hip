@@ -2574,13 +2574,30 @@ int pmx_autotune(pmx_ctx *c, unsigned flags, int reps, int *choice) {
     }
     c->list_mode = tc < tb ? 2 : 1;
   }
-  // PME on the second stream (stage 4, the whole call)
+  // PME on the second stream (stage 4, the whole call), with the pair
+  // kernels' wavefronts again: beside PME fewer of them leave the k-space
+  // kernels room on the CUs (4096 q-TIP4P/F at P = 32: 0.717 ms with 3072
+  // wavefronts and PME beside, 0.745 ms with the 8192 best alone)
   if (c->pme) {
-    double t0, t1;
-    if ((rc = stage_ms(4, flags, &t0))) return rc;
+    double tbest;
+    if ((rc = stage_ms(4, flags, &tbest))) return rc;
+    const int base = c->pair_waves;
+    int best_w = base, best_ov = 0;
+    const int cand[] = {base, 2048, 3072, 4096, 6144, 8192};
     c->overlap = 1;
-    if ((rc = stage_ms(4, flags, &t1))) return rc;
-    c->overlap = t1 < t0 ? 1 : 0;
+    for (int w : cand) {
+      if (w > PAIR_MAX_WAVES || (w > c->s.N && w != base)) continue;
+      c->pair_waves = w;
+      double t;
+      if ((rc = stage_ms(4, flags, &t))) return rc;
+      if (t < tbest) {
+        tbest = t;
+        best_w = w;
+        best_ov = 1;
+      }
+    }
+    c->overlap = best_ov;
+    c->pair_waves = best_w;
   }
   c->epoch++;
   if ((rc = pmx_energy_control(c, flags | PMX_EVAL_FORCE_LIST, &e))) return rc;
