@@ -57,6 +57,7 @@ struct pmx_ctx {
   bool keep_fxt = false;       // this sequence must not overwrite fxt
   float4 *ctab = nullptr;      // mixed precision: erfc(alp r)/r as cubic pieces [CT_N]
   float4 *ftab = nullptr;      // mixed precision: the switched pair functions [nftab][CT_N] (null if too many)
+  float4 *cswtab = nullptr;    // mixed precision: the switched, shifted erfc term [CT_N] when every type pair shares it
   int nftab = 0;
   std::vector<int2> ftab_off;  // per type pair: (erfc table, LJ table or -1), offsets in pieces
   float ct_r0 = 0.f, ct_idr = 0.f;
@@ -706,7 +707,14 @@ int build_pair_tables(pmx_ctx *c) {
   }
   dfree(c, c->ctab);
   dfree(c, c->ftab);
+  dfree(c, c->cswtab);
   int rc = upload(c, &c->ctab, ct.data(), ct.size());
+  // One erfc function for every type pair (one cutoff and shift): the mixed
+  // pair kernel reads it switched and shifted from its table and evaluates
+  // the switch only for the LJ term.
+  bool one_c = s.npair > 0;
+  for (int t = 1; t < s.npair; t++) one_c = one_c && c->ftab_off[t].x == c->ftab_off[0].x;
+  if (!rc && one_c) rc = upload(c, &c->cswtab, ft.data() + c->ftab_off[0].x, (size_t)CT_N);
   // The whole-function tables are built but used only on request
   // (PMX_MIXED_TAB=1): on the 4096-molecule, 32-bead box the pair kernel
   // takes 0.656 ms with them against 0.373 ms with LJ and the switch
@@ -968,7 +976,13 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(upload(c, &c->pair_pp_res, ppr.data(), ppr.size()));
     TRY(dalloc(c, &c->site, N));
     c->pair_mode = (c->par.precision == PMX_PREC_MIXED) ? 2 : (c->par.pair_eval == PMX_PAIR_DIRECT ? 1 : 0);
-    if (c->pair_mode == 2 && P > 1) TRY(dalloc(c, &c->disp, NP));
+    if (c->pair_mode == 2 && P > 1) {
+      if (NP * sizeof(float4) > (size_t)UINT32_MAX) {  // the pair kernel reads them at 32-bit byte offsets
+        set_err(c, "%d sites x %d beads exceed the 2^28 bead displacements of the mixed precision", N, P);
+        return PMX_ERR_RANGE;
+      }
+      TRY(dalloc(c, &c->disp, NP));
+    }
     if (c->pair_mode == 2) TRY(build_pair_tables(c));
     if (P > 1) TRY(dalloc(c, &c->umax, (size_t)N));
   }
@@ -1676,6 +1690,7 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     a.two_alpha2 = 2.0 * s.alp_ewd * s.alp_ewd;
     a.spalp = 2.0 * s.alp_ewd / sqrt(M_PI);
     a.ctab = c->ctab;
+    a.cswtab = c->cswtab;
     a.ct_r0 = c->ct_r0;
     a.ct_idr = c->ct_idr;
     a.ftab = c->ftab;
@@ -3237,6 +3252,7 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
     a.two_alpha2 = 2.0 * s.alp_ewd * s.alp_ewd;
     a.spalp = 2.0 * s.alp_ewd / sqrt(M_PI);
     a.ctab = c->ctab;
+    a.cswtab = c->cswtab;
     a.ct_r0 = c->ct_r0;
     a.ct_idr = c->ct_idr;
     a.ftab = c->ftab;

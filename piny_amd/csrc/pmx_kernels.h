@@ -71,6 +71,9 @@ struct PairArgs {
   // CT_N intervals from ct_r0 with 1/width ct_idr: (c0, c1, c2, c3) of
   // c0 + c1 t + c2 t^2 + c3 t^3, t in [0, 1] across the interval
   const float4 *ctab;
+  // the same erfc term switched and shifted as the type pairs' tables hold
+  // it (build_pair_tables), when every type pair shares one; else null
+  const float4 *cswtab;
   float ct_r0, ct_idr;
   // the switched, shifted pair functions whole (erfc term per unit charge
   // product, LJ term) on the same grid: nftab tables of CT_N pieces, and per

@@ -1011,11 +1011,14 @@ __global__ void __launch_bounds__(256) k_pair_half(PairArgs a) {
 
 // The FP32 pair term of pair_direct with the type pair's constants in
 // registers: A = (cut^2, cut, C12, C6), B = (LJ shift, Coulomb shift, LJ
-// flag, 0), read from LDS as two 16-byte loads per pair.
+// flag, 0), read from LDS as two 16-byte loads per pair.  ct holds the erfc
+// term's cubic pieces: with CSW switched and shifted already (the switch is
+// then evaluated for the LJ term only), else erfc(alp r)/r itself.
+template <bool CSW>
 __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4 &A, const float4 &B, float q2,
                                                  float r, float rinv, float &vlj, float &vc, const float4 *ct) {
   float sw = 1.f, dsw = 0.f;
-  if (a.swit) {
+  auto switch_at = [&]() {
     const float rheal = (float)a.rheal, irheal = (float)a.irheal;
     const float rs = r - A.y + rheal;
     if (rs > 0.f) {  // inside the switching shell (below it sw = 1, dsw = 0)
@@ -1024,12 +1027,14 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
       sw = 1.f + rp * rp * (2.f * rp - 3.f);
       dsw = -6.f * (rp * irheal) * (rp - 1.f) * rinv;
     }
-  }
+  };
+  if (!CSW && a.swit) switch_at();
   const float r2i = rinv * rinv;
   float dvt = 0.f;
   vlj = 0.f;
   vc = 0.f;
   if (B.z != 0.f) {  // r^-6 (C12 r^-6 - C6) and 6 r^-6 (2 C12 r^-6 - C6) / r^2
+    if (CSW && a.swit) switch_at();
     const float r6i = r2i * r2i * r2i, t12 = A.z * r6i;
     const float v = r6i * (t12 - A.w);
     const float dv = (6.f * r6i) * (2.f * t12 - A.w) * r2i;
@@ -1037,16 +1042,26 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
     dvt = dv * sw + v * dsw;
   }
   if (q2 != 0.f) {
-    // erfc(alp r)/r and its r derivative from the cubic piece of the table
+    // the erfc term and its r derivative from the cubic piece of the table
     // u < 0 (r below the table) converts to k = 0 and extrapolates the first piece
     const float u = r * a.ct_idr - a.ct_r0 * a.ct_idr;
     const unsigned k = min(__float2uint_rz(u), (unsigned)(CT_N - 1));
     const float t = u - (float)k;
     const float4 cc = ct[k];
-    const float v = cc.x + t * (cc.y + t * (cc.z + t * cc.w));
-    const float dv = -(cc.y + t * (2.f * cc.z + 3.f * t * cc.w)) * a.ct_idr * rinv;  // -(1/r) dV/dr
-    vc = q2 * (v * sw - B.y);
-    dvt += q2 * (dv * sw + v * dsw);
+    // value and t derivative by one Horner scheme
+    const float p1 = fmaf(cc.w, t, cc.z);
+    const float d1 = fmaf(cc.w, t, p1);
+    const float p2 = fmaf(p1, t, cc.y);
+    const float d2 = fmaf(d1, t, p2);
+    const float v = fmaf(p2, t, cc.x);
+    const float dv = -(d2 * a.ct_idr) * rinv;  // -(1/r) dV/dr
+    if (CSW) {
+      vc = q2 * v;
+      dvt += q2 * dv;
+    } else {
+      vc = q2 * (v * sw - B.y);
+      dvt += q2 * (dv * sw + v * dsw);
+    }
   }
   return dvt;
 }
@@ -1063,19 +1078,27 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
 // Energies, virial and forces are summed in FP32 within a chunk and in FP64
 // across chunks and lanes.
 //
-// With TAB the LJ and erfc terms come whole from the switched, shifted
-// function tables (build_pair_tables): per (pair, bead) one table index and
-// one cubic piece per term, no switch arithmetic and no branch on the
-// switching shell.  Without TAB (more distinct functions than MAX_FTAB
-// tables) pair_direct_f32 evaluates LJ and the switch directly with the
-// erfc table.
-template <int B, bool PIMD, bool TAB, bool RESPA>
+// TM selects how the terms are formed.  TM = 1: the LJ and erfc terms come
+// whole from the switched, shifted function tables (build_pair_tables): per
+// (pair, bead) one table index and one cubic piece per term, no switch
+// arithmetic.  TM = 2 (every type pair with the same erfc function): the erfc
+// term from its switched, shifted table, LJ and its switch evaluated
+// directly (pair_direct_f32<true>).  TM = 0: LJ, the erfc table and the
+// switch combined in pair_direct_f32<false>.
+template <int B, bool PIMD, int TM, bool RESPA>
 __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
+  constexpr bool TAB = TM == 1;
+  constexpr int S = 64 / B;
+  // the pair loop loads displacements up to 4 S entries past a chunk's end
+  constexpr int SPAD = 64 + 4 * S;
   __shared__ float4 sppA[MAX_PAIRS], sppB[MAX_PAIRS];  // pair_direct_f32's constants per type pair
   __shared__ int4 stt[MAX_PAIRS];  // (cut^2 as bits, TAB: erfc table, LJ table or -1, culling radius as bits)
   extern __shared__ float4 sct[];                       // the tables: TAB nftab * CT_N, else CT_N pieces
   __shared__ float4 sdq[4][64];
-  __shared__ int2 sent[4][64];  // (j * P, type pair) of the staged entries
+  __shared__ int styp[4][64];  // type pair of the staged entries
+  // byte offset of site j's first bead displacement in u per staged entry,
+  // 0 past the chunk's end (a valid address whose value goes unused)
+  __shared__ unsigned soff[4][PIMD ? SPAD : 1];
   for (int t = threadIdx.x; t < a.npair; t += blockDim.x) {
     const double *pp = a.pp + (size_t)t * PP_NUM;
     sppA[t] = make_float4((float)pp[PP_CUT2], (float)pp[PP_CUT], (float)pp[PP_C12], (float)pp[PP_C6]);
@@ -1084,12 +1107,13 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
                        __float_as_int((float)pp[PP_CULL]));
   }
   const int ntab = TAB ? a.nftab * CT_N : CT_N;
-  const float4 *src = TAB ? a.ftab : a.ctab;
+  const float4 *src = TAB ? a.ftab : (TM == 2 ? a.cswtab : a.ctab);
   for (int k = threadIdx.x; k < ntab; k += blockDim.x) sct[k] = src[k];
   __syncthreads();
   const float r0i = a.ct_r0 * a.ct_idr;
-  constexpr int S = 64 / B;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (PIMD)
+    for (int k = 64 + lane; k < SPAD; k += 64) soff[w][k] = 0u;
   const int slot = lane / B;
   const int bead = (lane % B) + blockIdx.y * B;
   const bool bact = bead < a.P;
@@ -1134,22 +1158,24 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         q2 = si.w * sj.w;
       }
       const uint64_t mk = __ballot(keep);
+      const int nk = __popcll(mk);
       if (keep) {
         const int pos = __popcll(mk & below);
         sdq[w][pos] = make_float4((float)dx, (float)dy, (float)dz, (float)q2);
-        sent[w][pos] = make_int2((e & ENTRY_MASK) * P, e >> ENTRY_SHIFT);
+        styp[w][pos] = e >> ENTRY_SHIFT;
+        if (PIMD) soff[w][pos] = (unsigned)(e & ENTRY_MASK) * (unsigned)P * (unsigned)sizeof(float4);
       }
+      if (PIMD && lane >= nk) soff[w][lane] = 0u;
       wave_sync();
-      const int nk = __popcll(mk);
       float cvlj = 0.f, cvc = 0.f, c11 = 0.f, c22 = 0.f, c33 = 0.f, c12 = 0.f, c13 = 0.f, c23 = 0.f;
       float cfx = 0.f, cfy = 0.f, cfz = 0.f;
       int cn = 0;
-      const int bsel = bact ? bead : 0;
+      const unsigned boff = (unsigned)(bact ? bead : 0) * (unsigned)sizeof(float4);
       // one pair of the chunk for this lane's bead, with the displacement uj
       // of site j
       auto pair = [&](int p, const float4 &uj) {
         const float4 d = sdq[w][p];
-        const int t = sent[w][p].y;
+        const int t = styp[w][p];
         float dx = d.x, dy = d.y, dz = d.z;
         if (PIMD) {
           dx += ui.x - uj.x;
@@ -1184,7 +1210,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           const float4 A = sppA[t], B2 = sppB[t];
           if (!bact || !(r2 < A.x)) return;
           const float rinv = __frsqrt_rn(r2);
-          dvd = pair_direct_f32(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c, sct);
+          dvd = pair_direct_f32<TM == 2>(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c, sct);
           if constexpr (RESPA) {  // respa_weight in FP32; B2.w is the RESPA cutoff
             float fm = (float)a.w_full;
             if (a.respa != RESPA_FULL) {
@@ -1210,17 +1236,15 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         c23 += dy * fz1;
         cn++;
       };
-      // the displacement of pair p into u (left as it is past the chunk's
-      // end, where it is not used)
-      // The load is issued on every path (past the chunk's end it reads this
-      // bead of site 0): with the load under a branch the compiler waits for
-      // every outstanding load before each pair (vmcnt(0)), including the
-      // prefetches, and the pair loop runs at the memory latency.
+      // the displacement of pair p into u (p < SPAD; past the chunk's end
+      // it reads this bead of site 0, unused)
+      // The load is issued on every path: with the load under a branch the
+      // compiler waits for every outstanding load before each pair
+      // (vmcnt(0)), including the prefetches, and the pair loop runs at the
+      // memory latency.  A 32-bit byte offset from the kernel argument lets
+      // the load take the scalar base address.
       auto fetch = [&](float4 &u, int p) {
-        if (PIMD) {
-          const int e = sent[w][min(p, 63)].x;
-          u = a.u[(unsigned)((p < nk ? e : 0) + bsel)];
-        }
+        if (PIMD) u = *(const float4 *)((const char *)a.u + (soff[w][p] + boff));
       };
       // four pairs per iteration, each displacement loaded a whole iteration
       // ahead into a register of its own: no register copies, which would
@@ -2616,6 +2640,20 @@ int pair_grid_x(int N, int max_waves) {
   return (waves + 3) / 4;
 }
 
+// k_pair_mixed with the erfc term formed as TM says, with or without beads
+// and RESPA weights
+template <int B, int TM>
+void launch_pair_mixed_tm(const PairArgs &a, dim3 grid, size_t lds, hipStream_t st) {
+  const dim3 block(256);
+  if (a.respa) {
+    if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, TM, true>), grid, block, lds, st, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, TM, true>), grid, block, lds, st, a);
+  } else {
+    if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, TM, false>), grid, block, lds, st, a);
+    else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, TM, false>), grid, block, lds, st, a);
+  }
+}
+
 template <int B>
 void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_t st) {
   const dim3 block(256);
@@ -2625,22 +2663,18 @@ void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_
     else if (a.ftab && !a.respa) {
       const size_t lds = (size_t)a.nftab * CT_N * sizeof(float4);
       if (a.pimd) {
-        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, true, true, false>,
+        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, true, 1, false>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, true, false>), grid, block, lds, st, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, 1, false>), grid, block, lds, st, a);
       } else {
-        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, false, true, false>,
+        (void)hipFuncSetAttribute((const void *)k_pair_mixed<B, false, 1, false>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, true, false>), grid, block, lds, st, a);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, 1, false>), grid, block, lds, st, a);
       }
-    } else if (a.respa) {
-      const size_t lds = (size_t)CT_N * sizeof(float4);
-      if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, false, true>), grid, block, lds, st, a);
-      else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, false, true>), grid, block, lds, st, a);
     } else {
       const size_t lds = (size_t)CT_N * sizeof(float4);
-      if (a.pimd) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, true, false, false>), grid, block, lds, st, a);
-      else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, false, false>), grid, block, lds, st, a);
+      if (a.cswtab) launch_pair_mixed_tm<B, 2>(a, grid, lds, st);
+      else launch_pair_mixed_tm<B, 0>(a, grid, lds, st);
     }
   } else {
     if (mode == 0) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_half<B, 0>), grid, block, 0, st, a);

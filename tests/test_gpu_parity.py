@@ -3,6 +3,7 @@
 Bars: Verlet lists, k-vectors, grid sizes and spline tables bit-exact;
 energies, forces and virials in FP64 within 1e-9 relative (the north-star
 bound for the FP64 path is a force RMS error below 1e-5 relative)."""
+import dataclasses
 import sys
 
 import numpy as np
@@ -91,6 +92,34 @@ def test_pair_direct_and_mixed(case, mode):
     assert rms < ftol
     pv_scale = np.max(np.abs(eo["pvten"]))
     assert np.max(np.abs(eg["pvten"] - eo["pvten"])) <= vtol * pv_scale
+
+
+@pytest.mark.parametrize("case", ["spc64", "tip64_p4"])
+def test_mixed_cutoff_per_type_pair(case):
+    """Type pairs with cutoffs of their own: no one switched erfc table
+    serves them all, so the mixed pair kernel switches the erfc term itself
+    (with one cutoff it reads the switched term from its table, as every
+    other mixed test does).  Against the oracle on the same system."""
+    c = cases.get(case)
+    cut = np.array(c.system.inter_cutoff, dtype=float).copy()
+    cut[0] *= 0.92
+    s = dataclasses.replace(c.system, inter_cutoff=cut)
+    flags = abi.PMX_EVAL_PIMD if s.pi_beads > 1 else 0
+    print(f"[case {case} with cutoffs {cut}]", flush=True)
+    o = Oracle(s, c.x, c.y, c.z)
+    eo, fo, _ = o.energy_control(c.x, c.y, c.z, flags)
+    g = Engine(s, c.x, c.y, c.z, precision=abi.PMX_PREC_MIXED)
+    g.set_positions(c.x, c.y, c.z)
+    eg = g.energy_control(flags)
+    fg = g.forces()
+    scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
+    for k in abi.ENERGY_KEYS:
+        assert abs(eg[k] - eo[k]) <= MIXED_ETOL * max(abs(eo[k]), scale), (k, eg[k], eo[k])
+    rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
+    print(f"  force rms rel err {rms:.3e}", flush=True)
+    assert rms < MIXED_FTOL
+    pv_scale = np.max(np.abs(eo["pvten"]))
+    assert np.max(np.abs(eg["pvten"] - eo["pvten"])) <= MIXED_VTOL * pv_scale
 
 
 @pytest.mark.parametrize("mode", ["table", "mixed"])
