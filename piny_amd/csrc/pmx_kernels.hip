@@ -1013,7 +1013,9 @@ __global__ void __launch_bounds__(256) k_pair_half(PairArgs a) {
 // registers: A = (cut^2, cut, C12, C6), B = (LJ shift, Coulomb shift, LJ
 // flag, 0), read from LDS as two 16-byte loads per pair.  ct holds the erfc
 // term's cubic pieces: with CSW switched and shifted already (the switch is
-// then evaluated for the LJ term only), else erfc(alp r)/r itself.
+// then evaluated for the LJ term only), else erfc(alp r)/r itself.  The
+// energies are added to vlj and vc; the erfc term is formed for every pair
+// (q2 = 0 adds zeros), so only the LJ term takes a branch.
 template <bool CSW>
 __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4 &A, const float4 &B, float q2,
                                                  float r, float rinv, float &vlj, float &vc, const float4 *ct) {
@@ -1029,19 +1031,8 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
     }
   };
   if (!CSW && a.swit) switch_at();
-  const float r2i = rinv * rinv;
-  float dvt = 0.f;
-  vlj = 0.f;
-  vc = 0.f;
-  if (B.z != 0.f) {  // r^-6 (C12 r^-6 - C6) and 6 r^-6 (2 C12 r^-6 - C6) / r^2
-    if (CSW && a.swit) switch_at();
-    const float r6i = r2i * r2i * r2i, t12 = A.z * r6i;
-    const float v = r6i * (t12 - A.w);
-    const float dv = (6.f * r6i) * (2.f * t12 - A.w) * r2i;
-    vlj = v * sw - B.x;
-    dvt = dv * sw + v * dsw;
-  }
-  if (q2 != 0.f) {
+  float dvt;
+  {
     // the erfc term and its r derivative from the cubic piece of the table
     // u < 0 (r below the table) converts to k = 0 and extrapolates the first piece
     const float u = r * a.ct_idr - a.ct_r0 * a.ct_idr;
@@ -1056,12 +1047,21 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
     const float v = fmaf(p2, t, cc.x);
     const float dv = -(d2 * a.ct_idr) * rinv;  // -(1/r) dV/dr
     if (CSW) {
-      vc = q2 * v;
-      dvt += q2 * dv;
+      vc += q2 * v;
+      dvt = q2 * dv;
     } else {
-      vc = q2 * (v * sw - B.y);
-      dvt += q2 * (dv * sw + v * dsw);
+      vc += q2 * (v * sw - B.y);
+      dvt = q2 * (dv * sw + v * dsw);
     }
+  }
+  if (B.z != 0.f) {  // r^-6 (C12 r^-6 - C6) and 6 r^-6 (2 C12 r^-6 - C6) / r^2
+    if (CSW && a.swit) switch_at();
+    const float r2i = rinv * rinv;
+    const float r6i = r2i * r2i * r2i, t12 = A.z * r6i;
+    const float v = r6i * (t12 - A.w);
+    const float dv = (6.f * r6i) * (2.f * t12 - A.w) * r2i;
+    vlj += v * sw - B.x;
+    dvt += dv * sw + v * dsw;
   }
   return dvt;
 }
@@ -1183,7 +1183,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           dz += ui.z - uj.z;
         }
         const float r2 = dx * dx + dy * dy + dz * dz;
-        float ev_lj, ev_c, dvd;
+        float dvd;
         if constexpr (TAB) {
           const int4 T = stt[t];
           if (!bact || !(r2 < __int_as_float(T.x))) return;
@@ -1193,16 +1193,14 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           const unsigned k = min(__float2uint_rz(u), (unsigned)(CT_N - 1));
           const float tt = u - (float)k;
           float dv = 0.f;  // dV/dt of the pair
-          ev_c = 0.f;
-          ev_lj = 0.f;
           if (d.w != 0.f) {
             const float4 cc = sct[T.y + k];
-            ev_c = d.w * (cc.x + tt * (cc.y + tt * (cc.z + tt * cc.w)));
+            cvc += d.w * (cc.x + tt * (cc.y + tt * (cc.z + tt * cc.w)));
             dv = d.w * (cc.y + tt * (2.f * cc.z + 3.f * tt * cc.w));
           }
           if (T.z >= 0) {
             const float4 cl = sct[T.z + k];
-            ev_lj = cl.x + tt * (cl.y + tt * (cl.z + tt * cl.w));
+            cvlj += cl.x + tt * (cl.y + tt * (cl.z + tt * cl.w));
             dv += cl.y + tt * (2.f * cl.z + 3.f * tt * cl.w);
           }
           dvd = -(dv * a.ct_idr) * rinv;  // -(1/r) dV/dr
@@ -1210,7 +1208,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           const float4 A = sppA[t], B2 = sppB[t];
           if (!bact || !(r2 < A.x)) return;
           const float rinv = __frsqrt_rn(r2);
-          dvd = pair_direct_f32<TM == 2>(a, A, B2, d.w, r2 * rinv, rinv, ev_lj, ev_c, sct);
+          dvd = pair_direct_f32<TM == 2>(a, A, B2, d.w, r2 * rinv, rinv, cvlj, cvc, sct);
           if constexpr (RESPA) {  // respa_weight in FP32; B2.w is the RESPA cutoff
             float fm = (float)a.w_full;
             if (a.respa != RESPA_FULL) {
@@ -1223,8 +1221,6 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           }
         }
         const float fx1 = dx * dvd, fy1 = dy * dvd, fz1 = dz * dvd;
-        cvlj += ev_lj;
-        cvc += ev_c;
         cfx += fx1;
         cfy += fy1;
         cfz += fz1;
