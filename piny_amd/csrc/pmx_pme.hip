@@ -713,6 +713,70 @@ __device__ void lds_fft(typename Cx<T>::C *buf, int K, int nlines, int stride, i
   }
 }
 
+// stockham_pass with every size a compile-time constant (transform length
+// K, NL lines, element stride ST, line distance DS, sub-transform length NS,
+// NT lanes): the same loads, twiddles, butterflies and stores in the same
+// order, with the index arithmetic folded to shifts and constants.  NS = 1
+// multiplies by no twiddle (all of them are 1 there, also in stockham_pass,
+// exactly).
+template <int R, typename T, int NT, int K, int NL, int ST, int DS, int NS>
+__device__ __forceinline__ void stockham_pass_c(typename Cx<T>::C *buf, int sign,
+                                                const typename Cx<T>::C *__restrict__ tw) {
+  using C = typename Cx<T>::C;
+  constexpr int KR = K / R, NB = NL * KR, MAXB = (NB + NT - 1) / NT;
+  constexpr bool LINES_FAST = ST != 1;
+  constexpr int KIN = KR * ST, KOUT = NS * ST, TSTEP = K / (NS * R);
+  const int tid = threadIdx.x;
+  C v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int bi = tid + q * NT;
+    if (NB % NT == 0 || bi < NB) {
+      const int l = LINES_FAST ? bi % NL : bi / KR, j = LINES_FAST ? bi / NL : bi % KR;
+      const C *p = buf + (l * DS + j * ST);
+#pragma unroll
+      for (int k = 0; k < R; k++) v[q][k] = p[k * KIN];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int bi = tid + q * NT;
+    if (NB % NT == 0 || bi < NB) {
+      const int l = LINES_FAST ? bi % NL : bi / KR, j = LINES_FAST ? bi / NL : bi % KR;
+      const int jq = j / NS, jm = j % NS;
+      if (NS > 1) {
+        const int jt = jm * TSTEP;
+#pragma unroll
+        for (int k = 1; k < R; k++) {  // twiddle exp(sign 2 pi i jm k / (NS R))
+          const C t = tw[jt * k];
+          const T wr = t.x, wi = sign < 0 ? t.y : -t.y;
+          const C a = v[q][k];
+          v[q][k] = Cx<T>::make(a.x * wr - a.y * wi, a.x * wi + a.y * wr);
+        }
+      }
+      dft_small<R, T>(v[q], sign);
+      C *p = buf + (l * DS + (jq * NS * R + jm) * ST);
+#pragma unroll
+      for (int k = 0; k < R; k++) p[k * KOUT] = v[q][k];
+    }
+  }
+  __syncthreads();
+}
+
+// A 64-point transform of NL lines as two radix-8 passes with constant
+// sizes: what lds_fft does for a plan of radices (8, 8).
+template <typename T, int NT, int NL, int ST, int DS>
+__device__ __forceinline__ void lds_fft64(typename Cx<T>::C *buf, int sign, const typename Cx<T>::C *tw) {
+  stockham_pass_c<8, T, NT, 64, NL, ST, DS, 1>(buf, sign, tw);
+  stockham_pass_c<8, T, NT, 64, NL, ST, DS, 8>(buf, sign, tw);
+}
+
+// the plan of axis d is radix 8 twice (a length of 64; FP32 grids)
+__device__ __forceinline__ bool plan_88(const FftArgs &a, int d) {
+  return a.npass[d] == 2 && a.radix[d][0] == 8 && a.radix[d][1] == 8;
+}
+
 template <typename T>
 __device__ __forceinline__ const typename Cx<T>::C *twiddles(const FftArgs &a, int d) {
   if constexpr (sizeof(T) == 8)
@@ -824,8 +888,13 @@ __global__ void __launch_bounds__(NT) k_fft_fwd_ab(FftArgs a) {
   stage_twiddles<T>(a, 1, twb);
   fold_plane<T>(a, c, b, sm);
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, -1, twa, a.npass[0], a.radix[0]);
-  lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, -1, twb, a.npass[1], a.radix[1]);
+  if (sizeof(T) == 4 && NT == 256 && plan_88(a, 0) && plan_88(a, 1)) {  // 64 x 64 planes
+    lds_fft64<T, NT, 64, 1, 64>(sm, -1, twa);
+    lds_fft64<T, NT, 33, 64, 1>(sm, -1, twb);
+  } else {
+    lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, -1, twa, a.npass[0], a.radix[0]);
+    lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, -1, twb, a.npass[1], a.radix[1]);
+  }
   C *sp = (C *)a.spec + (size_t)b * a.sstride + (size_t)c * Kb * Ha;
   const FastDiv dh(Ha);
   for (int w = threadIdx.x; w < Kb * Ha; w += blockDim.x) {
@@ -866,7 +935,9 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, -1, twc, a.npass[2], a.radix[2]);
+  const bool c64 = sizeof(T) == 4 && NT == 256 && plan_88(a, 0) && plan_88(a, 2);  // Ha = 33, Kc = 64
+  if (c64) lds_fft64<T, NT, 33, 33, 1>(sm, -1, twc);
+  else lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, -1, twc, a.npass[2], a.radix[2]);
   const double *hi = a.cell.hi;
   const double tpi = 2.0 * M_PI, falp2 = a.falp2;
   const int kb = bb <= Kb / 2 ? bb : bb - Kb;
@@ -904,7 +975,8 @@ __global__ void __launch_bounds__(NT) k_fft_c_conv(FftArgs a) {
     block_partials<FFT_NV>(vals, 1, b, a.P, a.part, bb, a.K[1]);
   }
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, +1, twc, a.npass[2], a.radix[2]);
+  if (c64) lds_fft64<T, NT, 33, 33, 1>(sm, +1, twc);
+  else lds_fft<T, NT, MAXP>(sm, Kc, Ha, Ha, 1, +1, twc, a.npass[2], a.radix[2]);
   for (int w = threadIdx.x; w < Kc * Ha; w += blockDim.x) {
     const int cc = dh.q(w), aa = w - cc * Ha;
     sp[((size_t)cc * Kb + bb) * Ha + aa] = sm[w];
@@ -940,7 +1012,9 @@ __global__ void __launch_bounds__(NT) k_fft_inv_ba(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, +1, twb, a.npass[1], a.radix[1]);
+  const bool ab64 = sizeof(T) == 4 && NT == 256 && plan_88(a, 0) && plan_88(a, 1);  // 64 x 64 planes
+  if (ab64) lds_fft64<T, NT, 33, 64, 1>(sm, +1, twb);
+  else lds_fft<T, NT, MAXP>(sm, Kb, Ha, Ka, 1, +1, twb, a.npass[1], a.radix[1]);
   // Hermitian completion along a: X[Ka - a'] = conj(X[a'])
   for (int w = threadIdx.x; w < Kb * Ka; w += blockDim.x) {
     const int bb = dka.q(w), aa = w - bb * Ka;
@@ -950,7 +1024,8 @@ __global__ void __launch_bounds__(NT) k_fft_inv_ba(FftArgs a) {
     }
   }
   __syncthreads();
-  lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, +1, twa, a.npass[0], a.radix[0]);
+  if (ab64) lds_fft64<T, NT, 64, 1, 64>(sm, +1, twa);
+  else lds_fft<T, NT, MAXP>(sm, Ka, Kb, 1, Ka, +1, twa, a.npass[0], a.radix[0]);
   T *g = (T *)a.grid + (size_t)b * a.gstride + (size_t)c * Ka * Kb;
   for (int w = threadIdx.x; w < Ka * Kb; w += blockDim.x) g[w] = sm[w].x;
 }
