@@ -254,10 +254,15 @@ __device__ __forceinline__ void load_halo(const PmeArgs &a, int b, const int (&o
       if (k < hvol) {
         const int q0 = (int)__umulhi((unsigned)k, a.mH0), lc = (int)__umulhi((unsigned)q0, a.mH1);
         const int la = k - q0 * H0, lb = q0 - lc * H1;
+        // org < K and l < H <= K + n - 1, so org + l < 3 K (n - 1 <= K on
+        // every grid): two conditional subtractions wrap it into the grid
         int ga = org[0] + la, gb = org[1] + lb, gc = org[2] + lc;
-        while (ga >= Ka) ga -= Ka;
-        while (gb >= Kb) gb -= Kb;
-        while (gc >= Kc) gc -= Kc;
+        ga -= ga >= Ka ? Ka : 0;
+        gb -= gb >= Kb ? Kb : 0;
+        gc -= gc >= Kc ? Kc : 0;
+        ga -= ga >= Ka ? Ka : 0;
+        gb -= gb >= Kb ? Kb : 0;
+        gc -= gc >= Kc ? Kc : 0;
         v[u] = g[((size_t)gc * Kb + gb) * Ka + ga];
       }
     }
@@ -269,10 +274,19 @@ __device__ __forceinline__ void load_halo(const PmeArgs &a, int b, const int (&o
   }
 }
 
+// lanes per charge in the gather: with FP32 grids two, each summing half
+// of the support planes (twice the wavefronts on the LDS the halo box
+// allows); with FP64 one
+template <typename T>
+constexpr int gather_lanes() {
+  return sizeof(T) == 4 ? 2 : 1;
+}
+
 // one workgroup per (tile, bead): the potential on the halo box into LDS,
-// then one lane per charge (ewald3d_pme.c:534-560)
+// then L lanes per charge (ewald3d_pme.c:534-560); lane part sums the planes
+// mc = part * NI / 2 ... (part + 1) * NI / 2 - 1 and the pair adds its sums
 template <int NI, typename T>
-__global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
+__global__ void __launch_bounds__(gather_lanes<T>() * PME_BLOCK) k_pme_gather(PmeArgs a) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   T *box = (T *)lds_raw;
   int tl, b;
@@ -288,16 +302,36 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
   load_halo<T>(a, b, org, box);
   __syncthreads();
   const double *hi = a.cell.hi;
-  for (int c = tid; c < cnt; c += blockDim.x) {
+  constexpr int L = gather_lanes<T>();
+  constexpr int HP = L == 1 ? 0 : NI / 2;  // planes of lane 0 when split
+  constexpr int KP = NI - HP;              // planes per lane (lane 0 pads odd orders with zero weights)
+  const int part = L == 1 ? 0 : (tid & 1);
+  for (int c = tid / L; c < cnt; c += blockDim.x / L) {
     const int it = a.sorted[lo + c];
     const int ic = it % a.nchrg, bead = it / a.nchrg;
     const int i = a.ichrg[ic];
     T w[3][NI], dw[3][NI];
     int l[3];
     item_weights<NI, true, T>(a, it, org, w, dw, l);
+    // this lane's planes: weights picked once per charge
+    T w2[KP], dw2[KP];
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+      if (L == 1) {
+        w2[k] = w[2][k];
+        dw2[k] = dw[2][k];
+      } else {
+        const T a0 = k < HP ? w[2][k] : (T)0, d0 = k < HP ? dw[2][k] : (T)0;
+        w2[k] = part ? w[2][HP + k] : a0;
+        dw2[k] = part ? dw[2][HP + k] : d0;
+      }
+    }
+    const int mc0 = part * HP;
     // row sums in T, accumulated over rows in FP64
     double da = 0.0, db = 0.0, dc = 0.0;
-    for (int mc = 0; mc < NI; mc++) {
+#pragma unroll
+    for (int k = 0; k < KP; k++) {
+      const int mc = min(mc0 + k, NI - 1);
 #pragma unroll
       for (int mb = 0; mb < NI; mb++) {
         const T *row = box + (l[2] + mc) * H01 + (l[1] + mb) * H0 + l[0];
@@ -308,20 +342,27 @@ __global__ void __launch_bounds__(PME_BLOCK) k_pme_gather(PmeArgs a) {
           r0 += dw[0][ma] * phi;
           r1 += w[0][ma] * phi;
         }
-        da += (double)(r0 * (w[1][mb] * w[2][mc]));
-        db += (double)(r1 * (dw[1][mb] * w[2][mc]));
-        dc += (double)(r1 * (w[1][mb] * dw[2][mc]));
+        da += (double)(r0 * (w[1][mb] * w2[k]));
+        db += (double)(r1 * (dw[1][mb] * w2[k]));
+        dc += (double)(r1 * (w[1][mb] * dw2[k]));
       }
     }
-    const double q = a.q[i];
-    da *= (double)Ka * q;
-    db *= (double)Kb * q;
-    dc *= (double)Kc * q;
-    // bead-major, so the lines of one bead are written from its own XCD
-    const size_t ib = (size_t)bead * a.N + i;
-    a.rx[ib] = -(da * hi[0] + db * hi[1] + dc * hi[2]);
-    a.ry[ib] = -(da * hi[3] + db * hi[4] + dc * hi[5]);
-    a.rz[ib] = -(da * hi[6] + db * hi[7] + dc * hi[8]);
+    if (L == 2) {  // the pair's lanes hold the same charge
+      da += __shfl_xor(da, 1, 64);
+      db += __shfl_xor(db, 1, 64);
+      dc += __shfl_xor(dc, 1, 64);
+    }
+    if (part == 0) {
+      const double q = a.q[i];
+      da *= (double)Ka * q;
+      db *= (double)Kb * q;
+      dc *= (double)Kc * q;
+      // bead-major, so the lines of one bead are written from its own XCD
+      const size_t ib = (size_t)bead * a.N + i;
+      a.rx[ib] = -(da * hi[0] + db * hi[1] + dc * hi[2]);
+      a.ry[ib] = -(da * hi[3] + db * hi[4] + dc * hi[5]);
+      a.rz[ib] = -(da * hi[6] + db * hi[7] + dc * hi[8]);
+    }
   }
 }
 
@@ -394,7 +435,7 @@ hipError_t launch_spread_t(const PmeArgs &a, hipStream_t st) {
 
 template <typename T>
 hipError_t launch_gather_t(const PmeArgs &a, hipStream_t st) {
-  const dim3 grid(xcd_grid(a.ntile, a.P)), block(PME_BLOCK);
+  const dim3 grid(xcd_grid(a.ntile, a.P)), block(gather_lanes<T>() * PME_BLOCK);
   const size_t lds = pme_lds_bytes(a, true);
   switch (a.n) {
 #define X(NI) \
