@@ -164,6 +164,7 @@ struct pmx_ctx {
   // kernel is bound by vector ALU issue, PME by memory and LDS)
   int overlap = 0;
   bool last_overlap = false;  // the last force sequence ran PME on sk
+  bool pme_deferred = false;  // PME on sk still to be launched, by main_sequence after the pair kernel
   hipStream_t sk = nullptr;
   hipEvent_t kfork = nullptr, kjoin = nullptr;
   hipEvent_t mxev = nullptr;  // the list check's maxima are in mx_pin
@@ -1496,6 +1497,67 @@ int list_stage_finish(pmx_ctx *c, unsigned flags, pmx_energies *out) {
   return PMX_OK;
 }
 
+// The PME chain of a force call (energy_control_inter_recip.c:119-355 with
+// ewald3d_pme.c): binning, spread, FFT and convolution, gather into frec,
+// launched on sk.
+int launch_pme(pmx_ctx *c, hipStream_t sk) {
+  Setup &s = c->s;
+  const int P = s.P;
+  const size_t NP = (size_t)s.N * P;
+  const int nchrg = (int)s.ichrg.size();
+  PmeArgs a = c->pa;
+  a.x = c->x;
+  a.y = c->y;
+  a.z = c->z;
+  a.cell = c->cell;
+  a.rx = c->frec;
+  a.ry = c->frec + NP;
+  a.rz = c->frec + 2 * NP;
+  const size_t nitem = (size_t)P * nchrg, nbin = (size_t)P * a.ntile;
+  hipLaunchKernelGGL(k_fill_int, dim3(blocks(nbin)), dim3(256), 0, sk, c->bin_cnt, 0, nbin);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_pme_bin, dim3(blocks(nchrg), P), dim3(256), (size_t)a.ntile * sizeof(int), sk, a);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_pme_scan, dim3(P), dim3(256), 0, sk, a);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_pme_scatter, dim3(blocks(nitem)), dim3(256), 0, sk, a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(launch_pme_spread(a, sk));
+  FftArgs f{};
+  f.P = P;
+  f.f32 = c->pme_f32;
+  for (int d = 0; d < 3; d++) {
+    f.K[d] = s.ngrid[d];
+    f.npass[d] = c->npass[d];
+    for (int k = 0; k < c->npass[d]; k++) f.radix[d][k] = c->radix[d][k];
+    f.tw[d] = c->tw[d];
+    f.twf[d] = c->twf[d];
+  }
+  f.gstride = c->gstride;
+  f.sstride = c->sstride;
+  f.grid = c->grid;
+  f.spec = c->spec;
+  f.infl = c->infl;
+  f.falp2 = 4.0 * s.alp_ewd * s.alp_ewd;
+  f.cell = c->cell;
+  f.part = c->part + c->off_fft;
+  f.hbuf = a.hbuf;
+  for (int d = 0; d < 3; d++) {
+    f.T[d] = a.T[d];
+    f.nt[d] = a.nt[d];
+    f.H[d] = a.H[d];
+    f.nbr[d] = a.nbr[d];
+  }
+  f.ntile = a.ntile;
+  f.hvol = a.hvol;
+  f.mKa = (unsigned)((1ull << 32) / (unsigned)s.ngrid[0] + 1ull);
+  f.mT0 = a.mT0;
+  f.mT1 = a.mT1;
+  HIPCHK(launch_pme_fft(f, sk));
+  HIPCHK(launch_pme_gather(a, sk));
+  return PMX_OK;
+}
+
 // The device work of a force call that does not depend on the Verlet list,
 // launched on st before the host reads the list check, so the device keeps
 // working meanwhile: the force and kinetic clears, the site packing and
@@ -1543,71 +1605,15 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
   const int nchrg = (int)s.ichrg.size();
   const bool full = respa_call(c, flags).full;  // the k space only in full calls
   const bool pme = c->pme && !(flags & PMX_EVAL_NO_RECIP) && full;
-  auto launch_pme = [&](hipStream_t sk) -> int {
-    PmeArgs a = c->pa;
-    a.x = c->x;
-    a.y = c->y;
-    a.z = c->z;
-    a.cell = c->cell;
-    a.rx = c->frec;
-    a.ry = c->frec + NP;
-    a.rz = c->frec + 2 * NP;
-    const size_t nitem = (size_t)P * nchrg, nbin = (size_t)P * a.ntile;
-    hipLaunchKernelGGL(k_fill_int, dim3(blocks(nbin)), dim3(256), 0, sk, c->bin_cnt, 0, nbin);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_pme_bin, dim3(blocks(nchrg), P), dim3(256), (size_t)a.ntile * sizeof(int), sk, a);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_pme_scan, dim3(P), dim3(256), 0, sk, a);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_pme_scatter, dim3(blocks(nitem)), dim3(256), 0, sk, a);
-    HIPCHK(hipGetLastError());
-    HIPCHK(launch_pme_spread(a, sk));
-    FftArgs f{};
-    f.P = P;
-    f.f32 = c->pme_f32;
-    for (int d = 0; d < 3; d++) {
-      f.K[d] = s.ngrid[d];
-      f.npass[d] = c->npass[d];
-      for (int k = 0; k < c->npass[d]; k++) f.radix[d][k] = c->radix[d][k];
-      f.tw[d] = c->tw[d];
-      f.twf[d] = c->twf[d];
-    }
-    f.gstride = c->gstride;
-    f.sstride = c->sstride;
-    f.grid = c->grid;
-    f.spec = c->spec;
-    f.infl = c->infl;
-    f.falp2 = 4.0 * s.alp_ewd * s.alp_ewd;
-    f.cell = c->cell;
-    f.part = c->part + c->off_fft;
-    f.hbuf = a.hbuf;
-    for (int d = 0; d < 3; d++) {
-      f.T[d] = a.T[d];
-      f.nt[d] = a.nt[d];
-      f.H[d] = a.H[d];
-      f.nbr[d] = a.nbr[d];
-    }
-    f.ntile = a.ntile;
-    f.hvol = a.hvol;
-    f.mKa = (unsigned)((1ull << 32) / (unsigned)s.ngrid[0] + 1ull);
-    f.mT0 = a.mT0;
-    f.mT1 = a.mT1;
-    HIPCHK(launch_pme_fft(f, sk));
-    HIPCHK(launch_pme_gather(a, sk));
-    return PMX_OK;
-  };
   const bool overlap = pme && c->overlap && fork;
   c->last_overlap = overlap;
-  if (overlap) {
+  c->pme_deferred = overlap;
+  if (overlap) {  // PME follows the pair kernel's launch (main_sequence), from this point of st
     HIPCHK(hipEventRecord(c->kfork, st));
-    HIPCHK(hipStreamWaitEvent(c->sk, c->kfork, 0));
-    if (int rc = launch_pme(c->sk)) return rc;
-    if (timing) HIPCHK(hipEventRecord(c->ev[2], c->sk));
-    HIPCHK(hipEventRecord(c->kjoin, c->sk));
     return PMX_OK;
   }
   if (pme)
-    if (int rc = launch_pme(st)) return rc;
+    if (int rc = launch_pme(c, st)) return rc;
   if (c->ewald && !(flags & PMX_EVAL_NO_RECIP) && full) {
     EwaldArgs a{};
     a.N = N;
@@ -1725,6 +1731,16 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
     HIPCHK(launch_pair(a, c->pair_mode, c->full_list, st));
     add_stage(red, c->off_pair, pair_grid_x(N, c->pair_waves), PAIR_NV,
               {ACC_VVDW, ACC_VCOUL, ACC_P11, ACC_P22, ACC_P33, ACC_P12, ACC_P13, ACC_P23, ACC_NPAIR});
+  }
+  if (c->pme_deferred) {
+    // PME on the second stream, launched once the pair kernel is queued:
+    // the pair kernel, the longest kernel of the call, starts as soon as
+    // the list decision allows, and the k space fills the CUs beside it
+    c->pme_deferred = false;
+    HIPCHK(hipStreamWaitEvent(c->sk, c->kfork, 0));
+    if (int rc = launch_pme(c, c->sk)) return rc;
+    if (timing) HIPCHK(hipEventRecord(c->ev[2], c->sk));
+    HIPCHK(hipEventRecord(c->kjoin, c->sk));
   }
   if (timing) HIPCHK(hipEventRecord(c->ev[4], st));
   if (kspace) {
