@@ -125,7 +125,7 @@ struct pmx_ctx {
   bool no_alloc = false;   // inside pmx_energy_control_noalloc: lists may not grow
   size_t off_pair = 0, off_fft = 0, off_ecor = 0, off_bond = 0, off_bend = 0, off_fin = 0, off_mode = 0;
   int g_intra = 1024;
-  double *acc = nullptr, *ake = nullptr;
+  double *acc = nullptr, *ake = nullptr;  // ake: the entry after acc's P * ACC_NUM
   std::vector<double> acc_h;
   // pinned host staging of the asynchronous readbacks: list-check maxima,
   // per-bead sums [P*ACC_NUM] followed by the spring energy
@@ -165,6 +165,7 @@ struct pmx_ctx {
   int overlap = 0;
   bool last_overlap = false;  // the last force sequence ran PME on sk
   bool pme_deferred = false;  // PME on sk still to be launched, by main_sequence after the pair kernel
+  bool fuse_pack = false;     // k_list_check writes site, disp and umax (mixed precision, P a power of 2 <= 64)
   hipStream_t sk = nullptr;
   hipEvent_t kfork = nullptr, kjoin = nullptr;
   hipEvent_t mxev = nullptr;  // the list check's maxima are in mx_pin
@@ -983,6 +984,7 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
         return PMX_ERR_RANGE;
       }
       TRY(dalloc(c, &c->disp, NP));
+      c->fuse_pack = P <= 64 && (64 % P) == 0;  // the list check packs the sites and displacements
     }
     if (c->pair_mode == 2) TRY(build_pair_tables(c));
     if (P > 1) TRY(dalloc(c, &c->umax, (size_t)N));
@@ -1141,8 +1143,9 @@ int build_device(pmx_ctx *c, const pmx_system *sys) {
     TRY(dalloc(c, &c->frec, 3 * NP));
     HIPCHK(hipMemset(c->frec, 0, 3 * NP * sizeof(double)));
   }
-  TRY(dalloc(c, &c->acc, (size_t)P * ACC_NUM));
-  TRY(dalloc(c, &c->ake, 1));
+  // the per-bead sums and, right after them, the spring energy: one readback
+  TRY(dalloc(c, &c->acc, (size_t)P * ACC_NUM + 1));
+  c->ake = c->acc + (size_t)P * ACC_NUM;
   c->acc_h.assign((size_t)P * ACC_NUM, 0.0);
   HIPCHK(hipHostMalloc((void **)&c->mx_pin, 2 * LIST_CHECK_BLOCKS * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void **)&c->acc_pin, ((size_t)P * ACC_NUM + 1) * sizeof(double), hipHostMallocDefault));
@@ -1451,7 +1454,8 @@ int list_stage_launch(pmx_ctx *c, hipStream_t st, bool timing) {
   c->check_blocks = (int)check_blocks;
   hipLaunchKernelGGL(k_list_check, dim3(check_blocks), dim3(256), 0, st, N, P, c->x, c->y, c->z, c->x0,
                      c->y0, c->z0,
-                     c->b0x, c->b0y, c->b0z, c->maxes, xmod_kind(c), c->xm, c->ym, c->zm);
+                     c->b0x, c->b0y, c->b0z, c->maxes, xmod_kind(c), c->xm, c->ym, c->zm,
+                     c->fuse_pack ? CheckPack{c->disp, c->umax, c->site, c->q} : CheckPack{});
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(c->mx_pin, c->maxes, 2 * check_blocks * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(c->mxev, st));
@@ -1580,11 +1584,16 @@ int pre_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing, bool f
       HIPCHK(hipGetLastError());
     }
   }
-  hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
-                     c->q, c->site);
-  HIPCHK(hipGetLastError());
-  c->disp_packed = false;
-  if (c->disp && !clu_active(c)) {
+  // with fuse_pack the list check has packed the sites and displacements
+  // (list_stage_launch precedes every force sequence; a weighted call's
+  // second pass reuses the first pass's, the positions being the same)
+  if (!c->fuse_pack) {
+    hipLaunchKernelGGL(k_pack_sites, dim3(blocks(N)), dim3(256), 0, st, N, P > 1 ? 1 : 0, c->xm, c->ym, c->zm,
+                       c->q, c->site);
+    HIPCHK(hipGetLastError());
+  }
+  c->disp_packed = c->fuse_pack;
+  if (c->disp && !clu_active(c) && !c->fuse_pack) {
     c->disp_packed = true;
     if (P <= 64) {
       const size_t waves = ((size_t)N + 64 / P - 1) / (64 / P);
@@ -1898,8 +1907,7 @@ int force_sequence(pmx_ctx *c, unsigned flags, hipStream_t st) {
 // Readback of the per-bead sums into pinned host memory, launched on st.
 int readback_launch(pmx_ctx *c, hipStream_t st) {
   if (c->clu_used) HIPCHK(hipMemcpyAsync(c->clu_pin + 1, c->clu_flag + 1, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(c->acc_pin, c->acc, c->acc_h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(c->acc_pin + c->acc_h.size(), c->ake, sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(c->acc_pin, c->acc, (c->acc_h.size() + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
   return PMX_OK;
 }
 

@@ -340,9 +340,17 @@ __global__ void k_pack_disp_umax(int N, int P, const double *x, const double *y,
                                  const double *ym, const double *zm, float4 *u, double *umax);
 __global__ void k_site_umax64(int N, int P, const double *x, const double *y, const double *z, const double *xm,
                               const double *ym, const double *zm, double *umax);
+// Pack: what k_pack_sites and k_pack_disp_umax write, formed in the same
+// pass (disp non-null: mixed precision with P a power of two up to 64).
+struct CheckPack {
+  float4 *disp;
+  double *umax;
+  double4 *site;
+  const double *q;
+};
 __global__ void k_list_check(int N, int P, const double *x, const double *y, const double *z, const double *x0,
                              const double *y0, const double *z0, double *b0x, double *b0y, double *b0z,
-                             double *maxes, int centroid, double *xm, double *ym, double *zm);
+                             double *maxes, int centroid, double *xm, double *ym, double *zm, CheckPack pk);
 constexpr int LIST_CHECK_BLOCKS = 512;  // workgroups of k_list_check, one pair of maxima each
 __global__ void k_list_build(ListArgs a);
 __global__ void k_cell_count(ListArgs a);

@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
                                                     const double *__restrict__ x0, const double *__restrict__ y0,
                                                     const double *__restrict__ z0, double *b0x, double *b0y,
                                                     double *b0z, double *maxes, int centroid,
-                                                    double *xm, double *ym, double *zm) {
+                                                    double *xm, double *ym, double *zm, CheckPack pk) {
 #pragma clang fp contract(off)
   __shared__ double red[2][4];
   __shared__ double stage[4][3 * 64];  // one chunk of bead positions per wavefront
@@ -202,6 +202,7 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
     const bool act = g < S && i < N;
     const size_t base = (size_t)(act ? i : 0) * P;
     double sx = 0.0, sy = 0.0, sz = 0.0, xi = 0.0, yi = 0.0, zi = 0.0;
+    double xk = 0.0, yk = 0.0, zk = 0.0;  // this lane's bead of the first chunk
     for (int c0 = 0; c0 < P; c0 += G) {
       const bool on = act && c0 + k < P;
       const double xb = on ? x[base + c0 + k] : 0.0, yb = on ? y[base + c0 + k] : 0.0,
@@ -210,6 +211,9 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
         xi = shfl_d(xb, lead);
         yi = shfl_d(yb, lead);
         zi = shfl_d(zb, lead);
+        xk = xb;
+        yk = yb;
+        zk = zb;
       }
       if (on) {
         const double ex = xb - xi, ey = yb - yi, ez = zb - zi;
@@ -259,6 +263,30 @@ __global__ void __launch_bounds__(256) k_list_check(int N, int P, const double *
         xm[i] = xi;
         ym[i] = yi;
         zm[i] = zi;
+      }
+    }
+    if (pk.disp) {  // uniform; P divides 64, so the site's beads are this group's lanes
+      // the site's xmod as the lead lane has it (formed above, or given)
+      double rx = 0.0, ry = 0.0, rz = 0.0;
+      if (act && k == 0) {
+        rx = centroid == 1 ? sx / P : centroid == 2 ? xi : xm[i];
+        ry = centroid == 1 ? sy / P : centroid == 2 ? yi : ym[i];
+        rz = centroid == 1 ? sz / P : centroid == 2 ? zi : zm[i];
+      }
+      rx = shfl_d(rx, lead);
+      ry = shfl_d(ry, lead);
+      rz = shfl_d(rz, lead);
+      // k_pack_disp_umax and k_pack_sites
+      float m = 0.0f;
+      if (act) {
+        const float4 v = make_float4((float)(xk - rx), (float)(yk - ry), (float)(zk - rz), 0.0f);
+        pk.disp[base + k] = v;
+        m = v.x * v.x + v.y * v.y + v.z * v.z;
+      }
+      for (int off = 1; off < G; off <<= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+      if (act && k == 0) {
+        pk.umax[i] = sqrt((double)m);
+        pk.site[i] = make_double4(rx, ry, rz, pk.q[i]);
       }
     }
   }
