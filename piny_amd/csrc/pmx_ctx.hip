@@ -1891,7 +1891,7 @@ int main_sequence(pmx_ctx *c, unsigned flags, hipStream_t st, bool timing) {
   red.spart = c->part + c->off_mode;  // the spring energy (0 without modes)
   red.sG = modes ? mode_blocks(N, P) : 0;
   red.sacc = c->ake;
-  hipLaunchKernelGGL(k_reduce_partials, dim3(blocks(((size_t)P * ACC_NUM + 1) * 64)), dim3(256), 0, st, red);
+  hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)P * ACC_NUM + 1), dim3(256), 0, st, red);
   HIPCHK(hipGetLastError());
   if (timing) HIPCHK(hipEventRecord(c->ev[5], st));
   return PMX_OK;
@@ -3310,7 +3310,7 @@ int pmx_get_param_grad(pmx_ctx *c, pmx_param_grad *out) {
     for (int v = 0; v < PAIRG_NV; v++) red.st[0].slot[v] = v;
     red.part = c->gpart;
     red.acc = c->gacc;
-    hipLaunchKernelGGL(k_reduce_partials, dim3(blocks((size_t)P * PAIRG_NV * 64)), dim3(256), 0, st, red);
+    hipLaunchKernelGGL(k_reduce_partials, dim3((unsigned)P * PAIRG_NV + 1), dim3(256), 0, st, red);
     HIPCHK(hipGetLastError());
   }
   const int nchrg = (int)s.ichrg.size();

@@ -321,7 +321,7 @@ struct RedArgs {
   const double *part;
   double *acc;  // [P][nslot], every entry written
   // one scalar besides: *sacc = the sum of spart[0 .. sG) (0 when sG = 0),
-  // by the wavefront after the per-bead ones
+  // by the workgroup after the per-bead ones
   const double *spart;
   int sG;
   double *sacc;

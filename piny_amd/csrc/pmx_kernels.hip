@@ -27,42 +27,46 @@ __global__ void k_fill_double(double *p, double v, size_t n) {
   if (t < n) p[t] = v;
 }
 
-// Sums the partial rows of every stage into the per-bead accumulators, one
-// wavefront per (bead, slot); blocks are summed in a fixed order.
+// The per-bead sums from the partial rows: one workgroup per (bead, slot)
+// output (and one more for the scalar *sacc = the sum of spart[0 .. sG)),
+// its 256 lanes over the rows' entries, then a fixed-order tree in LDS, so
+// the result does not depend on timing.
 __global__ void __launch_bounds__(256) k_reduce_partials(RedArgs a) {
-  const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = threadIdx.x & 63;
-  if (w == a.P * a.nslot) {  // the scalar
-    if (!a.sacc) return;
-    double s = 0.0;
-    for (int g = lane; g < a.sG; g += 64) s += a.spart[g];
-    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) *a.sacc = s;
-    return;
-  }
-  if (w >= a.P * a.nslot) return;  // uniform over the wavefront
-  const int b = w / a.nslot, slot = w % a.nslot;
-  // four independent sums per lane keep four loads in flight
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int si = 0; si < a.nstage; si++) {
-    const RedStage &r = a.st[si];
-    for (int v = 0; v < r.nv; v++) {
-      if (r.slot[v] != slot) continue;
-      const double *row = a.part + r.off + ((size_t)v * a.P + b) * r.G;
-      // a weight of 1 leaves every sum as it is (the products are exact)
-      const double w = v == 0 ? 1.0 : r.wv;
-      int g = lane;
-      for (; g + 192 < r.G; g += 256) {
-        s0 += w * row[g];
-        s1 += w * row[g + 64];
-        s2 += w * row[g + 128];
-        s3 += w * row[g + 192];
+  __shared__ double red[256];
+  const int o = blockIdx.x, tid = threadIdx.x;
+  const bool scalar = o == a.P * a.nslot;
+  if (scalar && !a.sacc) return;  // uniform over the workgroup
+  double s0 = 0.0, s1 = 0.0;
+  if (scalar) {
+    for (int g = tid; g < a.sG; g += 256) s0 += a.spart[g];
+  } else {
+    const int b = o / a.nslot, slot = o % a.nslot;
+    for (int si = 0; si < a.nstage; si++) {
+      const RedStage &r = a.st[si];
+      for (int v = 0; v < r.nv; v++) {
+        if (r.slot[v] != slot) continue;
+        const double *row = a.part + r.off + ((size_t)v * a.P + b) * r.G;
+        // a weight of 1 leaves every sum as it is (the products are exact)
+        const double w = v == 0 ? 1.0 : r.wv;
+        int g = tid;
+        for (; g + 256 < r.G; g += 512) {
+          s0 += w * row[g];
+          s1 += w * row[g + 256];
+        }
+        for (; g < r.G; g += 256) s0 += w * row[g];
       }
-      for (; g < r.G; g += 64) s0 += w * row[g];
     }
   }
-  double s = (s0 + s1) + (s2 + s3);
-  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane == 0) a.acc[(size_t)b * a.nslot + slot] = s;
+  red[tid] = s0 + s1;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) red[tid] += red[tid + h];
+    __syncthreads();
+  }
+  if (tid == 0) {
+    if (scalar) *a.sacc = red[0];
+    else a.acc[(size_t)(o / a.nslot) * a.nslot + (o % a.nslot)] = red[0];
+  }
 }
 
 // ---------------------------------------------------------------------------
