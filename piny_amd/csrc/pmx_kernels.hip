@@ -1041,17 +1041,20 @@ __global__ void __launch_bounds__(256) k_pair_half(PairArgs a) {
   block_partials<PAIR_NV>(vals, B, blockIdx.y * B, P, a.part, blockIdx.x, gridDim.x);
 }
 
-// The FP32 pair term of pair_direct with the type pair's constants in
-// registers: A = (cut^2, cut, C12, C6), B = (LJ shift, Coulomb shift, LJ
-// flag, 0), read from LDS as two 16-byte loads per pair.  ct holds the erfc
-// term's cubic pieces: with CSW switched and shifted already (the switch is
-// then evaluated for the LJ term only), else erfc(alp r)/r itself.  The
-// energies are added to vlj and vc; the erfc term is formed for every pair
-// (q2 = 0 adds zeros), so only the LJ term takes a branch.
+// The FP32 pair term of pair_direct with type pair t's constants from LDS:
+// sA[t] = (cut^2, cut, C12, C6), sB[t] = (LJ shift, Coulomb shift, LJ flag,
+// RESPA cutoff); lj says whether the pair has an LJ term.  ct holds the
+// erfc term's cubic pieces: with CSW switched and shifted already (the
+// switch and the constants are then read only for the LJ term), else
+// erfc(alp r)/r itself.  The energies are added to vlj and vc; the erfc term
+// is formed for every pair (q2 = 0 adds zeros), so only the LJ term takes a
+// branch.
 template <bool CSW>
-__device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4 &A, const float4 &B, float q2,
-                                                 float r, float rinv, float &vlj, float &vc, const float4 *ct) {
+__device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4 *sA, const float4 *sB, int t,
+                                                 bool lj, float q2, float r, float rinv, float &vlj, float &vc,
+                                                 const float4 *ct) {
   float sw = 1.f, dsw = 0.f;
+  float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
   auto switch_at = [&]() {
     const float rheal = (float)a.rheal, irheal = (float)a.irheal;
     const float rs = r - A.y + rheal;
@@ -1062,21 +1065,25 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
       dsw = -6.f * (rp * irheal) * (rp - 1.f) * rinv;
     }
   };
-  if (!CSW && a.swit) switch_at();
+  if (!CSW) {
+    A = sA[t];
+    B = sB[t];
+    if (a.swit) switch_at();
+  }
   float dvt;
   {
     // the erfc term and its r derivative from the cubic piece of the table
     // u < 0 (r below the table) converts to k = 0 and extrapolates the first piece
     const float u = r * a.ct_idr - a.ct_r0 * a.ct_idr;
     const unsigned k = min(__float2uint_rz(u), (unsigned)(CT_N - 1));
-    const float t = u - (float)k;
+    const float tt = u - (float)k;
     const float4 cc = ct[k];
     // value and t derivative by one Horner scheme
-    const float p1 = fmaf(cc.w, t, cc.z);
-    const float d1 = fmaf(cc.w, t, p1);
-    const float p2 = fmaf(p1, t, cc.y);
-    const float d2 = fmaf(d1, t, p2);
-    const float v = fmaf(p2, t, cc.x);
+    const float p1 = fmaf(cc.w, tt, cc.z);
+    const float d1 = fmaf(cc.w, tt, p1);
+    const float p2 = fmaf(p1, tt, cc.y);
+    const float d2 = fmaf(d1, tt, p2);
+    const float v = fmaf(p2, tt, cc.x);
     const float dv = -(d2 * a.ct_idr) * rinv;  // -(1/r) dV/dr
     if (CSW) {
       vc += q2 * v;
@@ -1086,8 +1093,12 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
       dvt = q2 * (dv * sw + v * dsw);
     }
   }
-  if (B.z != 0.f) {  // r^-6 (C12 r^-6 - C6) and 6 r^-6 (2 C12 r^-6 - C6) / r^2
-    if (CSW && a.swit) switch_at();
+  if (lj) {  // r^-6 (C12 r^-6 - C6) and 6 r^-6 (2 C12 r^-6 - C6) / r^2
+    if (CSW) {
+      A = sA[t];
+      B = sB[t];
+      if (a.swit) switch_at();
+    }
     const float r2i = rinv * rinv;
     const float r6i = r2i * r2i * r2i, t12 = A.z * r6i;
     const float v = r6i * (t12 - A.w);
@@ -1128,6 +1139,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   extern __shared__ float4 sct[];                       // the tables: TAB nftab * CT_N, else CT_N pieces
   __shared__ float4 sdq[4][64];
   __shared__ int styp[4][64];  // type pair of the staged entries
+  __shared__ float scut[4][64];  // their cut^2, negated for a type pair with an LJ term (not TAB)
   // byte offset of site j's first bead displacement in u per staged entry,
   // 0 past the chunk's end (a valid address whose value goes unused)
   __shared__ unsigned soff[4][PIMD ? SPAD : 1];
@@ -1195,6 +1207,11 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         const int pos = __popcll(mk & below);
         sdq[w][pos] = make_float4((float)dx, (float)dy, (float)dz, (float)q2);
         styp[w][pos] = e >> ENTRY_SHIFT;
+        if (!TAB) {
+          const int t = e >> ENTRY_SHIFT;
+          const float c2 = sppA[t].x;
+          scut[w][pos] = sppB[t].z != 0.f ? -c2 : c2;
+        }
         if (PIMD) soff[w][pos] = (unsigned)(e & ENTRY_MASK) * (unsigned)P * (unsigned)sizeof(float4);
       }
       if (PIMD && lane >= nk) soff[w][lane] = 0u;
@@ -1237,14 +1254,14 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           }
           dvd = -(dv * a.ct_idr) * rinv;  // -(1/r) dV/dr
         } else {
-          const float4 A = sppA[t], B2 = sppB[t];
-          if (!bact || !(r2 < A.x)) return;
+          const float c2 = scut[w][p];
+          if (!bact || !(r2 < fabsf(c2))) return;
           const float rinv = __frsqrt_rn(r2);
-          dvd = pair_direct_f32<TM == 2>(a, A, B2, d.w, r2 * rinv, rinv, cvlj, cvc, sct);
-          if constexpr (RESPA) {  // respa_weight in FP32; B2.w is the RESPA cutoff
+          dvd = pair_direct_f32<TM == 2>(a, sppA, sppB, t, c2 < 0.f, d.w, r2 * rinv, rinv, cvlj, cvc, sct);
+          if constexpr (RESPA) {  // respa_weight in FP32; sppB[t].w is the RESPA cutoff
             float fm = (float)a.w_full;
             if (a.respa != RESPA_FULL) {
-              float br = (r2 * rinv - B2.w + (float)a.rheal) * (float)a.irheal;
+              float br = (r2 * rinv - sppB[t].w + (float)a.rheal) * (float)a.irheal;
               br = fminf(fmaxf(br, 0.f), 1.f);
               const float sw = br * br * (br * 2.f - 3.f) + 1.f;
               fm = a.respa == RESPA_RES ? sw * (float)a.w_res : sw * (float)a.w_res + (1.f - sw) * (float)a.w_full;
