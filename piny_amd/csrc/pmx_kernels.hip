@@ -1139,7 +1139,9 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   extern __shared__ float4 sct[];                       // the tables: TAB nftab * CT_N, else CT_N pieces
   __shared__ float4 sdq[4][64];
   __shared__ int styp[4][64];  // type pair of the staged entries
-  __shared__ float scut[4][64];  // their cut^2, negated for a type pair with an LJ term (not TAB)
+  // not TAB: sdq's w holds the entry's cut^2, negated for a type pair with
+  // an LJ term, and sq2 its charge product (read only inside the cutoff)
+  __shared__ float sq2[4][64];
   // byte offset of site j's first bead displacement in u per staged entry,
   // 0 past the chunk's end (a valid address whose value goes unused)
   __shared__ unsigned soff[4][PIMD ? SPAD : 1];
@@ -1205,13 +1207,15 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
       const int nk = __popcll(mk);
       if (keep) {
         const int pos = __popcll(mk & below);
-        sdq[w][pos] = make_float4((float)dx, (float)dy, (float)dz, (float)q2);
+        float dw = (float)q2;
         styp[w][pos] = e >> ENTRY_SHIFT;
         if (!TAB) {
           const int t = e >> ENTRY_SHIFT;
           const float c2 = sppA[t].x;
-          scut[w][pos] = sppB[t].z != 0.f ? -c2 : c2;
+          sq2[w][pos] = dw;
+          dw = sppB[t].z != 0.f ? -c2 : c2;
         }
+        sdq[w][pos] = make_float4((float)dx, (float)dy, (float)dz, dw);
         if (PIMD) soff[w][pos] = (unsigned)(e & ENTRY_MASK) * (unsigned)P * (unsigned)sizeof(float4);
       }
       if (PIMD && lane >= nk) soff[w][lane] = 0u;
@@ -1254,10 +1258,10 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
           }
           dvd = -(dv * a.ct_idr) * rinv;  // -(1/r) dV/dr
         } else {
-          const float c2 = scut[w][p];
+          const float c2 = d.w;
           if (!bact || !(r2 < fabsf(c2))) return;
           const float rinv = __frsqrt_rn(r2);
-          dvd = pair_direct_f32<TM == 2>(a, sppA, sppB, t, c2 < 0.f, d.w, r2 * rinv, rinv, cvlj, cvc, sct);
+          dvd = pair_direct_f32<TM == 2>(a, sppA, sppB, t, c2 < 0.f, sq2[w][p], r2 * rinv, rinv, cvlj, cvc, sct);
           if constexpr (RESPA) {  // respa_weight in FP32; sppB[t].w is the RESPA cutoff
             float fm = (float)a.w_full;
             if (a.respa != RESPA_FULL) {
