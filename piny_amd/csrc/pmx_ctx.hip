@@ -55,9 +55,9 @@ struct pmx_ctx {
   bool last_weighted = false;  // the last call's forces carried RESPA weights other than 1
   bool fxt_kept = false;       // fxt holds the last call's unweighted forces (a weighted full call's first pass)
   bool keep_fxt = false;       // this sequence must not overwrite fxt
-  float4 *ctab = nullptr;      // mixed precision: erfc(alp r)/r as cubic pieces [CT_N]
+  float4 *ctab = nullptr;      // mixed precision: erfc(alp r)/r as cubic pieces [CT_N + 1]
   float4 *ftab = nullptr;      // mixed precision: the switched pair functions [nftab][CT_N] (null if too many)
-  float4 *cswtab = nullptr;    // mixed precision: the switched, shifted erfc term [CT_N] when every type pair shares it
+  float4 *cswtab = nullptr;    // mixed precision: the switched, shifted erfc term [CT_N + 1] when every type pair shares it
   int nftab = 0;
   std::vector<int2> ftab_off;  // per type pair: (erfc table, LJ table or -1), offsets in pieces
   float ct_r0 = 0.f, ct_idr = 0.f;
@@ -710,13 +710,21 @@ int build_pair_tables(pmx_ctx *c) {
   dfree(c, c->ctab);
   dfree(c, c->ftab);
   dfree(c, c->cswtab);
+  // the erfc tables of the per-site mixed kernel hold one more piece, the
+  // last repeated, so an r that rounds to the grid's end indexes a piece
+  // without a clamp
+  ct.push_back(ct.back());
   int rc = upload(c, &c->ctab, ct.data(), ct.size());
   // One erfc function for every type pair (one cutoff and shift): the mixed
   // pair kernel reads it switched and shifted from its table and evaluates
   // the switch only for the LJ term.
   bool one_c = s.npair > 0;
   for (int t = 1; t < s.npair; t++) one_c = one_c && c->ftab_off[t].x == c->ftab_off[0].x;
-  if (!rc && one_c) rc = upload(c, &c->cswtab, ft.data() + c->ftab_off[0].x, (size_t)CT_N);
+  if (!rc && one_c) {
+    std::vector<float4> cs(ft.begin() + c->ftab_off[0].x, ft.begin() + c->ftab_off[0].x + CT_N);
+    cs.push_back(cs.back());
+    rc = upload(c, &c->cswtab, cs.data(), cs.size());
+  }
   // The whole-function tables are built but used only on request
   // (PMX_MIXED_TAB=1): on the 4096-molecule, 32-bead box the pair kernel
   // takes 0.656 ms with them against 0.373 ms with LJ and the switch

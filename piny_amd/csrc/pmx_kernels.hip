@@ -1051,8 +1051,10 @@ __global__ void __launch_bounds__(256) k_pair_half(PairArgs a) {
 // branch.
 template <bool CSW>
 __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4 *sA, const float4 *sB, int t,
-                                                 bool lj, float q2, float r, float rinv, float &vlj, float &vc,
+                                                 bool lj, float2 q, float r, float rinv, float &vlj, float &vc,
                                                  const float4 *ct) {
+  // q = (q_i q_j, -q_i q_j / dr of the table grid)
+  const float q2 = q.x;
   float sw = 1.f, dsw = 0.f;
   float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = A;
   auto switch_at = [&]() {
@@ -1075,7 +1077,8 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
     // the erfc term and its r derivative from the cubic piece of the table
     // u < 0 (r below the table) converts to k = 0 and extrapolates the first piece
     const float u = r * a.ct_idr - a.ct_r0 * a.ct_idr;
-    const unsigned k = min(__float2uint_rz(u), (unsigned)(CT_N - 1));
+    // r < r_c <= the grid's end: k <= CT_N, the repeated last piece
+    const unsigned k = __float2uint_rz(u);
     const float tt = u - (float)k;
     const float4 cc = ct[k];
     // value and t derivative by one Horner scheme
@@ -1084,11 +1087,11 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
     const float p2 = fmaf(p1, tt, cc.y);
     const float d2 = fmaf(d1, tt, p2);
     const float v = fmaf(p2, tt, cc.x);
-    const float dv = -(d2 * a.ct_idr) * rinv;  // -(1/r) dV/dr
     if (CSW) {
       vc += q2 * v;
-      dvt = q2 * dv;
+      dvt = (d2 * q.y) * rinv;  // -(q2 / r) dV/dr
     } else {
+      const float dv = -(d2 * a.ct_idr) * rinv;  // -(1/r) dV/dr
       vc += q2 * (v * sw - B.y);
       dvt = q2 * (dv * sw + v * dsw);
     }
@@ -1136,12 +1139,13 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
   constexpr int SPAD = 64 + 4 * S;
   __shared__ float4 sppA[MAX_PAIRS], sppB[MAX_PAIRS];  // pair_direct_f32's constants per type pair
   __shared__ int4 stt[MAX_PAIRS];  // (cut^2 as bits, TAB: erfc table, LJ table or -1, culling radius as bits)
-  extern __shared__ float4 sct[];                       // the tables: TAB nftab * CT_N, else CT_N pieces
+  extern __shared__ float4 sct[];                       // the tables: TAB nftab * CT_N, else CT_N + 1 pieces
   __shared__ float4 sdq[4][64];
   __shared__ int styp[4][64];  // type pair of the staged entries
   // not TAB: sdq's w holds the entry's cut^2, negated for a type pair with
-  // an LJ term, and sq2 its charge product (read only inside the cutoff)
-  __shared__ float sq2[4][64];
+  // an LJ term, and sq2 its charge product and that times -1/dr of the erfc
+  // table (read only inside the cutoff)
+  __shared__ float2 sq2[4][64];
   // byte offset of site j's first bead displacement in u per staged entry,
   // 0 past the chunk's end (a valid address whose value goes unused)
   __shared__ unsigned soff[4][PIMD ? SPAD : 1];
@@ -1152,7 +1156,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
     stt[t] = make_int4(__float_as_int((float)pp[PP_CUT2]), a.ftoff[t].x, a.ftoff[t].y,
                        __float_as_int((float)pp[PP_CULL]));
   }
-  const int ntab = TAB ? a.nftab * CT_N : CT_N;
+  const int ntab = TAB ? a.nftab * CT_N : CT_N + 1;
   const float4 *src = TAB ? a.ftab : (TM == 2 ? a.cswtab : a.ctab);
   for (int k = threadIdx.x; k < ntab; k += blockDim.x) sct[k] = src[k];
   __syncthreads();
@@ -1212,7 +1216,7 @@ __global__ void __launch_bounds__(256) k_pair_mixed(PairArgs a) {
         if (!TAB) {
           const int t = e >> ENTRY_SHIFT;
           const float c2 = sppA[t].x;
-          sq2[w][pos] = dw;
+          sq2[w][pos] = make_float2(dw, -dw * a.ct_idr);
           dw = sppB[t].z != 0.f ? -c2 : c2;
         }
         sdq[w][pos] = make_float4((float)dx, (float)dy, (float)dz, dw);
@@ -2721,7 +2725,7 @@ void launch_pair_b(const PairArgs &a, int mode, bool full, dim3 grid, hipStream_
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pair_mixed<B, false, 1, false>), grid, block, lds, st, a);
       }
     } else {
-      const size_t lds = (size_t)CT_N * sizeof(float4);
+      const size_t lds = (size_t)(CT_N + 1) * sizeof(float4);
       if (a.cswtab) launch_pair_mixed_tm<B, 2>(a, grid, lds, st);
       else launch_pair_mixed_tm<B, 0>(a, grid, lds, st);
     }
