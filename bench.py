@@ -44,7 +44,7 @@ PEAK = {"f64": 78.6, "f32": 157.3}   # MI355X vector TFLOP/s (FP64 vector = FP64
 # HBM bytes per launch of the pair kernel, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (separate runs) scaled by a 512 MiB copy probe (FETCH x 2.00,
 # WRITE x 1.00); profiles/kernels_q-tip4p-f_4096_p32_{mixed,fp64}.md
-PAIR_TRAFFIC = {"mixed": (59934 * 2.0 + 21859) * 1024, "fp64": (129379 * 2.0 + 22829) * 1024}
+PAIR_TRAFFIC = {"mixed": (58888 * 2.0 + 21779) * 1024, "fp64": (129379 * 2.0 + 22829) * 1024}
 FLOP_PER_PAIR = 100.0                 # nominal algorithmic flops per pair term inside the cutoff
 DT_FS = 0.25
 
