@@ -1117,10 +1117,12 @@ __device__ __forceinline__ float pair_direct_f32(const PairArgs &a, const float4
 // entry: the bead-independent part of the separation, the imaged centroid
 // separation D_ij (FP64: xmod_i - xmod_j - h nint(hi (xmod_i - xmod_j)),
 // period.c:121-206; without PIMD the imaged x_i - x_j itself), rounded to
-// FP32, and q_i q_j, into the wavefront's LDS slice.  Then the (slot, bead)
-// lanes evaluate the chunk's pairs in FP32: r = u_i - u_j + D_ij with the
-// bead displacements u = x - xmod, the LJ and erfc terms of pair_direct
-// (erfc(alp r)/r from an LDS table of cubic pieces of PINY's polynomial).
+// FP32, with the type pair's cutoff (not TM = 1) and q_i q_j, and the byte
+// offset of site j's displacements, into the wavefront's LDS slice.  Then
+// the (slot, bead) lanes evaluate the chunk's pairs in FP32: r = u_i - u_j +
+// D_ij with the bead displacements u = x - xmod, the LJ and erfc terms of
+// pair_direct (erfc(alp r)/r from an LDS table of cubic pieces of PINY's
+// polynomial).
 // Energies, virial and forces are summed in FP32 within a chunk and in FP64
 // across chunks and lanes.
 //
