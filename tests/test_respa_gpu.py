@@ -146,3 +146,33 @@ def test_respa_fxt_and_integrators():
     assert np.isfinite(e["pi_ke_vir"])
     g.fxt()
     g.close()
+
+
+def test_respa_mixed_cutoff_per_type_pair():
+    """Mixed precision under RESPA with type pairs of different cutoffs (the
+    kernel switches the erfc term itself, as without one erfc function for
+    every type pair): full, short-range and both, against the oracle."""
+    import dataclasses
+    c = cases.get("tip64_p4_respa")
+    cut = np.array(c.system.inter_cutoff, dtype=float).copy()
+    cut[0] *= 0.92
+    s = dataclasses.replace(c.system, inter_cutoff=cut)
+    print(f"[respa mixed, cutoffs {cut}]", flush=True)
+    o = Oracle(s, c.x, c.y, c.z)
+    o.set_respa(abi.respa_params(s))
+    g = Engine(s, c.x, c.y, c.z, precision=abi.PMX_PREC_MIXED)
+    g.set_respa(abi.respa_params(s))
+    g.set_positions(c.x, c.y, c.z)
+    for full, res in ((1, 1), (0, 1), (1, 0)):
+        fl = flags_of(full, res, 1, s.pi_beads)
+        eo, fo, _ = o.energy_control(c.x, c.y, c.z, fl)
+        eg = g.energy_control(fl)
+        fg = g.forces()
+        rms = np.sqrt(((fg - fo) ** 2).sum() / (fo ** 2).sum())
+        print(f"  flags ({full}, {res}): force rms rel err {rms:.2e}", flush=True)
+        assert rms < MIXED_FTOL
+        assert np.max(np.abs(eg["pvten"] - eo["pvten"])) <= MIXED_VTOL * np.max(np.abs(eo["pvten"]))
+        scale = max(abs(eo["vintert"]), abs(eo["vintrat"]), 1e-3)
+        for k in ("vintert", "vintrat"):
+            assert abs(eg[k] - eo[k]) <= MIXED_ETOL * max(abs(eo[k]), scale), k
+    g.close()
