@@ -149,7 +149,7 @@ def run_path(cfg, steps, warmup, rebuild_every=20, tune=True, **engine_kw):
     tuned = None
     if tune:  # the project's autotuner picks the launch shapes first (untimed)
         t0 = time.perf_counter()
-        tuned = eng.autotune(reps=3, flags=flags)
+        tuned = eng.autotune(reps=10, flags=flags)
         tuned["seconds"] = time.perf_counter() - t0
     for _ in range(warmup):
         e = eng.energy_control(flags)
@@ -249,7 +249,7 @@ def run_batch(steps, warmup, nbox=64):
         b = Batch(engs) if batched else None
         if tune:  # the batch autotuner picks the launch shapes first
             t0 = time.perf_counter()
-            tuned = b.autotune(reps=2)
+            tuned = b.autotune(reps=5)
             tuned["seconds"] = time.perf_counter() - t0
 
         def step():
