@@ -2156,6 +2156,52 @@ __global__ void __launch_bounds__(256) k_mode_forces(ModeArgs a) {
           F.z = pf[j].z + (r * F.z);
         }
       }
+    } else if (PT >= 2) {
+      // lanes work in pairs over half the beads each: (0, P-1) sum with the
+      // weights 1 and cos(pi b) of frequencies 0 and P/2, (2k-1, 2k) with the
+      // cosine and sine of frequency k; each lane hands its partner the sum
+      // the partner keeps, so every bead row and twiddle is read once per pair
+      const bool edge = (m == 0 || m == P - 1);
+      const int k = m == 0 ? P / 2 : (m == P - 1 ? P / 2 : (m + 1) / 2);
+      const int h = (m != 0) && (m == P - 1 || !(m & 1));  // half of the beads
+      const int pm = m == 0 ? P - 1 : (m == P - 1 ? 0 : ((m & 1) ? m + 1 : m - 1));
+      double3 uA = make_double3(0.0, 0.0, 0.0), uB = uA, FA = uA, FB = uA;
+#pragma unroll
+      for (int j = 0; j < PT / 2; j++) {
+        const int b = h * (PT / 2) + j;
+        const double2 cs = stw[(k * b) & (PT - 1)];
+        const double wa = edge ? 1.0 : cs.x, wb = edge ? cs.x : cs.y;
+        const double3 xv = px[b], fv = pf[b];
+        uA.x += xv.x * wa;
+        uA.y += xv.y * wa;
+        uA.z += xv.z * wa;
+        uB.x += xv.x * wb;
+        uB.y += xv.y * wb;
+        uB.z += xv.z * wb;
+        FA.x += fv.x * wa;
+        FA.y += fv.y * wa;
+        FA.z += fv.z * wa;
+        FB.x += fv.x * wb;
+        FB.y += fv.y * wb;
+        FB.z += fv.z * wb;
+      }
+      // lane 0 and the cosine lanes keep the A sums, lane P-1 and the sine
+      // lanes the B sums
+      const bool keepA = (m == 0) || ((m & 1) && m != P - 1);
+      const double3 um = keepA ? uA : uB, ut = keepA ? uB : uA;
+      const double3 Fm = keepA ? FA : FB, Ft = keepA ? FB : FA;
+      const int src = lead + pm;
+      u = make_double3(um.x + __shfl(ut.x, src, 64), um.y + __shfl(ut.y, src, 64), um.z + __shfl(ut.z, src, 64));
+      F = make_double3(Fm.x + __shfl(Ft.x, src, 64), Fm.y + __shfl(Ft.y, src, 64), Fm.z + __shfl(Ft.z, src, 64));
+      const double dP = (double)P;
+      u.x /= dP;
+      u.y /= dP;
+      u.z /= dP;
+      if (!edge) {
+        F.x *= 2.0;
+        F.y *= 2.0;
+        F.z *= 2.0;
+      }
     } else {
       int k = 0, im = 0;  // frequency and real (0) / imaginary (1) part of mode m
       if (m == P - 1) k = P / 2;
